@@ -1,0 +1,124 @@
+"""ctypes binding of libmgpbd.so (declarations in include/mgpbd.h).
+
+The library is the product: when it is missing the package raises instead of falling back
+to any CPU path. Device pointers come from torch tensors on the current HIP device; torch is
+used for device memory and streams only.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libmgpbd.so")
+
+c_int, c_float, c_double, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+c_int64 = ctypes.c_int64
+I32P = ctypes.POINTER(ctypes.c_int32)
+F32P = ctypes.POINTER(ctypes.c_float)
+F64P = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); device pointers are passed as c_void_p
+_V = c_void_p
+SIGNATURES = {
+    "mgpbd_last_error": (ctypes.c_char_p, []),
+    "mgpbd_abi_version": (c_int, []),
+    # host setup
+    "mgpbd_pattern_build": (_V, [c_int, c_int, I32P]),
+    "mgpbd_pattern_nnz": (c_int64, [_V]),
+    "mgpbd_pattern_copy": (c_int, [_V, I32P, I32P]),
+    "mgpbd_pattern_free": (None, [_V]),
+    "mgpbd_face_adjacency": (c_int, [c_int, I32P, I32P, I32P]),
+    "mgpbd_tet_aggregate": (c_int, [c_int, I32P, I32P, c_int, I32P, I32P]),
+    "mgpbd_graph_aggregate": (c_int, [c_int, I32P, I32P, c_int, I32P, I32P]),
+    "mgpbd_graph_coarsen": (_V, [c_int, I32P, I32P, I32P, c_int]),
+    "mgpbd_graph_nnz": (c_int64, [_V]),
+    "mgpbd_graph_copy": (c_int, [_V, I32P, I32P]),
+    "mgpbd_graph_free": (None, [_V]),
+    "mgpbd_galerkin_build": (_V, [c_int, I32P, I32P, F32P, I32P, c_int]),
+    "mgpbd_galerkin_sizes": (c_int, [_V, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "mgpbd_galerkin_copy": (c_int, [_V, I32P, I32P, I32P, I32P, I32P]),
+    "mgpbd_galerkin_free": (None, [_V]),
+    "mgpbd_vertex_incidence": (c_int, [c_int, c_int, I32P, I32P, I32P]),
+    "mgpbd_sell_layout": (c_int, [c_int, I32P, I32P]),
+    "mgpbd_sell_map": (c_int, [c_int, I32P, I32P, I32P]),
+    "mgpbd_sell_columns": (c_int, [c_int, I32P, I32P, I32P, I32P]),
+    # context
+    "mgpbd_ctx_create": (_V, [_V]),
+    "mgpbd_ctx_set_stream": (c_int, [_V, _V]),
+    "mgpbd_ctx_destroy": (None, [_V]),
+    "mgpbd_ctx_sync": (c_int, [_V]),
+    "mgpbd_ctx_profile": (c_int, [_V, c_int]),
+    "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
+    # per-tet / per-vertex
+    "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float, _V, _V, _V]),
+    "mgpbd_predict": (c_int, [_V, c_int, _V, _V, _V, _V, _V, c_float, c_float, c_float,
+                              c_float, c_float, c_int]),
+    "mgpbd_update_vel": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float]),
+    "mgpbd_constraints": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_fill_A": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_scatter": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_dual_norms": (c_int, [_V, c_int, _V, _V, _V, c_float, _V, _V, _V, _V, c_int, _V]),
+    "mgpbd_apply_dpos": (c_int, [_V, c_int, _V, _V, _V, c_float]),
+    # sparse / BLAS-1
+    "mgpbd_spmv": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
+    "mgpbd_jacobi": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, c_float]),
+    "mgpbd_diag_inv": (c_int, [_V, c_int, _V, _V, _V]),
+    "mgpbd_coarse_refill": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
+    "mgpbd_restrict": (c_int, [_V, c_int, _V, _V, _V, _V]),
+    "mgpbd_prolong": (c_int, [_V, c_int, _V, _V, _V]),
+    "mgpbd_dot": (c_int, [_V, c_int, _V, _V, _V]),
+    "mgpbd_axpy": (c_int, [_V, c_int, c_float, _V, _V]),
+    "mgpbd_xpay": (c_int, [_V, c_int, c_float, _V, _V]),
+    "mgpbd_copy": (c_int, [_V, c_int, _V, _V]),
+    "mgpbd_fill": (c_int, [_V, c_int, c_float, _V]),
+    "mgpbd_scale": (c_int, [_V, c_int, c_float, _V]),
+    "mgpbd_negate_and_add": (c_int, [_V, c_int, _V, _V]),
+    # multigrid / PCG
+    "mgpbd_mg_configure": (c_int, [_V, c_int, c_int, c_float, c_int, c_int]),
+    "mgpbd_mg_set_level": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, _V, _V, _V, c_int]),
+    "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
+    "mgpbd_pcg_solve": (c_int, [_V, _V, _V, _V, c_int, c_double, c_int,
+                                ctypes.POINTER(c_int), F64P]),
+}
+
+_lib = None
+
+
+class MGPBDError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libmgpbd.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MGPBDError(f"{LIB_PATH} is missing: build it with "
+                         "`python -m controllable_pbd_3d_amd.build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = load().mgpbd_last_error().decode(errors="replace")
+        raise MGPBDError(f"{what or 'libmgpbd'} failed ({status}): {msg}")
+
+
+def np_ptr(a, P=I32P):
+    return a.ctypes.data_as(P)
+
+
+def dptr(t) -> int | None:
+    """Device address of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()

@@ -1,0 +1,1177 @@
+// gfx950 kernels of the MGPBD global XPBD solve and their C-ABI launchers.
+//
+// Layout in HBM (one GPU):
+//   positions / velocities      float[3 * n_verts]   (AoS, the caller's (n, 3) arrays)
+//   t_i                         int32[4 * n_tets]    (loaded as int4)
+//   Dm_inv                      float[9 * n_tets]    row-major 3x3
+//   gradC                       float[12 * n_tets]   4 vertex gradients x 3
+//   level matrices              SELL-64: per 64-row slice, entry k of row 64s+l at
+//                               slice_ptr[s] + 64k + l; int32 columns, float values
+//
+// Arithmetic conventions shared with the CPU restatement in oracle/: products and sums are
+// written out with explicit fmaf where the order matters, so the sparse kernels (SpMV,
+// Jacobi, refill, restrict, prolong) and the A fill produce identical FP32 bits on both
+// sides for identical inputs. Reductions accumulate in FP64 with a fixed tree order
+// (deterministic run to run).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mgpbd_internal.h"
+#include "svd3.h"
+
+namespace mgpbd {
+
+constexpr int kBlock = 256;           // 4 wavefronts of 64
+constexpr int kMaxRedBlocks = 2048;   // grid cap for reduction kernels (partials size)
+
+// ------------------------------------------------------------------------------------------
+// Deterministic FP64 reductions: block tree -> per-block partial -> the last block to arrive
+// (atomic ticket) sums the partials in a fixed order and runs the scalar epilogue.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Returns the block total in thread 0 (other threads: unspecified).
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[kBlock / 64];
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return s;
+}
+
+struct RedWS {
+  double* partials;       // kMaxRedBlocks * 4
+  unsigned int* ticket;   // 1 counter, 0 between kernels
+};
+
+// Thread 0 of every block stores its partial(s); returns true (in all threads) in the last
+// block to finish, after which the partials of all blocks are visible.
+__device__ __forceinline__ bool grid_arrive(RedWS ws) {
+  __shared__ bool last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned int prev = atomicAdd(ws.ticket, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ __forceinline__ double grid_total(RedWS ws, int slot) {
+  const double* p = ws.partials + (size_t)slot * kMaxRedBlocks;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+    s += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return block_sum(s);
+}
+
+__device__ __forceinline__ void grid_reset(RedWS ws) {
+  if (threadIdx.x == 0) __hip_atomic_store(ws.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_partial(RedWS ws, int slot, double v) {
+  if (threadIdx.x == 0) ws.partials[(size_t)slot * kMaxRedBlocks + blockIdx.x] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Per-tet kernels
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void load3(const float* __restrict__ p, int v, float* o) {
+  o[0] = p[3 * (size_t)v + 0];
+  o[1] = p[3 * (size_t)v + 1];
+  o[2] = p[3 * (size_t)v + 2];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_rest_shape(int n, const int4* __restrict__ t_i, const float* __restrict__ xr,
+             const float* __restrict__ mu_t, float mu, float dt, float* __restrict__ Dm_inv,
+             float* __restrict__ vol_out, float* __restrict__ alpha) {
+  int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  int4 tv = t_i[t];
+  float r0[3], r1[3], r2[3], r3[3];
+  load3(xr, tv.x, r0); load3(xr, tv.y, r1); load3(xr, tv.z, r2); load3(xr, tv.w, r3);
+  float D[3][3];   // Dm = [r1-r0, r2-r0, r3-r0] as columns; D[row][col]
+  for (int r = 0; r < 3; ++r) {
+    D[r][0] = r1[r] - r0[r];
+    D[r][1] = r2[r] - r0[r];
+    D[r][2] = r3[r] - r0[r];
+  }
+  float c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+  float c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+  float c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+  float det = D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02;
+  float id = 1.0f / det;
+  float* o = Dm_inv + 9 * (size_t)t;
+  // inverse = adjugate / det, adj[i][j] = cofactor[j][i]
+  o[0] = c00 * id;
+  o[1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) * id;
+  o[2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) * id;
+  o[3] = c01 * id;
+  o[4] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
+  o[5] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
+  o[6] = c02 * id;
+  o[7] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
+  o[8] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
+  float vol = fabsf(det) / 6.0f;
+  vol_out[t] = vol;
+  float m = mu_t ? mu_t[t] : mu;
+  alpha[t] = 1.0f / (m * dt * dt * vol);
+}
+
+// F = Ds Dm^-1 for tet t at positions x (+ omega * dx when dx != nullptr).
+__device__ __forceinline__ void deformation_gradient(const int4 tv, const float* __restrict__ x,
+                                                     const float* __restrict__ dx, float omega,
+                                                     const float* __restrict__ B, float F[9]) {
+  float p[4][3];
+  int vs[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    load3(x, vs[l], p[l]);
+    if (dx) {
+      float d[3];
+      load3(dx, vs[l], d);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) p[l][r] = p[l][r] + omega * d[r];
+    }
+  }
+  float Ds[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    Ds[r][0] = p[1][r] - p[0][r];
+    Ds[r][1] = p[2][r] - p[0][r];
+    Ds[r][2] = p[3][r] - p[0][r];
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      F[3 * r + c] = fmaf(Ds[r][2], B[6 + c], fmaf(Ds[r][1], B[3 + c], Ds[r][0] * B[c]));
+}
+
+__device__ __forceinline__ float arap_C(const float s[3]) {
+  float a = s[0] - 1.0f, b = s[1] - 1.0f, c = s[2] - 1.0f;
+  return sqrtf(fmaf(c, c, fmaf(b, b, a * a)));
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
+              const float* __restrict__ Dm_inv, const float* __restrict__ alpha,
+              const float* __restrict__ lambda, float* __restrict__ C_out,
+              float* __restrict__ gradC, float* __restrict__ rhs, RedWS ws,
+              double* __restrict__ norm2) {
+  double acc = 0.0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    int4 tv = t_i[t];
+    float B[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
+    float F[9];
+    deformation_gradient(tv, x, nullptr, 0.0f, B, F);
+    float U[3][3], s[3], V[3][3];
+    svd3(F, U, s, V);
+    float C = arap_C(s);
+    float g[12];
+    if (C > 1e-6f) {
+      float d0 = (s[0] - 1.0f) / C, d1 = (s[1] - 1.0f) / C, d2 = (s[2] - 1.0f) / C;
+      float G[3][3];   // dC/dF = U diag(d) V^T
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          G[r][c] = fmaf(U[2][r] * V[2][c], d2, fmaf(U[1][r] * V[1][c], d1, U[0][r] * V[0][c] * d0));
+      // g_k = G B^T[:, k-1] (k = 1..3), g_0 = -(g_1 + g_2 + g_3)
+#pragma unroll
+      for (int k = 1; k <= 3; ++k)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          g[3 * k + r] = fmaf(G[r][2], B[3 * (k - 1) + 2],
+                              fmaf(G[r][1], B[3 * (k - 1) + 1], G[r][0] * B[3 * (k - 1)]));
+#pragma unroll
+      for (int r = 0; r < 3; ++r) g[r] = -g[3 + r] - g[6 + r] - g[9 + r];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) g[k] = 0.0f;
+    }
+    C_out[t] = C;
+    float4* go = reinterpret_cast<float4*>(gradC + 12 * (size_t)t);
+    go[0] = make_float4(g[0], g[1], g[2], g[3]);
+    go[1] = make_float4(g[4], g[5], g[6], g[7]);
+    go[2] = make_float4(g[8], g[9], g[10], g[11]);
+    float b = -(C + alpha[t] * lambda[t]);
+    rhs[t] = b;
+    acc += (double)b * (double)b;
+  }
+  double bs = block_sum(acc);
+  store_partial(ws, 0, bs);
+  if (grid_arrive(ws)) {
+    double tot = grid_total(ws, 0);
+    if (threadIdx.x == 0) *norm2 = tot;
+    grid_reset(ws);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
+             const float* __restrict__ dx, float omega, const float* __restrict__ Dm_inv,
+             const float* __restrict__ alpha, const float* __restrict__ lambda,
+             const float* __restrict__ C_cur, int with_current, RedWS ws,
+             double* __restrict__ out) {
+  double acc_cur = 0.0, acc_trial = 0.0;
+  for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    int4 tv = t_i[t];
+    float B[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
+    float F[9];
+    deformation_gradient(tv, x, dx, omega, B, F);
+    float U[3][3], s[3], V[3][3];
+    svd3(F, U, s, V);
+    float c = arap_C(s);
+    float al = alpha[t] * lambda[t];
+    float r = -(c + al);
+    acc_trial += (double)r * (double)r;
+    if (with_current) {
+      float rc = -(C_cur[t] + al);
+      acc_cur += (double)rc * (double)rc;
+    }
+  }
+  double b0 = block_sum(acc_cur);
+  double b1 = block_sum(acc_trial);
+  store_partial(ws, 0, b0);
+  store_partial(ws, 1, b1);
+  if (grid_arrive(ws)) {
+    double t0 = grid_total(ws, 0);
+    double t1 = grid_total(ws, 1);
+    if (threadIdx.x == 0) {
+      if (with_current) out[0] = t0;
+      out[1] = t1;
+    }
+    grid_reset(ws);
+  }
+}
+
+// A values, one thread per row (tet) i; the row's entries are column-adjacent in SELL-64 so
+// every store of a wavefront is one contiguous 256-byte segment.
+__global__ void __launch_bounds__(kBlock)
+k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+         const int* __restrict__ row_len, const int4* __restrict__ t_i,
+         const float* __restrict__ invm, const float* __restrict__ gradC,
+         const float* __restrict__ alpha, float* __restrict__ val, float* __restrict__ dinv) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int base = slice_ptr[i >> 6] + (i & 63);
+  const int len = row_len[i];
+  const int4 ti = t_i[i];
+  const int vi[4] = {ti.x, ti.y, ti.z, ti.w};
+  float w[4], gi[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    w[a] = invm[vi[a]];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) gi[a][r] = gradC[12 * (size_t)i + 3 * a + r];
+  }
+  float diag = 0.0f;
+  for (int k = 0; k < len; ++k) {
+    const int p = base + 64 * k;
+    const int j = idx[p];
+    float v;
+    if (j == i) {
+      v = w[0] * dot3(gi[0], gi[0]);
+      v = fmaf(w[1], dot3(gi[1], gi[1]), v);
+      v = fmaf(w[2], dot3(gi[2], gi[2]), v);
+      v = fmaf(w[3], dot3(gi[3], gi[3]), v);
+      v = v + alpha[i];
+      diag = v;
+    } else {
+      const int4 tj = t_i[j];
+      const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
+      const float* gj = gradC + 12 * (size_t)j;
+      v = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (vi[a] == vj[b]) {
+            float g[3] = {gj[3 * b], gj[3 * b + 1], gj[3 * b + 2]};
+            v = fmaf(w[a], dot3(gi[a], g), v);
+          }
+    }
+    val[p] = v;
+  }
+  dinv[i] = fabsf(diag) > 1e-30f ? 1.0f / diag : 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_scatter(int nv, const int* __restrict__ vptr, const int* __restrict__ vinc,
+          const float* __restrict__ invm, const float* __restrict__ gradC,
+          const float* __restrict__ dlam, float* __restrict__ dpos) {
+  int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nv) return;
+  const float w = invm[v];
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  for (int k = vptr[v]; k < vptr[v + 1]; ++k) {
+    const int e = vinc[k];
+    const int t = e >> 2;
+    const float s = w * dlam[t];
+    const float* g = gradC + 12 * (size_t)t + 3 * (e & 3);
+    a0 = fmaf(s, g[0], a0);
+    a1 = fmaf(s, g[1], a1);
+    a2 = fmaf(s, g[2], a2);
+  }
+  dpos[3 * (size_t)v + 0] = a0;
+  dpos[3 * (size_t)v + 1] = a1;
+  dpos[3 * (size_t)v + 2] = a2;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_add_inplace(int n, const float* __restrict__ d, float* __restrict__ y) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] = y[i] + d[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_apply_dpos(int nv, float* __restrict__ x, const float* __restrict__ dpos,
+             const float* __restrict__ invm, float omega) {
+  int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nv || !(invm[v] > 0.0f)) return;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) x[3 * (size_t)v + r] = fmaf(omega, dpos[3 * (size_t)v + r], x[3 * (size_t)v + r]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_predict(int nv, float* __restrict__ x, float* __restrict__ vel, float* __restrict__ x_old,
+          const float* __restrict__ invm, const float* __restrict__ fext, float gx, float gy,
+          float gz, float dt, float damp, int hold_pinned) {
+  int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= nv) return;
+  const float g[3] = {gx, gy, gz};
+  const bool pinned = hold_pinned && !(invm[v] > 0.0f);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    size_t q = 3 * (size_t)v + r;
+    float xv = x[q];
+    if (pinned) {
+      vel[q] = 0.0f;
+      x_old[q] = xv;
+      continue;
+    }
+    float vv = vel[q] + dt * g[r];
+    if (fext) vv = vv + dt * invm[v] * fext[q];
+    vv = vv * damp;
+    vel[q] = vv;
+    x_old[q] = xv;
+    x[q] = xv + dt * vv;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_update_vel(int nv, const float* __restrict__ x, const float* __restrict__ x_old,
+             float* __restrict__ vel, float dt, float damp) {
+  int q = blockIdx.x * kBlock + threadIdx.x;
+  if (q >= 3 * nv) return;
+  vel[q] = damp * (x[q] - x_old[q]) / dt;
+}
+
+// ------------------------------------------------------------------------------------------
+// SELL-64 sparse kernels. One thread per row; the wavefront owns one slice, so every column
+// and value load is one 256-byte coalesced segment and rows are summed sequentially in CSR
+// order (the order of the reference's row-per-thread kernels).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float sell_row(int i, const int* __restrict__ slice_ptr,
+                                          const int* __restrict__ idx,
+                                          const float* __restrict__ val,
+                                          const float* __restrict__ x) {
+  const int s = i >> 6;
+  const int p0 = slice_ptr[s] + (i & 63);
+  const int w = (slice_ptr[s + 1] - slice_ptr[s]) >> 6;
+  float acc = 0.0f;
+#pragma unroll 8
+  for (int k = 0; k < w; ++k) {
+    const int p = p0 + 64 * k;
+    acc = fmaf(val[p], x[idx[p]], acc);
+  }
+  return acc;
+}
+
+// MODE 0: y = A x;  MODE 1: y = x + (omega dinv) (b - A x);  MODE 2: y = b - A x
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_sell(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+       const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y,
+       const float* __restrict__ dinv, const float* __restrict__ b, float omega,
+       const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const float ax = sell_row(i, slice_ptr, idx, val, x);
+  if (i >= n) return;
+  if (MODE == 0) y[i] = ax;
+  else if (MODE == 1) y[i] = fmaf(omega * dinv[i], b[i] - ax, x[i]);
+  else y[i] = b[i] - ax;
+}
+
+// Several Jacobi sweeps on a diagonal-only matrix in one pass (rows are independent, so this
+// is the same arithmetic as `sweeps` separate launches).
+__global__ void __launch_bounds__(kBlock)
+k_jacobi_diag(int n, const int* __restrict__ slice_ptr, const float* __restrict__ val,
+              const float* __restrict__ dinv, const float* __restrict__ b,
+              float* __restrict__ x, float omega, int sweeps, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float a = val[slice_ptr[i >> 6] + (i & 63)];
+  const float wd = omega * dinv[i];
+  const float bi = b[i];
+  float xi = x[i];
+  for (int s = 0; s < sweeps; ++s) xi = fmaf(wd, bi - fmaf(a, xi, 0.0f), xi);
+  x[i] = xi;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_diag_inv(int n, const int* __restrict__ diag_pos, const float* __restrict__ val,
+           float* __restrict__ dinv) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  int p = diag_pos[i];
+  float d = p >= 0 ? val[p] : 0.0f;
+  dinv[i] = fabsf(d) > 1e-30f ? 1.0f / d : 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_refill(int nc, const int* __restrict__ off, const int* __restrict__ fpos,
+         const int* __restrict__ cpos, const float* __restrict__ fine, float* __restrict__ coarse) {
+  int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  float v = 0.0f;
+  for (int k = off[c]; k < off[c + 1]; ++k) v = v + fine[fpos[k]];
+  coarse[cpos[c]] = v;
+}
+
+// b_c[I] = sum of r over the members of aggregate I (ascending); with dinv_c given, also the
+// coarse level's first Jacobi sweep from zero, x_c = (omega dinv_c) b_c.
+__global__ void __launch_bounds__(kBlock)
+k_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
+           const float* __restrict__ r, float* __restrict__ bc, const float* __restrict__ dinv_c,
+           float* __restrict__ xc, float omega, const int* __restrict__ done) {
+  if (done && *done) return;
+  int I = blockIdx.x * kBlock + threadIdx.x;
+  if (I >= nc) return;
+  float s = 0.0f;
+  for (int m = mptr[I]; m < mptr[I + 1]; ++m) s = s + r[mem[m]];
+  bc[I] = s;
+  if (xc) xc[I] = (omega * dinv_c[I]) * s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_prolong(int n, const int* __restrict__ agg, const float* __restrict__ xc,
+          float* __restrict__ x, const int* __restrict__ done) {
+  if (done && *done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = x[i] + xc[agg[i]];
+}
+
+// x = (omega dinv) b : the first Jacobi sweep from x = 0
+__global__ void __launch_bounds__(kBlock)
+k_first_sweep(int n, const float* __restrict__ dinv, const float* __restrict__ b,
+              float* __restrict__ x, float omega, const int* __restrict__ done) {
+  if (done && *done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = (omega * dinv[i]) * b[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// BLAS-1
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_blas1(int op, int n, float alpha, const float* __restrict__ a, float* __restrict__ y) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  switch (op) {
+    case 0: y[i] = fmaf(alpha, a[i], y[i]); break;      // axpy  y += alpha x
+    case 1: y[i] = fmaf(alpha, y[i], a[i]); break;      // xpay  y = x + alpha y
+    case 2: y[i] = a[i]; break;                         // copy
+    case 3: y[i] = alpha; break;                        // fill
+    case 4: y[i] = y[i] * alpha; break;                 // scale
+    case 5: y[i] = a[i] - y[i]; break;                  // negate_and_add  r = b - r
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_dot(int n, const float* __restrict__ a, const float* __restrict__ b, RedWS ws,
+      double* __restrict__ out) {
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    acc += (double)a[i] * (double)b[i];
+  double bs = block_sum(acc);
+  store_partial(ws, 0, bs);
+  if (grid_arrive(ws)) {
+    double tot = grid_total(ws, 0);
+    if (threadIdx.x == 0) *out = tot;
+    grid_reset(ws);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// PCG (cons/mgpcg_gpu.py:261-329) with the scalars kept on the device.
+// ------------------------------------------------------------------------------------------
+struct PcgState {
+  double rz, pAp, alpha, beta, rr, bb, tol2, tol;
+  int done;     // 0 running, 1 converged, 2 |pAp| < 1e-30, 3 |rz| < 1e-30
+  int iters;    // completed iterations (the reference's cg_iters)
+  int k;        // iteration index
+  int relative;
+};
+
+__global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
+  st->done = 0; st->iters = 0; st->k = 0; st->relative = relative; st->tol = tol;
+  st->rz = st->pAp = st->alpha = st->beta = st->rr = st->bb = 0.0;
+  st->tol2 = tol * tol;
+}
+
+// r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), bb = b.b
+__global__ void __launch_bounds__(kBlock)
+k_pcg_init(int n, const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
+           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, RedWS ws,
+           PcgState* st) {
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    float bi = b[i];
+    r[i] = bi;
+    x[i] = 0.0f;
+    if (xv) xv[i] = (omega * dinv0[i]) * bi;
+    acc += (double)bi * (double)bi;
+  }
+  double bs = block_sum(acc);
+  store_partial(ws, 0, bs);
+  if (grid_arrive(ws)) {
+    double tot = grid_total(ws, 0);
+    if (threadIdx.x == 0) {
+      st->bb = tot;
+      if (st->relative) st->tol2 = st->tol * st->tol * tot;
+    }
+    grid_reset(ws);
+  }
+}
+
+// MODE 0: rz = r.z (initial);  MODE 1: rz_new = r.z, breakdown test on the previous rz, beta.
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_pcg_rz(int n, const float* __restrict__ r, const float* __restrict__ z, RedWS ws,
+         PcgState* st) {
+  if (st->done) return;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    acc += (double)r[i] * (double)z[i];
+  double bs = block_sum(acc);
+  store_partial(ws, 0, bs);
+  if (grid_arrive(ws)) {
+    double tot = grid_total(ws, 0);
+    if (threadIdx.x == 0) {
+      if (MODE == 0) {
+        st->rz = tot;
+      } else {
+        st->iters = st->k + 1;
+        if (fabs(st->rz) < 1e-30) {
+          st->done = 3;
+        } else {
+          st->beta = tot / st->rz;
+          st->rz = tot;
+          st->k = st->k + 1;
+        }
+      }
+    }
+    grid_reset(ws);
+  }
+}
+
+// Ap = A p, pAp = p.Ap, alpha = rz / pAp
+__global__ void __launch_bounds__(kBlock)
+k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
+               const int* __restrict__ idx, const float* __restrict__ val,
+               const float* __restrict__ p, float* __restrict__ Ap, RedWS ws, PcgState* st) {
+  if (st->done) return;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; (i >> 6) < n_slices; i += gridDim.x * kBlock) {
+    float ax = sell_row(i, slice_ptr, idx, val, p);
+    if (i < n) {
+      Ap[i] = ax;
+      acc += (double)p[i] * (double)ax;
+    }
+  }
+  double bs = block_sum(acc);
+  store_partial(ws, 0, bs);
+  if (grid_arrive(ws)) {
+    double tot = grid_total(ws, 0);
+    if (threadIdx.x == 0) {
+      st->pAp = tot;
+      if (fabs(tot) < 1e-30) st->done = 2;
+      else st->alpha = st->rz / tot;
+    }
+    grid_reset(ws);
+  }
+}
+
+// x += alpha p, r -= alpha Ap, rr = r.r; also xv = (omega dinv0) r for the next V-cycle
+__global__ void __launch_bounds__(kBlock)
+k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
+             const float* __restrict__ Ap, const float* __restrict__ dinv0,
+             float* __restrict__ xv, float omega, RedWS ws, PcgState* st) {
+  if (st->done) return;
+  const float al = (float)st->alpha;
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    x[i] = fmaf(al, p[i], x[i]);
+    float ri = fmaf(-al, Ap[i], r[i]);
+    r[i] = ri;
+    if (xv) xv[i] = (omega * dinv0[i]) * ri;
+    acc += (double)ri * (double)ri;
+  }
+  double bs = block_sum(acc);
+  store_partial(ws, 0, bs);
+  if (grid_arrive(ws)) {
+    double tot = grid_total(ws, 0);
+    if (threadIdx.x == 0) {
+      st->rr = tot;
+      if (tot < st->tol2) {
+        st->done = 1;
+        st->iters = st->k + 1;
+      }
+    }
+    grid_reset(ws);
+  }
+}
+
+// p = z + beta p  (MODE 0: p = z)
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_pcg_p(int n, const float* __restrict__ z, float* __restrict__ p, const PcgState* st) {
+  if (st->done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (MODE == 0) p[i] = z[i];
+  else p[i] = fmaf((float)st->beta, p[i], z[i]);
+}
+
+}  // namespace mgpbd
+
+using namespace mgpbd;
+
+// ============================================================================================
+// Context
+// ============================================================================================
+struct MgLevel {
+  int n = 0, n_slices = 0;
+  const int* slice_ptr = nullptr;
+  const int* idx = nullptr;
+  const float* val = nullptr;
+  const float* dinv = nullptr;
+  const int* agg = nullptr;
+  const int* mptr = nullptr;
+  const int* mem = nullptr;
+  float* work = nullptr;   // b, x, x', r
+  int diagonal = 0;
+};
+
+struct mgpbd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  RedWS ws{};
+  PcgState* st = nullptr;
+  PcgState* st_host = nullptr;   // pinned mirror
+  int n_levels = 0, n_smooth = 2, coarse_mode = 0, coarse_sweeps = 50;
+  float omega = 2.0f / 3.0f;
+  MgLevel L[MGPBD_MAX_LEVELS];
+  int poll = 2;
+  // optional per-kernel timing with HIP events on the context stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev;     // 2 per recorded launch
+  std::vector<int> ev_kind;
+  int ev_used = 0;
+  double prof_ms[MGPBD_PROF_KINDS] = {};
+  int64_t prof_cnt[MGPBD_PROF_KINDS] = {};
+};
+
+static int prof_flush(mgpbd_ctx* c) {
+  if (c->ev_used == 0) return 0;
+  MGPBD_HIP_TRY(hipEventSynchronize(c->ev[2 * (c->ev_used - 1) + 1]));
+  for (int i = 0; i < c->ev_used; ++i) {
+    float ms = 0.f;
+    MGPBD_HIP_TRY(hipEventElapsedTime(&ms, c->ev[2 * i], c->ev[2 * i + 1]));
+    c->prof_ms[c->ev_kind[i]] += ms;
+    c->prof_cnt[c->ev_kind[i]] += 1;
+  }
+  c->ev_used = 0;
+  return 0;
+}
+
+// Brackets one launch with events when profiling is on; returns the slot or -1.
+static int prof_begin(mgpbd_ctx* c, int kind) {
+  if (!c->prof) return -1;
+  if (c->ev_used * 2 + 2 > (int)c->ev.size()) {
+    if (prof_flush(c)) return -1;
+  }
+  int i = c->ev_used++;
+  c->ev_kind[i] = kind;
+  (void)hipEventRecord(c->ev[2 * i], c->stream);
+  return i;
+}
+static void prof_end(mgpbd_ctx* c, int i) {
+  if (i >= 0) (void)hipEventRecord(c->ev[2 * i + 1], c->stream);
+}
+
+static inline int nblocks(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+static inline int red_blocks(int64_t n) { return std::min(kMaxRedBlocks, nblocks(n)); }
+
+extern "C" {
+
+mgpbd_ctx* mgpbd_ctx_create(void* stream) {
+  auto* c = new (std::nothrow) mgpbd_ctx;
+  if (!c) { mgpbd_set_error("mgpbd_ctx_create: out of memory"); return nullptr; }
+  c->stream = (hipStream_t)stream;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipMalloc(&c->ws.partials, sizeof(double) * kMaxRedBlocks * 4);
+  if (e == hipSuccess) e = hipMalloc(&c->ws.ticket, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMemset(c->ws.ticket, 0, sizeof(unsigned int));
+  if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(PcgState));
+  if (e == hipSuccess) e = hipMemset(c->st, 0, sizeof(PcgState));
+  if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(PcgState), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    mgpbd_set_error(std::string("mgpbd_ctx_create: ") + hipGetErrorString(e));
+    if (c->ws.partials) (void)hipFree(c->ws.partials);
+    if (c->ws.ticket) (void)hipFree(c->ws.ticket);
+    if (c->st) (void)hipFree(c->st);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+int mgpbd_ctx_set_stream(mgpbd_ctx* c, void* stream) {
+  if (!c) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_set_stream: null context");
+  c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+void mgpbd_ctx_destroy(mgpbd_ctx* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  (void)hipFree(c->ws.partials);
+  (void)hipFree(c->ws.ticket);
+  (void)hipFree(c->st);
+  (void)hipHostFree(c->st_host);
+  delete c;
+}
+
+int mgpbd_ctx_sync(mgpbd_ctx* c) {
+  if (!c) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_sync: null context");
+  MGPBD_HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mgpbd_ctx_profile(mgpbd_ctx* c, int enable) {
+  if (!c) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_profile: null context");
+  if (int e = prof_flush(c)) return e;
+  if (enable && c->ev.empty()) {
+    c->ev.resize(2 * 2048);
+    c->ev_kind.resize(2048);
+    for (auto& e : c->ev) MGPBD_HIP_TRY(hipEventCreate(&e));
+  }
+  c->prof = enable != 0;
+  for (int k = 0; k < MGPBD_PROF_KINDS; ++k) { c->prof_ms[k] = 0.0; c->prof_cnt[k] = 0; }
+  return 0;
+}
+
+int mgpbd_ctx_profile_read(mgpbd_ctx* c, double* ms, int64_t* count) {
+  if (!c || !ms || !count) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_profile_read: invalid arguments");
+  MGPBD_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (int e = prof_flush(c)) return e;
+  for (int k = 0; k < MGPBD_PROF_KINDS; ++k) { ms[k] = c->prof_ms[k]; count[k] = c->prof_cnt[k]; }
+  return 0;
+}
+
+#define CTX_CHECK(fn)                                                           \
+  if (!ctx) return mgpbd_fail(MGPBD_EINVAL, std::string(fn) + ": null context")
+
+int mgpbd_rest_shape(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_ref,
+                     const float* mu_t, float mu, float dt, float* Dm_inv, float* vol,
+                     float* alpha) {
+  CTX_CHECK("mgpbd_rest_shape");
+  if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_rest_shape: n_tets < 0");
+  if (n_tets == 0) return 0;
+  k_rest_shape<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, reinterpret_cast<const int4*>(t_i), v_ref, mu_t, mu, dt, Dm_inv, vol, alpha);
+  MGPBD_LAUNCH_CHECK("mgpbd_rest_shape");
+  return 0;
+}
+
+int mgpbd_predict(mgpbd_ctx* ctx, int n_verts, float* v_p, float* v_v, float* v_p_old,
+                  const float* invm, const float* fext, float gx, float gy, float gz, float dt,
+                  float damp, int hold_pinned) {
+  CTX_CHECK("mgpbd_predict");
+  if (n_verts <= 0) return n_verts < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_predict: n < 0") : 0;
+  k_predict<<<nblocks(n_verts), kBlock, 0, ctx->stream>>>(n_verts, v_p, v_v, v_p_old, invm,
+                                                          fext, gx, gy, gz, dt, damp, hold_pinned);
+  MGPBD_LAUNCH_CHECK("mgpbd_predict");
+  return 0;
+}
+
+int mgpbd_update_vel(mgpbd_ctx* ctx, int n_verts, const float* v_p, const float* v_p_old,
+                     float* v_v, float dt, float damp) {
+  CTX_CHECK("mgpbd_update_vel");
+  if (n_verts <= 0) return n_verts < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_update_vel: n < 0") : 0;
+  k_update_vel<<<nblocks(3LL * n_verts), kBlock, 0, ctx->stream>>>(n_verts, v_p, v_p_old, v_v, dt, damp);
+  MGPBD_LAUNCH_CHECK("mgpbd_update_vel");
+  return 0;
+}
+
+int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_p,
+                      const float* Dm_inv, const float* alpha, const float* lambda, float* C,
+                      float* gradC, float* rhs, double* norm2) {
+  CTX_CHECK("mgpbd_constraints");
+  if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints: n_tets < 0");
+  int pe = prof_begin(ctx, MGPBD_PROF_CONSTRAINTS);
+  k_constraints<<<red_blocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv, alpha, lambda, C, gradC, rhs,
+      ctx->ws, norm2);
+  prof_end(ctx, pe);
+  MGPBD_LAUNCH_CHECK("mgpbd_constraints");
+  return 0;
+}
+
+int mgpbd_fill_A(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr, const int32_t* idx,
+                 const int32_t* row_len, const int32_t* t_i, const float* invm,
+                 const float* gradC, const float* alpha, float* val, float* dinv) {
+  CTX_CHECK("mgpbd_fill_A");
+  if (n_tets <= 0) return n_tets < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_fill_A: n < 0") : 0;
+  int pe = prof_begin(ctx, MGPBD_PROF_FILL_A);
+  k_fill_A<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, slice_ptr, idx, row_len, reinterpret_cast<const int4*>(t_i), invm, gradC, alpha,
+      val, dinv);
+  prof_end(ctx, pe);
+  MGPBD_LAUNCH_CHECK("mgpbd_fill_A");
+  return 0;
+}
+
+int mgpbd_scatter(mgpbd_ctx* ctx, int n_tets, int n_verts, const int32_t* vptr,
+                  const int32_t* vinc, const float* invm, const float* gradC, const float* dlam,
+                  float* lambda, float* dpos) {
+  CTX_CHECK("mgpbd_scatter");
+  if (n_tets < 0 || n_verts < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_scatter: n < 0");
+  if (n_tets > 0) {
+    k_add_inplace<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(n_tets, dlam, lambda);
+    MGPBD_LAUNCH_CHECK("mgpbd_scatter(lambda)");
+  }
+  if (n_verts > 0) {
+    k_scatter<<<nblocks(n_verts), kBlock, 0, ctx->stream>>>(n_verts, vptr, vinc, invm, gradC, dlam, dpos);
+    MGPBD_LAUNCH_CHECK("mgpbd_scatter");
+  }
+  return 0;
+}
+
+int mgpbd_dual_norms(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_p,
+                     const float* dpos, float omega, const float* Dm_inv, const float* alpha,
+                     const float* lambda, const float* C, int with_current, double* out) {
+  CTX_CHECK("mgpbd_dual_norms");
+  if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_dual_norms: n < 0");
+  k_dual_norms<<<red_blocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, reinterpret_cast<const int4*>(t_i), v_p, dpos, omega, Dm_inv, alpha, lambda, C,
+      with_current, ctx->ws, out);
+  MGPBD_LAUNCH_CHECK("mgpbd_dual_norms");
+  return 0;
+}
+
+int mgpbd_apply_dpos(mgpbd_ctx* ctx, int n_verts, float* v_p, const float* dpos,
+                     const float* invm, float omega) {
+  CTX_CHECK("mgpbd_apply_dpos");
+  if (n_verts <= 0) return n_verts < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_apply_dpos: n < 0") : 0;
+  k_apply_dpos<<<nblocks(n_verts), kBlock, 0, ctx->stream>>>(n_verts, v_p, dpos, invm, omega);
+  MGPBD_LAUNCH_CHECK("mgpbd_apply_dpos");
+  return 0;
+}
+
+int mgpbd_spmv(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr, const int32_t* idx,
+               const float* val, const float* x, float* y) {
+  CTX_CHECK("mgpbd_spmv");
+  if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_spmv: n < 0") : 0;
+  int ns = (n + 63) / 64;
+  k_sell<0><<<nblocks(64LL * ns), kBlock, 0, ctx->stream>>>(n, ns, slice_ptr, idx, val, x, y,
+                                                           nullptr, nullptr, 0.f, nullptr);
+  MGPBD_LAUNCH_CHECK("mgpbd_spmv");
+  return 0;
+}
+
+int mgpbd_jacobi(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr, const int32_t* idx,
+                 const float* val, const float* dinv, const float* b, const float* x,
+                 float* x_new, float omega) {
+  CTX_CHECK("mgpbd_jacobi");
+  if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_jacobi: n < 0") : 0;
+  if (x == x_new) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_jacobi: x and x_new must differ");
+  int ns = (n + 63) / 64;
+  k_sell<1><<<nblocks(64LL * ns), kBlock, 0, ctx->stream>>>(n, ns, slice_ptr, idx, val, x, x_new,
+                                                           dinv, b, omega, nullptr);
+  MGPBD_LAUNCH_CHECK("mgpbd_jacobi");
+  return 0;
+}
+
+int mgpbd_diag_inv(mgpbd_ctx* ctx, int n, const int32_t* diag_pos, const float* val, float* dinv) {
+  CTX_CHECK("mgpbd_diag_inv");
+  if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_diag_inv: n < 0") : 0;
+  k_diag_inv<<<nblocks(n), kBlock, 0, ctx->stream>>>(n, diag_pos, val, dinv);
+  MGPBD_LAUNCH_CHECK("mgpbd_diag_inv");
+  return 0;
+}
+
+int mgpbd_coarse_refill(mgpbd_ctx* ctx, int nc, const int32_t* off, const int32_t* fpos,
+                        const int32_t* cpos, const float* fine, float* coarse) {
+  CTX_CHECK("mgpbd_coarse_refill");
+  if (nc <= 0) return nc < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_coarse_refill: n < 0") : 0;
+  int pe = prof_begin(ctx, MGPBD_PROF_REFILL);
+  k_refill<<<nblocks(nc), kBlock, 0, ctx->stream>>>(nc, off, fpos, cpos, fine, coarse);
+  prof_end(ctx, pe);
+  MGPBD_LAUNCH_CHECK("mgpbd_coarse_refill");
+  return 0;
+}
+
+int mgpbd_restrict(mgpbd_ctx* ctx, int nc, const int32_t* mptr, const int32_t* mem,
+                   const float* r, float* bc) {
+  CTX_CHECK("mgpbd_restrict");
+  if (nc <= 0) return nc < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_restrict: n < 0") : 0;
+  k_restrict<<<nblocks(nc), kBlock, 0, ctx->stream>>>(nc, mptr, mem, r, bc, nullptr, nullptr, 0.f, nullptr);
+  MGPBD_LAUNCH_CHECK("mgpbd_restrict");
+  return 0;
+}
+
+int mgpbd_prolong(mgpbd_ctx* ctx, int n, const int32_t* agg, const float* xc, float* x) {
+  CTX_CHECK("mgpbd_prolong");
+  if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_prolong: n < 0") : 0;
+  k_prolong<<<nblocks(n), kBlock, 0, ctx->stream>>>(n, agg, xc, x, nullptr);
+  MGPBD_LAUNCH_CHECK("mgpbd_prolong");
+  return 0;
+}
+
+static int blas1(mgpbd_ctx* ctx, const char* name, int op, int n, float alpha, const float* a,
+                 float* y) {
+  CTX_CHECK(name);
+  if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, std::string(name) + ": n < 0") : 0;
+  k_blas1<<<nblocks(n), kBlock, 0, ctx->stream>>>(op, n, alpha, a, y);
+  MGPBD_LAUNCH_CHECK(name);
+  return 0;
+}
+
+int mgpbd_axpy(mgpbd_ctx* ctx, int n, float alpha, const float* x, float* y) { return blas1(ctx, "mgpbd_axpy", 0, n, alpha, x, y); }
+int mgpbd_xpay(mgpbd_ctx* ctx, int n, float alpha, const float* x, float* y) { return blas1(ctx, "mgpbd_xpay", 1, n, alpha, x, y); }
+int mgpbd_copy(mgpbd_ctx* ctx, int n, const float* src, float* dst) { return blas1(ctx, "mgpbd_copy", 2, n, 0.f, src, dst); }
+int mgpbd_fill(mgpbd_ctx* ctx, int n, float value, float* x) { return blas1(ctx, "mgpbd_fill", 3, n, value, nullptr, x); }
+int mgpbd_scale(mgpbd_ctx* ctx, int n, float alpha, float* x) { return blas1(ctx, "mgpbd_scale", 4, n, alpha, nullptr, x); }
+int mgpbd_negate_and_add(mgpbd_ctx* ctx, int n, const float* b, float* r) { return blas1(ctx, "mgpbd_negate_and_add", 5, n, 0.f, b, r); }
+
+int mgpbd_dot(mgpbd_ctx* ctx, int n, const float* a, const float* b, double* out) {
+  CTX_CHECK("mgpbd_dot");
+  if (n < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_dot: n < 0");
+  k_dot<<<red_blocks(n), kBlock, 0, ctx->stream>>>(n, a, b, ctx->ws, out);
+  MGPBD_LAUNCH_CHECK("mgpbd_dot");
+  return 0;
+}
+
+// ============================================================================================
+// Multigrid V-cycle and PCG
+// ============================================================================================
+int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega, int coarse_mode,
+                       int coarse_sweeps) {
+  CTX_CHECK("mgpbd_mg_configure");
+  if (n_levels < 1 || n_levels > MGPBD_MAX_LEVELS || n_smooth < 0 || coarse_sweeps < 1 ||
+      coarse_mode != MGPBD_COARSE_JACOBI)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_configure: invalid arguments");
+  ctx->n_levels = n_levels;
+  ctx->n_smooth = n_smooth;
+  ctx->omega = omega;
+  ctx->coarse_mode = coarse_mode;
+  ctx->coarse_sweeps = coarse_sweeps;
+  for (int l = 0; l < MGPBD_MAX_LEVELS; ++l) ctx->L[l] = MgLevel{};
+  return 0;
+}
+
+int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* slice_ptr,
+                       const int32_t* idx, const float* val, const float* dinv,
+                       const int32_t* agg, const int32_t* mptr, const int32_t* mem,
+                       float* work, int diagonal_only) {
+  CTX_CHECK("mgpbd_mg_set_level");
+  if (level < 0 || level >= ctx->n_levels || n_rows <= 0 || !slice_ptr || !idx || !val ||
+      !dinv || !work)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level: invalid arguments");
+  bool coarsest = (level == ctx->n_levels - 1);
+  if (!coarsest && (!agg || !mptr || !mem))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level: transfer arrays required above the coarsest level");
+  MgLevel& L = ctx->L[level];
+  L.n = n_rows;
+  L.n_slices = (n_rows + 63) / 64;
+  L.slice_ptr = slice_ptr; L.idx = idx; L.val = val; L.dinv = dinv;
+  L.agg = agg; L.mptr = mptr; L.mem = mem; L.work = work;
+  L.diagonal = diagonal_only;
+  return 0;
+}
+
+static int check_levels(mgpbd_ctx* ctx) {
+  if (ctx->n_levels < 1) return mgpbd_fail(MGPBD_EINVAL, "multigrid: not configured");
+  for (int l = 0; l < ctx->n_levels; ++l)
+    if (ctx->L[l].n <= 0) return mgpbd_fail(MGPBD_EINVAL, "multigrid: level " + std::to_string(l) + " not set");
+  return 0;
+}
+
+// Jacobi sweeps on level l from the first-sweep result already in *cur; returns the buffer
+// holding the result (cur or the level's other x buffer).
+static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count, const int* done) {
+  MgLevel& L = ctx->L[l];
+  float* xa = L.work + (size_t)L.n;
+  float* xb = L.work + 2 * (size_t)L.n;
+  if (count <= 0) return cur;
+  if (L.diagonal) {
+    k_jacobi_diag<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.val, L.dinv, b,
+                                                            cur, ctx->omega, count, done);
+    return cur;
+  }
+  for (int s = 0; s < count; ++s) {
+    float* nxt = (cur == xa) ? xb : xa;
+    int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
+    k_sell<1><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
+        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, done);
+    prof_end(ctx, pe);
+    cur = nxt;
+  }
+  return cur;
+}
+
+// V-cycle on level l >= 0 for rhs b with the first pre-smooth sweep (x = omega D^-1 b, or
+// x = 0 when n_smooth == 0) already in x_first (one of the level's x buffers). Returns the
+// buffer holding the result.
+static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first, const int* done) {
+  MgLevel& L = ctx->L[l];
+  const bool coarsest = (l == ctx->n_levels - 1);
+  if (coarsest) {
+    int total = std::min(ctx->coarse_sweeps, L.n);
+    return sweeps(ctx, l, b, x_first, total - 1, done);
+  }
+  float* x = sweeps(ctx, l, b, x_first, ctx->n_smooth - 1, done);
+  float* r = L.work + 3 * (size_t)L.n;
+  MgLevel& C = ctx->L[l + 1];
+  // residual r = b - A x (x == 0 when n_smooth == 0: then r = b)
+  if (ctx->n_smooth > 0) {
+    int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
+    k_sell<2><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
+        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr, b, 0.f, done);
+    prof_end(ctx, pe);
+  } else {
+    k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, b, r);
+  }
+  // restrict + the coarse level's first sweep from zero
+  float* bc = C.work;
+  float* xc = C.work + (size_t)C.n;
+  const bool c_coarsest = (l + 1 == ctx->n_levels - 1);
+  const bool c_first = c_coarsest || ctx->n_smooth > 0;
+  k_restrict<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, L.mptr, L.mem, r, bc,
+                                                       C.dinv, c_first ? xc : nullptr,
+                                                       ctx->omega, done);
+  if (!c_first) k_blas1<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(3, C.n, 0.f, nullptr, xc);
+  float* xcr = vcycle_from(ctx, l + 1, bc, xc, done);
+  k_prolong<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.agg, xcr, x, done);
+  return sweeps(ctx, l, b, x, ctx->n_smooth, done);
+}
+
+int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
+  CTX_CHECK("mgpbd_mg_vcycle");
+  if (int e = check_levels(ctx)) return e;
+  MgLevel& L = ctx->L[0];
+  float* x1 = L.work + (size_t)L.n;
+  const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
+  if (first) k_first_sweep<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.dinv, b, x1, ctx->omega, nullptr);
+  else k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(3, L.n, 0.f, nullptr, x1);
+  float* res = vcycle_from(ctx, 0, b, x1, nullptr);
+  k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, res, x_out);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_vcycle");
+  return 0;
+}
+
+int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int maxiter,
+                    double tol, int relative, int* iters_out, double* info_out) {
+  CTX_CHECK("mgpbd_pcg_solve");
+  if (int e = check_levels(ctx)) return e;
+  if (!b || !x || !work || maxiter < 0)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_solve: invalid arguments");
+  MgLevel& L = ctx->L[0];
+  const int n = L.n;
+  float* r = work;                  // work[n, 2n) is reserved (unused)
+  float* p = work + 2 * (size_t)n;
+  float* Ap = work + 3 * (size_t)n;
+  float* xv = L.work + (size_t)n;   // level-0 x buffer receiving the first sweep
+  const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
+  const int* done = &ctx->st->done;
+  hipStream_t s = ctx->stream;
+
+  k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
+  k_pcg_init<<<red_blocks(n), kBlock, 0, s>>>(n, b, x, r, L.dinv, first ? xv : nullptr,
+                                              ctx->omega, ctx->ws, ctx->st);
+  if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
+  float* zr = vcycle_from(ctx, 0, r, xv, done);
+  k_pcg_rz<0><<<red_blocks(n), kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
+  k_pcg_p<0><<<nblocks(n), kBlock, 0, s>>>(n, zr, p, ctx->st);
+  MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
+
+  int issued = 0;
+  while (issued < maxiter) {
+    int chunk = std::min(ctx->poll, maxiter - issued);
+    for (int c = 0; c < chunk; ++c) {
+      int pe = prof_begin(ctx, MGPBD_PROF_SPMV_L0);
+      k_pcg_spmv_dot<<<red_blocks(64LL * L.n_slices), kBlock, 0, s>>>(
+          n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap, ctx->ws, ctx->st);
+      prof_end(ctx, pe);
+      k_pcg_update<<<red_blocks(n), kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr,
+                                                    ctx->omega, ctx->ws, ctx->st);
+      if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
+      float* zz = vcycle_from(ctx, 0, r, xv, done);
+      k_pcg_rz<1><<<red_blocks(n), kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
+      k_pcg_p<1><<<nblocks(n), kBlock, 0, s>>>(n, zz, p, ctx->st);
+    }
+    MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
+    issued += chunk;
+    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    if (int e = prof_flush(ctx)) return e;
+    if (ctx->st_host->done) break;
+  }
+  if (issued == 0) {
+    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    MGPBD_HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (iters_out) *iters_out = ctx->st_host->iters;
+  if (info_out) {
+    info_out[0] = ctx->st_host->rr;
+    info_out[1] = ctx->st_host->bb;
+    info_out[2] = (double)ctx->st_host->done;
+    info_out[3] = 0.0;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,287 @@
+"""MGPBD global XPBD solver on one MI355X: drop-in for the reference's ``MGPBDSolverGPU``
+(cons/mgpbd_gpu.py:20-446) -- same constructor, defaults, methods and attributes.
+
+Every kernel of the substep is libmgpbd (HIP, gfx950); Python runs the Newton loop exactly
+as the reference does (cons/mgpbd_gpu.py:397-446), reading back one scalar per Newton
+iteration plus the line-search norms.
+
+Reference behaviour kept by default (parity mode):
+* lambda is zeroed at the start of every substep and every PCG starts from dlambda = 0;
+* ``make_prediction`` moves every vertex, those with ``v_invm == 0`` included
+  (cons/mgpbd_gpu.py:206-213 has no invm guard); only the correction skips them;
+* tet aggregation gives at most two levels and the coarse Galerkin pattern is fixed by the
+  values of A at ``init`` (scipy drops entries that sum to 0.0);
+* the PCG stops on the absolute test ||r||^2 < tol_cg^2; the coarsest level gets
+  min(50, n) weighted-Jacobi sweeps.
+
+Keyword-only extensions (no reference counterpart; defaults keep the reference's
+behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
+('reference' | 'extended'), ``galerkin`` ('values' | 'structural'), ``max_coarse``, ``max_levels``,
+``f_ext`` (external vertex forces), ``device``.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, dptr, np_ptr
+from .mgpcg_gpu import GPUMultigridSolver
+from .runtime import get_context, to_device
+from .sparse_gpu import GPUSparseMatrix, gpu_fill_zero
+from . import tet_aggregation
+
+
+class _Scalar:
+    """Stands in for the reference's 0-d Taichi field (``self.damp[None]``)."""
+
+    def __init__(self, v: float):
+        self.value = float(v)
+
+    def __getitem__(self, key):
+        return self.value
+
+    def __setitem__(self, key, v):
+        self.value = float(v)
+
+
+class MGPBDSolverGPU:
+
+    def __init__(self, v_p, v_p_ref, v_invm, t_i, t_m, gravity, dt: float, mu: float = 1e6,
+                 damp: float = 1.0, maxiter: int = 20, atol: float = 1e-4, rtol: float = 1e-2,
+                 maxiter_cg: int = 100, tol_cg: float = 1e-5, n_smooth: int = 2,
+                 omega_jacobi: float = 2.0 / 3.0, setup_interval: int = 10000,
+                 use_line_search: bool = True, aggregation: str = "tet", benchmark=None, *,
+                 hold_pinned: bool = False, mu_per_tet=None, relative_tol: bool = False,
+                 hierarchy: str = "reference", galerkin: str = "values",
+                 max_coarse: int = 400, max_levels: int | None = None,
+                 coarse_sweeps: int = 50, f_ext=None,
+                 device=None, verbose: bool = True) -> None:
+        if aggregation != "tet":
+            raise NotImplementedError("aggregation='pyamg' (smoothed aggregation) is not "
+                                      "available; only the tet-aggregation hierarchy is built")
+        self.ctx = get_context(device)
+        dev = self.device = self.ctx.device
+        self.v_p = to_device(v_p, torch.float32, dev, (-1, 3))
+        self.v_p_ref = to_device(v_p_ref, torch.float32, dev, (-1, 3))
+        self.v_invm = to_device(v_invm, torch.float32, dev, (-1,))
+        self.t_i = to_device(t_i, torch.int32, dev, (-1,))
+        self.t_m = to_device(t_m, torch.float32, dev, (-1,))
+        self.n_verts = int(self.v_p.shape[0])
+        self.n_tets = int(self.t_i.shape[0] // 4)
+        g = gravity.to_numpy() if hasattr(gravity, "to_numpy") else np.asarray(gravity, np.float64)
+        self.gravity = tuple(float(x) for x in np.asarray(g).reshape(3))
+        self.dt = float(dt)
+        self.mu = float(mu)
+        self.maxiter = int(maxiter)
+        self.atol = float(atol)
+        self.rtol = float(rtol)
+        self.maxiter_cg = int(maxiter_cg)
+        self.tol_cg = float(tol_cg)
+        self.n_smooth = int(n_smooth)
+        self.omega_jacobi = float(omega_jacobi)
+        self.setup_interval = setup_interval      # unused, as in the reference GPU class
+        self.use_line_search = bool(use_line_search)
+        self.aggregation = aggregation
+        self.benchmark = benchmark
+        self.damp = _Scalar(damp)
+        # extensions
+        self.hold_pinned = bool(hold_pinned)
+        self.relative_tol = bool(relative_tol)
+        self.hierarchy = hierarchy
+        self.galerkin = galerkin
+        self.max_coarse = int(max_coarse)
+        self.max_levels = max_levels
+        self.coarse_sweeps = int(coarse_sweeps)
+        self.verbose = verbose
+        self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
+        self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
+
+        nv, nt = self.n_verts, self.n_tets
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.v_v = torch.zeros(nv, 3, **f32)
+        self.v_p_old = torch.zeros(nv, 3, **f32)
+        self.v_predict = torch.zeros(nv, 3, **f32)
+        self.dpos = torch.zeros(nv, 3, **f32)
+        self.Dm_inv = torch.zeros(nt, 3, 3, **f32)
+        self.rest_vol = torch.zeros(nt, **f32)
+        self.alpha_tilde = torch.zeros(nt, **f32)
+        self.lambdaf = torch.zeros(nt, **f32)
+        self.constraints = torch.zeros(nt, **f32)
+        self.gradC = torch.zeros(nt, 4, 3, **f32)
+        self._b_gpu = torch.zeros(nt, **f32)
+        self._dlam_gpu = torch.zeros(nt, **f32)
+        self._norms = torch.zeros(4, dtype=torch.float64, device=dev)
+        self._gpu_solver = None
+        self._amg_Ps = None
+        self.last_stats = []
+
+    # ------------------------------------------------------------------ setup
+    def init(self):
+        """Rest shape, pattern, hierarchy, multigrid setup (cons/mgpbd_gpu.py:92-114)."""
+        self._precompute_rest_shape()
+        self._build_sparsity_pattern()
+        self._alpha_tilde_np = self.alpha_tilde.cpu().numpy()
+        self._compute_C_and_gradC()
+        self._fill_A()
+        self._build_hierarchy()
+        self._gpu_solver = GPUMultigridSolver(
+            n_smooth=self.n_smooth, omega_jacobi=self.omega_jacobi,
+            maxiter_cg=self.maxiter_cg, tol_cg=self.tol_cg, relative=self.relative_tol,
+            coarse_sweeps=self.coarse_sweeps, galerkin=self.galerkin,
+            device=self.device.index)
+        self._gpu_solver.setup(self._A, self._aggs)
+        self._build_incidence()
+        if self.verbose:
+            print(f"MGPBD-GPU: {self.n_verts} verts, {self.n_tets} tets, {self._nnz} nnz, "
+                  f"aggregation={self.aggregation}, levels={self._gpu_solver._level_sizes}")
+
+    def _precompute_rest_shape(self):
+        self.ctx("mgpbd_rest_shape", self.n_tets, dptr(self.t_i), dptr(self.v_p_ref),
+                 dptr(self.mu_per_tet), self.mu, self.dt, dptr(self.Dm_inv),
+                 dptr(self.rest_vol), dptr(self.alpha_tilde))
+
+    def _build_sparsity_pattern(self):
+        """CSR pattern of A (cons/mgpbd_gpu.py:153-194) in the C++ setup code."""
+        lib = _lib.load()
+        t = self.t_i.cpu().numpy()
+        self._t_i_np = t
+        p = lib.mgpbd_pattern_build(self.n_tets, self.n_verts, np_ptr(t))
+        if not p:
+            check(-1, "mgpbd_pattern_build")
+        nnz = int(lib.mgpbd_pattern_nnz(p))
+        indptr = np.empty(self.n_tets + 1, np.int32)
+        indices = np.empty(max(nnz, 1), np.int32)
+        check(lib.mgpbd_pattern_copy(p, np_ptr(indptr), np_ptr(indices)), "mgpbd_pattern_copy")
+        lib.mgpbd_pattern_free(p)
+        self._A_indptr, self._A_indices = indptr, indices[:nnz]
+        self._nnz = nnz
+        self._A = GPUSparseMatrix(self.n_tets, nnz, device=self.device.index)
+        self._A.set_pattern(indptr, indices[:nnz], diag_csr_pos=indptr[1:] - 1)
+
+    def _build_hierarchy(self):
+        if self.hierarchy == "reference":
+            self._aggs = tet_aggregation.reference_aggregates(self._t_i_np, self.n_tets,
+                                                              max_coarse=self.max_coarse)
+        elif self.hierarchy == "extended":
+            self._aggs = tet_aggregation.build_extended_hierarchy(
+                self._t_i_np, self.n_tets, max_coarse=self.max_coarse,
+                max_levels=(self.max_levels - 1) if self.max_levels else 8)
+        else:
+            raise ValueError("hierarchy must be 'reference' or 'extended'")
+        self._amg_Ps = None   # built lazily by amg_Ps (scipy objects, for inspection)
+
+    @property
+    def amg_Ps(self):
+        if self._amg_Ps is None:
+            self._amg_Ps = [tet_aggregation.build_prolongation_from_aggregates(a, len(a), n)
+                            for a, n in self._aggs]
+        return self._amg_Ps
+
+    def _build_incidence(self):
+        lib = _lib.load()
+        vptr = np.empty(self.n_verts + 1, np.int32)
+        vinc = np.empty(max(4 * self.n_tets, 1), np.int32)
+        check(lib.mgpbd_vertex_incidence(self.n_tets, self.n_verts, np_ptr(self._t_i_np),
+                                         np_ptr(vptr), np_ptr(vinc)), "mgpbd_vertex_incidence")
+        self._vptr = torch.from_numpy(vptr).to(self.device)
+        self._vinc = torch.from_numpy(vinc).to(self.device)
+
+    # ------------------------------------------------------------------ time stepping
+    def step(self, frame: int = 0):
+        """predict -> solve -> velocity update (cons/mgpbd_gpu.py:200-204)."""
+        self.make_prediction()
+        self.solve(frame)
+        self.update_vel()
+
+    def make_prediction(self):
+        gx, gy, gz = self.gravity
+        self.ctx("mgpbd_predict", self.n_verts, dptr(self.v_p), dptr(self.v_v),
+                 dptr(self.v_p_old), dptr(self.v_invm), dptr(self.f_ext), gx, gy, gz,
+                 self.dt, self.damp[None], int(self.hold_pinned))
+
+    def update_vel(self):
+        self.ctx("mgpbd_update_vel", self.n_verts, dptr(self.v_p), dptr(self.v_p_old),
+                 dptr(self.v_v), self.dt, self.damp[None])
+
+    # ------------------------------------------------------------------ kernels
+    def _compute_C_and_gradC(self):
+        """C, gradC, rhs b = -(C + alpha lambda) and ||b||^2 in one pass."""
+        self.ctx("mgpbd_constraints", self.n_tets, dptr(self.t_i), dptr(self.v_p),
+                 dptr(self.Dm_inv), dptr(self.alpha_tilde), dptr(self.lambdaf),
+                 dptr(self.constraints), dptr(self.gradC), dptr(self._b_gpu), dptr(self._norms))
+
+    def _residual_norm(self) -> float:
+        return math.sqrt(float(self._norms[0].item()))
+
+    def _fill_A(self):
+        A = self._A
+        self.ctx("mgpbd_fill_A", self.n_tets, dptr(A.slice_ptr), dptr(A.sell_idx),
+                 dptr(A.row_len), dptr(self.t_i), dptr(self.v_invm), dptr(self.gradC),
+                 dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
+
+    def _fill_A_and_update_gpu(self):
+        self._fill_A()
+        self._gpu_solver.update_all_coarse_matrices()
+
+    def _scatter_dlam_gpu(self):
+        self.ctx("mgpbd_scatter", self.n_tets, self.n_verts, dptr(self._vptr), dptr(self._vinc),
+                 dptr(self.v_invm), dptr(self.gradC), dptr(self._dlam_gpu), dptr(self.lambdaf),
+                 dptr(self.dpos))
+
+    def _apply_dpos(self, omega: float):
+        self.ctx("mgpbd_apply_dpos", self.n_verts, dptr(self.v_p), dptr(self.dpos),
+                 dptr(self.v_invm), float(omega))
+
+    def _dual_norms(self, omega: float, with_current: bool):
+        self.ctx("mgpbd_dual_norms", self.n_tets, dptr(self.t_i), dptr(self.v_p),
+                 dptr(self.dpos), float(omega), dptr(self.Dm_inv), dptr(self.alpha_tilde),
+                 dptr(self.lambdaf), dptr(self.constraints), int(with_current),
+                 dptr(self._norms[1:3]))
+        return self._norms[1:3].tolist()
+
+    def _line_search(self, beta=0.5, max_steps=10):
+        """Backtracking on the dual residual norm (cons/mgpbd_gpu.py:379-391); the current
+        norm uses the C stored for this Newton iterate (same positions, same kernel)."""
+        cur2, trial2 = self._dual_norms(1.0, True)
+        omega = 1.0
+        for k in range(max_steps):
+            if k > 0:
+                _, trial2 = self._dual_norms(omega, False)
+            if trial2 < cur2:
+                return omega
+            omega *= beta
+        return omega
+
+    # ------------------------------------------------------------------ Newton loop
+    def solve(self, frame: int = 0):
+        """MGPBD outer loop (cons/mgpbd_gpu.py:397-446)."""
+        if self.benchmark is not None:
+            self.benchmark.begin_frame(frame)
+        self.lambdaf.zero_()
+        r0 = 0.0
+        stats = []
+        for ite in range(self.maxiter):
+            self._compute_C_and_gradC()
+            r = self._residual_norm()
+            if ite == 0:
+                r0 = r
+            if r < self.atol or (r0 > 1e-12 and r < self.rtol * r0):
+                stats.append((ite, r, 0, 1.0))
+                if self.benchmark is not None:
+                    self.benchmark.record_iteration(ite, r)
+                break
+            self._fill_A_and_update_gpu()
+            cg_iters = self._gpu_solver.solve(self._b_gpu, self._dlam_gpu)
+            self._scatter_dlam_gpu()
+            omega = self._line_search() if self.use_line_search else 1.0
+            self._apply_dpos(omega)
+            stats.append((ite, r, cg_iters, omega))
+            if self.benchmark is not None:
+                self.benchmark.record_iteration(ite, r, cg_iters=cg_iters, omega=omega)
+        self.last_stats = stats
+        if self.benchmark is not None:
+            self.benchmark.end_frame()

@@ -1,0 +1,201 @@
+"""Multigrid-preconditioned CG on the GPU, same interface as the reference's
+``GPUMultigridSolver`` (cons/mgpcg_gpu.py:18-329).
+
+Differences in mechanism, not in results:
+* coarse patterns and refill maps come from libmgpbd's O(nnz) C++ Galerkin builder, which
+  reproduces scipy's ``P^T A P`` pattern (including the entries it drops because they sum to
+  0.0 at setup) and the reference's contribution order;
+* the PCG keeps alpha, beta and the stopping test on the device (FP64 reductions) and the
+  host polls a done flag every few iterations instead of syncing three times per iteration;
+* Jacobi ping-pongs two buffers instead of copying x_new back to x.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib
+from ._lib import check, dptr, np_ptr
+from .runtime import get_context
+from .sparse_gpu import GPUSparseMatrix
+
+
+def _members(agg: np.ndarray, n_c: int):
+    order = np.argsort(agg, kind="stable").astype(np.int32)
+    ptr = np.zeros(n_c + 1, np.int32)
+    ptr[1:] = np.cumsum(np.bincount(agg, minlength=n_c))
+    return ptr, order
+
+
+def _aggs_from_Ps(Ps):
+    """Piecewise-constant unit P (one 1.0 per row) -> aggregate maps."""
+    out = []
+    for P in Ps:
+        P = sp.csr_matrix(P)
+        if not (np.all(np.diff(P.indptr) == 1) and np.all(P.data == 1.0)):
+            raise ValueError("GPUMultigridSolver supports piecewise-constant P with unit "
+                             "entries (one 1.0 per row), as tet aggregation builds")
+        out.append((P.indices.astype(np.int32), P.shape[1]))
+    return out
+
+
+class GPUMultigridSolver:
+    """Multigrid-preconditioned CG solver, fully GPU-resident (cons/mgpcg_gpu.py:18)."""
+
+    def __init__(self, n_smooth: int = 2, omega_jacobi: float = 2.0 / 3.0,
+                 maxiter_cg: int = 100, tol_cg: float = 1e-5, *, relative: bool = False,
+                 coarse_sweeps: int = 50, galerkin: str = "values", poll: int = 2,
+                 device=None):
+        self.n_smooth = int(n_smooth)
+        self.omega_jacobi = float(omega_jacobi)
+        self.maxiter_cg = int(maxiter_cg)
+        self.tol_cg = float(tol_cg)
+        self.relative = bool(relative)
+        self.coarse_sweeps = int(coarse_sweeps)
+        if galerkin not in ("values", "structural"):
+            raise ValueError("galerkin must be 'values' (reference) or 'structural'")
+        self.galerkin = galerkin
+        self.n_levels = 0
+        self._initialized = False
+        self.ctx = get_context(device)
+        self.device = self.ctx.device
+        self.last_info = None
+
+    def setup(self, A_level0, Ps_or_aggs: list):
+        """Hierarchy from the fine matrix and prolongations (cons/mgpcg_gpu.py:30-173).
+
+        ``A_level0``: scipy CSR (values used for the pattern as in the reference) or a
+        ``GPUSparseMatrix`` already holding level-0 values (used in place, no copy).
+        ``Ps_or_aggs``: scipy P matrices or ``(agg, n_coarse)`` pairs."""
+        lib = _lib.load()
+        if isinstance(A_level0, GPUSparseMatrix):
+            A0 = A_level0
+        else:
+            A = sp.csr_matrix(A_level0)
+            A0 = GPUSparseMatrix(A.shape[0], A.nnz, device=self.device.index)
+            A0.from_scipy(A)
+        if Ps_or_aggs and not isinstance(Ps_or_aggs[0], tuple):
+            aggs = _aggs_from_Ps(Ps_or_aggs)
+        else:
+            aggs = [(np.ascontiguousarray(a, np.int32), int(n)) for a, n in Ps_or_aggs]
+        self.levels = [A0]
+        self.transfers = []
+        self.refill = []
+        fine_vals = A0.csr_data() if self.galerkin == "values" else None
+        F = A0
+        for agg, n_c in aggs:
+            assert agg.shape[0] == F.n_rows
+            vals = None if fine_vals is None else np.ascontiguousarray(fine_vals, np.float32)
+            g = lib.mgpbd_galerkin_build(F.n_rows, np_ptr(F.indptr), np_ptr(F.indices),
+                                         None if vals is None else vals.ctypes.data_as(_lib.F32P),
+                                         np_ptr(agg), n_c)
+            if not g:
+                check(-1, "mgpbd_galerkin_build")
+            cnnz, ncon = ctypes.c_int64(), ctypes.c_int64()
+            check(lib.mgpbd_galerkin_sizes(g, ctypes.byref(cnnz), ctypes.byref(ncon)),
+                  "mgpbd_galerkin_sizes")
+            c_indptr = np.empty(n_c + 1, np.int32)
+            c_indices = np.empty(max(cnnz.value, 1), np.int32)
+            c_diag = np.empty(n_c, np.int32)
+            cidx = np.empty(max(ncon.value, 1), np.int32)
+            coff = np.empty(cnnz.value + 1, np.int32)
+            check(lib.mgpbd_galerkin_copy(g, np_ptr(c_indptr), np_ptr(c_indices),
+                                          np_ptr(c_diag), np_ptr(cidx), np_ptr(coff)),
+                  "mgpbd_galerkin_copy")
+            lib.mgpbd_galerkin_free(g)
+            c_indices, cidx = c_indices[:cnnz.value], cidx[:ncon.value]
+            C = GPUSparseMatrix(n_c, int(cnnz.value), device=self.device.index)
+            C.set_pattern(c_indptr, c_indices, diag_csr_pos=c_diag)
+            dev = self.device
+            fpos = F.sell_pos_host[cidx] if len(cidx) else np.zeros(1, np.int32)
+            self.refill.append({
+                "n": int(cnnz.value),
+                "off": torch.from_numpy(coff).to(dev),
+                "fpos": torch.from_numpy(np.ascontiguousarray(fpos, np.int32)).to(dev),
+                "cpos": torch.from_numpy(np.ascontiguousarray(C.sell_pos_host, np.int32)).to(dev),
+            })
+            mptr, mem = _members(agg, n_c)
+            self.transfers.append({
+                "agg": torch.from_numpy(agg).to(dev),
+                "mptr": torch.from_numpy(mptr).to(dev),
+                "mem": torch.from_numpy(mem).to(dev),
+                "contrib_idx": cidx, "contrib_off": coff,
+            })
+            self.levels.append(C)
+            if fine_vals is not None:   # coarse values for the next level's pattern
+                fine_vals = self._host_refill(fine_vals, cidx, coff)
+            F = C
+        self.n_levels = len(self.levels)
+        self._level_sizes = [L.n_rows for L in self.levels]
+        self.update_all_coarse_matrices()
+        # work vectors per level: b, x, x', r
+        self._work = [torch.zeros(4 * L.n_rows, dtype=torch.float32, device=self.device)
+                      for L in self.levels]
+        n0 = self._level_sizes[0]
+        self._pcg_work = torch.zeros(4 * n0, dtype=torch.float32, device=self.device)
+        self._b = [w[:L.n_rows] for w, L in zip(self._work, self.levels)]
+        self._x = [torch.zeros(L.n_rows, dtype=torch.float32, device=self.device)
+                   for L in self.levels]
+        self._configure()
+        self._initialized = True
+
+    @staticmethod
+    def _host_refill(fine_vals, cidx, coff):
+        """Coarse values on the host (FP64 sums of the FP32 fine values, in map order)."""
+        nc = len(coff) - 1
+        seg = np.repeat(np.arange(nc), np.diff(coff))
+        out = np.bincount(seg, weights=fine_vals[cidx].astype(np.float64), minlength=nc)
+        return out.astype(np.float32)
+
+    def _configure(self):
+        ctx = self.ctx
+        ctx("mgpbd_mg_configure", self.n_levels, self.n_smooth, self.omega_jacobi, 0,
+            self.coarse_sweeps)
+        for l, L in enumerate(self.levels):
+            t = self.transfers[l] if l < len(self.transfers) else None
+            ctx("mgpbd_mg_set_level", l, L.n_rows, dptr(L.slice_ptr), dptr(L.sell_idx),
+                dptr(L.data), dptr(L.diag_inv),
+                dptr(t["agg"]) if t else None, dptr(t["mptr"]) if t else None,
+                dptr(t["mem"]) if t else None, dptr(self._work[l]), int(L.is_diagonal))
+
+    # -- per-iteration updates -------------------------------------------------------------
+    def update_fine_matrix_data(self, data):
+        """New level-0 values (cons/mgpcg_gpu.py:175-181): numpy in CSR order, or a device
+        tensor in CSR order; refreshes 1/diag."""
+        self.levels[0].update_data(data)
+        self.levels[0].cache_diag_inv()
+
+    def update_all_coarse_matrices(self):
+        """Coarse values from the fine ones through the refill maps
+        (cons/mgpcg_gpu.py:183-212)."""
+        for l, rf in enumerate(self.refill):
+            F, C = self.levels[l], self.levels[l + 1]
+            if rf["n"] > 0:
+                self.ctx("mgpbd_coarse_refill", rf["n"], dptr(rf["off"]), dptr(rf["fpos"]),
+                         dptr(rf["cpos"]), dptr(F.data), dptr(C.data))
+            C.cache_diag_inv()
+
+    # -- solve -----------------------------------------------------------------------------
+    def vcycle(self, level: int = 0):
+        """self._x[0] = V-cycle(self._b[0]) from zero (cons/mgpcg_gpu.py:214-259)."""
+        if level != 0:
+            raise ValueError("only the level-0 V-cycle is exposed")
+        self.ctx("mgpbd_mg_vcycle", dptr(self._b[0]), dptr(self._x[0]))
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor):
+        """z = M^-1 r (one V-cycle)."""
+        self.ctx("mgpbd_mg_vcycle", dptr(r), dptr(z))
+
+    def solve(self, b_field: torch.Tensor, x_field: torch.Tensor) -> int:
+        """PCG for A x = b from x = 0; returns cg_iters (cons/mgpcg_gpu.py:261-329)."""
+        it = ctypes.c_int(0)
+        info = np.zeros(4, np.float64)
+        self.ctx("mgpbd_pcg_solve", dptr(b_field), dptr(x_field), dptr(self._pcg_work),
+                 self.maxiter_cg, self.tol_cg, int(self.relative), ctypes.byref(it),
+                 info.ctypes.data_as(_lib.F64P))
+        self.last_info = {"rr": float(info[0]), "bb": float(info[1]), "reason": int(info[2])}
+        return int(it.value)

@@ -1,0 +1,259 @@
+/*
+ * mgpbd.h -- C ABI of libmgpbd.so, the MI355X (gfx950) backend of the MGPBD global XPBD solve.
+ *
+ * The reference implements this path in Python + Taichi (cons/mgpbd_gpu.py, cons/mgpcg_gpu.py,
+ * cons/sparse_gpu.py, cons/tet_aggregation.py); its "FFI" is Taichi's kernel launch from
+ * Python. This library replaces those kernels and host loops; the Python package
+ * controllable_pbd_3d_amd binds it with ctypes behind the reference's class interface.
+ * Each declaration cites the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns int status: 0 = ok, < 0 = error (MGPBD_E*); the message is in
+ *    mgpbd_last_error() (thread-local). No exception crosses the ABI.
+ *  - Pointers named *_dev are device pointers (hipMalloc / torch cuda tensors); all other
+ *    pointers are host pointers. The library never frees caller memory.
+ *  - Device work is queued on the context's stream (mgpbd_ctx_create / _set_stream) and is
+ *    asynchronous unless stated; functions that return host values synchronise that stream.
+ *  - Matrices on the device use the sliced layout "SELL-64": rows are binned 64 per slice
+ *    (one gfx950 wavefront); slice s holds width_s = (slice_ptr[s+1]-slice_ptr[s])/64
+ *    entries per row, stored k-major (entry k of row 64s+l at slice_ptr[s] + 64k + l).
+ *    Rows shorter than width_s are padded with value 0 and column = the row itself.
+ *    Within a row, entries keep the CSR order of the reference (level 0: ascending
+ *    neighbours then the diagonal; coarse levels: ascending columns).
+ */
+#ifndef MGPBD_H
+#define MGPBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MGPBD_ABI_VERSION 1
+
+#define MGPBD_OK 0
+#define MGPBD_EINVAL (-1)     /* invalid argument / shape */
+#define MGPBD_EHIP (-2)       /* HIP runtime error */
+#define MGPBD_ENONFINITE (-3) /* NaN/Inf met where a finite value is required */
+#define MGPBD_EINTERNAL (-4)
+#define MGPBD_ENOMEM (-5)
+
+const char* mgpbd_last_error(void);
+int mgpbd_abi_version(void);
+
+/* ===================================================================================== */
+/* Host-side setup (CPU, run once at solver init)                                         */
+/* ===================================================================================== */
+
+typedef struct mgpbd_pattern mgpbd_pattern;
+/* CSR pattern of A = J M^-1 J^T + alpha: row t = vertex-sharing tets ascending, then t.
+ * Replaces MGPBDSolverGPU._build_sparsity_pattern (cons/mgpbd_gpu.py:153-194). */
+mgpbd_pattern* mgpbd_pattern_build(int n_tets, int n_verts, const int32_t* t_i);
+int64_t mgpbd_pattern_nnz(const mgpbd_pattern* p);
+int mgpbd_pattern_copy(const mgpbd_pattern* p, int32_t* indptr /*n_tets+1*/, int32_t* indices /*nnz*/);
+void mgpbd_pattern_free(mgpbd_pattern* p);
+
+/* Face adjacency with the neighbour order CPython's sets give the reference.
+ * Replaces build_tet_face_adjacency (cons/tet_aggregation.py:11-37).
+ * adj: 4*n_tets (-1 padded), deg: n_tets. */
+int mgpbd_face_adjacency(int n_tets, const int32_t* t_i, int32_t* adj, int32_t* deg);
+
+/* Greedy face-adjacent aggregation. Replaces tet_geometry_aggregation
+ * (cons/tet_aggregation.py:40-127). */
+int mgpbd_tet_aggregate(int n_tets, const int32_t* adj, const int32_t* deg, int max_agg,
+                        int32_t* agg_ids, int32_t* n_agg);
+
+/* Same growth rule on a general sorted CSR graph (deeper levels of the extended hierarchy;
+ * no reference counterpart: the reference's coarse "virtual tets" never aggregate,
+ * cons/tet_aggregation.py:199-208). */
+int mgpbd_graph_aggregate(int n, const int32_t* ptr, const int32_t* idx, int max_agg,
+                          int32_t* agg_ids, int32_t* n_agg);
+typedef struct mgpbd_graph mgpbd_graph;
+mgpbd_graph* mgpbd_graph_coarsen(int n_f, const int32_t* ptr, const int32_t* idx,
+                                 const int32_t* agg, int n_c);
+int64_t mgpbd_graph_nnz(const mgpbd_graph* g);
+int mgpbd_graph_copy(const mgpbd_graph* g, int32_t* ptr, int32_t* idx);
+void mgpbd_graph_free(mgpbd_graph* g);
+
+/* Galerkin coarse level for P[i, agg[i]] = 1 and its refill map.
+ * Replaces GPUMultigridSolver.setup's P^T A P and contribution maps
+ * (cons/mgpcg_gpu.py:46-99). values != NULL reproduces scipy's dropping of entries that
+ * sum to 0.0 (value-dependent pattern); values == NULL gives the structural pattern. */
+typedef struct mgpbd_galerkin mgpbd_galerkin;
+mgpbd_galerkin* mgpbd_galerkin_build(int n_f, const int32_t* indptr, const int32_t* indices,
+                                     const float* values, const int32_t* agg, int n_c);
+int mgpbd_galerkin_sizes(const mgpbd_galerkin* g, int64_t* c_nnz, int64_t* n_contrib);
+int mgpbd_galerkin_copy(const mgpbd_galerkin* g, int32_t* c_indptr, int32_t* c_indices,
+                        int32_t* c_diag, int32_t* contrib_idx, int32_t* contrib_off);
+void mgpbd_galerkin_free(mgpbd_galerkin* g);
+
+/* vertex -> (4*tet + slot) incidence, ascending; used for the deterministic scatter that
+ * replaces the atomics of _scatter_dlam_gpu (cons/mgpbd_gpu.py:330-343). */
+int mgpbd_vertex_incidence(int n_tets, int n_verts, const int32_t* t_i, int32_t* vptr,
+                           int32_t* vinc);
+
+/* SELL-64 layout of a CSR matrix: slice_ptr (n_slices+1) and, per CSR position, its
+ * position in the sliced arrays (sell_pos, nnz). No reference counterpart (layout only). */
+int mgpbd_sell_layout(int n_rows, const int32_t* indptr, int32_t* slice_ptr);
+int mgpbd_sell_map(int n_rows, const int32_t* indptr, const int32_t* slice_ptr,
+                   int32_t* sell_pos);
+/* Column array of the sliced layout (padding points at the row itself; rows past n_rows at
+ * column 0), sell_idx sized slice_ptr[n_slices]. */
+int mgpbd_sell_columns(int n_rows, const int32_t* indptr, const int32_t* indices,
+                       const int32_t* slice_ptr, int32_t* sell_idx);
+
+/* ===================================================================================== */
+/* Device context                                                                        */
+/* ===================================================================================== */
+
+typedef struct mgpbd_ctx mgpbd_ctx;
+/* Binds to the current HIP device; `stream` is a hipStream_t (NULL = null stream). */
+mgpbd_ctx* mgpbd_ctx_create(void* stream);
+int mgpbd_ctx_set_stream(mgpbd_ctx* ctx, void* stream);
+void mgpbd_ctx_destroy(mgpbd_ctx* ctx);
+int mgpbd_ctx_sync(mgpbd_ctx* ctx);
+
+/* Optional timing of the main kernels with HIP events on the context stream (no reference
+ * counterpart; the reference times whole frames in utils/bench.py:24-55). enable resets the
+ * counters; read synchronises and returns total milliseconds and launch counts per kind. */
+#define MGPBD_PROF_SPMV_L0 0      /* PCG A p (+ p.Ap) on level 0 */
+#define MGPBD_PROF_JACOBI_L0 1    /* weighted-Jacobi sweep on level 0 */
+#define MGPBD_PROF_RESID_L0 2     /* V-cycle residual b - A x on level 0 */
+#define MGPBD_PROF_FILL_A 3
+#define MGPBD_PROF_CONSTRAINTS 4
+#define MGPBD_PROF_REFILL 5
+#define MGPBD_PROF_COARSE 6       /* sweeps and residuals on levels >= 1 */
+#define MGPBD_PROF_KINDS 8
+int mgpbd_ctx_profile(mgpbd_ctx* ctx, int enable);
+int mgpbd_ctx_profile_read(mgpbd_ctx* ctx, double* ms_out, int64_t* count_out);
+
+/* ===================================================================================== */
+/* Per-tet / per-vertex kernels (cons/mgpbd_gpu.py)                                      */
+/* ===================================================================================== */
+
+/* Dm^-1 (row-major 3x3 per tet), |det Dm|/6, alpha = 1/(mu dt^2 vol); mu_dev (per tet)
+ * overrides the scalar mu when not NULL. Replaces _precompute_rest_shape (:136-151). */
+int mgpbd_rest_shape(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev,
+                     const float* v_ref_dev, const float* mu_dev, float mu, float dt,
+                     float* Dm_inv_dev, float* rest_vol_dev, float* alpha_dev);
+
+/* v += dt g (+ dt invm f_ext); v *= damp; x_old = x; x += dt v.
+ * Replaces make_prediction (:206-213), which moves every vertex, pinned ones included;
+ * hold_pinned != 0 (extension) keeps vertices with invm == 0 in place with zero velocity.
+ * f_ext_dev (3 per vertex) may be NULL. */
+int mgpbd_predict(mgpbd_ctx* ctx, int n_verts, float* v_p_dev, float* v_v_dev,
+                  float* v_p_old_dev, const float* v_invm_dev, const float* f_ext_dev,
+                  float gx, float gy, float gz, float dt, float damp, int hold_pinned);
+
+/* v = damp (x - x_old) / dt. Replaces update_vel (:215-219). */
+int mgpbd_update_vel(mgpbd_ctx* ctx, int n_verts, const float* v_p_dev,
+                     const float* v_p_old_dev, float* v_v_dev, float dt, float damp);
+
+/* C = ||sigma(F) - 1||, gradC (4 x vec3 per tet, 0 when C <= 1e-6), rhs = -(C + alpha lambda)
+ * and ||rhs||^2 (FP64, written to *norm2_dev). Replaces _compute_C_and_gradC (:225-247),
+ * _compute_rhs_gpu (:253-258) and _residual_norm_gpu (:260-267). */
+int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
+                      const float* Dm_inv_dev, const float* alpha_dev, const float* lambda_dev,
+                      float* C_dev, float* gradC_dev, float* rhs_dev, double* norm2_dev);
+
+/* A values on the level-0 sliced pattern plus 1/diag (0 when |diag| <= 1e-30).
+ * Replaces _fill_A_kernel (:273-309) with the shared-vertex sum order of _intersect_tets
+ * (cons/mgpbd.py:71-84) and cache_diag_inv (cons/sparse_gpu.py:107-121). */
+int mgpbd_fill_A(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr_dev,
+                 const int32_t* sell_idx_dev, const int32_t* row_len_dev,
+                 const int32_t* t_i_dev, const float* v_invm_dev, const float* gradC_dev,
+                 const float* alpha_dev, float* sell_val_dev, float* diag_inv_dev);
+
+/* lambda += dlambda; dpos_v = sum over (t, slot) at v of invm_v dlambda_t gradC_t,slot in
+ * ascending (t, slot) order. Replaces _zero_dpos + _scatter_dlam_gpu (:330-354). */
+int mgpbd_scatter(mgpbd_ctx* ctx, int n_tets, int n_verts, const int32_t* vptr_dev,
+                  const int32_t* vinc_dev, const float* v_invm_dev, const float* gradC_dev,
+                  const float* dlam_dev, float* lambda_dev, float* dpos_dev);
+
+/* Dual-residual norms for the line search (cons/mgpbd_gpu.py:360-391): out[0] =
+ * ||C(x) + alpha lambda||^2 from the stored C, out[1] = same at x + omega dpos (C
+ * recomputed). FP64, device out (2 doubles). */
+int mgpbd_dual_norms(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
+                     const float* dpos_dev, float omega, const float* Dm_inv_dev,
+                     const float* alpha_dev, const float* lambda_dev, const float* C_dev,
+                     int with_current, double* out_dev);
+
+/* x += omega dpos where invm > 0. Replaces _apply_dpos (:345-349). */
+int mgpbd_apply_dpos(mgpbd_ctx* ctx, int n_verts, float* v_p_dev, const float* dpos_dev,
+                     const float* v_invm_dev, float omega);
+
+/* ===================================================================================== */
+/* Sparse / BLAS-1 kernels (cons/sparse_gpu.py, cons/mgpcg_gpu.py)                       */
+/* ===================================================================================== */
+
+/* y = A x. Replaces GPUSparseMatrix.spmv (cons/sparse_gpu.py:123-132). */
+int mgpbd_spmv(mgpbd_ctx* ctx, int n_rows, const int32_t* slice_ptr_dev,
+               const int32_t* sell_idx_dev, const float* sell_val_dev, const float* x_dev,
+               float* y_dev);
+/* x_new = x + omega D^-1 (b - A x). Replaces jacobi_smooth (cons/sparse_gpu.py:134-144). */
+int mgpbd_jacobi(mgpbd_ctx* ctx, int n_rows, const int32_t* slice_ptr_dev,
+                 const int32_t* sell_idx_dev, const float* sell_val_dev,
+                 const float* diag_inv_dev, const float* b_dev, const float* x_dev,
+                 float* x_new_dev, float omega);
+/* diag_inv[i] = 1/val[diag_pos[i]] (0 if |.| <= 1e-30 or diag_pos[i] < 0).
+ * Replaces cache_diag_inv (cons/sparse_gpu.py:107-121). */
+int mgpbd_diag_inv(mgpbd_ctx* ctx, int n_rows, const int32_t* diag_pos_dev,
+                   const float* sell_val_dev, float* diag_inv_dev);
+/* Coarse values: val_c[cpos[c]] = sum_{k in [off[c], off[c+1])} val_f[fpos[k]] in order.
+ * Replaces _fill_coarse_values_kernel (cons/mgpcg_gpu.py:183-198) for unit P entries. */
+int mgpbd_coarse_refill(mgpbd_ctx* ctx, int n_coarse_nnz, const int32_t* off_dev,
+                        const int32_t* fpos_dev, const int32_t* cpos_dev,
+                        const float* fine_val_dev, float* coarse_val_dev);
+/* b_c[I] = sum_{i in I, ascending} r[i] (P^T r for unit piecewise-constant P).
+ * Replaces the P^T spmv of vcycle (cons/mgpcg_gpu.py:243). */
+int mgpbd_restrict(mgpbd_ctx* ctx, int n_coarse, const int32_t* mem_ptr_dev,
+                   const int32_t* mem_dev, const float* r_dev, float* b_c_dev);
+/* x[i] += x_c[agg[i]]. Replaces P spmv + axpy of vcycle (cons/mgpcg_gpu.py:252-253). */
+int mgpbd_prolong(mgpbd_ctx* ctx, int n_fine, const int32_t* agg_dev, const float* x_c_dev,
+                  float* x_dev);
+
+/* BLAS-1 (cons/sparse_gpu.py:17-69). Reductions are FP64 and deterministic and write their
+ * result to a device double. */
+int mgpbd_dot(mgpbd_ctx* ctx, int n, const float* a_dev, const float* b_dev, double* out_dev);
+int mgpbd_axpy(mgpbd_ctx* ctx, int n, float alpha, const float* x_dev, float* y_dev);
+int mgpbd_xpay(mgpbd_ctx* ctx, int n, float alpha, const float* x_dev, float* y_dev);
+int mgpbd_copy(mgpbd_ctx* ctx, int n, const float* src_dev, float* dst_dev);
+int mgpbd_fill(mgpbd_ctx* ctx, int n, float value, float* x_dev);
+int mgpbd_scale(mgpbd_ctx* ctx, int n, float alpha, float* x_dev);
+int mgpbd_negate_and_add(mgpbd_ctx* ctx, int n, const float* b_dev, float* r_dev);
+
+/* ===================================================================================== */
+/* Multigrid-preconditioned CG (cons/mgpcg_gpu.py GPUMultigridSolver)                    */
+/* ===================================================================================== */
+
+#define MGPBD_MAX_LEVELS 12
+#define MGPBD_COARSE_JACOBI 0   /* min(coarse_sweeps, n) weighted-Jacobi sweeps from 0 */
+
+/* (Re)declare the hierarchy: n_levels levels, level 0 the finest. */
+int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega,
+                       int coarse_mode, int coarse_sweeps);
+/* Level l operator and transfer to level l+1 (agg/mem_ptr/mem NULL on the coarsest).
+ * work_dev: 4 * n_rows floats owned by the caller (b, x, x', r of the level).
+ * diagonal_only != 0 declares that every row holds only its diagonal entry (then repeated
+ * Jacobi sweeps run as one elementwise pass with identical arithmetic). */
+int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* slice_ptr_dev,
+                       const int32_t* sell_idx_dev, const float* sell_val_dev,
+                       const float* diag_inv_dev, const int32_t* agg_dev,
+                       const int32_t* mem_ptr_dev, const int32_t* mem_dev, float* work_dev,
+                       int diagonal_only);
+/* x = M^-1 b with one V-cycle from x = 0. Replaces vcycle(0) (cons/mgpcg_gpu.py:214-259). */
+int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
+/* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:
+ * 261-329: stop when ||r||^2 < tol^2, breakdown on |pAp| < 1e-30 and on the previous
+ * |rz| < 1e-30). relative != 0 (extension) stops on ||r|| < tol ||b||. work_dev: 4*n floats
+ * (r, z, p, Ap). Synchronises; *iters_out = completed iterations (reference count);
+ * info_out (host, 4 doubles) = {||r||^2, ||b||^2, stop reason, 0}. */
+int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* work_dev,
+                    int maxiter, double tol, int relative, int* iters_out, double* info_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGPBD_H */

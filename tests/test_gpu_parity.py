@@ -1,0 +1,240 @@
+"""GPU parity: libmgpbd kernels (through the C ABI) against the CPU oracle on the same inputs.
+
+Bars (written in each test):
+* integer / structure work: bit-exact;
+* SpMV, Jacobi, A fill, coarse refill, restriction, prolongation, scatter, V-cycle: bit-exact
+  FP32 for identical inputs (both sides use the same explicit-FMA operation order);
+* C and dC/dx (3x3 SVD): |err| <= 2e-5 * max(1, |value|) against the FP64-SVD oracle;
+* PCG and full substeps: tolerance against the FP32 oracle and the FP64 solve.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from oracle import mgpbd_oracle as om          # noqa: E402
+from oracle import structure                   # noqa: E402
+from tests.scenarios import deformed, small_block, solver_kwargs   # noqa: E402
+
+
+def _solver(mesh, v0=None, **kw):
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    args = solver_kwargs(mesh, **kw)
+    if v0 is not None:
+        args["v_p"] = v0.copy()
+    s = MGPBDSolverGPU(**args, verbose=False)
+    s.init()
+    return s
+
+
+def _oracle(mesh, v0=None, precision="f32", **kw):
+    args = solver_kwargs(mesh, **kw)
+    if v0 is not None:
+        args["v_p"] = v0.copy()
+    o = om.OracleMGPBD(**args, precision=precision)
+    o.init()
+    return o
+
+
+@pytest.fixture(scope="module")
+def scene():
+    mesh = small_block(5, seed=3)
+    v0 = deformed(mesh, seed=3)
+    print(f"[scene] box5 n_tets={mesh.n_tets} deformed init", flush=True)
+    return mesh, v0
+
+
+def test_structure_matches_oracle(scene):
+    mesh, v0 = scene
+    s = _solver(mesh, v0)
+    indptr, indices = structure.build_sparsity_pattern(mesh.tets, mesh.n_tets)
+    assert np.array_equal(s._A_indptr, indptr)
+    assert np.array_equal(s._A_indices, indices)
+    Ps, sizes = structure.build_tet_hierarchy(mesh.tets, mesh.verts.astype(np.float64), mesh.n_tets)
+    assert s._gpu_solver._level_sizes == sizes
+    assert np.array_equal(s._aggs[0][0], Ps[0].indices)
+    # value-dependent Galerkin pattern equals scipy's on the same FP32 values
+    A = sp.csr_matrix((s._A.csr_data().astype(np.float64), indices, indptr),
+                      shape=(mesh.n_tets, mesh.n_tets))
+    Ac = (Ps[0].T @ A @ Ps[0]).tocsr()
+    C = s._gpu_solver.levels[1]
+    assert np.array_equal(C.indptr, Ac.indptr) and np.array_equal(C.indices, Ac.indices)
+
+
+def test_constraints_vs_fp64_svd(scene):
+    mesh, v0 = scene
+    s = _solver(mesh, v0)
+    o = _oracle(mesh, v0, precision="f64")
+    # same Dm^-1 on both sides
+    s.Dm_inv.copy_(torch.from_numpy(o.Dm_inv.astype(np.float32).reshape(-1, 3, 3)))
+    s._compute_C_and_gradC()
+    o._compute_C_and_gradC()
+    C_g = s.constraints.cpu().numpy()
+    g_g = s.gradC.cpu().numpy().reshape(-1)
+    assert np.all(o.constraints > 1e-6)
+    np.testing.assert_allclose(C_g, o.constraints, rtol=2e-5, atol=2e-5)
+    np.testing.assert_allclose(g_g, o.gradC, rtol=0, atol=2e-5 * max(1.0, np.abs(o.gradC).max()))
+
+
+def test_sparse_kernels_bitexact(scene):
+    """fill_A, diag_inv, SpMV, Jacobi, refill, restrict, prolong, scatter: identical FP32."""
+    mesh, v0 = scene
+    s = _solver(mesh, v0)
+    o = _oracle(mesh, v0, precision="f32")
+    L, R = o.L, o.R
+    # identical inputs: oracle's alpha and gradC pushed to the GPU solver
+    s.alpha_tilde.copy_(torch.from_numpy(o.alpha_tilde))
+    s.gradC.copy_(torch.from_numpy(o.gradC.reshape(-1, 4, 3)))
+    s._fill_A()
+    data = np.zeros(len(o.indices), np.float32)
+    L.orc_fill_A(o.n_tets, om.ptr(o.indptr, om.I32P), om.ptr(o.indices, om.I32P),
+                 om.ptr(o.t_i, om.I32P), om.ptr(o.v_invm, R), om.ptr(o.gradC, R),
+                 om.ptr(o.alpha_tilde, R), om.ptr(data, R))
+    got = s._A.csr_data()
+    assert np.array_equal(got.view(np.uint32), data.view(np.uint32)), "fill_A bits"
+    dinv = np.zeros(o.n_tets, np.float32)
+    L.orc_cache_diag_inv(o.n_tets, om.ptr(o.indptr, om.I32P), om.ptr(o.indices, om.I32P),
+                         om.ptr(data, R), om.ptr(dinv, R))
+    assert np.array_equal(s._A.diag_inv.cpu().numpy().view(np.uint32), dinv.view(np.uint32))
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(o.n_tets).astype(np.float32)
+    b = rng.standard_normal(o.n_tets).astype(np.float32)
+    xt, bt = torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda()
+    y = torch.zeros_like(xt)
+    s._A.spmv(xt, y)
+    y_o = np.zeros_like(x)
+    L.orc_spmv(o.n_tets, om.ptr(o.indptr, om.I32P), om.ptr(o.indices, om.I32P),
+               om.ptr(data, R), om.ptr(x, R), om.ptr(y_o, R))
+    assert np.array_equal(y.cpu().numpy().view(np.uint32), y_o.view(np.uint32)), "spmv bits"
+    xn = torch.zeros_like(xt)
+    s._A.jacobi_smooth(bt, xt, xn, 2.0 / 3.0)
+    xn_o = np.zeros_like(x)
+    L.orc_jacobi(o.n_tets, om.ptr(o.indptr, om.I32P), om.ptr(o.indices, om.I32P),
+                 om.ptr(data, R), om.ptr(dinv, R), om.ptr(b, R), om.ptr(x, R), om.ptr(xn_o, R),
+                 2.0 / 3.0)
+    assert np.array_equal(xn.cpu().numpy().view(np.uint32), xn_o.view(np.uint32)), "jacobi bits"
+    # coarse refill through the contribution maps
+    mg = s._gpu_solver
+    mg.update_all_coarse_matrices()
+    t = mg.transfers[0]
+    coarse_o = np.zeros(mg.levels[1].nnz, np.float32)
+    ones = np.ones(max(len(t["contrib_idx"]), 1), np.float32)
+    L.orc_refill(mg.levels[1].nnz, om.ptr(t["contrib_off"], om.I32P),
+                 om.ptr(np.ascontiguousarray(t["contrib_idx"]), om.I32P), om.ptr(ones, R),
+                 om.ptr(ones, R), om.ptr(data, R), om.ptr(coarse_o, R))
+    assert np.array_equal(mg.levels[1].csr_data().view(np.uint32), coarse_o.view(np.uint32))
+    # restriction / prolongation with unit P
+    agg = s._aggs[0][0]
+    n_c = s._aggs[0][1]
+    P = sp.csr_matrix((np.ones(o.n_tets, np.float32), (np.arange(o.n_tets), agg)),
+                      shape=(o.n_tets, n_c))
+    PT = P.T.tocsr()
+    bc = torch.zeros(n_c, dtype=torch.float32, device="cuda")
+    s.ctx("mgpbd_restrict", n_c, t["mptr"].data_ptr(), t["mem"].data_ptr(), xt.data_ptr(),
+          bc.data_ptr())
+    bc_o = np.zeros(n_c, np.float32)
+    L.orc_spmv(n_c, om.ptr(PT.indptr.astype(np.int32), om.I32P),
+               om.ptr(PT.indices.astype(np.int32), om.I32P), om.ptr(PT.data, R),
+               om.ptr(x, R), om.ptr(bc_o, R))
+    assert np.array_equal(bc.cpu().numpy().view(np.uint32), bc_o.view(np.uint32))
+    xc = torch.from_numpy(bc_o).cuda()
+    xp = xt.clone()
+    s.ctx("mgpbd_prolong", o.n_tets, t["agg"].data_ptr(), xc.data_ptr(), xp.data_ptr())
+    assert np.array_equal(xp.cpu().numpy(), x + bc_o[agg])
+    # scatter
+    dl = rng.standard_normal(o.n_tets).astype(np.float32) * 1e-3
+    s._dlam_gpu.copy_(torch.from_numpy(dl))
+    s.lambdaf.zero_()
+    s.v_invm.copy_(torch.from_numpy(o.v_invm))
+    s._scatter_dlam_gpu()
+    lam_o = np.zeros(o.n_tets, np.float32)
+    dpos_o = np.zeros(3 * o.n_verts, np.float32)
+    L.orc_scatter(o.n_tets, o.n_verts, om.ptr(o.t_i, om.I32P), om.ptr(o.v_invm, R),
+                  om.ptr(o.gradC, R), om.ptr(dl, R), om.ptr(lam_o, R), om.ptr(dpos_o, R))
+    assert np.array_equal(s.dpos.cpu().numpy().reshape(-1).view(np.uint32), dpos_o.view(np.uint32))
+    assert np.array_equal(s.lambdaf.cpu().numpy(), lam_o)
+
+
+def test_vcycle_bitexact(scene):
+    mesh, v0 = scene
+    s = _solver(mesh, v0)
+    o = _oracle(mesh, v0, precision="f32")
+    # identical level-0 values on both sides, then each side refills its coarse level
+    o.mg.update_fine_matrix_data(s._A.csr_data())
+    o.mg.update_all_coarse_matrices()
+    rng = np.random.default_rng(1)
+    r = rng.standard_normal(o.n_tets).astype(np.float32)
+    z_o = o.mg.vcycle(r)
+    rt = torch.from_numpy(r).cuda()
+    zt = torch.zeros_like(rt)
+    s._gpu_solver.apply(rt, zt)
+    assert np.array_equal(zt.cpu().numpy().view(np.uint32), z_o.view(np.uint32))
+
+
+@pytest.mark.parametrize("relative", [False, True])
+def test_pcg_vs_oracle_and_fp64(scene, relative):
+    mesh, v0 = scene
+    tol = 1e-6 if relative else 1e-5
+    s = _solver(mesh, v0, relative_tol=relative, tol_cg=tol)
+    o = _oracle(mesh, v0, precision="f32", relative_tol=relative, tol_cg=tol)
+    o.mg.update_fine_matrix_data(s._A.csr_data())
+    o.mg.update_all_coarse_matrices()
+    s._compute_C_and_gradC()
+    b = s._b_gpu.cpu().numpy()
+    it = s._gpu_solver.solve(s._b_gpu, s._dlam_gpu)
+    x = s._dlam_gpu.cpu().numpy()
+    x_o = np.zeros_like(b)
+    it_o = o.mg.solve(b.copy(), x_o)
+    print(f"[pcg] relative={relative} iters gpu={it} oracle={it_o}", flush=True)
+    assert abs(it - it_o) <= 1
+    A = s._A.to_scipy().astype(np.float64)
+    res = np.linalg.norm(b - A @ x.astype(np.float64))
+    if relative:
+        assert res <= 5e-6 * np.linalg.norm(b)
+    x64 = sp.linalg.spsolve(A.tocsc(), b.astype(np.float64))
+    assert np.linalg.norm(x - x64) <= 1e-3 * np.linalg.norm(x64)
+    assert np.linalg.norm(x - x_o) <= 1e-3 * np.linalg.norm(x_o)
+
+
+@pytest.mark.parametrize("mode", ["reference", "extended"])
+def test_substeps_positions(mode):
+    """Positions after 10 substeps within 1e-4 h of the FP64 oracle.
+    reference: the reference's semantics (pinned vertices predicted, two levels, init-time
+    Galerkin pattern, absolute PCG tolerance) from a deformed start; extended: held pins,
+    deeper hierarchy, structural Galerkin, relative 1e-6 PCG, from rest under gravity."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(5, seed=5)
+    if mode == "reference":
+        v0 = deformed(mesh, seed=5, amp=0.02, twist=0.05)
+        kw = dict(mu_per_tet=mu_t)
+        okw = dict(mu_per_tet=mu_t)
+    else:
+        v0 = None
+        kw = dict(mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended",
+                  galerkin="structural", relative_tol=True, tol_cg=1e-6, max_coarse=64)
+        okw = dict(mu_per_tet=mu_t, hold_pinned=True, relative_tol=True, tol_cg=1e-6)
+    s = _solver(mesh, v0, **kw)
+    o = _oracle(mesh, v0, precision="f64", **okw)
+    if mode == "extended":
+        # the oracle's own init restates the reference hierarchy; give it the same levels
+        A = sp.csr_matrix((np.asarray(o.A_data, np.float64), o.indices, o.indptr),
+                          shape=(o.n_tets, o.n_tets))
+        Ps = [sp.csr_matrix((np.ones(len(a)), (np.arange(len(a)), a)), shape=(len(a), n))
+              for a, n in s._aggs]
+        o.mg = om.OracleMG(2, 2.0 / 3.0, 100, 1e-6, "f64", relative=True, structural=True)
+        o.mg.setup(A, Ps)
+    worst = 0.0
+    for k in range(10):
+        s.step(0)
+        o.step(0)
+        err = np.abs(s.v_p.cpu().numpy() - o.v_p).max()
+        worst = max(worst, err)
+    print(f"[substeps] mode={mode} max|dx|/h = {worst / mesh.h:.3e} "
+          f"gpu={s.last_stats} oracle={o.history[-1]}", flush=True)
+    assert worst <= 1e-4 * mesh.h
