@@ -52,7 +52,7 @@ SIGNATURES = {
     "mgpbd_ctx_profile": (c_int, [_V, c_int]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
-    "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float, _V, _V, _V]),
+    "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),
     "mgpbd_predict": (c_int, [_V, c_int, _V, _V, _V, _V, _V, c_float, c_float, c_float,
                               c_float, c_float, c_int]),
     "mgpbd_update_vel": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float]),

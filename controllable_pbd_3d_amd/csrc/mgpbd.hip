@@ -30,7 +30,8 @@
 namespace mgpbd {
 
 constexpr int kBlock = 256;           // 4 wavefronts of 64
-constexpr int kMaxRedBlocks = 2048;   // grid cap for reduction kernels (partials size)
+constexpr int kMaxRedBlocks = 2048;   // grid cap for grid-stride reduction kernels
+constexpr int kPartialStride = 65536; // partials per reduction slot (full-grid kernels)
 
 // ------------------------------------------------------------------------------------------
 // Deterministic FP64 reductions: block tree -> per-block partial -> the last block to arrive
@@ -56,7 +57,7 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 struct RedWS {
-  double* partials;       // kMaxRedBlocks * 4
+  double* partials;       // kPartialStride * 4
   unsigned int* ticket;   // 1 counter, 0 between kernels
 };
 
@@ -76,7 +77,7 @@ __device__ __forceinline__ bool grid_arrive(RedWS ws) {
 }
 
 __device__ __forceinline__ double grid_total(RedWS ws, int slot) {
-  const double* p = ws.partials + (size_t)slot * kMaxRedBlocks;
+  const double* p = ws.partials + (size_t)slot * kPartialStride;
   double s = 0.0;
   for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
     s += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -88,7 +89,7 @@ __device__ __forceinline__ void grid_reset(RedWS ws) {
 }
 
 __device__ __forceinline__ void store_partial(RedWS ws, int slot, double v) {
-  if (threadIdx.x == 0) ws.partials[(size_t)slot * kMaxRedBlocks + blockIdx.x] = v;
+  if (threadIdx.x == 0) ws.partials[(size_t)slot * kPartialStride + blockIdx.x] = v;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -100,60 +101,67 @@ __device__ __forceinline__ void load3(const float* __restrict__ p, int v, float*
   o[2] = p[3 * (size_t)v + 2];
 }
 
+// Rest shape in FP64 (rounded once to FP32): Dm^-1 = adj(Dm) / det, vol = |det| / 6,
+// alpha = 1 / (mu dt^2 vol).
 __global__ void __launch_bounds__(kBlock)
 k_rest_shape(int n, const int4* __restrict__ t_i, const float* __restrict__ xr,
-             const float* __restrict__ mu_t, float mu, float dt, float* __restrict__ Dm_inv,
+             const float* __restrict__ mu_t, double mu, double dt, float* __restrict__ Dm_inv,
              float* __restrict__ vol_out, float* __restrict__ alpha) {
   int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
   int4 tv = t_i[t];
   float r0[3], r1[3], r2[3], r3[3];
   load3(xr, tv.x, r0); load3(xr, tv.y, r1); load3(xr, tv.z, r2); load3(xr, tv.w, r3);
-  float D[3][3];   // Dm = [r1-r0, r2-r0, r3-r0] as columns; D[row][col]
+  double D[3][3];   // Dm = [r1-r0, r2-r0, r3-r0] as columns; D[row][col]
   for (int r = 0; r < 3; ++r) {
-    D[r][0] = r1[r] - r0[r];
-    D[r][1] = r2[r] - r0[r];
-    D[r][2] = r3[r] - r0[r];
+    D[r][0] = (double)r1[r] - (double)r0[r];
+    D[r][1] = (double)r2[r] - (double)r0[r];
+    D[r][2] = (double)r3[r] - (double)r0[r];
   }
-  float c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
-  float c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
-  float c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
-  float det = D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02;
-  float id = 1.0f / det;
+  double c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+  double c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+  double c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+  double det = D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02;
   float* o = Dm_inv + 9 * (size_t)t;
   // inverse = adjugate / det, adj[i][j] = cofactor[j][i]
-  o[0] = c00 * id;
-  o[1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) * id;
-  o[2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) * id;
-  o[3] = c01 * id;
-  o[4] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * id;
-  o[5] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * id;
-  o[6] = c02 * id;
-  o[7] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * id;
-  o[8] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * id;
-  float vol = fabsf(det) / 6.0f;
-  vol_out[t] = vol;
-  float m = mu_t ? mu_t[t] : mu;
-  alpha[t] = 1.0f / (m * dt * dt * vol);
+  o[0] = (float)(c00 / det);
+  o[1] = (float)((D[0][2] * D[2][1] - D[0][1] * D[2][2]) / det);
+  o[2] = (float)((D[0][1] * D[1][2] - D[0][2] * D[1][1]) / det);
+  o[3] = (float)(c01 / det);
+  o[4] = (float)((D[0][0] * D[2][2] - D[0][2] * D[2][0]) / det);
+  o[5] = (float)((D[0][2] * D[1][0] - D[0][0] * D[1][2]) / det);
+  o[6] = (float)(c02 / det);
+  o[7] = (float)((D[0][1] * D[2][0] - D[0][0] * D[2][1]) / det);
+  o[8] = (float)((D[0][0] * D[1][1] - D[0][1] * D[1][0]) / det);
+  double vol = fabs(det) / 6.0;
+  vol_out[t] = (float)vol;
+  double m = mu_t ? (double)mu_t[t] : mu;
+  alpha[t] = (float)(1.0 / (m * dt * dt * vol));
 }
 
-// F = Ds Dm^-1 for tet t at positions x (+ omega * dx when dx != nullptr).
+// F = Ds Dm^-1 (FP64) for tet t at positions x (+ omega * dx in FP32 when dx != nullptr,
+// the trial positions of the line search).
 __device__ __forceinline__ void deformation_gradient(const int4 tv, const float* __restrict__ x,
                                                      const float* __restrict__ dx, float omega,
-                                                     const float* __restrict__ B, float F[9]) {
-  float p[4][3];
+                                                     const float* __restrict__ Bf, double F[9]) {
+  double p[4][3];
   int vs[4] = {tv.x, tv.y, tv.z, tv.w};
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
-    load3(x, vs[l], p[l]);
+    float q[3];
+    load3(x, vs[l], q);
     if (dx) {
       float d[3];
       load3(dx, vs[l], d);
 #pragma unroll
-      for (int r = 0; r < 3; ++r) p[l][r] = p[l][r] + omega * d[r];
+      for (int r = 0; r < 3; ++r) q[r] = q[r] + omega * d[r];
     }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) p[l][r] = (double)q[r];
   }
-  float Ds[3][3];
+  double Ds[3][3], B[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) B[k] = (double)Bf[k];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     Ds[r][0] = p[1][r] - p[0][r];
@@ -164,12 +172,12 @@ __device__ __forceinline__ void deformation_gradient(const int4 tv, const float*
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      F[3 * r + c] = fmaf(Ds[r][2], B[6 + c], fmaf(Ds[r][1], B[3 + c], Ds[r][0] * B[c]));
+      F[3 * r + c] = Ds[r][0] * B[c] + Ds[r][1] * B[3 + c] + Ds[r][2] * B[6 + c];
 }
 
-__device__ __forceinline__ float arap_C(const float s[3]) {
-  float a = s[0] - 1.0f, b = s[1] - 1.0f, c = s[2] - 1.0f;
-  return sqrtf(fmaf(c, c, fmaf(b, b, a * a)));
+__device__ __forceinline__ double arap_C(const double s[3]) {
+  double a = s[0] - 1.0, b = s[1] - 1.0, c = s[2] - 1.0;
+  return sqrt(a * a + b * b + c * c);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -181,32 +189,38 @@ k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
   double acc = 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
     int4 tv = t_i[t];
-    float B[9];
+    float Bf[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
-    float F[9];
-    deformation_gradient(tv, x, nullptr, 0.0f, B, F);
-    float U[3][3], s[3], V[3][3];
+    for (int k = 0; k < 9; ++k) Bf[k] = Dm_inv[9 * (size_t)t + k];
+    double F[9];
+    deformation_gradient(tv, x, nullptr, 0.0f, Bf, F);
+    double U[3][3], s[3], V[3][3];
     svd3(F, U, s, V);
-    float C = arap_C(s);
+    double Cd = arap_C(s);
+    float C = (float)Cd;
     float g[12];
     if (C > 1e-6f) {
-      float d0 = (s[0] - 1.0f) / C, d1 = (s[1] - 1.0f) / C, d2 = (s[2] - 1.0f) / C;
-      float G[3][3];   // dC/dF = U diag(d) V^T
+      double d[3] = {(s[0] - 1.0) / Cd, (s[1] - 1.0) / Cd, (s[2] - 1.0) / Cd};
+      double G[3][3];   // dC/dF = U diag(d) V^T
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          G[r][c] = fmaf(U[2][r] * V[2][c], d2, fmaf(U[1][r] * V[1][c], d1, U[0][r] * V[0][c] * d0));
+          G[r][c] = U[0][r] * V[0][c] * d[0] + U[1][r] * V[1][c] * d[1] + U[2][r] * V[2][c] * d[2];
       // g_k = G B^T[:, k-1] (k = 1..3), g_0 = -(g_1 + g_2 + g_3)
+      double gd[4][3];
 #pragma unroll
       for (int k = 1; k <= 3; ++k)
 #pragma unroll
         for (int r = 0; r < 3; ++r)
-          g[3 * k + r] = fmaf(G[r][2], B[3 * (k - 1) + 2],
-                              fmaf(G[r][1], B[3 * (k - 1) + 1], G[r][0] * B[3 * (k - 1)]));
+          gd[k][r] = G[r][0] * (double)Bf[3 * (k - 1)] + G[r][1] * (double)Bf[3 * (k - 1) + 1] +
+                     G[r][2] * (double)Bf[3 * (k - 1) + 2];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) g[r] = -g[3 + r] - g[6 + r] - g[9 + r];
+      for (int r = 0; r < 3; ++r) gd[0][r] = -gd[1][r] - gd[2][r] - gd[3][r];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) g[3 * k + r] = (float)gd[k][r];
     } else {
 #pragma unroll
       for (int k = 0; k < 12; ++k) g[k] = 0.0f;
@@ -241,11 +255,11 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
     float B[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
-    float F[9];
+    double F[9];
     deformation_gradient(tv, x, dx, omega, B, F);
-    float U[3][3], s[3], V[3][3];
+    double U[3][3], s[3], V[3][3];
     svd3(F, U, s, V);
-    float c = arap_C(s);
+    float c = (float)arap_C(s);
     float al = alpha[t] * lambda[t];
     float r = -(c + al);
     acc_trial += (double)r * (double)r;
@@ -603,18 +617,19 @@ k_pcg_rz(int n, const float* __restrict__ r, const float* __restrict__ z, RedWS 
   }
 }
 
-// Ap = A p, pAp = p.Ap, alpha = rz / pAp
+// Ap = A p, pAp = p.Ap, alpha = rz / pAp (one row per thread, one slice per wavefront)
 __global__ void __launch_bounds__(kBlock)
 k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
                const int* __restrict__ idx, const float* __restrict__ val,
                const float* __restrict__ p, float* __restrict__ Ap, RedWS ws, PcgState* st) {
   if (st->done) return;
   double acc = 0.0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; (i >> 6) < n_slices; i += gridDim.x * kBlock) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
     float ax = sell_row(i, slice_ptr, idx, val, p);
     if (i < n) {
       Ap[i] = ax;
-      acc += (double)p[i] * (double)ax;
+      acc = (double)p[i] * (double)ax;
     }
   }
   double bs = block_sum(acc);
@@ -748,7 +763,7 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (!c) { mgpbd_set_error("mgpbd_ctx_create: out of memory"); return nullptr; }
   c->stream = (hipStream_t)stream;
   hipError_t e = hipGetDevice(&c->device);
-  if (e == hipSuccess) e = hipMalloc(&c->ws.partials, sizeof(double) * kMaxRedBlocks * 4);
+  if (e == hipSuccess) e = hipMalloc(&c->ws.partials, sizeof(double) * kPartialStride * 4);
   if (e == hipSuccess) e = hipMalloc(&c->ws.ticket, sizeof(unsigned int));
   if (e == hipSuccess) e = hipMemset(c->ws.ticket, 0, sizeof(unsigned int));
   if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(PcgState));
@@ -815,7 +830,7 @@ int mgpbd_ctx_profile_read(mgpbd_ctx* c, double* ms, int64_t* count) {
   if (!ctx) return mgpbd_fail(MGPBD_EINVAL, std::string(fn) + ": null context")
 
 int mgpbd_rest_shape(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_ref,
-                     const float* mu_t, float mu, float dt, float* Dm_inv, float* vol,
+                     const float* mu_t, double mu, double dt, float* Dm_inv, float* vol,
                      float* alpha) {
   CTX_CHECK("mgpbd_rest_shape");
   if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_rest_shape: n_tets < 0");
@@ -1121,6 +1136,8 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_solve: invalid arguments");
   MgLevel& L = ctx->L[0];
   const int n = L.n;
+  if (nblocks(64LL * L.n_slices) > kPartialStride)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_solve: level 0 exceeds the reduction capacity");
   float* r = work;                  // work[n, 2n) is reserved (unused)
   float* p = work + 2 * (size_t)n;
   float* Ap = work + 3 * (size_t)n;
@@ -1143,7 +1160,7 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
     int chunk = std::min(ctx->poll, maxiter - issued);
     for (int c = 0; c < chunk; ++c) {
       int pe = prof_begin(ctx, MGPBD_PROF_SPMV_L0);
-      k_pcg_spmv_dot<<<red_blocks(64LL * L.n_slices), kBlock, 0, s>>>(
+      k_pcg_spmv_dot<<<nblocks(64LL * L.n_slices), kBlock, 0, s>>>(
           n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap, ctx->ws, ctx->st);
       prof_end(ctx, pe);
       k_pcg_update<<<red_blocks(n), kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr,

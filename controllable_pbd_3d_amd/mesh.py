@@ -170,7 +170,7 @@ def read_medit(path: str):
     (the reference loads them with meshio in ``utils/io.py:11-19``). Returns
     ``(verts float64 (n,3), tets int32 flat, faces int32 flat)``, ids made 0-based."""
     with open(path, "r") as f:
-        toks = f.read().split()
+        toks = " ".join(line.split("#", 1)[0] for line in f).split()
     i = 0
     verts = tets = faces = None
     while i < len(toks):

@@ -132,10 +132,11 @@ int mgpbd_ctx_profile_read(mgpbd_ctx* ctx, double* ms_out, int64_t* count_out);
 /* Per-tet / per-vertex kernels (cons/mgpbd_gpu.py)                                      */
 /* ===================================================================================== */
 
-/* Dm^-1 (row-major 3x3 per tet), |det Dm|/6, alpha = 1/(mu dt^2 vol); mu_dev (per tet)
- * overrides the scalar mu when not NULL. Replaces _precompute_rest_shape (:136-151). */
+/* Dm^-1 (row-major 3x3 per tet), |det Dm|/6, alpha = 1/(mu dt^2 vol), evaluated in FP64 and
+ * rounded to FP32; mu_dev (per tet) overrides the scalar mu when not NULL.
+ * Replaces _precompute_rest_shape (:136-151). */
 int mgpbd_rest_shape(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev,
-                     const float* v_ref_dev, const float* mu_dev, float mu, float dt,
+                     const float* v_ref_dev, const float* mu_dev, double mu, double dt,
                      float* Dm_inv_dev, float* rest_vol_dev, float* alpha_dev);
 
 /* v += dt g (+ dt invm f_ext); v *= damp; x_old = x; x += dt v.
@@ -151,7 +152,8 @@ int mgpbd_update_vel(mgpbd_ctx* ctx, int n_verts, const float* v_p_dev,
                      const float* v_p_old_dev, float* v_v_dev, float dt, float damp);
 
 /* C = ||sigma(F) - 1||, gradC (4 x vec3 per tet, 0 when C <= 1e-6), rhs = -(C + alpha lambda)
- * and ||rhs||^2 (FP64, written to *norm2_dev). Replaces _compute_C_and_gradC (:225-247),
+ * and ||rhs||^2 (FP64, written to *norm2_dev). F, its SVD, C and gradC are evaluated in FP64
+ * from the FP32 positions and rounded once to FP32. Replaces _compute_C_and_gradC (:225-247),
  * _compute_rhs_gpu (:253-258) and _residual_norm_gpu (:260-267). */
 int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
                       const float* Dm_inv_dev, const float* alpha_dev, const float* lambda_dev,

@@ -178,11 +178,16 @@ def test_vcycle_bitexact(scene):
 
 
 @pytest.mark.parametrize("relative", [False, True])
-def test_pcg_vs_oracle_and_fp64(scene, relative):
-    mesh, v0 = scene
+def test_pcg_vs_oracle_and_fp64(relative):
+    """Converging solve (physical material): same iteration count as the FP32 oracle (+-1),
+    solution within 1e-4 (relative) of the FP64 direct solve, and for the relative test the
+    FP64-evaluated residual ||b - A x|| <= 2e-6 ||b||."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(5, seed=11)
+    v0 = deformed(mesh, seed=11, amp=0.05, twist=0.1)
     tol = 1e-6 if relative else 1e-5
-    s = _solver(mesh, v0, relative_tol=relative, tol_cg=tol)
-    o = _oracle(mesh, v0, precision="f32", relative_tol=relative, tol_cg=tol)
+    s = _solver(mesh, v0, relative_tol=relative, tol_cg=tol, mu_per_tet=mu_t)
+    o = _oracle(mesh, v0, precision="f32", relative_tol=relative, tol_cg=tol, mu_per_tet=mu_t)
     o.mg.update_fine_matrix_data(s._A.csr_data())
     o.mg.update_all_coarse_matrices()
     s._compute_C_and_gradC()
@@ -192,13 +197,33 @@ def test_pcg_vs_oracle_and_fp64(scene, relative):
     x_o = np.zeros_like(b)
     it_o = o.mg.solve(b.copy(), x_o)
     print(f"[pcg] relative={relative} iters gpu={it} oracle={it_o}", flush=True)
-    assert abs(it - it_o) <= 1
+    assert 0 < it < 100 and abs(it - it_o) <= 1
     A = s._A.to_scipy().astype(np.float64)
     res = np.linalg.norm(b - A @ x.astype(np.float64))
     if relative:
-        assert res <= 5e-6 * np.linalg.norm(b)
+        assert res <= 2e-6 * np.linalg.norm(b)
+    else:
+        assert res <= 2e-5
     x64 = sp.linalg.spsolve(A.tocsc(), b.astype(np.float64))
-    assert np.linalg.norm(x - x64) <= 1e-3 * np.linalg.norm(x64)
+    assert np.linalg.norm(x - x64) <= 1e-4 * np.linalg.norm(x64)
+    assert np.linalg.norm(x - x_o) <= 1e-4 * np.linalg.norm(x_o)
+
+
+def test_pcg_reference_scene_maxiter(scene):
+    """The reference's own setting (mu = 1e6, unit density, strained start): neither side
+    converges in 100 iterations; the GPU iterate stays within 1e-3 of the FP32 oracle's."""
+    mesh, v0 = scene
+    s = _solver(mesh, v0)
+    o = _oracle(mesh, v0, precision="f32")
+    o.mg.update_fine_matrix_data(s._A.csr_data())
+    o.mg.update_all_coarse_matrices()
+    s._compute_C_and_gradC()
+    b = s._b_gpu.cpu().numpy()
+    it = s._gpu_solver.solve(s._b_gpu, s._dlam_gpu)
+    x = s._dlam_gpu.cpu().numpy()
+    x_o = np.zeros_like(b)
+    it_o = o.mg.solve(b.copy(), x_o)
+    assert it == it_o == 100
     assert np.linalg.norm(x - x_o) <= 1e-3 * np.linalg.norm(x_o)
 
 

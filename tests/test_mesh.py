@@ -1,0 +1,63 @@
+"""Synthetic workload generator and Medit loader."""
+
+import os
+
+import numpy as np
+import pytest
+
+from controllable_pbd_3d_amd.mesh import box_mesh, ellipsoid_mesh, load_medit_mesh, vertex_mass
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+
+
+@pytest.mark.parametrize("n,verts,tets", [(8, 729, 3072), (32, 35937, 196608)])
+def test_box_counts(n, verts, tets):
+    m = box_mesh(n, 0.03, seed=0)
+    assert (m.n_verts, m.n_tets) == (verts, tets)
+    assert int(m.pinned.sum()) == (n + 1) ** 2
+
+
+def test_box_geometry_and_mass():
+    m = box_mesh(4, 0.05, seed=3, rho=1000.0)
+    t = m.tets.reshape(-1, 4)
+    x = m.verts.astype(np.float64)
+    det = np.linalg.det(np.stack([x[t[:, 1]] - x[t[:, 0]], x[t[:, 2]] - x[t[:, 0]],
+                                  x[t[:, 3]] - x[t[:, 0]]], axis=2))
+    assert np.all(det > 0)                                    # positively oriented
+    assert np.sum(det) / 6.0 == pytest.approx(0.2 ** 3, rel=1e-5)  # volume preserved
+    vm, vol = vertex_mass(m.verts, m.tets, 1000.0)
+    assert vm.sum() == pytest.approx(1000.0 * 0.2 ** 3, rel=1e-5)
+    assert np.all(m.invm[m.pinned] == 0) and np.all(m.invm[~m.pinned] > 0)
+    # jitter only moves interior vertices, by at most 0.05 h per axis
+    grid = np.round(x / m.h)
+    d = np.abs(x - grid * m.h)
+    assert d.max() <= 0.05 * m.h + 1e-7
+    on_boundary = np.any((grid == 0) | (grid == 4), axis=1)
+    assert np.all(d[on_boundary] < 1e-7)
+
+
+def test_box_seeded_and_E_log_uniform():
+    a = box_mesh(6, 0.02, seed=5, E_range=(1e4, 1e6))
+    b = box_mesh(6, 0.02, seed=5, E_range=(1e4, 1e6))
+    assert np.array_equal(a.verts, b.verts) and np.array_equal(a.E, b.E)
+    assert a.E.min() >= 1e4 * (1 - 1e-6) and a.E.max() <= 1e6 * (1 + 1e-6)
+    assert abs(np.mean(np.log10(a.E)) - 5.0) < 0.1
+
+
+def test_medit_fixtures_and_reference_mass_convention():
+    m = load_medit_mesh(os.path.join(GOLD, "simple_sphere.mesh"), fixed_ids=[0, 11])
+    assert (m.n_verts, m.n_tets) == (43, 80)
+    assert m.invm[0] == 0 and m.invm[11] == 0
+    vm, vol = vertex_mass(m.verts.astype(np.float64), m.tets, 1.0)
+    assert vm.sum() == pytest.approx(vol.sum(), rel=1e-12)   # rho = 1: mass = volume
+    for name, nv, nt in (("spot", 837, 2288), ("cube_tet", 1926, 8736)):
+        mm = load_medit_mesh(os.path.join(GOLD, name + ".mesh"))
+        assert (mm.n_verts, mm.n_tets) == (nv, nt)
+
+
+def test_ellipsoid_small():
+    m = ellipsoid_mesh(0.1, radii=(0.5, 0.3, 0.2), seed=0)
+    assert m.n_tets > 0 and m.n_tets % 6 == 0
+    assert m.tets.max() == m.n_verts - 1
+    assert np.isclose(m.pinned.mean(), 0.05, atol=0.01)
+    assert set(np.unique(m.E)) <= {1e5, 1e8}

@@ -1,0 +1,231 @@
+"""Integer / structure parity on the CPU (bit-exact): the oracle's literal restatement is
+pinned by brute force, and the product's C++ setup code must equal the oracle.
+
+Pins:
+* CSR pattern == nonzeros of B B^T (B = tet-vertex incidence), rows ascending, diagonal last;
+* face adjacency == pairs with (B B^T)_ij == 3 (three shared vertices = one shared face);
+* O(deg) candidate sets == the literal O(n) set difference (same CPython iteration order);
+* Galerkin pattern == scipy P^T A P (explicit zeros, value dropping included);
+* contribution maps == the reference's triple loop; refill sums == P^T A P values.
+"""
+
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from controllable_pbd_3d_amd import tet_aggregation as ta
+from controllable_pbd_3d_amd import _lib
+from controllable_pbd_3d_amd.mesh import box_mesh, load_medit_mesh
+from oracle import structure
+from oracle.mgpbd_oracle import galerkin_contribution_maps_fast
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "meshes")
+
+
+def incidence(tets, n_verts):
+    t = tets.reshape(-1, 4)
+    rows = np.repeat(np.arange(t.shape[0]), 4)
+    return sp.csr_matrix((np.ones(t.size), (rows, t.reshape(-1))), shape=(t.shape[0], n_verts))
+
+
+def meshes():
+    out = [("box3", box_mesh(3, 0.1, seed=0)), ("box5", box_mesh(5, 0.05, seed=1)),
+           ("box4x2x3", box_mesh(4, 0.05, seed=2, ny=2, nz=3))]
+    for f in ("simple_sphere", "spot", "cube_tet"):
+        out.append((f, load_medit_mesh(os.path.join(GOLD, f + ".mesh"))))
+    return out
+
+
+MESHES = meshes()
+
+
+def product_pattern(m):
+    lib = _lib.load()
+    p = lib.mgpbd_pattern_build(m.n_tets, m.n_verts, _lib.np_ptr(m.tets))
+    nnz = lib.mgpbd_pattern_nnz(p)
+    indptr = np.empty(m.n_tets + 1, np.int32)
+    indices = np.empty(nnz, np.int32)
+    _lib.check(lib.mgpbd_pattern_copy(p, _lib.np_ptr(indptr), _lib.np_ptr(indices)))
+    lib.mgpbd_pattern_free(p)
+    return indptr, indices
+
+
+@pytest.mark.parametrize("name,m", MESHES, ids=[n for n, _ in MESHES])
+def test_pattern(name, m):
+    indptr, indices = structure.build_sparsity_pattern(m.tets, m.n_tets)
+    B = incidence(m.tets, m.n_verts)
+    G = (B @ B.T).tocsr()
+    G.sort_indices()
+    for i in range(m.n_tets):
+        row = indices[indptr[i]:indptr[i + 1]]
+        assert row[-1] == i
+        nb = G.indices[G.indptr[i]:G.indptr[i + 1]]
+        assert np.array_equal(row[:-1], nb[nb != i])
+    pi, px = product_pattern(m)
+    assert np.array_equal(pi, indptr) and np.array_equal(px, indices)
+
+
+def test_pattern_nnz_of_the_8cube_block():
+    m = box_mesh(8, 0.05, seed=0)
+    indptr, _ = product_pattern(m)
+    assert (m.n_verts, m.n_tets, int(indptr[-1])) == (729, 3072, 174000)
+    assert int(np.diff(indptr).max()) == 71
+
+
+@pytest.mark.parametrize("name,m", MESHES, ids=[n for n, _ in MESHES])
+def test_face_adjacency(name, m):
+    adj = structure.build_tet_face_adjacency(m.tets, m.n_tets)
+    B = incidence(m.tets, m.n_verts)
+    G = (B @ B.T).tocoo()
+    pairs = {(int(i), int(j)) for i, j, v in zip(G.row, G.col, G.data) if v == 3 and i != j}
+    got = {(t, u) for t, s in adj.items() for u in s}
+    assert got == pairs
+    a, d = ta.face_adjacency_arrays(m.tets, m.n_tets)
+    for t in range(m.n_tets):
+        assert list(adj[t]) == list(a[4 * t:4 * t + d[t]]), "CPython set iteration order"
+
+
+@pytest.mark.parametrize("name,m", MESHES, ids=[n for n, _ in MESHES])
+@pytest.mark.parametrize("max_agg", [2, 5, 8])
+def test_aggregation(name, m, max_agg):
+    a_lit, n_lit = structure.tet_geometry_aggregation(m.tets, m.verts, m.n_tets, max_agg,
+                                                      literal=True)
+    a_fast, n_fast = structure.tet_geometry_aggregation(m.tets, m.verts, m.n_tets, max_agg,
+                                                        literal=False)
+    assert n_lit == n_fast and np.array_equal(a_lit, a_fast)
+    a_p, n_p = ta.tet_geometry_aggregation(m.tets, m.verts, m.n_tets, max_agg)
+    assert n_p == n_lit and np.array_equal(a_p, a_lit)
+    sizes = np.bincount(a_p)
+    assert sizes.min() >= 1 and sizes.max() <= max_agg
+
+
+@pytest.mark.parametrize("name,m", MESHES, ids=[n for n, _ in MESHES])
+def test_hierarchy(name, m):
+    Ps, sizes = structure.build_tet_hierarchy(m.tets, m.verts.astype(np.float64), m.n_tets)
+    Pp, sizes_p = ta.build_tet_hierarchy(m.tets, m.verts, m.n_tets)
+    assert sizes == sizes_p and len(Ps) <= 1
+    for P, Q in zip(Ps, Pp):
+        assert (P != Q).nnz == 0
+
+
+def _galerkin_product(indptr, indices, values, agg, n_c):
+    lib = _lib.load()
+    import ctypes
+    v = None if values is None else np.ascontiguousarray(values, np.float32)
+    g = lib.mgpbd_galerkin_build(len(indptr) - 1, _lib.np_ptr(indptr), _lib.np_ptr(indices),
+                                 None if v is None else v.ctypes.data_as(_lib.F32P),
+                                 _lib.np_ptr(agg), n_c)
+    assert g
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(lib.mgpbd_galerkin_sizes(g, ctypes.byref(a), ctypes.byref(b)))
+    ci = np.empty(n_c + 1, np.int32)
+    cx = np.empty(max(a.value, 1), np.int32)
+    cd = np.empty(n_c, np.int32)
+    fi = np.empty(max(b.value, 1), np.int32)
+    off = np.empty(a.value + 1, np.int32)
+    _lib.check(lib.mgpbd_galerkin_copy(g, _lib.np_ptr(ci), _lib.np_ptr(cx), _lib.np_ptr(cd),
+                                       _lib.np_ptr(fi), _lib.np_ptr(off)))
+    lib.mgpbd_galerkin_free(g)
+    return ci, cx[:a.value], cd, fi[:b.value], off
+
+
+@pytest.mark.parametrize("case", ["rest", "random", "sparse_zeros"])
+def test_galerkin_value_pattern(case):
+    m = box_mesh(5, 0.05, seed=4)
+    indptr, indices = structure.build_sparsity_pattern(m.tets, m.n_tets)
+    rng = np.random.default_rng(7)
+    rows = np.repeat(np.arange(m.n_tets), np.diff(indptr))
+    if case == "rest":       # A = diag(alpha) with explicit zero off-diagonals (init at rest)
+        vals = np.where(indices == rows, rng.uniform(1, 2, len(indices)), 0.0)
+    elif case == "random":
+        vals = rng.standard_normal(len(indices))
+    else:                     # some off-diagonals zero, some pairs that cancel exactly
+        vals = rng.standard_normal(len(indices))
+        vals[rng.random(len(indices)) < 0.6] = 0.0
+        vals[indices == rows] = 4.0
+    vals = vals.astype(np.float32)
+    A = sp.csr_matrix((vals.astype(np.float64), indices, indptr), shape=(m.n_tets,) * 2)
+    Ps, _ = structure.build_tet_hierarchy(m.tets, m.verts, m.n_tets)
+    P = Ps[0]
+    Ac = (P.T @ A @ P).tocsr()
+    agg = P.indices.astype(np.int32)
+    ci, cx, cd, fi, off = _galerkin_product(indptr, indices, vals, agg, P.shape[1])
+    assert np.array_equal(ci, Ac.indptr) and np.array_equal(cx, Ac.indices)
+    if case == "rest":
+        assert Ac.nnz == P.shape[1], "the coarse matrix of a rest-state A is diagonal"
+    # literal contribution maps (cons/mgpcg_gpu.py:71-92) and the vectorised form
+    Ac2, fine_idx, p_ki, p_lj, offsets = structure.galerkin_contribution_maps(A, P)
+    assert np.array_equal(fi, fine_idx) and np.array_equal(off, offsets)
+    fi2, off2 = galerkin_contribution_maps_fast(A, agg, Ac)
+    assert np.array_equal(fi2, fine_idx) and np.array_equal(off2, offsets)
+    assert np.all(p_ki == 1.0) and np.all(p_lj == 1.0)
+    # refill sums reproduce P^T A P
+    sums = np.add.reduceat(vals.astype(np.float64)[fi], off[:-1]) if len(fi) else []
+    np.testing.assert_allclose(sums, Ac.data, rtol=1e-12, atol=1e-12)
+    # the diagonal position of every coarse row
+    for I in range(P.shape[1]):
+        row = cx[ci[I]:ci[I + 1]]
+        assert cd[I] == (ci[I] + int(np.nonzero(row == I)[0][0]) if np.any(row == I) else -1)
+
+
+def test_galerkin_structural_pattern():
+    m = box_mesh(5, 0.05, seed=4)
+    indptr, indices = structure.build_sparsity_pattern(m.tets, m.n_tets)
+    S = sp.csr_matrix((np.ones(len(indices)), indices, indptr), shape=(m.n_tets,) * 2)
+    Ps, _ = structure.build_tet_hierarchy(m.tets, m.verts, m.n_tets)
+    Ac = (Ps[0].T @ S @ Ps[0]).tocsr()
+    ci, cx, cd, fi, off = _galerkin_product(indptr, indices, None,
+                                            Ps[0].indices.astype(np.int32), Ps[0].shape[1])
+    assert np.array_equal(ci, Ac.indptr) and np.array_equal(cx, Ac.indices)
+    assert len(fi) == len(indices)   # every fine entry lands in some coarse entry
+    assert np.array_equal(np.sort(fi), np.arange(len(indices)))
+
+
+def test_extended_hierarchy_invariants():
+    m = box_mesh(8, 0.05, seed=0)
+    aggs = ta.build_extended_hierarchy(m.tets, m.n_tets, max_coarse=32)
+    sizes = [m.n_tets] + [n for _, n in aggs]
+    assert sizes[-1] <= 32 and all(b < a for a, b in zip(sizes, sizes[1:]))
+    ref = ta.reference_aggregates(m.tets, m.n_tets)
+    assert np.array_equal(aggs[0][0], ref[0][0])       # level 0 is the reference's
+    for (agg, n_c), n_f in zip(aggs, sizes):
+        assert agg.shape == (n_f,) and agg.min() == 0 and agg.max() == n_c - 1
+        assert np.all(np.bincount(agg, minlength=n_c) >= 1)
+
+
+def test_sell_layout_roundtrip():
+    from controllable_pbd_3d_amd.sparse_gpu import sell_layout
+    m = box_mesh(4, 0.05, seed=0)
+    indptr, indices = product_pattern(m)
+    n = m.n_tets
+    slice_ptr, sell_idx, pos = sell_layout(indptr, indices, n)
+    assert len(np.unique(pos)) == len(pos)
+    assert np.array_equal(sell_idx[pos], indices)
+    for s in range((n + 63) // 64):
+        w = (slice_ptr[s + 1] - slice_ptr[s]) // 64
+        lens = np.diff(indptr)[64 * s:64 * s + 64]
+        assert w == lens.max()
+        for l in range(64):
+            r = 64 * s + l
+            for k in range(w):
+                c = sell_idx[slice_ptr[s] + 64 * k + l]
+                if r >= n:
+                    assert c == 0
+                elif k >= lens[l]:
+                    assert c == r
+
+
+def test_vertex_incidence():
+    m = box_mesh(3, 0.1, seed=0)
+    lib = _lib.load()
+    vptr = np.empty(m.n_verts + 1, np.int32)
+    vinc = np.empty(4 * m.n_tets, np.int32)
+    _lib.check(lib.mgpbd_vertex_incidence(m.n_tets, m.n_verts, _lib.np_ptr(m.tets),
+                                          _lib.np_ptr(vptr), _lib.np_ptr(vinc)))
+    for v in range(m.n_verts):
+        e = vinc[vptr[v]:vptr[v + 1]]
+        assert np.all(np.diff(e) > 0)
+        assert np.all(m.tets[e] == v)
+    assert vptr[-1] == 4 * m.n_tets
