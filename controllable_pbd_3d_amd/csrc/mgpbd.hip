@@ -34,8 +34,11 @@ constexpr int kMaxRedBlocks = 2048;   // grid cap for grid-stride reduction kern
 constexpr int kPartialStride = 65536; // partials per reduction slot (full-grid kernels)
 
 // ------------------------------------------------------------------------------------------
-// Deterministic FP64 reductions: block tree -> per-block partial -> the last block to arrive
-// (atomic ticket) sums the partials in a fixed order and runs the scalar epilogue.
+// Deterministic FP64 reductions in two phases. Phase 1: every block of the producing kernel
+// stores its block total with a plain store. Phase 2: a one-block kernel sums the per-block
+// totals in a fixed order and runs the scalar epilogue. The kernel boundary orders the two
+// phases, so no block pays a device-scope fence (on gfx950 an agent-scope release writes
+// back the XCD's L2, which costs ~20 ns per block) and the result is the same every run.
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -57,39 +60,28 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 struct RedWS {
-  double* partials;       // kPartialStride * 4
-  unsigned int* ticket;   // 1 counter, 0 between kernels
+  double* partials;       // kPartialStride per slot, 4 slots
 };
-
-// Thread 0 of every block stores its partial(s); returns true (in all threads) in the last
-// block to finish, after which the partials of all blocks are visible.
-__device__ __forceinline__ bool grid_arrive(RedWS ws) {
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(ws.ticket, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ __forceinline__ double grid_total(RedWS ws, int slot) {
-  const double* p = ws.partials + (size_t)slot * kPartialStride;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
-    s += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return block_sum(s);
-}
-
-__device__ __forceinline__ void grid_reset(RedWS ws) {
-  if (threadIdx.x == 0) __hip_atomic_store(ws.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 __device__ __forceinline__ void store_partial(RedWS ws, int slot, double v) {
   if (threadIdx.x == 0) ws.partials[(size_t)slot * kPartialStride + blockIdx.x] = v;
+}
+
+// Phase 2 helper (one block): total of slot `slot` over `nparts` block partials (thread 0).
+__device__ __forceinline__ double sum_partials(RedWS ws, int slot, int nparts) {
+  const double* p = ws.partials + (size_t)slot * kPartialStride;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) s += p[i];
+  return block_sum(s);
+}
+
+// out[s] = total of slot s, for s in [first, nslots)
+__global__ void __launch_bounds__(kBlock)
+k_fin_sum(RedWS ws, int nparts, int first, int nslots, double* __restrict__ out) {
+  for (int sl = first; sl < nslots; ++sl) {
+    double t = sum_partials(ws, sl, nparts);
+    if (threadIdx.x == 0) out[sl] = t;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -184,8 +176,7 @@ __global__ void __launch_bounds__(kBlock)
 k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
               const float* __restrict__ Dm_inv, const float* __restrict__ alpha,
               const float* __restrict__ lambda, float* __restrict__ C_out,
-              float* __restrict__ gradC, float* __restrict__ rhs, RedWS ws,
-              double* __restrict__ norm2) {
+              float* __restrict__ gradC, float* __restrict__ rhs, RedWS ws) {
   double acc = 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
     int4 tv = t_i[t];
@@ -236,19 +227,13 @@ k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
   }
   double bs = block_sum(acc);
   store_partial(ws, 0, bs);
-  if (grid_arrive(ws)) {
-    double tot = grid_total(ws, 0);
-    if (threadIdx.x == 0) *norm2 = tot;
-    grid_reset(ws);
-  }
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
              const float* __restrict__ dx, float omega, const float* __restrict__ Dm_inv,
              const float* __restrict__ alpha, const float* __restrict__ lambda,
-             const float* __restrict__ C_cur, int with_current, RedWS ws,
-             double* __restrict__ out) {
+             const float* __restrict__ C_cur, int with_current, RedWS ws) {
   double acc_cur = 0.0, acc_trial = 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
     int4 tv = t_i[t];
@@ -272,15 +257,6 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
   double b1 = block_sum(acc_trial);
   store_partial(ws, 0, b0);
   store_partial(ws, 1, b1);
-  if (grid_arrive(ws)) {
-    double t0 = grid_total(ws, 0);
-    double t1 = grid_total(ws, 1);
-    if (threadIdx.x == 0) {
-      if (with_current) out[0] = t0;
-      out[1] = t1;
-    }
-    grid_reset(ws);
-  }
 }
 
 // A values, one thread per row (tet) i; the row's entries are column-adjacent in SELL-64 so
@@ -460,6 +436,44 @@ k_jacobi_diag(int n, const int* __restrict__ slice_ptr, const float* __restrict_
   x[i] = xi;
 }
 
+// All `sweeps` Jacobi sweeps of a small level (n <= kSmallRows) in one workgroup: x lives in
+// LDS (two buffers), a __syncthreads separates sweeps. Each row does the same arithmetic as
+// k_sell<1>, so the result equals `sweeps` separate launches bit for bit.
+constexpr int kSmallRows = 4096;
+constexpr int kSmallBlock = 1024;
+
+__global__ void __launch_bounds__(kSmallBlock)
+k_jacobi_small(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+               const float* __restrict__ val, const float* __restrict__ dinv,
+               const float* __restrict__ b, float* __restrict__ x, float omega, int sweeps,
+               const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float xs[2][kSmallRows];
+  const int n_slices = (n + 63) >> 6;
+  for (int i = threadIdx.x; i < 64 * n_slices; i += kSmallBlock) {
+    xs[0][i] = i < n ? x[i] : 0.0f;
+    xs[1][i] = 0.0f;
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int sw = 0; sw < sweeps; ++sw) {
+    for (int i = threadIdx.x; i < 64 * n_slices; i += kSmallBlock) {
+      const int sl = i >> 6;
+      const int p0 = slice_ptr[sl] + (i & 63);
+      const int w = (slice_ptr[sl + 1] - slice_ptr[sl]) >> 6;
+      float acc = 0.0f;
+      for (int k = 0; k < w; ++k) {
+        const int p = p0 + 64 * k;
+        acc = fmaf(val[p], xs[cur][idx[p]], acc);
+      }
+      if (i < n) xs[cur ^ 1][i] = fmaf(omega * dinv[i], b[i] - acc, xs[cur][i]);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (int i = threadIdx.x; i < n; i += kSmallBlock) x[i] = xs[cur][i];
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_diag_inv(int n, const int* __restrict__ diag_pos, const float* __restrict__ val,
            float* __restrict__ dinv) {
@@ -530,18 +544,12 @@ k_blas1(int op, int n, float alpha, const float* __restrict__ a, float* __restri
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_dot(int n, const float* __restrict__ a, const float* __restrict__ b, RedWS ws,
-      double* __restrict__ out) {
+k_dot(int n, const float* __restrict__ a, const float* __restrict__ b, RedWS ws) {
   double acc = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     acc += (double)a[i] * (double)b[i];
   double bs = block_sum(acc);
   store_partial(ws, 0, bs);
-  if (grid_arrive(ws)) {
-    double tot = grid_total(ws, 0);
-    if (threadIdx.x == 0) *out = tot;
-    grid_reset(ws);
-  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -561,11 +569,10 @@ __global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
   st->tol2 = tol * tol;
 }
 
-// r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), bb = b.b
+// r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), partial b.b
 __global__ void __launch_bounds__(kBlock)
 k_pcg_init(int n, const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
-           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, RedWS ws,
-           PcgState* st) {
+           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, RedWS ws) {
   double acc = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     float bi = b[i];
@@ -574,54 +581,26 @@ k_pcg_init(int n, const float* __restrict__ b, float* __restrict__ x, float* __r
     if (xv) xv[i] = (omega * dinv0[i]) * bi;
     acc += (double)bi * (double)bi;
   }
-  double bs = block_sum(acc);
-  store_partial(ws, 0, bs);
-  if (grid_arrive(ws)) {
-    double tot = grid_total(ws, 0);
-    if (threadIdx.x == 0) {
-      st->bb = tot;
-      if (st->relative) st->tol2 = st->tol * st->tol * tot;
-    }
-    grid_reset(ws);
-  }
+  store_partial(ws, 0, block_sum(acc));
 }
 
-// MODE 0: rz = r.z (initial);  MODE 1: rz_new = r.z, breakdown test on the previous rz, beta.
-template <int MODE>
+// partial r.z
 __global__ void __launch_bounds__(kBlock)
 k_pcg_rz(int n, const float* __restrict__ r, const float* __restrict__ z, RedWS ws,
-         PcgState* st) {
+         const PcgState* st) {
   if (st->done) return;
   double acc = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     acc += (double)r[i] * (double)z[i];
-  double bs = block_sum(acc);
-  store_partial(ws, 0, bs);
-  if (grid_arrive(ws)) {
-    double tot = grid_total(ws, 0);
-    if (threadIdx.x == 0) {
-      if (MODE == 0) {
-        st->rz = tot;
-      } else {
-        st->iters = st->k + 1;
-        if (fabs(st->rz) < 1e-30) {
-          st->done = 3;
-        } else {
-          st->beta = tot / st->rz;
-          st->rz = tot;
-          st->k = st->k + 1;
-        }
-      }
-    }
-    grid_reset(ws);
-  }
+  store_partial(ws, 0, block_sum(acc));
 }
 
-// Ap = A p, pAp = p.Ap, alpha = rz / pAp (one row per thread, one slice per wavefront)
+// Ap = A p and partial p.Ap (one row per thread, one slice per wavefront)
 __global__ void __launch_bounds__(kBlock)
 k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
                const int* __restrict__ idx, const float* __restrict__ val,
-               const float* __restrict__ p, float* __restrict__ Ap, RedWS ws, PcgState* st) {
+               const float* __restrict__ p, float* __restrict__ Ap, RedWS ws,
+               const PcgState* st) {
   if (st->done) return;
   double acc = 0.0;
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -632,24 +611,14 @@ k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
       acc = (double)p[i] * (double)ax;
     }
   }
-  double bs = block_sum(acc);
-  store_partial(ws, 0, bs);
-  if (grid_arrive(ws)) {
-    double tot = grid_total(ws, 0);
-    if (threadIdx.x == 0) {
-      st->pAp = tot;
-      if (fabs(tot) < 1e-30) st->done = 2;
-      else st->alpha = st->rz / tot;
-    }
-    grid_reset(ws);
-  }
+  store_partial(ws, 0, block_sum(acc));
 }
 
-// x += alpha p, r -= alpha Ap, rr = r.r; also xv = (omega dinv0) r for the next V-cycle
+// x += alpha p, r -= alpha Ap, partial r.r; also xv = (omega dinv0) r for the next V-cycle
 __global__ void __launch_bounds__(kBlock)
 k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
              const float* __restrict__ Ap, const float* __restrict__ dinv0,
-             float* __restrict__ xv, float omega, RedWS ws, PcgState* st) {
+             float* __restrict__ xv, float omega, RedWS ws, const PcgState* st) {
   if (st->done) return;
   const float al = (float)st->alpha;
   double acc = 0.0;
@@ -660,18 +629,42 @@ k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* _
     if (xv) xv[i] = (omega * dinv0[i]) * ri;
     acc += (double)ri * (double)ri;
   }
-  double bs = block_sum(acc);
-  store_partial(ws, 0, bs);
-  if (grid_arrive(ws)) {
-    double tot = grid_total(ws, 0);
-    if (threadIdx.x == 0) {
-      st->rr = tot;
-      if (tot < st->tol2) {
-        st->done = 1;
-        st->iters = st->k + 1;
-      }
+  store_partial(ws, 0, block_sum(acc));
+}
+
+// Scalar epilogues of the PCG (cons/mgpcg_gpu.py:286-327), one block.
+enum PcgFin { FIN_BB = 0, FIN_RZ0 = 1, FIN_ALPHA = 2, FIN_RR = 3, FIN_BETA = 4 };
+
+template <int KIND>
+__global__ void __launch_bounds__(kBlock)
+k_pcg_fin(RedWS ws, int nparts, PcgState* st) {
+  if (KIND != FIN_BB && st->done) return;
+  double tot = sum_partials(ws, 0, nparts);
+  if (threadIdx.x != 0) return;
+  if (KIND == FIN_BB) {
+    st->bb = tot;
+    if (st->relative) st->tol2 = st->tol * st->tol * tot;
+  } else if (KIND == FIN_RZ0) {
+    st->rz = tot;
+  } else if (KIND == FIN_ALPHA) {
+    st->pAp = tot;
+    if (fabs(tot) < 1e-30) st->done = 2;
+    else st->alpha = st->rz / tot;
+  } else if (KIND == FIN_RR) {
+    st->rr = tot;
+    if (tot < st->tol2) {
+      st->done = 1;
+      st->iters = st->k + 1;
     }
-    grid_reset(ws);
+  } else {   // FIN_BETA: rz_new = tot; breakdown test on the previous rz
+    st->iters = st->k + 1;
+    if (fabs(st->rz) < 1e-30) {
+      st->done = 3;
+    } else {
+      st->beta = tot / st->rz;
+      st->rz = tot;
+      st->k = st->k + 1;
+    }
   }
 }
 
@@ -764,8 +757,6 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   c->stream = (hipStream_t)stream;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipMalloc(&c->ws.partials, sizeof(double) * kPartialStride * 4);
-  if (e == hipSuccess) e = hipMalloc(&c->ws.ticket, sizeof(unsigned int));
-  if (e == hipSuccess) e = hipMemset(c->ws.ticket, 0, sizeof(unsigned int));
   if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(PcgState));
   if (e == hipSuccess) e = hipMemset(c->st, 0, sizeof(PcgState));
   if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(PcgState), hipHostMallocDefault);
@@ -773,7 +764,6 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (e != hipSuccess) {
     mgpbd_set_error(std::string("mgpbd_ctx_create: ") + hipGetErrorString(e));
     if (c->ws.partials) (void)hipFree(c->ws.partials);
-    if (c->ws.ticket) (void)hipFree(c->ws.ticket);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
     delete c;
@@ -793,7 +783,6 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   (void)hipFree(c->ws.partials);
-  (void)hipFree(c->ws.ticket);
   (void)hipFree(c->st);
   (void)hipHostFree(c->st_host);
   delete c;
@@ -867,9 +856,11 @@ int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const floa
   CTX_CHECK("mgpbd_constraints");
   if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints: n_tets < 0");
   int pe = prof_begin(ctx, MGPBD_PROF_CONSTRAINTS);
-  k_constraints<<<red_blocks(n_tets), kBlock, 0, ctx->stream>>>(
+  const int g = red_blocks(n_tets);
+  k_constraints<<<g, kBlock, 0, ctx->stream>>>(
       n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv, alpha, lambda, C, gradC, rhs,
-      ctx->ws, norm2);
+      ctx->ws);
+  k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, 0, 1, norm2);
   prof_end(ctx, pe);
   MGPBD_LAUNCH_CHECK("mgpbd_constraints");
   return 0;
@@ -910,9 +901,11 @@ int mgpbd_dual_norms(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float
                      const float* lambda, const float* C, int with_current, double* out) {
   CTX_CHECK("mgpbd_dual_norms");
   if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_dual_norms: n < 0");
-  k_dual_norms<<<red_blocks(n_tets), kBlock, 0, ctx->stream>>>(
+  const int g = red_blocks(n_tets);
+  k_dual_norms<<<g, kBlock, 0, ctx->stream>>>(
       n_tets, reinterpret_cast<const int4*>(t_i), v_p, dpos, omega, Dm_inv, alpha, lambda, C,
-      with_current, ctx->ws, out);
+      with_current, ctx->ws);
+  k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, with_current ? 0 : 1, 2, out);
   MGPBD_LAUNCH_CHECK("mgpbd_dual_norms");
   return 0;
 }
@@ -1005,7 +998,9 @@ int mgpbd_negate_and_add(mgpbd_ctx* ctx, int n, const float* b, float* r) { retu
 int mgpbd_dot(mgpbd_ctx* ctx, int n, const float* a, const float* b, double* out) {
   CTX_CHECK("mgpbd_dot");
   if (n < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_dot: n < 0");
-  k_dot<<<red_blocks(n), kBlock, 0, ctx->stream>>>(n, a, b, ctx->ws, out);
+  const int g = red_blocks(n);
+  k_dot<<<g, kBlock, 0, ctx->stream>>>(n, a, b, ctx->ws);
+  k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, 0, 1, out);
   MGPBD_LAUNCH_CHECK("mgpbd_dot");
   return 0;
 }
@@ -1065,6 +1060,13 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
   if (L.diagonal) {
     k_jacobi_diag<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.val, L.dinv, b,
                                                             cur, ctx->omega, count, done);
+    return cur;
+  }
+  if (L.n <= kSmallRows && count > 1) {
+    int pe = prof_begin(ctx, MGPBD_PROF_COARSE);
+    k_jacobi_small<<<1, kSmallBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.idx, L.val, L.dinv,
+                                                       b, cur, ctx->omega, count, done);
+    prof_end(ctx, pe);
     return cur;
   }
   for (int s = 0; s < count; ++s) {
@@ -1146,12 +1148,15 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
   const int* done = &ctx->st->done;
   hipStream_t s = ctx->stream;
 
+  const int gn = red_blocks(n);
+  const int gs = nblocks(64LL * L.n_slices);
   k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
-  k_pcg_init<<<red_blocks(n), kBlock, 0, s>>>(n, b, x, r, L.dinv, first ? xv : nullptr,
-                                              ctx->omega, ctx->ws, ctx->st);
+  k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, first ? xv : nullptr, ctx->omega, ctx->ws);
+  k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
   if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
   float* zr = vcycle_from(ctx, 0, r, xv, done);
-  k_pcg_rz<0><<<red_blocks(n), kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
+  k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
+  k_pcg_fin<FIN_RZ0><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
   k_pcg_p<0><<<nblocks(n), kBlock, 0, s>>>(n, zr, p, ctx->st);
   MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
 
@@ -1160,14 +1165,17 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
     int chunk = std::min(ctx->poll, maxiter - issued);
     for (int c = 0; c < chunk; ++c) {
       int pe = prof_begin(ctx, MGPBD_PROF_SPMV_L0);
-      k_pcg_spmv_dot<<<nblocks(64LL * L.n_slices), kBlock, 0, s>>>(
-          n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap, ctx->ws, ctx->st);
+      k_pcg_spmv_dot<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
+                                           ctx->ws, ctx->st);
       prof_end(ctx, pe);
-      k_pcg_update<<<red_blocks(n), kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr,
-                                                    ctx->omega, ctx->ws, ctx->st);
+      k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
+      k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr,
+                                         ctx->omega, ctx->ws, ctx->st);
+      k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
       if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
       float* zz = vcycle_from(ctx, 0, r, xv, done);
-      k_pcg_rz<1><<<red_blocks(n), kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
+      k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
+      k_pcg_fin<FIN_BETA><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
       k_pcg_p<1><<<nblocks(n), kBlock, 0, s>>>(n, zz, p, ctx->st);
     }
     MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");

@@ -420,3 +420,49 @@ void orc_scatter(int nt, int nv, const int32_t* t_i, const real* invm, const rea
     }
   }
 }
+
+/* ---------------------------------------------------------------- CSR pattern */
+/* cons/mgpbd_gpu.py:153-194 for large meshes: row t = every other tet sharing a vertex with
+ * t, ascending, then t. Pass indices == NULL to get row lengths (into indptr[1..]) only. */
+static int cmp_i32(const void* a, const void* b) {
+  int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+  return (x > y) - (x < y);
+}
+
+int64_t orc_pattern(int nt, int nv, const int32_t* t_i, int32_t* indptr, int32_t* indices) {
+  int32_t* vptr = calloc((size_t)nv + 1, sizeof(int32_t));
+  int32_t* vt = malloc(sizeof(int32_t) * 4 * (size_t)nt);
+  int32_t* fill = malloc(sizeof(int32_t) * ((size_t)nv + 1));
+  for (int64_t k = 0; k < 4 * (int64_t)nt; ++k) vptr[t_i[k] + 1]++;
+  for (int v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
+  memcpy(fill, vptr, sizeof(int32_t) * nv);
+  for (int t = 0; t < nt; ++t)
+    for (int l = 0; l < 4; ++l) vt[fill[t_i[4 * (int64_t)t + l]]++] = t;
+  int cap = 1024;
+  int32_t* buf = malloc(sizeof(int32_t) * cap);
+  int64_t nnz = 0;
+  if (!indices) indptr[0] = 0;
+  for (int t = 0; t < nt; ++t) {
+    int m = 0;
+    for (int l = 0; l < 4; ++l) {
+      int v = t_i[4 * (int64_t)t + l];
+      int len = vptr[v + 1] - vptr[v];
+      if (m + len > cap) { cap = 2 * (m + len); buf = realloc(buf, sizeof(int32_t) * cap); }
+      memcpy(buf + m, vt + vptr[v], sizeof(int32_t) * len);
+      m += len;
+    }
+    qsort(buf, m, sizeof(int32_t), cmp_i32);
+    int u = 0;
+    for (int k = 0; k < m; ++k)
+      if ((k == 0 || buf[k] != buf[k - 1]) && buf[k] != t) buf[u++] = buf[k];
+    if (indices) {
+      memcpy(indices + indptr[t], buf, sizeof(int32_t) * u);
+      indices[indptr[t] + u] = t;
+    } else {
+      indptr[t + 1] = indptr[t] + u + 1;
+    }
+    nnz += u + 1;
+  }
+  free(buf); free(fill); free(vt); free(vptr);
+  return nnz;
+}

@@ -61,6 +61,8 @@ def lib(precision: str = "f32"):
                               ctypes.c_double, ctypes.c_int, R, R, R, F64P]
         L.orc_pcg.restype = ctypes.c_int
         L.orc_scatter.argtypes = [ctypes.c_int, ctypes.c_int, I32P, R, R, R, R, R]
+        L.orc_pattern.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, I32P]
+        L.orc_pattern.restype = ctypes.c_int64
         _LIBS[precision] = (L, R, np.float32 if precision == "f32" else np.float64)
     return _LIBS[precision]
 
@@ -72,12 +74,25 @@ def ptr(a: np.ndarray | None, P):
     return a.ctypes.data_as(P)
 
 
+def sparsity_pattern(t_i: np.ndarray, n_tets: int, n_verts: int):
+    """CSR pattern of A (cons/mgpbd_gpu.py:153-194): the literal set-based restatement for
+    small meshes, the C restatement ``orc_pattern`` (same rows) above 20,000 tets."""
+    if n_tets <= 20000:
+        return structure.build_sparsity_pattern(t_i, n_tets)
+    L, _, _ = lib("f32")
+    indptr = np.zeros(n_tets + 1, np.int32)
+    nnz = L.orc_pattern(n_tets, n_verts, ptr(t_i, I32P), ptr(indptr, I32P), ptr(None, I32P))
+    indices = np.zeros(nnz, np.int32)
+    L.orc_pattern(n_tets, n_verts, ptr(t_i, I32P), ptr(indptr, I32P), ptr(indices, I32P))
+    return indptr, indices
+
+
 def galerkin_contribution_maps_fast(A_fine: sp.csr_matrix, agg: np.ndarray, Ac: sp.csr_matrix):
     """Vectorised equivalent of ``structure.galerkin_contribution_maps`` for unit
     piecewise-constant P: the contributions of coarse entry (I, J) are the fine CSR positions
     a with (agg[row(a)], agg[col(a)]) == (I, J) in ascending a -- the reference's (fine row of
     I ascending, CSR entry) order, because CSR positions grow with the row. Checked against
-    the literal triple loop in tests/test_oracle_structure.py."""
+    the literal triple loop in tests/test_structure.py."""
     n_f = A_fine.shape[0]
     n_c = Ac.shape[0]
     rows = np.repeat(np.arange(n_f, dtype=np.int64), np.diff(A_fine.indptr))
@@ -263,23 +278,28 @@ class OracleMGPBD:
     def _p(self, a):
         return ptr(a, self.R)
 
-    def init(self):
-        """cons/mgpbd_gpu.py:92-114."""
+    def init(self, Ps=None, structural=False):
+        """cons/mgpbd_gpu.py:92-114. ``Ps`` replaces the reference tet hierarchy (used to give
+        the oracle the same extended levels as a GPU run); ``structural`` builds coarse
+        patterns from the structure of A instead of its init-time values."""
         L = self.L
         L.orc_rest_shape(self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p_ref),
                          self._p(self.mu_per_tet), float(self.mu), float(self.dt),
                          self._p(self.Dm_inv), self._p(self.rest_vol), self._p(self.alpha_tilde))
-        self.indptr, self.indices = structure.build_sparsity_pattern(self.t_i, self.n_tets)
+        self.indptr, self.indices = sparsity_pattern(self.t_i, self.n_tets, self.n_verts)
         self._compute_C_and_gradC()
         data = np.zeros(len(self.indices), self.dtype)
         self._fill_A(data)
         A = sp.csr_matrix((data.astype(np.float32).astype(np.float64), self.indices.copy(),
                            self.indptr.copy()), shape=(self.n_tets, self.n_tets))
-        Ps, sizes = structure.build_tet_hierarchy(self.t_i, self.v_p_ref.astype(np.float64),
-                                                  self.n_tets)
+        if Ps is None:
+            Ps, sizes = structure.build_tet_hierarchy(self.t_i, self.v_p_ref.astype(np.float64),
+                                                      self.n_tets)
+        else:
+            sizes = [self.n_tets] + [P.shape[1] for P in Ps]
         self.Ps, self.sizes = Ps, sizes
         self.mg = OracleMG(self.n_smooth, self.omega_jacobi, self.maxiter_cg, self.tol_cg,
-                           self.precision, relative=self.relative_tol)
+                           self.precision, relative=self.relative_tol, structural=structural)
         self.mg.setup(A, Ps)
         self.A_data = data
 

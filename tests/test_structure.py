@@ -229,3 +229,18 @@ def test_vertex_incidence():
         assert np.all(np.diff(e) > 0)
         assert np.all(m.tets[e] == v)
     assert vptr[-1] == 4 * m.n_tets
+
+
+def test_oracle_c_pattern_equals_literal():
+    from oracle import mgpbd_oracle as om
+    import ctypes
+    for name, m in MESHES:
+        L, _, _ = om.lib("f32")
+        indptr = np.zeros(m.n_tets + 1, np.int32)
+        nnz = L.orc_pattern(m.n_tets, m.n_verts, om.ptr(m.tets, om.I32P),
+                            om.ptr(indptr, om.I32P), om.ptr(None, om.I32P))
+        indices = np.zeros(nnz, np.int32)
+        L.orc_pattern(m.n_tets, m.n_verts, om.ptr(m.tets, om.I32P), om.ptr(indptr, om.I32P),
+                      om.ptr(indices, om.I32P))
+        ip, ix = structure.build_sparsity_pattern(m.tets, m.n_tets)
+        assert np.array_equal(indptr, ip) and np.array_equal(indices, ix), name
