@@ -50,6 +50,7 @@ SIGNATURES = {
     "mgpbd_ctx_destroy": (None, [_V]),
     "mgpbd_ctx_sync": (c_int, [_V]),
     "mgpbd_ctx_profile": (c_int, [_V, c_int]),
+    "mgpbd_ctx_use_graphs": (c_int, [_V, c_int, c_int]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),
@@ -78,6 +79,7 @@ SIGNATURES = {
     # multigrid / PCG
     "mgpbd_mg_configure": (c_int, [_V, c_int, c_int, c_float, c_int, c_int]),
     "mgpbd_mg_set_level": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, _V, _V, _V, c_int]),
+    "mgpbd_mg_factor_coarsest": (c_int, [_V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_pcg_solve": (c_int, [_V, _V, _V, _V, c_int, c_double, c_int,
                                 ctypes.POINTER(c_int), F64P]),

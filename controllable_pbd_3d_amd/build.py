@@ -43,7 +43,8 @@ def build(verbose: bool = False) -> str:
              verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"], verbose)
+    _run([hipcc, "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp", "-L/opt/rocm/lib",
+          "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

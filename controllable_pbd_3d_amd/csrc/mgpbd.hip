@@ -24,6 +24,8 @@
 #include <string>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "mgpbd_internal.h"
 #include "svd3.h"
 
@@ -474,6 +476,52 @@ k_jacobi_small(int n, const int* __restrict__ slice_ptr, const int* __restrict__
   for (int i = threadIdx.x; i < n; i += kSmallBlock) x[i] = xs[cur][i];
 }
 
+// Coarsest-level direct solve (FP64): the level's SELL values are expanded into a dense
+// column-major n x n matrix, factored and inverted by rocSOLVER (dpotrf + dpotri, lower
+// triangle), mirrored to a full symmetric inverse, and every V-cycle applies it as a dense
+// mat-vec with FP64 accumulation.
+__global__ void __launch_bounds__(kBlock)
+k_dense_from_sell(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+                  const float* __restrict__ val, double* __restrict__ D) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  for (int j = 0; j < n; ++j) D[(size_t)j * n + i] = 0.0;
+  const int sl = i >> 6;
+  const int p0 = slice_ptr[sl] + (i & 63);
+  const int w = (slice_ptr[sl + 1] - slice_ptr[sl]) >> 6;
+  for (int k = 0; k < w; ++k) {
+    const int p = p0 + 64 * k;
+    const float v = val[p];
+    if (v != 0.0f) D[(size_t)idx[p] * n + i] += (double)v;   // column idx, row i
+  }
+}
+
+// copy the lower triangle (column-major) onto the upper one
+__global__ void __launch_bounds__(kBlock)
+k_symmetrize_lower(int n, double* __restrict__ D) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n * n) return;
+  const int i = t % n, j = t / n;   // element (i, j)
+  if (i < j) D[(size_t)j * n + i] = D[(size_t)i * n + j];
+}
+
+// x = D b for a dense symmetric D (column-major): 64 rows per block, the block's 4 waves
+// split the columns, FP64 accumulation.
+__global__ void __launch_bounds__(kBlock)
+k_dense_matvec(int n, const double* __restrict__ D, const float* __restrict__ b,
+               float* __restrict__ x, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + lane;
+  double acc = 0.0;
+  if (i < n)
+    for (int j = w; j < n; j += 4) acc += D[(size_t)j * n + i] * (double)b[j];
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && i < n) x[i] = (float)((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_diag_inv(int n, const int* __restrict__ diag_pos, const float* __restrict__ val,
            float* __restrict__ dinv) {
@@ -706,9 +754,22 @@ struct mgpbd_ctx {
   PcgState* st = nullptr;
   PcgState* st_host = nullptr;   // pinned mirror
   int n_levels = 0, n_smooth = 2, coarse_mode = 0, coarse_sweeps = 50;
+  // dense coarsest-level inverse (coarse_mode == MGPBD_COARSE_CHOLESKY)
+  rocblas_handle blas = nullptr;
+  double* dense = nullptr;
+  size_t dense_cap = 0;
+  rocblas_int* dense_info = nullptr;
+  bool dense_ready = false;
   float omega = 2.0f / 3.0f;
   MgLevel L[MGPBD_MAX_LEVELS];
-  int poll = 2;
+  int poll = 4;
+  // HIP graph of `poll` PCG iterations (+ the state copy), replayed on `work` stream
+  bool use_graph = true;
+  hipStream_t work = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t graph = nullptr;
+  const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  int graph_chunk = 0;
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   std::vector<hipEvent_t> ev;     // 2 per recorded launch
@@ -760,6 +821,9 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(PcgState));
   if (e == hipSuccess) e = hipMemset(c->st, 0, sizeof(PcgState));
   if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(PcgState), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->work, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     mgpbd_set_error(std::string("mgpbd_ctx_create: ") + hipGetErrorString(e));
@@ -782,6 +846,13 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+  if (c->ev_out) (void)hipEventDestroy(c->ev_out);
+  if (c->work) (void)hipStreamDestroy(c->work);
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
+  if (c->dense) (void)hipFree(c->dense);
+  if (c->dense_info) (void)hipFree(c->dense_info);
   (void)hipFree(c->ws.partials);
   (void)hipFree(c->st);
   (void)hipHostFree(c->st_host);
@@ -1012,8 +1083,10 @@ int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega, 
                        int coarse_sweeps) {
   CTX_CHECK("mgpbd_mg_configure");
   if (n_levels < 1 || n_levels > MGPBD_MAX_LEVELS || n_smooth < 0 || coarse_sweeps < 1 ||
-      coarse_mode != MGPBD_COARSE_JACOBI)
+      (coarse_mode != MGPBD_COARSE_JACOBI && coarse_mode != MGPBD_COARSE_CHOLESKY))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_configure: invalid arguments");
+  ctx->dense_ready = false;
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   ctx->n_levels = n_levels;
   ctx->n_smooth = n_smooth;
   ctx->omega = omega;
@@ -1034,6 +1107,11 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
   bool coarsest = (level == ctx->n_levels - 1);
   if (!coarsest && (!agg || !mptr || !mem))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level: transfer arrays required above the coarsest level");
+  if (coarsest && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && n_rows > MGPBD_MAX_DENSE)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level: coarsest level too large for the dense "
+                                    "solve (" + std::to_string(n_rows) + " > " +
+                                    std::to_string(MGPBD_MAX_DENSE) + ")");
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   MgLevel& L = ctx->L[level];
   L.n = n_rows;
   L.n_slices = (n_rows + 63) / 64;
@@ -1047,6 +1125,16 @@ static int check_levels(mgpbd_ctx* ctx) {
   if (ctx->n_levels < 1) return mgpbd_fail(MGPBD_EINVAL, "multigrid: not configured");
   for (int l = 0; l < ctx->n_levels; ++l)
     if (ctx->L[l].n <= 0) return mgpbd_fail(MGPBD_EINVAL, "multigrid: level " + std::to_string(l) + " not set");
+  return 0;
+}
+
+static int check_ready(mgpbd_ctx* ctx) {
+  if (int e = check_levels(ctx)) return e;
+  if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && !ctx->dense_ready)
+    return mgpbd_fail(MGPBD_EINVAL, "multigrid: coarsest level not factored "
+                                    "(call mgpbd_mg_factor_coarsest after its values change)");
+  if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels < 2)
+    return mgpbd_fail(MGPBD_EINVAL, "multigrid: the dense coarsest solve needs two levels or more");
   return 0;
 }
 
@@ -1080,6 +1168,44 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
   return cur;
 }
 
+int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
+  CTX_CHECK("mgpbd_mg_factor_coarsest");
+  if (int e = check_levels(ctx)) return e;
+  if (ctx->coarse_mode != MGPBD_COARSE_CHOLESKY) return 0;
+  MgLevel& L = ctx->L[ctx->n_levels - 1];
+  const int n = L.n;
+  const size_t need = (size_t)n * n;
+  if (need > ctx->dense_cap) {
+    if (ctx->dense) MGPBD_HIP_TRY(hipFree(ctx->dense));
+    ctx->dense = nullptr;
+    MGPBD_HIP_TRY(hipMalloc(&ctx->dense, need * sizeof(double)));
+    ctx->dense_cap = need;
+  }
+  if (!ctx->dense_info) MGPBD_HIP_TRY(hipMalloc(&ctx->dense_info, sizeof(rocblas_int)));
+  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
+    return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_create_handle failed");
+  if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+    return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_set_stream failed");
+  k_dense_from_sell<<<nblocks(n), kBlock, 0, ctx->stream>>>(n, L.slice_ptr, L.idx, L.val, ctx->dense);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(assemble)");
+  if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
+      rocblas_status_success)
+    return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotrf failed to launch");
+  rocblas_int info = 0;
+  MGPBD_HIP_TRY(hipMemcpyAsync(&info, ctx->dense_info, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
+  MGPBD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (info != 0)
+    return mgpbd_fail(MGPBD_ENONFINITE, "coarsest level is not positive definite (dpotrf info " +
+                                            std::to_string(info) + ")");
+  if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
+      rocblas_status_success)
+    return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotri failed to launch");
+  k_symmetrize_lower<<<nblocks((int64_t)n * n), kBlock, 0, ctx->stream>>>(n, ctx->dense);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(symmetrize)");
+  ctx->dense_ready = true;
+  return 0;
+}
+
 // V-cycle on level l >= 0 for rhs b with the first pre-smooth sweep (x = omega D^-1 b, or
 // x = 0 when n_smooth == 0) already in x_first (one of the level's x buffers). Returns the
 // buffer holding the result.
@@ -1087,6 +1213,10 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   MgLevel& L = ctx->L[l];
   const bool coarsest = (l == ctx->n_levels - 1);
   if (coarsest) {
+    if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY) {
+      k_dense_matvec<<<(L.n + 63) / 64, kBlock, 0, ctx->stream>>>(L.n, ctx->dense, b, x_first, done);
+      return x_first;
+    }
     int total = std::min(ctx->coarse_sweeps, L.n);
     return sweeps(ctx, l, b, x_first, total - 1, done);
   }
@@ -1106,11 +1236,12 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   float* bc = C.work;
   float* xc = C.work + (size_t)C.n;
   const bool c_coarsest = (l + 1 == ctx->n_levels - 1);
-  const bool c_first = c_coarsest || ctx->n_smooth > 0;
+  const bool c_dense = c_coarsest && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY;
+  const bool c_first = !c_dense && (c_coarsest || ctx->n_smooth > 0);
   k_restrict<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, L.mptr, L.mem, r, bc,
                                                        C.dinv, c_first ? xc : nullptr,
                                                        ctx->omega, done);
-  if (!c_first) k_blas1<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(3, C.n, 0.f, nullptr, xc);
+  if (!c_first && !c_dense) k_blas1<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(3, C.n, 0.f, nullptr, xc);
   float* xcr = vcycle_from(ctx, l + 1, bc, xc, done);
   k_prolong<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.agg, xcr, x, done);
   return sweeps(ctx, l, b, x, ctx->n_smooth, done);
@@ -1118,7 +1249,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
 
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   CTX_CHECK("mgpbd_mg_vcycle");
-  if (int e = check_levels(ctx)) return e;
+  if (int e = check_ready(ctx)) return e;
   MgLevel& L = ctx->L[0];
   float* x1 = L.work + (size_t)L.n;
   const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
@@ -1130,10 +1261,44 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   return 0;
 }
 
+// One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
+static void pcg_iteration(mgpbd_ctx* ctx, float* x, float* r, float* p, float* Ap, float* xv,
+                          bool first, int gn, int gs, bool prof) {
+  MgLevel& L = ctx->L[0];
+  const int n = L.n;
+  const int* done = &ctx->st->done;
+  hipStream_t s = ctx->stream;
+  int pe = prof ? prof_begin(ctx, MGPBD_PROF_SPMV_L0) : -1;
+  k_pcg_spmv_dot<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
+                                       ctx->ws, ctx->st);
+  prof_end(ctx, pe);
+  k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
+  k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr, ctx->omega,
+                                     ctx->ws, ctx->st);
+  k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+  if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
+  float* zz = vcycle_from(ctx, 0, r, xv, done);
+  k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
+  k_pcg_fin<FIN_BETA><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+  k_pcg_p<1><<<nblocks(n), kBlock, 0, s>>>(n, zz, p, ctx->st);
+}
+
+int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll) {
+  CTX_CHECK("mgpbd_ctx_use_graphs");
+  if (poll < 1 || poll > 64) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_use_graphs: poll in [1, 64]");
+  ctx->use_graph = enable != 0;
+  if (ctx->poll != poll && ctx->graph) {
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  ctx->poll = poll;
+  return 0;
+}
+
 int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int maxiter,
                     double tol, int relative, int* iters_out, double* info_out) {
   CTX_CHECK("mgpbd_pcg_solve");
-  if (int e = check_levels(ctx)) return e;
+  if (int e = check_ready(ctx)) return e;
   if (!b || !x || !work || maxiter < 0)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_solve: invalid arguments");
   MgLevel& L = ctx->L[0];
@@ -1146,10 +1311,47 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
   float* xv = L.work + (size_t)n;   // level-0 x buffer receiving the first sweep
   const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
   const int* done = &ctx->st->done;
-  hipStream_t s = ctx->stream;
-
   const int gn = red_blocks(n);
   const int gs = nblocks(64LL * L.n_slices);
+  // Graph replay runs on the library's own stream, ordered after the caller's stream; the
+  // profiling mode launches eagerly on the caller's stream (events cannot be captured).
+  const bool graph_mode = ctx->use_graph && !ctx->prof;
+  hipStream_t caller = ctx->stream;
+  if (graph_mode) {
+    MGPBD_HIP_TRY(hipEventRecord(ctx->ev_in, caller));
+    MGPBD_HIP_TRY(hipStreamWaitEvent(ctx->work, ctx->ev_in, 0));
+    ctx->stream = ctx->work;
+  }
+  struct Restore {
+    mgpbd_ctx* c; hipStream_t s;
+    ~Restore() { c->stream = s; }
+  } restore{ctx, caller};
+  hipStream_t s = ctx->stream;
+
+  // capture (or reuse) the iteration graph before any work is queued on the stream
+  if (graph_mode) {
+    const void* key[4] = {b, x, work, L.work};
+    bool stale = !ctx->graph || ctx->graph_chunk != ctx->poll ||
+                 std::memcmp(key, ctx->graph_key, sizeof(key)) != 0;
+    if (stale) {
+      if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+      hipGraph_t g = nullptr;
+      MGPBD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      for (int c = 0; c < ctx->poll; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, first, gn, gs, false);
+      (void)hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s);
+      hipError_t ce = hipStreamEndCapture(s, &g);
+      if (ce != hipSuccess) return mgpbd_fail(MGPBD_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
+      ce = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ce != hipSuccess) {
+        ctx->graph = nullptr;
+        return mgpbd_fail(MGPBD_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ce));
+      }
+      std::memcpy(ctx->graph_key, key, sizeof(key));
+      ctx->graph_chunk = ctx->poll;
+    }
+  }
+
   k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
   k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, first ? xv : nullptr, ctx->omega, ctx->ws);
   k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
@@ -1162,25 +1364,16 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
 
   int issued = 0;
   while (issued < maxiter) {
-    int chunk = std::min(ctx->poll, maxiter - issued);
-    for (int c = 0; c < chunk; ++c) {
-      int pe = prof_begin(ctx, MGPBD_PROF_SPMV_L0);
-      k_pcg_spmv_dot<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
-                                           ctx->ws, ctx->st);
-      prof_end(ctx, pe);
-      k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
-      k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr,
-                                         ctx->omega, ctx->ws, ctx->st);
-      k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-      if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
-      float* zz = vcycle_from(ctx, 0, r, xv, done);
-      k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
-      k_pcg_fin<FIN_BETA><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-      k_pcg_p<1><<<nblocks(n), kBlock, 0, s>>>(n, zz, p, ctx->st);
+    if (graph_mode && maxiter - issued >= ctx->poll) {
+      MGPBD_HIP_TRY(hipGraphLaunch(ctx->graph, s));
+      issued += ctx->poll;
+    } else {
+      int chunk = std::min(ctx->poll, maxiter - issued);
+      for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, first, gn, gs, true);
+      MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
+      issued += chunk;
+      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     }
-    MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
-    issued += chunk;
-    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     MGPBD_HIP_TRY(hipStreamSynchronize(s));
     if (int e = prof_flush(ctx)) return e;
     if (ctx->st_host->done) break;
@@ -1188,6 +1381,10 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
   if (issued == 0) {
     MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     MGPBD_HIP_TRY(hipStreamSynchronize(s));
+  }
+  if (graph_mode) {
+    MGPBD_HIP_TRY(hipEventRecord(ctx->ev_out, s));
+    MGPBD_HIP_TRY(hipStreamWaitEvent(caller, ctx->ev_out, 0));
   }
   if (iters_out) *iters_out = ctx->st_host->iters;
   if (info_out) {

@@ -8,6 +8,9 @@ Differences in mechanism, not in results:
 * the PCG keeps alpha, beta and the stopping test on the device (FP64 reductions) and the
   host polls a done flag every few iterations instead of syncing three times per iteration;
 * Jacobi ping-pongs two buffers instead of copying x_new back to x.
+
+Extension (keyword ``coarse_solver='cholesky'``): the coarsest level is solved exactly with
+an FP64 dense inverse (refactored on every coarse refill) instead of min(50, n) sweeps.
 """
 
 from __future__ import annotations
@@ -48,8 +51,8 @@ class GPUMultigridSolver:
 
     def __init__(self, n_smooth: int = 2, omega_jacobi: float = 2.0 / 3.0,
                  maxiter_cg: int = 100, tol_cg: float = 1e-5, *, relative: bool = False,
-                 coarse_sweeps: int = 50, galerkin: str = "values", poll: int = 2,
-                 device=None):
+                 coarse_sweeps: int = 50, galerkin: str = "values",
+                 coarse_solver: str = "jacobi", poll: int = 2, device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
         self.maxiter_cg = int(maxiter_cg)
@@ -59,6 +62,9 @@ class GPUMultigridSolver:
         if galerkin not in ("values", "structural"):
             raise ValueError("galerkin must be 'values' (reference) or 'structural'")
         self.galerkin = galerkin
+        if coarse_solver not in ("jacobi", "cholesky"):
+            raise ValueError("coarse_solver must be 'jacobi' (reference) or 'cholesky'")
+        self.coarse_solver = coarse_solver
         self.n_levels = 0
         self._initialized = False
         self.ctx = get_context(device)
@@ -131,7 +137,6 @@ class GPUMultigridSolver:
             F = C
         self.n_levels = len(self.levels)
         self._level_sizes = [L.n_rows for L in self.levels]
-        self.update_all_coarse_matrices()
         # work vectors per level: b, x, x', r
         self._work = [torch.zeros(4 * L.n_rows, dtype=torch.float32, device=self.device)
                       for L in self.levels]
@@ -141,6 +146,7 @@ class GPUMultigridSolver:
         self._x = [torch.zeros(L.n_rows, dtype=torch.float32, device=self.device)
                    for L in self.levels]
         self._configure()
+        self.update_all_coarse_matrices()
         self._initialized = True
 
     @staticmethod
@@ -153,8 +159,10 @@ class GPUMultigridSolver:
 
     def _configure(self):
         ctx = self.ctx
-        ctx("mgpbd_mg_configure", self.n_levels, self.n_smooth, self.omega_jacobi, 0,
+        mode = 1 if (self.coarse_solver == "cholesky" and self.n_levels > 1) else 0
+        ctx("mgpbd_mg_configure", self.n_levels, self.n_smooth, self.omega_jacobi, mode,
             self.coarse_sweeps)
+        self._coarse_mode = mode
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None
             ctx("mgpbd_mg_set_level", l, L.n_rows, dptr(L.slice_ptr), dptr(L.sell_idx),
@@ -178,6 +186,8 @@ class GPUMultigridSolver:
                 self.ctx("mgpbd_coarse_refill", rf["n"], dptr(rf["off"]), dptr(rf["fpos"]),
                          dptr(rf["cpos"]), dptr(F.data), dptr(C.data))
             C.cache_diag_inv()
+        if getattr(self, "_coarse_mode", 0) == 1:
+            self.ctx("mgpbd_mg_factor_coarsest")
 
     # -- solve -----------------------------------------------------------------------------
     def vcycle(self, level: int = 0):

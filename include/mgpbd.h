@@ -231,6 +231,8 @@ int mgpbd_negate_and_add(mgpbd_ctx* ctx, int n, const float* b_dev, float* r_dev
 
 #define MGPBD_MAX_LEVELS 12
 #define MGPBD_COARSE_JACOBI 0   /* min(coarse_sweeps, n) weighted-Jacobi sweeps from 0 */
+#define MGPBD_COARSE_CHOLESKY 1 /* exact FP64 solve with the dense inverse (extension) */
+#define MGPBD_MAX_DENSE 4096    /* largest coarsest level for MGPBD_COARSE_CHOLESKY */
 
 /* (Re)declare the hierarchy: n_levels levels, level 0 the finest. */
 int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega,
@@ -244,6 +246,11 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
                        const float* diag_inv_dev, const int32_t* agg_dev,
                        const int32_t* mem_ptr_dev, const int32_t* mem_dev, float* work_dev,
                        int diagonal_only);
+/* Coarsest level with MGPBD_COARSE_CHOLESKY: assemble it densely in FP64, factor and invert
+ * (rocSOLVER dpotrf/dpotri), keep the symmetric inverse. Call after the coarsest values
+ * change (every coarse refill). Synchronises; MGPBD_ENONFINITE if not positive definite.
+ * No reference counterpart (the reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
+int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
 /* x = M^-1 b with one V-cycle from x = 0. Replaces vcycle(0) (cons/mgpcg_gpu.py:214-259). */
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
 /* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:
@@ -253,6 +260,11 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
  * info_out (host, 4 doubles) = {||r||^2, ||b||^2, stop reason, 0}. */
 int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* work_dev,
                     int maxiter, double tol, int relative, int* iters_out, double* info_out);
+/* PCG launch mode (no reference counterpart): with enable != 0 (default) the iterations run
+ * as a captured HIP graph of `poll` iterations on a library-owned stream ordered after the
+ * context stream; the host checks the done flag after every `poll` iterations (default 4).
+ * Iterations past convergence are skipped on the device by the done flag. */
+int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll);
 
 #ifdef __cplusplus
 }

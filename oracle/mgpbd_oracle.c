@@ -318,6 +318,7 @@ typedef struct {
   const int32_t *tip, *tix; /* P_l^T */
   const real* ta;
   real *b, *x, *xn, *r, *tmp;
+  const double* dense;      /* coarsest only: dense column-major inverse, or NULL (sweeps) */
 } OrcLevel;
 
 typedef struct {
@@ -329,6 +330,15 @@ typedef struct {
 static void vc(const OrcMG* mg, int l) {
   const OrcLevel* A = &mg->L[l];
   int n = A->n;
+  if (l == mg->nl - 1 && A->dense) {
+    /* x = A^-1 b with FP64 accumulation; columns split in four interleaved partial sums */
+    for (int i = 0; i < n; ++i) {
+      double p[4] = {0, 0, 0, 0};
+      for (int j = 0; j < n; ++j) p[j & 3] += A->dense[(int64_t)j * n + i] * (double)A->b[j];
+      A->x[i] = (real)((p[0] + p[1]) + (p[2] + p[3]));
+    }
+    return;
+  }
   if (l == mg->nl - 1) {
     int it = n < 50 ? n : 50;
     for (int s = 0; s < it; ++s) {

@@ -121,7 +121,8 @@ def _level_struct(R):
                     ("ip", I32P), ("ix", I32P), ("a", R), ("dinv", R),
                     ("pip", I32P), ("pix", I32P), ("pa", R),
                     ("tip", I32P), ("tix", I32P), ("ta", R),
-                    ("b", R), ("x", R), ("xn", R), ("r", R), ("tmp", R)]
+                    ("b", R), ("x", R), ("xn", R), ("r", R), ("tmp", R),
+                    ("dense", F64P)]
     return OrcLevel
 
 
@@ -130,7 +131,7 @@ class OracleMG:
     value-dependent pattern, contribution-map refill, V-cycle, PCG."""
 
     def __init__(self, n_smooth=2, omega_jacobi=2.0 / 3.0, maxiter_cg=100, tol_cg=1e-5,
-                 precision="f32", relative=False, structural=False):
+                 precision="f32", relative=False, structural=False, coarse_solver="jacobi"):
         self.n_smooth = n_smooth
         self.omega = omega_jacobi
         self.maxiter_cg = maxiter_cg
@@ -138,6 +139,7 @@ class OracleMG:
         self.precision = precision
         self.relative = relative
         self.structural = structural
+        self.coarse_solver = coarse_solver
         self.L, self.R, self.dt = lib(precision)
 
     def setup(self, A0: sp.csr_matrix, Ps: list):
@@ -194,6 +196,7 @@ class OracleMG:
                 setattr(s, k, ptr(lv.get(k), R))
             for k in ("b", "x", "xn", "r", "tmp"):
                 setattr(s, k, ptr(lv[k], R))
+            s.dense = ptr(lv.get("dense"), F64P)
         self._structs = arr
 
     def update_fine_matrix_data(self, data):
@@ -212,6 +215,21 @@ class OracleMG:
                               ptr(ones, self.R), ptr(f["a"], self.R), ptr(c["a"], self.R))
             self.L.orc_cache_diag_inv(c["n"], ptr(c["ip"], I32P), ptr(c["ix"], I32P),
                                       ptr(c["a"], self.R), ptr(c["dinv"], self.R))
+        self._factor_coarsest()
+
+    def _factor_coarsest(self):
+        """Dense FP64 inverse of the coarsest level (numpy Cholesky), for coarse_solver
+        'cholesky' (extension; the reference smooths the coarsest level instead)."""
+        if self.coarse_solver != "cholesky" or len(self.levels) < 2:
+            return
+        c = self.levels[-1]
+        A = sp.csr_matrix((c["a"].astype(np.float64), c["ix"], c["ip"]),
+                          shape=(c["n"], c["n"])).toarray()
+        Lc = np.linalg.cholesky(A)
+        Li = np.linalg.inv(Lc)
+        inv = Li.T @ Li
+        c["dense"] = np.asfortranarray(inv).reshape(-1, order="F").copy()
+        self._build_structs()
 
     def vcycle(self, r):
         z = np.zeros_like(r)

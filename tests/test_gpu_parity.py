@@ -272,3 +272,66 @@ def test_substeps_positions(mode):
           flush=True)
     assert worst["f32"] <= 1e-4 * mesh.h
     assert worst["f64"] <= 1e-3 * mesh.h
+
+
+def _extended_pair(n=6, seed=2, coarse_solver="cholesky"):
+    """GPU solver on the extended hierarchy and an FP32 oracle with the same levels and the
+    same level-0 values."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(n, seed=seed)
+    v0 = deformed(mesh, seed=seed, amp=0.05, twist=0.1)
+    s = _solver(mesh, v0, mu_per_tet=mu_t, hierarchy="extended", galerkin="structural",
+                max_coarse=64, coarse_solver=coarse_solver, relative_tol=True, tol_cg=1e-6)
+    o = _oracle(mesh, v0, precision="f32", mu_per_tet=mu_t, relative_tol=True, tol_cg=1e-6)
+    A = sp.csr_matrix((s._A.csr_data().astype(np.float64), o.indices, o.indptr),
+                      shape=(o.n_tets, o.n_tets))
+    Ps = [sp.csr_matrix((np.ones(len(a)), (np.arange(len(a)), a)), shape=(len(a), m))
+          for a, m in s._aggs]
+    o.mg = om.OracleMG(2, 2.0 / 3.0, 100, 1e-6, "f32", relative=True, structural=True,
+                       coarse_solver=coarse_solver)
+    o.mg.setup(A, Ps)
+    o.mg.update_fine_matrix_data(s._A.csr_data())
+    o.mg.update_all_coarse_matrices()
+    return mesh, s, o
+
+
+def test_extended_levels_and_cholesky_vcycle():
+    """Extended hierarchy: coarse values bit-exact on every level; V-cycle with the FP64
+    dense coarsest solve within 1e-5 (relative) of the oracle's (numpy Cholesky inverse)."""
+    mesh, s, o = _extended_pair()
+    mg = s._gpu_solver
+    assert mg.n_levels == len(o.mg.levels) >= 3
+    for l in range(1, mg.n_levels):
+        got = mg.levels[l].csr_data()
+        assert np.array_equal(got.view(np.uint32), o.mg.levels[l]["a"].view(np.uint32)), l
+    rng = np.random.default_rng(4)
+    r = rng.standard_normal(o.n_tets).astype(np.float32)
+    z_o = o.mg.vcycle(r)
+    rt = torch.from_numpy(r).cuda()
+    zt = torch.zeros_like(rt)
+    mg.apply(rt, zt)
+    z = zt.cpu().numpy()
+    assert np.linalg.norm(z - z_o) <= 1e-5 * np.linalg.norm(z_o)
+
+
+@pytest.mark.parametrize("coarse_solver", ["jacobi", "cholesky"])
+def test_graph_replay_equals_eager(coarse_solver):
+    """The PCG replayed as HIP graphs gives the same bits as eager launches, over several
+    solves with eager work in between (each solve replays the graph several times)."""
+    mesh, s, o = _extended_pair(coarse_solver=coarse_solver)
+    mg = s._gpu_solver
+    ctx = s.ctx
+    results = {}
+    for mode in (0, 1, 0, 1):
+        ctx("mgpbd_ctx_use_graphs", mode, 4)
+        s._compute_C_and_gradC()
+        s._fill_A_and_update_gpu()
+        it = mg.solve(s._b_gpu, s._dlam_gpu)
+        torch.cuda.synchronize()
+        x = s._dlam_gpu.cpu().numpy().copy()
+        if mode in results:
+            assert results[mode][0] == it and np.array_equal(results[mode][1], x)
+        results[mode] = (it, x)
+    assert results[0][0] == results[1][0] > 4
+    assert np.array_equal(results[0][1].view(np.uint32), results[1][1].view(np.uint32))
+    ctx("mgpbd_ctx_use_graphs", 1, 4)
