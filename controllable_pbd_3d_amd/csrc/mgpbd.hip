@@ -87,6 +87,19 @@ k_fin_sum(RedWS ws, int nparts, int first, int nslots, double* __restrict__ out)
 }
 
 // ------------------------------------------------------------------------------------------
+// SELL-64x4: entry k of row r at slice_ptr[r/64] + 256*(k/4) + 4*(r%64) + (k%4).
+// ------------------------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int sell_base(const int* __restrict__ slice_ptr, int r) {
+  return slice_ptr[r >> 6] + 4 * (r & 63);
+}
+__device__ __forceinline__ int sell_width(const int* __restrict__ slice_ptr, int r) {
+  return (slice_ptr[(r >> 6) + 1] - slice_ptr[r >> 6]) >> 6;
+}
+
+// ------------------------------------------------------------------------------------------
 // Per-tet kernels
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ void load3(const float* __restrict__ p, int v, float* o) {
@@ -261,8 +274,35 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
   store_partial(ws, 1, b1);
 }
 
-// A values, one thread per row (tet) i; the row's entries are column-adjacent in SELL-64 so
-// every store of a wavefront is one contiguous 256-byte segment.
+// One A entry (row i, column j): the diagonal sum_v w_v |g_iv|^2 + alpha_i, or the sum over
+// the vertices i and j share of w_v g_iv . g_jv in the slot order of _intersect_tets.
+__device__ __forceinline__ float a_entry(int i, int j, const int (&vi)[4], const float (&w)[4],
+                                         const float (&gi)[4][3], float alpha_i,
+                                         const int4* __restrict__ t_i,
+                                         const float* __restrict__ gradC) {
+  if (j == i) {
+    float v = w[0] * dot3(gi[0], gi[0]);
+    v = fmaf(w[1], dot3(gi[1], gi[1]), v);
+    v = fmaf(w[2], dot3(gi[2], gi[2]), v);
+    v = fmaf(w[3], dot3(gi[3], gi[3]), v);
+    return v + alpha_i;
+  }
+  const int4 tj = t_i[j];
+  const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
+  const float* gj = gradC + 12 * (size_t)j;
+  float v = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (vi[a] == vj[b]) {
+        float g[3] = {gj[3 * b], gj[3 * b + 1], gj[3 * b + 2]};
+        v = fmaf(w[a], dot3(gi[a], g), v);
+      }
+  return v;
+}
+
+// A values, one thread per row (tet) i, four entries per 16-byte store (padding stays 0).
 __global__ void __launch_bounds__(kBlock)
 k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
          const int* __restrict__ row_len, const int4* __restrict__ t_i,
@@ -270,7 +310,7 @@ k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
          const float* __restrict__ alpha, float* __restrict__ val, float* __restrict__ dinv) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const int base = slice_ptr[i >> 6] + (i & 63);
+  const int base = sell_base(slice_ptr, i);
   const int len = row_len[i];
   const int4 ti = t_i[i];
   const int vi[4] = {ti.x, ti.y, ti.z, ti.w};
@@ -281,33 +321,22 @@ k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
 #pragma unroll
     for (int r = 0; r < 3; ++r) gi[a][r] = gradC[12 * (size_t)i + 3 * a + r];
   }
+  const float al = alpha[i];
   float diag = 0.0f;
-  for (int k = 0; k < len; ++k) {
-    const int p = base + 64 * k;
-    const int j = idx[p];
-    float v;
-    if (j == i) {
-      v = w[0] * dot3(gi[0], gi[0]);
-      v = fmaf(w[1], dot3(gi[1], gi[1]), v);
-      v = fmaf(w[2], dot3(gi[2], gi[2]), v);
-      v = fmaf(w[3], dot3(gi[3], gi[3]), v);
-      v = v + alpha[i];
-      diag = v;
-    } else {
-      const int4 tj = t_i[j];
-      const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
-      const float* gj = gradC + 12 * (size_t)j;
-      v = 0.0f;
+  for (int k0 = 0; k0 < len; k0 += 4) {
+    const int p = base + 64 * k0;   // 256 * (k0 / 4)
+    const int4 c = *reinterpret_cast<const int4*>(idx + p);
+    const int cj[4] = {c.x, c.y, c.z, c.w};
+    float v[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (vi[a] == vj[b]) {
-            float g[3] = {gj[3 * b], gj[3 * b + 1], gj[3 * b + 2]};
-            v = fmaf(w[a], dot3(gi[a], g), v);
-          }
+    for (int q = 0; q < 4; ++q) {
+      v[q] = 0.0f;
+      if (k0 + q < len) {
+        v[q] = a_entry(i, cj[q], vi, w, gi, al, t_i, gradC);
+        if (cj[q] == i) diag = v[q];
+      }
     }
-    val[p] = v;
+    *reinterpret_cast<float4*>(val + p) = make_float4(v[0], v[1], v[2], v[3]);
   }
   dinv[i] = fabsf(diag) > 1e-30f ? 1.0f / diag : 0.0f;
 }
@@ -384,22 +413,28 @@ k_update_vel(int nv, const float* __restrict__ x, const float* __restrict__ x_ol
 }
 
 // ------------------------------------------------------------------------------------------
-// SELL-64 sparse kernels. One thread per row; the wavefront owns one slice, so every column
-// and value load is one 256-byte coalesced segment and rows are summed sequentially in CSR
+// SELL-64x4 sparse kernels. One thread per row; the wavefront owns one slice, every lane
+// loads four columns and four values (16 bytes each) at a time, so a wavefront's load is
+// one contiguous 1 KiB segment; the matrix streams are read with non-temporal loads (each
+// byte is used once per pass) to keep x resident in L2. Rows are summed sequentially in CSR
 // order (the order of the reference's row-per-thread kernels).
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ float sell_row(int i, const int* __restrict__ slice_ptr,
                                           const int* __restrict__ idx,
                                           const float* __restrict__ val,
                                           const float* __restrict__ x) {
-  const int s = i >> 6;
-  const int p0 = slice_ptr[s] + (i & 63);
-  const int w = (slice_ptr[s + 1] - slice_ptr[s]) >> 6;
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
   float acc = 0.0f;
-#pragma unroll 8
-  for (int k = 0; k < w; ++k) {
-    const int p = p0 + 64 * k;
-    acc = fmaf(val[p], x[idx[p]], acc);
+#pragma unroll 4
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const int p = p0 + 64 * k0;
+    const v4i c = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(idx + p));
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(val + p));
+    acc = fmaf(v.x, x[c.x], acc);
+    acc = fmaf(v.y, x[c.y], acc);
+    acc = fmaf(v.z, x[c.z], acc);
+    acc = fmaf(v.w, x[c.w], acc);
   }
   return acc;
 }
@@ -430,7 +465,7 @@ k_jacobi_diag(int n, const int* __restrict__ slice_ptr, const float* __restrict_
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float a = val[slice_ptr[i >> 6] + (i & 63)];
+  const float a = val[sell_base(slice_ptr, i)];
   const float wd = omega * dinv[i];
   const float bi = b[i];
   float xi = x[i];
@@ -460,12 +495,11 @@ k_jacobi_small(int n, const int* __restrict__ slice_ptr, const int* __restrict__
   int cur = 0;
   for (int sw = 0; sw < sweeps; ++sw) {
     for (int i = threadIdx.x; i < 64 * n_slices; i += kSmallBlock) {
-      const int sl = i >> 6;
-      const int p0 = slice_ptr[sl] + (i & 63);
-      const int w = (slice_ptr[sl + 1] - slice_ptr[sl]) >> 6;
+      const int p0 = sell_base(slice_ptr, i);
+      const int w = sell_width(slice_ptr, i);
       float acc = 0.0f;
       for (int k = 0; k < w; ++k) {
-        const int p = p0 + 64 * k;
+        const int p = p0 + 64 * (k & ~3) + (k & 3);
         acc = fmaf(val[p], xs[cur][idx[p]], acc);
       }
       if (i < n) xs[cur ^ 1][i] = fmaf(omega * dinv[i], b[i] - acc, xs[cur][i]);
@@ -486,11 +520,10 @@ k_dense_from_sell(int n, const int* __restrict__ slice_ptr, const int* __restric
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   for (int j = 0; j < n; ++j) D[(size_t)j * n + i] = 0.0;
-  const int sl = i >> 6;
-  const int p0 = slice_ptr[sl] + (i & 63);
-  const int w = (slice_ptr[sl + 1] - slice_ptr[sl]) >> 6;
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
   for (int k = 0; k < w; ++k) {
-    const int p = p0 + 64 * k;
+    const int p = p0 + 64 * (k & ~3) + (k & 3);
     const float v = val[p];
     if (v != 0.0f) D[(size_t)idx[p] * n + i] += (double)v;   // column idx, row i
   }
@@ -505,21 +538,20 @@ k_symmetrize_lower(int n, double* __restrict__ D) {
   if (i < j) D[(size_t)j * n + i] = D[(size_t)i * n + j];
 }
 
-// x = D b for a dense symmetric D (column-major): 64 rows per block, the block's 4 waves
-// split the columns, FP64 accumulation.
+// x = D b for a dense symmetric D (column-major): one wavefront per row i reads column i
+// (= row i) contiguously, FP64 accumulation, fixed shuffle-tree reduction.
 __global__ void __launch_bounds__(kBlock)
 k_dense_matvec(int n, const double* __restrict__ D, const float* __restrict__ b,
                float* __restrict__ x, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double part[4][64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + lane;
+  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* col = D + (size_t)i * n;
   double acc = 0.0;
-  if (i < n)
-    for (int j = w; j < n; j += 4) acc += D[(size_t)j * n + i] * (double)b[j];
-  part[w][lane] = acc;
-  __syncthreads();
-  if (w == 0 && i < n) x[i] = (float)((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+  for (int j = lane; j < n; j += 64) acc += col[j] * (double)b[j];
+  acc = wave_sum(acc);
+  if (lane == 0) x[i] = (float)acc;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -1150,7 +1182,7 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
                                                             cur, ctx->omega, count, done);
     return cur;
   }
-  if (L.n <= kSmallRows && count > 1) {
+  if (L.n <= kSmallRows && count >= 8) {   // many sweeps of a small level: one launch
     int pe = prof_begin(ctx, MGPBD_PROF_COARSE);
     k_jacobi_small<<<1, kSmallBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.idx, L.val, L.dinv,
                                                        b, cur, ctx->omega, count, done);
@@ -1214,7 +1246,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   const bool coarsest = (l == ctx->n_levels - 1);
   if (coarsest) {
     if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY) {
-      k_dense_matvec<<<(L.n + 63) / 64, kBlock, 0, ctx->stream>>>(L.n, ctx->dense, b, x_first, done);
+      k_dense_matvec<<<(L.n + 3) / 4, kBlock, 0, ctx->stream>>>(L.n, ctx->dense, b, x_first, done);
       return x_first;
     }
     int total = std::min(ctx->coarse_sweeps, L.n);

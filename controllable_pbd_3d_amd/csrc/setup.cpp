@@ -543,8 +543,16 @@ int mgpbd_vertex_incidence(int n_tets, int n_verts, const int32_t* t_i, int32_t*
 }
 
 // ---------------------------------------------------------------------------------------
-// SELL-64 layout: 64 rows per slice (one wavefront), entries k-major within the slice.
+// SELL-64x4 layout: rows binned 64 per slice (one wavefront); a slice's width is its longest
+// row rounded up to a multiple of 4; entry k of row 64s+l sits at
+//   slice_ptr[s] + 256*(k/4) + 4*l + (k%4)
+// so each lane reads its row four entries (16 bytes) at a time and a wavefront's load is one
+// contiguous 1 KiB segment.
 // ---------------------------------------------------------------------------------------
+static inline int64_t sell_pos_of(const int32_t* slice_ptr, int r, int k) {
+  return (int64_t)slice_ptr[r / 64] + 256LL * (k / 4) + 4 * (r % 64) + (k % 4);
+}
+
 int mgpbd_sell_layout(int n_rows, const int32_t* indptr, int32_t* slice_ptr) {
   if (n_rows < 0 || (n_rows > 0 && !indptr) || !slice_ptr)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_layout: invalid arguments");
@@ -554,6 +562,7 @@ int mgpbd_sell_layout(int n_rows, const int32_t* indptr, int32_t* slice_ptr) {
   for (int s = 0; s < n_slices; ++s) {
     int w = 0;
     for (int r = 64 * s; r < std::min(n_rows, 64 * s + 64); ++r) w = std::max(w, indptr[r + 1] - indptr[r]);
+    w = (w + 3) / 4 * 4;
     acc += 64LL * w;
     if (acc > INT32_MAX) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_layout: padded size exceeds int32");
     slice_ptr[s + 1] = (int32_t)acc;
@@ -564,15 +573,14 @@ int mgpbd_sell_layout(int n_rows, const int32_t* indptr, int32_t* slice_ptr) {
 int mgpbd_sell_map(int n_rows, const int32_t* indptr, const int32_t* slice_ptr, int32_t* sell_pos) {
   if (n_rows < 0 || (n_rows > 0 && (!indptr || !slice_ptr || !sell_pos)))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_map: invalid arguments");
-  for (int r = 0; r < n_rows; ++r) {
-    int32_t base = slice_ptr[r / 64] + (r % 64);
-    for (int32_t a = indptr[r]; a < indptr[r + 1]; ++a) sell_pos[a] = base + 64 * (a - indptr[r]);
-  }
+  for (int r = 0; r < n_rows; ++r)
+    for (int32_t a = indptr[r]; a < indptr[r + 1]; ++a)
+      sell_pos[a] = (int32_t)sell_pos_of(slice_ptr, r, a - indptr[r]);
   return 0;
 }
 
-// Column array of the SELL-64 layout: real entries at their mapped positions, padding
-// entries of a row point at the row itself, rows past n_rows at column 0.
+// Column array of the layout: real entries at their positions, padding entries of a row
+// point at the row itself, rows past n_rows at column 0.
 int mgpbd_sell_columns(int n_rows, const int32_t* indptr, const int32_t* indices,
                        const int32_t* slice_ptr, int32_t* sell_idx) {
   if (n_rows < 0 || (n_rows > 0 && (!indptr || !indices || !slice_ptr || !sell_idx)))
@@ -588,7 +596,7 @@ int mgpbd_sell_columns(int n_rows, const int32_t* indptr, const int32_t* indices
         if (r >= n_rows) c = 0;
         else if (k < indptr[r + 1] - indptr[r]) c = indices[indptr[r] + k];
         else c = r;
-        sell_idx[slice_ptr[s] + 64 * k + l] = c;
+        sell_idx[sell_pos_of(slice_ptr, 64 * s + l, k)] = c;
       }
     }
   }

@@ -14,9 +14,10 @@
  *    pointers are host pointers. The library never frees caller memory.
  *  - Device work is queued on the context's stream (mgpbd_ctx_create / _set_stream) and is
  *    asynchronous unless stated; functions that return host values synchronise that stream.
- *  - Matrices on the device use the sliced layout "SELL-64": rows are binned 64 per slice
+ *  - Matrices on the device use the sliced layout "SELL-64x4": rows are binned 64 per slice
  *    (one gfx950 wavefront); slice s holds width_s = (slice_ptr[s+1]-slice_ptr[s])/64
- *    entries per row, stored k-major (entry k of row 64s+l at slice_ptr[s] + 64k + l).
+ *    entries per row (a multiple of 4); entry k of row 64s+l is at
+ *    slice_ptr[s] + 256*(k/4) + 4*l + (k%4), so a lane loads 4 entries (16 bytes) at a time.
  *    Rows shorter than width_s are padded with value 0 and column = the row itself.
  *    Within a row, entries keep the CSR order of the reference (level 0: ascending
  *    neighbours then the diagonal; coarse levels: ascending columns).
@@ -93,7 +94,7 @@ void mgpbd_galerkin_free(mgpbd_galerkin* g);
 int mgpbd_vertex_incidence(int n_tets, int n_verts, const int32_t* t_i, int32_t* vptr,
                            int32_t* vinc);
 
-/* SELL-64 layout of a CSR matrix: slice_ptr (n_slices+1) and, per CSR position, its
+/* SELL-64x4 layout of a CSR matrix: slice_ptr (n_slices+1) and, per CSR position, its
  * position in the sliced arrays (sell_pos, nnz). No reference counterpart (layout only). */
 int mgpbd_sell_layout(int n_rows, const int32_t* indptr, int32_t* slice_ptr);
 int mgpbd_sell_map(int n_rows, const int32_t* indptr, const int32_t* slice_ptr,

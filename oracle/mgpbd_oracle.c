@@ -331,11 +331,11 @@ static void vc(const OrcMG* mg, int l) {
   const OrcLevel* A = &mg->L[l];
   int n = A->n;
   if (l == mg->nl - 1 && A->dense) {
-    /* x = A^-1 b with FP64 accumulation; columns split in four interleaved partial sums */
+    /* x = A^-1 b with FP64 accumulation */
     for (int i = 0; i < n; ++i) {
-      double p[4] = {0, 0, 0, 0};
-      for (int j = 0; j < n; ++j) p[j & 3] += A->dense[(int64_t)j * n + i] * (double)A->b[j];
-      A->x[i] = (real)((p[0] + p[1]) + (p[2] + p[3]));
+      double acc = 0;
+      for (int j = 0; j < n; ++j) acc += A->dense[(int64_t)j * n + i] * (double)A->b[j];
+      A->x[i] = (real)acc;
     }
     return;
   }

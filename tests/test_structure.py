@@ -196,6 +196,8 @@ def test_extended_hierarchy_invariants():
 
 
 def test_sell_layout_roundtrip():
+    """SELL-64x4: width a multiple of 4 covering the longest row of the slice; entry k of row
+    r at slice_ptr[r/64] + 256 (k/4) + 4 (r%64) + k%4; padding points at the row itself."""
     from controllable_pbd_3d_amd.sparse_gpu import sell_layout
     m = box_mesh(4, 0.05, seed=0)
     indptr, indices = product_pattern(m)
@@ -203,18 +205,21 @@ def test_sell_layout_roundtrip():
     slice_ptr, sell_idx, pos = sell_layout(indptr, indices, n)
     assert len(np.unique(pos)) == len(pos)
     assert np.array_equal(sell_idx[pos], indices)
+    lens = np.diff(indptr)
     for s in range((n + 63) // 64):
         w = (slice_ptr[s + 1] - slice_ptr[s]) // 64
-        lens = np.diff(indptr)[64 * s:64 * s + 64]
-        assert w == lens.max()
+        assert w % 4 == 0 and w - 4 < lens[64 * s:64 * s + 64].max() <= w
         for l in range(64):
             r = 64 * s + l
             for k in range(w):
-                c = sell_idx[slice_ptr[s] + 64 * k + l]
+                p = slice_ptr[s] + 256 * (k // 4) + 4 * l + k % 4
+                c = sell_idx[p]
                 if r >= n:
                     assert c == 0
-                elif k >= lens[l]:
+                elif k >= lens[r]:
                     assert c == r
+                else:
+                    assert p == pos[indptr[r] + k] and c == indices[indptr[r] + k]
 
 
 def test_vertex_incidence():
