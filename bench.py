@@ -53,14 +53,15 @@ def solver_args(mesh, mu_t):
                 n_smooth=2, omega_jacobi=2.0 / 3.0, use_line_search=True, mu_per_tet=mu_t)
 
 
-def kernel_bytes(n_rows, nnz, kind):
-    """Algorithmic HBM bytes per launch (FP32 values, int32 columns; each array once)."""
+def kernel_bytes(n_rows, nnz, kind, per_nnz=8):
+    """Algorithmic HBM bytes per launch; per_nnz = 8 for FP32 values + int32 columns, 4 for
+    FP16/BF16 values + 16-bit column deltas (each array counted once)."""
     if kind == "spmv":      # values + columns + row pointers + x + y
-        return 8 * nnz + 4 * (n_rows + 1) + 4 * n_rows + 4 * n_rows
-    if kind == "jacobi":    # + b, D^-1, x (own) -> x_new
-        return 8 * nnz + 4 * (n_rows + 1) + 16 * n_rows
+        return per_nnz * nnz + 4 * (n_rows + 1) + 4 * n_rows + 4 * n_rows
+    if kind == "jacobi":    # + b, D^-1 (FP32 only: the scaled levels have unit diagonal), x -> x_new
+        return per_nnz * nnz + 4 * (n_rows + 1) + (16 if per_nnz == 8 else 12) * n_rows
     if kind == "resid":     # values + columns + ptr + x + b -> r
-        return 8 * nnz + 4 * (n_rows + 1) + 12 * n_rows
+        return per_nnz * nnz + 4 * (n_rows + 1) + 12 * n_rows
     raise ValueError(kind)
 
 
@@ -114,12 +115,75 @@ def cpu_baseline(mesh, mu_t, solver, newton_per_substep, pcg_per_substep, pcg_sa
                        f"{setup_s:.1f} s excluded; host has {os.cpu_count()} cores")}
 
 
+def kernel_stats(ctx, n, nnz, precision):
+    from controllable_pbd_3d_amd import _lib
+    ms = np.zeros(8, np.float64)
+    cnt = np.zeros(8, np.int64)
+    ctx("mgpbd_ctx_profile_read", ms.ctypes.data_as(_lib.F64P),
+        cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    kern = {}
+    # the outer PCG SpMV is FP32; the V-cycle passes use the precision under test
+    vb = 8 if precision == "fp32" else 4
+    for kind, slot, per in (("spmv", 0, 8), ("jacobi", 1, vb), ("resid", 2, vb)):
+        if cnt[slot] > 0:
+            avg = ms[slot] / cnt[slot] * 1e-3
+            by = kernel_bytes(n, nnz, kind, per)
+            kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
+                          "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
+    for kind, slot in (("fill_A", 3), ("constraints", 4), ("refill", 5), ("coarse", 6)):
+        if cnt[slot] > 0:
+            kern[kind] = {"avg_us": ms[slot] / cnt[slot] * 1e3, "launches": int(cnt[slot]),
+                          "total_ms": float(ms[slot])}
+    return kern
+
+
+def run_config(mesh, mu_t, cfg, precision, steps, warmup, profile_steps):
+    import torch
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    from controllable_pbd_3d_amd.runtime import get_context
+    ctx = get_context(0)
+    t0 = time.perf_counter()
+    s = MGPBDSolverGPU(**solver_args(mesh, mu_t), hold_pinned=True, relative_tol=True,
+                       tol_cg=1e-6, hierarchy="extended", galerkin="structural",
+                       max_coarse=64, max_levels=cfg["max_levels"], coarse_solver="cholesky",
+                       precision=precision, verbose=False)
+    s.init()
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f"[bench] {precision}: setup {setup_s:.1f} s, levels {s._gpu_solver._level_sizes}",
+          file=sys.stderr, flush=True)
+    for k in range(warmup):
+        s.step(0)
+    torch.cuda.synchronize()
+    iters, newton = [], []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        s.step(0)
+        iters.append(sum(c for _, _, c, _ in s.last_stats))
+        newton.append(len(s.last_stats))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    print(f"[bench] {precision}: {steps / elapsed:.3f} substeps/s", file=sys.stderr, flush=True)
+    # Kernel durations: HIP events around each launch of the main kernels, on the stream they
+    # run on, over `profile_steps` further substeps (the event-bracketed PCG launches eagerly
+    # instead of replaying its graph; kernel durations are unchanged, only gaps differ).
+    ctx("mgpbd_ctx_profile", 1)
+    for k in range(profile_steps):
+        s.step(0)
+    kern = kernel_stats(ctx, s.n_tets, s._nnz, precision)
+    ctx("mgpbd_ctx_profile", 0)
+    return s, {"value": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+               "setup_s": setup_s, "pcg_iters_per_substep": float(np.mean(iters)),
+               "newton_iters_per_substep": float(np.mean(newton)), "kernels": kern}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="all", choices=["all", "fp32", "fp16", "bf16"])
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
@@ -130,87 +194,60 @@ def main():
         sys.exit(2)
 
     import torch
-    from controllable_pbd_3d_amd import MGPBDSolverGPU
-    from controllable_pbd_3d_amd.runtime import get_context
-    from controllable_pbd_3d_amd import _lib
-
     cfg = CONFIGS[args.config]
     print(f"[bench] building {cfg['label']}", file=sys.stderr, flush=True)
     mesh, mu_t = build_workload(cfg)
     torch.cuda.set_device(0)
-    ctx = get_context(0)
-    t0 = time.perf_counter()
-    s = MGPBDSolverGPU(**solver_args(mesh, mu_t), hold_pinned=True, relative_tol=True,
-                       tol_cg=1e-6, hierarchy="extended", galerkin="structural",
-                       max_coarse=64, max_levels=cfg["max_levels"], coarse_solver="cholesky",
-                       verbose=False)
-    s.init()
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
-    levels = s._gpu_solver._level_sizes
-    print(f"[bench] setup {setup_s:.1f} s, levels {levels}, nnz {s._nnz}", file=sys.stderr,
-          flush=True)
-    for k in range(args.warmup):
-        s.step(0)
-    torch.cuda.synchronize()
-    iters = []
-    newton = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        s.step(0)
-        iters.append(sum(c for _, _, c, _ in s.last_stats))
-        newton.append(len(s.last_stats))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    # Kernel durations: HIP events around each launch of the main kernels, on the stream they
-    # run on, over `--profile-steps` further substeps (the event-bracketed PCG launches eagerly
-    # instead of replaying its graph; kernel durations are unchanged, only gaps differ).
-    ctx("mgpbd_ctx_profile", 1)
-    for k in range(args.profile_steps):
-        s.step(0)
-    ms = np.zeros(8, np.float64)
-    cnt = np.zeros(8, np.int64)
-    ctx("mgpbd_ctx_profile_read", ms.ctypes.data_as(_lib.F64P),
-        cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
-    ctx("mgpbd_ctx_profile", 0)
-    n, nnz = s.n_tets, s._nnz
-    kern = {}
-    for kind, slot in (("spmv", 0), ("jacobi", 1), ("resid", 2)):
-        if cnt[slot] > 0:
-            avg = ms[slot] / cnt[slot] * 1e-3
-            by = kernel_bytes(n, nnz, kind)
-            kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
-                          "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
-    for kind, slot in (("fill_A", 3), ("constraints", 4), ("refill", 5), ("coarse", 6)):
-        if cnt[slot] > 0:
-            kern[kind] = {"avg_us": ms[slot] / cnt[slot] * 1e3, "launches": int(cnt[slot]),
-                          "total_ms": float(ms[slot])}
+    precisions = ["fp32", "fp16", "bf16"] if args.precision == "all" else [args.precision]
+    runs = {}
+    solver32 = None
+    for prec in precisions:
+        s, res = run_config(mesh, mu_t, cfg, prec, args.steps, args.warmup, args.profile_steps)
+        runs[prec] = res
+        if prec == "fp32":
+            solver32 = s
+        else:
+            del s
+        torch.cuda.empty_cache()
+    head = "fp16" if "fp16" in runs else precisions[0]
+    best = runs[head]
+    kern = best["kernels"]
     mat = {k: v for k, v in kern.items() if "bytes" in v}
     dom = max(mat, key=lambda k: mat[k]["total_ms"]) if mat else None
-    value = args.steps / elapsed
     out = {
-        "metric": "MGPBD substeps/s at 1M tets to 1e-6 residual (plain FP32 HIP MGPCG)",
-        "value": value, "unit": "substeps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "metric": "MGPBD substeps/s at 1M tets to 1e-6 residual: mixed-precision V-cycle vs "
+                  "plain FP32 GPU",
+        "value": best["value"], "unit": "substeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head == "fp32" else f"f32 outer PCG, {head} V-cycle",
         "data": "synthetic",
-        "config": {"workload": cfg["label"], "n_tets": n, "n_verts": s.n_verts, "nnz_A0": nnz,
-                   "levels": levels, "precision": "fp32", "pcg_rel_tol": 1e-6,
-                   "E_range": cfg["E"], "setup_s": setup_s,
-                   "pcg_iters_per_substep": float(np.mean(iters)),
-                   "newton_iters_per_substep": float(np.mean(newton)),
-                   "kernels": kern},
+        "config": {"workload": cfg["label"], "n_tets": mesh.n_tets, "n_verts": mesh.n_verts,
+                   "nnz_A0": int(solver32._nnz) if solver32 else None,
+                   "levels": solver32._gpu_solver._level_sizes if solver32 else None,
+                   "pcg_rel_tol": 1e-6, "E_range": cfg["E"], "reported": head,
+                   "runs": runs,
+                   "speedup_vs_fp32": (best["value"] / runs["fp32"]["value"]
+                                       if "fp32" in runs else None)},
     }
     if dom:
-        k = kern[dom]
-        out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": k["GBs"],
+        k = mat[dom]
+        out["roofline"] = {"bound": "hbm", "kernel": f"{dom} ({head})", "achieved": k["GBs"],
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": None}
     if args.cpu_baseline:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(mesh, mu_t, s, float(np.mean(newton)),
-                                           float(np.mean(iters)))
+        ref = runs.get("fp32", best)
+        s_for = solver32
+        if s_for is None:
+            from controllable_pbd_3d_amd import tet_aggregation as ta
+
+            class _Aggs:
+                _aggs = ta.build_extended_hierarchy(mesh.tets, mesh.n_tets, max_coarse=64,
+                                                    max_levels=cfg["max_levels"] - 1)
+            s_for = _Aggs
+        out["cpu_baseline"] = cpu_baseline(mesh, mu_t, s_for, ref["newton_iters_per_substep"],
+                                           ref["pcg_iters_per_substep"])
     print(json.dumps(out), flush=True)
 
 

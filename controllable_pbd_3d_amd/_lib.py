@@ -80,6 +80,9 @@ SIGNATURES = {
     "mgpbd_mg_configure": (c_int, [_V, c_int, c_int, c_float, c_int, c_int]),
     "mgpbd_mg_set_level": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, _V, _V, _V, c_int]),
     "mgpbd_mg_factor_coarsest": (c_int, [_V]),
+    "mgpbd_mg_set_precision": (c_int, [_V, c_int, c_int]),
+    "mgpbd_mg_set_level_half": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
+    "mgpbd_mg_convert_half": (c_int, [_V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_pcg_solve": (c_int, [_V, _V, _V, _V, c_int, c_double, c_int,
                                 ctypes.POINTER(c_int), F64P]),

@@ -759,6 +759,218 @@ k_pcg_p(int n, const float* __restrict__ z, float* __restrict__ p, const PcgStat
   else p[i] = fmaf((float)st->beta, p[i], z[i]);
 }
 
+// ------------------------------------------------------------------------------------------
+// Reduced-precision V-cycle (extension). Each level is used in its symmetrically scaled form
+// A^ = D^-1/2 A D^-1/2 (unit diagonal, |a^_ij| <= 1 for SPD A, so FP16's range suffices),
+// values stored as FP16 or BF16 in the level's SELL positions, products accumulated in FP32.
+// With x = D^-1/2 x^ and b = D^1/2 b^ the weighted-Jacobi V-cycle on A^ is the V-cycle on A:
+//   sweep     x^' = x^ + omega (b^ - A^ x^)                 (diag(A^) = 1)
+//   restrict  b^_c[I] = isd_c[I] sum_{i in I} sd_i r^_i
+//   prolong   x^_i += sd_i isd_c[agg_i] x^_c[agg_i]
+// ------------------------------------------------------------------------------------------
+typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
+typedef short v4s16 __attribute__((ext_vector_type(4)));
+
+template <int VT>   // 1: FP16, 2: BF16
+__device__ __forceinline__ float h2f(unsigned short u) {
+  if (VT == 1) return (float)__builtin_bit_cast(_Float16, u);
+  return __uint_as_float((unsigned int)u << 16);
+}
+template <int VT>
+__device__ __forceinline__ unsigned short f2h(float f) {
+  if (VT == 1) return __builtin_bit_cast(unsigned short, (_Float16)f);
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+  return (unsigned short)(u >> 16);
+}
+
+template <int VT, bool DELTA>
+__device__ __forceinline__ float sellh_row(int i, const int* __restrict__ slice_ptr,
+                                           const int* __restrict__ idx,
+                                           const int16_t* __restrict__ col16,
+                                           const uint16_t* __restrict__ hval,
+                                           const float* __restrict__ x) {
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  float acc = 0.0f;
+#pragma unroll 4
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const int p = p0 + 64 * k0;
+    const v4u16 hv = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(hval + p));
+    int c0, c1, c2, c3;
+    if (DELTA) {
+      const v4s16 d = __builtin_nontemporal_load(reinterpret_cast<const v4s16*>(col16 + p));
+      c0 = i + d.x; c1 = i + d.y; c2 = i + d.z; c3 = i + d.w;
+    } else {
+      const v4i c = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(idx + p));
+      c0 = c.x; c1 = c.y; c2 = c.z; c3 = c.w;
+    }
+    acc = fmaf(h2f<VT>(hv.x), x[c0], acc);
+    acc = fmaf(h2f<VT>(hv.y), x[c1], acc);
+    acc = fmaf(h2f<VT>(hv.z), x[c2], acc);
+    acc = fmaf(h2f<VT>(hv.w), x[c3], acc);
+  }
+  return acc;
+}
+
+// MODE 1: y = x + omega (b - A^x);  MODE 2: y = b - A^x;  MODE 3: y = isd (x + omega (b - A^x))
+template <int VT, bool DELTA, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+        const int16_t* __restrict__ col16, const uint16_t* __restrict__ hval,
+        const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ b,
+        float omega, const float* __restrict__ isd, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const float ax = sellh_row<VT, DELTA>(i, slice_ptr, idx, col16, hval, x);
+  if (i >= n) return;
+  if (MODE == 2) {
+    y[i] = b[i] - ax;
+  } else {
+    const float xn = fmaf(omega, b[i] - ax, x[i]);
+    y[i] = MODE == 3 ? isd[i] * xn : xn;
+  }
+}
+
+// sd = sqrt(A_ii), isd = 1 / sd (1 where A_ii <= 0)
+__global__ void __launch_bounds__(kBlock)
+k_half_scales(int n, const int* __restrict__ diag_pos, const float* __restrict__ val,
+              float* __restrict__ sd, float* __restrict__ isd) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int p = diag_pos[i];
+  const float d = p >= 0 ? val[p] : 0.0f;
+  const float r = d > 0.0f ? sqrtf(d) : 1.0f;
+  sd[i] = r;
+  isd[i] = 1.0f / r;
+}
+
+// hval = half(val_ij isd_i isd_j) over the whole padded row; col16 = j - i (padding: 0)
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
+              const int* __restrict__ idx, const float* __restrict__ val,
+              const float* __restrict__ isd, uint16_t* __restrict__ hval,
+              int16_t* __restrict__ col16) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  const float si = i < n ? isd[i] : 0.0f;
+  for (int k = 0; k < w; ++k) {
+    const int p = p0 + 64 * (k & ~3) + (k & 3);
+    const int j = idx[p];
+    const float v = i < n ? val[p] * si * isd[j] : 0.0f;
+    hval[p] = f2h<VT>(v);
+    if (col16) col16[p] = (int16_t)(i < n ? j - i : 0);
+  }
+}
+
+// x^0 = omega isd r (first sweep on the scaled level 0) and b^0 = isd r
+__global__ void __launch_bounds__(kBlock)
+k_half_first(int n, const float* __restrict__ isd, const float* __restrict__ r,
+             float* __restrict__ bh, float* __restrict__ xh, float omega,
+             const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float v = isd[i] * r[i];
+  bh[i] = v;
+  xh[i] = omega * v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_restrict_h(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
+             const float* __restrict__ sd_f, const float* __restrict__ r,
+             const float* __restrict__ isd_c, float* __restrict__ bc, float* __restrict__ xc,
+             float omega, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int I = blockIdx.x * kBlock + threadIdx.x;
+  if (I >= nc) return;
+  float s = 0.0f;
+  for (int m = mptr[I]; m < mptr[I + 1]; ++m) {
+    const int i = mem[m];
+    s = fmaf(sd_f[i], r[i], s);
+  }
+  const float v = isd_c[I] * s;
+  bc[I] = v;
+  if (xc) xc[I] = omega * v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_prolong_h(int n, const int* __restrict__ agg, const float* __restrict__ sd_f,
+            const float* __restrict__ isd_c, const float* __restrict__ xc,
+            float* __restrict__ x, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int a = agg[i];
+  x[i] = fmaf(sd_f[i] * isd_c[a], xc[a], x[i]);
+}
+
+// x^ = sd (D (sd b^)) for the dense coarsest inverse D of the unscaled level
+__global__ void __launch_bounds__(kBlock)
+k_dense_matvec_scaled(int n, const double* __restrict__ D, const float* __restrict__ sd,
+                      const float* __restrict__ b, float* __restrict__ x,
+                      const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  const double* col = D + (size_t)i * n;
+  double acc = 0.0;
+  for (int j = lane; j < n; j += 64) acc += col[j] * ((double)sd[j] * (double)b[j]);
+  acc = wave_sum(acc);
+  if (lane == 0) x[i] = (float)(acc * (double)sd[i]);
+}
+
+// Flexible PCG (Polak-Ribiere): partial r.z and r.z_old
+__global__ void __launch_bounds__(kBlock)
+k_pcg_rz2(int n, const float* __restrict__ r, const float* __restrict__ z,
+          const float* __restrict__ zo, RedWS ws, const PcgState* st) {
+  if (st->done) return;
+  double a = 0.0, c = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double ri = r[i];
+    a += ri * (double)z[i];
+    c += ri * (double)zo[i];
+  }
+  const double ba = block_sum(a);
+  const double bc = block_sum(c);
+  store_partial(ws, 0, ba);
+  store_partial(ws, 1, bc);
+}
+
+// beta = (r.z - r.z_old) / rz_prev; breakdown test on rz_prev as in the reference
+__global__ void __launch_bounds__(kBlock)
+k_pcg_fin_flex(RedWS ws, int nparts, PcgState* st) {
+  if (st->done) return;
+  const double rz = sum_partials(ws, 0, nparts);
+  const double rzo = sum_partials(ws, 1, nparts);
+  if (threadIdx.x != 0) return;
+  st->iters = st->k + 1;
+  if (fabs(st->rz) < 1e-30) {
+    st->done = 3;
+  } else {
+    st->beta = (rz - rzo) / st->rz;
+    st->rz = rz;
+    st->k = st->k + 1;
+  }
+}
+
+// p = z + beta p and z_old = z
+__global__ void __launch_bounds__(kBlock)
+k_pcg_p_flex(int n, const float* __restrict__ z, float* __restrict__ p, float* __restrict__ zo,
+             const PcgState* st, int first) {
+  if (st->done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float zi = z[i];
+  p[i] = first ? zi : fmaf((float)st->beta, p[i], zi);
+  zo[i] = zi;
+}
+
 }  // namespace mgpbd
 
 using namespace mgpbd;
@@ -775,8 +987,16 @@ struct MgLevel {
   const int* agg = nullptr;
   const int* mptr = nullptr;
   const int* mem = nullptr;
-  float* work = nullptr;   // b, x, x', r
+  float* work = nullptr;   // b, x, x', r, each 64 * n_slices long (padded rows addressable)
   int diagonal = 0;
+  // reduced-precision copy for the V-cycle (MGPBD_PREC_FP16 / _BF16): values of the scaled
+  // level D^-1/2 A D^-1/2 in the SELL positions, optional 16-bit column deltas
+  uint16_t* hval = nullptr;
+  int16_t* col16 = nullptr;
+  float* sd = nullptr;      // sqrt(A_ii)
+  float* isd = nullptr;     // 1 / sqrt(A_ii)
+  const int* diag_pos = nullptr;
+  float* wb(int k) const { return work + (size_t)k * 64 * n_slices; }
 };
 
 struct mgpbd_ctx {
@@ -794,6 +1014,8 @@ struct mgpbd_ctx {
   bool dense_ready = false;
   float omega = 2.0f / 3.0f;
   MgLevel L[MGPBD_MAX_LEVELS];
+  int precision = MGPBD_PREC_FP32;   // V-cycle storage precision
+  int flexible = 0;                   // Polak-Ribiere beta
   int poll = 4;
   // HIP graph of `poll` PCG iterations (+ the state copy), replayed on `work` stream
   bool use_graph = true;
@@ -1167,6 +1389,10 @@ static int check_ready(mgpbd_ctx* ctx) {
                                     "(call mgpbd_mg_factor_coarsest after its values change)");
   if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels < 2)
     return mgpbd_fail(MGPBD_EINVAL, "multigrid: the dense coarsest solve needs two levels or more");
+  if (ctx->precision != MGPBD_PREC_FP32)
+    for (int l = 0; l < ctx->n_levels; ++l)
+      if (!ctx->L[l].hval)
+        return mgpbd_fail(MGPBD_EINVAL, "multigrid: reduced precision without half buffers on level " + std::to_string(l));
   return 0;
 }
 
@@ -1174,8 +1400,8 @@ static int check_ready(mgpbd_ctx* ctx) {
 // holding the result (cur or the level's other x buffer).
 static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count, const int* done) {
   MgLevel& L = ctx->L[l];
-  float* xa = L.work + (size_t)L.n;
-  float* xb = L.work + 2 * (size_t)L.n;
+  float* xa = L.wb(1);
+  float* xb = L.wb(2);
   if (count <= 0) return cur;
   if (L.diagonal) {
     k_jacobi_diag<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.val, L.dinv, b,
@@ -1253,7 +1479,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     return sweeps(ctx, l, b, x_first, total - 1, done);
   }
   float* x = sweeps(ctx, l, b, x_first, ctx->n_smooth - 1, done);
-  float* r = L.work + 3 * (size_t)L.n;
+  float* r = L.wb(3);
   MgLevel& C = ctx->L[l + 1];
   // residual r = b - A x (x == 0 when n_smooth == 0: then r = b)
   if (ctx->n_smooth > 0) {
@@ -1265,8 +1491,8 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, b, r);
   }
   // restrict + the coarse level's first sweep from zero
-  float* bc = C.work;
-  float* xc = C.work + (size_t)C.n;
+  float* bc = C.wb(0);
+  float* xc = C.wb(1);
   const bool c_coarsest = (l + 1 == ctx->n_levels - 1);
   const bool c_dense = c_coarsest && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY;
   const bool c_first = !c_dense && (c_coarsest || ctx->n_smooth > 0);
@@ -1279,11 +1505,134 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   return sweeps(ctx, l, b, x, ctx->n_smooth, done);
 }
 
+// ---------------- reduced-precision (scaled) V-cycle ----------------
+extern "C++" {
+template <int VT>
+static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
+                         const float* b, const int* done) {
+  const int g = nblocks(64LL * L.n_slices);
+  hipStream_t s = ctx->stream;
+  const float w = ctx->omega;
+#define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
+                                                         L.col16, L.hval, x, y, b, w, L.isd, done)
+  if (L.col16) {
+    if (mode == 1) SELLH(true, 1); else if (mode == 2) SELLH(true, 2); else SELLH(true, 3);
+  } else {
+    if (mode == 1) SELLH(false, 1); else if (mode == 2) SELLH(false, 2); else SELLH(false, 3);
+  }
+#undef SELLH
+}
+}  // extern "C++"
+
+static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
+                  const float* b, const int* done) {
+  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done);
+  else launch_sellh<2>(ctx, L, mode, x, y, b, done);
+}
+
+// `count` scaled sweeps from cur; with z_out the last one writes isd * x into z_out instead.
+static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count,
+                       const int* done, float* z_out) {
+  MgLevel& L = ctx->L[l];
+  float* xa = L.wb(1);
+  float* xb = L.wb(2);
+  for (int sw = 0; sw < count; ++sw) {
+    const bool last_out = z_out && sw == count - 1;
+    float* nxt = last_out ? z_out : ((cur == xa) ? xb : xa);
+    int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
+    sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done);
+    prof_end(ctx, pe);
+    cur = nxt;
+  }
+  return cur;
+}
+
+// Scaled V-cycle on level l for rhs b^ with the first sweep (x^ = omega b^) in x_first.
+// On level 0 the result is returned unscaled in z_out.
+static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, const int* done,
+                       float* z_out) {
+  MgLevel& L = ctx->L[l];
+  hipStream_t s = ctx->stream;
+  const bool coarsest = (l == ctx->n_levels - 1);
+  if (coarsest) {
+    if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY) {
+      k_dense_matvec_scaled<<<(L.n + 3) / 4, kBlock, 0, s>>>(L.n, ctx->dense, L.sd, b, x_first, done);
+      return x_first;
+    }
+    const int total = std::min(ctx->coarse_sweeps, L.n);
+    return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr);
+  }
+  float* x = sweeps_h(ctx, l, b, x_first, ctx->n_smooth - 1, done, nullptr);
+  float* r = L.wb(3);
+  int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
+  sellh(ctx, L, 2, x, r, b, done);
+  prof_end(ctx, pe);
+  MgLevel& C = ctx->L[l + 1];
+  float* bc = C.wb(0);
+  float* xc = C.wb(1);
+  const bool c_dense = (l + 1 == ctx->n_levels - 1) && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY;
+  k_restrict_h<<<nblocks(C.n), kBlock, 0, s>>>(C.n, L.mptr, L.mem, L.sd, r, C.isd, bc,
+                                               c_dense ? nullptr : xc, ctx->omega, done);
+  float* xcr = vcycle_h(ctx, l + 1, bc, xc, done, nullptr);
+  k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x, done);
+  return sweeps_h(ctx, l, b, x, ctx->n_smooth, done, z_out);
+}
+
+int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible) {
+  CTX_CHECK("mgpbd_mg_set_precision");
+  if (precision != MGPBD_PREC_FP32 && precision != MGPBD_PREC_FP16 && precision != MGPBD_PREC_BF16)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_precision: unknown precision");
+  if (precision != MGPBD_PREC_FP32 && ctx->n_smooth < 1)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_precision: the reduced-precision V-cycle needs n_smooth >= 1");
+  ctx->precision = precision;
+  ctx->flexible = flexible != 0;
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  return 0;
+}
+
+int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, int16_t* col16,
+                            float* sd, float* isd, const int32_t* diag_pos) {
+  CTX_CHECK("mgpbd_mg_set_level_half");
+  if (level < 0 || level >= ctx->n_levels || !hval || !sd || !isd || !diag_pos)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_half: invalid arguments");
+  MgLevel& L = ctx->L[level];
+  L.hval = hval; L.col16 = col16; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  return 0;
+}
+
+int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
+  CTX_CHECK("mgpbd_mg_convert_half");
+  if (int e = check_levels(ctx)) return e;
+  if (ctx->precision == MGPBD_PREC_FP32) return 0;
+  for (int l = 0; l < ctx->n_levels; ++l) {
+    MgLevel& L = ctx->L[l];
+    if (!L.hval) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
+    k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
+    const int g = nblocks(64LL * L.n_slices);
+    if (ctx->precision == MGPBD_PREC_FP16)
+      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.col16);
+    else
+      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.col16);
+  }
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_convert_half");
+  return 0;
+}
+
+static bool half_mode(const mgpbd_ctx* ctx) { return ctx->precision != MGPBD_PREC_FP32; }
+
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   CTX_CHECK("mgpbd_mg_vcycle");
   if (int e = check_ready(ctx)) return e;
   MgLevel& L = ctx->L[0];
-  float* x1 = L.work + (size_t)L.n;
+  if (half_mode(ctx)) {
+    k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, b, L.wb(0), L.wb(1), ctx->omega, nullptr);
+    float* z = vcycle_h(ctx, 0, L.wb(0), L.wb(1), nullptr, L.wb(3));
+    k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, z, x_out);
+    MGPBD_LAUNCH_CHECK("mgpbd_mg_vcycle");
+    return 0;
+  }
+  float* x1 = L.wb(1);
   const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
   if (first) k_first_sweep<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.dinv, b, x1, ctx->omega, nullptr);
   else k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(3, L.n, 0.f, nullptr, x1);
@@ -1294,25 +1643,44 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
 }
 
 // One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
+// z = M^-1 r for the PCG (first sweep included), FP32 or scaled reduced-precision V-cycle.
+static float* pcg_precondition(mgpbd_ctx* ctx, float* r, float* xv, bool first) {
+  MgLevel& L = ctx->L[0];
+  const int* done = &ctx->st->done;
+  if (half_mode(ctx)) {
+    k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, r, L.wb(0), L.wb(1),
+                                                           ctx->omega, done);
+    return vcycle_h(ctx, 0, L.wb(0), L.wb(1), done, L.wb(3));
+  }
+  if (!first) k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(3, L.n, 0.f, nullptr, xv);
+  return vcycle_from(ctx, 0, r, xv, done);
+}
+
+// One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
 static void pcg_iteration(mgpbd_ctx* ctx, float* x, float* r, float* p, float* Ap, float* xv,
-                          bool first, int gn, int gs, bool prof) {
+                          float* zold, bool first, int gn, int gs, bool prof) {
   MgLevel& L = ctx->L[0];
   const int n = L.n;
-  const int* done = &ctx->st->done;
   hipStream_t s = ctx->stream;
   int pe = prof ? prof_begin(ctx, MGPBD_PROF_SPMV_L0) : -1;
   k_pcg_spmv_dot<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
                                        ctx->ws, ctx->st);
   prof_end(ctx, pe);
   k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
-  k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, first ? xv : nullptr, ctx->omega,
-                                     ctx->ws, ctx->st);
+  const bool fuse_first = first && !half_mode(ctx);
+  k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, fuse_first ? xv : nullptr,
+                                     ctx->omega, ctx->ws, ctx->st);
   k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
-  float* zz = vcycle_from(ctx, 0, r, xv, done);
-  k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
-  k_pcg_fin<FIN_BETA><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  k_pcg_p<1><<<nblocks(n), kBlock, 0, s>>>(n, zz, p, ctx->st);
+  float* zz = pcg_precondition(ctx, r, xv, first);
+  if (ctx->flexible) {
+    k_pcg_rz2<<<gn, kBlock, 0, s>>>(n, r, zz, zold, ctx->ws, ctx->st);
+    k_pcg_fin_flex<<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+    k_pcg_p_flex<<<nblocks(n), kBlock, 0, s>>>(n, zz, p, zold, ctx->st, 0);
+  } else {
+    k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zz, ctx->ws, ctx->st);
+    k_pcg_fin<FIN_BETA><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+    k_pcg_p<1><<<nblocks(n), kBlock, 0, s>>>(n, zz, p, ctx->st);
+  }
 }
 
 int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll) {
@@ -1337,10 +1705,11 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
   const int n = L.n;
   if (nblocks(64LL * L.n_slices) > kPartialStride)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_solve: level 0 exceeds the reduction capacity");
-  float* r = work;                  // work[n, 2n) is reserved (unused)
+  float* r = work;
+  float* zold = work + (size_t)n;   // previous z (flexible PCG)
   float* p = work + 2 * (size_t)n;
   float* Ap = work + 3 * (size_t)n;
-  float* xv = L.work + (size_t)n;   // level-0 x buffer receiving the first sweep
+  float* xv = L.wb(1);              // level-0 x buffer receiving the first sweep
   const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
   const int* done = &ctx->st->done;
   const int gn = red_blocks(n);
@@ -1369,7 +1738,7 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
       if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
       hipGraph_t g = nullptr;
       MGPBD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int c = 0; c < ctx->poll; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, first, gn, gs, false);
+      for (int c = 0; c < ctx->poll; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, zold, first, gn, gs, false);
       (void)hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s);
       hipError_t ce = hipStreamEndCapture(s, &g);
       if (ce != hipSuccess) return mgpbd_fail(MGPBD_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -1385,13 +1754,14 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
   }
 
   k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
-  k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, first ? xv : nullptr, ctx->omega, ctx->ws);
+  const bool fuse_first = first && !half_mode(ctx);
+  k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr, ctx->omega, ctx->ws);
   k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  if (!first) k_blas1<<<nblocks(n), kBlock, 0, s>>>(3, n, 0.f, nullptr, xv);
-  float* zr = vcycle_from(ctx, 0, r, xv, done);
+  float* zr = pcg_precondition(ctx, r, xv, first);
   k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
   k_pcg_fin<FIN_RZ0><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  k_pcg_p<0><<<nblocks(n), kBlock, 0, s>>>(n, zr, p, ctx->st);
+  if (ctx->flexible) k_pcg_p_flex<<<nblocks(n), kBlock, 0, s>>>(n, zr, p, zold, ctx->st, 1);
+  else k_pcg_p<0><<<nblocks(n), kBlock, 0, s>>>(n, zr, p, ctx->st);
   MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
 
   int issued = 0;
@@ -1401,7 +1771,7 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
       issued += ctx->poll;
     } else {
       int chunk = std::min(ctx->poll, maxiter - issued);
-      for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, first, gn, gs, true);
+      for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, zold, first, gn, gs, true);
       MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
       issued += chunk;
       MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));

@@ -17,7 +17,7 @@ Reference behaviour kept by default (parity mode):
 Keyword-only extensions (no reference counterpart; defaults keep the reference's
 behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('reference' | 'extended'), ``galerkin`` ('values' | 'structural'), ``max_coarse``, ``max_levels``,
-``coarse_solver`` ('jacobi' | 'cholesky'), ``f_ext`` (external vertex forces), ``device``.
+``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'), ``f_ext`` (external vertex forces), ``device``.
 """
 
 from __future__ import annotations
@@ -58,7 +58,8 @@ class MGPBDSolverGPU:
                  hold_pinned: bool = False, mu_per_tet=None, relative_tol: bool = False,
                  hierarchy: str = "reference", galerkin: str = "values",
                  max_coarse: int = 400, max_levels: int | None = None,
-                 coarse_sweeps: int = 50, coarse_solver: str = "jacobi", f_ext=None,
+                 coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
+                 precision: str = "fp32", f_ext=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation != "tet":
             raise NotImplementedError("aggregation='pyamg' (smoothed aggregation) is not "
@@ -97,6 +98,7 @@ class MGPBDSolverGPU:
         self.max_levels = max_levels
         self.coarse_sweeps = int(coarse_sweeps)
         self.coarse_solver = coarse_solver
+        self.precision = precision
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -133,7 +135,8 @@ class MGPBDSolverGPU:
             n_smooth=self.n_smooth, omega_jacobi=self.omega_jacobi,
             maxiter_cg=self.maxiter_cg, tol_cg=self.tol_cg, relative=self.relative_tol,
             coarse_sweeps=self.coarse_sweeps, galerkin=self.galerkin,
-            coarse_solver=self.coarse_solver, device=self.device.index)
+            coarse_solver=self.coarse_solver, precision=self.precision,
+            device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()
         if self.verbose:

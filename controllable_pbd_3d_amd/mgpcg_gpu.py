@@ -9,8 +9,12 @@ Differences in mechanism, not in results:
   host polls a done flag every few iterations instead of syncing three times per iteration;
 * Jacobi ping-pongs two buffers instead of copying x_new back to x.
 
-Extension (keyword ``coarse_solver='cholesky'``): the coarsest level is solved exactly with
-an FP64 dense inverse (refactored on every coarse refill) instead of min(50, n) sweeps.
+Extensions (keyword-only, no reference counterpart):
+* ``coarse_solver='cholesky'``: the coarsest level is solved exactly with an FP64 dense
+  inverse (refactored on every coarse refill) instead of min(50, n) sweeps;
+* ``precision='fp16' | 'bf16'``: the V-cycle runs on the diagonally scaled levels stored in
+  half precision (FP32 accumulation), inside a flexible (Polak-Ribiere) FP32 PCG with FP64
+  dot products; the outer SpMV stays FP32.
 """
 
 from __future__ import annotations
@@ -52,7 +56,8 @@ class GPUMultigridSolver:
     def __init__(self, n_smooth: int = 2, omega_jacobi: float = 2.0 / 3.0,
                  maxiter_cg: int = 100, tol_cg: float = 1e-5, *, relative: bool = False,
                  coarse_sweeps: int = 50, galerkin: str = "values",
-                 coarse_solver: str = "jacobi", poll: int = 2, device=None):
+                 coarse_solver: str = "jacobi", precision: str = "fp32",
+                 flexible: bool | None = None, device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
         self.maxiter_cg = int(maxiter_cg)
@@ -65,6 +70,10 @@ class GPUMultigridSolver:
         if coarse_solver not in ("jacobi", "cholesky"):
             raise ValueError("coarse_solver must be 'jacobi' (reference) or 'cholesky'")
         self.coarse_solver = coarse_solver
+        if precision not in ("fp32", "fp16", "bf16"):
+            raise ValueError("precision must be 'fp32', 'fp16' or 'bf16'")
+        self.precision = precision
+        self.flexible = (precision != "fp32") if flexible is None else bool(flexible)
         self.n_levels = 0
         self._initialized = False
         self.ctx = get_context(device)
@@ -138,16 +147,39 @@ class GPUMultigridSolver:
         self.n_levels = len(self.levels)
         self._level_sizes = [L.n_rows for L in self.levels]
         # work vectors per level: b, x, x', r
-        self._work = [torch.zeros(4 * L.n_rows, dtype=torch.float32, device=self.device)
-                      for L in self.levels]
+        self._work = [torch.zeros(4 * 64 * ((L.n_rows + 63) // 64), dtype=torch.float32,
+                                  device=self.device) for L in self.levels]
         n0 = self._level_sizes[0]
         self._pcg_work = torch.zeros(4 * n0, dtype=torch.float32, device=self.device)
         self._b = [w[:L.n_rows] for w, L in zip(self._work, self.levels)]
         self._x = [torch.zeros(L.n_rows, dtype=torch.float32, device=self.device)
                    for L in self.levels]
         self._configure()
+        self._setup_half()
         self.update_all_coarse_matrices()
         self._initialized = True
+
+    def _setup_half(self):
+        """Reduced-precision copies of every level for the V-cycle (extension)."""
+        prec = {"fp32": 0, "fp16": 1, "bf16": 2}[self.precision]
+        self.ctx("mgpbd_mg_set_precision", prec, int(self.flexible))
+        self._half = []
+        if prec == 0:
+            return
+        dev = self.device
+        for l, L in enumerate(self.levels):
+            rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
+            delta = L.indices.astype(np.int64) - rows
+            use16 = bool(len(delta) == 0 or np.abs(delta).max() < 32768)
+            hb = {
+                "hval": torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev),
+                "col16": torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev) if use16 else None,
+                "sd": torch.ones(L.n_rows, dtype=torch.float32, device=dev),
+                "isd": torch.ones(L.n_rows, dtype=torch.float32, device=dev),
+            }
+            self.ctx("mgpbd_mg_set_level_half", l, dptr(hb["hval"]), dptr(hb["col16"]),
+                     dptr(hb["sd"]), dptr(hb["isd"]), dptr(L.diag_pos))
+            self._half.append(hb)
 
     @staticmethod
     def _host_refill(fine_vals, cidx, coff):
@@ -188,6 +220,8 @@ class GPUMultigridSolver:
             C.cache_diag_inv()
         if getattr(self, "_coarse_mode", 0) == 1:
             self.ctx("mgpbd_mg_factor_coarsest")
+        if getattr(self, "_half", None):
+            self.ctx("mgpbd_mg_convert_half")
 
     # -- solve -----------------------------------------------------------------------------
     def vcycle(self, level: int = 0):

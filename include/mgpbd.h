@@ -239,7 +239,8 @@ int mgpbd_negate_and_add(mgpbd_ctx* ctx, int n, const float* b_dev, float* r_dev
 int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega,
                        int coarse_mode, int coarse_sweeps);
 /* Level l operator and transfer to level l+1 (agg/mem_ptr/mem NULL on the coarsest).
- * work_dev: 4 * n_rows floats owned by the caller (b, x, x', r of the level).
+ * work_dev: 4 * 64 * ceil(n_rows / 64) floats owned by the caller (b, x, x', r of the level,
+ * each padded to whole 64-row slices).
  * diagonal_only != 0 declares that every row holds only its diagonal entry (then repeated
  * Jacobi sweeps run as one elementwise pass with identical arithmetic). */
 int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* slice_ptr_dev,
@@ -252,6 +253,23 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
  * change (every coarse refill). Synchronises; MGPBD_ENONFINITE if not positive definite.
  * No reference counterpart (the reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
+/* Reduced-precision V-cycle (extension; no reference counterpart). Every level is used in its
+ * scaled form D^-1/2 A D^-1/2 (unit diagonal), values stored FP16 or BF16 in the level's
+ * SELL positions, products accumulated in FP32; the outer PCG SpMV stays FP32. flexible != 0
+ * selects the Polak-Ribiere beta (flexible PCG) for the variable preconditioner. */
+#define MGPBD_PREC_FP32 0
+#define MGPBD_PREC_FP16 1
+#define MGPBD_PREC_BF16 2
+int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible);
+/* Per-level buffers: hval_dev (uint16, one per SELL slot), col16_dev (int16 column - row per
+ * SELL slot, or NULL to use the int32 columns; only valid when |col - row| < 32768),
+ * sd_dev / isd_dev (n_rows floats, written by mgpbd_mg_convert_half), diag_pos_dev (SELL
+ * position of each row's diagonal). */
+int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, int16_t* col16_dev,
+                            float* sd_dev, float* isd_dev, const int32_t* diag_pos_dev);
+/* Recomputes scales and reduced-precision values of every level from the FP32 values; call
+ * after the values change (after the coarse refill). */
+int mgpbd_mg_convert_half(mgpbd_ctx* ctx);
 /* x = M^-1 b with one V-cycle from x = 0. Replaces vcycle(0) (cons/mgpcg_gpu.py:214-259). */
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
 /* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:

@@ -335,3 +335,80 @@ def test_graph_replay_equals_eager(coarse_solver):
     assert results[0][0] == results[1][0] > 4
     assert np.array_equal(results[0][1].view(np.uint32), results[1][1].view(np.uint32))
     ctx("mgpbd_ctx_use_graphs", 1, 4)
+
+
+def _bf16_to_f32(u16):
+    return (u16.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_half_levels_are_scaled_and_rounded(precision):
+    """Every level's reduced-precision copy equals D^-1/2 A D^-1/2 rounded once (|err| <= half
+    an ulp of the format), with unit diagonal; level-0 column deltas equal col - row."""
+    mesh, s, o = _extended_pair(n=5, seed=3)
+    s2 = _solver(mesh, s.v_p.cpu().numpy(), mu_per_tet=o.mu_per_tet, hierarchy="extended",
+                 galerkin="structural", max_coarse=64, coarse_solver="cholesky",
+                 precision=precision, relative_tol=True, tol_cg=1e-6)
+    mg = s2._gpu_solver
+    eps = 2.0 ** -11 if precision == "fp16" else 2.0 ** -8
+    for l, (L, hb) in enumerate(zip(mg.levels, mg._half)):
+        A = L.to_scipy().astype(np.float64)
+        d = A.diagonal()
+        isd = 1.0 / np.sqrt(d)
+        S = sp.diags(isd) @ A @ sp.diags(isd)
+        S = S.tocsr()
+        raw = hb["hval"].cpu().numpy().view(np.uint16)[L.sell_pos_host]
+        got = raw.view(np.float16).astype(np.float64) if precision == "fp16" else \
+            _bf16_to_f32(raw).astype(np.float64)
+        want = np.asarray(S[np.repeat(np.arange(L.n_rows), np.diff(L.indptr)), L.indices]).reshape(-1)
+        assert np.all(np.abs(got - want) <= eps * np.abs(want) + 1e-30), l
+        assert np.all(np.abs(S.diagonal() - 1.0) < 1e-5)
+        if l == 0:
+            rows = np.repeat(np.arange(L.n_rows), np.diff(L.indptr))
+            c16 = hb["col16"].cpu().numpy()[L.sell_pos_host].astype(np.int64)
+            assert np.array_equal(c16, L.indices - rows)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision):
+    """Reduced-precision V-cycle inside the flexible FP32 PCG: relative residual <= 2e-6 when
+    re-evaluated in FP64, solution within 1e-5 of scipy's FP64 direct solve, and at most 25%
+    more iterations than the FP32 V-cycle on the same system."""
+    mesh, s, o = _extended_pair(n=6, seed=7)
+    v0 = s.v_p.cpu().numpy()
+    s2 = _solver(mesh, v0, mu_per_tet=o.mu_per_tet, hierarchy="extended", galerkin="structural",
+                 max_coarse=64, coarse_solver="cholesky", precision=precision,
+                 relative_tol=True, tol_cg=1e-6)
+    s._compute_C_and_gradC()
+    s2._compute_C_and_gradC()
+    b = s._b_gpu.cpu().numpy()
+    assert np.array_equal(b, s2._b_gpu.cpu().numpy())
+    it32 = s._gpu_solver.solve(s._b_gpu, s._dlam_gpu)
+    it = s2._gpu_solver.solve(s2._b_gpu, s2._dlam_gpu)
+    x = s2._dlam_gpu.cpu().numpy().astype(np.float64)
+    A = s2._A.to_scipy().astype(np.float64)
+    res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    x64 = sp.linalg.spsolve(A.tocsc(), b.astype(np.float64))
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    print(f"[mixed] {precision}: iters {it} (fp32 {it32}) rel residual {res:.2e} err {err:.2e}",
+          flush=True)
+    assert res <= 2e-6 and err <= 1e-5
+    assert it <= int(np.ceil(1.25 * it32)) + 1
+
+
+def test_mixed_precision_substeps_match_fp32():
+    """Ten substeps with the FP16 V-cycle stay within 1e-4 h of the FP32 V-cycle run (both
+    solve every system to the same 1e-6 relative PCG tolerance)."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(6, seed=9)
+    kw = dict(mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended", galerkin="structural",
+              relative_tol=True, tol_cg=1e-6, max_coarse=64, coarse_solver="cholesky")
+    a = _solver(mesh, None, **kw)
+    b = _solver(mesh, None, precision="fp16", **kw)
+    worst = 0.0
+    for k in range(10):
+        a.step(0)
+        b.step(0)
+        worst = max(worst, float((a.v_p - b.v_p).abs().max()))
+    print(f"[mixed] substeps max|dx|/h = {worst / mesh.h:.2e}", flush=True)
+    assert worst <= 1e-4 * mesh.h
