@@ -86,6 +86,23 @@ SIGNATURES = {
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_pcg_solve": (c_int, [_V, _V, _V, _V, c_int, c_double, c_int,
                                 ctypes.POINTER(c_int), F64P]),
+    # warm start
+    "mgpbd_pcg_solve_guess": (c_int, [_V, _V, _V, _V, _V, c_int, c_double, c_int,
+                                      ctypes.POINTER(c_int), F64P]),
+    "mgpbd_spring_drag": (c_int, [_V, c_int, _V, _V, c_int, c_float, c_float, c_float, c_float]),
+    # learned warm start
+    "mgpbd_gnn_param_count": (c_int, []),
+    "mgpbd_gnn_workspace_bytes": (c_int64, [c_int]),
+    "mgpbd_gnn_features": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_forward": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_loss_mse": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_loss_residual": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_backward": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_adam": (c_int, [_V, _V, _V, _V, _V, c_float, c_float, c_float, c_float, c_int,
+                               _V, _V]),
+    "mgpbd_gnn_pack": (c_int, [_V, _V, _V, _V]),
+    "mgpbd_gnn_gemm64": (c_int, [_V, c_int, _V, _V, _V, _V, _V, c_int, _V]),
+    "mgpbd_gnn_wgrad64": (c_int, [_V, c_int, _V, _V, _V, _V]),
 }
 
 _lib = None

@@ -1,0 +1,701 @@
+// Learned warm start for the PCG (extension; no reference counterpart): a message-passing
+// network over the tet face-adjacency graph that predicts dlambda of the first Newton
+// iteration from per-constraint features. Forward and backward run here; the three
+// 128 -> 64 message-passing layers use BF16 MFMA (v_mfma_f32_32x32x8bf16_1k) with FP32
+// accumulation, the 8 -> 64 encoder and 64 -> 1 decoder are FP32.
+//
+// Network (n tets, H = 64):
+//   h0 = relu(x W_in + b_in)                         x: n x 8 features
+//   m_l = mean_{j in N(i)} h_{l-1,j}                 N = face neighbours (<= 4)
+//   h_l = relu(h_{l-1} W_l^self + m_l W_l^nbr + b_l)   l = 1..3
+//   y   = h_3 w_out + b_out
+// Parameters (FP32, flattened): W_in[8][64], b_in[64], {W_l[128][64] (rows 0..63 self,
+// 64..127 neighbour), b_l[64]} x 3, w_out[64], b_out[1].
+// Activations are stored as BF16 (n x 64 row-major); BF16 weight copies are kept both
+// transposed (out-major, the forward B operand) and as-is (the backward B operand).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "mgpbd_internal.h"
+
+namespace mgpbd_gnn {
+
+constexpr int H = MGPBD_GNN_HIDDEN;       // 64
+constexpr int F = MGPBD_GNN_FEATURES;     // 8
+constexpr int NL = MGPBD_GNN_LAYERS;      // 3
+constexpr int kBlock = 256;
+
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+typedef float v16f32 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float bf2f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// offsets into the flattened FP32 parameter vector
+__host__ __device__ constexpr int off_win() { return 0; }
+__host__ __device__ constexpr int off_bin() { return F * H; }
+__host__ __device__ constexpr int off_wl(int l) { return F * H + H + l * (2 * H * H + H); }
+__host__ __device__ constexpr int off_bl(int l) { return off_wl(l) + 2 * H * H; }
+__host__ __device__ constexpr int off_wout() { return off_wl(NL); }
+__host__ __device__ constexpr int off_bout() { return off_wout() + H; }
+__host__ __device__ constexpr int n_params() { return off_bout() + 1; }
+
+// ------------------------------------------------------------------------------------------
+// MFMA GEMM: Y[n x 64] = act(A1 W1 + A2 W2 + bias), A* BF16 n x 64, W* as out-major BF16
+// [64 out][64 in]. One wavefront computes 32 rows x 64 columns (two 32x32 MFMA tiles).
+// MFMA 32x32x8 operand layout (wave64): A fragment lane l holds A[l%32][4(l/32) .. +3],
+// B fragment holds B[4(l/32) .. +3][l%32]; result register r of lane l is
+// D[8(r/4) + 4(l/32) + r%4][l%32].
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ v4i16 load4(const uint16_t* p) {
+  return *reinterpret_cast<const v4i16*>(p);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_gemm64(int n, const uint16_t* __restrict__ A1, const uint16_t* __restrict__ W1t,
+         const uint16_t* __restrict__ A2, const uint16_t* __restrict__ W2t,
+         const float* __restrict__ bias, int relu, uint16_t* __restrict__ Ybf,
+         float* __restrict__ Yf) {
+  const int lane = threadIdx.x & 63;
+  const int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
+  if (base >= n) return;
+  const int ra = base + (lane & 31);
+  const int kq = 4 * (lane >> 5);
+  v16f32 acc0 = {}, acc1 = {};
+  const v4i16 zero = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < H / 8; ++s) {
+    const v4i16 a = ra < n ? load4(A1 + (size_t)ra * H + 8 * s + kq) : zero;
+    const v4i16 b0 = load4(W1t + (size_t)(lane & 31) * H + 8 * s + kq);
+    const v4i16 b1 = load4(W1t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+  }
+  if (A2) {
+#pragma unroll
+    for (int s = 0; s < H / 8; ++s) {
+      const v4i16 a = ra < n ? load4(A2 + (size_t)ra * H + 8 * s + kq) : zero;
+      const v4i16 b0 = load4(W2t + (size_t)(lane & 31) * H + 8 * s + kq);
+      const v4i16 b1 = load4(W2t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = base + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+    if (row >= n) continue;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = 32 * c + (lane & 31);
+      float v = (c == 0 ? acc0[r] : acc1[r]) + (bias ? bias[col] : 0.0f);
+      if (relu) v = fmaxf(v, 0.0f);
+      if (Ybf) Ybf[(size_t)row * H + col] = f2bf(v);
+      if (Yf) Yf[(size_t)row * H + col] = v;
+    }
+  }
+}
+
+// Weight gradient partials: P[block][k][j] = sum over the block's rows i of A[i][k] dZ[i][j]
+// (A, dZ BF16 n x 64). Each of the 4 wavefronts owns one 32x32 quadrant of the 64x64 result;
+// the MFMA inner dimension runs over rows.
+constexpr int kWgradRows = 1024;
+
+__device__ __forceinline__ v4i16 load4_strided(const uint16_t* p, int stride, int valid) {
+  v4i16 v;
+  v.x = valid > 0 ? (short)p[0] : (short)0;
+  v.y = valid > 1 ? (short)p[stride] : (short)0;
+  v.z = valid > 2 ? (short)p[2 * stride] : (short)0;
+  v.w = valid > 3 ? (short)p[3 * stride] : (short)0;
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
+        float* __restrict__ partial) {
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int qk = w >> 1, qj = w & 1;
+  const int r0 = blockIdx.x * kWgradRows;
+  const int r1 = min(n, r0 + kWgradRows);
+  const int kq = 4 * (lane >> 5);
+  v16f32 acc = {};
+  for (int i0 = r0; i0 < r1; i0 += 8) {
+    const int i = i0 + kq;
+    const int valid = r1 - i;
+    const v4i16 a = load4_strided(A + (size_t)i * H + 32 * qk + (lane & 31), H, valid);
+    const v4i16 b = load4_strided(dZ + (size_t)i * H + 32 * qj + (lane & 31), H, valid);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  float* P = partial + (size_t)blockIdx.x * H * H;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int k = 32 * qk + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+    const int j = 32 * qj + (lane & 31);
+    P[k * H + j] = acc[r];
+  }
+}
+
+// out[e] (+)= sum over partial blocks (fixed order)
+__global__ void __launch_bounds__(kBlock)
+k_sum_partials(int nblk, int len, const float* __restrict__ partial, float* __restrict__ out,
+               int accumulate) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= len) return;
+  float s = 0.0f;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * len + e];
+  out[e] = accumulate ? out[e] + s : s;
+}
+
+// ------------------------------------------------------------------------------------------
+// Elementwise / small kernels
+// ------------------------------------------------------------------------------------------
+// m_i = mean of h_j over the face neighbours (BF16 in/out, FP32 sum in neighbour order)
+__global__ void __launch_bounds__(kBlock)
+k_mean_nbr(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ h,
+           uint16_t* __restrict__ m) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;   // one thread per (row, 4 columns)
+  const int i = t >> 4;
+  const int c = 4 * (t & 15);
+  if (i >= n) return;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  int d = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = nbr[4 * (size_t)i + q];
+    if (j < 0) continue;
+    ++d;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) s[e] += bf2f(h[(size_t)j * H + c + e]);
+  }
+  const float inv = d > 0 ? 1.0f / (float)d : 0.0f;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) m[(size_t)i * H + c + e] = f2bf(s[e] * inv);
+}
+
+// Backward of the mean: dh_j += sum_{i in N(j)} dm_i / deg_i (the face graph is symmetric),
+// written into dh (FP32) on top of its current value.
+__global__ void __launch_bounds__(kBlock)
+k_mean_nbr_bwd(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ dm,
+               float* __restrict__ dh) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int j = t >> 4;
+  const int c = 4 * (t & 15);
+  if (j >= n) return;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = nbr[4 * (size_t)j + q];
+    if (i < 0) continue;
+    int d = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) d += nbr[4 * (size_t)i + u] >= 0;
+    const float inv = 1.0f / (float)d;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) s[e] = fmaf(bf2f(dm[(size_t)i * H + c + e]), inv, s[e]);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) dh[(size_t)j * H + c + e] += s[e];
+}
+
+// h0 = relu(x W_in + b_in), FP32 weights, BF16 output
+__global__ void __launch_bounds__(kBlock)
+k_encoder(int n, const float* __restrict__ x, const float* __restrict__ P,
+          uint16_t* __restrict__ h0) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int i = t >> 6, c = t & 63;
+  if (i >= n) return;
+  float v = P[off_bin() + c];
+#pragma unroll
+  for (int k = 0; k < F; ++k) v = fmaf(x[(size_t)i * F + k], P[off_win() + k * H + c], v);
+  h0[(size_t)i * H + c] = f2bf(fmaxf(v, 0.0f));
+}
+
+// y = h3 w_out + b_out
+__global__ void __launch_bounds__(kBlock)
+k_decoder(int n, const uint16_t* __restrict__ h3, const float* __restrict__ P,
+          float* __restrict__ y) {
+  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  float v = bf2f(h3[(size_t)i * H + lane]) * P[off_wout() + lane];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) y[i] = v + P[off_bout()];
+}
+
+// dh3[i][c] = dy_i w_out[c]; masked by h3 > 0 later with the layer's ReLU
+__global__ void __launch_bounds__(kBlock)
+k_decoder_bwd(int n, const float* __restrict__ dy, const float* __restrict__ P,
+              float* __restrict__ dh3) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int i = t >> 6, c = t & 63;
+  if (i >= n) return;
+  dh3[(size_t)i * H + c] = dy[i] * P[off_wout() + c];
+}
+
+// dZ = dH (FP32) masked by h > 0, stored BF16
+__global__ void __launch_bounds__(kBlock)
+k_relu_bwd(int n, const float* __restrict__ dh, const uint16_t* __restrict__ h,
+           uint16_t* __restrict__ dz) {
+  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= (size_t)n * H) return;
+  dz[e] = f2bf(bf2f(h[e]) > 0.0f ? dh[e] : 0.0f);
+}
+
+// column sums over rows (bias / decoder gradients), block partials: P[b][c]
+__global__ void __launch_bounds__(kBlock)
+k_colsum_partial(int n, int width, const uint16_t* __restrict__ Zbf, const float* __restrict__ Zf,
+                 const float* __restrict__ rowscale, const uint16_t* __restrict__ Sbf,
+                 float* __restrict__ partial) {
+  // out[c] = sum_i (Z[i][c] or rowscale_i * S[i][c]) over the block's 1024 rows
+  const int c = threadIdx.x;
+  if (c >= width) return;
+  const int r0 = blockIdx.x * 1024, r1 = min(n, r0 + 1024);
+  float s = 0.0f;
+  for (int i = r0; i < r1; ++i) {
+    float v;
+    if (Zbf) v = bf2f(Zbf[(size_t)i * width + c]);
+    else if (Zf) v = Zf[(size_t)i * width + c];
+    else v = rowscale[i] * bf2f(Sbf[(size_t)i * H + c]);
+    s += v;
+  }
+  partial[(size_t)blockIdx.x * width + c] = s;
+}
+
+// encoder weight gradient partials: P[b][k][c] = sum_i x[i][k] dz0[i][c]
+__global__ void __launch_bounds__(kBlock)
+k_encoder_wgrad(int n, const float* __restrict__ x, const uint16_t* __restrict__ dz0,
+                float* __restrict__ partial) {
+  const int e = threadIdx.x;   // 256 threads: k = e / 64 (+4), c = e % 64
+  const int r0 = blockIdx.x * 1024, r1 = min(n, r0 + 1024);
+  const int c = e & 63;
+  const int k0 = e >> 6;       // 0..3
+  float s0 = 0.f, s1 = 0.f;
+  for (int i = r0; i < r1; ++i) {
+    const float dz = bf2f(dz0[(size_t)i * H + c]);
+    s0 = fmaf(x[(size_t)i * F + k0], dz, s0);
+    s1 = fmaf(x[(size_t)i * F + k0 + 4], dz, s1);
+  }
+  partial[(size_t)blockIdx.x * F * H + k0 * H + c] = s0;
+  partial[(size_t)blockIdx.x * F * H + (k0 + 4) * H + c] = s1;
+}
+
+// Adam on the FP32 parameters, then BF16 copies of the message-passing weights:
+// wt[l][0|1][out][in] (forward operand) and w[l][0|1][in][out] (backward operand).
+__global__ void __launch_bounds__(kBlock)
+k_adam(int np, float* __restrict__ P, const float* __restrict__ g, float* __restrict__ m,
+       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= np) return;
+  const float gi = g[e];
+  const float mi = b1 * m[e] + (1.0f - b1) * gi;
+  const float vi = b2 * v[e] + (1.0f - b2) * gi * gi;
+  m[e] = mi;
+  v[e] = vi;
+  P[e] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_pack_bf16(const float* __restrict__ P, uint16_t* __restrict__ wt, uint16_t* __restrict__ w) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;   // over NL * 2 * H * H
+  if (e >= NL * 2 * H * H) return;
+  const int l = e / (2 * H * H);
+  const int rem = e % (2 * H * H);
+  const int half = rem / (H * H);
+  const int in = (rem % (H * H)) / H, out = rem % H;
+  const float val = P[off_wl(l) + (half * H + in) * H + out];
+  wt[(size_t)(l * 2 + half) * H * H + out * H + in] = f2bf(val);
+  w[(size_t)(l * 2 + half) * H * H + in * H + out] = f2bf(val);
+}
+
+// ------------------------------------------------------------------------------------------
+// Features. x[i] = [jac/s, lp/s, C/sC, std log alpha, std log d, vol/mean - 1, deg/4, 1]
+// with jac = b/A_ii, lp = previous lambda, s = rms(jac) (or rms(lp) when larger); the
+// scalars come from FP64 sums passed in by the host-side reduction helper.
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_feature_sums(int n, const float* __restrict__ b, const float* __restrict__ dinv,
+               const float* __restrict__ lp, const float* __restrict__ C,
+               const float* __restrict__ alpha, const float* __restrict__ vol,
+               double* __restrict__ partial) {
+  // per block: [sum jac^2, sum lp^2, sum C^2, sum la, sum la^2, sum ld, sum ld^2, sum vol]
+  __shared__ double sh[8][kBlock];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double jac = (double)b[i] * (double)dinv[i];
+    const double l = lp ? (double)lp[i] : 0.0;
+    const double la = log((double)alpha[i]);
+    const double ld = dinv[i] > 0.0f ? -log((double)dinv[i]) : 0.0;
+    acc[0] += jac * jac; acc[1] += l * l; acc[2] += (double)C[i] * C[i];
+    acc[3] += la; acc[4] += la * la; acc[5] += ld; acc[6] += ld * ld; acc[7] += vol[i];
+  }
+  for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int st = kBlock / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 8; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) partial[blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_feature_stats(int n, int nblk, const double* __restrict__ partial,
+                                float* __restrict__ stats) {
+  // stats: [s, sC, mean la, std la, mean ld, std ld, mean vol, unused]
+  if (threadIdx.x != 0) return;
+  double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int bk = 0; bk < nblk; ++bk)
+    for (int q = 0; q < 8; ++q) t[q] += partial[bk * 8 + q];
+  const double nn = (double)(n > 0 ? n : 1);
+  const double sj = sqrt(t[0] / nn), sl = sqrt(t[1] / nn);
+  const double s = fmax(fmax(sj, sl), 1e-30);
+  const double ma = t[3] / nn, md = t[5] / nn;
+  stats[0] = (float)s;
+  stats[1] = (float)fmax(sqrt(t[2] / nn), 1e-30);
+  stats[2] = (float)ma;
+  stats[3] = (float)fmax(sqrt(fmax(t[4] / nn - ma * ma, 0.0)), 1e-6);
+  stats[4] = (float)md;
+  stats[5] = (float)fmax(sqrt(fmax(t[6] / nn - md * md, 0.0)), 1e-6);
+  stats[6] = (float)fmax(t[7] / nn, 1e-30);
+  stats[7] = 0.0f;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_features(int n, const float* __restrict__ b, const float* __restrict__ dinv,
+           const float* __restrict__ lp, const float* __restrict__ C,
+           const float* __restrict__ alpha, const float* __restrict__ vol,
+           const int32_t* __restrict__ nbr, const float* __restrict__ st,
+           float* __restrict__ x) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float s = st[0];
+  int d = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) d += nbr[4 * (size_t)i + q] >= 0;
+  const float la = logf(alpha[i]);
+  const float ld = dinv[i] > 0.0f ? -logf(dinv[i]) : st[4];
+  float* o = x + (size_t)i * F;
+  o[0] = b[i] * dinv[i] / s;
+  o[1] = lp ? lp[i] / s : 0.0f;
+  o[2] = C[i] / st[1];
+  o[3] = (la - st[2]) / st[3];
+  o[4] = (ld - st[4]) / st[5];
+  o[5] = vol[i] / st[6] - 1.0f;
+  o[6] = 0.25f * (float)d;
+  o[7] = 1.0f;
+}
+
+// prediction dlambda = s (lp/s + y) = lp + s y ; loss gradient dy = 2 (y - t) / n with the
+// normalised target t = (dlambda* - lp) / s; per-block partial of the squared error
+__global__ void __launch_bounds__(kBlock)
+k_predict(int n, const float* __restrict__ y, const float* __restrict__ lp,
+          const float* __restrict__ st, float* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  out[i] = (lp ? lp[i] : 0.0f) + st[0] * y[i];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_loss_mse(int n, const float* __restrict__ y, const float* __restrict__ target,
+           const float* __restrict__ lp, const float* __restrict__ st, float* __restrict__ dy,
+           double* __restrict__ partial) {
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float t = (target[i] - (lp ? lp[i] : 0.0f)) / st[0];
+    const float e = y[i] - t;
+    dy[i] = 2.0f * e / (float)n;
+    acc += (double)e * (double)e;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// r = A x0 - b in place over ax, partials of r.r and b.b (2 per block)
+__global__ void __launch_bounds__(kBlock)
+k_resid_norms(int n, float* __restrict__ ax, const float* __restrict__ b,
+              double* __restrict__ partial) {
+  __shared__ double sh[2][kBlock];
+  double rr = 0.0, bb = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float r = ax[i] - b[i];
+    ax[i] = r;
+    rr += (double)r * r;
+    bb += (double)b[i] * b[i];
+  }
+  sh[0][threadIdx.x] = rr;
+  sh[1][threadIdx.x] = bb;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + s];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sh[0][0]; partial[2 * blockIdx.x + 1] = sh[1][0]; }
+}
+
+// scal[0] = r.r / b.b (the loss), scal[1] = 2 s / b.b (d loss / d y = scal[1] A r)
+__global__ void k_resid_fin(int nblk, const double* __restrict__ partial,
+                            const float* __restrict__ st, double* __restrict__ scal,
+                            double* __restrict__ loss) {
+  if (threadIdx.x != 0) return;
+  double rr = 0.0, bb = 0.0;
+  for (int b = 0; b < nblk; ++b) { rr += partial[2 * b]; bb += partial[2 * b + 1]; }
+  const double inv = bb > 0.0 ? 1.0 / bb : 0.0;
+  scal[0] = rr * inv;
+  scal[1] = 2.0 * (double)st[0] * inv;
+  if (loss) *loss = rr * inv;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_resid_dy(int n, const float* __restrict__ ar, const double* __restrict__ scal,
+           float* __restrict__ dy) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  dy[i] = (float)(scal[1] * (double)ar[i]);
+}
+
+__global__ void k_sum_double(int nblk, const double* __restrict__ partial, double* __restrict__ out,
+                             double scale) {
+  if (threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += partial[b];
+  *out = s * scale;
+}
+
+}  // namespace mgpbd_gnn
+
+using namespace mgpbd_gnn;
+
+static inline int nb(int64_t n) { return (int)std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+
+struct GnnWorkspace {
+  // activations: h[0..NL] and m[1..NL] BF16 n x 64; y n; dh FP32 n x 64; dz, dm BF16 n x 64
+  uint16_t* h[NL + 1];
+  uint16_t* m[NL + 1];
+  float* y;
+  float* dh;
+  float* dh2;
+  uint16_t* dz;
+  uint16_t* dmb;
+  float* partial;   // max(blocks) * 64 * 64
+  double* dpartial;
+  double* scal;
+};
+
+static void carve(void* ws, int n, GnnWorkspace& W) {
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return (void*)q; };
+  const size_t act = (size_t)n * H * sizeof(uint16_t);
+  for (int l = 0; l <= NL; ++l) W.h[l] = (uint16_t*)take(act);
+  W.m[0] = nullptr;
+  for (int l = 1; l <= NL; ++l) W.m[l] = (uint16_t*)take(act);
+  W.y = (float*)take((size_t)n * 4);
+  W.dh = (float*)take((size_t)n * H * 4);
+  W.dh2 = (float*)take((size_t)n * H * 4);
+  W.dz = (uint16_t*)take(act);
+  W.dmb = (uint16_t*)take(act);
+  const int nblk = (n + kWgradRows - 1) / kWgradRows;
+  W.partial = (float*)take((size_t)std::max(nblk, 1) * H * H * 4);
+  W.dpartial = (double*)take((size_t)std::max(nb(n), 1) * 8 * 8);
+  W.scal = (double*)take(64);
+}
+
+static size_t workspace_bytes(int n) {
+  const size_t act = ((size_t)n * H * 2 + 255) / 256 * 256;
+  const size_t f4 = ((size_t)n * 4 + 255) / 256 * 256;
+  const size_t fh = ((size_t)n * H * 4 + 255) / 256 * 256;
+  const int nblk = (n + kWgradRows - 1) / kWgradRows;
+  return act * (2 * NL + 1) + f4 + 2 * fh + 2 * act +
+         (((size_t)std::max(nblk, 1) * H * H * 4 + 255) / 256 * 256) +
+         (((size_t)std::max(nb(n), 1) * 64 + 255) / 256 * 256) + 256;
+}
+
+extern "C" {
+
+int mgpbd_gnn_param_count(void) { return n_params(); }
+int64_t mgpbd_gnn_workspace_bytes(int n) { return n < 0 ? -1 : (int64_t)workspace_bytes(n); }
+
+int mgpbd_gnn_features(mgpbd_ctx* ctx, int n, const float* b, const float* dinv,
+                       const float* lam_prev, const float* C, const float* alpha,
+                       const float* vol, const int32_t* nbr, float* x, float* stats, void* ws) {
+  if (!ctx || n <= 0 || !b || !dinv || !C || !alpha || !vol || !nbr || !x || !stats || !ws)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_features: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  const int g = std::min(nb(n), 1024);
+  k_feature_sums<<<g, kBlock, 0, s>>>(n, b, dinv, lam_prev, C, alpha, vol, W.dpartial);
+  k_feature_stats<<<1, 64, 0, s>>>(n, g, W.dpartial, stats);
+  k_features<<<nb(n), kBlock, 0, s>>>(n, b, dinv, lam_prev, C, alpha, vol, nbr, stats, x);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_features");
+  return 0;
+}
+
+int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
+                      const float* stats, const float* lam_prev, const float* params,
+                      const uint16_t* wt, void* ws, float* dlam_out) {
+  if (!ctx || n <= 0 || !nbr || !x || !stats || !params || !wt || !ws)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_forward: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  k_encoder<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, x, params, W.h[0]);
+  const int gg = (n + 127) / 128;   // 4 wavefronts x 32 rows per block
+  for (int l = 0; l < NL; ++l) {
+    k_mean_nbr<<<nb((int64_t)n * 16), kBlock, 0, s>>>(n, nbr, W.h[l], W.m[l + 1]);
+    const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
+    const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
+    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.h[l], w_self, W.m[l + 1], w_nbr, params + off_bl(l), 1,
+                                   W.h[l + 1], nullptr);
+  }
+  k_decoder<<<nb((int64_t)n * 64), kBlock, 0, s>>>(n, W.h[NL], params, W.y);
+  if (dlam_out) k_predict<<<nb(n), kBlock, 0, s>>>(n, W.y, lam_prev, stats, dlam_out);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_forward");
+  return 0;
+}
+
+int mgpbd_gnn_loss_mse(mgpbd_ctx* ctx, int n, const float* stats, const float* lam_prev,
+                       const float* target, void* ws, float* dy, double* loss_out) {
+  if (!ctx || n <= 0 || !stats || !target || !ws || !dy)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_loss_mse: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  const int gl = std::min(nb(n), 1024);
+  k_loss_mse<<<gl, kBlock, 0, s>>>(n, W.y, target, lam_prev, stats, dy, W.dpartial);
+  if (loss_out) k_sum_double<<<1, 64, 0, s>>>(gl, W.dpartial, loss_out, 1.0 / n);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_loss_mse");
+  return 0;
+}
+
+int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
+                            const int32_t* sell_idx, const float* sell_val, const float* b,
+                            const float* dlam, const float* stats, float* tmp, void* ws,
+                            float* dy, double* loss_out) {
+  if (!ctx || n <= 0 || !slice_ptr || !sell_idx || !sell_val || !b || !dlam || !stats || !tmp ||
+      !ws || !dy)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_loss_residual: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  float* r = tmp;
+  float* ar = tmp + n;
+  if (int e = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, dlam, r)) return e;
+  const int gl = std::min(nb(n), 1024);
+  k_resid_norms<<<gl, kBlock, 0, s>>>(n, r, b, W.dpartial);
+  k_resid_fin<<<1, 64, 0, s>>>(gl, W.dpartial, stats, W.scal, loss_out);
+  if (int e = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, r, ar)) return e;
+  k_resid_dy<<<nb(n), kBlock, 0, s>>>(n, ar, W.scal, dy);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_loss_residual");
+  return 0;
+}
+
+int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
+                       const float* params, const uint16_t* w, const float* dy, void* ws,
+                       float* grad) {
+  if (!ctx || n <= 0 || !nbr || !x || !params || !w || !dy || !ws || !grad)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_backward: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  const int nblk = (n + kWgradRows - 1) / kWgradRows;
+  const int ncs = (n + 1023) / 1024;
+  // decoder: d w_out = sum_i dy_i h3_i, d b_out = sum_i dy_i, dh3 = dy w_out^T
+  k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, nullptr, nullptr, dy, W.h[NL], W.partial);
+  k_sum_partials<<<1, kBlock, 0, s>>>(ncs, H, W.partial, grad + off_wout(), 0);
+  k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, 1, nullptr, dy, nullptr, nullptr, W.partial);
+  k_sum_partials<<<1, kBlock, 0, s>>>(ncs, 1, W.partial, grad + off_bout(), 0);
+  k_decoder_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, dy, params, W.dh);
+  const int gg = (n + 127) / 128;
+  for (int l = NL - 1; l >= 0; --l) {
+    // dZ = dh_{l+1} * (h_{l+1} > 0)
+    k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[l + 1], W.dz);
+    // weight and bias gradients
+    k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.h[l], W.dz, W.partial);
+    k_sum_partials<<<nb(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l), 0);
+    k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.m[l + 1], W.dz, W.partial);
+    k_sum_partials<<<nb(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l) + H * H, 0);
+    k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
+    k_sum_partials<<<1, kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bl(l), 0);
+    // input gradients: dh_l = dZ W_self^T + mean-backward(dZ W_nbr^T)
+    const uint16_t* w_self = w + (size_t)(2 * l) * H * H;
+    const uint16_t* w_nbr = w + (size_t)(2 * l + 1) * H * H;
+    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_nbr, nullptr, nullptr, nullptr, 0, W.dmb, nullptr);
+    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_self, nullptr, nullptr, nullptr, 0, nullptr, W.dh2);
+    k_mean_nbr_bwd<<<nb((int64_t)n * 16), kBlock, 0, s>>>(n, nbr, W.dmb, W.dh2);
+    std::swap(W.dh, W.dh2);
+  }
+  // encoder: dZ0 = dh0 * (h0 > 0); dW_in = x^T dZ0; db_in = sum dZ0
+  k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[0], W.dz);
+  k_encoder_wgrad<<<ncs, kBlock, 0, s>>>(n, x, W.dz, W.partial);
+  k_sum_partials<<<nb(F * H), kBlock, 0, s>>>(ncs, F * H, W.partial, grad + off_win(), 0);
+  k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
+  k_sum_partials<<<1, kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bin(), 0);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_backward");
+  return 0;
+}
+
+int mgpbd_gnn_adam(mgpbd_ctx* ctx, float* params, const float* grad, float* m, float* v,
+                   float lr, float beta1, float beta2, float eps, int step, uint16_t* wt,
+                   uint16_t* w) {
+  if (!ctx || !params || !grad || !m || !v || step < 1)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_adam: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  k_adam<<<nb(n_params()), kBlock, 0, s>>>(n_params(), params, grad, m, v, lr, beta1, beta2, eps, c1, c2);
+  if (wt && w) k_pack_bf16<<<nb(NL * 2 * H * H), kBlock, 0, s>>>(params, wt, w);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_adam");
+  return 0;
+}
+
+int mgpbd_gnn_pack(mgpbd_ctx* ctx, const float* params, uint16_t* wt, uint16_t* w) {
+  if (!ctx || !params || !wt || !w) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_pack: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  k_pack_bf16<<<nb(NL * 2 * H * H), kBlock, 0, s>>>(params, wt, w);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_pack");
+  return 0;
+}
+
+// Unit entry for tests: Y = act(A1 W1 + A2 W2 + bias) with the MFMA kernel (BF16 operands).
+int mgpbd_gnn_gemm64(mgpbd_ctx* ctx, int n, const uint16_t* A1, const uint16_t* W1t,
+                     const uint16_t* A2, const uint16_t* W2t, const float* bias, int relu,
+                     float* Y) {
+  if (!ctx || n <= 0 || !A1 || !W1t || !Y || (A2 && !W2t))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_gemm64: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  k_gemm64<<<(n + 127) / 128, kBlock, 0, s>>>(n, A1, W1t, A2, W2t, bias, relu, nullptr, Y);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_gemm64");
+  return 0;
+}
+
+// Unit entry for tests: G[64][64] = A^T dZ (A, dZ BF16 n x 64) with the MFMA reduction.
+int mgpbd_gnn_wgrad64(mgpbd_ctx* ctx, int n, const uint16_t* A, const uint16_t* dZ,
+                      float* partial, float* G) {
+  if (!ctx || n <= 0 || !A || !dZ || !partial || !G)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_wgrad64: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  const int nblk = (n + kWgradRows - 1) / kWgradRows;
+  k_wgrad<<<nblk, kBlock, 0, s>>>(n, A, dZ, partial);
+  k_sum_partials<<<nb(H * H), kBlock, 0, s>>>(nblk, H * H, partial, G, 0);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_wgrad64");
+  return 0;
+}
+
+}  // extern "C"

@@ -378,6 +378,17 @@ k_apply_dpos(int nv, float* __restrict__ x, const float* __restrict__ dpos,
   for (int r = 0; r < 3; ++r) x[3 * (size_t)v + r] = fmaf(omega, dpos[3 * (size_t)v + r], x[3 * (size_t)v + r]);
 }
 
+// f_ext = 0 except the dragged vertex: k (target - x)
+__global__ void __launch_bounds__(kBlock)
+k_spring_drag(int nv, const float* __restrict__ x, float* __restrict__ f, int vertex, float tx,
+              float ty, float tz, float k) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= 3 * nv) return;
+  const int v = e / 3, c = e % 3;
+  const float t = c == 0 ? tx : (c == 1 ? ty : tz);
+  f[e] = v == vertex ? k * (t - x[e]) : 0.0f;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_predict(int nv, float* __restrict__ x, float* __restrict__ vel, float* __restrict__ x_old,
           const float* __restrict__ invm, const float* __restrict__ fext, float gx, float gy,
@@ -637,16 +648,77 @@ k_dot(int n, const float* __restrict__ a, const float* __restrict__ b, RedWS ws)
 // ------------------------------------------------------------------------------------------
 struct PcgState {
   double rz, pAp, alpha, beta, rr, bb, tol2, tol;
+  double rr0, rr_guess, rr_alt;   // initial residuals (started from a guess)
   int done;     // 0 running, 1 converged, 2 |pAp| < 1e-30, 3 |rz| < 1e-30
   int iters;    // completed iterations (the reference's cg_iters)
   int k;        // iteration index
   int relative;
+  int start;    // 0: x = 0, 1: the guess in x, 2: the alternative guess
 };
 
 __global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
   st->done = 0; st->iters = 0; st->k = 0; st->relative = relative; st->tol = tol;
   st->rz = st->pAp = st->alpha = st->beta = st->rr = st->bb = 0.0;
+  st->rr0 = st->rr_guess = st->rr_alt = 0.0;
+  st->start = 0;
   st->tol2 = tol * tol;
+}
+
+// Started from a guess: partials of b.b (slot 0), r.r (slot 1) and q.q (slot 2), where
+// r = b - A x_guess and q = b - A x_alt were written by k_sell<2>.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_guess_norms(int n, const float* __restrict__ b, const float* __restrict__ r,
+                  const float* __restrict__ q, RedWS ws) {
+  double bb = 0.0, rr = 0.0, qq = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double bi = b[i], ri = r[i];
+    bb += bi * bi;
+    rr += ri * ri;
+    if (q) { const double qi = q[i]; qq += qi * qi; }
+  }
+  const double b0 = block_sum(bb);
+  const double b1 = block_sum(rr);
+  const double b2 = block_sum(qq);
+  store_partial(ws, 0, b0);
+  store_partial(ws, 1, b1);
+  store_partial(ws, 2, b2);
+}
+
+// Picks the start with the smallest residual among the guess, the alternative guess and zero
+// (the guess wins a tie with the alternative; zero wins a tie and any non-finite residual).
+// The stopping test stays relative to ||b|| (or absolute), as from a zero start.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_fin_guess(RedWS ws, int nparts, int has_alt, PcgState* st) {
+  const double bb = sum_partials(ws, 0, nparts);
+  const double ra = sum_partials(ws, 1, nparts);
+  const double rb = sum_partials(ws, 2, nparts);
+  if (threadIdx.x != 0) return;
+  st->bb = bb;
+  if (st->relative) st->tol2 = st->tol * st->tol * bb;
+  st->rr_guess = ra;
+  st->rr_alt = has_alt ? rb : 0.0;
+  int start = 1;
+  double best = ra;
+  if (has_alt && rb < best) { start = 2; best = rb; }
+  if (!(best < bb)) { start = 0; best = bb; }   // also catches a non-finite guess
+  st->start = start;
+  st->rr0 = best;
+}
+
+// Applies the chosen start: x, r (and the fused first sweep xv = (omega dinv0) r).
+__global__ void __launch_bounds__(kBlock)
+k_pcg_select(int n, const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
+             const float* __restrict__ x_alt, const float* __restrict__ q,
+             const float* __restrict__ dinv0, float* __restrict__ xv, float omega,
+             const PcgState* __restrict__ st) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int start = st->start;
+  float ri;
+  if (start == 0) { x[i] = 0.0f; ri = b[i]; r[i] = ri; }
+  else if (start == 2) { x[i] = x_alt[i]; ri = q[i]; r[i] = ri; }
+  else ri = r[i];
+  if (xv) xv[i] = (omega * dinv0[i]) * ri;
 }
 
 // r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), partial b.b
@@ -1163,6 +1235,17 @@ int mgpbd_predict(mgpbd_ctx* ctx, int n_verts, float* v_p, float* v_v, float* v_
   k_predict<<<nblocks(n_verts), kBlock, 0, ctx->stream>>>(n_verts, v_p, v_v, v_p_old, invm,
                                                           fext, gx, gy, gz, dt, damp, hold_pinned);
   MGPBD_LAUNCH_CHECK("mgpbd_predict");
+  return 0;
+}
+
+int mgpbd_spring_drag(mgpbd_ctx* ctx, int n_verts, const float* v_p, float* f_ext, int vertex,
+                      float tx, float ty, float tz, float k) {
+  CTX_CHECK("mgpbd_spring_drag");
+  if (n_verts <= 0 || !v_p || !f_ext || vertex < 0 || vertex >= n_verts)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_spring_drag: invalid arguments");
+  k_spring_drag<<<nblocks(3LL * n_verts), kBlock, 0, ctx->stream>>>(n_verts, v_p, f_ext, vertex,
+                                                                    tx, ty, tz, k);
+  MGPBD_LAUNCH_CHECK("mgpbd_spring_drag");
   return 0;
 }
 
@@ -1695,11 +1778,10 @@ int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll) {
   return 0;
 }
 
-int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int maxiter,
-                    double tol, int relative, int* iters_out, double* info_out) {
-  CTX_CHECK("mgpbd_pcg_solve");
-  if (int e = check_ready(ctx)) return e;
-  if (!b || !x || !work || maxiter < 0)
+static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_alt, bool guess,
+                     float* work, int maxiter, double tol, int relative, int* iters_out,
+                     double* info_out, int n_info) {
+  if (!b || !x || !work || maxiter < 0 || (x_alt && (!guess || x_alt == x)))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_solve: invalid arguments");
   MgLevel& L = ctx->L[0];
   const int n = L.n;
@@ -1755,8 +1837,22 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
 
   k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
   const bool fuse_first = first && !half_mode(ctx);
-  k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr, ctx->omega, ctx->ws);
-  k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+  if (guess) {
+    // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
+    // start with the smallest residual
+    k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
+                                    b, 0.f, nullptr);
+    if (x_alt)
+      k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x_alt, Ap,
+                                      nullptr, b, 0.f, nullptr);
+    k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, x_alt ? Ap : nullptr, ctx->ws);
+    k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, x_alt != nullptr, ctx->st);
+    k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, x_alt, Ap, L.dinv,
+                                               fuse_first ? xv : nullptr, ctx->omega, ctx->st);
+  } else {
+    k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr, ctx->omega, ctx->ws);
+    k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+  }
   float* zr = pcg_precondition(ctx, r, xv, first);
   k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
   k_pcg_fin<FIN_RZ0><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
@@ -1790,12 +1886,31 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int m
   }
   if (iters_out) *iters_out = ctx->st_host->iters;
   if (info_out) {
-    info_out[0] = ctx->st_host->rr;
-    info_out[1] = ctx->st_host->bb;
-    info_out[2] = (double)ctx->st_host->done;
-    info_out[3] = 0.0;
+    const PcgState& h = *ctx->st_host;
+    const double v[8] = {h.rr, h.bb, (double)h.done, guess ? h.rr0 : h.bb, (double)h.start,
+                         h.rr_guess, h.rr_alt, 0.0};
+    for (int i = 0; i < n_info; ++i) info_out[i] = v[i];
   }
   return 0;
 }
+
+int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, float* work, int maxiter,
+                    double tol, int relative, int* iters_out, double* info_out) {
+  CTX_CHECK("mgpbd_pcg_solve");
+  if (int e = check_ready(ctx)) return e;
+  return pcg_solve(ctx, b, x, nullptr, false, work, maxiter, tol, relative, iters_out,
+                   info_out, 4);
+}
+
+int mgpbd_pcg_solve_guess(mgpbd_ctx* ctx, const float* b, float* x, const float* x_alt,
+                          float* work, int maxiter, double tol, int relative, int* iters_out,
+                          double* info_out) {
+  CTX_CHECK("mgpbd_pcg_solve_guess");
+  if (int e = check_ready(ctx)) return e;
+  return pcg_solve(ctx, b, x, x_alt, true, work, maxiter, tol, relative, iters_out, info_out,
+                   8);
+}
+
+void* mgpbd_internal_stream(mgpbd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 }  // extern "C"

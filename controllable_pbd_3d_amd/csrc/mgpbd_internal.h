@@ -24,3 +24,6 @@ int mgpbd_fail(int code, const std::string& msg);
     if (_e != hipSuccess)                                                            \
       return mgpbd_fail(MGPBD_EHIP, std::string(name) + ": " + hipGetErrorString(_e)); \
   } while (0)
+
+// The stream a context queues its work on (the caller's stream outside a PCG solve).
+extern "C" void* mgpbd_internal_stream(mgpbd_ctx* ctx);

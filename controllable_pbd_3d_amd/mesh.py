@@ -219,3 +219,17 @@ def load_medit_mesh(path: str, scale: float = 1.0, repose=(0.0, 0.0, 0.0),
         invm[pinned] = 0.0
     return TetMesh(verts=verts, tets=tets, invm=invm, tet_mass=vol.astype(np.float32),
                    pinned=pinned, name=path.rsplit("/", 1)[-1])
+
+
+def drag_schedule(mesh: TetMesh, seed: int, n_frames: int, box: float = 0.5):
+    """Spring-drag loading, one (vertex, target) per frame from ``seed``: a uniformly drawn
+    free vertex (``invm > 0``) and a target uniform in the axis-aligned cube of side ``box``
+    (metres) centred on that vertex's rest position."""
+    rng = np.random.default_rng(seed)
+    free = np.flatnonzero(mesh.invm > 0)
+    out = []
+    for _ in range(n_frames):
+        v = int(free[rng.integers(len(free))])
+        t = mesh.verts[v].astype(np.float64) + rng.uniform(-0.5 * box, 0.5 * box, 3)
+        out.append((v, t))
+    return out

@@ -17,7 +17,12 @@ Reference behaviour kept by default (parity mode):
 Keyword-only extensions (no reference counterpart; defaults keep the reference's
 behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('reference' | 'extended'), ``galerkin`` ('values' | 'structural'), ``max_coarse``, ``max_levels``,
-``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'), ``f_ext`` (external vertex forces), ``device``.
+``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'), ``f_ext`` (external vertex forces), ``device``,
+``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
+``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
+substep's lambda, or from the network's prediction with the previous lambda as the fallback;
+the device keeps the start with the smaller residual and iterates to the same tolerance.
+``record_samples`` (a list) collects training pairs from the first Newton iteration.
 """
 
 from __future__ import annotations
@@ -31,7 +36,7 @@ from . import _lib
 from ._lib import check, dptr, np_ptr
 from .mgpcg_gpu import GPUMultigridSolver
 from .runtime import get_context, to_device
-from .sparse_gpu import GPUSparseMatrix, gpu_fill_zero
+from .sparse_gpu import GPUSparseMatrix, gpu_copy, gpu_fill_zero
 from . import tet_aggregation
 
 
@@ -59,7 +64,8 @@ class MGPBDSolverGPU:
                  hierarchy: str = "reference", galerkin: str = "values",
                  max_coarse: int = 400, max_levels: int | None = None,
                  coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
-                 precision: str = "fp32", f_ext=None,
+                 precision: str = "fp32", f_ext=None, warm_start: str = "none",
+                 warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation != "tet":
             raise NotImplementedError("aggregation='pyamg' (smoothed aggregation) is not "
@@ -102,6 +108,15 @@ class MGPBDSolverGPU:
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
+        if warm_start not in ("none", "previous", "learned"):
+            raise ValueError("warm_start must be 'none', 'previous' or 'learned'")
+        if warm_start == "learned" and warm_start_model is None:
+            raise ValueError("warm_start='learned' needs warm_start_model")
+        if record_samples is not None and warm_start_model is None:
+            raise ValueError("record_samples needs warm_start_model (it computes the features)")
+        self.warm_start = warm_start
+        self.warm_start_model = warm_start_model
+        self.record_samples = record_samples
 
         nv, nt = self.n_verts, self.n_tets
         f32 = dict(dtype=torch.float32, device=dev)
@@ -117,10 +132,15 @@ class MGPBDSolverGPU:
         self.gradC = torch.zeros(nt, 4, 3, **f32)
         self._b_gpu = torch.zeros(nt, **f32)
         self._dlam_gpu = torch.zeros(nt, **f32)
+        self._lam_prev = torch.zeros(nt, **f32)
         self._norms = torch.zeros(4, dtype=torch.float64, device=dev)
         self._gpu_solver = None
         self._amg_Ps = None
         self.last_stats = []
+        self.first_solve_info = {}
+        self._drag = None
+        if warm_start_model is not None and warm_start_model.n != nt:
+            raise ValueError("warm_start_model was built for a different tet count")
 
     # ------------------------------------------------------------------ setup
     def init(self):
@@ -201,7 +221,24 @@ class MGPBDSolverGPU:
         self.solve(frame)
         self.update_vel()
 
+    def set_drag(self, vertex, target=None, k: float = 1e3):
+        """Spring drag loading (extension): before every prediction, f_ext is zero except
+        f_ext[vertex] = k (target - x[vertex]); ``vertex=None`` removes it."""
+        if vertex is None:
+            self._drag = None
+            return
+        if not 0 <= int(vertex) < self.n_verts:
+            raise ValueError("drag vertex out of range")
+        if self.f_ext is None:
+            self.f_ext = torch.zeros(self.n_verts, 3, dtype=torch.float32, device=self.device)
+        t = np.asarray(target, np.float64).reshape(3)
+        self._drag = (int(vertex), float(t[0]), float(t[1]), float(t[2]), float(k))
+
     def make_prediction(self):
+        if self._drag is not None:
+            v, tx, ty, tz, k = self._drag
+            self.ctx("mgpbd_spring_drag", self.n_verts, dptr(self.v_p), dptr(self.f_ext), v,
+                     tx, ty, tz, k)
         gx, gy, gz = self.gravity
         self.ctx("mgpbd_predict", self.n_verts, dptr(self.v_p), dptr(self.v_v),
                  dptr(self.v_p_old), dptr(self.v_invm), dptr(self.f_ext), gx, gy, gz,
@@ -279,7 +316,16 @@ class MGPBDSolverGPU:
                     self.benchmark.record_iteration(ite, r)
                 break
             self._fill_A_and_update_gpu()
-            cg_iters = self._gpu_solver.solve(self._b_gpu, self._dlam_gpu)
+            if ite == 0 and self.warm_start != "none":
+                cg_iters = self._solve_warm_started()
+            else:
+                cg_iters = self._gpu_solver.solve(self._b_gpu, self._dlam_gpu)
+            if ite == 0 and self.record_samples is not None:
+                self.record_samples.append(self.warm_start_model.make_sample(
+                    self._b_gpu, self._A.diag_inv, self._lam_prev, self.constraints,
+                    self.alpha_tilde, self.rest_vol, self._dlam_gpu, self._A))
+            if ite == 0:
+                self.first_solve_info = dict(self._gpu_solver.last_info, cg_iters=cg_iters)
             self._scatter_dlam_gpu()
             omega = self._line_search() if self.use_line_search else 1.0
             self._apply_dpos(omega)
@@ -287,5 +333,18 @@ class MGPBDSolverGPU:
             if self.benchmark is not None:
                 self.benchmark.record_iteration(ite, r, cg_iters=cg_iters, omega=omega)
         self.last_stats = stats
+        if self.warm_start != "none" or self.record_samples is not None:
+            gpu_copy(self.lambdaf, self._lam_prev, self.n_tets)
         if self.benchmark is not None:
             self.benchmark.end_frame()
+
+    def _solve_warm_started(self) -> int:
+        """First PCG of the substep from the previous substep's lambda, or from the learned
+        prediction with the previous lambda as the alternative start."""
+        if self.warm_start == "previous":
+            gpu_copy(self._lam_prev, self._dlam_gpu, self.n_tets)
+            return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu)
+        self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
+                                      self.constraints, self.alpha_tilde, self.rest_vol,
+                                      self._dlam_gpu)
+        return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu, self._lam_prev)

@@ -243,3 +243,19 @@ class GPUMultigridSolver:
                  info.ctypes.data_as(_lib.F64P))
         self.last_info = {"rr": float(info[0]), "bb": float(info[1]), "reason": int(info[2])}
         return int(it.value)
+
+    def solve_guess(self, b_field: torch.Tensor, x_field: torch.Tensor,
+                    x_alt: torch.Tensor | None = None) -> int:
+        """PCG from the guess held in ``x_field`` (extension): the device takes whichever of
+        the guess, ``x_alt`` and zero has the smallest residual ||b - A x0||, then iterates to
+        the same stopping test as ``solve``. ``last_info['start']`` is 0 (zero), 1 (guess) or
+        2 (x_alt)."""
+        it = ctypes.c_int(0)
+        info = np.zeros(8, np.float64)
+        self.ctx("mgpbd_pcg_solve_guess", dptr(b_field), dptr(x_field), dptr(x_alt),
+                 dptr(self._pcg_work), self.maxiter_cg, self.tol_cg, int(self.relative),
+                 ctypes.byref(it), info.ctypes.data_as(_lib.F64P))
+        self.last_info = {"rr": float(info[0]), "bb": float(info[1]), "reason": int(info[2]),
+                          "rr0": float(info[3]), "start": int(info[4]),
+                          "rr_guess": float(info[5]), "rr_alt": float(info[6])}
+        return int(it.value)

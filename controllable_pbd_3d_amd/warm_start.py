@@ -1,0 +1,211 @@
+"""Learned warm start for the first PCG of every substep (extension; no reference
+counterpart -- the reference zeroes lambda per substep, cons/mgpbd_gpu.py:402, and starts every
+PCG from dlambda = 0, cons/mgpbd_gpu.py:426, cons/mgpcg_gpu.py:271-272).
+
+A three-layer message-passing network over the tet face-adjacency graph predicts dlambda of the
+first Newton iteration from per-tet features (b / A_ii, the previous substep's lambda, C,
+alpha~, A_ii, rest volume, degree). Inference and training both run on libmgpbd's kernels:
+feature and neighbour-mean gather kernels, BF16 MFMA layers with FP32 accumulation, their
+backward kernels and Adam. ``MGPBDSolverGPU(warm_start='learned', warm_start_model=...)``
+hands the prediction to ``mgpbd_pcg_solve_guess`` together with the previous lambda; the
+device keeps whichever start has the smaller residual, and every start is iterated to the
+same tolerance.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import dptr
+from .runtime import get_context, to_device
+
+HIDDEN, FEATURES, LAYERS = 64, 8, 3
+
+
+def _offsets():
+    o = {"W_in": (0, (FEATURES, HIDDEN)), "b_in": (FEATURES * HIDDEN, (HIDDEN,))}
+    p = FEATURES * HIDDEN + HIDDEN
+    for l in range(LAYERS):
+        o[f"W{l}"] = (p, (2 * HIDDEN, HIDDEN))
+        o[f"b{l}"] = (p + 2 * HIDDEN * HIDDEN, (HIDDEN,))
+        p += 2 * HIDDEN * HIDDEN + HIDDEN
+    o["w_out"] = (p, (HIDDEN,))
+    o["b_out"] = (p + HIDDEN, (1,))
+    return o, p + HIDDEN + 1
+
+
+def initial_parameters(seed: int = 0) -> np.ndarray:
+    """He-normal message-passing and encoder weights, zero biases and a zero decoder: the
+    untrained network predicts dlambda = previous lambda (the plain warm start)."""
+    o, n = _offsets()
+    rng = np.random.default_rng(seed)
+    P = np.zeros(n, np.float32)
+    a, s = o["W_in"]
+    P[a:a + s[0] * s[1]] = rng.normal(0, np.sqrt(2.0 / FEATURES), s).reshape(-1)
+    for l in range(LAYERS):
+        a, s = o[f"W{l}"]
+        P[a:a + s[0] * s[1]] = rng.normal(0, np.sqrt(2.0 / (2 * HIDDEN)), s).reshape(-1)
+    return P
+
+
+def neighbour_table(t_i, n_tets: int) -> np.ndarray:
+    """(n_tets, 4) int32 face neighbours, -1 padded (libmgpbd's face adjacency)."""
+    from .tet_aggregation import face_adjacency_arrays
+    adj, _ = face_adjacency_arrays(np.asarray(t_i, np.int32).reshape(-1), n_tets)
+    return adj.reshape(n_tets, 4)
+
+
+class LearnedWarmStart:
+    """Network state (FP32 parameters, Adam moments, BF16 weight copies) and workspaces for
+    graphs of up to ``n`` tets on one GPU."""
+
+    def __init__(self, nbr, *, params=None, seed: int = 0, device=None, lr: float = 1e-3,
+                 betas=(0.9, 0.999), eps: float = 1e-8):
+        self.ctx = get_context(device)
+        dev = self.device = self.ctx.device
+        lib = self.lib = self.ctx.lib
+        self.n_params = int(lib.mgpbd_gnn_param_count())
+        if self.n_params != _offsets()[1]:
+            raise _lib.MGPBDError("libmgpbd network layout differs from this module's")
+        self.nbr = to_device(nbr, torch.int32, dev, (-1, 4))
+        self.n = int(self.nbr.shape[0])
+        P = initial_parameters(seed) if params is None else np.asarray(params, np.float32)
+        if P.shape != (self.n_params,):
+            raise ValueError(f"params must have {self.n_params} values")
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.params = torch.from_numpy(P.copy()).to(dev)
+        self.grad = torch.zeros(self.n_params, **f32)
+        self.m = torch.zeros(self.n_params, **f32)
+        self.v = torch.zeros(self.n_params, **f32)
+        self.step_count = 0
+        self.lr, self.betas, self.eps = float(lr), tuple(betas), float(eps)
+        nw = 2 * LAYERS * HIDDEN * HIDDEN
+        self.wt = torch.zeros(nw, dtype=torch.int16, device=dev)
+        self.w = torch.zeros(nw, dtype=torch.int16, device=dev)
+        self.ctx("mgpbd_gnn_pack", dptr(self.params), dptr(self.wt), dptr(self.w))
+        ws = int(lib.mgpbd_gnn_workspace_bytes(self.n))
+        self.ws = torch.zeros(ws, dtype=torch.uint8, device=dev)
+        self.x = torch.zeros(self.n, FEATURES, **f32)
+        self.stats = torch.zeros(8, **f32)
+        self.dy = torch.zeros(self.n, **f32)
+        self.dlam = torch.zeros(self.n, **f32)
+        self.tmp = torch.zeros(2 * self.n, **f32)
+        self.loss = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    @classmethod
+    def for_mesh(cls, t_i, n_tets: int, **kw) -> "LearnedWarmStart":
+        return cls(neighbour_table(t_i, n_tets), **kw)
+
+    # ------------------------------------------------------------------ inference
+    def features(self, b, dinv, lam_prev, C, alpha, vol, x=None, stats=None):
+        x = self.x if x is None else x
+        stats = self.stats if stats is None else stats
+        self.ctx("mgpbd_gnn_features", self.n, dptr(b), dptr(dinv), dptr(lam_prev), dptr(C),
+                 dptr(alpha), dptr(vol), dptr(self.nbr), dptr(x), dptr(stats), dptr(self.ws))
+        return x, stats
+
+    def forward(self, x, stats, lam_prev, out=None):
+        self.ctx("mgpbd_gnn_forward", self.n, dptr(self.nbr), dptr(x), dptr(stats),
+                 dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
+        return out
+
+    def predict(self, b, dinv, lam_prev, C, alpha, vol, out):
+        """out = predicted dlambda of the first Newton iteration (device tensors, n floats)."""
+        x, st = self.features(b, dinv, lam_prev, C, alpha, vol)
+        return self.forward(x, st, lam_prev, out)
+
+    # ------------------------------------------------------------------ training
+    def make_sample(self, b, dinv, lam_prev, C, alpha, vol, target, matrix=None) -> dict:
+        """One training pair from a solved first Newton iteration: features, scale, previous
+        lambda, converged dlambda and (for the residual loss) the right-hand side and a copy
+        of the level-0 values of A (``matrix``: the GPUSparseMatrix of that solve)."""
+        x = torch.empty(self.n, FEATURES, dtype=torch.float32, device=self.device)
+        st = torch.empty(8, dtype=torch.float32, device=self.device)
+        self.features(b, dinv, lam_prev, C, alpha, vol, x, st)
+        s = {"x": x, "stats": st, "lam_prev": lam_prev.clone(), "target": target.clone()}
+        if matrix is not None:
+            s.update(b=b.clone(), val=matrix.data.clone(), slice_ptr=matrix.slice_ptr,
+                     sell_idx=matrix.sell_idx)
+        return s
+
+    def loss_and_grad(self, sample: dict, loss: str = "residual"):
+        """Forward, loss and backward for one sample; the gradient lands in ``self.grad`` and
+        the loss in ``self.loss`` (device)."""
+        self.forward(sample["x"], sample["stats"], sample["lam_prev"], self.dlam)
+        if loss == "mse":
+            self.ctx("mgpbd_gnn_loss_mse", self.n, dptr(sample["stats"]),
+                     dptr(sample["lam_prev"]), dptr(sample["target"]), dptr(self.ws),
+                     dptr(self.dy), dptr(self.loss))
+        elif loss == "residual":
+            self.ctx("mgpbd_gnn_loss_residual", self.n, dptr(sample["slice_ptr"]),
+                     dptr(sample["sell_idx"]), dptr(sample["val"]), dptr(sample["b"]),
+                     dptr(self.dlam), dptr(sample["stats"]), dptr(self.tmp), dptr(self.ws),
+                     dptr(self.dy), dptr(self.loss))
+        else:
+            raise ValueError("loss must be 'residual' or 'mse'")
+        self.ctx("mgpbd_gnn_backward", self.n, dptr(self.nbr), dptr(sample["x"]),
+                 dptr(self.params), dptr(self.w), dptr(self.dy), dptr(self.ws), dptr(self.grad))
+        return self.loss
+
+    def adam_step(self):
+        self.step_count += 1
+        b1, b2 = self.betas
+        self.ctx("mgpbd_gnn_adam", dptr(self.params), dptr(self.grad), dptr(self.m),
+                 dptr(self.v), self.lr, b1, b2, self.eps, self.step_count, dptr(self.wt),
+                 dptr(self.w))
+
+    def train(self, samples, epochs: int = 1, loss: str = "residual", seed: int = 0,
+              lr_decay: float = 1.0):
+        """Adam over the samples, one step per sample, shuffled per epoch; returns the mean
+        loss of every epoch (one host sync per epoch)."""
+        rng = np.random.default_rng(seed)
+        hist = []
+        acc = torch.zeros(1, dtype=torch.float64, device=self.device)
+        for _ in range(epochs):
+            acc.zero_()
+            for i in rng.permutation(len(samples)):
+                acc += self.loss_and_grad(samples[i], loss)
+                self.adam_step()
+            hist.append(float(acc.item()) / max(len(samples), 1))
+            self.lr *= lr_decay
+        return hist
+
+    def evaluate(self, samples, loss: str = "residual"):
+        """Mean loss over samples without updating the parameters."""
+        tot = 0.0
+        for s in samples:
+            self.forward(s["x"], s["stats"], s["lam_prev"], self.dlam)
+            if loss == "mse":
+                self.ctx("mgpbd_gnn_loss_mse", self.n, dptr(s["stats"]), dptr(s["lam_prev"]),
+                         dptr(s["target"]), dptr(self.ws), dptr(self.dy), dptr(self.loss))
+            else:
+                self.ctx("mgpbd_gnn_loss_residual", self.n, dptr(s["slice_ptr"]),
+                         dptr(s["sell_idx"]), dptr(s["val"]), dptr(s["b"]), dptr(self.dlam),
+                         dptr(s["stats"]), dptr(self.tmp), dptr(self.ws), dptr(self.dy),
+                         dptr(self.loss))
+            tot += float(self.loss.item())
+        return tot / max(len(samples), 1)
+
+    # ------------------------------------------------------------------ state
+    def state_dict(self) -> dict:
+        return {"params": self.params.cpu().numpy().copy()}
+
+    def load_state_dict(self, d: dict) -> None:
+        P = np.asarray(d["params"], np.float32)
+        if P.shape != (self.n_params,):
+            raise ValueError(f"params must have {self.n_params} values")
+        self.params.copy_(torch.from_numpy(P))
+        self.ctx("mgpbd_gnn_pack", dptr(self.params), dptr(self.wt), dptr(self.w))
+
+    def save(self, path: str) -> None:
+        np.savez(path, params=self.params.cpu().numpy(), hidden=HIDDEN, layers=LAYERS,
+                 features=FEATURES)
+
+    @staticmethod
+    def load_parameters(path: str) -> np.ndarray:
+        with np.load(path, allow_pickle=False) as z:
+            if int(z["hidden"]) != HIDDEN or int(z["layers"]) != LAYERS:
+                raise ValueError("checkpoint network shape differs")
+            return z["params"].astype(np.float32)

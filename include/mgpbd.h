@@ -285,6 +285,90 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* wor
  * Iterations past convergence are skipped on the device by the done flag. */
 int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll);
 
+/* ---- Warm start (extension; no reference counterpart: the reference zeroes lambda every
+ * substep, cons/mgpbd_gpu.py:402, and starts every PCG from dlambda = 0,
+ * cons/mgpbd_gpu.py:426 and cons/mgpcg_gpu.py:271-272) ---- */
+/* PCG for A x = b from a guess: x_dev holds the guess on entry, x_alt_dev (or NULL) a second
+ * one (it must not alias x_dev). The start with the smallest ||b - A x0|| among the guess,
+ * the alternative and zero is taken on the device (the guess wins a tie with the
+ * alternative, zero wins ties and non-finite residuals); from x0 = 0 the arithmetic is that of
+ * mgpbd_pcg_solve. The stopping test is that of mgpbd_pcg_solve (||r|| < tol, or
+ * ||r|| < tol ||b|| with relative != 0), so every start reaches the same tolerance.
+ * info_out (host, 8 doubles) = {||r||^2, ||b||^2, stop reason, ||r0||^2 of the start taken,
+ * start (0 zero, 1 guess, 2 alternative), ||b - A x_guess||^2, ||b - A x_alt||^2, 0}. */
+int mgpbd_pcg_solve_guess(mgpbd_ctx* ctx, const float* b_dev, float* x_dev,
+                          const float* x_alt_dev, float* work_dev, int maxiter, double tol,
+                          int relative, int* iters_out, double* info_out);
+
+/* Spring drag loading (extension): f_ext = 0 except f_ext[vertex] = k (target - x[vertex]).
+ * Used with mgpbd_predict's f_ext. */
+int mgpbd_spring_drag(mgpbd_ctx* ctx, int n_verts, const float* v_p_dev, float* f_ext_dev,
+                      int vertex, float tx, float ty, float tz, float k);
+
+/* ---- Learned warm start (extension): a message-passing network over the tet face-adjacency
+ * graph (nbr_dev: 4 neighbours per tet, -1 padded, as from mgpbd_face_adjacency) predicting
+ * dlambda of the first Newton iteration.
+ *   h0 = relu(x W_in + b_in); m_l = mean over face neighbours of h_{l-1};
+ *   h_l = relu(h_{l-1} W_l[0:64] + m_l W_l[64:128] + b_l), l = 1..3; y = h3 w_out + b_out;
+ *   dlambda = lam_prev + s y  (s = stats[0], the feature scale).
+ * Parameters: one FP32 vector of mgpbd_gnn_param_count() values in the order W_in[8][64],
+ * b_in[64], {W_l[128][64], b_l[64]} x 3, w_out[64], b_out. The three 128 -> 64 layers run on
+ * BF16 MFMA with FP32 accumulation from BF16 weight copies: wt_dev (layer, half, out, in) for
+ * the forward pass and w_dev (layer, half, in, out) for the backward pass, 6*64*64 uint16
+ * each, written by mgpbd_gnn_pack / mgpbd_gnn_adam. Activations are kept as BF16.
+ * ws_dev: mgpbd_gnn_workspace_bytes(n) bytes; mgpbd_gnn_backward reads the activations the
+ * preceding mgpbd_gnn_forward left there (same ws_dev, same inputs). */
+#define MGPBD_GNN_HIDDEN 64
+#define MGPBD_GNN_FEATURES 8
+#define MGPBD_GNN_LAYERS 3
+int mgpbd_gnn_param_count(void);
+int64_t mgpbd_gnn_workspace_bytes(int n);
+/* Per-tet features x_dev (n x 8): b/A_ii, lam_prev, C, log alpha, log A_ii, rest volume,
+ * degree, 1 -- normalised by FP64 statistics written to stats_dev (8 floats: the scale s of
+ * b/A_ii and lam_prev, the scale of C, mean/std of log alpha, mean/std of log A_ii, mean
+ * volume). lam_prev_dev may be NULL (zero). dinv_dev = 1/A_ii (as cached for the smoother). */
+int mgpbd_gnn_features(mgpbd_ctx* ctx, int n, const float* b_dev, const float* dinv_dev,
+                       const float* lam_prev_dev, const float* C_dev, const float* alpha_dev,
+                       const float* vol_dev, const int32_t* nbr_dev, float* x_dev,
+                       float* stats_dev, void* ws_dev);
+/* Forward pass; dlam_out_dev (n, may be NULL) = lam_prev + s y. */
+int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* x_dev,
+                      const float* stats_dev, const float* lam_prev_dev,
+                      const float* params_dev, const uint16_t* wt_dev, void* ws_dev,
+                      float* dlam_out_dev);
+/* Losses of the forward pass left in ws_dev; each writes dy_dev (n) = d loss / d y and, when
+ * loss_dev (device double) is given, the loss.
+ *   mse:      mean_i (y_i - (target_i - lam_prev_i) / s)^2
+ *   residual: ||A dlam - b||^2 / ||b||^2 with dlam = the forward's dlam_out, A the level-0
+ *             SELL-64x4 matrix of the same solve (tmp_dev: 2 n floats). This is the quantity
+ *             mgpbd_pcg_solve_guess compares between starts. */
+int mgpbd_gnn_loss_mse(mgpbd_ctx* ctx, int n, const float* stats_dev, const float* lam_prev_dev,
+                       const float* target_dev, void* ws_dev, float* dy_dev, double* loss_dev);
+int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr_dev,
+                            const int32_t* sell_idx_dev, const float* sell_val_dev,
+                            const float* b_dev, const float* dlam_dev, const float* stats_dev,
+                            float* tmp_dev, void* ws_dev, float* dy_dev, double* loss_dev);
+/* Backward pass from dy_dev = d loss / d y: grad_dev gets d loss / d params (FP32,
+ * param_count values, overwritten). */
+int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* x_dev,
+                       const float* params_dev, const uint16_t* w_dev, const float* dy_dev,
+                       void* ws_dev, float* grad_dev);
+/* One Adam step (bias-corrected, step >= 1) on the FP32 parameters; refreshes the BF16 copies
+ * when wt_dev and w_dev are given. */
+int mgpbd_gnn_adam(mgpbd_ctx* ctx, float* params_dev, const float* grad_dev, float* m_dev,
+                   float* v_dev, float lr, float beta1, float beta2, float eps, int step,
+                   uint16_t* wt_dev, uint16_t* w_dev);
+int mgpbd_gnn_pack(mgpbd_ctx* ctx, const float* params_dev, uint16_t* wt_dev, uint16_t* w_dev);
+/* The MFMA building blocks on their own: Y (n x 64 FP32) = act(A1 W1 + A2 W2 + bias) with A*
+ * BF16 n x 64 row-major and W*t BF16 [64 out][64 in] (A2/W2t/bias may be NULL, act = relu
+ * when relu != 0); and G (64 x 64 FP32) = A^T dZ over n rows (partial_dev:
+ * 4096 * ceil(n / 1024) floats). */
+int mgpbd_gnn_gemm64(mgpbd_ctx* ctx, int n, const uint16_t* A1_dev, const uint16_t* W1t_dev,
+                     const uint16_t* A2_dev, const uint16_t* W2t_dev, const float* bias_dev,
+                     int relu, float* Y_dev);
+int mgpbd_gnn_wgrad64(mgpbd_ctx* ctx, int n, const uint16_t* A_dev, const uint16_t* dZ_dev,
+                      float* partial_dev, float* G_dev);
+
 #ifdef __cplusplus
 }
 #endif

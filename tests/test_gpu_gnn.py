@@ -1,0 +1,305 @@
+"""GPU parity of the learned warm start and the guessed-start PCG (through the C ABI).
+
+Bars (written in each test):
+* MFMA tiles (BF16 operands, FP32 accumulation) against the FP64 product of the same BF16
+  values: |err| <= 1e-5 (gemm) / 1e-4 (weight gradient) of sum |a_k b_k| -- a wrong lane
+  layout or a transposed operand is O(1) off;
+* features against the oracle from the same FP32 inputs: 4 ulp-level (rtol 4e-6);
+* forward output and every parameter-gradient group against the BF16-emulating FP64 oracle:
+  relative L2 error <= 2e-3 (only accumulation order and rare BF16 rounding flips differ);
+* Adam: rtol 1e-6 against the oracle's update;
+* guessed-start PCG: bit-identical to the zero-start PCG whenever zero is the start taken,
+  same FP64-verified tolerance from every start.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from oracle import gnn_oracle as go             # noqa: E402
+from tests.scenarios import deformed, physical_block, solver_kwargs   # noqa: E402
+
+
+def _ctx():
+    from controllable_pbd_3d_amd.runtime import get_context
+    return get_context(0)
+
+
+def _bf16_bits(a):
+    f = np.ascontiguousarray(np.asarray(a, np.float32)).view(np.uint32).astype(np.uint64)
+    u = ((f + 0x7FFF + ((f >> 16) & 1)) >> 16).astype(np.uint16)
+    return torch.from_numpy(u.view(np.int16).copy()).cuda()
+
+
+def _dptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("n,two,relu", [(100, True, True), (1000, False, False), (4133, True, False)])
+def test_mfma_gemm_matches_fp64(n, two, relu):
+    rng = np.random.default_rng(n)
+    A1 = go.bf16(rng.normal(size=(n, 64)))
+    A2 = go.bf16(rng.normal(size=(n, 64)))
+    W1t = go.bf16(rng.normal(size=(64, 64)))
+    W2t = go.bf16(rng.normal(size=(64, 64)))
+    bias = rng.normal(size=64).astype(np.float32)
+    ref = A1 @ W1t.T + (A2 @ W2t.T if two else 0) + bias
+    mag = np.abs(A1) @ np.abs(W1t).T + (np.abs(A2) @ np.abs(W2t).T if two else 0) + np.abs(bias)
+    if relu:
+        ref = np.maximum(ref, 0)
+    Y = torch.zeros(n, 64, dtype=torch.float32, device="cuda")
+    # device copies stay referenced until the launch has completed
+    dA1, dW1t, dA2, dW2t = (_bf16_bits(a) for a in (A1, W1t, A2, W2t))
+    dbias = torch.from_numpy(bias).cuda()
+    ctx = _ctx()
+    ctx("mgpbd_gnn_gemm64", n, _dptr(dA1), _dptr(dW1t), _dptr(dA2) if two else None,
+        _dptr(dW2t) if two else None, _dptr(dbias), int(relu), _dptr(Y))
+    got = Y.cpu().numpy().astype(np.float64)
+    err = np.abs(got - ref) / mag
+    print(f"[gemm64] n={n} two={two} relu={relu} max err/mag {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-5
+
+
+@pytest.mark.parametrize("n", [1000, 2500, 8])
+def test_mfma_weight_gradient_matches_fp64(n):
+    rng = np.random.default_rng(7 + n)
+    A = go.bf16(rng.normal(size=(n, 64)))
+    dZ = go.bf16(rng.normal(size=(n, 64)))
+    ref = A.T @ dZ
+    mag = np.abs(A).T @ np.abs(dZ)
+    nblk = (n + 1023) // 1024
+    part = torch.zeros(nblk * 4096, dtype=torch.float32, device="cuda")
+    G = torch.zeros(64, 64, dtype=torch.float32, device="cuda")
+    dA, ddZ = _bf16_bits(A), _bf16_bits(dZ)
+    _ctx()("mgpbd_gnn_wgrad64", n, _dptr(dA), _dptr(ddZ), _dptr(part), _dptr(G))
+    err = np.abs(G.cpu().numpy() - ref) / mag
+    print(f"[wgrad64] n={n} max err/mag {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def solved_scene():
+    """A small strained block after one first-Newton-iteration solve: real b, 1/A_ii, C,
+    alpha~, volumes and the converged dlambda."""
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    from controllable_pbd_3d_amd.warm_start import LearnedWarmStart
+    mesh, mu_t = physical_block(5, seed=4)
+    v0 = deformed(mesh, seed=4, amp=0.03, twist=0.08)
+    kw = solver_kwargs(mesh, mu_per_tet=mu_t, relative_tol=True, tol_cg=1e-6)
+    kw["v_p"] = v0
+    model = LearnedWarmStart.for_mesh(mesh.tets, mesh.n_tets, seed=3)
+    rec = []
+    s = MGPBDSolverGPU(**kw, warm_start_model=model, record_samples=rec, verbose=False)
+    s.init()
+    s.step(0)
+    s.step(0)                      # the second substep has a nonzero previous lambda
+    print(f"[gnn scene] box5 n_tets={mesh.n_tets} samples={len(rec)}", flush=True)
+    return mesh, s, model, rec
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+def _trained_like_params(model, seed=11):
+    """Random parameters with a nonzero decoder (the initial decoder is zero)."""
+    rng = np.random.default_rng(seed)
+    d = go.unpack(_host(model.params).astype(np.float64))
+    d["w_out"] = rng.normal(0, 0.2, 64)
+    d["b_out"] = np.array([0.05])
+    for l in range(3):
+        d[f"b{l}"] = rng.normal(0, 0.05, 64)
+    d["b_in"] = rng.normal(0, 0.05, 64)
+    return go.pack(d).astype(np.float32)
+
+
+def _nbr(model):
+    return _host(model.nbr)
+
+
+def test_features_match_oracle(solved_scene):
+    mesh, s, model, rec = solved_scene
+    smp = rec[-1]
+    args = [_host(t) for t in (s._b_gpu, s._A.diag_inv, smp["lam_prev"], s.constraints,
+                               s.alpha_tilde, s.rest_vol)]
+    x_ref, st_ref = go.features(*args, _nbr(model))
+    x, st = model.features(s._b_gpu, s._A.diag_inv, smp["lam_prev"], s.constraints,
+                           s.alpha_tilde, s.rest_vol)
+    np.testing.assert_allclose(_host(st), st_ref, rtol=1e-6)
+    np.testing.assert_allclose(_host(x), x_ref, rtol=4e-6, atol=4e-6)
+    assert np.any(_host(smp["lam_prev"]) != 0)
+
+
+@pytest.mark.parametrize("loss", ["mse", "residual"])
+def test_forward_and_backward_match_oracle(solved_scene, loss):
+    mesh, s, model, rec = solved_scene
+    smp = rec[-1]
+    P = _trained_like_params(model)
+    model.load_state_dict({"params": P})
+    model.loss_and_grad(smp, loss)
+    g_gpu = _host(model.grad).astype(np.float64)
+    y_gpu_dlam = _host(model.dlam)
+    L_gpu = float(model.loss.item())
+
+    nbr = _nbr(model)
+    x, st, lp = _host(smp["x"]), _host(smp["stats"]), _host(smp["lam_prev"])
+    y, dlam, cache = go.forward(P, x, nbr, st, lp, emulate=True)
+    if loss == "mse":
+        L, dy = go.loss_mse(y, _host(smp["target"]), lp, st)
+    else:
+        csr = _host(smp["val"][s._A.sell_pos])        # A of the recorded solve, CSR order
+        A = sp.csr_matrix((csr.astype(np.float64), s._A_indices, s._A_indptr),
+                          shape=(mesh.n_tets, mesh.n_tets))
+        L, dy = go.loss_residual(A, _host(smp["b"]), dlam, st)
+    g = go.backward(P, cache, dy, nbr, emulate=True)
+    e_dlam = _rel(y_gpu_dlam, dlam)
+    print(f"[gnn {loss}] loss gpu {L_gpu:.6e} oracle {L:.6e} dlam rel {e_dlam:.2e}", flush=True)
+    assert e_dlam <= 2e-3
+    assert abs(L_gpu - L) <= 2e-3 * abs(L)
+    o, _ = go.offsets()
+    for k, (a, shp) in o.items():
+        m = int(np.prod(shp))
+        e = _rel(g_gpu[a:a + m], g[a:a + m])
+        print(f"    grad {k:6s} rel L2 {e:.2e}", flush=True)
+        assert e <= 2e-3, k
+
+
+def test_adam_and_weight_copies_match_oracle(solved_scene):
+    mesh, s, model, rec = solved_scene
+    rng = np.random.default_rng(5)
+    P = _trained_like_params(model, 12)
+    model.load_state_dict({"params": P})
+    model.m.zero_(); model.v.zero_(); model.step_count = 0
+    G = rng.normal(size=P.shape).astype(np.float32)
+    model.grad.copy_(torch.from_numpy(G))
+    Pr, m, v = P.astype(np.float64), np.zeros(P.shape), np.zeros(P.shape)
+    for step in (1, 2):
+        model.adam_step()
+        Pr, m, v = go.adam(Pr, G.astype(np.float64), m, v, step, lr=model.lr,
+                           beta1=model.betas[0], beta2=model.betas[1], eps=model.eps)
+    np.testing.assert_allclose(_host(model.params), Pr, rtol=1e-6, atol=1e-7)
+    # BF16 copies: wt[l][half][out][in] and w[l][half][in][out] of the updated weights
+    d = go.unpack(_host(model.params).astype(np.float64))
+    wt = _host(model.wt).view(np.uint16).reshape(3, 2, 64, 64)
+    w = _host(model.w).view(np.uint16).reshape(3, 2, 64, 64)
+    for l in range(3):
+        W = d[f"W{l}"]
+        for half in range(2):
+            Wh = W[64 * half:64 * (half + 1)]
+            ref = _host(_bf16_bits(Wh)).view(np.uint16)
+            assert np.array_equal(w[l, half], ref)
+            assert np.array_equal(wt[l, half], ref.T)
+
+
+def test_training_lowers_the_loss(solved_scene):
+    mesh, s, model, rec = solved_scene
+    model.load_state_dict({"params": go.init_params(0)})
+    model.m.zero_(); model.v.zero_(); model.step_count = 0
+    samples = [r for r in rec if "b" in r]
+    before = model.evaluate(samples)
+    model.lr = 3e-3
+    model.train(samples * 100, epochs=1)
+    after = model.evaluate(samples)
+    print(f"[gnn train] residual loss {before:.4e} -> {after:.4e}", flush=True)
+    assert after < 0.5 * before
+
+
+# ------------------------------------------------------------------------ guessed-start PCG
+@pytest.fixture(scope="module")
+def pcg_scene():
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    mesh, mu_t = physical_block(6, seed=8)
+    v0 = deformed(mesh, seed=8, amp=0.05, twist=0.1)
+    kw = solver_kwargs(mesh, mu_per_tet=mu_t, relative_tol=True, tol_cg=1e-6, maxiter_cg=300)
+    kw["v_p"] = v0
+    s = MGPBDSolverGPU(**kw, hierarchy="extended", galerkin="structural", max_coarse=64,
+                       coarse_solver="cholesky", verbose=False)
+    s.init()
+    s._compute_C_and_gradC()
+    s._fill_A_and_update_gpu()
+    csr = s._A.csr_data()
+    A = sp.csr_matrix((csr.astype(np.float64), s._A_indices, s._A_indptr),
+                      shape=(mesh.n_tets, mesh.n_tets))
+    b = _host(s._b_gpu).astype(np.float64)
+    import scipy.sparse.linalg as spla
+    x_star = spla.spsolve(A.tocsc(), b)
+    print(f"[pcg guess scene] box6 n_tets={mesh.n_tets}", flush=True)
+    return s, A, b, x_star
+
+
+def _solve(s, x0=None, alt=None):
+    x = torch.zeros(s.n_tets, dtype=torch.float32, device="cuda")
+    if x0 is None:
+        it = s._gpu_solver.solve(s._b_gpu, x)
+        info = dict(s._gpu_solver.last_info, start=0)
+    else:
+        x.copy_(torch.from_numpy(np.asarray(x0, np.float32)))
+        a = None if alt is None else torch.from_numpy(np.asarray(alt, np.float32)).cuda()
+        it = s._gpu_solver.solve_guess(s._b_gpu, x, a)
+        info = s._gpu_solver.last_info
+    return it, _host(x), info
+
+
+def test_guess_zero_is_the_zero_start(pcg_scene):
+    s, A, b, x_star = pcg_scene
+    it0, x0, _ = _solve(s)
+    it1, x1, info = _solve(s, np.zeros(len(b)))
+    assert info["start"] == 0 and it1 == it0
+    assert np.array_equal(x0, x1)
+
+
+def test_good_guess_is_taken_and_reaches_the_same_tolerance(pcg_scene):
+    s, A, b, x_star = pcg_scene
+    rng = np.random.default_rng(1)
+    it0, x0, _ = _solve(s)
+    guess = x_star * (1 + 1e-3 * rng.normal(size=len(b)))
+    it1, x1, info = _solve(s, guess)
+    r0 = np.linalg.norm(b - A @ x0) / np.linalg.norm(b)
+    r1 = np.linalg.norm(b - A @ x1) / np.linalg.norm(b)
+    print(f"[pcg guess] zero start {it0} it r={r0:.2e}; guess {it1} it r={r1:.2e} "
+          f"rr_guess/bb={info['rr_guess'] / info['bb']:.2e}", flush=True)
+    assert info["start"] == 1
+    assert it1 < it0
+    # same stopping test: FP32 recurrence residual < 1e-6 ||b||; the FP64 residual of the
+    # returned x agrees with it to within the FP32 drift of the recurrence
+    assert info["rr"] < 1e-12 * info["bb"]
+    assert r1 < 1e-5 and r0 < 1e-5
+
+
+def test_bad_guess_falls_back_to_alternative_or_zero(pcg_scene):
+    s, A, b, x_star = pcg_scene
+    rng = np.random.default_rng(2)
+    it0, x0, _ = _solve(s)
+    bad = 1e3 * np.abs(x_star).max() * rng.normal(size=len(b))
+    it1, x1, info = _solve(s, bad)
+    assert info["start"] == 0 and it1 == it0 and np.array_equal(x1, x0)
+    nan = bad.copy()
+    nan[3] = np.nan
+    it2, x2, info = _solve(s, nan)
+    assert info["start"] == 0 and np.array_equal(x2, x0)
+    good = x_star * (1 + 1e-3 * rng.normal(size=len(b)))
+    it3, x3, info = _solve(s, bad, alt=good)
+    assert info["start"] == 2 and it3 < it0
+    assert info["rr_guess"] > info["rr_alt"]
+
+
+def test_spring_drag_force():
+    from controllable_pbd_3d_amd.runtime import get_context
+    ctx = get_context(0)
+    x = torch.randn(50, 3, device="cuda")
+    f = torch.full((50, 3), 7.0, device="cuda")
+    ctx("mgpbd_spring_drag", 50, x.data_ptr(), f.data_ptr(), 17, 1.0, -2.0, 0.5, 1e3)
+    ref = torch.zeros(50, 3)
+    ref[17] = 1e3 * (torch.tensor([1.0, -2.0, 0.5]) - x[17].cpu())
+    torch.testing.assert_close(f.cpu(), ref, rtol=1e-6, atol=1e-4)

@@ -400,15 +400,19 @@ def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision):
 
 
 def test_mixed_precision_substeps_match_fp32():
-    """Ten substeps with the FP16 V-cycle stay within 1e-4 h of the FP32 V-cycle run. Both
-    solve every linear system to the same 1e-6 relative PCG tolerance and run Newton to
-    rtol 1e-6, so neither run can stop a Newton loop one iteration earlier than the other
-    (with the default rtol 1e-2 that decision flips on 1e-6-level differences)."""
+    """Ten substeps with the FP16 V-cycle stay within 1e-4 h of the FP32 V-cycle run when the
+    Newton path has no discrete choices: both solve every linear system to the same 1e-6
+    relative PCG tolerance, take full steps (no line search) and run a fixed five Newton
+    iterations. With the line search or an early exit the two runs can take different
+    branches on 1e-6-level differences, and the MGPBD iteration is path-dependent (lambda takes
+    the full dlambda while x moves by omega dx, and the primal residual is not driven to zero),
+    so branches settle on different states; per-solve accuracy of the mixed V-cycle is pinned
+    by the FP64-verified PCG test above."""
     from tests.scenarios import physical_block
     mesh, mu_t = physical_block(6, seed=9)
     kw = dict(mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended", galerkin="structural",
               relative_tol=True, tol_cg=1e-6, max_coarse=64, coarse_solver="cholesky",
-              rtol=1e-6, atol=1e-9, maxiter=30)
+              rtol=0.0, atol=0.0, maxiter=5, use_line_search=False)
     a = _solver(mesh, None, **kw)
     b = _solver(mesh, None, precision="fp16", **kw)
     worst = 0.0
