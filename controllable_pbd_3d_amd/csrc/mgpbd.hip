@@ -455,8 +455,9 @@ template <int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_sell(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
        const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y,
-       const float* __restrict__ dinv, const float* __restrict__ b, float omega,
+       const float* __restrict__ dinv, const float* __restrict__ b, float omega, const float* __restrict__ wp,
        const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
@@ -467,12 +468,84 @@ k_sell(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __rest
   else y[i] = b[i] - ax;
 }
 
+// Power iteration on D^-1 A (symmetric in the D inner product): w = D^-1 A v and the
+// partials v.Av (slot 0), v.Dv (slot 1), w.Dw = (Av)^2 / d (slot 2).
+__global__ void __launch_bounds__(kBlock)
+k_power_step(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+             const float* __restrict__ val, const float* __restrict__ dinv,
+             const float* __restrict__ v, float* __restrict__ w, RedWS ws) {
+  double vav = 0.0, vdv = 0.0, wdw = 0.0;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const float ax = sell_row(i, slice_ptr, idx, val, v);
+    if (i < n) {
+      const float di = dinv[i];
+      w[i] = di * ax;
+      vav = (double)v[i] * ax;
+      if (di > 0.0f) {
+        vdv = (double)v[i] * v[i] / di;
+        wdw = (double)ax * ax * di;
+      }
+    }
+  }
+  const double s0 = block_sum(vav);
+  const double s1 = block_sum(vdv);
+  const double s2 = block_sum(wdw);
+  store_partial(ws, 0, s0);
+  store_partial(ws, 1, s1);
+  store_partial(ws, 2, s2);
+}
+
+// pw[0] = lambda = v.Av / v.Dv, pw[1] = ||w||_D
+__global__ void __launch_bounds__(kBlock)
+k_power_fin(RedWS ws, int nparts, double* __restrict__ pw) {
+  const double vav = sum_partials(ws, 0, nparts);
+  const double vdv = sum_partials(ws, 1, nparts);
+  const double wdw = sum_partials(ws, 2, nparts);
+  if (threadIdx.x != 0) return;
+  pw[0] = vdv > 0.0 ? vav / vdv : 0.0;
+  pw[1] = sqrt(wdw);
+}
+
+// v = w / ||w||_D (v unchanged when the norm is 0 or not finite)
+__global__ void __launch_bounds__(kBlock)
+k_power_norm(int n, const float* __restrict__ w, float* __restrict__ v,
+             const double* __restrict__ pw) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const double nrm = pw[1];
+  if (i >= n || !(nrm > 0.0) || !(nrm < 1e300)) return;
+  v[i] = (float)(w[i] / nrm);
+}
+
+// deterministic start vector with entries of both signs
+__global__ void __launch_bounds__(kBlock)
+k_power_init(int n, float* __restrict__ v) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  uint32_t h = (uint32_t)i * 2654435761u;
+  h ^= h >> 15; h *= 2246822519u; h ^= h >> 13;
+  v[i] = 0.5f + (float)(h & 0xFFFFu) * (1.0f / 65536.0f) - ((h >> 16) & 1u ? 1.0f : 0.0f);
+}
+
+// omega_l = target / (safety lambda), or the fixed weight when lambda is not usable
+__global__ void k_power_omega(const double* __restrict__ pw, float target, float safety,
+                              float fallback, float* __restrict__ omega_l) {
+  if (threadIdx.x != 0) return;
+  const double lam = pw[0];
+  omega_l[0] = (lam > 1e-6 && lam < 1e30) ? (float)(target / (safety * lam)) : fallback;
+}
+
+__global__ void k_set_float(float* __restrict__ x, float v) {
+  if (threadIdx.x == 0) x[0] = v;
+}
+
 // Several Jacobi sweeps on a diagonal-only matrix in one pass (rows are independent, so this
 // is the same arithmetic as `sweeps` separate launches).
 __global__ void __launch_bounds__(kBlock)
 k_jacobi_diag(int n, const int* __restrict__ slice_ptr, const float* __restrict__ val,
               const float* __restrict__ dinv, const float* __restrict__ b,
-              float* __restrict__ x, float omega, int sweeps, const int* __restrict__ done) {
+              float* __restrict__ x, float omega, const float* __restrict__ wp, int sweeps, const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -493,8 +566,9 @@ constexpr int kSmallBlock = 1024;
 __global__ void __launch_bounds__(kSmallBlock)
 k_jacobi_small(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
                const float* __restrict__ val, const float* __restrict__ dinv,
-               const float* __restrict__ b, float* __restrict__ x, float omega, int sweeps,
+               const float* __restrict__ b, float* __restrict__ x, float omega, const float* __restrict__ wp, int sweeps,
                const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   __shared__ float xs[2][kSmallRows];
   const int n_slices = (n + 63) >> 6;
@@ -590,7 +664,8 @@ k_refill(int nc, const int* __restrict__ off, const int* __restrict__ fpos,
 __global__ void __launch_bounds__(kBlock)
 k_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
            const float* __restrict__ r, float* __restrict__ bc, const float* __restrict__ dinv_c,
-           float* __restrict__ xc, float omega, const int* __restrict__ done) {
+           float* __restrict__ xc, float omega, const float* __restrict__ wp, const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   int I = blockIdx.x * kBlock + threadIdx.x;
   if (I >= nc) return;
@@ -602,16 +677,40 @@ k_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
 
 __global__ void __launch_bounds__(kBlock)
 k_prolong(int n, const int* __restrict__ agg, const float* __restrict__ xc,
-          float* __restrict__ x, const int* __restrict__ done) {
+          float* __restrict__ x, float g, const int* __restrict__ done) {
   if (done && *done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) x[i] = x[i] + xc[agg[i]];
+  if (i < n) x[i] = fmaf(g, xc[agg[i]], x[i]);   // g = 1: x + xc, rounded once
+}
+
+// Second coarse cycle of a W-cycle: y = x (save), e = e + s (combine), with the done flag.
+__global__ void __launch_bounds__(kBlock)
+k_copy_d(int n, const float* __restrict__ x, float* __restrict__ y, const int* __restrict__ done) {
+  if (done && *done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+__global__ void __launch_bounds__(kBlock)
+k_add_d(int n, const float* __restrict__ s, float* __restrict__ e, const int* __restrict__ done) {
+  if (done && *done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) e[i] = e[i] + s[i];
+}
+// x = omega b: the first sweep from zero on a unit-diagonal (scaled) level
+__global__ void __launch_bounds__(kBlock)
+k_scale_d(int n, float omega, const float* __restrict__ wp, const float* __restrict__ b, float* __restrict__ x,
+          const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
+  if (done && *done) return;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = omega * b[i];
 }
 
 // x = (omega dinv) b : the first Jacobi sweep from x = 0
 __global__ void __launch_bounds__(kBlock)
 k_first_sweep(int n, const float* __restrict__ dinv, const float* __restrict__ b,
-              float* __restrict__ x, float omega, const int* __restrict__ done) {
+              float* __restrict__ x, float omega, const float* __restrict__ wp, const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n) x[i] = (omega * dinv[i]) * b[i];
@@ -709,8 +808,9 @@ k_pcg_fin_guess(RedWS ws, int nparts, int has_alt, PcgState* st) {
 __global__ void __launch_bounds__(kBlock)
 k_pcg_select(int n, const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
              const float* __restrict__ x_alt, const float* __restrict__ q,
-             const float* __restrict__ dinv0, float* __restrict__ xv, float omega,
+             const float* __restrict__ dinv0, float* __restrict__ xv, float omega, const float* __restrict__ wp,
              const PcgState* __restrict__ st) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int start = st->start;
@@ -724,7 +824,8 @@ k_pcg_select(int n, const float* __restrict__ b, float* __restrict__ x, float* _
 // r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), partial b.b
 __global__ void __launch_bounds__(kBlock)
 k_pcg_init(int n, const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
-           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, RedWS ws) {
+           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   double acc = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     float bi = b[i];
@@ -770,7 +871,8 @@ k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
 __global__ void __launch_bounds__(kBlock)
 k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
              const float* __restrict__ Ap, const float* __restrict__ dinv0,
-             float* __restrict__ xv, float omega, RedWS ws, const PcgState* st) {
+             float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws, const PcgState* st) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (st->done) return;
   const float al = (float)st->alpha;
   double acc = 0.0;
@@ -891,7 +993,8 @@ __global__ void __launch_bounds__(kBlock)
 k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
         const int16_t* __restrict__ col16, const uint16_t* __restrict__ hval,
         const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ b,
-        float omega, const float* __restrict__ isd, const int* __restrict__ done) {
+        float omega, const float* __restrict__ wp, const float* __restrict__ isd, const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
@@ -942,8 +1045,9 @@ k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
 // x^0 = omega isd r (first sweep on the scaled level 0) and b^0 = isd r
 __global__ void __launch_bounds__(kBlock)
 k_half_first(int n, const float* __restrict__ isd, const float* __restrict__ r,
-             float* __restrict__ bh, float* __restrict__ xh, float omega,
+             float* __restrict__ bh, float* __restrict__ xh, float omega, const float* __restrict__ wp,
              const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -956,7 +1060,8 @@ __global__ void __launch_bounds__(kBlock)
 k_restrict_h(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
              const float* __restrict__ sd_f, const float* __restrict__ r,
              const float* __restrict__ isd_c, float* __restrict__ bc, float* __restrict__ xc,
-             float omega, const int* __restrict__ done) {
+             float omega, const float* __restrict__ wp, const int* __restrict__ done) {
+  if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int I = blockIdx.x * kBlock + threadIdx.x;
   if (I >= nc) return;
@@ -973,12 +1078,12 @@ k_restrict_h(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
 __global__ void __launch_bounds__(kBlock)
 k_prolong_h(int n, const int* __restrict__ agg, const float* __restrict__ sd_f,
             const float* __restrict__ isd_c, const float* __restrict__ xc,
-            float* __restrict__ x, const int* __restrict__ done) {
+            float* __restrict__ x, float g, const int* __restrict__ done) {
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int a = agg[i];
-  x[i] = fmaf(sd_f[i] * isd_c[a], xc[a], x[i]);
+  x[i] = fmaf((sd_f[i] * isd_c[a]) * g, xc[a], x[i]);
 }
 
 // x^ = sd (D (sd b^)) for the dense coarsest inverse D of the unscaled level
@@ -1085,6 +1190,16 @@ struct mgpbd_ctx {
   rocblas_int* dense_info = nullptr;
   bool dense_ready = false;
   float omega = 2.0f / 3.0f;
+  int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
+  float overcorrect = 1.0f;   // scale of the prolongated coarse correction
+  // spectral smoother: per-level weights omega_l = target / (safety lambda_l) in device memory,
+  // lambda_l from a power iteration on D^-1 A warm-started across updates
+  bool spectral = false;
+  float sm_target = 4.0f / 3.0f, sm_safety = 1.1f;
+  int power_iters = 4;
+  float* omega_dev = nullptr;   // MGPBD_MAX_LEVELS
+  double* pw_dev = nullptr;     // 2 per level: lambda estimate, ||w||_D
+  bool pv_ready[MGPBD_MAX_LEVELS] = {};
   MgLevel L[MGPBD_MAX_LEVELS];
   int precision = MGPBD_PREC_FP32;   // V-cycle storage precision
   int flexible = 0;                   // Polak-Ribiere beta
@@ -1104,6 +1219,11 @@ struct mgpbd_ctx {
   double prof_ms[MGPBD_PROF_KINDS] = {};
   int64_t prof_cnt[MGPBD_PROF_KINDS] = {};
 };
+
+// Device pointer of level l's smoothing weight, or nullptr for the fixed host weight.
+static const float* wl(const mgpbd_ctx* c, int l) {
+  return c->spectral ? c->omega_dev + l : nullptr;
+}
 
 static int prof_flush(mgpbd_ctx* c) {
   if (c->ev_used == 0) return 0;
@@ -1150,12 +1270,16 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->work, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&c->omega_dev, sizeof(float) * MGPBD_MAX_LEVELS);
+  if (e == hipSuccess) e = hipMalloc(&c->pw_dev, sizeof(double) * 2 * MGPBD_MAX_LEVELS);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     mgpbd_set_error(std::string("mgpbd_ctx_create: ") + hipGetErrorString(e));
     if (c->ws.partials) (void)hipFree(c->ws.partials);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->omega_dev) (void)hipFree(c->omega_dev);
+    if (c->pw_dev) (void)hipFree(c->pw_dev);
     delete c;
     return nullptr;
   }
@@ -1182,6 +1306,8 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   (void)hipFree(c->ws.partials);
   (void)hipFree(c->st);
   (void)hipHostFree(c->st_host);
+  (void)hipFree(c->omega_dev);
+  (void)hipFree(c->pw_dev);
   delete c;
 }
 
@@ -1333,7 +1459,7 @@ int mgpbd_spmv(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr, const int32_t* i
   if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_spmv: n < 0") : 0;
   int ns = (n + 63) / 64;
   k_sell<0><<<nblocks(64LL * ns), kBlock, 0, ctx->stream>>>(n, ns, slice_ptr, idx, val, x, y,
-                                                           nullptr, nullptr, 0.f, nullptr);
+                                                           nullptr, nullptr, 0.f, nullptr, nullptr);
   MGPBD_LAUNCH_CHECK("mgpbd_spmv");
   return 0;
 }
@@ -1346,7 +1472,7 @@ int mgpbd_jacobi(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr, const int32_t*
   if (x == x_new) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_jacobi: x and x_new must differ");
   int ns = (n + 63) / 64;
   k_sell<1><<<nblocks(64LL * ns), kBlock, 0, ctx->stream>>>(n, ns, slice_ptr, idx, val, x, x_new,
-                                                           dinv, b, omega, nullptr);
+                                                           dinv, b, omega, nullptr, nullptr);
   MGPBD_LAUNCH_CHECK("mgpbd_jacobi");
   return 0;
 }
@@ -1374,7 +1500,7 @@ int mgpbd_restrict(mgpbd_ctx* ctx, int nc, const int32_t* mptr, const int32_t* m
                    const float* r, float* bc) {
   CTX_CHECK("mgpbd_restrict");
   if (nc <= 0) return nc < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_restrict: n < 0") : 0;
-  k_restrict<<<nblocks(nc), kBlock, 0, ctx->stream>>>(nc, mptr, mem, r, bc, nullptr, nullptr, 0.f, nullptr);
+  k_restrict<<<nblocks(nc), kBlock, 0, ctx->stream>>>(nc, mptr, mem, r, bc, nullptr, nullptr, 0.f, nullptr, nullptr);
   MGPBD_LAUNCH_CHECK("mgpbd_restrict");
   return 0;
 }
@@ -1382,7 +1508,7 @@ int mgpbd_restrict(mgpbd_ctx* ctx, int nc, const int32_t* mptr, const int32_t* m
 int mgpbd_prolong(mgpbd_ctx* ctx, int n, const int32_t* agg, const float* xc, float* x) {
   CTX_CHECK("mgpbd_prolong");
   if (n <= 0) return n < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_prolong: n < 0") : 0;
-  k_prolong<<<nblocks(n), kBlock, 0, ctx->stream>>>(n, agg, xc, x, nullptr);
+  k_prolong<<<nblocks(n), kBlock, 0, ctx->stream>>>(n, agg, xc, x, 1.0f, nullptr);
   MGPBD_LAUNCH_CHECK("mgpbd_prolong");
   return 0;
 }
@@ -1429,7 +1555,10 @@ int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega, 
   ctx->omega = omega;
   ctx->coarse_mode = coarse_mode;
   ctx->coarse_sweeps = coarse_sweeps;
-  for (int l = 0; l < MGPBD_MAX_LEVELS; ++l) ctx->L[l] = MgLevel{};
+  for (int l = 0; l < MGPBD_MAX_LEVELS; ++l) {
+    ctx->L[l] = MgLevel{};
+    ctx->pv_ready[l] = false;
+  }
   return 0;
 }
 
@@ -1488,13 +1617,13 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
   if (count <= 0) return cur;
   if (L.diagonal) {
     k_jacobi_diag<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.val, L.dinv, b,
-                                                            cur, ctx->omega, count, done);
+                                                            cur, ctx->omega, wl(ctx, l), count, done);
     return cur;
   }
   if (L.n <= kSmallRows && count >= 8) {   // many sweeps of a small level: one launch
     int pe = prof_begin(ctx, MGPBD_PROF_COARSE);
     k_jacobi_small<<<1, kSmallBlock, 0, ctx->stream>>>(L.n, L.slice_ptr, L.idx, L.val, L.dinv,
-                                                       b, cur, ctx->omega, count, done);
+                                                       b, cur, ctx->omega, wl(ctx, l), count, done);
     prof_end(ctx, pe);
     return cur;
   }
@@ -1502,7 +1631,8 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
     float* nxt = (cur == xa) ? xb : xa;
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
     k_sell<1><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
-        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, done);
+        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, wl(ctx, l),
+        done);
     prof_end(ctx, pe);
     cur = nxt;
   }
@@ -1568,7 +1698,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   if (ctx->n_smooth > 0) {
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
     k_sell<2><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
-        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr, b, 0.f, done);
+        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr, b, 0.f, nullptr, done);
     prof_end(ctx, pe);
   } else {
     k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, b, r);
@@ -1581,10 +1711,26 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   const bool c_first = !c_dense && (c_coarsest || ctx->n_smooth > 0);
   k_restrict<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, L.mptr, L.mem, r, bc,
                                                        C.dinv, c_first ? xc : nullptr,
-                                                       ctx->omega, done);
+                                                       ctx->omega, wl(ctx, l + 1), done);
   if (!c_first && !c_dense) k_blas1<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(3, C.n, 0.f, nullptr, xc);
   float* xcr = vcycle_from(ctx, l + 1, bc, xc, done);
-  k_prolong<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.agg, xcr, x, done);
+  if (ctx->cycle == 2 && !c_coarsest) {
+    // W-cycle: a second coarse cycle on the coarse residual, added to the first result
+    float* save = C.wb(5);
+    float* rc = C.wb(4);
+    k_copy_d<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, xcr, save, done);
+    k_sell<2><<<nblocks(64LL * C.n_slices), kBlock, 0, ctx->stream>>>(
+        C.n, C.n_slices, C.slice_ptr, C.idx, C.val, xcr, rc, nullptr, bc, 0.f, nullptr, done);
+    float* x2 = C.wb(1);
+    if (ctx->n_smooth > 0)
+      k_first_sweep<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, C.dinv, rc, x2, ctx->omega,
+                                                                 wl(ctx, l + 1), done);
+    else
+      k_blas1<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(3, C.n, 0.f, nullptr, x2);
+    xcr = vcycle_from(ctx, l + 1, rc, x2, done);
+    k_add_d<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, save, xcr, done);
+  }
+  k_prolong<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.agg, xcr, x, ctx->overcorrect, done);
   return sweeps(ctx, l, b, x, ctx->n_smooth, done);
 }
 
@@ -1592,12 +1738,12 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
 extern "C++" {
 template <int VT>
 static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
-                         const float* b, const int* done) {
+                         const float* b, const int* done, const float* wp) {
   const int g = nblocks(64LL * L.n_slices);
   hipStream_t s = ctx->stream;
   const float w = ctx->omega;
 #define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
-                                                         L.col16, L.hval, x, y, b, w, L.isd, done)
+                                                         L.col16, L.hval, x, y, b, w, wp, L.isd, done)
   if (L.col16) {
     if (mode == 1) SELLH(true, 1); else if (mode == 2) SELLH(true, 2); else SELLH(true, 3);
   } else {
@@ -1608,9 +1754,9 @@ static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float
 }  // extern "C++"
 
 static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
-                  const float* b, const int* done) {
-  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done);
-  else launch_sellh<2>(ctx, L, mode, x, y, b, done);
+                  const float* b, const int* done, const float* wp) {
+  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp);
+  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp);
 }
 
 // `count` scaled sweeps from cur; with z_out the last one writes isd * x into z_out instead.
@@ -1623,7 +1769,7 @@ static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int co
     const bool last_out = z_out && sw == count - 1;
     float* nxt = last_out ? z_out : ((cur == xa) ? xb : xa);
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
-    sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done);
+    sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l));
     prof_end(ctx, pe);
     cur = nxt;
   }
@@ -1648,17 +1794,117 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
   float* x = sweeps_h(ctx, l, b, x_first, ctx->n_smooth - 1, done, nullptr);
   float* r = L.wb(3);
   int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
-  sellh(ctx, L, 2, x, r, b, done);
+  sellh(ctx, L, 2, x, r, b, done, nullptr);
   prof_end(ctx, pe);
   MgLevel& C = ctx->L[l + 1];
   float* bc = C.wb(0);
   float* xc = C.wb(1);
   const bool c_dense = (l + 1 == ctx->n_levels - 1) && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY;
   k_restrict_h<<<nblocks(C.n), kBlock, 0, s>>>(C.n, L.mptr, L.mem, L.sd, r, C.isd, bc,
-                                               c_dense ? nullptr : xc, ctx->omega, done);
+                                               c_dense ? nullptr : xc, ctx->omega, wl(ctx, l + 1),
+                                               done);
   float* xcr = vcycle_h(ctx, l + 1, bc, xc, done, nullptr);
-  k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x, done);
+  if (ctx->cycle == 2 && l + 1 < ctx->n_levels - 1) {
+    float* save = C.wb(5);
+    float* rc = C.wb(4);
+    k_copy_d<<<nblocks(C.n), kBlock, 0, s>>>(C.n, xcr, save, done);
+    sellh(ctx, C, 2, xcr, rc, bc, done, nullptr);
+    float* x2 = C.wb(1);
+    k_scale_d<<<nblocks(C.n), kBlock, 0, s>>>(C.n, ctx->omega, wl(ctx, l + 1), rc, x2, done);
+    xcr = vcycle_h(ctx, l + 1, rc, x2, done, nullptr);
+    k_add_d<<<nblocks(C.n), kBlock, 0, s>>>(C.n, save, xcr, done);
+  }
+  k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x, ctx->overcorrect,
+                                              done);
   return sweeps_h(ctx, l, b, x, ctx->n_smooth, done, z_out);
+}
+
+int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect) {
+  CTX_CHECK("mgpbd_mg_set_cycle");
+  if ((cycle != 1 && cycle != 2) || !(overcorrect > 0.0f && overcorrect < 4.0f))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_cycle: cycle 1 (V) or 2 (W), overcorrect in (0, 4)");
+  ctx->cycle = cycle;
+  ctx->overcorrect = overcorrect;
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  return 0;
+}
+
+int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, int power_iters) {
+  CTX_CHECK("mgpbd_mg_set_smoother");
+  if ((mode != 0 && mode != 1) || !(target > 0.0f && target < 2.0f) || !(safety >= 1.0f) ||
+      power_iters < 1 || power_iters > 64)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_smoother: mode 0/1, target in (0, 2), "
+                                    "safety >= 1, power_iters in [1, 64]");
+  ctx->spectral = mode == 1;
+  ctx->sm_target = target;
+  ctx->sm_safety = safety;
+  ctx->power_iters = power_iters;
+  for (int l = 0; l < MGPBD_MAX_LEVELS; ++l) {
+    ctx->pv_ready[l] = false;
+    k_set_float<<<1, 64, 0, ctx->stream>>>(ctx->omega_dev + l, ctx->omega);
+  }
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_set_smoother");
+  return 0;
+}
+
+int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
+  CTX_CHECK("mgpbd_mg_update_smoother");
+  if (int e = check_levels(ctx)) return e;
+  if (!ctx->spectral) return 0;
+  hipStream_t s = ctx->stream;
+  for (int l = 0; l < ctx->n_levels; ++l) {
+    MgLevel& L = ctx->L[l];
+    const bool coarsest = l == ctx->n_levels - 1;
+    if (coarsest && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY) continue;
+    if (L.diagonal) {   // D^-1 A = I: Jacobi with weight 1 is exact
+      k_set_float<<<1, 64, 0, s>>>(ctx->omega_dev + l, 1.0f);
+      continue;
+    }
+    if (nblocks(64LL * L.n_slices) > kPartialStride)
+      return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_update_smoother: level exceeds the reduction capacity");
+    float* v = L.wb(6);
+    float* w = L.wb(7);
+    int iters = ctx->power_iters;
+    if (!ctx->pv_ready[l]) {
+      k_power_init<<<nblocks(L.n), kBlock, 0, s>>>(L.n, v);
+      iters = std::max(iters, 12);
+      ctx->pv_ready[l] = true;
+    }
+    const int g = nblocks(64LL * L.n_slices);
+    double* pw = ctx->pw_dev + 2 * l;
+    for (int it = 0; it < iters; ++it) {
+      k_power_step<<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.dinv, v, w,
+                                        ctx->ws);
+      k_power_fin<<<1, kBlock, 0, s>>>(ctx->ws, g, pw);
+      k_power_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, w, v, pw);
+    }
+    k_power_omega<<<1, 64, 0, s>>>(pw, ctx->sm_target, ctx->sm_safety, ctx->omega,
+                                   ctx->omega_dev + l);
+  }
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_update_smoother");
+  if (lambda_out) {
+    double tmp[2 * MGPBD_MAX_LEVELS];
+    MGPBD_HIP_TRY(hipMemcpyAsync(tmp, ctx->pw_dev, sizeof(tmp), hipMemcpyDeviceToHost, s));
+    MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    for (int l = 0; l < ctx->n_levels; ++l) lambda_out[l] = tmp[2 * l];
+  }
+  return 0;
+}
+
+int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out) {
+  CTX_CHECK("mgpbd_mg_get_smoother");
+  if (int e = check_levels(ctx)) return e;
+  if (!omega_out) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_get_smoother: null output");
+  if (!ctx->spectral) {
+    for (int l = 0; l < ctx->n_levels; ++l) omega_out[l] = ctx->omega;
+    return 0;
+  }
+  float tmp[MGPBD_MAX_LEVELS];
+  MGPBD_HIP_TRY(hipMemcpyAsync(tmp, ctx->omega_dev, sizeof(tmp), hipMemcpyDeviceToHost, ctx->stream));
+  MGPBD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int l = 0; l < ctx->n_levels; ++l) omega_out[l] = tmp[l];
+  return 0;
 }
 
 int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible) {
@@ -1709,7 +1955,8 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   if (int e = check_ready(ctx)) return e;
   MgLevel& L = ctx->L[0];
   if (half_mode(ctx)) {
-    k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, b, L.wb(0), L.wb(1), ctx->omega, nullptr);
+    k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, b, L.wb(0), L.wb(1), ctx->omega,
+                                                           wl(ctx, 0), nullptr);
     float* z = vcycle_h(ctx, 0, L.wb(0), L.wb(1), nullptr, L.wb(3));
     k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, z, x_out);
     MGPBD_LAUNCH_CHECK("mgpbd_mg_vcycle");
@@ -1717,7 +1964,7 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   }
   float* x1 = L.wb(1);
   const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
-  if (first) k_first_sweep<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.dinv, b, x1, ctx->omega, nullptr);
+  if (first) k_first_sweep<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.dinv, b, x1, ctx->omega, wl(ctx, 0), nullptr);
   else k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(3, L.n, 0.f, nullptr, x1);
   float* res = vcycle_from(ctx, 0, b, x1, nullptr);
   k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, res, x_out);
@@ -1732,7 +1979,7 @@ static float* pcg_precondition(mgpbd_ctx* ctx, float* r, float* xv, bool first) 
   const int* done = &ctx->st->done;
   if (half_mode(ctx)) {
     k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, r, L.wb(0), L.wb(1),
-                                                           ctx->omega, done);
+                                                           ctx->omega, wl(ctx, 0), done);
     return vcycle_h(ctx, 0, L.wb(0), L.wb(1), done, L.wb(3));
   }
   if (!first) k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(3, L.n, 0.f, nullptr, xv);
@@ -1752,7 +1999,7 @@ static void pcg_iteration(mgpbd_ctx* ctx, float* x, float* r, float* p, float* A
   k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
   const bool fuse_first = first && !half_mode(ctx);
   k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, fuse_first ? xv : nullptr,
-                                     ctx->omega, ctx->ws, ctx->st);
+                                     ctx->omega, wl(ctx, 0), ctx->ws, ctx->st);
   k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
   float* zz = pcg_precondition(ctx, r, xv, first);
   if (ctx->flexible) {
@@ -1841,16 +2088,18 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
     // start with the smallest residual
     k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
-                                    b, 0.f, nullptr);
+                                    b, 0.f, nullptr, nullptr);
     if (x_alt)
       k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x_alt, Ap,
-                                      nullptr, b, 0.f, nullptr);
+                                      nullptr, b, 0.f, nullptr, nullptr);
     k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, x_alt ? Ap : nullptr, ctx->ws);
     k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, x_alt != nullptr, ctx->st);
     k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, x_alt, Ap, L.dinv,
-                                               fuse_first ? xv : nullptr, ctx->omega, ctx->st);
+                                               fuse_first ? xv : nullptr, ctx->omega, wl(ctx, 0),
+                                               ctx->st);
   } else {
-    k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr, ctx->omega, ctx->ws);
+    k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr, ctx->omega,
+                                      wl(ctx, 0), ctx->ws);
     k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
   }
   float* zr = pcg_precondition(ctx, r, xv, first);

@@ -17,7 +17,10 @@ Reference behaviour kept by default (parity mode):
 Keyword-only extensions (no reference counterpart; defaults keep the reference's
 behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('reference' | 'extended'), ``galerkin`` ('values' | 'structural'), ``max_coarse``, ``max_levels``,
-``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'), ``f_ext`` (external vertex forces), ``device``,
+``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'),
+``cycle`` ('V' | 'W'), ``overcorrect`` (scale of the coarse correction), ``smoother``
+('jacobi' | 'spectral': per-level weights from lambda_max(D^-1 A)), ``f_ext``
+(external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
 substep's lambda, or from the network's prediction with the previous lambda as the fallback;
@@ -64,7 +67,9 @@ class MGPBDSolverGPU:
                  hierarchy: str = "reference", galerkin: str = "values",
                  max_coarse: int = 400, max_levels: int | None = None,
                  coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
-                 precision: str = "fp32", f_ext=None, warm_start: str = "none",
+                 precision: str = "fp32", cycle: str = "V", overcorrect: float = 1.0,
+                 smoother: str = "jacobi",
+                 f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation != "tet":
@@ -105,6 +110,9 @@ class MGPBDSolverGPU:
         self.coarse_sweeps = int(coarse_sweeps)
         self.coarse_solver = coarse_solver
         self.precision = precision
+        self.cycle = cycle
+        self.overcorrect = float(overcorrect)
+        self.smoother = smoother
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -155,8 +163,8 @@ class MGPBDSolverGPU:
             n_smooth=self.n_smooth, omega_jacobi=self.omega_jacobi,
             maxiter_cg=self.maxiter_cg, tol_cg=self.tol_cg, relative=self.relative_tol,
             coarse_sweeps=self.coarse_sweeps, galerkin=self.galerkin,
-            coarse_solver=self.coarse_solver, precision=self.precision,
-            device=self.device.index)
+            coarse_solver=self.coarse_solver, precision=self.precision, cycle=self.cycle,
+            overcorrect=self.overcorrect, smoother=self.smoother, device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()
         if self.verbose:

@@ -14,7 +14,11 @@ Extensions (keyword-only, no reference counterpart):
   inverse (refactored on every coarse refill) instead of min(50, n) sweeps;
 * ``precision='fp16' | 'bf16'``: the V-cycle runs on the diagonally scaled levels stored in
   half precision (FP32 accumulation), inside a flexible (Polak-Ribiere) FP32 PCG with FP64
-  dot products; the outer SpMV stays FP32.
+  dot products; the outer SpMV stays FP32;
+* ``smoother='spectral'``: per-level Jacobi weights target / (safety lambda_max(D^-1 A)) from a
+  device power iteration refreshed at every refill (the reference's fixed 2/3 makes the V-cycle
+  indefinite once lambda_max(D^-1 A) > 3, which per-tet stiffness contrast produces);
+* ``cycle='W'`` and ``overcorrect``: W-cycle and a scaled coarse correction.
 """
 
 from __future__ import annotations
@@ -27,7 +31,7 @@ import torch
 
 from . import _lib
 from ._lib import check, dptr, np_ptr
-from .runtime import get_context
+from .runtime import new_context
 from .sparse_gpu import GPUSparseMatrix
 
 
@@ -57,7 +61,9 @@ class GPUMultigridSolver:
                  maxiter_cg: int = 100, tol_cg: float = 1e-5, *, relative: bool = False,
                  coarse_sweeps: int = 50, galerkin: str = "values",
                  coarse_solver: str = "jacobi", precision: str = "fp32",
-                 flexible: bool | None = None, device=None):
+                 flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
+                 smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
+                 smoother_safety: float = 1.1, power_iters: int = 4, device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
         self.maxiter_cg = int(maxiter_cg)
@@ -74,9 +80,22 @@ class GPUMultigridSolver:
             raise ValueError("precision must be 'fp32', 'fp16' or 'bf16'")
         self.precision = precision
         self.flexible = (precision != "fp32") if flexible is None else bool(flexible)
+        if cycle not in ("V", "W"):
+            raise ValueError("cycle must be 'V' (reference) or 'W'")
+        self.cycle = cycle
+        self.overcorrect = float(overcorrect)
+        if smoother not in ("jacobi", "spectral"):
+            raise ValueError("smoother must be 'jacobi' (reference) or 'spectral'")
+        self.smoother = smoother
+        self.smoother_target = float(smoother_target)
+        self.smoother_safety = float(smoother_safety)
+        self.power_iters = int(power_iters)
+        self.last_lambda = None
         self.n_levels = 0
         self._initialized = False
-        self.ctx = get_context(device)
+        # the hierarchy, PCG state and PCG graph live in the context: one per solver, so that
+        # several solvers on one device do not overwrite each other's levels
+        self.ctx = new_context(device)
         self.device = self.ctx.device
         self.last_info = None
 
@@ -146,8 +165,8 @@ class GPUMultigridSolver:
             F = C
         self.n_levels = len(self.levels)
         self._level_sizes = [L.n_rows for L in self.levels]
-        # work vectors per level: b, x, x', r
-        self._work = [torch.zeros(4 * 64 * ((L.n_rows + 63) // 64), dtype=torch.float32,
+        # work vectors per level: b, x, x', r, two W-cycle and two power-iteration buffers
+        self._work = [torch.zeros(8 * 64 * ((L.n_rows + 63) // 64), dtype=torch.float32,
                                   device=self.device) for L in self.levels]
         n0 = self._level_sizes[0]
         self._pcg_work = torch.zeros(4 * n0, dtype=torch.float32, device=self.device)
@@ -194,6 +213,9 @@ class GPUMultigridSolver:
         mode = 1 if (self.coarse_solver == "cholesky" and self.n_levels > 1) else 0
         ctx("mgpbd_mg_configure", self.n_levels, self.n_smooth, self.omega_jacobi, mode,
             self.coarse_sweeps)
+        ctx("mgpbd_mg_set_cycle", 2 if self.cycle == "W" else 1, self.overcorrect)
+        ctx("mgpbd_mg_set_smoother", 1 if self.smoother == "spectral" else 0,
+            self.smoother_target, self.smoother_safety, self.power_iters)
         self._coarse_mode = mode
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None
@@ -222,6 +244,22 @@ class GPUMultigridSolver:
             self.ctx("mgpbd_mg_factor_coarsest")
         if getattr(self, "_half", None):
             self.ctx("mgpbd_mg_convert_half")
+        if self.smoother == "spectral":
+            self.ctx("mgpbd_mg_update_smoother", None)
+
+    def smoother_weights(self):
+        """Jacobi weight of each level as the V-cycle uses it now (synchronises)."""
+        w = np.zeros(self.n_levels, np.float32)
+        self.ctx("mgpbd_mg_get_smoother", w.ctypes.data_as(_lib.F32P))
+        return w
+
+    def smoother_estimates(self):
+        """Largest eigenvalue of D^-1 A per level from the spectral smoother's power iteration
+        (advances it by its usual iteration count; synchronises)."""
+        lam = np.zeros(self.n_levels, np.float64)
+        self.ctx("mgpbd_mg_update_smoother", lam.ctypes.data_as(_lib.F64P))
+        self.last_lambda = lam
+        return lam
 
     # -- solve -----------------------------------------------------------------------------
     def vcycle(self, level: int = 0):

@@ -53,6 +53,16 @@ def require_gpu() -> None:
                          "is False); there is no CPU fallback")
 
 
+def new_context(device: int | None = None) -> Context:
+    """A private context: its multigrid levels, PCG state and captured PCG graph belong to one
+    solver (the per-device shared context serves the stateless kernels)."""
+    require_gpu()
+    dev = torch.cuda.current_device() if device is None else int(device)
+    ctx = Context(dev)
+    ctx.bind_stream()
+    return ctx
+
+
 def get_context(device: int | None = None) -> Context:
     require_gpu()
     dev = torch.cuda.current_device() if device is None else int(device)

@@ -239,8 +239,9 @@ int mgpbd_negate_and_add(mgpbd_ctx* ctx, int n, const float* b_dev, float* r_dev
 int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega,
                        int coarse_mode, int coarse_sweeps);
 /* Level l operator and transfer to level l+1 (agg/mem_ptr/mem NULL on the coarsest).
- * work_dev: 4 * 64 * ceil(n_rows / 64) floats owned by the caller (b, x, x', r of the level,
- * each padded to whole 64-row slices).
+ * work_dev: 8 * 64 * ceil(n_rows / 64) floats owned by the caller (b, x, x', r of the level,
+ * two W-cycle buffers and the spectral smoother's power-iteration vectors, each padded to
+ * whole 64-row slices).
  * diagonal_only != 0 declares that every row holds only its diagonal entry (then repeated
  * Jacobi sweeps run as one elementwise pass with identical arithmetic). */
 int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* slice_ptr_dev,
@@ -253,6 +254,25 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
  * change (every coarse refill). Synchronises; MGPBD_ENONFINITE if not positive definite.
  * No reference counterpart (the reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
+/* Cycle shape (extension; no reference counterpart: cons/mgpcg_gpu.py:214-259 is a V-cycle
+ * adding the plain coarse correction). cycle 1 = V, 2 = W (each level below the finest that is
+ * not the coarsest is cycled twice, the second time on its residual); overcorrect scales the
+ * prolongated coarse correction (1 = the reference). The defaults are 1 and 1. */
+int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect);
+/* Smoother weights (extension). mode 0 = the reference's fixed weighted Jacobi (the omega of
+ * mgpbd_mg_configure on every level, cons/mgpcg_gpu.py:232-246). mode 1 = spectral: level l
+ * relaxes with omega_l = target / (safety lambda_l), lambda_l the largest eigenvalue of
+ * D^-1 A_l estimated by power iteration (12 iterations the first time, power_iters after,
+ * warm-started); a weight with omega_l lambda_max < 2 keeps the V-cycle an SPD preconditioner,
+ * which the fixed 2/3 loses once lambda_max(D^-1 A) > 3. Weights live in device memory, so
+ * captured PCG graphs need no recapture when they change. */
+int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, int power_iters);
+/* Re-estimates lambda_l and omega_l for the current level values (call after the coarse refill;
+ * no-op in mode 0). lambda_out (host, n_levels doubles, may be NULL) receives the estimates
+ * (then the call synchronises). */
+int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out);
+/* The weight each level relaxes with (host, n_levels floats; synchronises). */
+int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out);
 /* Reduced-precision V-cycle (extension; no reference counterpart). Every level is used in its
  * scaled form D^-1/2 A D^-1/2 (unit diagonal), values stored FP16 or BF16 in the level's
  * SELL positions, products accumulated in FP32; the outer PCG SpMV stays FP32. flexible != 0

@@ -319,6 +319,7 @@ typedef struct {
   const real* ta;
   real *b, *x, *xn, *r, *tmp;
   const double* dense;      /* coarsest only: dense column-major inverse, or NULL (sweeps) */
+  real omega;               /* this level's Jacobi weight; <= 0: the cycle's common weight */
 } OrcLevel;
 
 typedef struct {
@@ -330,6 +331,7 @@ typedef struct {
 static void vc(const OrcMG* mg, int l) {
   const OrcLevel* A = &mg->L[l];
   int n = A->n;
+  const real w = A->omega > 0 ? A->omega : mg->omega;
   if (l == mg->nl - 1 && A->dense) {
     /* x = A^-1 b with FP64 accumulation */
     for (int i = 0; i < n; ++i) {
@@ -342,14 +344,14 @@ static void vc(const OrcMG* mg, int l) {
   if (l == mg->nl - 1) {
     int it = n < 50 ? n : 50;
     for (int s = 0; s < it; ++s) {
-      orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, mg->omega);
+      orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, w);
       memcpy(A->x, A->xn, sizeof(real) * n);
     }
     return;
   }
   const OrcLevel* C = &mg->L[l + 1];
   for (int s = 0; s < mg->n_smooth; ++s) {
-    orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, mg->omega);
+    orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, w);
     memcpy(A->x, A->xn, sizeof(real) * n);
   }
   orc_spmv(n, A->ip, A->ix, A->a, A->x, A->r);
@@ -360,7 +362,7 @@ static void vc(const OrcMG* mg, int l) {
   orc_spmv(n, A->pip, A->pix, A->pa, C->x, A->tmp);
   for (int i = 0; i < n; ++i) A->x[i] = FMA((real)1, A->tmp[i], A->x[i]);
   for (int s = 0; s < mg->n_smooth; ++s) {
-    orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, mg->omega);
+    orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, w);
     memcpy(A->x, A->xn, sizeof(real) * n);
   }
 }

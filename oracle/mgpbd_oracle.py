@@ -115,14 +115,14 @@ class _OrcLevel(ctypes.Structure):
     pass
 
 
-def _level_struct(R):
+def _level_struct(R, scalar):
     class OrcLevel(ctypes.Structure):
         _fields_ = [("n", ctypes.c_int),
                     ("ip", I32P), ("ix", I32P), ("a", R), ("dinv", R),
                     ("pip", I32P), ("pix", I32P), ("pa", R),
                     ("tip", I32P), ("tix", I32P), ("ta", R),
                     ("b", R), ("x", R), ("xn", R), ("r", R), ("tmp", R),
-                    ("dense", F64P)]
+                    ("dense", F64P), ("omega", scalar)]
     return OrcLevel
 
 
@@ -181,8 +181,15 @@ class OracleMG:
                                       ptr(lv["a"], self.R), ptr(lv["dinv"], self.R))
         self._build_structs()
 
+    def set_level_omegas(self, omegas):
+        """Per-level Jacobi weights (None: the common ``omega`` on every level)."""
+        self.level_omegas = None if omegas is None else [float(w) for w in omegas]
+        self._build_structs()
+
     def _build_structs(self):
-        S = _level_struct(self.R)
+        scalar = ctypes.c_float if self.dt == np.float32 else ctypes.c_double
+        S = _level_struct(self.R, scalar)
+        oms = getattr(self, "level_omegas", None)
         arr = (S * len(self.levels))()
         R = self.R
         for i, lv in enumerate(self.levels):
@@ -197,6 +204,7 @@ class OracleMG:
             for k in ("b", "x", "xn", "r", "tmp"):
                 setattr(s, k, ptr(lv[k], R))
             s.dense = ptr(lv.get("dense"), F64P)
+            s.omega = oms[i] if oms is not None else 0.0
         self._structs = arr
 
     def update_fine_matrix_data(self, data):

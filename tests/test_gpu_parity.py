@@ -400,19 +400,16 @@ def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision):
 
 
 def test_mixed_precision_substeps_match_fp32():
-    """Ten substeps with the FP16 V-cycle stay within 1e-4 h of the FP32 V-cycle run when the
-    Newton path has no discrete choices: both solve every linear system to the same 1e-6
-    relative PCG tolerance, take full steps (no line search) and run a fixed five Newton
-    iterations. With the line search or an early exit the two runs can take different
-    branches on 1e-6-level differences, and the MGPBD iteration is path-dependent (lambda takes
-    the full dlambda while x moves by omega dx, and the primal residual is not driven to zero),
-    so branches settle on different states; per-solve accuracy of the mixed V-cycle is pinned
-    by the FP64-verified PCG test above."""
+    """Ten substeps with the FP16 V-cycle stay within 1e-4 h of the FP32 V-cycle run, the two
+    solvers stepping side by side on one device. Both solve every linear system to the same
+    1e-6 relative PCG tolerance and run Newton to rtol 1e-6, so neither run can stop a Newton
+    loop one iteration earlier than the other (with the default rtol 1e-2 that decision flips
+    on 1e-6-level differences)."""
     from tests.scenarios import physical_block
     mesh, mu_t = physical_block(6, seed=9)
     kw = dict(mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended", galerkin="structural",
               relative_tol=True, tol_cg=1e-6, max_coarse=64, coarse_solver="cholesky",
-              rtol=0.0, atol=0.0, maxiter=5, use_line_search=False)
+              rtol=1e-6, atol=1e-9, maxiter=30)
     a = _solver(mesh, None, **kw)
     b = _solver(mesh, None, precision="fp16", **kw)
     worst = 0.0
@@ -423,3 +420,108 @@ def test_mixed_precision_substeps_match_fp32():
     print(f"[mixed] substeps max|dx|/h = {worst / mesh.h:.2e} fp32 {a.last_stats} "
           f"fp16 {b.last_stats}", flush=True)
     assert worst <= 1e-4 * mesh.h
+
+
+def test_two_solvers_on_one_device_are_independent():
+    """Each multigrid solver keeps its own hierarchy, PCG state and graph: two solvers set up
+    on different matrices before either solves give the same bits as each one alone."""
+    from controllable_pbd_3d_amd import GPUMultigridSolver
+    from tests.scenarios import physical_block
+    systems = []
+    for seed in (21, 22):
+        mesh, mu_t = physical_block(5, seed=seed)
+        s = _solver(mesh, deformed(mesh, seed=seed, amp=0.05, twist=0.1), mu_per_tet=mu_t)
+        s._compute_C_and_gradC()
+        s._fill_A_and_update_gpu()
+        csr = s._A.csr_data().astype(np.float64)
+        A = sp.csr_matrix((csr, s._A_indices, s._A_indptr), shape=(mesh.n_tets,) * 2)
+        systems.append((A, s._aggs, s._b_gpu.clone()))
+
+    def make(A, aggs):
+        g = GPUMultigridSolver(2, 2.0 / 3.0, 100, 1e-6, relative=True, galerkin="structural")
+        g.setup(A, aggs)
+        return g
+
+    alone = []
+    for A, aggs, b in systems:
+        g = make(A, aggs)
+        x = torch.zeros_like(b)
+        it = g.solve(b, x)
+        alone.append((it, x.cpu().numpy()))
+        del g
+    both = [make(A, aggs) for A, aggs, _ in systems]
+    for (A, aggs, b), g, (it0, x0) in zip(systems, both, alone):
+        x = torch.zeros_like(b)
+        it = g.solve(b, x)
+        assert it == it0 and np.array_equal(x.cpu().numpy(), x0)
+
+
+@pytest.fixture(scope="module")
+def stiff_scene():
+    """Strained block with E log-uniform in [1e4, 1e8] Pa at h = 0.02: lambda_max(D^-1 A) near
+    7, where the reference's fixed 2/3-weighted Jacobi diverges on the top modes."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(6, seed=12, h=0.02, E_range=(1e4, 1e8))
+    v0 = deformed(mesh, seed=12, amp=0.05, twist=0.1)
+    print(f"[stiff scene] box6 h=0.02 n_tets={mesh.n_tets}", flush=True)
+    return mesh, mu_t, v0
+
+
+def test_spectral_smoother_weights_and_vcycle_bitexact(stiff_scene):
+    """Power-iteration estimates within [0.85, 1] of lambda_max(D^-1 A_l) (scipy), the weights
+    target / (safety lambda) as stated, omega lambda_max < 2 on every level, and the V-cycle
+    with those weights bit-identical to the FP32 oracle's per-level-weight V-cycle."""
+    import scipy.sparse.linalg as spla
+    mesh, mu_t, v0 = stiff_scene
+    s = _solver(mesh, v0, mu_per_tet=mu_t, smoother="spectral")
+    g = s._gpu_solver
+    lam_hat = g.smoother_estimates()
+    w = g.smoother_weights()
+    for l, L in enumerate(g.levels):
+        A = sp.csr_matrix((L.csr_data().astype(np.float64), L.indices, L.indptr),
+                          shape=(L.n_rows,) * 2)
+        d = A.diagonal()
+        S = sp.diags(1 / np.sqrt(d)) @ A @ sp.diags(1 / np.sqrt(d))
+        lam = spla.eigsh(S, k=1, which="LA", return_eigenvectors=False)[0]
+        exp_w = np.float32(np.float64(np.float32(4.0 / 3.0)) /
+                           (np.float64(np.float32(1.1)) * lam_hat[l]))
+        print(f"[spectral] level {l} n={L.n_rows} lambda {lam:.4f} estimate {lam_hat[l]:.4f} "
+              f"omega {w[l]:.4f}", flush=True)
+        assert 0.85 * lam <= lam_hat[l] <= lam * (1 + 1e-6)
+        assert w[l] == exp_w
+        assert w[l] * lam < 2.0
+    assert lam_hat[0] > 3.0
+    o = _oracle(mesh, v0, precision="f32", mu_per_tet=mu_t)
+    o.mg.update_fine_matrix_data(s._A.csr_data())
+    o.mg.update_all_coarse_matrices()
+    o.mg.set_level_omegas(w.tolist())
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(o.n_tets).astype(np.float32)
+    z_o = o.mg.vcycle(r)
+    rt = torch.from_numpy(r).cuda()
+    zt = torch.zeros_like(rt)
+    g.apply(rt, zt)
+    assert np.array_equal(zt.cpu().numpy().view(np.uint32), z_o.view(np.uint32))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_spectral_pcg_converges_where_fixed_jacobi_stalls(stiff_scene, precision):
+    """Relative 1e-6 within 100 iterations (FP64-verified) with spectral weights; the
+    reference's fixed 2/3 weight on the same system does not get there in 300."""
+    mesh, mu_t, v0 = stiff_scene
+    res = {}
+    for smoother, maxiter in (("jacobi", 300), ("spectral", 100)):
+        s = _solver(mesh, v0, mu_per_tet=mu_t, smoother=smoother, relative_tol=True,
+                    tol_cg=1e-6, maxiter_cg=maxiter, precision=precision, hierarchy="extended",
+                    galerkin="structural", max_coarse=64, coarse_solver="cholesky")
+        A = sp.csr_matrix((s._A.csr_data().astype(np.float64), s._A_indices, s._A_indptr),
+                          shape=(mesh.n_tets,) * 2)
+        b = s._b_gpu.cpu().numpy().astype(np.float64)
+        x = torch.zeros_like(s._b_gpu)
+        it = s._gpu_solver.solve(s._b_gpu, x)
+        rel = np.linalg.norm(b - A @ x.cpu().numpy().astype(np.float64)) / np.linalg.norm(b)
+        res[smoother] = (it, rel, s._gpu_solver.last_info["reason"])
+    print(f"[spectral pcg {precision}] {res}", flush=True)
+    it, rel, reason = res["spectral"]
+    assert reason == 1 and it < 100 and rel < 2e-6
+    assert res["jacobi"][2] != 1 and res["jacobi"][1] > 1e-4

@@ -264,3 +264,60 @@ def test_line_search_falls_back_to_2_pow_minus_10():
     assert o._line_search() == 0.5 ** 10
     o.dpos[:] = 0.0
     assert o._line_search() == 0.5 ** 10      # equal norms never pass the strict test
+
+
+def _stiff_contrast_system(n=5, seed=0):
+    """A strained block with per-tet E log-uniform in [1e4, 1e8] Pa (h = 0.02): the
+    stiffness contrast pushes lambda_max(D^-1 A) well above 3."""
+    from controllable_pbd_3d_amd.mesh import box_mesh as bm
+    from tests.scenarios import solver_kwargs, deformed
+    mesh = bm(n, 0.02, jitter=0.05, seed=seed, rho=1000.0, E_range=(1e4, 1e8))
+    mu_t = (mesh.E / 2.9).astype(np.float32)
+    kw = solver_kwargs(mesh, mu_per_tet=mu_t)
+    kw["v_p"] = deformed(mesh, seed=seed + 1, amp=0.05, twist=0.1)
+    o = om.OracleMGPBD(**kw, precision="f64", hold_pinned=True)
+    o.init()
+    o._compute_C_and_gradC()
+    o._fill_A(o.A_data)
+    o.mg.update_fine_matrix_data(o.A_data)
+    o.mg.update_all_coarse_matrices()
+    return o
+
+
+def _vcycle_operator(mg, n):
+    M = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        M[:, j] = mg.vcycle(e)
+    return M
+
+
+def test_fixed_jacobi_vcycle_is_indefinite_and_spectral_weights_fix_it():
+    """The V-cycle preconditioner with the fixed weight 2/3 is symmetric but indefinite once
+    lambda_max(D^-1 A) > 3 (weighted Jacobi then amplifies the top mode, |1 - 2/3 lambda| > 1);
+    with per-level weights (4/3) / lambda_max(D^-1 A_l) it is positive definite. Eigenvalues
+    from numpy on the explicitly assembled operator (FP64 oracle)."""
+    o = _stiff_contrast_system()
+    n = o.n_tets
+    lam = []
+    for lv in o.mg.levels:
+        A = sp.csr_matrix((lv["a"].astype(np.float64), lv["ix"], lv["ip"]), shape=(lv["n"],) * 2)
+        d = A.diagonal()
+        S = (sp.diags(1 / np.sqrt(d)) @ A @ sp.diags(1 / np.sqrt(d))).toarray()
+        lam.append(np.linalg.eigvalsh(S).max())
+    assert len(lam) >= 2 and lam[0] > 3.5, lam
+    M = _vcycle_operator(o.mg, n)
+    assert np.abs(M - M.T).max() <= 1e-9 * np.abs(M).max()
+    ev_fixed = np.linalg.eigvalsh(0.5 * (M + M.T))
+    o.mg.set_level_omegas([(4.0 / 3.0) / l for l in lam])
+    M2 = _vcycle_operator(o.mg, n)
+    assert np.abs(M2 - M2.T).max() <= 1e-9 * np.abs(M2).max()
+    ev_spec = np.linalg.eigvalsh(0.5 * (M2 + M2.T))
+    print(f"lambda_max(D^-1 A_l) {np.round(lam, 3)}; eig M^-1 fixed [{ev_fixed.min():.3e}, "
+          f"{ev_fixed.max():.3e}] spectral [{ev_spec.min():.3e}, {ev_spec.max():.3e}]")
+    assert ev_fixed.min() < 0
+    assert ev_spec.min() > 0
+    # the common weight on every level reproduces the fixed-weight cycle exactly
+    o.mg.set_level_omegas([2.0 / 3.0] * len(lam))
+    assert np.array_equal(_vcycle_operator(o.mg, n), M)
