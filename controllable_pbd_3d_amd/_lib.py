@@ -51,6 +51,7 @@ SIGNATURES = {
     "mgpbd_ctx_sync": (c_int, [_V]),
     "mgpbd_ctx_profile": (c_int, [_V, c_int]),
     "mgpbd_ctx_use_graphs": (c_int, [_V, c_int, c_int]),
+    "mgpbd_pcg_set_verify": (c_int, [_V, c_int]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),

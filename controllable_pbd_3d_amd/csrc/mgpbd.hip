@@ -748,6 +748,7 @@ k_dot(int n, const float* __restrict__ a, const float* __restrict__ b, RedWS ws)
 struct PcgState {
   double rz, pAp, alpha, beta, rr, bb, tol2, tol;
   double rr0, rr_guess, rr_alt;   // initial residuals (started from a guess)
+  double rr_true;                 // FP64 ||b - A x||^2 of the returned x (verification)
   int done;     // 0 running, 1 converged, 2 |pAp| < 1e-30, 3 |rz| < 1e-30
   int iters;    // completed iterations (the reference's cg_iters)
   int k;        // iteration index
@@ -759,6 +760,7 @@ __global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
   st->done = 0; st->iters = 0; st->k = 0; st->relative = relative; st->tol = tol;
   st->rz = st->pAp = st->alpha = st->beta = st->rr = st->bb = 0.0;
   st->rr0 = st->rr_guess = st->rr_alt = 0.0;
+  st->rr_true = -1.0;
   st->start = 0;
   st->tol2 = tol * tol;
 }
@@ -819,6 +821,41 @@ k_pcg_select(int n, const float* __restrict__ b, float* __restrict__ x, float* _
   else if (start == 2) { x[i] = x_alt[i]; ri = q[i]; r[i] = ri; }
   else ri = r[i];
   if (xv) xv[i] = (omega * dinv0[i]) * ri;
+}
+
+// FP64 residual of the FP32 iterate: products and sums in double over the FP32 A, x, b;
+// partial ||b - A x||^2 in slot 0
+__global__ void __launch_bounds__(kBlock)
+k_true_resid(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+             const float* __restrict__ val, const float* __restrict__ x,
+             const float* __restrict__ b, RedWS ws) {
+  double acc = 0.0;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const int p0 = sell_base(slice_ptr, i);
+    const int w = sell_width(slice_ptr, i);
+    double ax = 0.0;
+    for (int k0 = 0; k0 < w; k0 += 4) {
+      const int p = p0 + 64 * k0;
+      const v4i c = *reinterpret_cast<const v4i*>(idx + p);
+      const v4f v = *reinterpret_cast<const v4f*>(val + p);
+      ax = fma((double)v.x, (double)x[c.x], ax);
+      ax = fma((double)v.y, (double)x[c.y], ax);
+      ax = fma((double)v.z, (double)x[c.z], ax);
+      ax = fma((double)v.w, (double)x[c.w], ax);
+    }
+    if (i < n) {
+      const double ri = (double)b[i] - ax;
+      acc = ri * ri;
+    }
+  }
+  store_partial(ws, 0, block_sum(acc));
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fin_true(RedWS ws, int nparts, PcgState* st) {
+  const double t = sum_partials(ws, 0, nparts);
+  if (threadIdx.x == 0) st->rr_true = t;
 }
 
 // r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), partial b.b
@@ -1204,6 +1241,7 @@ struct mgpbd_ctx {
   int precision = MGPBD_PREC_FP32;   // V-cycle storage precision
   int flexible = 0;                   // Polak-Ribiere beta
   int poll = 4;
+  int verify = 0;   // 1: FP64 check of the converged x, restart from x when it fails; 2: check only
   // HIP graph of `poll` PCG iterations (+ the state copy), replayed on `work` stream
   bool use_graph = true;
   hipStream_t work = nullptr;
@@ -2082,64 +2120,105 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     }
   }
 
-  k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
   const bool fuse_first = first && !half_mode(ctx);
-  if (guess) {
-    // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
-    // start with the smallest residual
-    k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
-                                    b, 0.f, nullptr, nullptr);
-    if (x_alt)
-      k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x_alt, Ap,
-                                      nullptr, b, 0.f, nullptr, nullptr);
-    k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, x_alt ? Ap : nullptr, ctx->ws);
-    k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, x_alt != nullptr, ctx->st);
-    k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, x_alt, Ap, L.dinv,
-                                               fuse_first ? xv : nullptr, ctx->omega, wl(ctx, 0),
-                                               ctx->st);
-  } else {
-    k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr, ctx->omega,
-                                      wl(ctx, 0), ctx->ws);
-    k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  }
-  float* zr = pcg_precondition(ctx, r, xv, first);
-  k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
-  k_pcg_fin<FIN_RZ0><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  if (ctx->flexible) k_pcg_p_flex<<<nblocks(n), kBlock, 0, s>>>(n, zr, p, zold, ctx->st, 1);
-  else k_pcg_p<0><<<nblocks(n), kBlock, 0, s>>>(n, zr, p, ctx->st);
-  MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
-
-  int issued = 0;
-  while (issued < maxiter) {
-    if (graph_mode && maxiter - issued >= ctx->poll) {
-      MGPBD_HIP_TRY(hipGraphLaunch(ctx->graph, s));
-      issued += ctx->poll;
+  int issued = 0, total_iters = 0, restarts = 0;
+  bool from_guess = guess;
+  const float* alt = x_alt;
+  double first_info[4] = {0.0, 0.0, 0.0, 0.0};   // rr0, start, rr_guess, rr_alt of round 0
+  double rr_true = -1.0;
+  for (;;) {
+    k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
+    if (from_guess) {
+      // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
+      // start with the smallest residual
+      k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
+                                      b, 0.f, nullptr, nullptr);
+      if (alt)
+        k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, alt, Ap,
+                                        nullptr, b, 0.f, nullptr, nullptr);
+      k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, alt ? Ap : nullptr, ctx->ws);
+      k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, alt != nullptr, ctx->st);
+      k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, alt, Ap, L.dinv,
+                                                 fuse_first ? xv : nullptr, ctx->omega,
+                                                 wl(ctx, 0), ctx->st);
     } else {
-      int chunk = std::min(ctx->poll, maxiter - issued);
-      for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, zold, first, gn, gs, true);
-      MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
-      issued += chunk;
-      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+      k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr,
+                                        ctx->omega, wl(ctx, 0), ctx->ws);
+      k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
     }
-    MGPBD_HIP_TRY(hipStreamSynchronize(s));
-    if (int e = prof_flush(ctx)) return e;
-    if (ctx->st_host->done) break;
-  }
-  if (issued == 0) {
+    float* zr = pcg_precondition(ctx, r, xv, first);
+    k_pcg_rz<<<gn, kBlock, 0, s>>>(n, r, zr, ctx->ws, ctx->st);
+    k_pcg_fin<FIN_RZ0><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
+    if (ctx->flexible) k_pcg_p_flex<<<nblocks(n), kBlock, 0, s>>>(n, zr, p, zold, ctx->st, 1);
+    else k_pcg_p<0><<<nblocks(n), kBlock, 0, s>>>(n, zr, p, ctx->st);
+    MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
+
+    int round_issued = 0;
+    while (issued < maxiter) {
+      if (graph_mode && maxiter - issued >= ctx->poll) {
+        MGPBD_HIP_TRY(hipGraphLaunch(ctx->graph, s));
+        issued += ctx->poll;
+        round_issued += ctx->poll;
+      } else {
+        int chunk = std::min(ctx->poll, maxiter - issued);
+        for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, zold, first, gn, gs, true);
+        MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
+        issued += chunk;
+        round_issued += chunk;
+        MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+      }
+      MGPBD_HIP_TRY(hipStreamSynchronize(s));
+      if (int e = prof_flush(ctx)) return e;
+      if (ctx->st_host->done) break;
+    }
+    if (round_issued == 0) {
+      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+      MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (restarts == 0) {
+      const PcgState& h = *ctx->st_host;
+      first_info[0] = guess ? h.rr0 : h.bb;
+      first_info[1] = (double)h.start;
+      first_info[2] = h.rr_guess;
+      first_info[3] = h.rr_alt;
+    }
+    total_iters += ctx->st_host->iters;
+    if (!ctx->verify || (ctx->verify == 1 && ctx->st_host->done != 1)) break;
+    // FP64 residual of the returned x; a restart from x when it misses the tolerance
+    k_true_resid<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, b, ctx->ws);
+    k_fin_true<<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
+    MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(verify)");
     MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    rr_true = ctx->st_host->rr_true;
+    if (ctx->verify == 2 || ctx->st_host->done != 1 || rr_true < ctx->st_host->tol2 ||
+        issued >= maxiter || restarts >= 3)
+      break;
+    ++restarts;
+    from_guess = true;
+    alt = nullptr;
   }
   if (graph_mode) {
     MGPBD_HIP_TRY(hipEventRecord(ctx->ev_out, s));
     MGPBD_HIP_TRY(hipStreamWaitEvent(caller, ctx->ev_out, 0));
   }
-  if (iters_out) *iters_out = ctx->st_host->iters;
+  if (iters_out) *iters_out = total_iters;
   if (info_out) {
     const PcgState& h = *ctx->st_host;
-    const double v[8] = {h.rr, h.bb, (double)h.done, guess ? h.rr0 : h.bb, (double)h.start,
-                         h.rr_guess, h.rr_alt, 0.0};
-    for (int i = 0; i < n_info; ++i) info_out[i] = v[i];
+    double reason = (double)h.done;
+    if (ctx->verify == 1 && h.done == 1 && !(rr_true < h.tol2)) reason = 0.0;   // not verified
+    const double v4[4] = {h.rr, h.bb, reason, rr_true};
+    const double v8[8] = {h.rr, h.bb, reason, first_info[0], first_info[1], first_info[2],
+                          first_info[3], rr_true};
+    for (int i = 0; i < n_info; ++i) info_out[i] = n_info == 4 ? v4[i] : v8[i];
   }
+  return 0;
+}
+
+int mgpbd_pcg_set_verify(mgpbd_ctx* ctx, int enable) {
+  CTX_CHECK("mgpbd_pcg_set_verify");
+  if (enable < 0 || enable > 2) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_set_verify: 0, 1 or 2");
+  ctx->verify = enable;
   return 0;
 }
 

@@ -19,7 +19,9 @@ behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('reference' | 'extended'), ``galerkin`` ('values' | 'structural'), ``max_coarse``, ``max_levels``,
 ``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'),
 ``cycle`` ('V' | 'W'), ``overcorrect`` (scale of the coarse correction), ``smoother``
-('jacobi' | 'spectral': per-level weights from lambda_max(D^-1 A)), ``f_ext``
+('jacobi' | 'spectral': per-level weights from lambda_max(D^-1 A)), ``verify_residual``
+(1: FP64 check of every converged PCG solution, restart when it fails; 2: measure only),
+``f_ext``
 (external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
@@ -68,7 +70,7 @@ class MGPBDSolverGPU:
                  max_coarse: int = 400, max_levels: int | None = None,
                  coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
                  precision: str = "fp32", cycle: str = "V", overcorrect: float = 1.0,
-                 smoother: str = "jacobi",
+                 smoother: str = "jacobi", verify_residual: int = 0,
                  f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
@@ -113,6 +115,7 @@ class MGPBDSolverGPU:
         self.cycle = cycle
         self.overcorrect = float(overcorrect)
         self.smoother = smoother
+        self.verify_residual = int(verify_residual)
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -146,6 +149,7 @@ class MGPBDSolverGPU:
         self._amg_Ps = None
         self.last_stats = []
         self.first_solve_info = {}
+        self.last_solve_info = []
         self._drag = None
         if warm_start_model is not None and warm_start_model.n != nt:
             raise ValueError("warm_start_model was built for a different tet count")
@@ -164,7 +168,8 @@ class MGPBDSolverGPU:
             maxiter_cg=self.maxiter_cg, tol_cg=self.tol_cg, relative=self.relative_tol,
             coarse_sweeps=self.coarse_sweeps, galerkin=self.galerkin,
             coarse_solver=self.coarse_solver, precision=self.precision, cycle=self.cycle,
-            overcorrect=self.overcorrect, smoother=self.smoother, device=self.device.index)
+            overcorrect=self.overcorrect, smoother=self.smoother, verify=self.verify_residual,
+            device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()
         if self.verbose:
@@ -313,6 +318,7 @@ class MGPBDSolverGPU:
         self.lambdaf.zero_()
         r0 = 0.0
         stats = []
+        self.last_solve_info = []
         for ite in range(self.maxiter):
             self._compute_C_and_gradC()
             r = self._residual_norm()
@@ -332,8 +338,9 @@ class MGPBDSolverGPU:
                 self.record_samples.append(self.warm_start_model.make_sample(
                     self._b_gpu, self._A.diag_inv, self._lam_prev, self.constraints,
                     self.alpha_tilde, self.rest_vol, self._dlam_gpu, self._A))
+            self.last_solve_info.append(dict(self._gpu_solver.last_info, cg_iters=cg_iters))
             if ite == 0:
-                self.first_solve_info = dict(self._gpu_solver.last_info, cg_iters=cg_iters)
+                self.first_solve_info = self.last_solve_info[0]
             self._scatter_dlam_gpu()
             omega = self._line_search() if self.use_line_search else 1.0
             self._apply_dpos(omega)

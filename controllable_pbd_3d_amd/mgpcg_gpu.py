@@ -18,7 +18,9 @@ Extensions (keyword-only, no reference counterpart):
 * ``smoother='spectral'``: per-level Jacobi weights target / (safety lambda_max(D^-1 A)) from a
   device power iteration refreshed at every refill (the reference's fixed 2/3 makes the V-cycle
   indefinite once lambda_max(D^-1 A) > 3, which per-tet stiffness contrast produces);
-* ``cycle='W'`` and ``overcorrect``: W-cycle and a scaled coarse correction.
+* ``cycle='W'`` and ``overcorrect``: W-cycle and a scaled coarse correction;
+* ``verify=1`` (or True): a converged solve is checked with an FP64 residual of the returned x
+  and restarted from x when it misses the tolerance; ``verify=2`` only measures that residual.
 """
 
 from __future__ import annotations
@@ -63,7 +65,8 @@ class GPUMultigridSolver:
                  coarse_solver: str = "jacobi", precision: str = "fp32",
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
-                 smoother_safety: float = 1.1, power_iters: int = 4, device=None):
+                 smoother_safety: float = 1.1, power_iters: int = 4, verify: int = 0,
+                 device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
         self.maxiter_cg = int(maxiter_cg)
@@ -90,6 +93,7 @@ class GPUMultigridSolver:
         self.smoother_target = float(smoother_target)
         self.smoother_safety = float(smoother_safety)
         self.power_iters = int(power_iters)
+        self.verify = int(verify)   # 0 off, 1 verify + restart (True), 2 measure only
         self.last_lambda = None
         self.n_levels = 0
         self._initialized = False
@@ -216,6 +220,7 @@ class GPUMultigridSolver:
         ctx("mgpbd_mg_set_cycle", 2 if self.cycle == "W" else 1, self.overcorrect)
         ctx("mgpbd_mg_set_smoother", 1 if self.smoother == "spectral" else 0,
             self.smoother_target, self.smoother_safety, self.power_iters)
+        ctx("mgpbd_pcg_set_verify", int(self.verify))
         self._coarse_mode = mode
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None
@@ -279,7 +284,8 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_pcg_solve", dptr(b_field), dptr(x_field), dptr(self._pcg_work),
                  self.maxiter_cg, self.tol_cg, int(self.relative), ctypes.byref(it),
                  info.ctypes.data_as(_lib.F64P))
-        self.last_info = {"rr": float(info[0]), "bb": float(info[1]), "reason": int(info[2])}
+        self.last_info = {"rr": float(info[0]), "bb": float(info[1]), "reason": int(info[2]),
+                          "rr_true": float(info[3])}
         return int(it.value)
 
     def solve_guess(self, b_field: torch.Tensor, x_field: torch.Tensor,
@@ -295,5 +301,6 @@ class GPUMultigridSolver:
                  ctypes.byref(it), info.ctypes.data_as(_lib.F64P))
         self.last_info = {"rr": float(info[0]), "bb": float(info[1]), "reason": int(info[2]),
                           "rr0": float(info[3]), "start": int(info[4]),
-                          "rr_guess": float(info[5]), "rr_alt": float(info[6])}
+                          "rr_guess": float(info[5]), "rr_alt": float(info[6]),
+                          "rr_true": float(info[7])}
         return int(it.value)

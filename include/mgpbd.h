@@ -296,7 +296,8 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
  * 261-329: stop when ||r||^2 < tol^2, breakdown on |pAp| < 1e-30 and on the previous
  * |rz| < 1e-30). relative != 0 (extension) stops on ||r|| < tol ||b||. work_dev: 4*n floats
  * (r, z, p, Ap). Synchronises; *iters_out = completed iterations (reference count);
- * info_out (host, 4 doubles) = {||r||^2, ||b||^2, stop reason, 0}. */
+ * info_out (host, 4 doubles) = {||r||^2, ||b||^2, stop reason, FP64 ||b - A x||^2 when
+ * verification is on, else -1}. */
 int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* work_dev,
                     int maxiter, double tol, int relative, int* iters_out, double* info_out);
 /* PCG launch mode (no reference counterpart): with enable != 0 (default) the iterations run
@@ -304,6 +305,13 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* wor
  * context stream; the host checks the done flag after every `poll` iterations (default 4).
  * Iterations past convergence are skipped on the device by the done flag. */
 int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll);
+/* Converged-solution verification (extension; the reference stops on the recurrence residual,
+ * cons/mgpcg_gpu.py:306-309): with enable != 0, a solve whose recurrence residual met the
+ * tolerance computes ||b - A x|| with FP64 products and sums; if that misses the tolerance the
+ * PCG restarts from x (at most 3 times, within maxiter) and the stop reason is 0 when it never
+ * verifies. Iteration counts include the restarts. enable = 2 computes the FP64 residual of
+ * every returned x (converged or not) without restarting. */
+int mgpbd_pcg_set_verify(mgpbd_ctx* ctx, int enable);
 
 /* ---- Warm start (extension; no reference counterpart: the reference zeroes lambda every
  * substep, cons/mgpbd_gpu.py:402, and starts every PCG from dlambda = 0,
@@ -315,7 +323,8 @@ int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll);
  * mgpbd_pcg_solve. The stopping test is that of mgpbd_pcg_solve (||r|| < tol, or
  * ||r|| < tol ||b|| with relative != 0), so every start reaches the same tolerance.
  * info_out (host, 8 doubles) = {||r||^2, ||b||^2, stop reason, ||r0||^2 of the start taken,
- * start (0 zero, 1 guess, 2 alternative), ||b - A x_guess||^2, ||b - A x_alt||^2, 0}. */
+ * start (0 zero, 1 guess, 2 alternative), ||b - A x_guess||^2, ||b - A x_alt||^2, FP64
+ * ||b - A x||^2 when verification is on (mgpbd_pcg_set_verify), else -1}. */
 int mgpbd_pcg_solve_guess(mgpbd_ctx* ctx, const float* b_dev, float* x_dev,
                           const float* x_alt_dev, float* work_dev, int maxiter, double tol,
                           int relative, int* iters_out, double* info_out);

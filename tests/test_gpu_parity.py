@@ -525,3 +525,24 @@ def test_spectral_pcg_converges_where_fixed_jacobi_stalls(stiff_scene, precision
     it, rel, reason = res["spectral"]
     assert reason == 1 and it < 100 and rel < 2e-6
     assert res["jacobi"][2] != 1 and res["jacobi"][1] > 1e-4
+
+
+def test_verified_pcg_meets_the_tolerance_in_fp64(stiff_scene):
+    """With verification on, the returned x meets relative 1e-6 in an FP64 residual computed
+    by numpy from the same FP32 A and b, and the library's own FP64 residual agrees with it."""
+    mesh, mu_t, v0 = stiff_scene
+    s = _solver(mesh, v0, mu_per_tet=mu_t, smoother="spectral", relative_tol=True, tol_cg=1e-6,
+                maxiter_cg=200, hierarchy="extended", galerkin="structural", max_coarse=64,
+                coarse_solver="cholesky", precision="fp16", verify_residual=True)
+    A = sp.csr_matrix((s._A.csr_data().astype(np.float64), s._A_indices, s._A_indptr),
+                      shape=(mesh.n_tets,) * 2)
+    b = s._b_gpu.cpu().numpy().astype(np.float64)
+    x = torch.zeros_like(s._b_gpu)
+    it = s._gpu_solver.solve(s._b_gpu, x)
+    info = s._gpu_solver.last_info
+    r = b - A @ x.cpu().numpy().astype(np.float64)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"[verify] {it} it, reason {info['reason']}, FP64 rel {rel:.3e}, library "
+          f"{np.sqrt(info['rr_true'] / info['bb']):.3e}", flush=True)
+    assert info["reason"] == 1 and rel < 1e-6
+    assert abs(info["rr_true"] - r @ r) <= 1e-6 * (r @ r)
