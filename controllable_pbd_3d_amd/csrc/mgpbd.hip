@@ -289,7 +289,10 @@ __device__ __forceinline__ float a_entry(int i, int j, const int (&vi)[4], const
   }
   const int4 tj = t_i[j];
   const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
-  const float* gj = gradC + 12 * (size_t)j;
+  // the 48 bytes of gradC_j in three 16-byte loads
+  const float4* g4 = reinterpret_cast<const float4*>(gradC) + 3 * (size_t)j;
+  const float4 q0 = g4[0], q1 = g4[1], q2 = g4[2];
+  const float gj[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
   float v = 0.0f;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -612,6 +615,24 @@ k_dense_from_sell(int n, const int* __restrict__ slice_ptr, const int* __restric
     const float v = val[p];
     if (v != 0.0f) D[(size_t)idx[p] * n + i] += (double)v;   // column idx, row i
   }
+}
+
+// R = I (column-major n x n)
+__global__ void __launch_bounds__(kBlock)
+k_set_identity(int n, double* __restrict__ R) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  R[t] = (t % n == t / n) ? 1.0 : 0.0;
+}
+
+// partial sums of squares of len doubles (slot 0)
+__global__ void __launch_bounds__(kBlock)
+k_sumsq(int64_t len, const double* __restrict__ x, RedWS ws) {
+  double acc = 0.0;
+  for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < len;
+       t += (int64_t)gridDim.x * kBlock)
+    acc += x[t] * x[t];
+  store_partial(ws, 0, block_sum(acc));
 }
 
 // copy the lower triangle (column-major) onto the upper one
@@ -1058,7 +1079,11 @@ k_half_scales(int n, const int* __restrict__ diag_pos, const float* __restrict__
   isd[i] = 1.0f / r;
 }
 
-// hval = half(val_ij isd_i isd_j) over the whole padded row; col16 = j - i (padding: 0)
+// hval = half(val_ij isd_i isd_j) over the whole padded row, four entries (one 16-byte index
+// and value load, one 8-byte store) at a time; col16 = j - i (padding: 0) when col16 != NULL
+// (the pattern is fixed, so the deltas are written once).
+typedef uint16_t v4u16 __attribute__((ext_vector_type(4)));
+typedef int16_t v4s16 __attribute__((ext_vector_type(4)));
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
@@ -1069,13 +1094,26 @@ k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
   if ((i >> 6) >= n_slices) return;
   const int p0 = sell_base(slice_ptr, i);
   const int w = sell_width(slice_ptr, i);
-  const float si = i < n ? isd[i] : 0.0f;
-  for (int k = 0; k < w; ++k) {
-    const int p = p0 + 64 * (k & ~3) + (k & 3);
-    const int j = idx[p];
-    const float v = i < n ? val[p] * si * isd[j] : 0.0f;
-    hval[p] = f2h<VT>(v);
-    if (col16) col16[p] = (int16_t)(i < n ? j - i : 0);
+  const bool row = i < n;
+  const float si = row ? isd[i] : 0.0f;
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const int p = p0 + 64 * k0;
+    const v4i c = *reinterpret_cast<const v4i*>(idx + p);
+    const v4f v = *reinterpret_cast<const v4f*>(val + p);
+    v4u16 h;
+    h.x = f2h<VT>(row ? v.x * si * isd[c.x] : 0.0f);
+    h.y = f2h<VT>(row ? v.y * si * isd[c.y] : 0.0f);
+    h.z = f2h<VT>(row ? v.z * si * isd[c.z] : 0.0f);
+    h.w = f2h<VT>(row ? v.w * si * isd[c.w] : 0.0f);
+    *reinterpret_cast<v4u16*>(hval + p) = h;
+    if (col16) {
+      v4s16 d;
+      d.x = (int16_t)(row ? c.x - i : 0);
+      d.y = (int16_t)(row ? c.y - i : 0);
+      d.z = (int16_t)(row ? c.z - i : 0);
+      d.w = (int16_t)(row ? c.w - i : 0);
+      *reinterpret_cast<v4s16*>(col16 + p) = d;
+    }
   }
 }
 
@@ -1201,7 +1239,8 @@ struct MgLevel {
   const int* agg = nullptr;
   const int* mptr = nullptr;
   const int* mem = nullptr;
-  float* work = nullptr;   // b, x, x', r, each 64 * n_slices long (padded rows addressable)
+  float* work = nullptr;   // b, x, x', r, two W-cycle and two power-iteration vectors, each
+                           // 64 * n_slices long (padded rows addressable)
   int diagonal = 0;
   // reduced-precision copy for the V-cycle (MGPBD_PREC_FP16 / _BF16): values of the scaled
   // level D^-1/2 A D^-1/2 in the SELL positions, optional 16-bit column deltas
@@ -1210,6 +1249,7 @@ struct MgLevel {
   float* sd = nullptr;      // sqrt(A_ii)
   float* isd = nullptr;     // 1 / sqrt(A_ii)
   const int* diag_pos = nullptr;
+  bool col16_ready = false;  // the deltas depend on the pattern only: written once
   float* wb(int k) const { return work + (size_t)k * 64 * n_slices; }
 };
 
@@ -1226,6 +1266,14 @@ struct mgpbd_ctx {
   size_t dense_cap = 0;
   rocblas_int* dense_info = nullptr;
   bool dense_ready = false;
+  // Newton-Schulz refresh of the coarsest inverse from the previous one:
+  // X <- X + X (I - A X) while ||I - A X||_F < 0.3, else a new Cholesky inverse
+  bool ns_update = true;
+  double* dA = nullptr;      // assembled coarsest matrix
+  double* dR = nullptr;      // I - A X
+  double* dX2 = nullptr;     // next X
+  double* dnorm = nullptr;   // ||R||_F^2 (device)
+  int ns_steps = 0, ns_refactors = 0;
   float omega = 2.0f / 3.0f;
   int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
   float overcorrect = 1.0f;   // scale of the prolongated coarse correction
@@ -1341,6 +1389,10 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (c->blas) (void)rocblas_destroy_handle(c->blas);
   if (c->dense) (void)hipFree(c->dense);
   if (c->dense_info) (void)hipFree(c->dense_info);
+  if (c->dA) (void)hipFree(c->dA);
+  if (c->dR) (void)hipFree(c->dR);
+  if (c->dX2) (void)hipFree(c->dX2);
+  if (c->dnorm) (void)hipFree(c->dnorm);
   (void)hipFree(c->ws.partials);
   (void)hipFree(c->st);
   (void)hipHostFree(c->st_host);
@@ -1684,34 +1736,83 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
   MgLevel& L = ctx->L[ctx->n_levels - 1];
   const int n = L.n;
   const size_t need = (size_t)n * n;
+  hipStream_t s = ctx->stream;
   if (need > ctx->dense_cap) {
-    if (ctx->dense) MGPBD_HIP_TRY(hipFree(ctx->dense));
-    ctx->dense = nullptr;
-    MGPBD_HIP_TRY(hipMalloc(&ctx->dense, need * sizeof(double)));
+    for (double** b : {&ctx->dense, &ctx->dA, &ctx->dR, &ctx->dX2}) {
+      if (*b) MGPBD_HIP_TRY(hipFree(*b));
+      *b = nullptr;
+    }
+    for (double** b : {&ctx->dense, &ctx->dA, &ctx->dR, &ctx->dX2})
+      MGPBD_HIP_TRY(hipMalloc(b, need * sizeof(double)));
     ctx->dense_cap = need;
+    ctx->dense_ready = false;
   }
   if (!ctx->dense_info) MGPBD_HIP_TRY(hipMalloc(&ctx->dense_info, sizeof(rocblas_int)));
+  if (!ctx->dnorm) MGPBD_HIP_TRY(hipMalloc(&ctx->dnorm, sizeof(double)));
   if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
     return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_create_handle failed");
-  if (rocblas_set_stream(ctx->blas, ctx->stream) != rocblas_status_success)
+  if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success)
     return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_set_stream failed");
-  k_dense_from_sell<<<nblocks(n), kBlock, 0, ctx->stream>>>(n, L.slice_ptr, L.idx, L.val, ctx->dense);
+  k_dense_from_sell<<<nblocks(n), kBlock, 0, s>>>(n, L.slice_ptr, L.idx, L.val, ctx->dA);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(assemble)");
-  if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
-      rocblas_status_success)
-    return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotrf failed to launch");
-  rocblas_int info = 0;
-  MGPBD_HIP_TRY(hipMemcpyAsync(&info, ctx->dense_info, sizeof(info), hipMemcpyDeviceToHost, ctx->stream));
-  MGPBD_HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (info != 0)
-    return mgpbd_fail(MGPBD_ENONFINITE, "coarsest level is not positive definite (dpotrf info " +
-                                            std::to_string(info) + ")");
-  if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
-      rocblas_status_success)
-    return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotri failed to launch");
-  k_symmetrize_lower<<<nblocks((int64_t)n * n), kBlock, 0, ctx->stream>>>(n, ctx->dense);
-  MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(symmetrize)");
+  bool refactor = !ctx->dense_ready || !ctx->ns_update;
+  if (!refactor) {
+    const int gq = std::min(nblocks((int64_t)need), 1024);
+    for (int step = 0; step < 2; ++step) {
+      // R = I - A X
+      k_set_identity<<<nblocks((int64_t)need), kBlock, 0, s>>>(n, ctx->dR);
+      const double m1 = -1.0, p1 = 1.0;
+      if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &m1,
+                        ctx->dA, n, ctx->dense, n, &p1, ctx->dR, n) != rocblas_status_success)
+        return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_dgemm failed to launch");
+      k_sumsq<<<gq, kBlock, 0, s>>>((int64_t)need, ctx->dR, ctx->ws);
+      k_fin_sum<<<1, kBlock, 0, s>>>(ctx->ws, gq, 0, 1, ctx->dnorm);
+      MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(residual)");
+      double f2 = 0.0;
+      MGPBD_HIP_TRY(hipMemcpyAsync(&f2, ctx->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+      MGPBD_HIP_TRY(hipStreamSynchronize(s));
+      if (!(f2 < 0.09)) { refactor = true; break; }   // ||R||_F >= 0.3 (or not finite)
+      if (f2 < 1e-12) break;
+      // X2 = X + X R, then X <- X2
+      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dX2, ctx->dense, need * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+      if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &p1,
+                        ctx->dense, n, ctx->dR, n, &p1, ctx->dX2, n) != rocblas_status_success)
+        return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_dgemm failed to launch");
+      // copied back rather than swapped: captured PCG graphs hold the address of ctx->dense
+      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, ctx->dX2, need * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+      ++ctx->ns_steps;
+    }
+  }
+  if (refactor) {
+    ++ctx->ns_refactors;
+    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, ctx->dA, need * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s));
+    if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
+        rocblas_status_success)
+      return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotrf failed to launch");
+    rocblas_int info = 0;
+    MGPBD_HIP_TRY(hipMemcpyAsync(&info, ctx->dense_info, sizeof(info), hipMemcpyDeviceToHost, s));
+    MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    if (info != 0) {
+      ctx->dense_ready = false;
+      return mgpbd_fail(MGPBD_ENONFINITE, "coarsest level is not positive definite (dpotrf info " +
+                                              std::to_string(info) + ")");
+    }
+    if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
+        rocblas_status_success)
+      return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotri failed to launch");
+    k_symmetrize_lower<<<nblocks((int64_t)n * n), kBlock, 0, s>>>(n, ctx->dense);
+    MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(symmetrize)");
+  }
   ctx->dense_ready = true;
+  return 0;
+}
+
+int mgpbd_mg_set_coarse_update(mgpbd_ctx* ctx, int newton_schulz) {
+  CTX_CHECK("mgpbd_mg_set_coarse_update");
+  ctx->ns_update = newton_schulz != 0;
   return 0;
 }
 
@@ -1964,6 +2065,7 @@ int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, int16_t* 
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_half: invalid arguments");
   MgLevel& L = ctx->L[level];
   L.hval = hval; L.col16 = col16; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
+  L.col16_ready = false;
   if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   return 0;
 }
@@ -1977,10 +2079,12 @@ int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
     if (!L.hval) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
     k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
     const int g = nblocks(64LL * L.n_slices);
+    int16_t* cols = L.col16_ready ? nullptr : L.col16;
     if (ctx->precision == MGPBD_PREC_FP16)
-      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.col16);
+      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, cols);
     else
-      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.col16);
+      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, cols);
+    L.col16_ready = true;
   }
   MGPBD_LAUNCH_CHECK("mgpbd_mg_convert_half");
   return 0;

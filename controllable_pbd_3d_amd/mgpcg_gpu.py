@@ -65,7 +65,7 @@ class GPUMultigridSolver:
                  coarse_solver: str = "jacobi", precision: str = "fp32",
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
-                 smoother_safety: float = 1.1, power_iters: int = 4, verify: int = 0,
+                 smoother_safety: float = 1.1, power_iters: int = 2, verify: int = 0,
                  device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
