@@ -164,7 +164,7 @@ def make_solver(mesh, mu_t, cfg, kw, model=None, record=None):
 
 def train_learned(cfg, seeds, substeps, epochs):
     """Warm-start network trained on first-Newton-iteration pairs the solver produces on
-    other seeds of the same workload (mesh jitter and per-tet E differ), residual loss."""
+    other seeds of the same workload (mesh jitter and per-tet E differ), A-norm loss."""
     import torch
     from controllable_pbd_3d_amd.warm_start import LearnedWarmStart
     t0 = time.perf_counter()
@@ -184,11 +184,11 @@ def train_learned(cfg, seeds, substeps, epochs):
     gen_s = time.perf_counter() - t0
     t1 = time.perf_counter()
     before = model.evaluate(samples)
-    hist = model.train(samples, epochs=epochs, loss="residual", seed=0)
+    hist = model.train(samples, epochs=epochs, loss="energy", seed=0)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t1
     info = {"train_seeds": list(seeds), "substeps_per_seed": substeps, "samples": len(samples),
-            "epochs": epochs, "residual_loss_untrained": before, "residual_loss_epochs": hist,
+            "epochs": epochs, "energy_loss_untrained": before, "energy_loss_epochs": hist,
             "generate_s": gen_s, "train_s": train_s}
     del samples
     torch.cuda.empty_cache()

@@ -463,6 +463,31 @@ __global__ void k_resid_fin(int nblk, const double* __restrict__ partial,
   if (loss) *loss = rr * inv;
 }
 
+// e = dlam - target (into e)
+__global__ void __launch_bounds__(kBlock)
+k_energy_err(int n, const float* __restrict__ dlam, const float* __restrict__ target,
+             float* __restrict__ e) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) e[i] = dlam[i] - target[i];
+}
+
+// partial sums of a.b (one per block) into partial[slot + 2 * block]
+__global__ void __launch_bounds__(kBlock)
+k_pair_dot(int n, const float* __restrict__ a, const float* __restrict__ b,
+           double* __restrict__ partial, int slot) {
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    acc += (double)a[i] * b[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[2 * blockIdx.x + slot] = sh[0];
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_resid_dy(int n, const float* __restrict__ ar, const double* __restrict__ scal,
            float* __restrict__ dy) {
@@ -604,6 +629,31 @@ int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
   if (int e = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, r, ar)) return e;
   k_resid_dy<<<nb(n), kBlock, 0, s>>>(n, ar, W.scal, dy);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_loss_residual");
+  return 0;
+}
+
+int mgpbd_gnn_loss_energy(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
+                          const int32_t* sell_idx, const float* sell_val, const float* target,
+                          const float* dlam, const float* stats, float* tmp, void* ws, float* dy,
+                          double* loss_out) {
+  if (!ctx || n <= 0 || !slice_ptr || !sell_idx || !sell_val || !target || !dlam || !stats ||
+      !tmp || !ws || !dy)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_loss_energy: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  float* e = tmp;
+  float* ae = tmp + n;
+  const int gl = std::min(nb(n), 1024);
+  k_energy_err<<<nb(n), kBlock, 0, s>>>(n, dlam, target, e);
+  if (int r = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, e, ae)) return r;
+  k_pair_dot<<<gl, kBlock, 0, s>>>(n, e, ae, W.dpartial, 0);          // e.Ae
+  if (int r = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, target, e)) return r;
+  k_pair_dot<<<gl, kBlock, 0, s>>>(n, target, e, W.dpartial, 1);     // t.At
+  // same finalisation as the residual loss: scal = {e.Ae / t.At, 2 s / t.At}
+  k_resid_fin<<<1, 64, 0, s>>>(gl, W.dpartial, stats, W.scal, loss_out);
+  k_resid_dy<<<nb(n), kBlock, 0, s>>>(n, ae, W.scal, dy);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_loss_energy");
   return 0;
 }
 

@@ -119,8 +119,8 @@ class LearnedWarmStart:
     # ------------------------------------------------------------------ training
     def make_sample(self, b, dinv, lam_prev, C, alpha, vol, target, matrix=None) -> dict:
         """One training pair from a solved first Newton iteration: features, scale, previous
-        lambda, converged dlambda and (for the residual loss) the right-hand side and a copy
-        of the level-0 values of A (``matrix``: the GPUSparseMatrix of that solve)."""
+        lambda, converged dlambda and (for the energy and residual losses) the right-hand side
+        and a copy of the level-0 values of A (``matrix``: the GPUSparseMatrix of that solve)."""
         x = torch.empty(self.n, FEATURES, dtype=torch.float32, device=self.device)
         st = torch.empty(8, dtype=torch.float32, device=self.device)
         self.features(b, dinv, lam_prev, C, alpha, vol, x, st)
@@ -130,10 +130,7 @@ class LearnedWarmStart:
                      sell_idx=matrix.sell_idx)
         return s
 
-    def loss_and_grad(self, sample: dict, loss: str = "residual"):
-        """Forward, loss and backward for one sample; the gradient lands in ``self.grad`` and
-        the loss in ``self.loss`` (device)."""
-        self.forward(sample["x"], sample["stats"], sample["lam_prev"], self.dlam)
+    def _loss(self, sample: dict, loss: str):
         if loss == "mse":
             self.ctx("mgpbd_gnn_loss_mse", self.n, dptr(sample["stats"]),
                      dptr(sample["lam_prev"]), dptr(sample["target"]), dptr(self.ws),
@@ -143,8 +140,21 @@ class LearnedWarmStart:
                      dptr(sample["sell_idx"]), dptr(sample["val"]), dptr(sample["b"]),
                      dptr(self.dlam), dptr(sample["stats"]), dptr(self.tmp), dptr(self.ws),
                      dptr(self.dy), dptr(self.loss))
+        elif loss == "energy":
+            self.ctx("mgpbd_gnn_loss_energy", self.n, dptr(sample["slice_ptr"]),
+                     dptr(sample["sell_idx"]), dptr(sample["val"]), dptr(sample["target"]),
+                     dptr(self.dlam), dptr(sample["stats"]), dptr(self.tmp), dptr(self.ws),
+                     dptr(self.dy), dptr(self.loss))
         else:
-            raise ValueError("loss must be 'residual' or 'mse'")
+            raise ValueError("loss must be 'energy', 'residual' or 'mse'")
+
+    def loss_and_grad(self, sample: dict, loss: str = "energy"):
+        """Forward, loss and backward for one sample; the gradient lands in ``self.grad`` and
+        the loss in ``self.loss`` (device). 'energy' (default) is the A-norm error of the start
+        relative to the converged dlambda, the quantity that bounds CG's iteration count;
+        'residual' is ||A dlambda - b||^2 / ||b||^2; 'mse' the plain squared error."""
+        self.forward(sample["x"], sample["stats"], sample["lam_prev"], self.dlam)
+        self._loss(sample, loss)
         self.ctx("mgpbd_gnn_backward", self.n, dptr(self.nbr), dptr(sample["x"]),
                  dptr(self.params), dptr(self.w), dptr(self.dy), dptr(self.ws), dptr(self.grad))
         return self.loss
@@ -156,7 +166,7 @@ class LearnedWarmStart:
                  dptr(self.v), self.lr, b1, b2, self.eps, self.step_count, dptr(self.wt),
                  dptr(self.w))
 
-    def train(self, samples, epochs: int = 1, loss: str = "residual", seed: int = 0,
+    def train(self, samples, epochs: int = 1, loss: str = "energy", seed: int = 0,
               lr_decay: float = 1.0):
         """Adam over the samples, one step per sample, shuffled per epoch; returns the mean
         loss of every epoch (one host sync per epoch)."""
@@ -172,19 +182,12 @@ class LearnedWarmStart:
             self.lr *= lr_decay
         return hist
 
-    def evaluate(self, samples, loss: str = "residual"):
+    def evaluate(self, samples, loss: str = "energy"):
         """Mean loss over samples without updating the parameters."""
         tot = 0.0
         for s in samples:
             self.forward(s["x"], s["stats"], s["lam_prev"], self.dlam)
-            if loss == "mse":
-                self.ctx("mgpbd_gnn_loss_mse", self.n, dptr(s["stats"]), dptr(s["lam_prev"]),
-                         dptr(s["target"]), dptr(self.ws), dptr(self.dy), dptr(self.loss))
-            else:
-                self.ctx("mgpbd_gnn_loss_residual", self.n, dptr(s["slice_ptr"]),
-                         dptr(s["sell_idx"]), dptr(s["val"]), dptr(s["b"]), dptr(self.dlam),
-                         dptr(s["stats"]), dptr(self.tmp), dptr(self.ws), dptr(self.dy),
-                         dptr(self.loss))
+            self._loss(s, loss)
             tot += float(self.loss.item())
         return tot / max(len(samples), 1)
 

@@ -375,13 +375,19 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float
  *   mse:      mean_i (y_i - (target_i - lam_prev_i) / s)^2
  *   residual: ||A dlam - b||^2 / ||b||^2 with dlam = the forward's dlam_out, A the level-0
  *             SELL-64x4 matrix of the same solve (tmp_dev: 2 n floats). This is the quantity
- *             mgpbd_pcg_solve_guess compares between starts. */
+ *             mgpbd_pcg_solve_guess compares between starts.
+ *   energy:   (dlam - t)^T A (dlam - t) / t^T A t with t = target (the converged dlambda): the
+ *             A-norm error that bounds CG's convergence from that start (tmp_dev: 2 n). */
 int mgpbd_gnn_loss_mse(mgpbd_ctx* ctx, int n, const float* stats_dev, const float* lam_prev_dev,
                        const float* target_dev, void* ws_dev, float* dy_dev, double* loss_dev);
 int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr_dev,
                             const int32_t* sell_idx_dev, const float* sell_val_dev,
                             const float* b_dev, const float* dlam_dev, const float* stats_dev,
                             float* tmp_dev, void* ws_dev, float* dy_dev, double* loss_dev);
+int mgpbd_gnn_loss_energy(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr_dev,
+                          const int32_t* sell_idx_dev, const float* sell_val_dev,
+                          const float* target_dev, const float* dlam_dev, const float* stats_dev,
+                          float* tmp_dev, void* ws_dev, float* dy_dev, double* loss_dev);
 /* Backward pass from dy_dev = d loss / d y: grad_dev gets d loss / d params (FP32,
  * param_count values, overwritten). */
 int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* x_dev,

@@ -1,6 +1,6 @@
 """ORACLE (test infrastructure only) -- FP64 numpy statement of the learned warm start:
-the message-passing network over the tet face-adjacency graph, its per-tet features, the two
-training losses, the hand-derived backward pass and the Adam update.
+the message-passing network over the tet face-adjacency graph, its per-tet features, the three
+training losses (squared error, residual, A-norm energy), the hand-derived backward pass and the Adam update.
 
 The network has no counterpart in the reference (it solves every PCG from dlambda = 0,
 cons/mgpbd_gpu.py:426 and cons/mgpcg_gpu.py:271-272); its definition is the build's own:
@@ -174,6 +174,16 @@ def loss_residual(A, b, dlam, stats):
     r = A @ np.asarray(dlam, np.float64) - b
     bb = float(b @ b)
     return float(r @ r) / bb, (2.0 * s / bb) * (A.T @ r)
+
+
+def loss_energy(A, target, dlam, stats):
+    """(dlam - t)^T A (dlam - t) / t^T A t and its gradient with respect to y."""
+    s = float(np.asarray(stats, np.float32)[0])
+    t = np.asarray(target, np.float64)
+    e = np.asarray(dlam, np.float64) - t
+    Ae = A @ e
+    tAt = float(t @ (A @ t))
+    return float(e @ Ae) / tAt, (2.0 * s / tAt) * Ae
 
 
 def backward(params, cache, dy, nbr, emulate=False):

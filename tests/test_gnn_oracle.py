@@ -67,10 +67,12 @@ def total_loss(P, x, nbr, stats, lp, kind, target=None, A=None, b=None):
     y, dlam, _ = go.forward(P, x, nbr, stats, lp)
     if kind == "mse":
         return go.loss_mse(y, target, lp, stats)[0]
+    if kind == "energy":
+        return go.loss_energy(A, target, dlam, stats)[0]
     return go.loss_residual(A, b, dlam, stats)[0]
 
 
-@pytest.mark.parametrize("kind", ["mse", "residual"])
+@pytest.mark.parametrize("kind", ["mse", "residual", "energy"])
 def test_backward_matches_finite_differences(kind):
     _, nbr = mesh_graph(2)
     nt = len(nbr)
@@ -83,6 +85,8 @@ def test_backward_matches_finite_differences(kind):
     y, dlam, cache = go.forward(P, x, nbr, stats, lp)
     if kind == "mse":
         _, dy = go.loss_mse(y, target, lp, stats)
+    elif kind == "energy":
+        _, dy = go.loss_energy(A, target, dlam, stats)
     else:
         _, dy = go.loss_residual(A, b, dlam, stats)
     g = go.backward(P, cache, dy, nbr)
