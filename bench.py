@@ -233,6 +233,8 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None):
            "solves": len(solves), "solves_converged": float(conv.mean()) if len(conv) else None,
            "fp64_rel_residual_max": float(rel.max()) if len(rel) else None,
            "fp64_rel_residual_median": float(np.median(rel)) if len(rel) else None}
+    ns, refac = s._gpu_solver.coarse_stats()
+    res["coarsest_inverse"] = {"newton_schulz_steps": ns, "cholesky_refactors": refac}
     if kw.get("warm_start"):
         # start of each substep's first solve: 0 zero, 1 the guess, 2 the previous lambda
         res["first_solve_starts"] = np.bincount(starts, minlength=3).tolist()

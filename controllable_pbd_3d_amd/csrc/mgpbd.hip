@@ -499,13 +499,14 @@ k_power_step(int n, int n_slices, const int* __restrict__ slice_ptr, const int* 
   store_partial(ws, 2, s2);
 }
 
-// pw[0] = lambda = v.Av / v.Dv, pw[1] = ||w||_D
+// pw[0] = lambda = v.Av / v.Dv, pw[1] = ||w||_D, pw[2] = the previous iteration's lambda
 __global__ void __launch_bounds__(kBlock)
 k_power_fin(RedWS ws, int nparts, double* __restrict__ pw) {
   const double vav = sum_partials(ws, 0, nparts);
   const double vdv = sum_partials(ws, 1, nparts);
   const double wdw = sum_partials(ws, 2, nparts);
   if (threadIdx.x != 0) return;
+  pw[2] = pw[0];
   pw[0] = vdv > 0.0 ? vav / vdv : 0.0;
   pw[1] = sqrt(wdw);
 }
@@ -530,11 +531,14 @@ k_power_init(int n, float* __restrict__ v) {
   v[i] = 0.5f + (float)(h & 0xFFFFu) * (1.0f / 65536.0f) - ((h >> 16) & 1u ? 1.0f : 0.0f);
 }
 
-// omega_l = target / (safety lambda), or the fixed weight when lambda is not usable
+// omega_l = target / (safety lambda); while the power iteration is still moving (the last two
+// Rayleigh quotients differ by more than 2 %) the estimate is raised by a quarter; the fixed
+// weight when lambda is not usable
 __global__ void k_power_omega(const double* __restrict__ pw, float target, float safety,
                               float fallback, float* __restrict__ omega_l) {
   if (threadIdx.x != 0) return;
-  const double lam = pw[0];
+  double lam = pw[0];
+  if (!(fabs(lam - pw[2]) <= 0.02 * lam)) lam *= 1.25;
   omega_l[0] = (lam > 1e-6 && lam < 1e30) ? (float)(target / (safety * lam)) : fallback;
 }
 
@@ -1266,14 +1270,15 @@ struct mgpbd_ctx {
   size_t dense_cap = 0;
   rocblas_int* dense_info = nullptr;
   bool dense_ready = false;
-  // Newton-Schulz refresh of the coarsest inverse from the previous one:
-  // X <- X + X (I - A X) while ||I - A X||_F < 0.3, else a new Cholesky inverse
+  // Newton-Schulz refresh of the coarsest inverse from the previous one: two steps
+  // X <- X + X (I - A X), kept when ||I - A X||_F < 0.1 afterwards, else a Cholesky inverse
   bool ns_update = true;
   double* dA = nullptr;      // assembled coarsest matrix
   double* dR = nullptr;      // I - A X
-  double* dX2 = nullptr;     // next X
+  double* dX2 = nullptr;     // X (Newton-Schulz iterate)
+  double* dT = nullptr;      // next X
   double* dnorm = nullptr;   // ||R||_F^2 (device)
-  int ns_steps = 0, ns_refactors = 0;
+  int64_t ns_steps = 0, ns_refactors = 0;
   float omega = 2.0f / 3.0f;
   int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
   float overcorrect = 1.0f;   // scale of the prolongated coarse correction
@@ -1283,8 +1288,10 @@ struct mgpbd_ctx {
   float sm_target = 4.0f / 3.0f, sm_safety = 1.1f;
   int power_iters = 4;
   float* omega_dev = nullptr;   // MGPBD_MAX_LEVELS
-  double* pw_dev = nullptr;     // 2 per level: lambda estimate, ||w||_D
+  double* pw_dev = nullptr;     // 4 per level: lambda estimate, ||w||_D, previous estimate
   bool pv_ready[MGPBD_MAX_LEVELS] = {};
+  int pv_updates[MGPBD_MAX_LEVELS] = {};   // updates since the level was set
+  double* pw_host = nullptr;    // pinned copy of pw_dev from the previous update (read lagged)
   MgLevel L[MGPBD_MAX_LEVELS];
   int precision = MGPBD_PREC_FP32;   // V-cycle storage precision
   int flexible = 0;                   // Polak-Ribiere beta
@@ -1357,7 +1364,9 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->omega_dev, sizeof(float) * MGPBD_MAX_LEVELS);
-  if (e == hipSuccess) e = hipMalloc(&c->pw_dev, sizeof(double) * 2 * MGPBD_MAX_LEVELS);
+  if (e == hipSuccess) e = hipMalloc(&c->pw_dev, sizeof(double) * 4 * MGPBD_MAX_LEVELS);
+  if (e == hipSuccess) e = hipHostMalloc(&c->pw_host, sizeof(double) * 4 * MGPBD_MAX_LEVELS, hipHostMallocDefault);
+  if (e == hipSuccess) std::memset(c->pw_host, 0, sizeof(double) * 4 * MGPBD_MAX_LEVELS);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     mgpbd_set_error(std::string("mgpbd_ctx_create: ") + hipGetErrorString(e));
@@ -1366,6 +1375,7 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->omega_dev) (void)hipFree(c->omega_dev);
     if (c->pw_dev) (void)hipFree(c->pw_dev);
+    if (c->pw_host) (void)hipHostFree(c->pw_host);
     delete c;
     return nullptr;
   }
@@ -1392,12 +1402,14 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (c->dA) (void)hipFree(c->dA);
   if (c->dR) (void)hipFree(c->dR);
   if (c->dX2) (void)hipFree(c->dX2);
+  if (c->dT) (void)hipFree(c->dT);
   if (c->dnorm) (void)hipFree(c->dnorm);
   (void)hipFree(c->ws.partials);
   (void)hipFree(c->st);
   (void)hipHostFree(c->st_host);
   (void)hipFree(c->omega_dev);
   (void)hipFree(c->pw_dev);
+  (void)hipHostFree(c->pw_host);
   delete c;
 }
 
@@ -1738,11 +1750,11 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
   const size_t need = (size_t)n * n;
   hipStream_t s = ctx->stream;
   if (need > ctx->dense_cap) {
-    for (double** b : {&ctx->dense, &ctx->dA, &ctx->dR, &ctx->dX2}) {
+    for (double** b : {&ctx->dense, &ctx->dA, &ctx->dR, &ctx->dX2, &ctx->dT}) {
       if (*b) MGPBD_HIP_TRY(hipFree(*b));
       *b = nullptr;
     }
-    for (double** b : {&ctx->dense, &ctx->dA, &ctx->dR, &ctx->dX2})
+    for (double** b : {&ctx->dense, &ctx->dA, &ctx->dR, &ctx->dX2, &ctx->dT})
       MGPBD_HIP_TRY(hipMalloc(b, need * sizeof(double)));
     ctx->dense_cap = need;
     ctx->dense_ready = false;
@@ -1757,32 +1769,39 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(assemble)");
   bool refactor = !ctx->dense_ready || !ctx->ns_update;
   if (!refactor) {
+    // two Newton-Schulz steps from the previous inverse, X <- X + X (I - A X), then an a
+    // posteriori check: ||I - A X||_F < 0.1 bounds the spectral radius of I - A X, so the
+    // kept X is SPD with every eigenvalue of X A within 0.1 of 1
     const int gq = std::min(nblocks((int64_t)need), 1024);
-    for (int step = 0; step < 2; ++step) {
-      // R = I - A X
+    const double m1 = -1.0, p1 = 1.0;
+    double* X = ctx->dX2;
+    double* T = ctx->dT;
+    MGPBD_HIP_TRY(hipMemcpyAsync(X, ctx->dense, need * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int step = 0; step <= 2; ++step) {
       k_set_identity<<<nblocks((int64_t)need), kBlock, 0, s>>>(n, ctx->dR);
-      const double m1 = -1.0, p1 = 1.0;
       if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &m1,
-                        ctx->dA, n, ctx->dense, n, &p1, ctx->dR, n) != rocblas_status_success)
+                        ctx->dA, n, X, n, &p1, ctx->dR, n) != rocblas_status_success)
         return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_dgemm failed to launch");
-      k_sumsq<<<gq, kBlock, 0, s>>>((int64_t)need, ctx->dR, ctx->ws);
-      k_fin_sum<<<1, kBlock, 0, s>>>(ctx->ws, gq, 0, 1, ctx->dnorm);
-      MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(residual)");
-      double f2 = 0.0;
-      MGPBD_HIP_TRY(hipMemcpyAsync(&f2, ctx->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
-      MGPBD_HIP_TRY(hipStreamSynchronize(s));
-      if (!(f2 < 0.09)) { refactor = true; break; }   // ||R||_F >= 0.3 (or not finite)
-      if (f2 < 1e-12) break;
-      // X2 = X + X R, then X <- X2
-      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dX2, ctx->dense, need * sizeof(double),
-                                   hipMemcpyDeviceToDevice, s));
+      if (step == 2) break;
+      MGPBD_HIP_TRY(hipMemcpyAsync(T, X, need * sizeof(double), hipMemcpyDeviceToDevice, s));
       if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &p1,
-                        ctx->dense, n, ctx->dR, n, &p1, ctx->dX2, n) != rocblas_status_success)
+                        X, n, ctx->dR, n, &p1, T, n) != rocblas_status_success)
         return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_dgemm failed to launch");
-      // copied back rather than swapped: captured PCG graphs hold the address of ctx->dense
-      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, ctx->dX2, need * sizeof(double),
+      std::swap(X, T);
+    }
+    k_sumsq<<<gq, kBlock, 0, s>>>((int64_t)need, ctx->dR, ctx->ws);
+    k_fin_sum<<<1, kBlock, 0, s>>>(ctx->ws, gq, 0, 1, ctx->dnorm);
+    MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(newton-schulz)");
+    double f2 = 0.0;
+    MGPBD_HIP_TRY(hipMemcpyAsync(&f2, ctx->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+    MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    if (f2 < 0.01) {
+      // copied rather than swapped: captured PCG graphs hold the address of ctx->dense
+      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, X, need * sizeof(double),
                                    hipMemcpyDeviceToDevice, s));
-      ++ctx->ns_steps;
+      ctx->ns_steps += 2;
+    } else {
+      refactor = true;   // also when the iteration diverged (f2 not finite)
     }
   }
   if (refactor) {
@@ -1807,6 +1826,13 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
     MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(symmetrize)");
   }
   ctx->dense_ready = true;
+  return 0;
+}
+
+int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors) {
+  CTX_CHECK("mgpbd_mg_coarse_stats");
+  if (newton_schulz_steps) *newton_schulz_steps = ctx->ns_steps;
+  if (refactors) *refactors = ctx->ns_refactors;
   return 0;
 }
 
@@ -2007,11 +2033,20 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
     int iters = ctx->power_iters;
     if (!ctx->pv_ready[l]) {
       k_power_init<<<nblocks(L.n), kBlock, 0, s>>>(L.n, v);
-      iters = std::max(iters, 12);
       ctx->pv_ready[l] = true;
+      ctx->pv_updates[l] = 0;
     }
+    // the first updates after a (re)set start far from the top eigenvector (at rest the
+    // level-0 matrix is diagonal): longer runs until the vector has followed the matrix; also
+    // after an update whose last two estimates still differed by more than 2 % (read from the
+    // pinned copy of that update, which the Newton loop's own synchronisations have completed)
+    const double* ph = ctx->pw_host + 4 * l;
+    const bool moving = !(std::fabs(ph[0] - ph[2]) <= 0.02 * ph[0]);
+    if (ctx->pv_updates[l] < 4) iters = std::max(iters, 12);
+    else if (moving) iters = std::max(iters, 8);
+    ++ctx->pv_updates[l];
     const int g = nblocks(64LL * L.n_slices);
-    double* pw = ctx->pw_dev + 2 * l;
+    double* pw = ctx->pw_dev + 4 * l;
     for (int it = 0; it < iters; ++it) {
       k_power_step<<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.dinv, v, w,
                                         ctx->ws);
@@ -2022,11 +2057,13 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
                                    ctx->omega_dev + l);
   }
   MGPBD_LAUNCH_CHECK("mgpbd_mg_update_smoother");
+  MGPBD_HIP_TRY(hipMemcpyAsync(ctx->pw_host, ctx->pw_dev, sizeof(double) * 4 * MGPBD_MAX_LEVELS,
+                               hipMemcpyDeviceToHost, s));
   if (lambda_out) {
-    double tmp[2 * MGPBD_MAX_LEVELS];
+    double tmp[4 * MGPBD_MAX_LEVELS];
     MGPBD_HIP_TRY(hipMemcpyAsync(tmp, ctx->pw_dev, sizeof(tmp), hipMemcpyDeviceToHost, s));
     MGPBD_HIP_TRY(hipStreamSynchronize(s));
-    for (int l = 0; l < ctx->n_levels; ++l) lambda_out[l] = tmp[2 * l];
+    for (int l = 0; l < ctx->n_levels; ++l) lambda_out[l] = tmp[4 * l];
   }
   return 0;
 }

@@ -65,7 +65,7 @@ class GPUMultigridSolver:
                  coarse_solver: str = "jacobi", precision: str = "fp32",
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
-                 smoother_safety: float = 1.1, power_iters: int = 2, verify: int = 0,
+                 smoother_safety: float = 1.1, power_iters: int = 4, verify: int = 0,
                  device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
@@ -251,6 +251,12 @@ class GPUMultigridSolver:
             self.ctx("mgpbd_mg_convert_half")
         if self.smoother == "spectral":
             self.ctx("mgpbd_mg_update_smoother", None)
+
+    def coarse_stats(self):
+        """(Newton-Schulz steps kept, Cholesky refactorizations) of the coarsest inverse."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx("mgpbd_mg_coarse_stats", ctypes.byref(a), ctypes.byref(b))
+        return int(a.value), int(b.value)
 
     def smoother_weights(self):
         """Jacobi weight of each level as the V-cycle uses it now (synchronises)."""

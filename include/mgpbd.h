@@ -255,10 +255,13 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
  * No reference counterpart (the reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
 /* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level first
- * tries up to two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse (rocBLAS
- * dgemm) and refactors only when ||I - A X||_F >= 0.3; X stays symmetric positive definite
- * with eigenvalues of X A within 0.3^2 of 1, so the V-cycle stays an SPD preconditioner. */
+ * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse (rocBLAS dgemm)
+ * and keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue of X A lies
+ * within 0.1 of 1 and the V-cycle stays an SPD preconditioner); otherwise, or when the
+ * iteration diverged, it refactors with Cholesky. */
 int mgpbd_mg_set_coarse_update(mgpbd_ctx* ctx, int newton_schulz);
+/* Counts since the context was created: Newton-Schulz steps kept, Cholesky refactorizations. */
+int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors);
 /* Cycle shape (extension; no reference counterpart: cons/mgpcg_gpu.py:214-259 is a V-cycle
  * adding the plain coarse correction). cycle 1 = V, 2 = W (each level below the finest that is
  * not the coarsest is cycled twice, the second time on its residual); overcorrect scales the

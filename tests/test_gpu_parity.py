@@ -546,3 +546,26 @@ def test_verified_pcg_meets_the_tolerance_in_fp64(stiff_scene):
           f"{np.sqrt(info['rr_true'] / info['bb']):.3e}", flush=True)
     assert info["reason"] == 1 and rel < 1e-6
     assert abs(info["rr_true"] - r @ r) <= 1e-6 * (r @ r)
+
+
+def test_coarsest_inverse_refresh_keeps_solves_verified():
+    """Across substeps the coarsest inverse is refreshed by Newton-Schulz steps from the
+    previous one (kept only when ||I - A X||_F < 0.1); every solve still meets the FP64-verified
+    1e-6, and the refresh is actually taken on most refills."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(8, seed=14, h=0.03)
+    s = _solver(mesh, None, mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended",
+                galerkin="structural", max_coarse=64, coarse_solver="cholesky",
+                relative_tol=True, tol_cg=1e-6, smoother="spectral", verify_residual=1,
+                precision="fp16")
+    infos = []
+    for k in range(6):
+        s.step(0)
+        infos += s.last_solve_info
+    ns, refac = s._gpu_solver.coarse_stats()
+    print(f"[coarse refresh] {len(infos)} solves, Newton-Schulz steps {ns}, refactors {refac}; "
+          f"(reason, iters, rel) {[(d['reason'], d['cg_iters'], float(np.sqrt(max(d['rr_true'], 0) / d['bb']))) for d in infos]}",
+          flush=True)
+    assert all(d["reason"] == 1 for d in infos)
+    assert all(d["rr_true"] < 1e-12 * d["bb"] for d in infos)
+    assert ns >= 2 * (len(infos) // 2)
