@@ -115,6 +115,19 @@ def cpu_baseline(mesh, mu_t, solver, newton_per_substep, pcg_per_substep, pcg_sa
                        f"{setup_s:.1f} s excluded; host has {os.cpu_count()} cores")}
 
 
+def measured_traffic(kind, precision):
+    """HBM bytes per launch of a level-0 kernel from the committed rocprofv3 counter summaries
+    (``bench-traffic: <kind> <precision> <bytes>`` lines in profiles/*.md), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "bench-traffic:" and parts[1] == kind and \
+                    parts[2] == precision:
+                return float(parts[3])
+    return None
+
+
 def kernel_stats(ctx, n, nnz, precision):
     from controllable_pbd_3d_amd import _lib
     ms = np.zeros(8, np.float64)
@@ -325,7 +338,9 @@ def main():
         k = mat[dom]
         out["roofline"] = {"bound": "hbm", "kernel": f"{dom} ({head})", "achieved": k["GBs"],
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": k["GBs"] / HBM_PEAK_GBS, "traffic": None}
+                           "frac": k["GBs"] / HBM_PEAK_GBS,
+                           "algorithmic_bytes": k["bytes"],
+                           "traffic": measured_traffic(dom, prec)}
     if args.cpu_baseline:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         ref = runs.get("fp32_reference", best)
