@@ -154,13 +154,14 @@ RUNS = {
     # the reference's algorithm (fixed 2/3-weighted Jacobi); the FP64 residual of every solve
     # is measured, not enforced
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
-    # spectral smoother weights, every solve FP64-verified to the tolerance
-    "fp32": dict(precision="fp32", smoother="spectral", verify_residual=1),
-    "fp16": dict(precision="fp16", smoother="spectral", verify_residual=1),
-    "bf16": dict(precision="bf16", smoother="spectral", verify_residual=1),
-    "fp16_previous": dict(precision="fp16", smoother="spectral", verify_residual=1,
+    # Chebyshev smoothing on the device's spectral estimates (same 2 + 2 sweeps), every solve
+    # FP64-verified to the tolerance
+    "fp32": dict(precision="fp32", smoother="chebyshev", verify_residual=1),
+    "fp16": dict(precision="fp16", smoother="chebyshev", verify_residual=1),
+    "bf16": dict(precision="bf16", smoother="chebyshev", verify_residual=1),
+    "fp16_previous": dict(precision="fp16", smoother="chebyshev", verify_residual=1,
                           warm_start="previous"),
-    "fp16_learned": dict(precision="fp16", smoother="spectral", verify_residual=1,
+    "fp16_learned": dict(precision="fp16", smoother="chebyshev", verify_residual=1,
                          warm_start="learned"),
 }
 

@@ -453,13 +453,16 @@ __device__ __forceinline__ float sell_row(int i, const int* __restrict__ slice_p
   return acc;
 }
 
-// MODE 0: y = A x;  MODE 1: y = x + (omega dinv) (b - A x);  MODE 2: y = b - A x
+// MODE 0: y = A x;  MODE 1: y = x + (omega dinv) (b - A x);  MODE 2: y = b - A x;
+// MODE 3 (Chebyshev step): y = x + c1 (x - xp) + (c2 dinv) (b - A x), (c1, c2) = cc[0..1],
+// xp = the previous iterate (NULL: zero)
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_sell(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
-       const float* __restrict__ val, const float* __restrict__ x, float* __restrict__ y,
+       const float* __restrict__ val, const float* __restrict__ x, float* y,
        const float* __restrict__ dinv, const float* __restrict__ b, float omega, const float* __restrict__ wp,
-       const int* __restrict__ done) {
+       const int* __restrict__ done, const float* xp = nullptr,
+       const float* __restrict__ cc = nullptr) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -468,7 +471,12 @@ k_sell(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __rest
   if (i >= n) return;
   if (MODE == 0) y[i] = ax;
   else if (MODE == 1) y[i] = fmaf(omega * dinv[i], b[i] - ax, x[i]);
-  else y[i] = b[i] - ax;
+  else if (MODE == 2) y[i] = b[i] - ax;
+  else {
+    const float xi = x[i];
+    const float u = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
+    y[i] = fmaf(cc[1] * dinv[i], b[i] - ax, u);
+  }
 }
 
 // Power iteration on D^-1 A (symmetric in the D inner product): w = D^-1 A v and the
@@ -531,15 +539,40 @@ k_power_init(int n, float* __restrict__ v) {
   v[i] = 0.5f + (float)(h & 0xFFFFu) * (1.0f / 65536.0f) - ((h >> 16) & 1u ? 1.0f : 0.0f);
 }
 
-// omega_l = target / (safety lambda); while the power iteration is still moving (the last two
-// Rayleigh quotients differ by more than 2 %) the estimate is raised by a quarter; the fixed
-// weight when lambda is not usable
+// Level weights from the power-iteration estimate (raised by a quarter while the iteration is
+// still moving: the last two Rayleigh quotients differ by more than 2 %).
+//  Jacobi:    omega_l = target / (safety lambda).
+//  Chebyshev: on [a, b] = [safety lambda / ratio, safety lambda], theta = (b + a) / 2,
+//             delta = (b - a) / 2, sigma = theta / delta: the first step is Jacobi with
+//             omega_l = 1 / theta, step k >= 2 uses c1 = rho_k rho_{k-1}, c2 = 2 rho_k / delta with
+//             rho_1 = 1 / sigma, rho_k = 1 / (2 sigma - rho_{k-1}).
+// Not usable lambda: the fixed weight (and Chebyshev steps that reduce to it, c1 = 0).
 __global__ void k_power_omega(const double* __restrict__ pw, float target, float safety,
-                              float fallback, float* __restrict__ omega_l) {
+                              float fallback, int cheb, float ratio, float* __restrict__ omega_l,
+                              float* __restrict__ cheb_l) {
   if (threadIdx.x != 0) return;
   double lam = pw[0];
   if (!(fabs(lam - pw[2]) <= 0.02 * lam)) lam *= 1.25;
-  omega_l[0] = (lam > 1e-6 && lam < 1e30) ? (float)(target / (safety * lam)) : fallback;
+  const bool ok = lam > 1e-6 && lam < 1e30;
+  if (!cheb) {
+    omega_l[0] = ok ? (float)(target / (safety * lam)) : fallback;
+    return;
+  }
+  if (!ok) {
+    omega_l[0] = fallback;
+    for (int k = 0; k < 8; ++k) { cheb_l[2 * k] = 0.0f; cheb_l[2 * k + 1] = fallback; }
+    return;
+  }
+  const double b = safety * lam, a = b / ratio;
+  const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+  omega_l[0] = (float)(1.0 / theta);
+  double rho = 1.0 / sigma;
+  for (int k = 0; k < 8; ++k) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    cheb_l[2 * k] = (float)(rn * rho);
+    cheb_l[2 * k + 1] = (float)(2.0 * rn / delta);
+    rho = rn;
+  }
 }
 
 __global__ void k_set_float(float* __restrict__ x, float v) {
@@ -1049,13 +1082,16 @@ __device__ __forceinline__ float sellh_row(int i, const int* __restrict__ slice_
   return acc;
 }
 
-// MODE 1: y = x + omega (b - A^x);  MODE 2: y = b - A^x;  MODE 3: y = isd (x + omega (b - A^x))
+// MODE 1: y = x + omega (b - A^x);  MODE 2: y = b - A^x;  MODE 3: y = isd (x + omega (b - A^x));
+// MODE 4 (Chebyshev step, unit diagonal): y = x + c1 (x - xp) + c2 (b - A^x);  MODE 5: isd times
+// the MODE 4 value
 template <int VT, bool DELTA, int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
         const int16_t* __restrict__ col16, const uint16_t* __restrict__ hval,
-        const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ b,
-        float omega, const float* __restrict__ wp, const float* __restrict__ isd, const int* __restrict__ done) {
+        const float* __restrict__ x, float* y, const float* __restrict__ b,
+        float omega, const float* __restrict__ wp, const float* __restrict__ isd, const int* __restrict__ done,
+        const float* xp, const float* __restrict__ cc) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1064,9 +1100,14 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
   if (i >= n) return;
   if (MODE == 2) {
     y[i] = b[i] - ax;
-  } else {
+  } else if (MODE == 1 || MODE == 3) {
     const float xn = fmaf(omega, b[i] - ax, x[i]);
     y[i] = MODE == 3 ? isd[i] * xn : xn;
+  } else {
+    const float xi = x[i];
+    const float u = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
+    const float xn = fmaf(cc[1], b[i] - ax, u);
+    y[i] = MODE == 5 ? isd[i] * xn : xn;
   }
 }
 
@@ -1257,6 +1298,8 @@ struct MgLevel {
   float* wb(int k) const { return work + (size_t)k * 64 * n_slices; }
 };
 
+constexpr int kChebSteps = 8;
+
 struct mgpbd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1285,6 +1328,9 @@ struct mgpbd_ctx {
   // spectral smoother: per-level weights omega_l = target / (safety lambda_l) in device memory,
   // lambda_l from a power iteration on D^-1 A warm-started across updates
   bool spectral = false;
+  bool chebyshev = false;       // spectral weights as a Chebyshev polynomial smoother
+  float cheb_ratio = 12.0f;     // smoothed interval [lambda_max / ratio, lambda_max]
+  float* cheb_dev = nullptr;    // MGPBD_MAX_LEVELS x kChebSteps x (c1, c2)
   float sm_target = 4.0f / 3.0f, sm_safety = 1.1f;
   int power_iters = 4;
   float* omega_dev = nullptr;   // MGPBD_MAX_LEVELS
@@ -1316,6 +1362,14 @@ struct mgpbd_ctx {
 // Device pointer of level l's smoothing weight, or nullptr for the fixed host weight.
 static const float* wl(const mgpbd_ctx* c, int l) {
   return c->spectral ? c->omega_dev + l : nullptr;
+}
+// Device pointer of the (c1, c2) pair of Chebyshev step `step` (>= 2) on level l.
+static const float* cl(const mgpbd_ctx* c, int l, int step) {
+  return c->cheb_dev + 2 * (l * kChebSteps + std::min(step - 2, kChebSteps - 1));
+}
+// Chebyshev smoothing on level l (not on the coarsest level, which keeps Jacobi sweeps).
+static bool cheb_level(const mgpbd_ctx* c, int l) {
+  return c->chebyshev && l < c->n_levels - 1 && !c->L[l].diagonal;
 }
 
 static int prof_flush(mgpbd_ctx* c) {
@@ -1365,6 +1419,7 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->omega_dev, sizeof(float) * MGPBD_MAX_LEVELS);
   if (e == hipSuccess) e = hipMalloc(&c->pw_dev, sizeof(double) * 4 * MGPBD_MAX_LEVELS);
+  if (e == hipSuccess) e = hipMalloc(&c->cheb_dev, sizeof(float) * 2 * kChebSteps * MGPBD_MAX_LEVELS);
   if (e == hipSuccess) e = hipHostMalloc(&c->pw_host, sizeof(double) * 4 * MGPBD_MAX_LEVELS, hipHostMallocDefault);
   if (e == hipSuccess) std::memset(c->pw_host, 0, sizeof(double) * 4 * MGPBD_MAX_LEVELS);
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -1376,6 +1431,7 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
     if (c->omega_dev) (void)hipFree(c->omega_dev);
     if (c->pw_dev) (void)hipFree(c->pw_dev);
     if (c->pw_host) (void)hipHostFree(c->pw_host);
+    if (c->cheb_dev) (void)hipFree(c->cheb_dev);
     delete c;
     return nullptr;
   }
@@ -1410,6 +1466,7 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   (void)hipFree(c->omega_dev);
   (void)hipFree(c->pw_dev);
   (void)hipHostFree(c->pw_host);
+  (void)hipFree(c->cheb_dev);
   delete c;
 }
 
@@ -1712,7 +1769,14 @@ static int check_ready(mgpbd_ctx* ctx) {
 
 // Jacobi sweeps on level l from the first-sweep result already in *cur; returns the buffer
 // holding the result (cur or the level's other x buffer).
-static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count, const int* done) {
+// `count` smoothing steps on level l from the iterate in `cur` (one of the level's two x
+// buffers). first_step = the 1-based index of the first step within its smoothing phase (a
+// pre-smoothing phase from zero enters at step 2: its first step is fused elsewhere), 0 for
+// plain weighted-Jacobi sweeps (the coarsest level). With the Chebyshev smoother steps >= 2
+// need the previous iterate, which is the buffer being overwritten (zero at step 2 of a phase
+// that started from zero).
+static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count, const int* done,
+                     int first_step = 0, bool from_zero = false) {
   MgLevel& L = ctx->L[l];
   float* xa = L.wb(1);
   float* xb = L.wb(2);
@@ -1729,12 +1793,21 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
     prof_end(ctx, pe);
     return cur;
   }
+  const bool cheb = first_step > 0 && cheb_level(ctx, l);
   for (int s = 0; s < count; ++s) {
     float* nxt = (cur == xa) ? xb : xa;
+    const int step = first_step + s;
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
-    k_sell<1><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
-        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, wl(ctx, l),
-        done);
+    if (cheb && step >= 2) {
+      const float* xp = (step == 2 && from_zero) ? nullptr : nxt;
+      k_sell<3><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
+          L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, nullptr,
+          done, xp, cl(ctx, l, step));
+    } else {
+      k_sell<1><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
+          L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, wl(ctx, l),
+          done);
+    }
     prof_end(ctx, pe);
     cur = nxt;
   }
@@ -1856,7 +1929,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     int total = std::min(ctx->coarse_sweeps, L.n);
     return sweeps(ctx, l, b, x_first, total - 1, done);
   }
-  float* x = sweeps(ctx, l, b, x_first, ctx->n_smooth - 1, done);
+  float* x = sweeps(ctx, l, b, x_first, ctx->n_smooth - 1, done, 2, true);
   float* r = L.wb(3);
   MgLevel& C = ctx->L[l + 1];
   // residual r = b - A x (x == 0 when n_smooth == 0: then r = b)
@@ -1896,45 +1969,69 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     k_add_d<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, save, xcr, done);
   }
   k_prolong<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.agg, xcr, x, ctx->overcorrect, done);
-  return sweeps(ctx, l, b, x, ctx->n_smooth, done);
+  return sweeps(ctx, l, b, x, ctx->n_smooth, done, 1, false);
 }
 
 // ---------------- reduced-precision (scaled) V-cycle ----------------
 extern "C++" {
 template <int VT>
 static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
-                         const float* b, const int* done, const float* wp) {
+                         const float* b, const int* done, const float* wp, const float* xp,
+                         const float* cc) {
   const int g = nblocks(64LL * L.n_slices);
   hipStream_t s = ctx->stream;
   const float w = ctx->omega;
 #define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
-                                                         L.col16, L.hval, x, y, b, w, wp, L.isd, done)
+                                                         L.col16, L.hval, x, y, b, w, wp, L.isd, done, xp, cc)
   if (L.col16) {
-    if (mode == 1) SELLH(true, 1); else if (mode == 2) SELLH(true, 2); else SELLH(true, 3);
+    switch (mode) {
+      case 1: SELLH(true, 1); break;
+      case 2: SELLH(true, 2); break;
+      case 3: SELLH(true, 3); break;
+      case 4: SELLH(true, 4); break;
+      default: SELLH(true, 5); break;
+    }
   } else {
-    if (mode == 1) SELLH(false, 1); else if (mode == 2) SELLH(false, 2); else SELLH(false, 3);
+    switch (mode) {
+      case 1: SELLH(false, 1); break;
+      case 2: SELLH(false, 2); break;
+      case 3: SELLH(false, 3); break;
+      case 4: SELLH(false, 4); break;
+      default: SELLH(false, 5); break;
+    }
   }
 #undef SELLH
 }
 }  // extern "C++"
 
 static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
-                  const float* b, const int* done, const float* wp) {
-  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp);
-  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp);
+                  const float* b, const int* done, const float* wp, const float* xp = nullptr,
+                  const float* cc = nullptr) {
+  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp, xp, cc);
+  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp, xp, cc);
 }
 
 // `count` scaled sweeps from cur; with z_out the last one writes isd * x into z_out instead.
+// Scaled-level counterpart of `sweeps` (same step numbering); the last step of a phase that
+// ends the V-cycle on level 0 writes the unscaled result into z_out.
 static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count,
-                       const int* done, float* z_out) {
+                       const int* done, float* z_out, int first_step = 0, bool from_zero = false) {
   MgLevel& L = ctx->L[l];
   float* xa = L.wb(1);
   float* xb = L.wb(2);
+  const bool cheb = first_step > 0 && cheb_level(ctx, l);
   for (int sw = 0; sw < count; ++sw) {
     const bool last_out = z_out && sw == count - 1;
-    float* nxt = last_out ? z_out : ((cur == xa) ? xb : xa);
+    float* other = (cur == xa) ? xb : xa;   // holds the previous iterate
+    float* nxt = last_out ? z_out : other;
+    const int step = first_step + sw;
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
-    sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l));
+    if (cheb && step >= 2) {
+      const float* xp = (step == 2 && from_zero) ? nullptr : other;
+      sellh(ctx, L, last_out ? 5 : 4, cur, nxt, b, done, nullptr, xp, cl(ctx, l, step));
+    } else {
+      sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l));
+    }
     prof_end(ctx, pe);
     cur = nxt;
   }
@@ -1956,7 +2053,7 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
     const int total = std::min(ctx->coarse_sweeps, L.n);
     return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr);
   }
-  float* x = sweeps_h(ctx, l, b, x_first, ctx->n_smooth - 1, done, nullptr);
+  float* x = sweeps_h(ctx, l, b, x_first, ctx->n_smooth - 1, done, nullptr, 2, true);
   float* r = L.wb(3);
   int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
   sellh(ctx, L, 2, x, r, b, done, nullptr);
@@ -1981,7 +2078,7 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
   }
   k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x, ctx->overcorrect,
                                               done);
-  return sweeps_h(ctx, l, b, x, ctx->n_smooth, done, z_out);
+  return sweeps_h(ctx, l, b, x, ctx->n_smooth, done, z_out, 1, false);
 }
 
 int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect) {
@@ -1994,13 +2091,16 @@ int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect) {
   return 0;
 }
 
-int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, int power_iters) {
+int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, int power_iters,
+                          float cheb_ratio) {
   CTX_CHECK("mgpbd_mg_set_smoother");
-  if ((mode != 0 && mode != 1) || !(target > 0.0f && target < 2.0f) || !(safety >= 1.0f) ||
-      power_iters < 1 || power_iters > 64)
-    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_smoother: mode 0/1, target in (0, 2), "
-                                    "safety >= 1, power_iters in [1, 64]");
-  ctx->spectral = mode == 1;
+  if (mode < 0 || mode > 2 || !(target > 0.0f && target < 2.0f) || !(safety >= 1.0f) ||
+      power_iters < 1 || power_iters > 64 || !(cheb_ratio > 1.0f && cheb_ratio < 1e4f))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_smoother: mode 0/1/2, target in (0, 2), "
+                                    "safety >= 1, power_iters in [1, 64], cheb_ratio > 1");
+  ctx->spectral = mode >= 1;
+  ctx->chebyshev = mode == 2;
+  ctx->cheb_ratio = cheb_ratio;
   ctx->sm_target = target;
   ctx->sm_safety = safety;
   ctx->power_iters = power_iters;
@@ -2054,7 +2154,8 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
       k_power_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, w, v, pw);
     }
     k_power_omega<<<1, 64, 0, s>>>(pw, ctx->sm_target, ctx->sm_safety, ctx->omega,
-                                   ctx->omega_dev + l);
+                                   cheb_level(ctx, l) ? 1 : 0, ctx->cheb_ratio,
+                                   ctx->omega_dev + l, ctx->cheb_dev + 2 * kChebSteps * l);
   }
   MGPBD_LAUNCH_CHECK("mgpbd_mg_update_smoother");
   MGPBD_HIP_TRY(hipMemcpyAsync(ctx->pw_host, ctx->pw_dev, sizeof(double) * 4 * MGPBD_MAX_LEVELS,
@@ -2068,18 +2169,24 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
   return 0;
 }
 
-int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out) {
+int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out, float* cheb_out) {
   CTX_CHECK("mgpbd_mg_get_smoother");
   if (int e = check_levels(ctx)) return e;
   if (!omega_out) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_get_smoother: null output");
   if (!ctx->spectral) {
     for (int l = 0; l < ctx->n_levels; ++l) omega_out[l] = ctx->omega;
+    if (cheb_out) std::memset(cheb_out, 0, sizeof(float) * 2 * kChebSteps * ctx->n_levels);
     return 0;
   }
   float tmp[MGPBD_MAX_LEVELS];
   MGPBD_HIP_TRY(hipMemcpyAsync(tmp, ctx->omega_dev, sizeof(tmp), hipMemcpyDeviceToHost, ctx->stream));
+  if (cheb_out)
+    MGPBD_HIP_TRY(hipMemcpyAsync(cheb_out, ctx->cheb_dev, sizeof(float) * 2 * kChebSteps * ctx->n_levels,
+                                 hipMemcpyDeviceToHost, ctx->stream));
   MGPBD_HIP_TRY(hipStreamSynchronize(ctx->stream));
   for (int l = 0; l < ctx->n_levels; ++l) omega_out[l] = tmp[l];
+  if (cheb_out && !ctx->chebyshev)
+    std::memset(cheb_out, 0, sizeof(float) * 2 * kChebSteps * ctx->n_levels);
   return 0;
 }
 

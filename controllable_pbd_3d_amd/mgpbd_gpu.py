@@ -19,7 +19,8 @@ behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('reference' | 'extended'), ``galerkin`` ('values' | 'structural'), ``max_coarse``, ``max_levels``,
 ``coarse_solver`` ('jacobi' | 'cholesky'), ``precision`` ('fp32' | 'fp16' | 'bf16'),
 ``cycle`` ('V' | 'W'), ``overcorrect`` (scale of the coarse correction), ``smoother``
-('jacobi' | 'spectral': per-level weights from lambda_max(D^-1 A)), ``verify_residual``
+('jacobi' | 'spectral' | 'chebyshev': per-level weights or polynomial from
+lambda_max(D^-1 A)), ``verify_residual``
 (1: FP64 check of every converged PCG solution, restart when it fails; 2: measure only),
 ``f_ext``
 (external vertex forces), ``device``,

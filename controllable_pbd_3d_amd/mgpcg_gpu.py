@@ -18,6 +18,8 @@ Extensions (keyword-only, no reference counterpart):
 * ``smoother='spectral'``: per-level Jacobi weights target / (safety lambda_max(D^-1 A)) from a
   device power iteration refreshed at every refill (the reference's fixed 2/3 makes the V-cycle
   indefinite once lambda_max(D^-1 A) > 3, which per-tet stiffness contrast produces);
+  ``smoother='chebyshev'``: the same estimates drive a Chebyshev polynomial smoother on
+  [lambda_max / cheb_ratio, lambda_max] (same sweeps per phase, fewer PCG iterations);
 * ``cycle='W'`` and ``overcorrect``: W-cycle and a scaled coarse correction;
 * ``verify=1`` (or True): a converged solve is checked with an FP64 residual of the returned x
   and restarted from x when it misses the tolerance; ``verify=2`` only measures that residual.
@@ -65,7 +67,8 @@ class GPUMultigridSolver:
                  coarse_solver: str = "jacobi", precision: str = "fp32",
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
-                 smoother_safety: float = 1.1, power_iters: int = 4, verify: int = 0,
+                 smoother_safety: float = 1.1, power_iters: int = 4, cheb_ratio: float = 12.0,
+                 verify: int = 0,
                  device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
@@ -87,12 +90,13 @@ class GPUMultigridSolver:
             raise ValueError("cycle must be 'V' (reference) or 'W'")
         self.cycle = cycle
         self.overcorrect = float(overcorrect)
-        if smoother not in ("jacobi", "spectral"):
-            raise ValueError("smoother must be 'jacobi' (reference) or 'spectral'")
+        if smoother not in ("jacobi", "spectral", "chebyshev"):
+            raise ValueError("smoother must be 'jacobi' (reference), 'spectral' or 'chebyshev'")
         self.smoother = smoother
         self.smoother_target = float(smoother_target)
         self.smoother_safety = float(smoother_safety)
         self.power_iters = int(power_iters)
+        self.cheb_ratio = float(cheb_ratio)
         self.verify = int(verify)   # 0 off, 1 verify + restart (True), 2 measure only
         self.last_lambda = None
         self.n_levels = 0
@@ -218,8 +222,8 @@ class GPUMultigridSolver:
         ctx("mgpbd_mg_configure", self.n_levels, self.n_smooth, self.omega_jacobi, mode,
             self.coarse_sweeps)
         ctx("mgpbd_mg_set_cycle", 2 if self.cycle == "W" else 1, self.overcorrect)
-        ctx("mgpbd_mg_set_smoother", 1 if self.smoother == "spectral" else 0,
-            self.smoother_target, self.smoother_safety, self.power_iters)
+        ctx("mgpbd_mg_set_smoother", {"jacobi": 0, "spectral": 1, "chebyshev": 2}[self.smoother],
+            self.smoother_target, self.smoother_safety, self.power_iters, self.cheb_ratio)
         ctx("mgpbd_pcg_set_verify", int(self.verify))
         self._coarse_mode = mode
         for l, L in enumerate(self.levels):
@@ -249,7 +253,7 @@ class GPUMultigridSolver:
             self.ctx("mgpbd_mg_factor_coarsest")
         if getattr(self, "_half", None):
             self.ctx("mgpbd_mg_convert_half")
-        if self.smoother == "spectral":
+        if self.smoother != "jacobi":
             self.ctx("mgpbd_mg_update_smoother", None)
 
     def coarse_stats(self):
@@ -258,11 +262,14 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_mg_coarse_stats", ctypes.byref(a), ctypes.byref(b))
         return int(a.value), int(b.value)
 
-    def smoother_weights(self):
-        """Jacobi weight of each level as the V-cycle uses it now (synchronises)."""
+    def smoother_weights(self, chebyshev: bool = False):
+        """Jacobi weight of each level as the V-cycle uses it now (synchronises); with
+        ``chebyshev`` also the (c1, c2) of Chebyshev steps 2..9, shape (n_levels, 8, 2)."""
         w = np.zeros(self.n_levels, np.float32)
-        self.ctx("mgpbd_mg_get_smoother", w.ctypes.data_as(_lib.F32P))
-        return w
+        c = np.zeros((self.n_levels, 8, 2), np.float32)
+        self.ctx("mgpbd_mg_get_smoother", w.ctypes.data_as(_lib.F32P),
+                 c.ctypes.data_as(_lib.F32P) if chebyshev else None)
+        return (w, c) if chebyshev else w
 
     def smoother_estimates(self):
         """Largest eigenvalue of D^-1 A per level from the spectral smoother's power iteration

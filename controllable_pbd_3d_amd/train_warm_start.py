@@ -4,7 +4,7 @@ Workload: a jittered block of n^3 Kuhn cubes (6 tets each), per-tet E log-unifor
 [1e4, 1e6] Pa (nu = 0.45, rho = 1000), top face held; every frame a seeded random free vertex
 is pulled by a spring (k = 1e3 N/m) toward a target uniform in a 0.5 m cube around its rest
 position; 5 substeps per frame at dt = 1/300 s; PCG to a relative residual of 1e-6, every solve
-FP64-verified; spectral-weight smoother, FP16 V-cycle.
+FP64-verified; Chebyshev smoother on the spectral estimates, FP16 V-cycle.
 
 Training pairs come from the solver itself: the first Newton iteration of every substep on the
 training seeds (the solver runs from the previous lambda meanwhile). The network is trained on
@@ -48,7 +48,7 @@ def make_solver(mesh, mu_t, max_levels, warm_start="none", model=None, record=No
                    n_smooth=2, omega_jacobi=2.0 / 3.0, mu_per_tet=mu_t, hold_pinned=True,
                    relative_tol=True, tol_cg=1e-6, hierarchy="extended", galerkin="structural",
                    max_coarse=64, max_levels=max_levels, coarse_solver="cholesky",
-                   precision=precision, smoother="spectral", verify_residual=1,
+                   precision=precision, smoother="chebyshev", verify_residual=1,
                    warm_start=warm_start, warm_start_model=model, record_samples=record,
                    verbose=False)
     s.init()

@@ -267,20 +267,27 @@ int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t*
  * not the coarsest is cycled twice, the second time on its residual); overcorrect scales the
  * prolongated coarse correction (1 = the reference). The defaults are 1 and 1. */
 int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect);
-/* Smoother weights (extension). mode 0 = the reference's fixed weighted Jacobi (the omega of
- * mgpbd_mg_configure on every level, cons/mgpcg_gpu.py:232-246). mode 1 = spectral: level l
- * relaxes with omega_l = target / (safety lambda_l), lambda_l the largest eigenvalue of
- * D^-1 A_l estimated by power iteration (12 iterations the first time, power_iters after,
- * warm-started); a weight with omega_l lambda_max < 2 keeps the V-cycle an SPD preconditioner,
- * which the fixed 2/3 loses once lambda_max(D^-1 A) > 3. Weights live in device memory, so
- * captured PCG graphs need no recapture when they change. */
-int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, int power_iters);
+/* Smoother (extension). mode 0 = the reference's fixed weighted Jacobi (the omega of
+ * mgpbd_mg_configure on every level, cons/mgpcg_gpu.py:232-246). mode 1 = spectral Jacobi:
+ * level l relaxes with omega_l = target / (safety lambda_l), lambda_l the largest eigenvalue
+ * of D^-1 A_l estimated by power iteration (12 iterations for the first four updates,
+ * power_iters after, warm-started); omega_l lambda_max < 2 keeps the V-cycle an SPD
+ * preconditioner, which the fixed 2/3 loses once lambda_max(D^-1 A) > 3. mode 2 = Chebyshev:
+ * every smoothing phase of a level above the coarsest applies the Chebyshev polynomial of
+ * D^-1 A_l on [safety lambda_l / cheb_ratio, safety lambda_l] (step 1 a Jacobi sweep with
+ * 1 / theta, later steps x + c1 (x - x_prev) + c2 D^-1 (b - A x)); the same polynomial pre and
+ * post keeps the cycle symmetric. Weights and coefficients live in device memory, so captured
+ * PCG graphs need no recapture when they change. */
+int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, int power_iters,
+                          float cheb_ratio);
 /* Re-estimates lambda_l and omega_l for the current level values (call after the coarse refill;
  * no-op in mode 0). lambda_out (host, n_levels doubles, may be NULL) receives the estimates
  * (then the call synchronises). */
 int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out);
-/* The weight each level relaxes with (host, n_levels floats; synchronises). */
-int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out);
+/* The weight each level relaxes with (host, n_levels floats) and, when cheb_out is given,
+ * the Chebyshev (c1, c2) of steps 2..9 per level (n_levels x 16 floats, zero unless mode 2);
+ * synchronises. */
+int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out, float* cheb_out);
 /* Reduced-precision V-cycle (extension; no reference counterpart). Every level is used in its
  * scaled form D^-1/2 A D^-1/2 (unit diagonal), values stored FP16 or BF16 in the level's
  * SELL positions, products accumulated in FP32; the outer PCG SpMV stays FP32. flexible != 0

@@ -320,6 +320,7 @@ typedef struct {
   real *b, *x, *xn, *r, *tmp;
   const double* dense;      /* coarsest only: dense column-major inverse, or NULL (sweeps) */
   real omega;               /* this level's Jacobi weight; <= 0: the cycle's common weight */
+  const real* cheb;         /* Chebyshev (c1, c2) of smoothing steps 2..9, or NULL (Jacobi) */
 } OrcLevel;
 
 typedef struct {
@@ -327,6 +328,37 @@ typedef struct {
   real omega;
   OrcLevel* L;
 } OrcMG;
+
+/* x_new = x + c1 (x - xp) + (c2 dinv) (b - A x): step >= 2 of the Chebyshev smoother */
+static void orc_cheb_step(int n, const int32_t* indptr, const int32_t* indices, const real* data,
+                          const real* dinv, const real* b, const real* x, const real* xp,
+                          real* x_new, real c1, real c2) {
+  for (int i = 0; i < n; ++i) {
+    real s = 0;
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) s = FMA(data[k], x[indices[k]], s);
+    const real u = FMA(c1, x[i] - xp[i], x[i]);
+    x_new[i] = FMA(c2 * dinv[i], b[i] - s, u);
+  }
+}
+
+/* one smoothing phase of `count` steps from A->x (A->tmp keeps the previous iterate) */
+static void smooth_phase(const OrcLevel* A, int count, real w) {
+  const int n = A->n;
+  real* prev = A->tmp;
+  memcpy(prev, A->x, sizeof(real) * n);
+  for (int s = 0; s < count; ++s) {
+    const int step = 1 + s;
+    if (A->cheb && step >= 2) {
+      const int k = step - 2 < 7 ? step - 2 : 7;
+      orc_cheb_step(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, prev, A->xn, A->cheb[2 * k],
+                    A->cheb[2 * k + 1]);
+    } else {
+      orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, w);
+    }
+    memcpy(prev, A->x, sizeof(real) * n);
+    memcpy(A->x, A->xn, sizeof(real) * n);
+  }
+}
 
 static void vc(const OrcMG* mg, int l) {
   const OrcLevel* A = &mg->L[l];
@@ -350,10 +382,7 @@ static void vc(const OrcMG* mg, int l) {
     return;
   }
   const OrcLevel* C = &mg->L[l + 1];
-  for (int s = 0; s < mg->n_smooth; ++s) {
-    orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, w);
-    memcpy(A->x, A->xn, sizeof(real) * n);
-  }
+  smooth_phase(A, mg->n_smooth, w);
   orc_spmv(n, A->ip, A->ix, A->a, A->x, A->r);
   for (int i = 0; i < n; ++i) A->r[i] = A->b[i] - A->r[i];
   orc_spmv(C->n, A->tip, A->tix, A->ta, A->r, C->b);
@@ -361,10 +390,7 @@ static void vc(const OrcMG* mg, int l) {
   vc(mg, l + 1);
   orc_spmv(n, A->pip, A->pix, A->pa, C->x, A->tmp);
   for (int i = 0; i < n; ++i) A->x[i] = FMA((real)1, A->tmp[i], A->x[i]);
-  for (int s = 0; s < mg->n_smooth; ++s) {
-    orc_jacobi(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, A->xn, w);
-    memcpy(A->x, A->xn, sizeof(real) * n);
-  }
+  smooth_phase(A, mg->n_smooth, w);
 }
 
 static double dotd(int n, const real* a, const real* b) {

@@ -122,7 +122,7 @@ def _level_struct(R, scalar):
                     ("pip", I32P), ("pix", I32P), ("pa", R),
                     ("tip", I32P), ("tix", I32P), ("ta", R),
                     ("b", R), ("x", R), ("xn", R), ("r", R), ("tmp", R),
-                    ("dense", F64P), ("omega", scalar)]
+                    ("dense", F64P), ("omega", scalar), ("cheb", R)]
     return OrcLevel
 
 
@@ -181,9 +181,14 @@ class OracleMG:
                                       ptr(lv["a"], self.R), ptr(lv["dinv"], self.R))
         self._build_structs()
 
-    def set_level_omegas(self, omegas):
-        """Per-level Jacobi weights (None: the common ``omega`` on every level)."""
+    def set_level_omegas(self, omegas, cheb=None):
+        """Per-level Jacobi weights (None: the common ``omega`` on every level) and, optionally,
+        per-level Chebyshev (c1, c2) of smoothing steps 2..9 (arrays of shape (8, 2), or None for
+        a Jacobi level)."""
         self.level_omegas = None if omegas is None else [float(w) for w in omegas]
+        self.level_cheb = None if cheb is None else [
+            None if c is None else np.ascontiguousarray(np.asarray(c).reshape(-1), self.dt)
+            for c in cheb]
         self._build_structs()
 
     def _build_structs(self):
@@ -205,6 +210,8 @@ class OracleMG:
                 setattr(s, k, ptr(lv[k], R))
             s.dense = ptr(lv.get("dense"), F64P)
             s.omega = oms[i] if oms is not None else 0.0
+            ch = getattr(self, "level_cheb", None)
+            s.cheb = ptr(ch[i] if ch is not None else None, R)
         self._structs = arr
 
     def update_fine_matrix_data(self, data):

@@ -321,3 +321,57 @@ def test_fixed_jacobi_vcycle_is_indefinite_and_spectral_weights_fix_it():
     # the common weight on every level reproduces the fixed-weight cycle exactly
     o.mg.set_level_omegas([2.0 / 3.0] * len(lam))
     assert np.array_equal(_vcycle_operator(o.mg, n), M)
+
+
+def chebyshev_coefficients(lam, safety=1.1, ratio=12.0):
+    """(1/theta, [(c1, c2) of steps 2..9]) of the Chebyshev smoother on
+    [safety lam / ratio, safety lam] (Saad, Iterative Methods, Alg. 12.1 in three-term form)."""
+    b = safety * lam
+    a = b / ratio
+    theta, delta = 0.5 * (b + a), 0.5 * (b - a)
+    sigma = theta / delta
+    rho = 1.0 / sigma
+    c = []
+    for _ in range(8):
+        rn = 1.0 / (2.0 * sigma - rho)
+        c.append((rn * rho, 2.0 * rn / delta))
+        rho = rn
+    return 1.0 / theta, np.array(c)
+
+
+def test_chebyshev_smoother_is_the_chebyshev_polynomial_and_keeps_the_cycle_spd():
+    """Two smoothing steps from zero equal p(D^-1 A) applied as the Chebyshev polynomial
+    T2((b + a - 2 lambda) / (b - a)) / T2((b + a) / (b - a)) (checked on the error propagation
+    of a diagonal system, where it is the polynomial itself); and the V-cycle with Chebyshev
+    smoothing on every level above the coarsest is symmetric positive definite."""
+    # 1. the recurrence gives the Chebyshev polynomial: error after 2 steps on A = diag(lam)
+    lam_max = 5.0
+    w1, c = chebyshev_coefficients(lam_max / 1.1, ratio=12.0)
+    b_, a_ = lam_max, lam_max / 12.0
+    lam = np.linspace(0.01, lam_max, 50)
+    e0 = np.ones_like(lam)
+    e1 = e0 - w1 * lam * e0                       # Jacobi step (error form, b = 0)
+    e2 = e1 + c[0][0] * (e1 - e0) - c[0][1] * lam * e1
+    T2 = lambda t: 2 * t * t - 1
+    p = T2((b_ + a_ - 2 * lam) / (b_ - a_)) / T2((b_ + a_) / (b_ - a_))
+    np.testing.assert_allclose(e2, p, rtol=1e-10, atol=1e-12)
+    # 2. SPD V-cycle with Chebyshev smoothing
+    o = _stiff_contrast_system()
+    n = o.n_tets
+    lam_l, omegas, chebs = [], [], []
+    for i, lv in enumerate(o.mg.levels):
+        A = sp.csr_matrix((lv["a"].astype(np.float64), lv["ix"], lv["ip"]), shape=(lv["n"],) * 2)
+        d = A.diagonal()
+        S = (sp.diags(1 / np.sqrt(d)) @ A @ sp.diags(1 / np.sqrt(d))).toarray()
+        lm = np.linalg.eigvalsh(S).max()
+        if i < len(o.mg.levels) - 1:
+            w, cc = chebyshev_coefficients(lm)
+            omegas.append(w); chebs.append(cc)
+        else:
+            omegas.append((4.0 / 3.0) / (1.1 * lm)); chebs.append(None)
+    o.mg.set_level_omegas(omegas, chebs)
+    M = _vcycle_operator(o.mg, n)
+    assert np.abs(M - M.T).max() <= 1e-9 * np.abs(M).max()
+    ev = np.linalg.eigvalsh(0.5 * (M + M.T))
+    print(f"Chebyshev V-cycle eigenvalues [{ev.min():.3e}, {ev.max():.3e}]")
+    assert ev.min() > 0
