@@ -61,6 +61,14 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD b % 8), each XCD with
+// its own L2. Remapping block b to a contiguous range per XCD keeps neighbouring work (rows
+// sharing cache lines of their gathered inputs) on one L2; a bijection on [0, G).
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int x = b & 7, j = b >> 3, q = G >> 3, rem = G & 7;
+  return x * q + min(x, rem) + j;
+}
+
 struct RedWS {
   double* partials;       // kPartialStride per slot, 4 slots
 };
@@ -311,7 +319,7 @@ k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
          const int* __restrict__ row_len, const int4* __restrict__ t_i,
          const float* __restrict__ invm, const float* __restrict__ gradC,
          const float* __restrict__ alpha, float* __restrict__ val, float* __restrict__ dinv) {
-  int i = blockIdx.x * kBlock + threadIdx.x;
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const int base = sell_base(slice_ptr, i);
   const int len = row_len[i];
@@ -348,7 +356,7 @@ __global__ void __launch_bounds__(kBlock)
 k_scatter(int nv, const int* __restrict__ vptr, const int* __restrict__ vinc,
           const float* __restrict__ invm, const float* __restrict__ gradC,
           const float* __restrict__ dlam, float* __restrict__ dpos) {
-  int v = blockIdx.x * kBlock + threadIdx.x;
+  int v = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (v >= nv) return;
   const float w = invm[v];
   float a0 = 0.f, a1 = 0.f, a2 = 0.f;
@@ -710,10 +718,24 @@ k_diag_inv(int n, const int* __restrict__ diag_pos, const float* __restrict__ va
 __global__ void __launch_bounds__(kBlock)
 k_refill(int nc, const int* __restrict__ off, const int* __restrict__ fpos,
          const int* __restrict__ cpos, const float* __restrict__ fine, float* __restrict__ coarse) {
-  int c = blockIdx.x * kBlock + threadIdx.x;
+  int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (c >= nc) return;
+  // contributions in batches of 8: all index and value loads of a batch are issued before
+  // the adds, which stay in the map's order (the sum is the same as one add per load)
   float v = 0.0f;
-  for (int k = off[c]; k < off[c + 1]; ++k) v = v + fine[fpos[k]];
+  int k = off[c];
+  const int k1 = off[c + 1];
+  for (; k + 8 <= k1; k += 8) {
+    int q[8];
+    float f[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) q[u] = fpos[k + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) f[u] = fine[q[u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v = v + f[u];
+  }
+  for (; k < k1; ++k) v = v + fine[fpos[k]];
   coarse[cpos[c]] = v;
 }
 
@@ -2144,6 +2166,7 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
     const bool moving = !(std::fabs(ph[0] - ph[2]) <= 0.02 * ph[0]);
     if (ctx->pv_updates[l] < 4) iters = std::max(iters, 12);
     else if (moving) iters = std::max(iters, 8);
+    else iters = std::min(iters, 2);   // the previous refresh had settled
     ++ctx->pv_updates[l];
     const int g = nblocks(64LL * L.n_slices);
     double* pw = ctx->pw_dev + 4 * l;
