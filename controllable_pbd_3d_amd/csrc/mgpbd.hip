@@ -282,6 +282,24 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
   store_partial(ws, 1, b1);
 }
 
+// Off-diagonal entry from tet j's vertex ids and gradient (already loaded).
+__device__ __forceinline__ float a_offdiag(const int (&vi)[4], const float (&w)[4],
+                                           const float (&gi)[4][3], int4 tj, float4 q0,
+                                           float4 q1, float4 q2) {
+  const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
+  const float gj[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+  float v = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (vi[a] == vj[b]) {
+        float g[3] = {gj[3 * b], gj[3 * b + 1], gj[3 * b + 2]};
+        v = fmaf(w[a], dot3(gi[a], g), v);
+      }
+  return v;
+}
+
 // One A entry (row i, column j): the diagonal sum_v w_v |g_iv|^2 + alpha_i, or the sum over
 // the vertices i and j share of w_v g_iv . g_jv in the slot order of _intersect_tets.
 __device__ __forceinline__ float a_entry(int i, int j, const int (&vi)[4], const float (&w)[4],
@@ -296,22 +314,11 @@ __device__ __forceinline__ float a_entry(int i, int j, const int (&vi)[4], const
     return v + alpha_i;
   }
   const int4 tj = t_i[j];
-  const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
   // the 48 bytes of gradC_j in three 16-byte loads
   const float4* g4 = reinterpret_cast<const float4*>(gradC) + 3 * (size_t)j;
-  const float4 q0 = g4[0], q1 = g4[1], q2 = g4[2];
-  const float gj[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
-  float v = 0.0f;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (vi[a] == vj[b]) {
-        float g[3] = {gj[3 * b], gj[3 * b + 1], gj[3 * b + 2]};
-        v = fmaf(w[a], dot3(gi[a], g), v);
-      }
-  return v;
+  return a_offdiag(vi, w, gi, tj, g4[0], g4[1], g4[2]);
 }
+
 
 // A values, one thread per row (tet) i, four entries per 16-byte store (padding stays 0).
 __global__ void __launch_bounds__(kBlock)
@@ -334,17 +341,34 @@ k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
   }
   const float al = alpha[i];
   float diag = 0.0f;
+  const float4* g4 = reinterpret_cast<const float4*>(gradC);
   for (int k0 = 0; k0 < len; k0 += 4) {
     const int p = base + 64 * k0;   // 256 * (k0 / 4)
     const int4 c = *reinterpret_cast<const int4*>(idx + p);
     const int cj[4] = {c.x, c.y, c.z, c.w};
+    // the four neighbours' vertex ids and gradients, all loads issued before any arithmetic
+    // (padding slots and the diagonal load row i's own, unused)
+    int4 tj[4];
+    float4 q0[4], q1[4], q2[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = k0 + q < len ? cj[q] : i;
+      tj[q] = t_i[j];
+      q0[q] = g4[3 * (size_t)j];
+      q1[q] = g4[3 * (size_t)j + 1];
+      q2[q] = g4[3 * (size_t)j + 2];
+    }
     float v[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       v[q] = 0.0f;
       if (k0 + q < len) {
-        v[q] = a_entry(i, cj[q], vi, w, gi, al, t_i, gradC);
-        if (cj[q] == i) diag = v[q];
+        if (cj[q] == i) {
+          v[q] = a_entry(i, i, vi, w, gi, al, t_i, gradC);
+          diag = v[q];
+        } else {
+          v[q] = a_offdiag(vi, w, gi, tj[q], q0[q], q1[q], q2[q]);
+        }
       }
     }
     *reinterpret_cast<float4*>(val + p) = make_float4(v[0], v[1], v[2], v[3]);
