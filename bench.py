@@ -31,19 +31,52 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    # name: (cubes, h, E range, levels)
-    "c1": dict(n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets"),
-    "c3": dict(n=32, h=0.03, E=(1e4, 1e6), max_levels=4, label="box32_196608tets"),
-    "c0": dict(n=8, h=0.05, E=None, max_levels=3, label="box8_3072tets"),
+    # box: n^3 Kuhn cubes of edge h, per-tet E log-uniform in E (constant 1e5 when None), top
+    # face held; ellipsoid: grid of edge h clipped to radii (1.0, 0.6, 0.4) m, stiff sphere of
+    # radius 0.2 m (E 1e8) in an E 1e5 matrix, lowest 5% of vertices held. drag: every frame
+    # (5 substeps) a seeded random free vertex is pulled by a 1e3 N/m spring toward a target
+    # uniform in a 0.5 m cube around its rest position. maxiter_cg replaces the reference's
+    # 100-iteration PCG cap where some systems need more to reach 1e-6.
+    "c1": dict(kind="box", n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets"),
+    "c2": dict(kind="box", n=56, h=0.02, E=(1e4, 1e8), max_levels=5,
+               label="box56_1053696tets_E1e4-1e8"),
+    "c3": dict(kind="box", n=32, h=0.03, E=(1e4, 1e6), max_levels=4, drag=True,
+               label="box32_196608tets_drag"),
+    "c4": dict(kind="ellipsoid", h=0.01, E=(1e5, 1e8), max_levels=6, drag=True,
+               maxiter_cg=1000, label="ellipsoid_h0.01_inclusion_drag"),
+    "c0": dict(kind="box", n=8, h=0.05, E=None, max_levels=3, label="box8_3072tets"),
 }
+DRAG_K, DRAG_BOX, SUBSTEPS_PER_FRAME = 1e3, 0.5, 5
 
 
 def build_workload(cfg, seed=0):
-    from controllable_pbd_3d_amd.mesh import box_mesh
-    mesh = box_mesh(cfg["n"], cfg["h"], jitter=0.05, seed=seed, rho=1000.0,
-                    E_range=cfg["E"] if cfg["E"] else (1e5, 1e5))
+    from controllable_pbd_3d_amd.mesh import box_mesh, ellipsoid_mesh
+    if cfg["kind"] == "ellipsoid":
+        mesh = ellipsoid_mesh(cfg["h"], jitter=0.05, seed=seed, rho=1000.0,
+                              E_matrix=cfg["E"][0], E_inclusion=cfg["E"][1])
+    else:
+        mesh = box_mesh(cfg["n"], cfg["h"], jitter=0.05, seed=seed, rho=1000.0,
+                        E_range=cfg["E"] if cfg["E"] else (1e5, 1e5))
     mu_t = (mesh.E / (2.0 * (1.0 + 0.45))).astype(np.float32)
     return mesh, mu_t
+
+
+class Loading:
+    """Per-substep loading of a workload: nothing, or the drag schedule of ``seed``."""
+
+    def __init__(self, cfg, mesh, seed, substeps):
+        from controllable_pbd_3d_amd.mesh import drag_schedule
+        self.frames = None
+        if cfg.get("drag"):
+            self.frames = drag_schedule(mesh, seed, substeps // SUBSTEPS_PER_FRAME + 1, DRAG_BOX)
+        self.k = 0
+
+    def step(self, s):
+        if self.frames is not None and self.k % SUBSTEPS_PER_FRAME == 0:
+            v, target = self.frames[self.k // SUBSTEPS_PER_FRAME]
+            s.set_drag(v, target, DRAG_K)
+        s.step(self.k // SUBSTEPS_PER_FRAME)
+        self.k += 1
 
 
 def solver_args(mesh, mu_t):
@@ -154,21 +187,23 @@ RUNS = {
     # the reference's algorithm (fixed 2/3-weighted Jacobi); the FP64 residual of every solve
     # is measured, not enforced
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
-    # Chebyshev smoothing on the device's spectral estimates (same 2 + 2 sweeps), every solve
-    # FP64-verified to the tolerance
-    "fp32": dict(precision="fp32", smoother="chebyshev", verify_residual=1),
-    "fp16": dict(precision="fp16", smoother="chebyshev", verify_residual=1),
-    "bf16": dict(precision="bf16", smoother="chebyshev", verify_residual=1),
+    # Chebyshev smoothing on the device's spectral estimates (same 2 + 2 sweeps), FP64 PCG
+    # iterate, every solve FP64-verified to the tolerance
+    "fp32": dict(precision="fp32", smoother="chebyshev", verify_residual=1, fp64_iterate=True),
+    "fp16": dict(precision="fp16", smoother="chebyshev", verify_residual=1, fp64_iterate=True),
+    "bf16": dict(precision="bf16", smoother="chebyshev", verify_residual=1, fp64_iterate=True),
     "fp16_previous": dict(precision="fp16", smoother="chebyshev", verify_residual=1,
-                          warm_start="previous"),
+                          fp64_iterate=True, warm_start="previous"),
     "fp16_learned": dict(precision="fp16", smoother="chebyshev", verify_residual=1,
-                         warm_start="learned"),
+                         fp64_iterate=True, warm_start="learned"),
 }
 
 
 def make_solver(mesh, mu_t, cfg, kw, model=None, record=None):
     from controllable_pbd_3d_amd import MGPBDSolverGPU
-    s = MGPBDSolverGPU(**solver_args(mesh, mu_t), hold_pinned=True, relative_tol=True,
+    args = solver_args(mesh, mu_t)
+    args["maxiter_cg"] = cfg.get("maxiter_cg", args["maxiter_cg"])
+    s = MGPBDSolverGPU(**args, hold_pinned=True, relative_tol=True,
                        tol_cg=1e-6, hierarchy="extended", galerkin="structural",
                        max_coarse=64, max_levels=cfg["max_levels"], coarse_solver="cholesky",
                        warm_start_model=model, record_samples=record, verbose=False, **kw)
@@ -189,8 +224,9 @@ def train_learned(cfg, seeds, substeps, epochs):
         if model is None:
             model = LearnedWarmStart.for_mesh(mesh.tets, mesh.n_tets, seed=0, lr=1e-3)
         s = make_solver(mesh, mu_t, cfg, kw, model=model, record=samples)
+        load = Loading(cfg, mesh, seed, substeps)
         for k in range(substeps):
-            s.step(0)
+            load.step(s)
         del s
         torch.cuda.empty_cache()
         print(f"[bench] learned: seed {seed} -> {len(samples)} samples", file=sys.stderr,
@@ -209,7 +245,7 @@ def train_learned(cfg, seeds, substeps, epochs):
     return model, info
 
 
-def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None):
+def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, seed=0):
     import torch
     from controllable_pbd_3d_amd.runtime import get_context
     ctx = get_context(0)
@@ -220,13 +256,14 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None):
     setup_s = time.perf_counter() - t0
     print(f"[bench] {name}: setup {setup_s:.1f} s, levels {s._gpu_solver._level_sizes}",
           file=sys.stderr, flush=True)
+    load = Loading(cfg, mesh, seed, warmup + steps + profile_steps)
     for k in range(warmup):
-        s.step(0)
+        load.step(s)
     torch.cuda.synchronize()
     iters, newton, solves, starts, first_its = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(steps):
-        s.step(0)
+        load.step(s)
         iters.append(sum(c for _, _, c, _ in s.last_stats))
         newton.append(len(s.last_stats))
         solves.extend(s.last_solve_info)
@@ -247,6 +284,7 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None):
            "solves": len(solves), "solves_converged": float(conv.mean()) if len(conv) else None,
            "fp64_rel_residual_max": float(rel.max()) if len(rel) else None,
            "fp64_rel_residual_median": float(np.median(rel)) if len(rel) else None}
+    res["hbm_footprint_bytes"] = s.device_bytes()
     ns, refac = s._gpu_solver.coarse_stats()
     res["coarsest_inverse"] = {"newton_schulz_steps": ns, "cholesky_refactors": refac}
     if kw.get("warm_start"):
@@ -259,7 +297,7 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None):
     ctx("mgpbd_ctx_profile", 1)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 1)
     for k in range(profile_steps):
-        s.step(0)
+        load.step(s)
     kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"])
     kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"]).items()
                  if k not in kern})
@@ -274,7 +312,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS),
+                    help="c1 (default) 1M-tet block, E 1e4-1e6; c2 the same block with E "
+                         "1e4-1e8; c3 196,608-tet block with drag loading; c4 ~6M-tet "
+                         "ellipsoid with a stiff inclusion and drag loading; c0 3,072 tets")
+    ap.add_argument("--seed", type=int, default=0,
+                    help="mesh jitter / material / drag seed of the measured workload")
     ap.add_argument("--runs", default="all",
                     help="comma list of " + ",".join(RUNS) + " (default: all)")
     ap.add_argument("--profile-steps", type=int, default=2)
@@ -296,22 +339,29 @@ def main():
         if n not in RUNS:
             raise SystemExit(f"unknown run {n}")
     print(f"[bench] building {cfg['label']}", file=sys.stderr, flush=True)
-    mesh, mu_t = build_workload(cfg)
+    mesh, mu_t = build_workload(cfg, args.seed)
     torch.cuda.set_device(0)
     model, learn_info = None, None
     if "fp16_learned" in names:
         model, learn_info = train_learned(cfg, [int(x) for x in args.train_seeds.split(",")],
                                           args.train_substeps, args.train_epochs)
-    runs, keep = {}, None
+    runs, keep, pos = {}, None, {}
     for name in names:
         s, res = run_config(mesh, mu_t, cfg, name, args.steps, args.warmup, args.profile_steps,
-                            model)
+                            model, args.seed)
         runs[name] = res
+        pos[name] = s.v_p.detach().cpu().numpy().astype(np.float64)
         if keep is None:
             keep = s            # hierarchy and nnz for the report and the CPU baseline
         else:
             del s
         torch.cuda.empty_cache()
+    if "fp32" in pos:
+        # every run takes the same substeps under the same loading; positions after the last
+        # one, against the FP32 run with FP64-verified solves, in units of the edge length
+        for name in names:
+            d = np.abs(pos[name] - pos["fp32"]).max()
+            runs[name]["max_position_diff_vs_fp32_over_h"] = float(d / mesh.h)
     head = "fp16_learned" if "fp16_learned" in runs else names[-1]
     best = runs[head]
     kern = best["kernels"]
@@ -320,15 +370,19 @@ def main():
     prec = RUNS[head]["precision"]
     out = {
         "metric": "MGPBD substeps/s at 1M tets to 1e-6 relative residual (FP64-verified): "
-                  "mixed-precision V-cycle + learned warm start vs plain FP32 GPU MGPCG",
+                  "mixed-precision V-cycle + learned warm start vs plain FP32 GPU MGPCG"
+                  if args.config == "c1" else f"MGPBD substeps/s on {cfg['label']} to 1e-6 "
+                  "relative residual (FP64-verified)",
         "value": best["value"], "unit": "substeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if prec == "fp32" else f"f32 outer PCG (f64 dots), {prec} V-cycle",
         "data": "synthetic",
-        "config": {"workload": cfg["label"], "n_tets": mesh.n_tets, "n_verts": mesh.n_verts,
+        "config": {"workload": cfg["label"], "seed": args.seed, "drag": bool(cfg.get("drag")),
+                   "n_tets": mesh.n_tets, "n_verts": mesh.n_verts,
                    "nnz_A0": int(keep._nnz), "levels": keep._gpu_solver._level_sizes,
-                   "pcg_rel_tol": 1e-6, "E_range": cfg["E"], "reported": head,
+                   "pcg_rel_tol": 1e-6, "pcg_maxiter": cfg.get("maxiter_cg", 100),
+                   "E_range": cfg["E"], "reported": head,
                    "runs": runs, "learned_training": learn_info,
                    "speedup_vs_fp32_reference": (best["value"] / runs["fp32_reference"]["value"]
                                                  if "fp32_reference" in runs else None),

@@ -32,6 +32,7 @@ SIGNATURES = {
     "mgpbd_face_adjacency": (c_int, [c_int, I32P, I32P, I32P]),
     "mgpbd_tet_aggregate": (c_int, [c_int, I32P, I32P, c_int, I32P, I32P]),
     "mgpbd_graph_aggregate": (c_int, [c_int, I32P, I32P, c_int, I32P, I32P]),
+    "mgpbd_graph_merge_singletons": (c_int, [c_int, I32P, I32P, I32P, I32P]),
     "mgpbd_graph_coarsen": (_V, [c_int, I32P, I32P, I32P, c_int]),
     "mgpbd_graph_nnz": (c_int64, [_V]),
     "mgpbd_graph_copy": (c_int, [_V, I32P, I32P]),
@@ -49,9 +50,11 @@ SIGNATURES = {
     "mgpbd_ctx_set_stream": (c_int, [_V, _V]),
     "mgpbd_ctx_destroy": (None, [_V]),
     "mgpbd_ctx_sync": (c_int, [_V]),
+    "mgpbd_ctx_device_bytes": (c_int, [_V, ctypes.POINTER(c_int64)]),
     "mgpbd_ctx_profile": (c_int, [_V, c_int]),
     "mgpbd_ctx_use_graphs": (c_int, [_V, c_int, c_int]),
     "mgpbd_pcg_set_verify": (c_int, [_V, c_int]),
+    "mgpbd_pcg_set_fp64_iterate": (c_int, [_V, c_int]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),

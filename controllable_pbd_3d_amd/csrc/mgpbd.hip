@@ -485,6 +485,29 @@ __device__ __forceinline__ float sell_row(int i, const int* __restrict__ slice_p
   return acc;
 }
 
+// The same row product with FP64 products and sums (the FP32 x2 FP32 products are exact in
+// FP64), x in FP32 or FP64.
+template <typename XT>
+__device__ __forceinline__ double sell_row_d(int i, const int* __restrict__ slice_ptr,
+                                             const int* __restrict__ idx,
+                                             const float* __restrict__ val,
+                                             const XT* __restrict__ x) {
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  double acc = 0.0;
+#pragma unroll 4
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const int p = p0 + 64 * k0;
+    const v4i c = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(idx + p));
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(val + p));
+    acc = fma((double)v.x, (double)x[c.x], acc);
+    acc = fma((double)v.y, (double)x[c.y], acc);
+    acc = fma((double)v.z, (double)x[c.z], acc);
+    acc = fma((double)v.w, (double)x[c.w], acc);
+  }
+  return acc;
+}
+
 // MODE 0: y = A x;  MODE 1: y = x + (omega dinv) (b - A x);  MODE 2: y = b - A x;
 // MODE 3 (Chebyshev step): y = x + c1 (x - xp) + (c2 dinv) (b - A x), (c1, c2) = cc[0..1],
 // xp = the previous iterate (NULL: zero)
@@ -962,16 +985,46 @@ k_fin_true(RedWS ws, int nparts, PcgState* st) {
   if (threadIdx.x == 0) st->rr_true = t;
 }
 
+// Restart residual of the FP64 iterate: r = b - A x64 (FP64 products and sums), rounded once
+__global__ void __launch_bounds__(kBlock)
+k_resid64(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+          const float* __restrict__ val, const double* __restrict__ x64,
+          const float* __restrict__ b, float* __restrict__ r) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const double ax = sell_row_d(i, slice_ptr, idx, val, x64);
+  if (i < n) r[i] = (float)((double)b[i] - ax);
+}
+
+// FP64 iterate from the start the device selected: the FP32 start on the first round, and on
+// a restart only when the zero start was taken (otherwise x64 already holds the iterate)
+__global__ void __launch_bounds__(kBlock)
+k_x64_from_start(int n, const float* __restrict__ x, double* __restrict__ x64, int restart,
+                 const PcgState* __restrict__ st) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || (restart && st->start != 0)) return;
+  x64[i] = (double)x[i];
+}
+
+// the FP32 result of the FP64 iterate
+__global__ void __launch_bounds__(kBlock)
+k_x_from64(int n, const double* __restrict__ x64, float* __restrict__ x) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = (float)x64[i];
+}
+
 // r = b, x = 0, xv = (omega dinv0) r (first pre-smooth sweep, when wanted), partial b.b
 __global__ void __launch_bounds__(kBlock)
 k_pcg_init(int n, const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r,
-           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws) {
+           const float* __restrict__ dinv0, float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws,
+           double* __restrict__ x64) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   double acc = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     float bi = b[i];
     r[i] = bi;
     x[i] = 0.0f;
+    if (x64) x64[i] = 0.0;
     if (xv) xv[i] = (omega * dinv0[i]) * bi;
     acc += (double)bi * (double)bi;
   }
@@ -990,6 +1043,8 @@ k_pcg_rz(int n, const float* __restrict__ r, const float* __restrict__ z, RedWS 
 }
 
 // Ap = A p and partial p.Ap (one row per thread, one slice per wavefront)
+// ACC64: products and sums in FP64, Ap rounded once to FP32
+template <bool ACC64>
 __global__ void __launch_bounds__(kBlock)
 k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
                const int* __restrict__ idx, const float* __restrict__ val,
@@ -999,7 +1054,8 @@ k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
   double acc = 0.0;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) < n_slices) {
-    float ax = sell_row(i, slice_ptr, idx, val, p);
+    const float ax = ACC64 ? (float)sell_row_d(i, slice_ptr, idx, val, p)
+                           : sell_row(i, slice_ptr, idx, val, p);
     if (i < n) {
       Ap[i] = ax;
       acc = (double)p[i] * (double)ax;
@@ -1008,17 +1064,21 @@ k_pcg_spmv_dot(int n, int n_slices, const int* __restrict__ slice_ptr,
   store_partial(ws, 0, block_sum(acc));
 }
 
-// x += alpha p, r -= alpha Ap, partial r.r; also xv = (omega dinv0) r for the next V-cycle
+// x += alpha p, r -= alpha Ap, partial r.r; also xv = (omega dinv0) r for the next V-cycle.
+// x64 != NULL: the iterate is the FP64 x64 (x64 += alpha p in FP64; x is left alone)
 __global__ void __launch_bounds__(kBlock)
 k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
              const float* __restrict__ Ap, const float* __restrict__ dinv0,
-             float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws, const PcgState* st) {
+             float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws,
+             const PcgState* st, double* __restrict__ x64) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   if (st->done) return;
   const float al = (float)st->alpha;
+  const double ald = st->alpha;
   double acc = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    x[i] = fmaf(al, p[i], x[i]);
+    if (x64) x64[i] = fma(ald, (double)p[i], x64[i]);
+    else x[i] = fmaf(al, p[i], x[i]);
     float ri = fmaf(-al, Ap[i], r[i]);
     r[i] = ri;
     if (xv) xv[i] = (omega * dinv0[i]) * ri;
@@ -1389,6 +1449,7 @@ struct mgpbd_ctx {
   int flexible = 0;                   // Polak-Ribiere beta
   int poll = 4;
   int verify = 0;   // 1: FP64 check of the converged x, restart from x when it fails; 2: check only
+  bool fp64_iterate = false;   // FP64 PCG iterate, FP64-accumulated outer SpMV and restart residual
   // HIP graph of `poll` PCG iterations (+ the state copy), replayed on `work` stream
   bool use_graph = true;
   hipStream_t work = nullptr;
@@ -1514,6 +1575,18 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   (void)hipHostFree(c->pw_host);
   (void)hipFree(c->cheb_dev);
   delete c;
+}
+
+int mgpbd_ctx_device_bytes(mgpbd_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_device_bytes: invalid arguments");
+  int64_t t = (int64_t)sizeof(double) * kPartialStride * 4 + (int64_t)sizeof(PcgState) +
+              (int64_t)sizeof(float) * MGPBD_MAX_LEVELS * (1 + 2 * kChebSteps) +
+              (int64_t)sizeof(double) * 4 * MGPBD_MAX_LEVELS;
+  int nd = 0;
+  for (double* b : {c->dense, c->dA, c->dR, c->dX2, c->dT}) nd += b != nullptr;
+  t += (int64_t)nd * (int64_t)c->dense_cap * (int64_t)sizeof(double);
+  *bytes = t;
+  return 0;
 }
 
 int mgpbd_ctx_sync(mgpbd_ctx* c) {
@@ -2320,19 +2393,23 @@ static float* pcg_precondition(mgpbd_ctx* ctx, float* r, float* xv, bool first) 
 }
 
 // One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
-static void pcg_iteration(mgpbd_ctx* ctx, float* x, float* r, float* p, float* Ap, float* xv,
-                          float* zold, bool first, int gn, int gs, bool prof) {
+static void pcg_iteration(mgpbd_ctx* ctx, float* x, double* x64, float* r, float* p, float* Ap,
+                          float* xv, float* zold, bool first, int gn, int gs, bool prof) {
   MgLevel& L = ctx->L[0];
   const int n = L.n;
   hipStream_t s = ctx->stream;
   int pe = prof ? prof_begin(ctx, MGPBD_PROF_SPMV_L0) : -1;
-  k_pcg_spmv_dot<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
-                                       ctx->ws, ctx->st);
+  if (x64)
+    k_pcg_spmv_dot<true><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
+                                               ctx->ws, ctx->st);
+  else
+    k_pcg_spmv_dot<false><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
+                                                ctx->ws, ctx->st);
   prof_end(ctx, pe);
   k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
   const bool fuse_first = first && !half_mode(ctx);
   k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, fuse_first ? xv : nullptr,
-                                     ctx->omega, wl(ctx, 0), ctx->ws, ctx->st);
+                                     ctx->omega, wl(ctx, 0), ctx->ws, ctx->st, x64);
   k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
   float* zz = pcg_precondition(ctx, r, xv, first);
   if (ctx->flexible) {
@@ -2371,6 +2448,8 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
   float* zold = work + (size_t)n;   // previous z (flexible PCG)
   float* p = work + 2 * (size_t)n;
   float* Ap = work + 3 * (size_t)n;
+  // FP64 iterate (work + 4n floats, 8-byte aligned when work is)
+  double* x64 = ctx->fp64_iterate ? reinterpret_cast<double*>(work + 4 * (size_t)n) : nullptr;
   float* xv = L.wb(1);              // level-0 x buffer receiving the first sweep
   const bool first = (ctx->n_levels == 1) || ctx->n_smooth > 0;
   const int* done = &ctx->st->done;
@@ -2400,7 +2479,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
       hipGraph_t g = nullptr;
       MGPBD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int c = 0; c < ctx->poll; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, zold, first, gn, gs, false);
+      for (int c = 0; c < ctx->poll; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, false);
       (void)hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s);
       hipError_t ce = hipStreamEndCapture(s, &g);
       if (ce != hipSuccess) return mgpbd_fail(MGPBD_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -2425,9 +2504,12 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
     if (from_guess) {
       // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
-      // start with the smallest residual
-      k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
-                                      b, 0.f, nullptr, nullptr);
+      // start with the smallest residual; a restart of the FP64 iterate takes r = b - A x64
+      if (x64 && restarts > 0)
+        k_resid64<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x64, b, r);
+      else
+        k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
+                                        b, 0.f, nullptr, nullptr);
       if (alt)
         k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, alt, Ap,
                                         nullptr, b, 0.f, nullptr, nullptr);
@@ -2436,9 +2518,11 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, alt, Ap, L.dinv,
                                                  fuse_first ? xv : nullptr, ctx->omega,
                                                  wl(ctx, 0), ctx->st);
+      if (x64)
+        k_x64_from_start<<<nblocks(n), kBlock, 0, s>>>(n, x, x64, restarts > 0, ctx->st);
     } else {
       k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr,
-                                        ctx->omega, wl(ctx, 0), ctx->ws);
+                                        ctx->omega, wl(ctx, 0), ctx->ws, x64);
       k_pcg_fin<FIN_BB><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
     }
     float* zr = pcg_precondition(ctx, r, xv, first);
@@ -2456,7 +2540,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
         round_issued += ctx->poll;
       } else {
         int chunk = std::min(ctx->poll, maxiter - issued);
-        for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, r, p, Ap, xv, zold, first, gn, gs, true);
+        for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, true);
         MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
         issued += chunk;
         round_issued += chunk;
@@ -2470,6 +2554,10 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       MGPBD_HIP_TRY(hipStreamSynchronize(s));
     }
+    if (x64) {
+      k_x_from64<<<nblocks(n), kBlock, 0, s>>>(n, x64, x);
+      MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(x)");
+    }
     if (restarts == 0) {
       const PcgState& h = *ctx->st_host;
       first_info[0] = guess ? h.rr0 : h.bb;
@@ -2478,8 +2566,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       first_info[3] = h.rr_alt;
     }
     total_iters += ctx->st_host->iters;
-    if (!ctx->verify || (ctx->verify == 1 && ctx->st_host->done != 1)) break;
-    // FP64 residual of the returned x; a restart from x when it misses the tolerance
+    if (!ctx->verify) break;
+    // FP64 residual of the returned x (every solve); a restart from x when a converged one
+    // misses the tolerance
     k_true_resid<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, b, ctx->ws);
     k_fin_true<<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
     MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(verify)");
@@ -2507,6 +2596,17 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
                           first_info[3], rr_true};
     for (int i = 0; i < n_info; ++i) info_out[i] = n_info == 4 ? v4[i] : v8[i];
   }
+  return 0;
+}
+
+int mgpbd_pcg_set_fp64_iterate(mgpbd_ctx* ctx, int enable) {
+  CTX_CHECK("mgpbd_pcg_set_fp64_iterate");
+  const bool on = enable != 0;
+  if (on != ctx->fp64_iterate && ctx->graph) {   // the captured iteration changes
+    (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+  }
+  ctx->fp64_iterate = on;
   return 0;
 }
 

@@ -307,6 +307,45 @@ int mgpbd_graph_aggregate(int n, const int32_t* ptr, const int32_t* idx, int max
   return 0;
 }
 
+// Singleton merge for the graph levels of the extended hierarchy (no reference counterpart):
+// in node order, a node that is alone in its aggregate joins the smallest aggregate among its
+// neighbours' (ties: the lowest id; sizes as updated so far); aggregates are then renumbered
+// in ascending order of their old ids, empty ones dropped. Without it the greedy growth leaves
+// most nodes of the coarse graphs of irregular meshes as singletons and coarsening stalls.
+int mgpbd_graph_merge_singletons(int n, const int32_t* ptr, const int32_t* idx,
+                                 int32_t* agg_ids, int32_t* n_agg_inout) {
+  if (n < 0 || !n_agg_inout || *n_agg_inout < 0 || (n > 0 && (!ptr || !idx || !agg_ids)))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_graph_merge_singletons: invalid arguments");
+  const int32_t na = *n_agg_inout;
+  std::vector<int32_t> size((size_t)na, 0);
+  for (int i = 0; i < n; ++i) {
+    if (agg_ids[i] < 0 || agg_ids[i] >= na)
+      return mgpbd_fail(MGPBD_EINVAL, "mgpbd_graph_merge_singletons: aggregate id out of range");
+    ++size[agg_ids[i]];
+  }
+  for (int i = 0; i < n; ++i) {
+    const int32_t a = agg_ids[i];
+    if (size[a] != 1) continue;
+    int32_t best = -1;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int32_t c = agg_ids[idx[k]];
+      if (c == a) continue;
+      if (best < 0 || size[c] < size[best] || (size[c] == size[best] && c < best)) best = c;
+    }
+    if (best < 0) continue;
+    size[a] = 0;
+    ++size[best];
+    agg_ids[i] = best;
+  }
+  std::vector<int32_t> renum((size_t)na, -1);
+  int32_t m = 0;
+  for (int32_t a = 0; a < na; ++a)
+    if (size[a] > 0) renum[a] = m++;
+  for (int i = 0; i < n; ++i) agg_ids[i] = renum[agg_ids[i]];
+  *n_agg_inout = m;
+  return 0;
+}
+
 // Aggregate graph induced by a fine graph: I ~ J (I != J) when some fine edge joins them.
 struct mgpbd_graph {
   std::vector<int32_t> ptr, idx;

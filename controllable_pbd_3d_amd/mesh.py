@@ -131,17 +131,18 @@ def ellipsoid_mesh(h: float, radii=(1.0, 0.6, 0.4), jitter: float = 0.05, seed: 
                    rho: float = 1.0, pin_fraction: float = 0.05,
                    inclusion_radius: float = 0.2, E_inclusion: float = 1e8,
                    E_matrix: float = 1e5) -> TetMesh:
-    """Jittered Kuhn grid clipped to an ellipsoid (cubes whose centre lies inside are kept),
-    with a stiff spherical inclusion at the centre and the lowest ``pin_fraction`` of
-    vertices by height pinned. Unused grid vertices are dropped and ids compacted."""
+    """Jittered Kuhn grid clipped to an ellipsoid (cubes whose unjittered centre lies inside
+    are kept, so the connectivity does not depend on the seed), with a stiff spherical
+    inclusion at the centre and the lowest ``pin_fraction`` of vertices by height pinned.
+    Unused grid vertices are dropped and ids compacted."""
     rx, ry, rz = radii
     nx, ny, nz = [int(np.ceil(2 * r / h)) for r in radii]
     origin = (-nx * h / 2, -ny * h / 2, -nz * h / 2)
     full = box_mesh(nx, h, jitter=jitter, seed=seed, rho=rho, pin="none", origin=origin,
                     ny=ny, nz=nz)
     t = full.tets.reshape(-1, 4)
-    cen = full.verts.astype(np.float64)[t].mean(axis=1)
-    cube_c = cen.reshape(-1, 6, 3).mean(axis=1)
+    cx, cy, cz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cube_c = (np.stack([cx, cy, cz], axis=-1).reshape(-1, 3) + 0.5) * h + np.asarray(origin)
     inside = ((cube_c[:, 0] / rx) ** 2 + (cube_c[:, 1] / ry) ** 2 + (cube_c[:, 2] / rz) ** 2) <= 1.0
     keep = np.repeat(inside, 6)
     t = t[keep]

@@ -22,7 +22,7 @@ behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('jacobi' | 'spectral' | 'chebyshev': per-level weights or polynomial from
 lambda_max(D^-1 A)), ``verify_residual``
 (1: FP64 check of every converged PCG solution, restart when it fails; 2: measure only),
-``f_ext``
+``fp64_iterate`` (FP64 PCG iterate and outer SpMV accumulation), ``f_ext``
 (external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
@@ -72,7 +72,7 @@ class MGPBDSolverGPU:
                  coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
                  precision: str = "fp32", cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", verify_residual: int = 0,
-                 f_ext=None, warm_start: str = "none",
+                 fp64_iterate: bool = False, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation != "tet":
@@ -117,6 +117,7 @@ class MGPBDSolverGPU:
         self.overcorrect = float(overcorrect)
         self.smoother = smoother
         self.verify_residual = int(verify_residual)
+        self.fp64_iterate = bool(fp64_iterate)
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -170,7 +171,7 @@ class MGPBDSolverGPU:
             coarse_sweeps=self.coarse_sweeps, galerkin=self.galerkin,
             coarse_solver=self.coarse_solver, precision=self.precision, cycle=self.cycle,
             overcorrect=self.overcorrect, smoother=self.smoother, verify=self.verify_residual,
-            device=self.device.index)
+            fp64_iterate=self.fp64_iterate, device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()
         if self.verbose:
@@ -234,6 +235,20 @@ class MGPBDSolverGPU:
         self.make_prediction()
         self.solve(frame)
         self.update_vel()
+
+    def device_bytes(self) -> dict:
+        """HBM footprint in bytes: per-tet/per-vertex state and the level-0 pattern
+        ("state"), the multigrid hierarchy with its context ("hierarchy") and the warm-start
+        network when one is attached ("network"); each storage is counted once."""
+        from .runtime import device_bytes
+        skip = ("_gpu_solver", "warm_start_model", "record_samples")
+        seen = set()
+        out = {"hierarchy": self._gpu_solver.device_bytes(seen)}
+        out["state"] = device_bytes([v for k, v in vars(self).items() if k not in skip], seen=seen)
+        if self.warm_start_model is not None:
+            out["network"] = device_bytes(self.warm_start_model, seen=seen)
+        out["total"] = sum(out.values())
+        return out
 
     def set_drag(self, vertex, target=None, k: float = 1e3):
         """Spring drag loading (extension): before every prediction, f_ext is zero except

@@ -22,7 +22,10 @@ Extensions (keyword-only, no reference counterpart):
   [lambda_max / cheb_ratio, lambda_max] (same sweeps per phase, fewer PCG iterations);
 * ``cycle='W'`` and ``overcorrect``: W-cycle and a scaled coarse correction;
 * ``verify=1`` (or True): a converged solve is checked with an FP64 residual of the returned x
-  and restarted from x when it misses the tolerance; ``verify=2`` only measures that residual.
+  and restarted from x when it misses the tolerance; ``verify=2`` only measures that residual;
+* ``fp64_iterate=True``: the PCG iterate is accumulated in FP64 and the outer SpMV takes FP64
+  products and sums (on stiff high-contrast systems the rounding of an FP32 iterate alone keeps
+  the true residual above 1e-6).
 """
 
 from __future__ import annotations
@@ -68,7 +71,7 @@ class GPUMultigridSolver:
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
                  smoother_safety: float = 1.1, power_iters: int = 4, cheb_ratio: float = 12.0,
-                 verify: int = 0,
+                 verify: int = 0, fp64_iterate: bool = False,
                  device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
@@ -98,6 +101,7 @@ class GPUMultigridSolver:
         self.power_iters = int(power_iters)
         self.cheb_ratio = float(cheb_ratio)
         self.verify = int(verify)   # 0 off, 1 verify + restart (True), 2 measure only
+        self.fp64_iterate = bool(fp64_iterate)
         self.last_lambda = None
         self.n_levels = 0
         self._initialized = False
@@ -177,7 +181,8 @@ class GPUMultigridSolver:
         self._work = [torch.zeros(8 * 64 * ((L.n_rows + 63) // 64), dtype=torch.float32,
                                   device=self.device) for L in self.levels]
         n0 = self._level_sizes[0]
-        self._pcg_work = torch.zeros(4 * n0, dtype=torch.float32, device=self.device)
+        # r, z, p, Ap and the FP64 iterate
+        self._pcg_work = torch.zeros(6 * n0, dtype=torch.float32, device=self.device)
         self._b = [w[:L.n_rows] for w, L in zip(self._work, self.levels)]
         self._x = [torch.zeros(L.n_rows, dtype=torch.float32, device=self.device)
                    for L in self.levels]
@@ -185,6 +190,10 @@ class GPUMultigridSolver:
         self._setup_half()
         self.update_all_coarse_matrices()
         self._initialized = True
+
+    # reduced-precision levels store int16 column deltas j - i when every |j - i| is below
+    # this bound, int32 columns (shared with the FP32 copy) otherwise
+    col16_limit = 32768
 
     def _setup_half(self):
         """Reduced-precision copies of every level for the V-cycle (extension)."""
@@ -197,7 +206,7 @@ class GPUMultigridSolver:
         for l, L in enumerate(self.levels):
             rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
             delta = L.indices.astype(np.int64) - rows
-            use16 = bool(len(delta) == 0 or np.abs(delta).max() < 32768)
+            use16 = bool(len(delta) == 0 or np.abs(delta).max() < self.col16_limit)
             hb = {
                 "hval": torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev),
                 "col16": torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev) if use16 else None,
@@ -225,6 +234,7 @@ class GPUMultigridSolver:
         ctx("mgpbd_mg_set_smoother", {"jacobi": 0, "spectral": 1, "chebyshev": 2}[self.smoother],
             self.smoother_target, self.smoother_safety, self.power_iters, self.cheb_ratio)
         ctx("mgpbd_pcg_set_verify", int(self.verify))
+        ctx("mgpbd_pcg_set_fp64_iterate", int(self.fp64_iterate))
         self._coarse_mode = mode
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None
@@ -255,6 +265,16 @@ class GPUMultigridSolver:
             self.ctx("mgpbd_mg_convert_half")
         if self.smoother != "jacobi":
             self.ctx("mgpbd_mg_update_smoother", None)
+
+    def device_bytes(self, seen=None) -> int:
+        """HBM held by the hierarchy: level matrices (FP32 and reduced-precision copies),
+        transfers, refill maps, work vectors and the context's own buffers."""
+        from .runtime import device_bytes
+        ctx_bytes = ctypes.c_int64()
+        self.ctx("mgpbd_ctx_device_bytes", ctypes.byref(ctx_bytes))
+        return device_bytes(self.levels, self.transfers, self.refill, self._work,
+                            self._pcg_work, self._x, getattr(self, "_half", None),
+                            seen=seen) + ctx_bytes.value
 
     def coarse_stats(self):
         """(Newton-Schulz steps kept, Cholesky refactorizations) of the coarsest inverse."""

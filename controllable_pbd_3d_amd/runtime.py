@@ -88,3 +88,30 @@ def to_device(a, dtype, device, shape=None, copy=False):
     if shape is not None:
         t = t.reshape(shape)
     return t
+
+
+def device_bytes(*objs, seen=None) -> int:
+    """Bytes of the distinct device tensors reachable from ``objs`` (attributes, dicts, lists),
+    each storage counted once; storages whose address is in ``seen`` are skipped, and the
+    ones counted are added to it."""
+    seen_obj, total = set(), 0
+    seen_mem = set() if seen is None else seen
+    stack = list(objs)
+    while stack:
+        o = stack.pop()
+        if id(o) in seen_obj or o is None:
+            continue
+        seen_obj.add(id(o))
+        if isinstance(o, torch.Tensor):
+            if o.is_cuda:
+                st = o.untyped_storage()
+                if st.data_ptr() not in seen_mem:
+                    seen_mem.add(st.data_ptr())
+                    total += st.nbytes()
+        elif isinstance(o, dict):
+            stack.extend(o.values())
+        elif isinstance(o, (list, tuple)):
+            stack.extend(o)
+        elif hasattr(o, "__dict__") and not isinstance(o, (type, Context)):
+            stack.extend(vars(o).values())
+    return total

@@ -88,11 +88,12 @@ def _graph_from_face_adjacency(adj, deg, n):
 
 
 def build_extended_hierarchy(t_i_np, n_tets, max_coarse=512, max_levels=8, max_agg=8,
-                             first_level="reference"):
+                             first_level="reference", merge_singletons=True):
     """Aggregate maps [(agg_l, n_{l+1})] for a deeper hierarchy (extension, no reference
     counterpart). Level 0 uses the reference's tet aggregation; every further level
     aggregates the graph of aggregates joined by a shared face, with the reference's greedy
-    growth rule, until at most ``max_coarse`` unknowns remain."""
+    growth rule followed (``merge_singletons``) by merging every singleton into the smallest
+    neighbouring aggregate, until at most ``max_coarse`` unknowns remain."""
     lib = _lib.load()
     adj, deg = face_adjacency_arrays(t_i_np, n_tets)
     out = []
@@ -125,6 +126,9 @@ def build_extended_hierarchy(t_i_np, n_tets, max_coarse=512, max_levels=8, max_a
         agg2 = np.empty(nc, np.int32)
         check(lib.mgpbd_graph_aggregate(nc, np_ptr(cptr), np_ptr(cidx), int(max_agg),
                                         np_ptr(agg2), np_ptr(n_agg)), "mgpbd_graph_aggregate")
+        if merge_singletons:
+            check(lib.mgpbd_graph_merge_singletons(nc, np_ptr(cptr), np_ptr(cidx), np_ptr(agg2),
+                                                   np_ptr(n_agg)), "mgpbd_graph_merge_singletons")
         if int(n_agg[0]) >= nc:
             break
         out.append((agg2, int(n_agg[0])))

@@ -49,6 +49,7 @@ def make_solver(mesh, mu_t, max_levels, warm_start="none", model=None, record=No
                    relative_tol=True, tol_cg=1e-6, hierarchy="extended", galerkin="structural",
                    max_coarse=64, max_levels=max_levels, coarse_solver="cholesky",
                    precision=precision, smoother="chebyshev", verify_residual=1,
+                   fp64_iterate=True,
                    warm_start=warm_start, warm_start_model=model, record_samples=record,
                    verbose=False)
     s.init()

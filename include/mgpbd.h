@@ -70,6 +70,13 @@ int mgpbd_tet_aggregate(int n_tets, const int32_t* adj, const int32_t* deg, int 
  * cons/tet_aggregation.py:199-208). */
 int mgpbd_graph_aggregate(int n, const int32_t* ptr, const int32_t* idx, int max_agg,
                           int32_t* agg_ids, int32_t* n_agg);
+/* Extension (no reference counterpart): in node order, a node alone in its aggregate joins
+ * the smallest aggregate among its neighbours' (ties: lowest id); aggregates are renumbered in
+ * ascending order of their old ids with empty ones dropped; *n_agg_inout is updated. Applied
+ * to the graph levels of the extended hierarchy, where greedy growth alone leaves most nodes
+ * of the coarse graphs of irregular meshes as singletons. */
+int mgpbd_graph_merge_singletons(int n, const int32_t* ptr, const int32_t* idx,
+                                 int32_t* agg_ids, int32_t* n_agg_inout);
 typedef struct mgpbd_graph mgpbd_graph;
 mgpbd_graph* mgpbd_graph_coarsen(int n_f, const int32_t* ptr, const int32_t* idx,
                                  const int32_t* agg, int n_c);
@@ -114,6 +121,9 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream);
 int mgpbd_ctx_set_stream(mgpbd_ctx* ctx, void* stream);
 void mgpbd_ctx_destroy(mgpbd_ctx* ctx);
 int mgpbd_ctx_sync(mgpbd_ctx* ctx);
+/* Device memory the context itself holds (reduction partials, PCG state, smoother weights,
+ * coarsest-level dense buffers), in bytes; caller-owned arrays are not counted. */
+int mgpbd_ctx_device_bytes(mgpbd_ctx* ctx, int64_t* bytes);
 
 /* Optional timing of the main kernels with HIP events on the context stream (no reference
  * counterpart; the reference times whole frames in utils/bench.py:24-55). enable resets the
@@ -311,7 +321,7 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
 /* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:
  * 261-329: stop when ||r||^2 < tol^2, breakdown on |pAp| < 1e-30 and on the previous
  * |rz| < 1e-30). relative != 0 (extension) stops on ||r|| < tol ||b||. work_dev: 4*n floats
- * (r, z, p, Ap). Synchronises; *iters_out = completed iterations (reference count);
+ * (r, z, p, Ap), 6*n floats (8-byte aligned) with the FP64 iterate. Synchronises; *iters_out = completed iterations (reference count);
  * info_out (host, 4 doubles) = {||r||^2, ||b||^2, stop reason, FP64 ||b - A x||^2 when
  * verification is on, else -1}. */
 int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* work_dev,
@@ -322,12 +332,20 @@ int mgpbd_pcg_solve(mgpbd_ctx* ctx, const float* b_dev, float* x_dev, float* wor
  * Iterations past convergence are skipped on the device by the done flag. */
 int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll);
 /* Converged-solution verification (extension; the reference stops on the recurrence residual,
- * cons/mgpcg_gpu.py:306-309): with enable != 0, a solve whose recurrence residual met the
- * tolerance computes ||b - A x|| with FP64 products and sums; if that misses the tolerance the
- * PCG restarts from x (at most 3 times, within maxiter) and the stop reason is 0 when it never
- * verifies. Iteration counts include the restarts. enable = 2 computes the FP64 residual of
+ * cons/mgpcg_gpu.py:306-309): with enable != 0, every solve computes ||b - A x|| of the
+ * returned x with FP64 products and sums; when the recurrence residual met the tolerance but
+ * that misses it, the PCG restarts from x (at most 3 times, within maxiter) and the stop
+ * reason is 0 when it never verifies. Iteration counts include the restarts. enable = 2 computes the FP64 residual of
  * every returned x (converged or not) without restarting. */
 int mgpbd_pcg_set_verify(mgpbd_ctx* ctx, int enable);
+/* FP64 iterate (extension; the reference's PCG vectors are FP32, cons/mgpcg_gpu.py:264-272):
+ * with enable != 0 the PCG accumulates x in FP64 (x += alpha p with the FP64 alpha), the outer
+ * A p takes FP64 products and sums (rounded once to FP32), and a verification restart starts
+ * from r = b - A x computed the same way from the FP64 iterate; x_dev receives the FP32
+ * rounding of the iterate. The rounding errors of an FP32 x (one ulp of x per update, each
+ * worth up to kappa(A) eps in the relative residual) otherwise bound the attainable residual
+ * on stiff, high-contrast systems. Needs the 6*n-float work buffer. Default off. */
+int mgpbd_pcg_set_fp64_iterate(mgpbd_ctx* ctx, int enable);
 
 /* ---- Warm start (extension; no reference counterpart: the reference zeroes lambda every
  * substep, cons/mgpbd_gpu.py:402, and starts every PCG from dlambda = 0,

@@ -12,11 +12,17 @@ decides the aggregation's tie-breaks -- is the one the reference sees:
 * ``build_prolongation_from_aggregates`` cons/tet_aggregation.py:130-142
 * ``build_tet_hierarchy``           cons/tet_aggregation.py:145-211
 * ``galerkin_contribution_maps``    cons/mgpcg_gpu.py:43-99
+* ``graph_aggregate``               the growth rule of cons/tet_aggregation.py:80-111 on a CSR
+                                    graph (the extended hierarchy's coarse levels)
+* ``merge_singletons``              extension, no reference counterpart
 
-Pinning (tests/test_oracle_structure.py): the CSR pattern against brute-force ``B B^T != 0``
+Pinning (tests/test_structure.py): the CSR pattern against brute-force ``B B^T != 0``
 of the vertex incidence matrix B; face adjacency against ``(B B^T)_ij == 3``; the
 aggregation's O(deg) candidate form against the literal O(n) set difference; Galerkin
-patterns against scipy ``P^T A P``; contribution maps against a dense re-summation.
+patterns against scipy ``P^T A P``; contribution maps against a dense re-summation;
+``graph_aggregate`` / ``merge_singletons`` against the invariants of their rules (every node
+assigned once, aggregates connected and within max_agg, seeds in ascending degree, no
+singleton left beside a neighbour, compact ids in old-id order) checked by brute force.
 """
 
 from __future__ import annotations
@@ -202,3 +208,63 @@ def galerkin_contribution_maps(A_fine: sp.csr_matrix, P: sp.csr_matrix):
             offsets.append(len(fine_idx))
     return (Ac, np.array(fine_idx, dtype=np.int32), np.array(p_ki, dtype=np.float32),
             np.array(p_lj, dtype=np.float32), np.array(offsets, dtype=np.int32))
+
+
+def graph_aggregate(ptr, idx, max_agg):
+    """Greedy growth of the reference's tet aggregation (cons/tet_aggregation.py:80-111:
+    seeds in stable ascending-degree order; each seed takes its unvisited neighbours in stable
+    ascending-degree order and, while room remains, each taken neighbour's unvisited
+    neighbours in adjacency order, up to max_agg) applied to a CSR graph."""
+    n = len(ptr) - 1
+    deg = [ptr[i + 1] - ptr[i] for i in range(n)]
+    order = sorted(range(n), key=lambda t: deg[t])
+    visited = [False] * n
+    agg = [-1] * n
+    n_agg = 0
+    for seed in order:
+        if visited[seed]:
+            continue
+        agg[seed] = n_agg
+        visited[seed] = True
+        size = 1
+        cand = sorted([j for j in idx[ptr[seed]:ptr[seed + 1]] if not visited[j]],
+                      key=lambda t: deg[t])
+        for nb in cand:
+            if visited[nb]:
+                continue
+            if size >= max_agg:
+                break
+            agg[nb] = n_agg
+            visited[nb] = True
+            size += 1
+            if size < max_agg:
+                for nb2 in idx[ptr[nb]:ptr[nb + 1]]:
+                    if not visited[nb2] and size < max_agg:
+                        agg[nb2] = n_agg
+                        visited[nb2] = True
+                        size += 1
+        n_agg += 1
+    return agg, n_agg
+
+
+def merge_singletons(ptr, idx, agg, n_agg):
+    """Extension (no reference counterpart): in node order a singleton joins the smallest
+    neighbouring aggregate (ties: lowest id, sizes as updated so far); aggregates renumbered in
+    ascending old id, empty ones dropped."""
+    agg = list(agg)
+    size = [0] * n_agg
+    for a in agg:
+        size[a] += 1
+    for i in range(len(agg)):
+        a = agg[i]
+        if size[a] != 1:
+            continue
+        cands = sorted({agg[j] for j in idx[ptr[i]:ptr[i + 1]]} - {a}, key=lambda c: (size[c], c))
+        if not cands:
+            continue
+        size[a] = 0
+        size[cands[0]] += 1
+        agg[i] = cands[0]
+    keep = [a for a in range(n_agg) if size[a] > 0]
+    renum = {a: k for k, a in enumerate(keep)}
+    return [renum[a] for a in agg], len(keep)

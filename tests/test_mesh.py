@@ -61,3 +61,5 @@ def test_ellipsoid_small():
     assert m.tets.max() == m.n_verts - 1
     assert np.isclose(m.pinned.mean(), 0.05, atol=0.01)
     assert set(np.unique(m.E)) <= {1e5, 1e8}
+    m2 = ellipsoid_mesh(0.1, radii=(0.5, 0.3, 0.2), seed=1)
+    assert np.array_equal(m.tets, m2.tets) and not np.array_equal(m.verts, m2.verts)

@@ -195,6 +195,87 @@ def test_extended_hierarchy_invariants():
         assert np.all(np.bincount(agg, minlength=n_c) >= 1)
 
 
+def _random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, n, density=p, random_state=rng, format="csr")
+    M = ((M + M.T) != 0).astype(np.int32).tolil()
+    M.setdiag(0)
+    M = M.tocsr()
+    M.eliminate_zeros()
+    M.sort_indices()
+    return M.indptr.astype(np.int32), M.indices.astype(np.int32)
+
+
+def _check_aggregates(ptr, idx, agg, n_agg, max_size=None):
+    """Every node in one aggregate, ids compact, aggregates connected within themselves."""
+    agg = np.asarray(agg)
+    n = len(ptr) - 1
+    assert agg.shape == (n,) and (n == 0 or (agg.min() == 0 and agg.max() == n_agg - 1))
+    sizes = np.bincount(agg, minlength=n_agg)
+    assert np.all(sizes >= 1)
+    if max_size is not None:
+        assert sizes.max() <= max_size
+    for a in range(n_agg):
+        members = set(np.flatnonzero(agg == a).tolist())
+        start = next(iter(members))
+        seen, stack = {start}, [start]
+        while stack:
+            u = stack.pop()
+            for v in idx[ptr[u]:ptr[u + 1]]:
+                if v in members and v not in seen:
+                    seen.add(int(v))
+                    stack.append(int(v))
+        assert seen == members, a
+
+
+@pytest.mark.parametrize("n,p,seed,max_agg", [(60, 0.05, 0, 8), (200, 0.02, 1, 8),
+                                             (300, 0.01, 2, 4), (150, 0.1, 3, 8)])
+def test_graph_aggregate_and_singleton_merge_match_oracle(n, p, seed, max_agg):
+    """libmgpbd's graph aggregation and singleton merge equal the oracle restatements, and
+    both satisfy their rules' invariants: aggregates connected and within max_agg after the
+    growth, seeds (lowest-degree members) in ascending degree; after the merge aggregates
+    are still connected and no singleton has a neighbour."""
+    from oracle import structure as st
+    lib = _lib.load()
+    ptr, idx = _random_graph(n, p, seed)
+    agg = np.empty(n, np.int32)
+    na = np.zeros(1, np.int32)
+    _lib.check(lib.mgpbd_graph_aggregate(n, _lib.np_ptr(ptr), _lib.np_ptr(idx), max_agg,
+                                         _lib.np_ptr(agg), _lib.np_ptr(na)))
+    ref_agg, ref_n = st.graph_aggregate(ptr.tolist(), idx.tolist(), max_agg)
+    assert int(na[0]) == ref_n and agg.tolist() == ref_agg
+    _check_aggregates(ptr, idx, agg, int(na[0]), max_agg)
+    deg = np.diff(ptr)
+    seeds = [min(np.flatnonzero(agg == a), key=lambda t: (deg[t], t)) for a in range(int(na[0]))]
+    assert all(deg[s] <= deg[t] for s, t in zip(seeds, seeds[1:]))   # seeds by degree
+    merged = agg.copy()
+    nm = na.copy()
+    _lib.check(lib.mgpbd_graph_merge_singletons(n, _lib.np_ptr(ptr), _lib.np_ptr(idx),
+                                                _lib.np_ptr(merged), _lib.np_ptr(nm)))
+    ref_m, ref_nm = st.merge_singletons(ptr.tolist(), idx.tolist(), agg.tolist(), int(na[0]))
+    assert int(nm[0]) == ref_nm and merged.tolist() == ref_m
+    _check_aggregates(ptr, idx, merged, int(nm[0]))
+    sizes = np.bincount(merged, minlength=int(nm[0]))
+    for i in range(n):
+        if sizes[merged[i]] == 1:
+            assert ptr[i + 1] == ptr[i]          # only isolated nodes stay alone
+
+
+def test_extended_hierarchy_coarsens_an_irregular_mesh():
+    """On a clipped (ellipsoidal) grid the graph levels keep coarsening by at least 2x per
+    level down to max_coarse, which the greedy growth alone does not."""
+    from controllable_pbd_3d_amd.mesh import ellipsoid_mesh
+    m = ellipsoid_mesh(0.04, seed=0)
+    aggs = ta.build_extended_hierarchy(m.tets, m.n_tets, max_coarse=64, max_levels=8)
+    sizes = [m.n_tets] + [n for _, n in aggs]
+    plain = ta.build_extended_hierarchy(m.tets, m.n_tets, max_coarse=64, max_levels=8,
+                                        merge_singletons=False)
+    print(f"[ellipsoid h=0.04] levels {sizes}; without the merge "
+          f"{[m.n_tets] + [n for _, n in plain]}")
+    assert sizes[-1] <= 64
+    assert all(2 * b <= a for a, b in zip(sizes[1:], sizes[2:]))
+
+
 def test_sell_layout_roundtrip():
     """SELL-64x4: width a multiple of 4 covering the longest row of the slice; entry k of row
     r at slice_ptr[r/64] + 256 (k/4) + 4 (r%64) + k%4; padding points at the row itself."""
