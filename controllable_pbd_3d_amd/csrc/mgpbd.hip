@@ -1138,13 +1138,16 @@ k_pcg_p(int n, const float* __restrict__ z, float* __restrict__ p, const PcgStat
 // Reduced-precision V-cycle (extension). Each level is used in its symmetrically scaled form
 // A^ = D^-1/2 A D^-1/2 (unit diagonal, |a^_ij| <= 1 for SPD A, so FP16's range suffices),
 // values stored as FP16 or BF16 in the level's SELL positions, products accumulated in FP32.
+// Levels whose column offsets j - i fit 16 bits store one packed 32-bit word per entry,
+// (j - i) << 16 | half bits, so a lane's four entries arrive in one 16-byte load; the others
+// keep the FP32 level's int32 columns beside a 16-bit value array.
 // With x = D^-1/2 x^ and b = D^1/2 b^ the weighted-Jacobi V-cycle on A^ is the V-cycle on A:
 //   sweep     x^' = x^ + omega (b^ - A^ x^)                 (diag(A^) = 1)
 //   restrict  b^_c[I] = isd_c[I] sum_{i in I} sd_i r^_i
 //   prolong   x^_i += sd_i isd_c[agg_i] x^_c[agg_i]
 // ------------------------------------------------------------------------------------------
 typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
-typedef short v4s16 __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 
 template <int VT>   // 1: FP16, 2: BF16
 __device__ __forceinline__ float h2f(unsigned short u) {
@@ -1159,10 +1162,10 @@ __device__ __forceinline__ unsigned short f2h(float f) {
   return (unsigned short)(u >> 16);
 }
 
-template <int VT, bool DELTA>
+template <int VT, bool PACKED>
 __device__ __forceinline__ float sellh_row(int i, const int* __restrict__ slice_ptr,
                                            const int* __restrict__ idx,
-                                           const int16_t* __restrict__ col16,
+                                           const uint32_t* __restrict__ hpk,
                                            const uint16_t* __restrict__ hval,
                                            const float* __restrict__ x) {
   const int p0 = sell_base(slice_ptr, i);
@@ -1171,19 +1174,24 @@ __device__ __forceinline__ float sellh_row(int i, const int* __restrict__ slice_
 #pragma unroll 4
   for (int k0 = 0; k0 < w; k0 += 4) {
     const int p = p0 + 64 * k0;
-    const v4u16 hv = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(hval + p));
     int c0, c1, c2, c3;
-    if (DELTA) {
-      const v4s16 d = __builtin_nontemporal_load(reinterpret_cast<const v4s16*>(col16 + p));
-      c0 = i + d.x; c1 = i + d.y; c2 = i + d.z; c3 = i + d.w;
+    unsigned short h0, h1, h2, h3;
+    if (PACKED) {
+      const v4u32 e = __builtin_nontemporal_load(reinterpret_cast<const v4u32*>(hpk + p));
+      c0 = i + ((int)e.x >> 16); c1 = i + ((int)e.y >> 16);
+      c2 = i + ((int)e.z >> 16); c3 = i + ((int)e.w >> 16);
+      h0 = (unsigned short)e.x; h1 = (unsigned short)e.y;
+      h2 = (unsigned short)e.z; h3 = (unsigned short)e.w;
     } else {
+      const v4u16 hv = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(hval + p));
       const v4i c = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(idx + p));
       c0 = c.x; c1 = c.y; c2 = c.z; c3 = c.w;
+      h0 = hv.x; h1 = hv.y; h2 = hv.z; h3 = hv.w;
     }
-    acc = fmaf(h2f<VT>(hv.x), x[c0], acc);
-    acc = fmaf(h2f<VT>(hv.y), x[c1], acc);
-    acc = fmaf(h2f<VT>(hv.z), x[c2], acc);
-    acc = fmaf(h2f<VT>(hv.w), x[c3], acc);
+    acc = fmaf(h2f<VT>(h0), x[c0], acc);
+    acc = fmaf(h2f<VT>(h1), x[c1], acc);
+    acc = fmaf(h2f<VT>(h2), x[c2], acc);
+    acc = fmaf(h2f<VT>(h3), x[c3], acc);
   }
   return acc;
 }
@@ -1191,10 +1199,10 @@ __device__ __forceinline__ float sellh_row(int i, const int* __restrict__ slice_
 // MODE 1: y = x + omega (b - A^x);  MODE 2: y = b - A^x;  MODE 3: y = isd (x + omega (b - A^x));
 // MODE 4 (Chebyshev step, unit diagonal): y = x + c1 (x - xp) + c2 (b - A^x);  MODE 5: isd times
 // the MODE 4 value
-template <int VT, bool DELTA, int MODE>
+template <int VT, bool PACKED, int MODE>
 __global__ void __launch_bounds__(kBlock)
 k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
-        const int16_t* __restrict__ col16, const uint16_t* __restrict__ hval,
+        const uint32_t* __restrict__ hpk, const uint16_t* __restrict__ hval,
         const float* __restrict__ x, float* y, const float* __restrict__ b,
         float omega, const float* __restrict__ wp, const float* __restrict__ isd, const int* __restrict__ done,
         const float* xp, const float* __restrict__ cc) {
@@ -1202,7 +1210,7 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
-  const float ax = sellh_row<VT, DELTA>(i, slice_ptr, idx, col16, hval, x);
+  const float ax = sellh_row<VT, PACKED>(i, slice_ptr, idx, hpk, hval, x);
   if (i >= n) return;
   if (MODE == 2) {
     y[i] = b[i] - ax;
@@ -1230,17 +1238,15 @@ k_half_scales(int n, const int* __restrict__ diag_pos, const float* __restrict__
   isd[i] = 1.0f / r;
 }
 
-// hval = half(val_ij isd_i isd_j) over the whole padded row, four entries (one 16-byte index
-// and value load, one 8-byte store) at a time; col16 = j - i (padding: 0) when col16 != NULL
-// (the pattern is fixed, so the deltas are written once).
-typedef uint16_t v4u16 __attribute__((ext_vector_type(4)));
-typedef int16_t v4s16 __attribute__((ext_vector_type(4)));
+// half(val_ij isd_i isd_j) over the whole padded row, four entries (one 16-byte index and
+// value load) at a time, stored as packed words (j - i) << 16 | half (one 16-byte store;
+// padding: 0) when hpk != NULL, else into hval (one 8-byte store)
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
               const int* __restrict__ idx, const float* __restrict__ val,
               const float* __restrict__ isd, uint16_t* __restrict__ hval,
-              int16_t* __restrict__ col16) {
+              uint32_t* __restrict__ hpk) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
   const int p0 = sell_base(slice_ptr, i);
@@ -1256,14 +1262,15 @@ k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
     h.y = f2h<VT>(row ? v.y * si * isd[c.y] : 0.0f);
     h.z = f2h<VT>(row ? v.z * si * isd[c.z] : 0.0f);
     h.w = f2h<VT>(row ? v.w * si * isd[c.w] : 0.0f);
-    *reinterpret_cast<v4u16*>(hval + p) = h;
-    if (col16) {
-      v4s16 d;
-      d.x = (int16_t)(row ? c.x - i : 0);
-      d.y = (int16_t)(row ? c.y - i : 0);
-      d.z = (int16_t)(row ? c.z - i : 0);
-      d.w = (int16_t)(row ? c.w - i : 0);
-      *reinterpret_cast<v4s16*>(col16 + p) = d;
+    if (hpk) {
+      v4u32 e;
+      e.x = (row ? (uint32_t)(c.x - i) << 16 : 0u) | h.x;
+      e.y = (row ? (uint32_t)(c.y - i) << 16 : 0u) | h.y;
+      e.z = (row ? (uint32_t)(c.z - i) << 16 : 0u) | h.z;
+      e.w = (row ? (uint32_t)(c.w - i) << 16 : 0u) | h.w;
+      *reinterpret_cast<v4u32*>(hpk + p) = e;
+    } else {
+      *reinterpret_cast<v4u16*>(hval + p) = h;
     }
   }
 }
@@ -1394,13 +1401,13 @@ struct MgLevel {
                            // 64 * n_slices long (padded rows addressable)
   int diagonal = 0;
   // reduced-precision copy for the V-cycle (MGPBD_PREC_FP16 / _BF16): values of the scaled
-  // level D^-1/2 A D^-1/2 in the SELL positions, optional 16-bit column deltas
+  // level D^-1/2 A D^-1/2 in the SELL positions, packed with 16-bit column offsets (hpk) or
+  // beside the int32 columns (hval)
   uint16_t* hval = nullptr;
-  int16_t* col16 = nullptr;
+  uint32_t* hpk = nullptr;
   float* sd = nullptr;      // sqrt(A_ii)
   float* isd = nullptr;     // 1 / sqrt(A_ii)
   const int* diag_pos = nullptr;
-  bool col16_ready = false;  // the deltas depend on the pattern only: written once
   float* wb(int k) const { return work + (size_t)k * 64 * n_slices; }
 };
 
@@ -1881,7 +1888,7 @@ static int check_ready(mgpbd_ctx* ctx) {
     return mgpbd_fail(MGPBD_EINVAL, "multigrid: the dense coarsest solve needs two levels or more");
   if (ctx->precision != MGPBD_PREC_FP32)
     for (int l = 0; l < ctx->n_levels; ++l)
-      if (!ctx->L[l].hval)
+      if (!ctx->L[l].hval && !ctx->L[l].hpk)
         return mgpbd_fail(MGPBD_EINVAL, "multigrid: reduced precision without half buffers on level " + std::to_string(l));
   return 0;
 }
@@ -2101,8 +2108,8 @@ static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float
   hipStream_t s = ctx->stream;
   const float w = ctx->omega;
 #define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
-                                                         L.col16, L.hval, x, y, b, w, wp, L.isd, done, xp, cc)
-  if (L.col16) {
+                                                         L.hpk, L.hval, x, y, b, w, wp, L.isd, done, xp, cc)
+  if (L.hpk) {
     switch (mode) {
       case 1: SELLH(true, 1); break;
       case 2: SELLH(true, 2); break;
@@ -2322,14 +2329,13 @@ int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible) {
   return 0;
 }
 
-int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, int16_t* col16,
+int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, uint32_t* hpacked,
                             float* sd, float* isd, const int32_t* diag_pos) {
   CTX_CHECK("mgpbd_mg_set_level_half");
-  if (level < 0 || level >= ctx->n_levels || !hval || !sd || !isd || !diag_pos)
+  if (level < 0 || level >= ctx->n_levels || (!hval) == (!hpacked) || !sd || !isd || !diag_pos)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_half: invalid arguments");
   MgLevel& L = ctx->L[level];
-  L.hval = hval; L.col16 = col16; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
-  L.col16_ready = false;
+  L.hval = hpacked ? nullptr : hval; L.hpk = hpacked; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
   if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   return 0;
 }
@@ -2340,15 +2346,13 @@ int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
   if (ctx->precision == MGPBD_PREC_FP32) return 0;
   for (int l = 0; l < ctx->n_levels; ++l) {
     MgLevel& L = ctx->L[l];
-    if (!L.hval) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
+    if (!L.hval && !L.hpk) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
     k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
     const int g = nblocks(64LL * L.n_slices);
-    int16_t* cols = L.col16_ready ? nullptr : L.col16;
     if (ctx->precision == MGPBD_PREC_FP16)
-      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, cols);
+      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk);
     else
-      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, cols);
-    L.col16_ready = true;
+      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk);
   }
   MGPBD_LAUNCH_CHECK("mgpbd_mg_convert_half");
   return 0;

@@ -191,8 +191,8 @@ class GPUMultigridSolver:
         self.update_all_coarse_matrices()
         self._initialized = True
 
-    # reduced-precision levels store int16 column deltas j - i when every |j - i| is below
-    # this bound, int32 columns (shared with the FP32 copy) otherwise
+    # reduced-precision levels pack 16-bit column offsets j - i with the values when every
+    # |j - i| is below this bound, and use int32 columns (shared with the FP32 copy) otherwise
     col16_limit = 32768
 
     def _setup_half(self):
@@ -207,13 +207,15 @@ class GPUMultigridSolver:
             rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
             delta = L.indices.astype(np.int64) - rows
             use16 = bool(len(delta) == 0 or np.abs(delta).max() < self.col16_limit)
+            slots = max(L.padded, 1)
             hb = {
-                "hval": torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev),
-                "col16": torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev) if use16 else None,
+                # packed (col - row) << 16 | half words, or half values beside int32 columns
+                "hpk": torch.zeros(slots, dtype=torch.int32, device=dev) if use16 else None,
+                "hval": None if use16 else torch.zeros(slots, dtype=torch.int16, device=dev),
                 "sd": torch.ones(L.n_rows, dtype=torch.float32, device=dev),
                 "isd": torch.ones(L.n_rows, dtype=torch.float32, device=dev),
             }
-            self.ctx("mgpbd_mg_set_level_half", l, dptr(hb["hval"]), dptr(hb["col16"]),
+            self.ctx("mgpbd_mg_set_level_half", l, dptr(hb["hval"]), dptr(hb["hpk"]),
                      dptr(hb["sd"]), dptr(hb["isd"]), dptr(L.diag_pos))
             self._half.append(hb)
 

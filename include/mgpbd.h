@@ -307,11 +307,11 @@ int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out, float* cheb_out);
 #define MGPBD_PREC_FP16 1
 #define MGPBD_PREC_BF16 2
 int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible);
-/* Per-level buffers: hval_dev (uint16, one per SELL slot), col16_dev (int16 column - row per
- * SELL slot, or NULL to use the int32 columns; only valid when |col - row| < 32768),
- * sd_dev / isd_dev (n_rows floats, written by mgpbd_mg_convert_half), diag_pos_dev (SELL
- * position of each row's diagonal). */
-int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, int16_t* col16_dev,
+/* Per-level buffers, exactly one of: hpacked_dev (uint32 per SELL slot, (col - row) << 16 |
+ * half bits; only valid when |col - row| < 32768 on the whole level) or hval_dev (uint16 per
+ * SELL slot, with the level's int32 columns); sd_dev / isd_dev (n_rows floats, written by
+ * mgpbd_mg_convert_half), diag_pos_dev (SELL position of each row's diagonal). */
+int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, uint32_t* hpacked_dev,
                             float* sd_dev, float* isd_dev, const int32_t* diag_pos_dev);
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */

@@ -357,7 +357,11 @@ def test_half_levels_are_scaled_and_rounded(precision):
         isd = 1.0 / np.sqrt(d)
         S = sp.diags(isd) @ A @ sp.diags(isd)
         S = S.tocsr()
-        raw = hb["hval"].cpu().numpy().view(np.uint16)[L.sell_pos_host]
+        if hb["hpk"] is not None:
+            words = hb["hpk"].cpu().numpy()
+            raw = (words & 0xFFFF).astype(np.uint16)[L.sell_pos_host]
+        else:
+            raw = hb["hval"].cpu().numpy().view(np.uint16)[L.sell_pos_host]
         got = raw.view(np.float16).astype(np.float64) if precision == "fp16" else \
             _bf16_to_f32(raw).astype(np.float64)
         want = np.asarray(S[np.repeat(np.arange(L.n_rows), np.diff(L.indptr)), L.indices]).reshape(-1)
@@ -368,13 +372,14 @@ def test_half_levels_are_scaled_and_rounded(precision):
         assert np.all(np.abs(S.diagonal() - 1.0) < 1e-5)
         if l == 0:
             rows = np.repeat(np.arange(L.n_rows), np.diff(L.indptr))
-            c16 = hb["col16"].cpu().numpy()[L.sell_pos_host].astype(np.int64)
+            c16 = (hb["hpk"].cpu().numpy() >> 16)[L.sell_pos_host].astype(np.int64)
             assert np.array_equal(c16, L.indices - rows)
 
 
-def test_half_levels_with_int32_columns_match_int16_deltas(monkeypatch):
-    """Levels whose column deltas do not fit int16 keep int32 columns; the arithmetic is the
-    same, so a PCG solve with every level forced onto int32 columns is bit-identical."""
+def test_half_levels_with_int32_columns_match_packed_offsets(monkeypatch):
+    """Levels whose column offsets do not fit 16 bits keep int32 columns beside 16-bit values;
+    the arithmetic is that of the packed levels, so a PCG solve with every level forced onto
+    int32 columns is bit-identical."""
     from controllable_pbd_3d_amd.mgpcg_gpu import GPUMultigridSolver
     mesh, s, o = _extended_pair(n=6, seed=7)
     v0 = s.v_p.cpu().numpy()
@@ -384,8 +389,8 @@ def test_half_levels_with_int32_columns_match_int16_deltas(monkeypatch):
     s16 = _solver(mesh, v0, **kw)
     monkeypatch.setattr(GPUMultigridSolver, "col16_limit", 0)
     s32 = _solver(mesh, v0, **kw)
-    assert all(hb["col16"] is not None for hb in s16._gpu_solver._half)
-    assert all(hb["col16"] is None for hb in s32._gpu_solver._half)
+    assert all(hb["hpk"] is not None and hb["hval"] is None for hb in s16._gpu_solver._half)
+    assert all(hb["hpk"] is None and hb["hval"] is not None for hb in s32._gpu_solver._half)
     out = []
     for z in (s16, s32):
         z._compute_C_and_gradC()
