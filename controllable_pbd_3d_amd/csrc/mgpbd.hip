@@ -2270,7 +2270,7 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
     const bool moving = !(std::fabs(ph[0] - ph[2]) <= 0.02 * ph[0]);
     if (ctx->pv_updates[l] < 4) iters = std::max(iters, 12);
     else if (moving) iters = std::max(iters, 8);
-    else iters = std::min(iters, 2);   // the previous refresh had settled
+    // else power_iters steps: the settled vector follows the matrix from refresh to refresh
     ++ctx->pv_updates[l];
     const int g = nblocks(64LL * L.n_slices);
     double* pw = ctx->pw_dev + 4 * l;

@@ -22,7 +22,8 @@ behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 ('jacobi' | 'spectral' | 'chebyshev': per-level weights or polynomial from
 lambda_max(D^-1 A)), ``verify_residual``
 (1: FP64 check of every converged PCG solution, restart when it fails; 2: measure only),
-``fp64_iterate`` (FP64 PCG iterate and outer SpMV accumulation), ``f_ext``
+``fp64_iterate`` (FP64 PCG iterate and outer SpMV accumulation), ``power_iters`` (power
+steps per refresh of the spectral estimates once settled), ``f_ext``
 (external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
@@ -72,7 +73,7 @@ class MGPBDSolverGPU:
                  coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
                  precision: str = "fp32", cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", verify_residual: int = 0,
-                 fp64_iterate: bool = False, f_ext=None, warm_start: str = "none",
+                 fp64_iterate: bool = False, power_iters: int = 2, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation != "tet":
@@ -118,6 +119,7 @@ class MGPBDSolverGPU:
         self.smoother = smoother
         self.verify_residual = int(verify_residual)
         self.fp64_iterate = bool(fp64_iterate)
+        self.power_iters = int(power_iters)
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -171,7 +173,8 @@ class MGPBDSolverGPU:
             coarse_sweeps=self.coarse_sweeps, galerkin=self.galerkin,
             coarse_solver=self.coarse_solver, precision=self.precision, cycle=self.cycle,
             overcorrect=self.overcorrect, smoother=self.smoother, verify=self.verify_residual,
-            fp64_iterate=self.fp64_iterate, device=self.device.index)
+            fp64_iterate=self.fp64_iterate, power_iters=self.power_iters,
+            device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()
         if self.verbose:

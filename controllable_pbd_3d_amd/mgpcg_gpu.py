@@ -70,7 +70,7 @@ class GPUMultigridSolver:
                  coarse_solver: str = "jacobi", precision: str = "fp32",
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
-                 smoother_safety: float = 1.1, power_iters: int = 4, cheb_ratio: float = 12.0,
+                 smoother_safety: float = 1.1, power_iters: int = 2, cheb_ratio: float = 12.0,
                  verify: int = 0, fp64_iterate: bool = False,
                  device=None):
         self.n_smooth = int(n_smooth)

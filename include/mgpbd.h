@@ -282,7 +282,7 @@ int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect);
  * level l relaxes with omega_l = target / (safety lambda_l), lambda_l the largest eigenvalue
  * of D^-1 A_l estimated by power iteration, warm-started (12 iterations for the first four
  * updates; then 8 after an update whose last two estimates differed by more than 2 %, else
- * min(power_iters, 2); an unsettled estimate is raised by a quarter); omega_l lambda_max < 2 keeps the V-cycle an SPD
+ * power_iters; an unsettled estimate is raised by a quarter); omega_l lambda_max < 2 keeps the V-cycle an SPD
  * preconditioner, which the fixed 2/3 loses once lambda_max(D^-1 A) > 3. mode 2 = Chebyshev:
  * every smoothing phase of a level above the coarsest applies the Chebyshev polynomial of
  * D^-1 A_l on [safety lambda_l / cheb_ratio, safety lambda_l] (step 1 a Jacobi sweep with
