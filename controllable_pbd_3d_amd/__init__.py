@@ -5,6 +5,8 @@ Public API mirrors the reference's cons/ modules:
   GPUMultigridSolver  (cons/mgpcg_gpu.py)
   GPUSparseMatrix, gpu_* BLAS-1 (cons/sparse_gpu.py)
   tet_aggregation.*   (cons/tet_aggregation.py)
+  AdaptiveZone, MGPBDAdaptiveSolver (cons/adaptive_zone.py, cons/mgpbd_adaptive.py)
+  SolverBenchmark     (utils/bench.py)
 """
 
 from ._lib import MGPBDError, load as load_library  # noqa: F401
@@ -21,4 +23,10 @@ def __getattr__(name):
     if name == "GPUSparseMatrix":
         from .sparse_gpu import GPUSparseMatrix
         return GPUSparseMatrix
+    if name in ("AdaptiveZone", "MGPBDAdaptiveSolver"):
+        from . import adaptive
+        return getattr(adaptive, name)
+    if name == "SolverBenchmark":
+        from .benchmark import SolverBenchmark
+        return SolverBenchmark
     raise AttributeError(name)

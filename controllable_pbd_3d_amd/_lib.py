@@ -100,6 +100,11 @@ SIGNATURES = {
     "mgpbd_pcg_solve_guess": (c_int, [_V, _V, _V, _V, _V, c_int, c_double, c_int,
                                       ctypes.POINTER(c_int), F64P]),
     "mgpbd_spring_drag": (c_int, [_V, c_int, _V, _V, c_int, c_float, c_float, c_float, c_float]),
+    # spatial adaptivity
+    "mgpbd_zone_proximity": (c_int, [_V, c_int, _V, _V, c_float, c_float, c_float, c_float, _V]),
+    "mgpbd_zone_strain": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float, _V]),
+    "mgpbd_zone_expand": (c_int, [_V, c_int, _V, _V, _V, c_int]),
+    "mgpbd_zone_mask_alpha": (c_int, [_V, c_int, _V, _V, _V, c_float]),
     # learned warm start
     "mgpbd_gnn_param_count": (c_int, []),
     "mgpbd_gnn_workspace_bytes": (c_int64, [c_int]),

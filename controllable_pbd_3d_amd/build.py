@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmgpbd.so")
 
-HIP_SOURCES = ["mgpbd.hip", "gnn.hip"]
+HIP_SOURCES = ["mgpbd.hip", "gnn.hip", "adaptive.hip"]
 CXX_SOURCES = ["setup.cpp", "common.cpp"]
 # packed FP32 instructions are disabled for every device object (build flags)
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -368,6 +368,28 @@ int mgpbd_pcg_solve_guess(mgpbd_ctx* ctx, const float* b_dev, float* x_dev,
 int mgpbd_spring_drag(mgpbd_ctx* ctx, int n_verts, const float* v_p_dev, float* f_ext_dev,
                       int vertex, float tx, float ty, float tz, float k);
 
+/* ---- Spatial adaptivity (the callers of the solve in cons/adaptive_zone.py and
+ * cons/mgpbd_adaptive.py). The zone is one int32 flag per tet (1 active, 0 passive) on the
+ * device; all entry points queue on the context stream and do not synchronise. ---- */
+/* active[t] = 1 when some vertex of t lies strictly within radius of (cx, cy, cz), FP32
+ * |p - c|^2 < radius^2 (replaces AdaptiveZone.update_by_proximity, cons/adaptive_zone.py:35-48). */
+int mgpbd_zone_proximity(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
+                         float cx, float cy, float cz, float radius, int32_t* active_dev);
+/* active[t] = 1 when max over the vertices of t of |x - x_old| / dt exceeds threshold, else 0
+ * (replaces AdaptiveZone.update_by_strain, cons/adaptive_zone.py:50-64). dt > 0. */
+int mgpbd_zone_strain(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
+                      const float* v_p_old_dev, float threshold, float dt, int32_t* active_dev);
+/* n_rings rings of face neighbours added to the zone, in place; nbr_dev: n_tets x 4 face
+ * neighbours, -1 padded (mgpbd_face_adjacency); scratch_dev: n_tets ints, not aliasing active
+ * (replaces AdaptiveZone.expand_zone, cons/adaptive_zone.py:66-80). */
+int mgpbd_zone_expand(mgpbd_ctx* ctx, int n_tets, const int32_t* nbr_dev, int32_t* active_dev,
+                      int32_t* scratch_dev, int n_rings);
+/* saved = alpha; alpha[t] = passive_value where active[t] == 0 (replaces
+ * MGPBDAdaptiveSolver._mask_inactive_kernel, cons/mgpbd_adaptive.py:64-73, passive_value
+ * 1e20 there); the restore is a copy of saved back into alpha. */
+int mgpbd_zone_mask_alpha(mgpbd_ctx* ctx, int n_tets, const int32_t* active_dev, float* alpha_dev,
+                          float* saved_dev, float passive_value);
+
 /* ---- Learned warm start (extension): a message-passing network over the tet face-adjacency
  * graph (nbr_dev: 4 neighbours per tet, -1 padded, as from mgpbd_face_adjacency) predicting
  * dlambda of the first Newton iteration.

@@ -1,0 +1,101 @@
+"""GPU checks of the adaptive zone kernels and the adaptive wrapper against the oracle
+(cons/adaptive_zone.py, cons/mgpbd_adaptive.py), and of the SolverBenchmark hooks
+(utils/bench.py) on the GPU solver.
+
+Bars: zone flags bit-exact (integer results of FP32 comparisons evaluated in the same
+operation order); the restored alpha~ bit-exact; positions of adaptive substeps within
+1e-4 h of the FP32 oracle's adaptive substeps.
+"""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from oracle import adaptive_oracle as ao       # noqa: E402
+from oracle import mgpbd_oracle as om          # noqa: E402
+from oracle import structure                   # noqa: E402
+from tests.scenarios import deformed, physical_block, solver_kwargs   # noqa: E402
+
+
+def _solver(mesh, v0=None, **kw):
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    args = solver_kwargs(mesh, **kw)
+    if v0 is not None:
+        args["v_p"] = v0.copy()
+    s = MGPBDSolverGPU(**args, verbose=False)
+    s.init()
+    return s
+
+
+def test_zone_kernels_match_oracle():
+    from controllable_pbd_3d_amd.adaptive import AdaptiveZone
+    mesh, mu_t = physical_block(6, seed=21)
+    x = deformed(mesh, seed=21, amp=0.1, twist=0.2)
+    xd = torch.from_numpy(x).cuda()
+    zone = AdaptiveZone(mesh.n_tets, torch.from_numpy(mesh.tets).cuda(), xd)
+    assert zone.get_active_fraction() == 1.0
+    adj = structure.build_tet_face_adjacency(mesh.tets, mesh.n_tets)
+    rng = np.random.default_rng(0)
+    lo, hi = x.min(axis=0), x.max(axis=0)
+    for k in range(12):
+        c = lo + rng.random(3) * (hi - lo)
+        r = float(rng.uniform(0.02, 0.15))
+        zone.update_by_proximity(*c, r)
+        want = ao.proximity(mesh.tets, x, c.astype(np.float32), r)
+        got = zone.tet_active.cpu().numpy()
+        assert np.array_equal(got, want), k
+        rings = k % 4
+        zone.expand_zone(rings)
+        assert np.array_equal(zone.tet_active.cpu().numpy(), ao.expand(adj, want, rings)), k
+    xo = (x + rng.normal(0, 0.002, x.shape)).astype(np.float32)
+    for thr in (1.0, 1.5, 2.0):
+        zone.update_by_strain(torch.from_numpy(xo).cuda(), thr, 1.0 / 300.0)
+        assert np.array_equal(zone.tet_active.cpu().numpy(),
+                              ao.strain(mesh.tets, x, xo, thr, 1.0 / 300.0)), thr
+    print(f"[zone] last strain fraction {zone.get_active_fraction():.3f}", flush=True)
+
+
+def test_adaptive_substeps_match_oracle():
+    """Five adaptive substeps with the tool moving across the block: the zone, the restored
+    alpha~ and the positions agree with the oracle's adaptive substeps."""
+    from controllable_pbd_3d_amd.adaptive import AdaptiveZone, MGPBDAdaptiveSolver
+    mesh, mu_t = physical_block(5, seed=22)
+    v0 = deformed(mesh, seed=22, amp=0.02, twist=0.05)
+    kw = dict(mu_per_tet=mu_t, hold_pinned=True, relative_tol=True, tol_cg=1e-6)
+    s = _solver(mesh, v0, **kw)
+    o = om.OracleMGPBD(**solver_kwargs(mesh, v_p=v0.copy(), **kw), precision="f32")
+    o.init()
+    alpha0 = s.alpha_tilde.cpu().numpy().copy()
+    zone = AdaptiveZone(s.n_tets, s.t_i, s.v_p)
+    ad = MGPBDAdaptiveSolver(s, zone)
+    adj = structure.build_tet_face_adjacency(mesh.tets, mesh.n_tets)
+    c0 = mesh.verts.mean(axis=0)
+    worst = 0.0
+    for k in range(5):
+        tool = c0 + np.array([0.04 * (k - 2), 0.0, 0.02])
+        ad.step(0, tool_pos=tool, tool_radius=0.06)
+        act = ao.adaptive_step(o, adj, None, 0, tool_pos=tool.astype(np.float32), tool_radius=0.06)
+        assert np.array_equal(zone.tet_active.cpu().numpy(), act), k
+        assert 0.0 < act.mean() < 1.0
+        assert np.array_equal(s.alpha_tilde.cpu().numpy(), alpha0)
+        worst = max(worst, float(np.abs(s.v_p.cpu().numpy() - o.v_p).max()))
+    print(f"[adaptive] zone fraction {zone.get_active_fraction():.3f}, max|dx|/h "
+          f"{worst / mesh.h:.2e}, gpu {s.last_stats}, oracle {o.history[-1]}", flush=True)
+    assert worst <= 1e-4 * mesh.h
+
+
+def test_solver_benchmark_hooks_record_the_newton_loop():
+    from controllable_pbd_3d_amd.benchmark import SolverBenchmark
+    mesh, mu_t = physical_block(4, seed=23)
+    b = SolverBenchmark("MGPBD-GPU", "box4", {"n_tets": mesh.n_tets})
+    s = _solver(mesh, None, mu_per_tet=mu_t, benchmark=b)
+    for f in range(3):
+        s.step(f)
+        fr = b.frames[-1]
+        assert fr["frame_id"] == f and fr["t_end"] >= fr["t_start"]
+        assert [(d["iteration"], d["residual"], d["cg_iters"], d["omega"]) for d in fr["iterations"]] == \
+            [(i, float(r), c, float(w)) for i, r, c, w in s.last_stats]
+    assert b.summary()["n_frames"] == 3
