@@ -285,6 +285,7 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
            "fp64_rel_residual_max": float(rel.max()) if len(rel) else None,
            "fp64_rel_residual_median": float(np.median(rel)) if len(rel) else None}
     res["hbm_footprint_bytes"] = s.device_bytes()
+    res["pcg_rescues"], res["pcg_restarts"] = s._gpu_solver.pcg_counters()
     ns, refac = s._gpu_solver.coarse_stats()
     res["coarsest_inverse"] = {"newton_schulz_steps": ns, "cholesky_refactors": refac}
     if kw.get("warm_start"):

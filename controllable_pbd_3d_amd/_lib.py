@@ -55,6 +55,8 @@ SIGNATURES = {
     "mgpbd_ctx_use_graphs": (c_int, [_V, c_int, c_int]),
     "mgpbd_pcg_set_verify": (c_int, [_V, c_int]),
     "mgpbd_pcg_set_fp64_iterate": (c_int, [_V, c_int]),
+    "mgpbd_pcg_set_rescue": (c_int, [_V, c_int, c_float, c_double]),
+    "mgpbd_pcg_counters": (c_int, [_V, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),

@@ -1457,6 +1457,13 @@ struct mgpbd_ctx {
   int poll = 4;
   int verify = 0;   // 1: FP64 check of the converged x, restart from x when it fails; 2: check only
   bool fp64_iterate = false;   // FP64 PCG iterate, FP64-accumulated outer SpMV and restart residual
+  // stall rescue (spectral smoothers): a PCG whose residual falls less than 2x over 8
+  // iterations re-estimates every level's spectrum from a fresh start vector with the safety
+  // factor times rescue_boost and restarts from its iterate (once per solve)
+  bool rescue = false;
+  float rescue_boost = 1.25f;
+  double stall_ratio = 0.25;   // ||r||^2 ratio over 8 iterations counted as a stall
+  int64_t n_rescues = 0, n_restarts = 0;
   // HIP graph of `poll` PCG iterations (+ the state copy), replayed on `work` stream
   bool use_graph = true;
   hipStream_t work = nullptr;
@@ -2239,10 +2246,10 @@ int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, 
   return 0;
 }
 
-int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
-  CTX_CHECK("mgpbd_mg_update_smoother");
-  if (int e = check_levels(ctx)) return e;
-  if (!ctx->spectral) return 0;
+// Refresh of the spectral estimates and the weights/coefficients derived from them. fresh:
+// restart every level's power iteration from the deterministic start vector with 12 steps
+// (the stall rescue of the PCG); boost multiplies the safety factor of this refresh.
+static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
   hipStream_t s = ctx->stream;
   for (int l = 0; l < ctx->n_levels; ++l) {
     MgLevel& L = ctx->L[l];
@@ -2257,10 +2264,10 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
     float* v = L.wb(6);
     float* w = L.wb(7);
     int iters = ctx->power_iters;
-    if (!ctx->pv_ready[l]) {
+    if (!ctx->pv_ready[l] || fresh) {
       k_power_init<<<nblocks(L.n), kBlock, 0, s>>>(L.n, v);
+      if (!ctx->pv_ready[l]) ctx->pv_updates[l] = 0;
       ctx->pv_ready[l] = true;
-      ctx->pv_updates[l] = 0;
     }
     // the first updates after a (re)set start far from the top eigenvector (at rest the
     // level-0 matrix is diagonal): longer runs until the vector has followed the matrix; also
@@ -2268,7 +2275,7 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
     // pinned copy of that update, which the Newton loop's own synchronisations have completed)
     const double* ph = ctx->pw_host + 4 * l;
     const bool moving = !(std::fabs(ph[0] - ph[2]) <= 0.02 * ph[0]);
-    if (ctx->pv_updates[l] < 4) iters = std::max(iters, 12);
+    if (ctx->pv_updates[l] < 4 || fresh) iters = std::max(iters, 12);
     else if (moving) iters = std::max(iters, 8);
     // else power_iters steps: the settled vector follows the matrix from refresh to refresh
     ++ctx->pv_updates[l];
@@ -2280,13 +2287,22 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
       k_power_fin<<<1, kBlock, 0, s>>>(ctx->ws, g, pw);
       k_power_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, w, v, pw);
     }
-    k_power_omega<<<1, 64, 0, s>>>(pw, ctx->sm_target, ctx->sm_safety, ctx->omega,
+    k_power_omega<<<1, 64, 0, s>>>(pw, ctx->sm_target, ctx->sm_safety * boost, ctx->omega,
                                    cheb_level(ctx, l) ? 1 : 0, ctx->cheb_ratio,
                                    ctx->omega_dev + l, ctx->cheb_dev + 2 * kChebSteps * l);
   }
   MGPBD_LAUNCH_CHECK("mgpbd_mg_update_smoother");
   MGPBD_HIP_TRY(hipMemcpyAsync(ctx->pw_host, ctx->pw_dev, sizeof(double) * 4 * MGPBD_MAX_LEVELS,
                                hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
+  CTX_CHECK("mgpbd_mg_update_smoother");
+  if (int e = check_levels(ctx)) return e;
+  if (!ctx->spectral) return 0;
+  hipStream_t s = ctx->stream;
+  if (int e = smoother_refresh(ctx, false, 1.0f)) return e;
   if (lambda_out) {
     double tmp[4 * MGPBD_MAX_LEVELS];
     MGPBD_HIP_TRY(hipMemcpyAsync(tmp, ctx->pw_dev, sizeof(tmp), hipMemcpyDeviceToHost, s));
@@ -2500,6 +2516,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
 
   const bool fuse_first = first && !half_mode(ctx);
   int issued = 0, total_iters = 0, restarts = 0;
+  bool rescued = false, first_round = true;
   bool from_guess = guess;
   const float* alt = x_alt;
   double first_info[4] = {0.0, 0.0, 0.0, 0.0};   // rr0, start, rr_guess, rr_alt of round 0
@@ -2509,7 +2526,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     if (from_guess) {
       // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
       // start with the smallest residual; a restart of the FP64 iterate takes r = b - A x64
-      if (x64 && restarts > 0)
+      if (x64 && (restarts > 0 || rescued))
         k_resid64<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x64, b, r);
       else
         k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
@@ -2523,7 +2540,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
                                                  fuse_first ? xv : nullptr, ctx->omega,
                                                  wl(ctx, 0), ctx->st);
       if (x64)
-        k_x64_from_start<<<nblocks(n), kBlock, 0, s>>>(n, x, x64, restarts > 0, ctx->st);
+        k_x64_from_start<<<nblocks(n), kBlock, 0, s>>>(n, x, x64, restarts > 0 || rescued, ctx->st);
     } else {
       k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr,
                                         ctx->omega, wl(ctx, 0), ctx->ws, x64);
@@ -2537,6 +2554,11 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
 
     int round_issued = 0;
+    const bool watch = ctx->rescue && ctx->spectral && !rescued;
+    const int back = std::max(1, 8 / ctx->poll);   // polls spanning 8 iterations
+    double rr_poll[64];
+    int n_poll = 0;
+    bool stalled = false;
     while (issued < maxiter) {
       if (graph_mode && maxiter - issued >= ctx->poll) {
         MGPBD_HIP_TRY(hipGraphLaunch(ctx->graph, s));
@@ -2553,6 +2575,15 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       MGPBD_HIP_TRY(hipStreamSynchronize(s));
       if (int e = prof_flush(ctx)) return e;
       if (ctx->st_host->done) break;
+      if (watch) {
+        rr_poll[n_poll % 64] = ctx->st_host->rr;
+        ++n_poll;
+        if (round_issued >= 16 && n_poll > back &&
+            ctx->st_host->rr > ctx->stall_ratio * rr_poll[(n_poll - 1 - back) % 64]) {
+          stalled = true;
+          break;
+        }
+      }
     }
     if (round_issued == 0) {
       MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
@@ -2562,14 +2593,24 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       k_x_from64<<<nblocks(n), kBlock, 0, s>>>(n, x64, x);
       MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(x)");
     }
-    if (restarts == 0) {
+    if (first_round) {
+      first_round = false;
       const PcgState& h = *ctx->st_host;
       first_info[0] = guess ? h.rr0 : h.bb;
       first_info[1] = (double)h.start;
       first_info[2] = h.rr_guess;
       first_info[3] = h.rr_alt;
     }
-    total_iters += ctx->st_host->iters;
+    total_iters += stalled ? round_issued : ctx->st_host->iters;
+    if (stalled && issued < maxiter) {
+      // the V-cycle no longer contracts: the spectral estimates lag the matrix
+      rescued = true;
+      ++ctx->n_rescues;
+      if (int e = smoother_refresh(ctx, true, ctx->rescue_boost)) return e;
+      from_guess = true;
+      alt = nullptr;
+      continue;
+    }
     if (!ctx->verify) break;
     // FP64 residual of the returned x (every solve); a restart from x when a converged one
     // misses the tolerance
@@ -2583,6 +2624,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
         issued >= maxiter || restarts >= 3)
       break;
     ++restarts;
+    ++ctx->n_restarts;
     from_guess = true;
     alt = nullptr;
   }
@@ -2600,6 +2642,23 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
                           first_info[3], rr_true};
     for (int i = 0; i < n_info; ++i) info_out[i] = n_info == 4 ? v4[i] : v8[i];
   }
+  return 0;
+}
+
+int mgpbd_pcg_set_rescue(mgpbd_ctx* ctx, int enable, float boost, double stall_ratio) {
+  CTX_CHECK("mgpbd_pcg_set_rescue");
+  if (!(boost >= 1.0f && boost <= 4.0f) || !(stall_ratio >= 0.0 && stall_ratio <= 1.0))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_set_rescue: boost in [1, 4], stall_ratio in [0, 1]");
+  ctx->rescue = enable != 0;
+  ctx->rescue_boost = boost;
+  ctx->stall_ratio = stall_ratio;
+  return 0;
+}
+
+int mgpbd_pcg_counters(mgpbd_ctx* ctx, int64_t* rescues, int64_t* restarts) {
+  CTX_CHECK("mgpbd_pcg_counters");
+  if (rescues) *rescues = ctx->n_rescues;
+  if (restarts) *restarts = ctx->n_restarts;
   return 0;
 }
 

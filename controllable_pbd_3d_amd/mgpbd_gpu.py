@@ -23,7 +23,10 @@ behaviour): ``hold_pinned``, ``mu_per_tet``, ``relative_tol``, ``hierarchy``
 lambda_max(D^-1 A)), ``verify_residual``
 (1: FP64 check of every converged PCG solution, restart when it fails; 2: measure only),
 ``fp64_iterate`` (FP64 PCG iterate and outer SpMV accumulation), ``power_iters`` (power
-steps per refresh of the spectral estimates once settled), ``f_ext``
+steps per refresh of the spectral estimates once settled), ``smoother_safety`` and
+``cheb_ratio`` (upper end safety factor over the estimate and interval ratio of the spectral
+smoothers), ``stall_rescue`` (re-estimate the spectra and restart a PCG that stops
+contracting; spectral smoothers only), ``f_ext``
 (external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
@@ -73,7 +76,9 @@ class MGPBDSolverGPU:
                  coarse_sweeps: int = 50, coarse_solver: str = "jacobi",
                  precision: str = "fp32", cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", verify_residual: int = 0,
-                 fp64_iterate: bool = False, power_iters: int = 2, f_ext=None, warm_start: str = "none",
+                 fp64_iterate: bool = False, power_iters: int = 2,
+                 smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
+                 stall_rescue: bool = True, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation != "tet":
@@ -120,6 +125,9 @@ class MGPBDSolverGPU:
         self.verify_residual = int(verify_residual)
         self.fp64_iterate = bool(fp64_iterate)
         self.power_iters = int(power_iters)
+        self.smoother_safety = float(smoother_safety)
+        self.cheb_ratio = float(cheb_ratio)
+        self.stall_rescue = bool(stall_rescue)
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -174,6 +182,8 @@ class MGPBDSolverGPU:
             coarse_solver=self.coarse_solver, precision=self.precision, cycle=self.cycle,
             overcorrect=self.overcorrect, smoother=self.smoother, verify=self.verify_residual,
             fp64_iterate=self.fp64_iterate, power_iters=self.power_iters,
+            smoother_safety=self.smoother_safety, cheb_ratio=self.cheb_ratio,
+            stall_rescue=self.stall_rescue,
             device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()

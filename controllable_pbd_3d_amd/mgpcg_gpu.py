@@ -25,7 +25,9 @@ Extensions (keyword-only, no reference counterpart):
   and restarted from x when it misses the tolerance; ``verify=2`` only measures that residual;
 * ``fp64_iterate=True``: the PCG iterate is accumulated in FP64 and the outer SpMV takes FP64
   products and sums (on stiff high-contrast systems the rounding of an FP32 iterate alone keeps
-  the true residual above 1e-6).
+  the true residual above 1e-6);
+* ``stall_rescue`` (default on with the spectral smoothers): a PCG that stops contracting
+  re-estimates the spectra from a fresh vector with a raised safety factor and restarts.
 """
 
 from __future__ import annotations
@@ -71,7 +73,7 @@ class GPUMultigridSolver:
                  flexible: bool | None = None, cycle: str = "V", overcorrect: float = 1.0,
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
                  smoother_safety: float = 1.1, power_iters: int = 2, cheb_ratio: float = 12.0,
-                 verify: int = 0, fp64_iterate: bool = False,
+                 verify: int = 0, fp64_iterate: bool = False, stall_rescue: bool = True,
                  device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
@@ -102,6 +104,8 @@ class GPUMultigridSolver:
         self.cheb_ratio = float(cheb_ratio)
         self.verify = int(verify)   # 0 off, 1 verify + restart (True), 2 measure only
         self.fp64_iterate = bool(fp64_iterate)
+        self.stall_rescue = bool(stall_rescue)
+        self.stall_ratio = 0.25   # squared-residual ratio over 8 iterations counted as a stall
         self.last_lambda = None
         self.n_levels = 0
         self._initialized = False
@@ -237,6 +241,8 @@ class GPUMultigridSolver:
             self.smoother_target, self.smoother_safety, self.power_iters, self.cheb_ratio)
         ctx("mgpbd_pcg_set_verify", int(self.verify))
         ctx("mgpbd_pcg_set_fp64_iterate", int(self.fp64_iterate))
+        ctx("mgpbd_pcg_set_rescue", int(self.stall_rescue and self.smoother != "jacobi"), 1.25,
+            self.stall_ratio)
         self._coarse_mode = mode
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None
@@ -277,6 +283,12 @@ class GPUMultigridSolver:
         return device_bytes(self.levels, self.transfers, self.refill, self._work,
                             self._pcg_work, self._x, getattr(self, "_half", None),
                             seen=seen) + ctx_bytes.value
+
+    def pcg_counters(self):
+        """(stall rescues, verification restarts) over this solver's solves."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self.ctx("mgpbd_pcg_counters", ctypes.byref(a), ctypes.byref(b))
+        return int(a.value), int(b.value)
 
     def coarse_stats(self):
         """(Newton-Schulz steps kept, Cholesky refactorizations) of the coarsest inverse."""

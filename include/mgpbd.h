@@ -346,6 +346,18 @@ int mgpbd_pcg_set_verify(mgpbd_ctx* ctx, int enable);
  * worth up to kappa(A) eps in the relative residual) otherwise bound the attainable residual
  * on stiff, high-contrast systems. Needs the 6*n-float work buffer. Default off. */
 int mgpbd_pcg_set_fp64_iterate(mgpbd_ctx* ctx, int enable);
+/* Stall rescue (extension; spectral and Chebyshev smoothers only): with enable != 0, a PCG
+ * round whose squared recurrence residual falls by less than 1 / stall_ratio over 8
+ * iterations (stall_ratio 0.25: less than 2x in the norm; checked from the 16th iteration
+ * on, at the host's polls) stops, every level's spectral estimate is recomputed
+ * from a fresh start vector (12 power steps) with the safety factor times boost (in [1, 4]),
+ * and the PCG restarts from its iterate, at most once per solve, within maxiter. Under
+ * abrupt loading (a new spring drag) the warm-started estimates can lag the largest
+ * eigenvalue; the polynomial smoother then amplifies the modes above its interval and the
+ * V-cycle stops contracting. Default off. */
+int mgpbd_pcg_set_rescue(mgpbd_ctx* ctx, int enable, float boost, double stall_ratio);
+/* Cumulative counts over the context's solves: stall rescues and verification restarts. */
+int mgpbd_pcg_counters(mgpbd_ctx* ctx, int64_t* rescues, int64_t* restarts);
 
 /* ---- Warm start (extension; no reference counterpart: the reference zeroes lambda every
  * substep, cons/mgpbd_gpu.py:402, and starts every PCG from dlambda = 0,

@@ -614,6 +614,45 @@ def test_fp64_iterate_reaches_the_rounding_floor(stiff_scene):
     assert abs(info_g["rr_true"] / info_g["bb"] - rel_g ** 2) <= 1e-6 * rel_g ** 2
 
 
+def test_stall_rescue_restarts_and_still_verifies(stiff_scene):
+    """The stall rescue (fresh spectral estimates, raised safety, restart from the iterate),
+    forced at the 16th iteration by a zero stall ratio: the solve still meets the verified
+    tolerance, lands within the tolerance's reach of the unforced solution, and the counters
+    record exactly one rescue; an unforced solve afterwards converges as before."""
+    mesh, mu_t, v0 = stiff_scene
+    s = _solver(mesh, v0, mu_per_tet=mu_t, smoother="spectral", relative_tol=True, tol_cg=1e-6,
+                maxiter_cg=300, hierarchy="extended", galerkin="structural", max_coarse=64,
+                coarse_solver="cholesky", precision="fp16", verify_residual=1,
+                fp64_iterate=True, n_smooth=1)
+    g = s._gpu_solver
+    s._compute_C_and_gradC()
+    x1 = torch.zeros_like(s._b_gpu)
+    it1 = g.solve(s._b_gpu, x1)
+    info1 = dict(g.last_info)
+    assert g.pcg_counters()[0] == 0
+    g.ctx("mgpbd_pcg_set_rescue", 1, 1.25, 0.0)
+    x2 = torch.zeros_like(s._b_gpu)
+    it2 = g.solve(s._b_gpu, x2)
+    info2 = dict(g.last_info)
+    resc = g.pcg_counters()[0]
+    g.ctx("mgpbd_pcg_set_rescue", 1, 1.25, 0.25)
+    x3 = torch.zeros_like(s._b_gpu)
+    it3 = g.solve(s._b_gpu, x3)
+    info3 = dict(g.last_info)
+    A = sp.csr_matrix((s._A.csr_data().astype(np.float64), s._A_indices, s._A_indptr),
+                      shape=(mesh.n_tets,) * 2)
+    b = s._b_gpu.cpu().numpy().astype(np.float64)
+    rel = [np.linalg.norm(b - A @ x.cpu().numpy().astype(np.float64)) / np.linalg.norm(b)
+           for x in (x1, x2, x3)]
+    d = float((x2 - x1).norm() / x1.norm())
+    print(f"[rescue] iterations {it1} / forced {it2} / after {it3}; rescues {resc}; FP64 rel "
+          f"{rel}; |x2 - x1| / |x1| = {d:.2e}", flush=True)
+    assert it1 > 16
+    assert info1["reason"] == info2["reason"] == info3["reason"] == 1
+    assert resc == 1 and all(r < 1e-6 for r in rel)
+    assert d < 1e-2   # both within 1e-6 relative residual; kappa(A) ~ 1e4 on this block
+
+
 def test_coarsest_inverse_refresh_keeps_solves_verified():
     """Across substeps the coarsest inverse is refreshed by Newton-Schulz steps from the
     previous one (kept only when ||I - A X||_F < 0.1); every solve still meets the FP64-verified
