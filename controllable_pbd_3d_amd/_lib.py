@@ -32,6 +32,7 @@ SIGNATURES = {
     "mgpbd_face_adjacency": (c_int, [c_int, I32P, I32P, I32P]),
     "mgpbd_tet_aggregate": (c_int, [c_int, I32P, I32P, c_int, I32P, I32P]),
     "mgpbd_graph_aggregate": (c_int, [c_int, I32P, I32P, c_int, I32P, I32P]),
+    "mgpbd_standard_aggregate": (c_int, [c_int, I32P, I32P, I32P, I32P]),
     "mgpbd_graph_merge_singletons": (c_int, [c_int, I32P, I32P, I32P, I32P]),
     "mgpbd_graph_coarsen": (_V, [c_int, I32P, I32P, I32P, c_int]),
     "mgpbd_graph_nnz": (c_int64, [_V]),

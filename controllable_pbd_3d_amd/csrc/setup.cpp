@@ -307,6 +307,67 @@ int mgpbd_graph_aggregate(int n, const int32_t* ptr, const int32_t* idx, int max
   return 0;
 }
 
+// pyamg's standard aggregation (the `aggregation='pyamg'` branch, cons/mgpbd_gpu.py:124-134,
+// calls pyamg.smoothed_aggregation_solver with its default 'standard' aggregate on the
+// 'symmetric' strength graph; pyamg is not vendored: this restates its published three-pass
+// algorithm, Vanek-Mandel-Brezina 1996, in pyamg's node order):
+//  1. in node order, a node none of whose neighbours is aggregated yet forms an aggregate
+//     with all its neighbours; a node without off-diagonal neighbours stays out (isolated);
+//  2. in node order, every node still free joins the aggregate of its first neighbour (in
+//     adjacency order) that was aggregated in pass 1;
+//  3. in node order, a node still free starts a new aggregate with its still-free neighbours.
+// agg_ids[i] = aggregate of i, -1 for isolated nodes; aggregates numbered in creation order.
+int mgpbd_standard_aggregate(int n, const int32_t* ptr, const int32_t* idx, int32_t* agg_ids,
+                             int32_t* n_agg_out) {
+  if (n < 0 || !n_agg_out || (n > 0 && (!ptr || !idx || !agg_ids)))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_standard_aggregate: invalid arguments");
+  // x: 0 free, k > 0 member of pass-1 aggregate k-1, -k joined pass-1 aggregate k-1 in pass 2,
+  // -n-1 isolated (kept distinct from every -k)
+  std::vector<int64_t> x((size_t)n, 0);
+  const int64_t iso = -(int64_t)n - 1;
+  int64_t next = 1;
+  for (int i = 0; i < n; ++i) {
+    if (x[i]) continue;
+    bool has_nbr = false, has_agg_nbr = false;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int32_t j = idx[k];
+      if (j == i) continue;
+      has_nbr = true;
+      if (x[j]) { has_agg_nbr = true; break; }
+    }
+    if (!has_nbr) {
+      x[i] = iso;
+    } else if (!has_agg_nbr) {
+      x[i] = next;
+      for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) x[idx[k]] = next;
+      ++next;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    if (x[i]) continue;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int64_t xj = x[idx[k]];
+      if (xj > 0) { x[i] = -xj; break; }
+    }
+  }
+  // final ids of passes 1-2 (0-based, in creation order), -1 isolated, -2 still free
+  for (int i = 0; i < n; ++i) {
+    const int64_t xi = x[i];
+    x[i] = xi > 0 ? xi - 1 : xi == iso ? -1 : xi < 0 ? -xi - 1 : -2;
+  }
+  --next;   // aggregates made so far; pass 3 numbers on from here
+  for (int i = 0; i < n; ++i) {
+    if (x[i] != -2) continue;
+    x[i] = next;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (x[idx[k]] == -2) x[idx[k]] = next;
+    ++next;
+  }
+  for (int i = 0; i < n; ++i) agg_ids[i] = (int32_t)x[i];
+  *n_agg_out = (int32_t)next;
+  return 0;
+}
+
 // Singleton merge for the graph levels of the extended hierarchy (no reference counterpart):
 // in node order, a node that is alone in its aggregate joins the smallest aggregate among its
 // neighbours' (ties: the lowest id; sizes as updated so far); aggregates are then renumbered

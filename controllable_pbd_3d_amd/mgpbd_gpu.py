@@ -81,9 +81,8 @@ class MGPBDSolverGPU:
                  stall_rescue: bool = True, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
-        if aggregation != "tet":
-            raise NotImplementedError("aggregation='pyamg' (smoothed aggregation) is not "
-                                      "available; only the tet-aggregation hierarchy is built")
+        if aggregation not in ("tet", "pyamg"):
+            raise ValueError("aggregation must be 'tet' or 'pyamg'")
         self.ctx = get_context(device)
         dev = self.device = self.ctx.device
         self.v_p = to_device(v_p, torch.float32, dev, (-1, 3))
@@ -215,7 +214,14 @@ class MGPBDSolverGPU:
         self._A.set_pattern(indptr, indices[:nnz], diag_csr_pos=indptr[1:] - 1)
 
     def _build_hierarchy(self):
-        if self.hierarchy == "reference":
+        if self.aggregation == "pyamg":
+            # cons/mgpbd_gpu.py:124-134: pyamg standard aggregation on the matrix pattern,
+            # max_coarse 400, Galerkin patterns from the values at init (as pyamg's products)
+            vals = self._A.csr_data() if self.galerkin == "values" else None
+            self._aggs = tet_aggregation.build_pyamg_hierarchy(
+                self._A_indptr, self._A_indices, vals, max_coarse=400,
+                max_levels=self.max_levels or 10)
+        elif self.hierarchy == "reference":
             self._aggs = tet_aggregation.reference_aggregates(self._t_i_np, self.n_tets,
                                                               max_coarse=self.max_coarse)
         elif self.hierarchy == "extended":

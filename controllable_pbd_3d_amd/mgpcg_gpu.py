@@ -52,13 +52,21 @@ def _members(agg: np.ndarray, n_c: int):
 
 
 def _aggs_from_Ps(Ps):
-    """Piecewise-constant unit P (one 1.0 per row) -> aggregate maps."""
+    """Piecewise-constant P (one positive entry per row, equal within each column: the unit P
+    of tet aggregation or pyamg's tentative 1/sqrt(|aggregate|) one) -> aggregate maps. A
+    column scaling of P does not change the V-cycle, so both run on the unit-P kernels."""
     out = []
     for P in Ps:
         P = sp.csr_matrix(P)
-        if not (np.all(np.diff(P.indptr) == 1) and np.all(P.data == 1.0)):
-            raise ValueError("GPUMultigridSolver supports piecewise-constant P with unit "
-                             "entries (one 1.0 per row), as tet aggregation builds")
+        ok = np.all(np.diff(P.indptr) == 1) and np.all(P.data > 0)
+        if ok:
+            col = P.indices
+            first = np.full(P.shape[1], np.nan)
+            first[col[::-1]] = P.data[::-1]
+            ok = np.all(P.data == first[col])
+        if not ok:
+            raise ValueError("GPUMultigridSolver supports piecewise-constant P (one positive "
+                             "entry per row, constant within each column)")
         out.append((P.indices.astype(np.int32), P.shape[1]))
     return out
 

@@ -10,6 +10,14 @@ The reference hierarchy has at most two levels: its coarse "virtual tets" repeat
 times, so they have no shared faces, aggregate into singletons and the attempt is popped.
 ``build_extended_hierarchy`` (no reference counterpart) keeps coarsening on the aggregate
 graph induced by face adjacency, with the same greedy growth rule.
+
+``standard_aggregation`` / ``build_pyamg_hierarchy`` are the ``aggregation='pyamg'`` branch
+(cons/mgpbd_gpu.py:124-134): pyamg's standard aggregation on the symmetric strength graph with
+theta = 0 (every stored entry of the level matrix), level after level on the Galerkin
+patterns until at most ``max_coarse`` unknowns remain. pyamg's prolongator there
+(``smooth=None``) is the tentative one, 1/sqrt(|aggregate|) per row; it differs from the unit
+P of these maps by a diagonal column scaling, which leaves the V-cycle (Jacobi smoothing,
+Galerkin coarse levels, coarse correction) unchanged in exact arithmetic.
 """
 
 from __future__ import annotations
@@ -133,4 +141,65 @@ def build_extended_hierarchy(t_i_np, n_tets, max_coarse=512, max_levels=8, max_a
             break
         out.append((agg2, int(n_agg[0])))
         gptr, gidx, n = cptr, cidx, nc
+    return out
+
+
+def standard_aggregation(indptr, indices):
+    """pyamg's standard aggregation of a CSR graph: (agg int32[n] with -1 for isolated nodes,
+    n_agg)."""
+    lib = _lib.load()
+    indptr = np.ascontiguousarray(indptr, np.int32)
+    indices = np.ascontiguousarray(indices, np.int32)
+    n = len(indptr) - 1
+    agg = np.empty(max(n, 1), np.int32)
+    na = np.zeros(1, np.int32)
+    check(lib.mgpbd_standard_aggregate(n, np_ptr(indptr), np_ptr(indices), np_ptr(agg),
+                                       np_ptr(na)), "mgpbd_standard_aggregate")
+    return agg[:n], int(na[0])
+
+
+def build_pyamg_hierarchy(indptr, indices, values=None, max_coarse=400, max_levels=10):
+    """Aggregate maps [(agg_l, n_{l+1})] of the ``aggregation='pyamg'`` hierarchy for a level-0
+    matrix pattern. ``values`` (the level-0 values, optional) makes the coarse patterns drop
+    entries whose Galerkin sums are exactly zero, as pyamg's sparse products do; without them
+    the coarse patterns are structural. Isolated nodes (no off-diagonal entry), which pyamg
+    leaves out of every aggregate, become singleton aggregates here (the unit-P transfer needs
+    one aggregate per row); coarsening stops when a level does not shrink."""
+    import ctypes
+    lib = _lib.load()
+    ptr = np.ascontiguousarray(indptr, np.int32)
+    idx = np.ascontiguousarray(indices, np.int32)
+    vals = None if values is None else np.ascontiguousarray(values, np.float32)
+    n = len(ptr) - 1
+    out = []
+    while len(out) + 1 < max_levels and n > max_coarse:
+        agg, na = standard_aggregation(ptr, idx)
+        iso = np.flatnonzero(agg < 0)
+        if len(iso):
+            agg[iso] = na + np.arange(len(iso), dtype=np.int32)
+            na += len(iso)
+        if na >= n or na == 0:
+            break
+        out.append((agg, na))
+        g = lib.mgpbd_galerkin_build(n, np_ptr(ptr), np_ptr(idx),
+                                     None if vals is None else vals.ctypes.data_as(_lib.F32P),
+                                     np_ptr(agg), na)
+        if not g:
+            check(-1, "mgpbd_galerkin_build")
+        cnnz, ncon = ctypes.c_int64(), ctypes.c_int64()
+        check(lib.mgpbd_galerkin_sizes(g, ctypes.byref(cnnz), ctypes.byref(ncon)),
+              "mgpbd_galerkin_sizes")
+        cptr = np.empty(na + 1, np.int32)
+        cidx = np.empty(max(cnnz.value, 1), np.int32)
+        cdiag = np.empty(na, np.int32)
+        fidx = np.empty(max(ncon.value, 1), np.int32)
+        coff = np.empty(cnnz.value + 1, np.int32)
+        check(lib.mgpbd_galerkin_copy(g, np_ptr(cptr), np_ptr(cidx), np_ptr(cdiag), np_ptr(fidx),
+                                      np_ptr(coff)), "mgpbd_galerkin_copy")
+        lib.mgpbd_galerkin_free(g)
+        if vals is not None:   # coarse values: FP64 sums of the fine values in map order
+            seg = np.repeat(np.arange(cnnz.value), np.diff(coff))
+            vals = np.bincount(seg, weights=vals[fidx[:ncon.value]].astype(np.float64),
+                               minlength=cnnz.value).astype(np.float32)
+        ptr, idx, n = cptr, cidx[:cnnz.value], na
     return out

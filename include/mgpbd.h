@@ -70,6 +70,15 @@ int mgpbd_tet_aggregate(int n_tets, const int32_t* adj, const int32_t* deg, int 
  * cons/tet_aggregation.py:199-208). */
 int mgpbd_graph_aggregate(int n, const int32_t* ptr, const int32_t* idx, int max_agg,
                           int32_t* agg_ids, int32_t* n_agg);
+/* pyamg's standard aggregation (the aggregation='pyamg' branch, cons/mgpbd_gpu.py:124-134;
+ * pyamg itself is not vendored): pass 1 in node order makes a node whose neighbours are all
+ * free an aggregate with them (a node without off-diagonal neighbours stays out, id -1);
+ * pass 2 puts every free node into the pass-1 aggregate of its first such neighbour (adjacency
+ * order); pass 3 makes each still-free node an aggregate with its free neighbours. ptr/idx:
+ * the strength graph (pyamg's symmetric strength with theta = 0 keeps every stored entry of
+ * A, so the matrix pattern itself). */
+int mgpbd_standard_aggregate(int n, const int32_t* ptr, const int32_t* idx, int32_t* agg_ids,
+                             int32_t* n_agg_out);
 /* Extension (no reference counterpart): in node order, a node alone in its aggregate joins
  * the smallest aggregate among its neighbours' (ties: lowest id); aggregates are renumbered in
  * ascending order of their old ids with empty ones dropped; *n_agg_inout is updated. Applied

@@ -15,6 +15,10 @@ decides the aggregation's tie-breaks -- is the one the reference sees:
 * ``graph_aggregate``               the growth rule of cons/tet_aggregation.py:80-111 on a CSR
                                     graph (the extended hierarchy's coarse levels)
 * ``merge_singletons``              extension, no reference counterpart
+* ``standard_aggregation``          pyamg's standard aggregation (the aggregation='pyamg'
+                                    branch, cons/mgpbd_gpu.py:124-134; pyamg is absent: its
+                                    published three-pass algorithm, parity unpinned against
+                                    pyamg itself)
 
 Pinning (tests/test_structure.py): the CSR pattern against brute-force ``B B^T != 0``
 of the vertex incidence matrix B; face adjacency against ``(B B^T)_ij == 3``; the
@@ -22,7 +26,10 @@ aggregation's O(deg) candidate form against the literal O(n) set difference; Gal
 patterns against scipy ``P^T A P``; contribution maps against a dense re-summation;
 ``graph_aggregate`` / ``merge_singletons`` against the invariants of their rules (every node
 assigned once, aggregates connected and within max_agg, seeds in ascending degree, no
-singleton left beside a neighbour, compact ids in old-id order) checked by brute force.
+singleton left beside a neighbour, compact ids in old-id order) checked by brute force;
+``standard_aggregation`` against the invariants of its three passes (pass-1 aggregates are
+whole closed neighbourhoods of pairwise non-adjacent roots, each pass-2 node is adjacent to
+its aggregate's root neighbourhood, pass-3 aggregates are connected, isolated nodes -1).
 """
 
 from __future__ import annotations
@@ -268,3 +275,44 @@ def merge_singletons(ptr, idx, agg, n_agg):
     keep = [a for a in range(n_agg) if size[a] > 0]
     renum = {a: k for k, a in enumerate(keep)}
     return [renum[a] for a in agg], len(keep)
+
+
+def standard_aggregation(ptr, idx):
+    """pyamg's three-pass standard aggregation of a CSR graph (Vanek, Mandel, Brezina 1996):
+    pass 1, in node order: a node none of whose neighbours is aggregated forms an aggregate
+    with all of them (no off-diagonal neighbour: isolated, -1); pass 2, in node order: a free
+    node joins the pass-1 aggregate of its first neighbour, in adjacency order, that pass 1
+    aggregated; pass 3, in node order: a free node starts an aggregate with its free
+    neighbours. Returns (agg, n_agg)."""
+    n = len(ptr) - 1
+    FREE, ISO = None, -1
+    agg = [FREE] * n
+    pass1 = [False] * n
+    n_agg = 0
+    for i in range(n):
+        if agg[i] is not FREE:
+            continue
+        nbrs = [j for j in idx[ptr[i]:ptr[i + 1]] if j != i]
+        if not nbrs:
+            agg[i] = ISO
+        elif all(agg[j] is FREE for j in nbrs):
+            for j in [i] + list(idx[ptr[i]:ptr[i + 1]]):
+                agg[j] = n_agg
+                pass1[j] = True
+            n_agg += 1
+    for i in range(n):
+        if agg[i] is not FREE:
+            continue
+        for j in idx[ptr[i]:ptr[i + 1]]:
+            if pass1[j] and agg[j] is not FREE and agg[j] != ISO:
+                agg[i] = agg[j]
+                break
+    for i in range(n):
+        if agg[i] is not FREE:
+            continue
+        agg[i] = n_agg
+        for j in idx[ptr[i]:ptr[i + 1]]:
+            if agg[j] is FREE:
+                agg[j] = n_agg
+        n_agg += 1
+    return [int(a) for a in agg], n_agg

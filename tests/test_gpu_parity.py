@@ -274,6 +274,38 @@ def test_substeps_positions(mode):
     assert worst["f64"] <= 1e-3 * mesh.h
 
 
+def test_pyamg_aggregation_substeps_match_oracle():
+    """aggregation='pyamg' (pyamg's standard aggregation on the matrix pattern, structural
+    Galerkin levels): the hierarchy is the host builder's, and ten substeps stay within
+    1e-4 h of the FP32 oracle run on the same aggregates."""
+    from controllable_pbd_3d_amd import tet_aggregation as ta
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(8, seed=31)
+    kw = dict(mu_per_tet=mu_t, hold_pinned=True, galerkin="structural", relative_tol=True,
+              tol_cg=1e-6, aggregation="pyamg")
+    s = _solver(mesh, None, **kw)
+    want = ta.build_pyamg_hierarchy(s._A_indptr, s._A_indices, None, max_coarse=400)
+    assert [n for _, n in s._aggs] == [n for _, n in want]
+    assert all(np.array_equal(a, b) for (a, _), (b, _) in zip(s._aggs, want))
+    okw = {k: v for k, v in kw.items() if k not in ("galerkin", "aggregation")}
+    o = _oracle(mesh, None, precision="f32", **okw)
+    A = sp.csr_matrix((np.asarray(o.A_data, np.float64), o.indices, o.indptr),
+                      shape=(o.n_tets, o.n_tets))
+    Ps = [sp.csr_matrix((np.ones(len(a)), (np.arange(len(a)), a)), shape=(len(a), n))
+          for a, n in s._aggs]
+    o.mg = om.OracleMG(2, 2.0 / 3.0, 100, 1e-6, "f32", relative=True, structural=True)
+    o.mg.setup(A, Ps)
+    worst = 0.0
+    for k in range(10):
+        s.step(0)
+        o.step(0)
+        worst = max(worst, float(np.abs(s.v_p.cpu().numpy() - o.v_p).max()))
+    print(f"[pyamg] levels {s._gpu_solver._level_sizes} max|dx|/h {worst / mesh.h:.2e} "
+          f"gpu {s.last_stats}", flush=True)
+    assert len(s._gpu_solver._level_sizes) >= 2
+    assert worst <= 1e-4 * mesh.h
+
+
 def _extended_pair(n=6, seed=2, coarse_solver="cholesky"):
     """GPU solver on the extended hierarchy and an FP32 oracle with the same levels and the
     same level-0 values."""

@@ -276,6 +276,87 @@ def test_extended_hierarchy_coarsens_an_irregular_mesh():
     assert all(2 * b <= a for a, b in zip(sizes[1:], sizes[2:]))
 
 
+@pytest.mark.parametrize("n,p,seed", [(80, 0.05, 0), (300, 0.02, 1), (200, 0.004, 2),
+                                      (150, 0.1, 3)])
+def test_standard_aggregation_matches_oracle(n, p, seed):
+    """pyamg's standard aggregation (the aggregation='pyamg' branch): libmgpbd == the oracle's
+    three-pass restatement on random graphs with isolated nodes; every non-isolated node in
+    exactly one connected aggregate, isolated nodes (no off-diagonal neighbour) -1."""
+    from oracle import structure as st
+    ptr, idx = _random_graph(n, p, seed)
+    agg, na = ta.standard_aggregation(ptr, idx)
+    ref, ref_n = st.standard_aggregation(ptr.tolist(), idx.tolist())
+    assert na == ref_n and agg.tolist() == ref
+    deg = np.diff(ptr)
+    assert np.array_equal(agg < 0, deg == 0)
+    if na:
+        for a in range(na):
+            members = set(np.flatnonzero(agg == a).tolist())
+            assert members
+            start = next(iter(members))
+            seen, stack = {start}, [start]
+            while stack:
+                u = stack.pop()
+                for v in idx[ptr[u]:ptr[u + 1]]:
+                    if v in members and v not in seen:
+                        seen.add(int(v))
+                        stack.append(int(v))
+            assert seen == members
+
+
+def test_pyamg_hierarchy_on_a_block():
+    """The aggregation='pyamg' hierarchy on a block's matrix pattern: each level is pyamg's
+    standard aggregation of the previous level's Galerkin pattern, sizes shrink to <= 400;
+    with the values of A at rest (zero off-diagonals) the first coarse pattern is diagonal,
+    every coarse node is isolated and coarsening stops after one level."""
+    m = box_mesh(8, 0.05, seed=0)
+    indptr, indices = product_pattern(m)
+    aggs = ta.build_pyamg_hierarchy(indptr, indices, None, max_coarse=400)
+    sizes = [m.n_tets] + [n for _, n in aggs]
+    print(f"[pyamg hierarchy] {sizes}")
+    assert sizes[-1] <= 400 and all(b < a for a, b in zip(sizes, sizes[1:]))
+    agg0, n1 = ta.standard_aggregation(indptr, indices)
+    assert np.array_equal(aggs[0][0], agg0) and aggs[0][1] == n1
+    vals = np.where(indices == np.repeat(np.arange(m.n_tets), np.diff(indptr)), 1.0, 0.0)
+    at_rest = ta.build_pyamg_hierarchy(indptr, indices, vals.astype(np.float32), max_coarse=400)
+    assert len(at_rest) == 1 and np.array_equal(at_rest[0][0], agg0)
+
+
+def test_column_scaled_prolongation_gives_the_same_vcycle():
+    """pyamg's tentative prolongator (1/sqrt(|aggregate|) per row) and the unit P of the same
+    aggregates give the same two-level V-cycle (weighted Jacobi, Galerkin coarse level,
+    exact coarse solve) up to rounding: a column scaling S gives A_c' = S A_c S and
+    P' A_c'^-1 P'^T = P A_c^-1 P^T."""
+    rng = np.random.default_rng(0)
+    n = 120
+    M = sp.random(n, n, density=0.06, random_state=rng)
+    A = (M @ M.T + sp.eye(n) * 0.5).tocsr()
+    agg, na = ta.standard_aggregation(A.indptr, A.indices)
+    agg = np.where(agg < 0, na + np.arange(n), agg)
+    _, agg = np.unique(agg, return_inverse=True)
+    nc = agg.max() + 1
+    P = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, nc))
+    size = np.bincount(agg)
+    Ps = sp.csr_matrix((1 / np.sqrt(size[agg]), (np.arange(n), agg)), shape=(n, nc))
+    d = A.diagonal()
+
+    def vcycle(P, b):
+        x = np.zeros(n)
+        for _ in range(2):
+            x = x + (2 / 3) * (b - A @ x) / d
+        Ac = (P.T @ A @ P).toarray()
+        x = x + P @ np.linalg.solve(Ac, P.T @ (b - A @ x))
+        for _ in range(2):
+            x = x + (2 / 3) * (b - A @ x) / d
+        return x
+
+    b = rng.standard_normal(n)
+    x1, x2 = vcycle(P, b), vcycle(Ps, b)
+    assert np.abs(x1 - x2).max() <= 1e-10 * np.abs(x1).max()
+    from controllable_pbd_3d_amd.mgpcg_gpu import _aggs_from_Ps
+    assert np.array_equal(_aggs_from_Ps([Ps])[0][0], agg.astype(np.int32))
+
+
 def test_sell_layout_roundtrip():
     """SELL-64x4: width a multiple of 4 covering the longest row of the slice; entry k of row
     r at slice_ptr[r/64] + 256 (k/4) + 4 (r%64) + k%4; padding points at the row itself."""
