@@ -4,13 +4,15 @@ Default workload (the 1M-tet configuration): jittered block of 56^3 Kuhn cubes (
 vertices, 1,053,696 tets), h = 0.02 m, per-tet E log-uniform in [1e4, 1e6] Pa (seed 0),
 nu = 0.45, rho = 1000 kg/m^3, gravity -9.8 m/s^2, top face held, dt = 1/300 s (60 fps x 5
 substeps), Newton defaults of the reference (maxiter 20, atol 1e-4, rtol 1e-2, line search),
-PCG to relative residual 1e-6 (max 100 iterations), 2+2 weighted-Jacobi sweeps (omega 2/3),
-tet-aggregation hierarchy extended to 5 levels, coarsest level solved exactly (FP64 dense
-inverse). A step is one substep (predict, Newton/MGPCG
-solve, velocity update) from the state the previous step left; inputs are resident in HBM.
+PCG to relative residual 1e-6 (max 100 iterations), tet-aggregation hierarchy extended to
+5 levels, coarsest level solved exactly (FP64 dense inverse). Runs: the reference's algorithm
+(2+2 weighted-Jacobi sweeps, omega 2/3, FP32) and the Chebyshev-smoothed solver (3+3 steps on
+the device's spectral estimates, FP64 iterate, FP64-verified solves) in FP32, FP16 and BF16,
+and FP16 with the previous-lambda and the learned warm starts. A step is one substep
+(predict, Newton/MGPCG solve, velocity update) from the state the previous step left; inputs
+are resident in HBM.
 
-Prints one JSON line (see README for the fields). `--gpus N` with N > 1 exits: this build
-measures one GPU.
+Prints one JSON line. `--gpus N` with N > 1 exits: this build measures one GPU.
 """
 
 from __future__ import annotations
@@ -183,19 +185,19 @@ def kernel_stats(ctx, n, nnz, precision):
     return kern
 
 
+CHEBYSHEV = dict(smoother="chebyshev", n_smooth=3, cheb_ratio=20.0, smoother_safety=1.2,
+                 verify_residual=1, fp64_iterate=True)
 RUNS = {
     # the reference's algorithm (fixed 2/3-weighted Jacobi); the FP64 residual of every solve
     # is measured, not enforced
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
-    # Chebyshev smoothing on the device's spectral estimates (same 2 + 2 sweeps), FP64 PCG
-    # iterate, every solve FP64-verified to the tolerance
-    "fp32": dict(precision="fp32", smoother="chebyshev", verify_residual=1, fp64_iterate=True),
-    "fp16": dict(precision="fp16", smoother="chebyshev", verify_residual=1, fp64_iterate=True),
-    "bf16": dict(precision="bf16", smoother="chebyshev", verify_residual=1, fp64_iterate=True),
-    "fp16_previous": dict(precision="fp16", smoother="chebyshev", verify_residual=1,
-                          fp64_iterate=True, warm_start="previous"),
-    "fp16_learned": dict(precision="fp16", smoother="chebyshev", verify_residual=1,
-                         fp64_iterate=True, warm_start="learned"),
+    # Chebyshev smoothing on the device's spectral estimates (3 + 3 steps on
+    # [1.2 lambda / 20, 1.2 lambda]), FP64 PCG iterate, every solve FP64-verified
+    "fp32": dict(precision="fp32", **CHEBYSHEV),
+    "fp16": dict(precision="fp16", **CHEBYSHEV),
+    "bf16": dict(precision="bf16", **CHEBYSHEV),
+    "fp16_previous": dict(precision="fp16", warm_start="previous", **CHEBYSHEV),
+    "fp16_learned": dict(precision="fp16", warm_start="learned", **CHEBYSHEV),
 }
 
 
@@ -203,10 +205,12 @@ def make_solver(mesh, mu_t, cfg, kw, model=None, record=None):
     from controllable_pbd_3d_amd import MGPBDSolverGPU
     args = solver_args(mesh, mu_t)
     args["maxiter_cg"] = cfg.get("maxiter_cg", args["maxiter_cg"])
-    s = MGPBDSolverGPU(**args, hold_pinned=True, relative_tol=True,
-                       tol_cg=1e-6, hierarchy="extended", galerkin="structural",
-                       max_coarse=64, max_levels=cfg["max_levels"], coarse_solver="cholesky",
-                       warm_start_model=model, record_samples=record, verbose=False, **kw)
+    args.update(hold_pinned=True, relative_tol=True, tol_cg=1e-6, hierarchy="extended",
+                galerkin="structural", max_coarse=64, max_levels=cfg["max_levels"],
+                coarse_solver="cholesky", warm_start_model=model, record_samples=record,
+                verbose=False)
+    args.update(kw)
+    s = MGPBDSolverGPU(**args)
     s.init()
     return s
 
