@@ -1225,6 +1225,33 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
   }
 }
 
+// Power iteration on the scaled reduced-precision level (unit diagonal, so D = I in these
+// variables and the spectrum is that of D^-1 A as the V-cycle applies it): w = A^ v and the
+// partials v.A^v (slot 0), v.v (slot 1), w.w (slot 2) for k_power_fin.
+template <int VT, bool PACKED>
+__global__ void __launch_bounds__(kBlock)
+k_power_step_h(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+               const uint32_t* __restrict__ hpk, const uint16_t* __restrict__ hval,
+               const float* __restrict__ v, float* __restrict__ w, RedWS ws) {
+  double vav = 0.0, vv = 0.0, ww = 0.0;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const float ax = sellh_row<VT, PACKED>(i, slice_ptr, idx, hpk, hval, v);
+    if (i < n) {
+      w[i] = ax;
+      vav = (double)v[i] * ax;
+      vv = (double)v[i] * v[i];
+      ww = (double)ax * ax;
+    }
+  }
+  const double s0 = block_sum(vav);
+  const double s1 = block_sum(vv);
+  const double s2 = block_sum(ww);
+  store_partial(ws, 0, s0);
+  store_partial(ws, 1, s1);
+  store_partial(ws, 2, s2);
+}
+
 // sd = sqrt(A_ii), isd = 1 / sd (1 where A_ii <= 0)
 __global__ void __launch_bounds__(kBlock)
 k_half_scales(int n, const int* __restrict__ diag_pos, const float* __restrict__ val,
@@ -2282,8 +2309,21 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     const int g = nblocks(64LL * L.n_slices);
     double* pw = ctx->pw_dev + 4 * l;
     for (int it = 0; it < iters; ++it) {
-      k_power_step<<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.dinv, v, w,
-                                        ctx->ws);
+      if (ctx->precision == MGPBD_PREC_FP32)
+        k_power_step<<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.dinv, v,
+                                          w, ctx->ws);
+      else if (ctx->precision == MGPBD_PREC_FP16 && L.hpk)
+        k_power_step_h<1, true><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.hpk,
+                                                     L.hval, v, w, ctx->ws);
+      else if (ctx->precision == MGPBD_PREC_FP16)
+        k_power_step_h<1, false><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.hpk,
+                                                      L.hval, v, w, ctx->ws);
+      else if (L.hpk)
+        k_power_step_h<2, true><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.hpk,
+                                                     L.hval, v, w, ctx->ws);
+      else
+        k_power_step_h<2, false><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.hpk,
+                                                      L.hval, v, w, ctx->ws);
       k_power_fin<<<1, kBlock, 0, s>>>(ctx->ws, g, pw);
       k_power_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, w, v, pw);
     }
