@@ -94,6 +94,7 @@ SIGNATURES = {
     "mgpbd_mg_convert_half": (c_int, [_V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_mg_set_cycle": (c_int, [_V, c_int, c_float]),
+    "mgpbd_mg_set_level_smoothing": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_smoother": (c_int, [_V, c_int, c_float, c_float, c_int, c_float]),
     "mgpbd_mg_update_smoother": (c_int, [_V, F64P]),
     "mgpbd_mg_get_smoother": (c_int, [_V, F32P, F32P]),

@@ -1447,6 +1447,7 @@ struct mgpbd_ctx {
   PcgState* st = nullptr;
   PcgState* st_host = nullptr;   // pinned mirror
   int n_levels = 0, n_smooth = 2, coarse_mode = 0, coarse_sweeps = 50;
+  int level_smooth[MGPBD_MAX_LEVELS] = {};   // smoothing steps per phase on level l (0: n_smooth)
   // dense coarsest-level inverse (coarse_mode == MGPBD_COARSE_CHOLESKY)
   rocblas_handle blas = nullptr;
   double* dense = nullptr;
@@ -1508,6 +1509,11 @@ struct mgpbd_ctx {
 };
 
 // Device pointer of level l's smoothing weight, or nullptr for the fixed host weight.
+// smoothing steps per phase on level l
+static int ns(const mgpbd_ctx* ctx, int l) {
+  return ctx->level_smooth[l] > 0 ? ctx->level_smooth[l] : ctx->n_smooth;
+}
+
 static const float* wl(const mgpbd_ctx* c, int l) {
   return c->spectral ? c->omega_dev + l : nullptr;
 }
@@ -1871,6 +1877,7 @@ int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega, 
   if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   ctx->n_levels = n_levels;
   ctx->n_smooth = n_smooth;
+  for (int l = 0; l < MGPBD_MAX_LEVELS; ++l) ctx->level_smooth[l] = 0;
   ctx->omega = omega;
   ctx->coarse_mode = coarse_mode;
   ctx->coarse_sweeps = coarse_sweeps;
@@ -2089,7 +2096,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     int total = std::min(ctx->coarse_sweeps, L.n);
     return sweeps(ctx, l, b, x_first, total - 1, done);
   }
-  float* x = sweeps(ctx, l, b, x_first, ctx->n_smooth - 1, done, 2, true);
+  float* x = sweeps(ctx, l, b, x_first, ns(ctx, l) - 1, done, 2, true);
   float* r = L.wb(3);
   MgLevel& C = ctx->L[l + 1];
   // residual r = b - A x (x == 0 when n_smooth == 0: then r = b)
@@ -2129,7 +2136,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     k_add_d<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, save, xcr, done);
   }
   k_prolong<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.agg, xcr, x, ctx->overcorrect, done);
-  return sweeps(ctx, l, b, x, ctx->n_smooth, done, 1, false);
+  return sweeps(ctx, l, b, x, ns(ctx, l), done, 1, false);
 }
 
 // ---------------- reduced-precision (scaled) V-cycle ----------------
@@ -2213,7 +2220,7 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
     const int total = std::min(ctx->coarse_sweeps, L.n);
     return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr);
   }
-  float* x = sweeps_h(ctx, l, b, x_first, ctx->n_smooth - 1, done, nullptr, 2, true);
+  float* x = sweeps_h(ctx, l, b, x_first, ns(ctx, l) - 1, done, nullptr, 2, true);
   float* r = L.wb(3);
   int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
   sellh(ctx, L, 2, x, r, b, done, nullptr);
@@ -2238,7 +2245,20 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
   }
   k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x, ctx->overcorrect,
                                               done);
-  return sweeps_h(ctx, l, b, x, ctx->n_smooth, done, z_out, 1, false);
+  return sweeps_h(ctx, l, b, x, ns(ctx, l), done, z_out, 1, false);
+}
+
+int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps) {
+  CTX_CHECK("mgpbd_mg_set_level_smoothing");
+  // steps 0 restores the configured n_smooth; per-level counts need n_smooth >= 1 (the
+  // first step of every phase structure is laid out for a smoothed cycle)
+  if (level < 0 || level >= ctx->n_levels || steps < 0 || steps > 8 ||
+      (steps > 0 && ctx->n_smooth < 1))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_smoothing: level in range, steps in "
+                                    "[0, 8], n_smooth >= 1");
+  ctx->level_smooth[level] = steps;
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  return 0;
 }
 
 int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect) {

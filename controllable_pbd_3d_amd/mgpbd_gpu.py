@@ -26,7 +26,8 @@ lambda_max(D^-1 A)), ``verify_residual``
 steps per refresh of the spectral estimates once settled), ``smoother_safety`` and
 ``cheb_ratio`` (upper end safety factor over the estimate and interval ratio of the spectral
 smoothers), ``stall_rescue`` (re-estimate the spectra and restart a PCG that stops
-contracting; spectral smoothers only), ``f_ext``
+contracting; spectral smoothers only), ``smooth_schedule`` (smoothing steps per level, the
+last entry repeating), ``f_ext``
 (external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
@@ -78,7 +79,7 @@ class MGPBDSolverGPU:
                  smoother: str = "jacobi", verify_residual: int = 0,
                  fp64_iterate: bool = False, power_iters: int = 2,
                  smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
-                 stall_rescue: bool = True, f_ext=None, warm_start: str = "none",
+                 stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None,
                  device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
@@ -127,6 +128,7 @@ class MGPBDSolverGPU:
         self.smoother_safety = float(smoother_safety)
         self.cheb_ratio = float(cheb_ratio)
         self.stall_rescue = bool(stall_rescue)
+        self.smooth_schedule = smooth_schedule
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -182,7 +184,7 @@ class MGPBDSolverGPU:
             overcorrect=self.overcorrect, smoother=self.smoother, verify=self.verify_residual,
             fp64_iterate=self.fp64_iterate, power_iters=self.power_iters,
             smoother_safety=self.smoother_safety, cheb_ratio=self.cheb_ratio,
-            stall_rescue=self.stall_rescue,
+            stall_rescue=self.stall_rescue, smooth_schedule=self.smooth_schedule,
             device=self.device.index)
         self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()

@@ -82,7 +82,7 @@ class GPUMultigridSolver:
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
                  smoother_safety: float = 1.1, power_iters: int = 2, cheb_ratio: float = 12.0,
                  verify: int = 0, fp64_iterate: bool = False, stall_rescue: bool = True,
-                 device=None):
+                 smooth_schedule=None, device=None):
         self.n_smooth = int(n_smooth)
         self.omega_jacobi = float(omega_jacobi)
         self.maxiter_cg = int(maxiter_cg)
@@ -113,6 +113,8 @@ class GPUMultigridSolver:
         self.verify = int(verify)   # 0 off, 1 verify + restart (True), 2 measure only
         self.fp64_iterate = bool(fp64_iterate)
         self.stall_rescue = bool(stall_rescue)
+        # smoothing steps per level (the last entry repeats); None: n_smooth everywhere
+        self.smooth_schedule = None if smooth_schedule is None else [int(k) for k in smooth_schedule]
         self.stall_ratio = 0.25   # squared-residual ratio over 8 iterations counted as a stall
         self.last_lambda = None
         self.n_levels = 0
@@ -251,6 +253,10 @@ class GPUMultigridSolver:
         ctx("mgpbd_pcg_set_fp64_iterate", int(self.fp64_iterate))
         ctx("mgpbd_pcg_set_rescue", int(self.stall_rescue and self.smoother != "jacobi"), 1.25,
             self.stall_ratio)
+        if self.smooth_schedule:
+            for l in range(self.n_levels):
+                k = self.smooth_schedule[min(l, len(self.smooth_schedule) - 1)]
+                ctx("mgpbd_mg_set_level_smoothing", l, k)
         self._coarse_mode = mode
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None

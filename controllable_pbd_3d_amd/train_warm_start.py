@@ -45,11 +45,12 @@ def make_solver(mesh, mu_t, max_levels, warm_start="none", model=None, record=No
     s = solver_cls(v_p=mesh.verts.copy(), v_p_ref=mesh.verts.copy(), v_invm=mesh.invm.copy(),
                    t_i=mesh.tets.copy(), t_m=mesh.tet_mass.copy(), gravity=(0.0, -9.8, 0.0),
                    dt=1.0 / 300.0, maxiter=20, atol=1e-4, rtol=1e-2, maxiter_cg=100,
-                   n_smooth=3, omega_jacobi=2.0 / 3.0, mu_per_tet=mu_t, hold_pinned=True,
+                   n_smooth=5, omega_jacobi=2.0 / 3.0, mu_per_tet=mu_t, hold_pinned=True,
                    relative_tol=True, tol_cg=1e-6, hierarchy="extended", galerkin="structural",
                    max_coarse=64, max_levels=max_levels, coarse_solver="cholesky",
-                   precision=precision, smoother="chebyshev", cheb_ratio=20.0,
-                   smoother_safety=1.2, verify_residual=1, fp64_iterate=True,
+                   precision=precision, smoother="chebyshev", cheb_ratio=30.0,
+                   smoother_safety=1.2, smooth_schedule=[5, 1], verify_residual=1,
+                   fp64_iterate=True,
                    warm_start=warm_start, warm_start_model=model, record_samples=record,
                    verbose=False)
     s.init()

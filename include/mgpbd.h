@@ -281,6 +281,10 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
 int mgpbd_mg_set_coarse_update(mgpbd_ctx* ctx, int newton_schulz);
 /* Counts since the context was created: Newton-Schulz steps kept, Cholesky refactorizations. */
 int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors);
+/* Smoothing steps per phase on one level (extension; the reference uses n_smooth on every
+ * level, cons/mgpcg_gpu.py:232-246): steps in [1, 8], 0 restores n_smooth. Coarse levels are
+ * cheap in bytes but each step is a launch, so fewer steps there can pay. */
+int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps);
 /* Cycle shape (extension; no reference counterpart: cons/mgpcg_gpu.py:214-259 is a V-cycle
  * adding the plain coarse correction). cycle 1 = V, 2 = W (each level below the finest that is
  * not the coarsest is cycled twice, the second time on its residual); overcorrect scales the
