@@ -2615,8 +2615,8 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
 
     int round_issued = 0;
     const bool watch = ctx->rescue && ctx->spectral && !rescued;
-    const int back = std::max(1, 8 / ctx->poll);   // polls spanning 8 iterations
-    double rr_poll[64];
+    double rr_poll[64];    // recurrence residual and iteration count at each host poll
+    int it_poll[64];
     int n_poll = 0;
     bool stalled = false;
     while (issued < maxiter) {
@@ -2625,7 +2625,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
         issued += ctx->poll;
         round_issued += ctx->poll;
       } else {
-        int chunk = std::min(ctx->poll, maxiter - issued);
+        // profiling times every launch: one iteration per host check, so no launch past
+        // convergence (skipped by the done flag) dilutes the per-kernel averages
+        int chunk = std::min(ctx->prof ? 1 : ctx->poll, maxiter - issued);
         for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, true);
         MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
         issued += chunk;
@@ -2637,9 +2639,14 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       if (ctx->st_host->done) break;
       if (watch) {
         rr_poll[n_poll % 64] = ctx->st_host->rr;
+        it_poll[n_poll % 64] = round_issued;
         ++n_poll;
-        if (round_issued >= 16 && n_poll > back &&
-            ctx->st_host->rr > ctx->stall_ratio * rr_poll[(n_poll - 1 - back) % 64]) {
+        // the latest poll at least 8 iterations back (within the kept history)
+        int q = -1;
+        for (int k = n_poll - 2; k >= 0 && k >= n_poll - 64; --k)
+          if (it_poll[k % 64] <= round_issued - 8) { q = k; break; }
+        if (round_issued >= 16 && q >= 0 &&
+            ctx->st_host->rr > ctx->stall_ratio * rr_poll[q % 64]) {
           stalled = true;
           break;
         }
