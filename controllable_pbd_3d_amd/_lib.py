@@ -114,6 +114,7 @@ SIGNATURES = {
     "mgpbd_gnn_workspace_bytes": (c_int64, [c_int]),
     "mgpbd_gnn_features": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_forward": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_infer": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_loss_mse": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_loss_residual": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_loss_energy": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),

@@ -104,6 +104,74 @@ k_gemm64(int n, const uint16_t* __restrict__ A1, const uint16_t* __restrict__ W1
   }
 }
 
+// Inference layer with the neighbour mean fused in: Y = relu(H W_self + mean_nbr(H) W_nbr + b)
+// for the same 32-row MFMA tiles as k_gemm64. Each lane forms its own fragment of the mean
+// (the four neighbour rows summed in FP32 in neighbour order, times 1/deg, rounded to BF16:
+// the arithmetic of k_mean_nbr), so the output equals k_mean_nbr + k_gemm64 bit for bit
+// without the mean's n x 64 round trip through memory.
+__global__ void __launch_bounds__(kBlock)
+k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ Hin,
+             const uint16_t* __restrict__ Wst, const uint16_t* __restrict__ Wnt,
+             const float* __restrict__ bias, uint16_t* __restrict__ Hout) {
+  const int lane = threadIdx.x & 63;
+  const int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
+  if (base >= n) return;
+  const int ra = base + (lane & 31);
+  const int kq = 4 * (lane >> 5);
+  v16f32 acc0 = {}, acc1 = {};
+  const v4i16 zero = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < H / 8; ++s) {
+    const v4i16 a = ra < n ? load4(Hin + (size_t)ra * H + 8 * s + kq) : zero;
+    const v4i16 b0 = load4(Wst + (size_t)(lane & 31) * H + 8 * s + kq);
+    const v4i16 b1 = load4(Wst + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+  }
+  int nb4[4] = {-1, -1, -1, -1};
+  int d = 0;
+  if (ra < n) {
+    const int4 q = reinterpret_cast<const int4*>(nbr)[ra];
+    nb4[0] = q.x; nb4[1] = q.y; nb4[2] = q.z; nb4[3] = q.w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d += nb4[k] >= 0;
+  }
+  const float inv = d > 0 ? 1.0f / (float)d : 0.0f;
+#pragma unroll
+  for (int s = 0; s < H / 8; ++s) {
+    float m[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (nb4[k] < 0) continue;
+      const v4i16 h = load4(Hin + (size_t)nb4[k] * H + 8 * s + kq);
+      m[0] += bf2f((uint16_t)h.x);
+      m[1] += bf2f((uint16_t)h.y);
+      m[2] += bf2f((uint16_t)h.z);
+      m[3] += bf2f((uint16_t)h.w);
+    }
+    v4i16 a;
+    a.x = (short)f2bf(m[0] * inv);
+    a.y = (short)f2bf(m[1] * inv);
+    a.z = (short)f2bf(m[2] * inv);
+    a.w = (short)f2bf(m[3] * inv);
+    const v4i16 b0 = load4(Wnt + (size_t)(lane & 31) * H + 8 * s + kq);
+    const v4i16 b1 = load4(Wnt + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = base + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+    if (row >= n) continue;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = 32 * c + (lane & 31);
+      const float v = (c == 0 ? acc0[r] : acc1[r]) + bias[col];
+      Hout[(size_t)row * H + col] = f2bf(fmaxf(v, 0.0f));
+    }
+  }
+}
+
 // Weight gradient partials: P[block][k][j] = sum over the block's rows i of A[i][k] dZ[i][j]
 // (A, dZ BF16 n x 64). Each of the 4 wavefronts owns one 32x32 quadrant of the 64x64 result;
 // the MFMA inner dimension runs over rows.
@@ -217,6 +285,61 @@ k_encoder(int n, const float* __restrict__ x, const float* __restrict__ P,
 #pragma unroll
   for (int k = 0; k < F; ++k) v = fmaf(x[(size_t)i * F + k], P[off_win() + k * H + c], v);
   h0[(size_t)i * H + c] = f2bf(fmaxf(v, 0.0f));
+}
+
+// h0 for 8 channels per thread (same FP32 fma order as k_encoder), one 16-byte store
+__global__ void __launch_bounds__(kBlock)
+k_encoder8(int n, const float* __restrict__ x, const float* __restrict__ P,
+           uint16_t* __restrict__ h0) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  const int i = t >> 3, c0 = 8 * (t & 7);
+  if (i >= n) return;
+  const float4 xa = reinterpret_cast<const float4*>(x)[2 * (size_t)i];
+  const float4 xb = reinterpret_cast<const float4*>(x)[2 * (size_t)i + 1];
+  const float xi[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+  uint16_t out[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    float v = P[off_bin() + c0 + e];
+#pragma unroll
+    for (int k = 0; k < F; ++k) v = fmaf(xi[k], P[off_win() + k * H + c0 + e], v);
+    out[e] = f2bf(fmaxf(v, 0.0f));
+  }
+  uint4 pk;
+  pk.x = (uint32_t)out[0] | ((uint32_t)out[1] << 16);
+  pk.y = (uint32_t)out[2] | ((uint32_t)out[3] << 16);
+  pk.z = (uint32_t)out[4] | ((uint32_t)out[5] << 16);
+  pk.w = (uint32_t)out[6] | ((uint32_t)out[7] << 16);
+  reinterpret_cast<uint4*>(h0 + (size_t)i * H)[t & 7] = pk;
+}
+
+// dlambda = lam_prev + s (h3 w_out + b_out), one row per thread; the 64 products are summed in
+// the pairing order of k_decoder's butterfly (lane 0's view), so y matches it bit for bit
+__global__ void __launch_bounds__(kBlock)
+k_decoder_predict(int n, const uint16_t* __restrict__ h3, const float* __restrict__ P,
+                  const float* __restrict__ lp, const float* __restrict__ st,
+                  float* __restrict__ y, float* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float p[H];
+  const uint4* row = reinterpret_cast<const uint4*>(h3 + (size_t)i * H);
+#pragma unroll
+  for (int q = 0; q < H / 8; ++q) {
+    const uint4 v = row[q];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      p[8 * q + 2 * e] = bf2f((uint16_t)(w[e] & 0xFFFFu)) * P[off_wout() + 8 * q + 2 * e];
+      p[8 * q + 2 * e + 1] = bf2f((uint16_t)(w[e] >> 16)) * P[off_wout() + 8 * q + 2 * e + 1];
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < off; ++c) p[c] = p[c] + p[c + off];
+  const float yi = p[0] + P[off_bout()];
+  if (y) y[i] = yi;
+  if (out) out[i] = (lp ? lp[i] : 0.0f) + st[0] * yi;
 }
 
 // y = h3 w_out + b_out
@@ -593,6 +716,27 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
   k_decoder<<<nb((int64_t)n * 64), kBlock, 0, s>>>(n, W.h[NL], params, W.y);
   if (dlam_out) k_predict<<<nb(n), kBlock, 0, s>>>(n, W.y, lam_prev, stats, dlam_out);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_forward");
+  return 0;
+}
+
+int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
+                    const float* stats, const float* lam_prev, const float* params,
+                    const uint16_t* wt, void* ws, float* dlam_out) {
+  if (!ctx || n <= 0 || !nbr || !x || !stats || !params || !wt || !ws || !dlam_out)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_infer: invalid arguments");
+  hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
+  GnnWorkspace W;
+  carve(ws, n, W);
+  k_encoder8<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, x, params, W.h[0]);
+  const int gg = (n + 127) / 128;   // 4 wavefronts x 32 rows per block
+  for (int l = 0; l < NL; ++l) {
+    const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
+    const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
+    k_layer_mean<<<gg, kBlock, 0, s>>>(n, nbr, W.h[l], w_self, w_nbr, params + off_bl(l),
+                                       W.h[l + 1]);
+  }
+  k_decoder_predict<<<nb(n), kBlock, 0, s>>>(n, W.h[NL], params, lam_prev, stats, W.y, dlam_out);
+  MGPBD_LAUNCH_CHECK("mgpbd_gnn_infer");
   return 0;
 }
 

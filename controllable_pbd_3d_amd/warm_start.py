@@ -111,10 +111,17 @@ class LearnedWarmStart:
                  dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
         return out
 
+    def infer(self, x, stats, lam_prev, out):
+        """The forward's prediction through the fused inference kernels (bit-identical;
+        keeps no activations for a backward pass)."""
+        self.ctx("mgpbd_gnn_infer", self.n, dptr(self.nbr), dptr(x), dptr(stats),
+                 dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
+        return out
+
     def predict(self, b, dinv, lam_prev, C, alpha, vol, out):
         """out = predicted dlambda of the first Newton iteration (device tensors, n floats)."""
         x, st = self.features(b, dinv, lam_prev, C, alpha, vol)
-        return self.forward(x, st, lam_prev, out)
+        return self.infer(x, st, lam_prev, out)
 
     # ------------------------------------------------------------------ training
     def make_sample(self, b, dinv, lam_prev, C, alpha, vol, target, matrix=None) -> dict:

@@ -446,6 +446,13 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float
                       const float* stats_dev, const float* lam_prev_dev,
                       const float* params_dev, const uint16_t* wt_dev, void* ws_dev,
                       float* dlam_out_dev);
+/* Inference-only forward (extension): the same network and the same result bit for bit as
+ * mgpbd_gnn_forward's dlam_out, with the neighbour mean fused into each MFMA layer (no n x 64
+ * mean written or read back), the encoder at 8 channels per thread and the decoder plus
+ * prediction at one row per thread. Leaves no activations for a backward pass. */
+int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* x_dev,
+                    const float* stats_dev, const float* lam_prev_dev, const float* params_dev,
+                    const uint16_t* wt_dev, void* ws_dev, float* dlam_out_dev);
 /* Losses of the forward pass left in ws_dev; each writes dy_dev (n) = d loss / d y and, when
  * loss_dev (device double) is given, the loss.
  *   mse:      mean_i (y_i - (target_i - lam_prev_i) / s)^2

@@ -141,6 +141,22 @@ def test_features_match_oracle(solved_scene):
     assert np.any(_host(smp["lam_prev"]) != 0)
 
 
+def test_fused_inference_equals_forward(solved_scene):
+    """mgpbd_gnn_infer (mean fused into the MFMA layers, per-row decoder) gives the forward's
+    prediction bit for bit, and so agrees with the BF16-emulating oracle."""
+    mesh, s, model, rec = solved_scene
+    smp = rec[-1]
+    model.load_state_dict({"params": _trained_like_params(model, seed=5)})
+    a = torch.empty_like(smp["lam_prev"])
+    b = torch.empty_like(smp["lam_prev"])
+    model.forward(smp["x"], smp["stats"], smp["lam_prev"], a)
+    model.infer(smp["x"], smp["stats"], smp["lam_prev"], b)
+    assert torch.equal(a, b)
+    _, dlam, _ = go.forward(_host(model.params), _host(smp["x"]), _nbr(model), _host(smp["stats"]),
+                            _host(smp["lam_prev"]), emulate=True)
+    assert _rel(_host(b), dlam) <= 2e-3
+
+
 @pytest.mark.parametrize("loss", ["mse", "residual", "energy"])
 def test_forward_and_backward_match_oracle(solved_scene, loss):
     mesh, s, model, rec = solved_scene
