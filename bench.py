@@ -89,13 +89,17 @@ def solver_args(mesh, mu_t):
                 n_smooth=2, omega_jacobi=2.0 / 3.0, use_line_search=True, mu_per_tet=mu_t)
 
 
-def kernel_bytes(n_rows, nnz, kind, per_nnz=8):
+def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
     """Algorithmic HBM bytes per launch; per_nnz = 8 for FP32 values + int32 columns, 4 for
-    FP16/BF16 values + 16-bit column deltas (each array counted once)."""
+    FP16/BF16 values + 16-bit column deltas (each array counted once). With the Chebyshev
+    smoother the level-0 sweeps also read the previous iterate (all steps but the Jacobi start;
+    the last post-step reads the unscaling factor too): 4 n bytes more on average over a
+    V-cycle's 4 + 1 + 1 step kinds."""
     if kind == "spmv":      # values + columns + row pointers + x + y
         return per_nnz * nnz + 4 * (n_rows + 1) + 4 * n_rows + 4 * n_rows
     if kind == "jacobi":    # + b, D^-1 (FP32 only: the scaled levels have unit diagonal), x -> x_new
-        return per_nnz * nnz + 4 * (n_rows + 1) + (16 if per_nnz == 8 else 12) * n_rows
+        return per_nnz * nnz + 4 * (n_rows + 1) + (16 if per_nnz == 8 else 12) * n_rows + \
+            (4 * n_rows if chebyshev else 0)
     if kind == "resid":     # values + columns + ptr + x + b -> r
         return per_nnz * nnz + 4 * (n_rows + 1) + 12 * n_rows
     raise ValueError(kind)
@@ -164,7 +168,7 @@ def measured_traffic(kind, precision):
     return None
 
 
-def kernel_stats(ctx, n, nnz, precision):
+def kernel_stats(ctx, n, nnz, precision, chebyshev=False):
     from controllable_pbd_3d_amd import _lib
     ms = np.zeros(8, np.float64)
     cnt = np.zeros(8, np.int64)
@@ -176,7 +180,7 @@ def kernel_stats(ctx, n, nnz, precision):
     for kind, slot, per in (("spmv", 0, 8), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
-            by = kernel_bytes(n, nnz, kind, per)
+            by = kernel_bytes(n, nnz, kind, per, chebyshev)
             kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
                           "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
     for kind, slot in (("fill_A", 3), ("constraints", 4), ("refill", 5), ("coarse", 6)):
@@ -305,8 +309,9 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     s._gpu_solver.ctx("mgpbd_ctx_profile", 1)
     for k in range(profile_steps):
         load.step(s)
-    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"])
-    kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"]).items()
+    cheb = kw.get("smoother") == "chebyshev"
+    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"], cheb)
+    kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"], cheb).items()
                  if k not in kern})
     ctx("mgpbd_ctx_profile", 0)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 0)
