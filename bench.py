@@ -155,16 +155,17 @@ def cpu_baseline(mesh, mu_t, solver, newton_per_substep, pcg_per_substep, pcg_sa
                        f"{setup_s:.1f} s excluded; host has {os.cpu_count()} cores")}
 
 
-def measured_traffic(kind, precision):
-    """HBM bytes per launch of a level-0 kernel from the committed rocprofv3 counter summaries
-    (``bench-traffic: <kind> <precision> <bytes>`` lines in profiles/*.md), or None."""
+def measured_traffic(workload, kind, precision):
+    """HBM bytes per launch of a level-0 kernel of this workload from the committed rocprofv3
+    counter summaries (``bench-traffic: <workload> <kind> <precision> <bytes>`` lines in
+    profiles/*.md), or None when that workload was not profiled."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             parts = line.split()
-            if len(parts) == 4 and parts[0] == "bench-traffic:" and parts[1] == kind and \
-                    parts[2] == precision:
-                return float(parts[3])
+            if len(parts) == 5 and parts[0] == "bench-traffic:" and parts[1] == workload and \
+                    parts[2] == kind and parts[3] == precision:
+                return float(parts[4])
     return None
 
 
@@ -407,7 +408,7 @@ def main():
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": k["GBs"] / HBM_PEAK_GBS,
                            "algorithmic_bytes": k["bytes"],
-                           "traffic": measured_traffic(dom, prec)}
+                           "traffic": measured_traffic(cfg["label"], dom, prec)}
     if args.cpu_baseline:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         ref = runs.get("fp32_reference", best)
