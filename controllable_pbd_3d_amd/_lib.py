@@ -66,6 +66,8 @@ SIGNATURES = {
     "mgpbd_update_vel": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float]),
     "mgpbd_constraints": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_fill_A": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_fill_A_desc": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
+    "mgpbd_fill_A_shared": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_scatter": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_dual_norms": (c_int, [_V, c_int, _V, _V, _V, c_float, _V, _V, _V, _V, c_int, _V]),
     "mgpbd_apply_dpos": (c_int, [_V, c_int, _V, _V, _V, c_float]),

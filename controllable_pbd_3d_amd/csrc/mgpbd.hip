@@ -98,6 +98,7 @@ k_fin_sum(RedWS ws, int nparts, int first, int nslots, double* __restrict__ out)
 // SELL-64x4: entry k of row r at slice_ptr[r/64] + 256*(k/4) + 4*(r%64) + (k%4).
 // ------------------------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ int sell_base(const int* __restrict__ slice_ptr, int r) {
@@ -368,6 +369,126 @@ k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
           diag = v[q];
         } else {
           v[q] = a_offdiag(vi, w, gi, tj[q], q0[q], q1[q], q2[q]);
+        }
+      }
+    }
+    *reinterpret_cast<float4*>(val + p) = make_float4(v[0], v[1], v[2], v[3]);
+  }
+  dinv[i] = fabsf(diag) > 1e-30f ? 1.0f / diag : 0.0f;
+}
+
+// Shared-vertex descriptors of the level-0 pattern, one 16-bit word per SELL slot: bits 12-13
+// the number of vertices row i shares with column j (1-3), and per shared vertex, in ascending
+// local slot a of tet i, 4 bits (a | b << 2) with b the vertex's local slot in tet j;
+// kDescDiag marks the diagonal, 0 padding. The pattern and t_i fix them: built once.
+constexpr uint16_t kDescDiag = 0x8000;
+
+__global__ void __launch_bounds__(kBlock)
+k_fill_desc(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+            const int* __restrict__ row_len, const int4* __restrict__ t_i,
+            uint16_t* __restrict__ desc) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) >= (n + 63) >> 6 || i >= n) return;
+  const int base = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  const int len = row_len[i];
+  const int4 ti = t_i[i];
+  const int vi[4] = {ti.x, ti.y, ti.z, ti.w};
+  for (int k = 0; k < w; ++k) {
+    const int p = base + 64 * (k & ~3) + (k & 3);
+    uint16_t d = 0;
+    if (k < len) {
+      const int j = idx[p];
+      if (j == i) {
+        d = kDescDiag;
+      } else {
+        const int4 tj = t_i[j];
+        const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
+        int cnt = 0, code = 0;
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b)
+            if (vi[a] == vj[b]) { code |= (a | (b << 2)) << (4 * cnt); ++cnt; }
+        d = (uint16_t)((cnt << 12) | code);
+      }
+    }
+    desc[p] = d;
+  }
+}
+
+// The A fill from the descriptors: per off-diagonal slot only tet j's gradients at the shared
+// vertices are gathered (12 bytes each instead of j's 16-byte vertex ids and 48-byte
+// gradient); the products are summed in the same order as k_fill_A (ascending local slot of
+// tet i), so the values are identical.
+__global__ void __launch_bounds__(kBlock)
+k_fill_A_desc(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
+              const int* __restrict__ row_len, const uint16_t* __restrict__ desc,
+              const int4* __restrict__ t_i, const float* __restrict__ invm,
+              const float* __restrict__ gradC, const float* __restrict__ alpha,
+              float* __restrict__ val, float* __restrict__ dinv) {
+  int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int base = sell_base(slice_ptr, i);
+  const int len = row_len[i];
+  const int4 ti = t_i[i];
+  const int vi[4] = {ti.x, ti.y, ti.z, ti.w};
+  float w[4], gi[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    w[a] = invm[vi[a]];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) gi[a][r] = gradC[12 * (size_t)i + 3 * a + r];
+  }
+  const float al = alpha[i];
+  float diag = 0.0f;
+  for (int k0 = 0; k0 < len; k0 += 4) {
+    const int p = base + 64 * k0;
+    const v4i c = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(idx + p));
+    const v4u16 dd = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(desc + p));
+    const int cj[4] = {c.x, c.y, c.z, c.w};
+    const uint32_t dq[4] = {dd.x, dd.y, dd.z, dd.w};
+    // gathers of every shared vertex's gradient first, then the arithmetic
+    float g[4][3][3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t d = dq[q];
+      const int cnt = (d & kDescDiag) ? 0 : (int)((d >> 12) & 3u);
+      const float* gj = gradC + 12 * (size_t)cj[q];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const int b = (int)((d >> (4 * e + 2)) & 3u);
+        if (e < cnt) {
+          g[q][e][0] = gj[3 * b];
+          g[q][e][1] = gj[3 * b + 1];
+          g[q][e][2] = gj[3 * b + 2];
+        } else {
+          g[q][e][0] = g[q][e][1] = g[q][e][2] = 0.0f;
+        }
+      }
+    }
+    float v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t d = dq[q];
+      v[q] = 0.0f;
+      if (d == kDescDiag) {
+        float dv = w[0] * dot3(gi[0], gi[0]);
+        dv = fmaf(w[1], dot3(gi[1], gi[1]), dv);
+        dv = fmaf(w[2], dot3(gi[2], gi[2]), dv);
+        dv = fmaf(w[3], dot3(gi[3], gi[3]), dv);
+        v[q] = dv + al;
+        diag = v[q];
+      } else {
+        const int cnt = (int)((d >> 12) & 3u);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          if (e < cnt) {
+            const int a = (int)((d >> (4 * e)) & 3u);
+            const float ga[3] = {a == 0 ? gi[0][0] : a == 1 ? gi[1][0] : a == 2 ? gi[2][0] : gi[3][0],
+                                 a == 0 ? gi[0][1] : a == 1 ? gi[1][1] : a == 2 ? gi[2][1] : gi[3][1],
+                                 a == 0 ? gi[0][2] : a == 1 ? gi[1][2] : a == 2 ? gi[2][2] : gi[3][2]};
+            const float wa = a == 0 ? w[0] : a == 1 ? w[1] : a == 2 ? w[2] : w[3];
+            v[q] = fmaf(wa, dot3(ga, g[q][e]), v[q]);
+          }
         }
       }
     }
@@ -1146,7 +1267,6 @@ k_pcg_p(int n, const float* __restrict__ z, float* __restrict__ p, const PcgStat
 //   restrict  b^_c[I] = isd_c[I] sum_{i in I} sd_i r^_i
 //   prolong   x^_i += sd_i isd_c[agg_i] x^_c[agg_i]
 // ------------------------------------------------------------------------------------------
-typedef unsigned short v4u16 __attribute__((ext_vector_type(4)));
 typedef unsigned int v4u32 __attribute__((ext_vector_type(4)));
 
 template <int VT>   // 1: FP16, 2: BF16
@@ -1736,6 +1856,34 @@ int mgpbd_fill_A(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr, const int
       val, dinv);
   prof_end(ctx, pe);
   MGPBD_LAUNCH_CHECK("mgpbd_fill_A");
+  return 0;
+}
+
+int mgpbd_fill_A_desc(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr, const int32_t* idx,
+                      const int32_t* row_len, const int32_t* t_i, uint16_t* desc) {
+  CTX_CHECK("mgpbd_fill_A_desc");
+  if (n_tets < 0 || (n_tets > 0 && (!slice_ptr || !idx || !row_len || !t_i || !desc)))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_fill_A_desc: invalid arguments");
+  if (n_tets == 0) return 0;
+  k_fill_desc<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, slice_ptr, idx, row_len, reinterpret_cast<const int4*>(t_i), desc);
+  MGPBD_LAUNCH_CHECK("mgpbd_fill_A_desc");
+  return 0;
+}
+
+int mgpbd_fill_A_shared(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr,
+                        const int32_t* idx, const int32_t* row_len, const uint16_t* desc,
+                        const int32_t* t_i, const float* invm, const float* gradC,
+                        const float* alpha, float* val, float* dinv) {
+  CTX_CHECK("mgpbd_fill_A_shared");
+  if (n_tets <= 0) return n_tets < 0 ? mgpbd_fail(MGPBD_EINVAL, "mgpbd_fill_A_shared: n < 0") : 0;
+  if (!desc) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_fill_A_shared: no descriptors");
+  int pe = prof_begin(ctx, MGPBD_PROF_FILL_A);
+  k_fill_A_desc<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, slice_ptr, idx, row_len, desc, reinterpret_cast<const int4*>(t_i), invm, gradC,
+      alpha, val, dinv);
+  prof_end(ctx, pe);
+  MGPBD_LAUNCH_CHECK("mgpbd_fill_A_shared");
   return 0;
 }
 

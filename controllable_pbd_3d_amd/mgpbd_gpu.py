@@ -214,6 +214,11 @@ class MGPBDSolverGPU:
         self._nnz = nnz
         self._A = GPUSparseMatrix(self.n_tets, nnz, device=self.device.index)
         self._A.set_pattern(indptr, indices[:nnz], diag_csr_pos=indptr[1:] - 1)
+        # which vertices each (row, column) pair shares, per SELL slot: built once
+        A = self._A
+        self._A_desc = torch.zeros(max(A.padded, 1), dtype=torch.int16, device=self.device)
+        self.ctx("mgpbd_fill_A_desc", self.n_tets, dptr(A.slice_ptr), dptr(A.sell_idx),
+                 dptr(A.row_len), dptr(self.t_i), dptr(self._A_desc))
 
     def _build_hierarchy(self):
         if self.aggregation == "pyamg":
@@ -310,9 +315,9 @@ class MGPBDSolverGPU:
 
     def _fill_A(self):
         A = self._A
-        self.ctx("mgpbd_fill_A", self.n_tets, dptr(A.slice_ptr), dptr(A.sell_idx),
-                 dptr(A.row_len), dptr(self.t_i), dptr(self.v_invm), dptr(self.gradC),
-                 dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
+        self.ctx("mgpbd_fill_A_shared", self.n_tets, dptr(A.slice_ptr), dptr(A.sell_idx),
+                 dptr(A.row_len), dptr(self._A_desc), dptr(self.t_i), dptr(self.v_invm),
+                 dptr(self.gradC), dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
 
     def _fill_A_and_update_gpu(self):
         self._fill_A()

@@ -186,6 +186,21 @@ int mgpbd_fill_A(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr_dev,
                  const int32_t* sell_idx_dev, const int32_t* row_len_dev,
                  const int32_t* t_i_dev, const float* v_invm_dev, const float* gradC_dev,
                  const float* alpha_dev, float* sell_val_dev, float* diag_inv_dev);
+/* The same A fill (same values bit for bit) from per-slot shared-vertex descriptors
+ * (extension): mgpbd_fill_A_desc writes, once per pattern, one 16-bit word per SELL slot
+ * (desc_dev: padded slots, zero-initialised): the diagonal flag 0x8000, or the count of
+ * vertices rows i and j share (bits 12-13) and, per shared vertex in ascending local slot a of
+ * tet i, (a | b << 2) with b its slot in tet j; mgpbd_fill_A_shared then gathers only tet j's
+ * gradients at the shared vertices (12 bytes each instead of its vertex ids and whole
+ * gradient, 64 bytes). */
+int mgpbd_fill_A_desc(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr_dev,
+                      const int32_t* sell_idx_dev, const int32_t* row_len_dev,
+                      const int32_t* t_i_dev, uint16_t* desc_dev);
+int mgpbd_fill_A_shared(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr_dev,
+                        const int32_t* sell_idx_dev, const int32_t* row_len_dev,
+                        const uint16_t* desc_dev, const int32_t* t_i_dev, const float* v_invm_dev,
+                        const float* gradC_dev, const float* alpha_tilde_dev, float* val_dev,
+                        float* diag_inv_dev);
 
 /* lambda += dlambda; dpos_v = sum over (t, slot) at v of invm_v dlambda_t gradC_t,slot in
  * ascending (t, slot) order. Replaces _zero_dpos + _scatter_dlam_gpu (:330-354). */

@@ -82,6 +82,41 @@ def test_constraints_vs_fp64_svd(scene):
     np.testing.assert_allclose(g_g, o.gradC, rtol=0, atol=2e-5 * max(1.0, np.abs(o.gradC).max()))
 
 
+def test_shared_vertex_fill_equals_full_fill():
+    """The descriptor A fill (only the shared vertices' gradients of tet j gathered) writes the
+    same bits as the reference-structured fill, on a jittered block and on the reference's
+    irregular TetGen mesh (face, edge and vertex neighbours: 3, 2 and 1 shared vertices);
+    the descriptors hold the shared-vertex counts of B B^T."""
+    from controllable_pbd_3d_amd.mesh import load_medit_mesh
+    import os
+    gold = os.path.join(os.path.dirname(__file__), "golden", "meshes", "cube_tet.mesh")
+    for mesh in (small_block(6, seed=4), load_medit_mesh(gold)):
+        v0 = deformed(mesh, seed=4, amp=0.05, twist=0.1)
+        s = _solver(mesh, v0)
+        s._compute_C_and_gradC()
+        A = s._A
+        s._fill_A()
+        shared = A.data.clone()
+        d_shared = A.diag_inv.clone()
+        s.ctx("mgpbd_fill_A", s.n_tets, A.slice_ptr.data_ptr(), A.sell_idx.data_ptr(),
+              A.row_len.data_ptr(), s.t_i.data_ptr(), s.v_invm.data_ptr(), s.gradC.data_ptr(),
+              s.alpha_tilde.data_ptr(), A.data.data_ptr(), A.diag_inv.data_ptr())
+        assert torch.equal(shared.view(torch.int32), A.data.view(torch.int32))
+        assert torch.equal(d_shared.view(torch.int32), A.diag_inv.view(torch.int32))
+        desc = s._A_desc.cpu().numpy().view(np.uint16)[A.sell_pos_host].astype(np.int64)
+        t = mesh.tets.reshape(-1, 4).astype(np.int64)
+        B = sp.csr_matrix((np.ones(t.size), (np.repeat(np.arange(len(t)), 4), t.reshape(-1))),
+                          shape=(len(t), mesh.n_verts))
+        S = (B @ B.T).tocsr()
+        rows = np.repeat(np.arange(s.n_tets), np.diff(s._A_indptr))
+        cnt = np.asarray(S[rows, s._A_indices]).reshape(-1)
+        diag = rows == s._A_indices
+        assert np.all(desc[diag] == 0x8000)
+        assert np.array_equal((desc[~diag] >> 12) & 3, cnt[~diag])
+        print(f"[desc] {mesh.name}: shared-vertex counts {np.bincount(cnt[~diag].astype(int))}",
+              flush=True)
+
+
 def test_sparse_kernels_bitexact(scene):
     """fill_A, diag_inv, SpMV, Jacobi, refill, restrict, prolong, scatter: identical FP32."""
     mesh, v0 = scene
