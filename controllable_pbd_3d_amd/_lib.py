@@ -46,6 +46,7 @@ SIGNATURES = {
     "mgpbd_sell_layout": (c_int, [c_int, I32P, I32P]),
     "mgpbd_sell_map": (c_int, [c_int, I32P, I32P, I32P]),
     "mgpbd_sell_columns": (c_int, [c_int, I32P, I32P, I32P, I32P]),
+    "mgpbd_sell_window": (c_int, [c_int, I32P, I32P, I32P, I32P, c_int64]),
     # context
     "mgpbd_ctx_create": (_V, [_V]),
     "mgpbd_ctx_set_stream": (c_int, [_V, _V]),
@@ -93,6 +94,7 @@ SIGNATURES = {
     "mgpbd_mg_coarse_stats": (c_int, [_V, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "mgpbd_mg_set_precision": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_level_half": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
+    "mgpbd_mg_set_level_window": (c_int, [_V, c_int, _V, _V, c_int, c_int]),
     "mgpbd_mg_convert_half": (c_int, [_V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_mg_set_cycle": (c_int, [_V, c_int, c_float]),

@@ -1372,6 +1372,164 @@ k_power_step_h(int n, int n_slices, const int* __restrict__ slice_ptr, const int
   store_partial(ws, 2, s2);
 }
 
+// ------------------------------------------------------------------------------------------
+// Windowed packed levels: the x values a block of kBlock rows reads are staged in LDS first.
+// The block's window is the sorted list of the 32-column segments its rows' columns fall in
+// (segs[seg_ptr[blk] ..]), segment k at LDS floats [32 k, 32 k + 32); the high half of each
+// packed word holds the LDS byte offset of its column, (32 k + col % 32) * 4, instead of the
+// column offset j - i. A row then streams 4 bytes per entry from HBM and reads x from LDS, and
+// x is fetched once per block, 128 coalesced bytes per segment, instead of once per entry.
+// Sums run over the same words in the same order as the global-gather rows above, so results
+// are bit-identical to them. Blocks go to XCDs round-robin (block b on XCD b % 8): row block
+// for launch block b keeps each XCD on a contiguous range of rows, so its L2 holds the part
+// of x that its windows cover.
+// ------------------------------------------------------------------------------------------
+constexpr int kWinSeg = MGPBD_WINDOW_SEG;          // columns per segment
+constexpr int kWinMaxSegs = MGPBD_WINDOW_MAX_SEGS; // 48 KiB of LDS per block; offsets fit 16 bits
+static_assert(MGPBD_WINDOW_ROWS == kBlock, "window blocks are the sweep workgroups");
+static_assert(kWinSeg * kWinMaxSegs * 4 <= 65536, "LDS byte offsets fit 16 bits");
+
+__device__ __forceinline__ int xcd_row_block(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, k = b & 7;
+  return k * q + min(k, r) + (b >> 3);
+}
+
+// Stages block blk's window of x into xs (all threads of the block take part).
+__device__ __forceinline__ void stage_window(float* xs, const int* __restrict__ seg_ptr,
+                                             const int* __restrict__ segs, int blk,
+                                             const float* __restrict__ x) {
+  const int s0 = seg_ptr[blk];
+  const int ns = seg_ptr[blk + 1] - s0;   // >= 1: every block holds a row with a diagonal
+  const int sub = threadIdx.x >> 5, ln = threadIdx.x & 31;
+  constexpr int kPer = kBlock / kWinSeg;  // segments per pass
+  // passes past the window repeat its last segment: same slot, same value
+  for (int k0 = 0; k0 < ns; k0 += 8 * kPer) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = min(k0 + kPer * u + sub, ns - 1);
+      v[u] = x[segs[s0 + k] * kWinSeg + ln];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = min(k0 + kPer * u + sub, ns - 1);
+      xs[k * kWinSeg + ln] = v[u];
+    }
+  }
+  __syncthreads();
+}
+
+template <int VT>
+__device__ __forceinline__ float sellw_row(int i, const int* __restrict__ slice_ptr,
+                                           const uint32_t* __restrict__ hpk, const float* xs) {
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  const char* xl = reinterpret_cast<const char*>(xs);
+  float acc = 0.0f;
+#pragma unroll 4
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const v4u32 e = __builtin_nontemporal_load(reinterpret_cast<const v4u32*>(hpk + p0 + 64 * k0));
+    acc = fmaf(h2f<VT>((unsigned short)e.x), *reinterpret_cast<const float*>(xl + (e.x >> 16)), acc);
+    acc = fmaf(h2f<VT>((unsigned short)e.y), *reinterpret_cast<const float*>(xl + (e.y >> 16)), acc);
+    acc = fmaf(h2f<VT>((unsigned short)e.z), *reinterpret_cast<const float*>(xl + (e.z >> 16)), acc);
+    acc = fmaf(h2f<VT>((unsigned short)e.w), *reinterpret_cast<const float*>(xl + (e.w >> 16)), acc);
+  }
+  return acc;
+}
+
+// Windowed counterpart of k_sellh (same MODE semantics), dynamic LDS of max_segs * 128 bytes.
+template <int VT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* __restrict__ hpk,
+        const int* __restrict__ seg_ptr, const int* __restrict__ segs,
+        const float* __restrict__ x, float* y, const float* __restrict__ b,
+        float omega, const float* __restrict__ wp, const float* __restrict__ isd,
+        const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
+  extern __shared__ float xs[];
+  if (wp) omega = *wp;
+  if (done && *done) return;
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, x);
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const float ax = sellw_row<VT>(i, slice_ptr, hpk, xs);
+  if (i >= n) return;
+  if (MODE == 2) {
+    y[i] = b[i] - ax;
+  } else if (MODE == 1 || MODE == 3) {
+    const float xn = fmaf(omega, b[i] - ax, x[i]);
+    y[i] = MODE == 3 ? isd[i] * xn : xn;
+  } else {
+    const float xi = x[i];
+    const float u = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
+    const float xn = fmaf(cc[1], b[i] - ax, u);
+    y[i] = MODE == 5 ? isd[i] * xn : xn;
+  }
+}
+
+// Windowed counterpart of k_power_step_h; partials stored at the row block's slot, so the
+// totals are those of k_power_step_h.
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_power_step_w(int n, int n_slices, const int* __restrict__ slice_ptr,
+               const uint32_t* __restrict__ hpk, const int* __restrict__ seg_ptr,
+               const int* __restrict__ segs, const float* __restrict__ v, float* __restrict__ w,
+               RedWS ws) {
+  extern __shared__ float xs[];
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, v);
+  double vav = 0.0, vv = 0.0, ww = 0.0;
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const float ax = sellw_row<VT>(i, slice_ptr, hpk, xs);
+    if (i < n) {
+      w[i] = ax;
+      vav = (double)v[i] * ax;
+      vv = (double)v[i] * v[i];
+      ww = (double)ax * ax;
+    }
+  }
+  const double s0 = block_sum(vav);
+  const double s1 = block_sum(vv);
+  const double s2 = block_sum(ww);
+  if (threadIdx.x == 0) {
+    ws.partials[blk] = s0;
+    ws.partials[(size_t)kPartialStride + blk] = s1;
+    ws.partials[(size_t)2 * kPartialStride + blk] = s2;
+  }
+}
+
+// High halves of a windowed level's packed words: the LDS byte offset of each slot's column
+// in its block's window (binary search of the sorted segment list); 0 for padding rows, whose
+// words carry value 0. Low halves (the value bits) are kept.
+__global__ void __launch_bounds__(kBlock)
+k_window_offsets(int n, int n_slices, const int* __restrict__ slice_ptr,
+                 const int* __restrict__ idx, const int* __restrict__ seg_ptr,
+                 const int* __restrict__ segs, uint32_t* __restrict__ hpk, int* __restrict__ bad) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  const int blk = i / kBlock;
+  const int lo0 = seg_ptr[blk], hi0 = seg_ptr[blk + 1];
+  for (int k = 0; k < w; ++k) {
+    const int p = p0 + 64 * (k & ~3) + (k & 3);
+    uint32_t off = 0;
+    if (i < n) {
+      const int c = idx[p];
+      const int sg = c / kWinSeg;
+      int lo = lo0, hi = hi0;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (segs[mid] < sg) lo = mid + 1; else hi = mid;
+      }
+      if (lo < hi0 && segs[lo] == sg) off = (uint32_t)((lo - lo0) * kWinSeg + c % kWinSeg) * 4u;
+      else atomicOr(bad, 1);
+    }
+    hpk[p] = (off << 16) | (hpk[p] & 0xFFFFu);
+  }
+}
+
 // sd = sqrt(A_ii), isd = 1 / sd (1 where A_ii <= 0)
 __global__ void __launch_bounds__(kBlock)
 k_half_scales(int n, const int* __restrict__ diag_pos, const float* __restrict__ val,
@@ -1387,13 +1545,14 @@ k_half_scales(int n, const int* __restrict__ diag_pos, const float* __restrict__
 
 // half(val_ij isd_i isd_j) over the whole padded row, four entries (one 16-byte index and
 // value load) at a time, stored as packed words (j - i) << 16 | half (one 16-byte store;
-// padding: 0) when hpk != NULL, else into hval (one 8-byte store)
+// padding: 0) when hpk != NULL, else into hval (one 8-byte store); with keep_hi (windowed
+// levels) the words' high halves, the LDS offsets, are kept and only the values replaced
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
               const int* __restrict__ idx, const float* __restrict__ val,
               const float* __restrict__ isd, uint16_t* __restrict__ hval,
-              uint32_t* __restrict__ hpk) {
+              uint32_t* __restrict__ hpk, int keep_hi) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
   const int p0 = sell_base(slice_ptr, i);
@@ -1409,7 +1568,14 @@ k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
     h.y = f2h<VT>(row ? v.y * si * isd[c.y] : 0.0f);
     h.z = f2h<VT>(row ? v.z * si * isd[c.z] : 0.0f);
     h.w = f2h<VT>(row ? v.w * si * isd[c.w] : 0.0f);
-    if (hpk) {
+    if (hpk && keep_hi) {   // windowed level: LDS offsets stay
+      v4u32 e = *reinterpret_cast<const v4u32*>(hpk + p);
+      e.x = (e.x & 0xFFFF0000u) | h.x;
+      e.y = (e.y & 0xFFFF0000u) | h.y;
+      e.z = (e.z & 0xFFFF0000u) | h.z;
+      e.w = (e.w & 0xFFFF0000u) | h.w;
+      *reinterpret_cast<v4u32*>(hpk + p) = e;
+    } else if (hpk) {
       v4u32 e;
       e.x = (row ? (uint32_t)(c.x - i) << 16 : 0u) | h.x;
       e.y = (row ? (uint32_t)(c.y - i) << 16 : 0u) | h.y;
@@ -1552,6 +1718,10 @@ struct MgLevel {
   // beside the int32 columns (hval)
   uint16_t* hval = nullptr;
   uint32_t* hpk = nullptr;
+  // LDS window of each block of kBlock rows (windowed packed levels, else null)
+  const int* seg_ptr = nullptr;
+  const int* segs = nullptr;
+  int win_segs = 0;         // largest window (segments) of the level
   float* sd = nullptr;      // sqrt(A_ii)
   float* isd = nullptr;     // 1 / sqrt(A_ii)
   const int* diag_pos = nullptr;
@@ -2058,6 +2228,7 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
   L.slice_ptr = slice_ptr; L.idx = idx; L.val = val; L.dinv = dinv;
   L.agg = agg; L.mptr = mptr; L.mem = mem; L.work = work;
   L.diagonal = diagonal_only;
+  L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0;
   return 0;
 }
 
@@ -2298,7 +2469,18 @@ static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float
   const float w = ctx->omega;
 #define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
                                                          L.hpk, L.hval, x, y, b, w, wp, L.isd, done, xp, cc)
-  if (L.hpk) {
+#define SELLW(M) k_sellw<VT, M><<<g, kBlock, lds, s>>>(L.n, L.n_slices, L.slice_ptr, L.hpk, L.seg_ptr, \
+                                                     L.segs, x, y, b, w, wp, L.isd, done, xp, cc)
+  const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
+  if (L.hpk && L.segs) {
+    switch (mode) {
+      case 1: SELLW(1); break;
+      case 2: SELLW(2); break;
+      case 3: SELLW(3); break;
+      case 4: SELLW(4); break;
+      default: SELLW(5); break;
+    }
+  } else if (L.hpk) {
     switch (mode) {
       case 1: SELLH(true, 1); break;
       case 2: SELLH(true, 2); break;
@@ -2316,6 +2498,7 @@ static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float
     }
   }
 #undef SELLH
+#undef SELLW
 }
 }  // extern "C++"
 
@@ -2480,6 +2663,12 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
       if (ctx->precision == MGPBD_PREC_FP32)
         k_power_step<<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.dinv, v,
                                           w, ctx->ws);
+      else if (L.segs && ctx->precision == MGPBD_PREC_FP16)
+        k_power_step_w<1><<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
+            L.n, L.n_slices, L.slice_ptr, L.hpk, L.seg_ptr, L.segs, v, w, ctx->ws);
+      else if (L.segs)
+        k_power_step_w<2><<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
+            L.n, L.n_slices, L.slice_ptr, L.hpk, L.seg_ptr, L.segs, v, w, ctx->ws);
       else if (ctx->precision == MGPBD_PREC_FP16 && L.hpk)
         k_power_step_h<1, true><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.hpk,
                                                      L.hval, v, w, ctx->ws);
@@ -2560,6 +2749,35 @@ int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, uint32_t*
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_half: invalid arguments");
   MgLevel& L = ctx->L[level];
   L.hval = hpacked ? nullptr : hval; L.hpk = hpacked; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
+  L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0;
+  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  return 0;
+}
+
+int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr,
+                              const int32_t* segs, int n_blocks, int max_segs) {
+  CTX_CHECK("mgpbd_mg_set_level_window");
+  if (level < 0 || level >= ctx->n_levels || !seg_ptr || !segs)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: invalid arguments");
+  MgLevel& L = ctx->L[level];
+  if (!L.hpk || !L.idx || L.n <= 0)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: level needs packed half words "
+                                    "(mgpbd_mg_set_level_half with hpacked)");
+  if (n_blocks != nblocks(64LL * L.n_slices) || max_segs < 1 || max_segs > kWinMaxSegs)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: n_blocks must be ceil(64 n_slices / " +
+                                    std::to_string(kBlock) + "), max_segs in [1, " +
+                                    std::to_string(kWinMaxSegs) + "]");
+  int* bad = reinterpret_cast<int*>(ctx->ws.partials + (size_t)3 * kPartialStride);
+  MGPBD_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
+  k_window_offsets<<<n_blocks, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx,
+                                                           seg_ptr, segs, L.hpk, bad);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_set_level_window");
+  int hb = 0;
+  MGPBD_HIP_TRY(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  MGPBD_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (hb) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: a column lies outside its "
+                                          "block's window");
+  L.seg_ptr = seg_ptr; L.segs = segs; L.win_segs = max_segs;
   if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   return 0;
 }
@@ -2574,9 +2792,11 @@ int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
     k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
     const int g = nblocks(64LL * L.n_slices);
     if (ctx->precision == MGPBD_PREC_FP16)
-      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk);
+      k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk,
+                                                        L.segs != nullptr);
     else
-      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk);
+      k_half_values<2><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk,
+                                                        L.segs != nullptr);
   }
   MGPBD_LAUNCH_CHECK("mgpbd_mg_convert_half");
   return 0;

@@ -703,4 +703,56 @@ int mgpbd_sell_columns(int n_rows, const int32_t* indptr, const int32_t* indices
   return 0;
 }
 
+// LDS windows of the windowed packed sweeps: for each block of MGPBD_WINDOW_ROWS rows, the
+// sorted ids of the MGPBD_WINDOW_SEG-column segments holding its rows' columns and its rows'
+// own indices (the column of a real row's SELL padding). One pass over the entries with a
+// per-thread stamp array, then a sort of each block's (short) list.
+int mgpbd_sell_window(int n_rows, const int32_t* indptr, const int32_t* indices,
+                      int32_t* seg_ptr, int32_t* segs, int64_t cap) {
+  if (n_rows <= 0 || !indptr || !indices || !seg_ptr)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_window: invalid arguments");
+  const int rows = MGPBD_WINDOW_ROWS, seg = MGPBD_WINDOW_SEG;
+  const int64_t n_slices = (n_rows + 63) / 64;
+  const int nb = (int)((64 * n_slices + rows - 1) / rows);
+  int n_cols = n_rows;
+  for (int64_t e = 0; e < indptr[n_rows]; ++e) {
+    if (indices[e] < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_window: negative column");
+    n_cols = std::max(n_cols, indices[e] + 1);
+  }
+  const int n_seg = (n_cols + seg - 1) / seg;
+  std::vector<std::vector<int32_t>> lists(nb);
+#pragma omp parallel
+  {
+    std::vector<int32_t> stamp(n_seg, -1);
+#pragma omp for schedule(static)
+    for (int b = 0; b < nb; ++b) {
+      std::vector<int32_t>& out = lists[b];
+      const int r1 = std::min(n_rows, (b + 1) * rows);
+      for (int r = b * rows; r < r1; ++r) {
+        const int own = r / seg;
+        if (stamp[own] != b) { stamp[own] = b; out.push_back(own); }
+        for (int32_t e = indptr[r]; e < indptr[r + 1]; ++e) {
+          const int sg = indices[e] / seg;
+          if (stamp[sg] != b) { stamp[sg] = b; out.push_back(sg); }
+        }
+      }
+      if (out.empty()) out.push_back(0);   // blocks of padding rows only: one dummy segment
+      std::sort(out.begin(), out.end());
+    }
+  }
+  int64_t total = 0;
+  for (int b = 0; b < nb; ++b) {
+    seg_ptr[b] = (int32_t)total;
+    total += (int64_t)lists[b].size();
+  }
+  if (total > INT32_MAX) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_window: too many segments");
+  seg_ptr[nb] = (int32_t)total;
+  if (!segs) return 0;
+  if (cap < total) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_sell_window: segs capacity below seg_ptr[n_blocks]");
+#pragma omp parallel for schedule(static)
+  for (int b = 0; b < nb; ++b)
+    std::copy(lists[b].begin(), lists[b].end(), segs + seg_ptr[b]);
+  return 0;
+}
+
 }  // extern "C"

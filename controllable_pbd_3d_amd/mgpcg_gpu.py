@@ -208,6 +208,27 @@ class GPUMultigridSolver:
     # reduced-precision levels pack 16-bit column offsets j - i with the values when every
     # |j - i| is below this bound, and use int32 columns (shared with the FP32 copy) otherwise
     col16_limit = 32768
+    # packed levels whose blocks of 256 rows read x from at most this many 32-column segments
+    # stage them in LDS (windowed sweeps; the words then hold LDS offsets, so the 16-bit column
+    # offset bound does not apply); 0 disables
+    window_segs_limit = 384
+
+    def _window(self, L):
+        """(seg_ptr, segs, largest window) of a level's LDS windows, None above the limit."""
+        if self.window_segs_limit <= 0 or L.n_rows == 0:
+            return None
+        lib = _lib.load()
+        nb = (64 * ((L.n_rows + 63) // 64) + 255) // 256
+        seg_ptr = np.zeros(nb + 1, np.int32)
+        check(lib.mgpbd_sell_window(L.n_rows, np_ptr(L.indptr), np_ptr(L.indices),
+                                    np_ptr(seg_ptr), None, 0), "mgpbd_sell_window")
+        widest = int(np.diff(seg_ptr).max())
+        if widest > min(self.window_segs_limit, 384):
+            return None
+        segs = np.zeros(max(int(seg_ptr[-1]), 1), np.int32)
+        check(lib.mgpbd_sell_window(L.n_rows, np_ptr(L.indptr), np_ptr(L.indices),
+                                    np_ptr(seg_ptr), np_ptr(segs), len(segs)), "mgpbd_sell_window")
+        return seg_ptr, segs, widest
 
     def _setup_half(self):
         """Reduced-precision copies of every level for the V-cycle (extension)."""
@@ -220,7 +241,9 @@ class GPUMultigridSolver:
         for l, L in enumerate(self.levels):
             rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
             delta = L.indices.astype(np.int64) - rows
-            use16 = bool(len(delta) == 0 or np.abs(delta).max() < self.col16_limit)
+            win = self._window(L)
+            use16 = bool(win is not None or len(delta) == 0 or
+                         np.abs(delta).max() < self.col16_limit)
             slots = max(L.padded, 1)
             hb = {
                 # packed (col - row) << 16 | half words, or half values beside int32 columns
@@ -231,6 +254,12 @@ class GPUMultigridSolver:
             }
             self.ctx("mgpbd_mg_set_level_half", l, dptr(hb["hval"]), dptr(hb["hpk"]),
                      dptr(hb["sd"]), dptr(hb["isd"]), dptr(L.diag_pos))
+            hb["seg_ptr"] = hb["segs"] = None
+            if win is not None:
+                hb["seg_ptr"] = torch.from_numpy(win[0]).to(dev)
+                hb["segs"] = torch.from_numpy(win[1]).to(dev)
+                self.ctx("mgpbd_mg_set_level_window", l, dptr(hb["seg_ptr"]), dptr(hb["segs"]),
+                         len(win[0]) - 1, win[2])
             self._half.append(hb)
 
     @staticmethod

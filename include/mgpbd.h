@@ -119,6 +119,17 @@ int mgpbd_sell_map(int n_rows, const int32_t* indptr, const int32_t* slice_ptr,
  * column 0), sell_idx sized slice_ptr[n_slices]. */
 int mgpbd_sell_columns(int n_rows, const int32_t* indptr, const int32_t* indices,
                        const int32_t* slice_ptr, int32_t* sell_idx);
+/* LDS windows for the windowed reduced-precision sweeps (no reference counterpart: the
+ * reference's SpMV gathers x from global memory). Blocks of MGPBD_WINDOW_ROWS rows (the sweep
+ * kernels' workgroup, ceil(64 n_slices / MGPBD_WINDOW_ROWS) blocks); for each, the sorted ids
+ * of the MGPBD_WINDOW_SEG-column segments holding its rows' columns and own indices, in
+ * segs[seg_ptr[b] .. seg_ptr[b+1]). seg_ptr has n_blocks + 1 entries; with segs NULL only
+ * seg_ptr is filled (seg_ptr[n_blocks] = the size segs needs), else cap >= that size. */
+#define MGPBD_WINDOW_ROWS 256
+#define MGPBD_WINDOW_SEG 32
+#define MGPBD_WINDOW_MAX_SEGS 384
+int mgpbd_sell_window(int n_rows, const int32_t* indptr, const int32_t* indices,
+                      int32_t* seg_ptr, int32_t* segs, int64_t cap);
 
 /* ===================================================================================== */
 /* Device context                                                                        */
@@ -341,6 +352,15 @@ int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible);
  * mgpbd_mg_convert_half), diag_pos_dev (SELL position of each row's diagonal). */
 int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, uint32_t* hpacked_dev,
                             float* sd_dev, float* isd_dev, const int32_t* diag_pos_dev);
+/* Switches a packed level to the windowed sweeps (no reference counterpart; same results as
+ * the packed sweeps, bit for bit): each workgroup stages its block's window of x in LDS
+ * (seg_ptr_dev / segs_dev from mgpbd_sell_window on the level's CSR, n_blocks blocks, largest
+ * window max_segs <= MGPBD_WINDOW_MAX_SEGS segments) and the packed words' high halves become
+ * LDS byte offsets (rewritten here; mgpbd_mg_convert_half keeps them). The arrays must stay
+ * valid while the level is used; mgpbd_mg_set_level / _set_level_half reset the window.
+ * Synchronises the context's stream. */
+int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr_dev,
+                              const int32_t* segs_dev, int n_blocks, int max_segs);
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx);

@@ -411,3 +411,35 @@ def test_oracle_c_pattern_equals_literal():
                       om.ptr(indices, om.I32P))
         ip, ix = structure.build_sparsity_pattern(m.tets, m.n_tets)
         assert np.array_equal(indptr, ip) and np.array_equal(indices, ix), name
+
+
+@pytest.mark.parametrize("n_rows,seed", [(1, 0), (255, 1), (700, 2), (1000, 3)])
+def test_sell_window_lists_each_blocks_segments(n_rows, seed):
+    """LDS windows: for each block of 256 rows (over the slice-padded row range), the sorted
+    distinct 32-column segments of its rows' columns and own indices; checked against a set
+    computed row by row, on random sparse patterns with columns up to 4x the row count and
+    empty rows."""
+    from controllable_pbd_3d_amd import _lib
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 12, n_rows)
+    lens[rng.random(n_rows) < 0.1] = 0
+    indptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    indices = rng.integers(0, 4 * n_rows, int(indptr[-1])).astype(np.int32)
+    lib = _lib.load()
+    nb = (64 * ((n_rows + 63) // 64) + 255) // 256
+    seg_ptr = np.zeros(nb + 1, np.int32)
+    _lib.check(lib.mgpbd_sell_window(n_rows, _lib.np_ptr(indptr), _lib.np_ptr(indices),
+                                     _lib.np_ptr(seg_ptr), None, 0), "mgpbd_sell_window")
+    segs = np.zeros(max(int(seg_ptr[-1]), 1), np.int32)
+    assert lib.mgpbd_sell_window(n_rows, _lib.np_ptr(indptr), _lib.np_ptr(indices),
+                                 _lib.np_ptr(seg_ptr), _lib.np_ptr(segs), len(segs) - 1) == -1
+    _lib.check(lib.mgpbd_sell_window(n_rows, _lib.np_ptr(indptr), _lib.np_ptr(indices),
+                                     _lib.np_ptr(seg_ptr), _lib.np_ptr(segs), len(segs)),
+               "mgpbd_sell_window")
+    for b in range(nb):
+        want = set()
+        for r in range(256 * b, min(n_rows, 256 * b + 256)):
+            want.add(r // 32)
+            want.update((indices[indptr[r]:indptr[r + 1]] // 32).tolist())
+        got = segs[seg_ptr[b]:seg_ptr[b + 1]]
+        assert got.tolist() == sorted(want), b
