@@ -105,70 +105,132 @@ k_gemm64(int n, const uint16_t* __restrict__ A1, const uint16_t* __restrict__ W1
 }
 
 // Inference layer with the neighbour mean fused in: Y = relu(H W_self + mean_nbr(H) W_nbr + b)
-// for the same 32-row MFMA tiles as k_gemm64. Each lane forms its own fragment of the mean
-// (the four neighbour rows summed in FP32 in neighbour order, times 1/deg, rounded to BF16:
-// the arithmetic of k_mean_nbr), so the output equals k_mean_nbr + k_gemm64 bit for bit
-// without the mean's n x 64 round trip through memory.
+// over 32-row MFMA tiles, with the arithmetic of k_mean_nbr + k_gemm64 (same fragments, same
+// MFMA sequence), so the output equals theirs bit for bit without the mean's n x 64 round trip
+// through memory. Memory side: each block stages both weight matrices in LDS once and its
+// wavefronts loop over tiles; a tile's own rows are loaded as 16-byte row chunks (eight lanes
+// per 128-byte row), and each (row, 8-column chunk) pair gathers the same chunk of its four
+// neighbour rows and forms the mean there (the four rows summed in FP32 in neighbour order,
+// times 1/deg, rounded to BF16). MFMA operands are read from LDS rows padded to 68 halves
+// (8-byte aligned, rows two banks apart), and the output tile goes back through LDS to
+// 16-byte row-chunk stores.
+constexpr int kLdsRow = H + 4;
+constexpr int kTileWaves = kBlock / 64;
+constexpr int kLayerBlocks = 768;   // 3 blocks (52 KiB of LDS each) per CU on 256 CUs
+
+typedef unsigned short v8u16 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void lds_store8(uint16_t* dst, v8u16 v) {
+  reinterpret_cast<v4i16*>(dst)[0] = v4i16{(short)v[0], (short)v[1], (short)v[2], (short)v[3]};
+  reinterpret_cast<v4i16*>(dst)[1] = v4i16{(short)v[4], (short)v[5], (short)v[6], (short)v[7]};
+}
+__device__ __forceinline__ v8u16 lds_load8(const uint16_t* src) {
+  const v4i16 lo = reinterpret_cast<const v4i16*>(src)[0];
+  const v4i16 hi = reinterpret_cast<const v4i16*>(src)[1];
+  return v8u16{(unsigned short)lo.x, (unsigned short)lo.y, (unsigned short)lo.z, (unsigned short)lo.w,
+               (unsigned short)hi.x, (unsigned short)hi.y, (unsigned short)hi.z, (unsigned short)hi.w};
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ Hin,
              const uint16_t* __restrict__ Wst, const uint16_t* __restrict__ Wnt,
              const float* __restrict__ bias, uint16_t* __restrict__ Hout) {
-  const int lane = threadIdx.x & 63;
-  const int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
-  if (base >= n) return;
-  const int ra = base + (lane & 31);
-  const int kq = 4 * (lane >> 5);
-  v16f32 acc0 = {}, acc1 = {};
-  const v4i16 zero = {0, 0, 0, 0};
-#pragma unroll
-  for (int s = 0; s < H / 8; ++s) {
-    const v4i16 a = ra < n ? load4(Hin + (size_t)ra * H + 8 * s + kq) : zero;
-    const v4i16 b0 = load4(Wst + (size_t)(lane & 31) * H + 8 * s + kq);
-    const v4i16 b1 = load4(Wst + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+  __shared__ __align__(16) uint16_t wl[2][H * kLdsRow];
+  __shared__ __align__(16) uint16_t tl[kTileWaves][2][32 * kLdsRow];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int c = threadIdx.x; c < 2 * H * 8; c += kBlock) {   // 16-byte chunks of both matrices
+    const int m = c / (H * 8), r = (c / 8) % H, ch = c % 8;
+    const uint16_t* src = (m ? Wnt : Wst) + r * H + 8 * ch;
+    lds_store8(wl[m] + r * kLdsRow + 8 * ch, *reinterpret_cast<const v8u16*>(src));
   }
-  int nb4[4] = {-1, -1, -1, -1};
-  int d = 0;
-  if (ra < n) {
-    const int4 q = reinterpret_cast<const int4*>(nbr)[ra];
-    nb4[0] = q.x; nb4[1] = q.y; nb4[2] = q.z; nb4[3] = q.w;
+  const int n_tiles = (n + 31) / 32;
+  const int kq = 4 * (lane >> 5), rl = lane & 31;
+  uint16_t* ts = tl[w][0];
+  uint16_t* tm = tl[w][1];
+  const float bias0 = bias[rl], bias1 = bias[32 + rl];
+  // the loop bound is the block's, so every wavefront meets every barrier
+  for (int t0 = blockIdx.x * kTileWaves; t0 < n_tiles; t0 += gridDim.x * kTileWaves) {
+    const int base = (t0 + w) * 32;
+    // all loads of the tile are issued before any is used: rows past n read row n - 1 and
+    // absent neighbours re-read the row itself, and selects drop them (a dropped neighbour
+    // leaves the sum untouched, as skipping it does)
+    int rows[4];
+    int4 nq[4];
+    v8u16 hv[4], hn[4][4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) d += nb4[k] >= 0;
-  }
-  const float inv = d > 0 ? 1.0f / (float)d : 0.0f;
-#pragma unroll
-  for (int s = 0; s < H / 8; ++s) {
-    float m[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (nb4[k] < 0) continue;
-      const v4i16 h = load4(Hin + (size_t)nb4[k] * H + 8 * s + kq);
-      m[0] += bf2f((uint16_t)h.x);
-      m[1] += bf2f((uint16_t)h.y);
-      m[2] += bf2f((uint16_t)h.z);
-      m[3] += bf2f((uint16_t)h.w);
+    for (int q = 0; q < 4; ++q) {
+      const int c = lane + 64 * q;
+      rows[q] = min(base + (c >> 3), n - 1);
+      hv[q] = *reinterpret_cast<const v8u16*>(Hin + (size_t)rows[q] * H + 8 * (c & 7));
+      nq[q] = reinterpret_cast<const int4*>(nbr)[rows[q]];
     }
-    v4i16 a;
-    a.x = (short)f2bf(m[0] * inv);
-    a.y = (short)f2bf(m[1] * inv);
-    a.z = (short)f2bf(m[2] * inv);
-    a.w = (short)f2bf(m[3] * inv);
-    const v4i16 b0 = load4(Wnt + (size_t)(lane & 31) * H + 8 * s + kq);
-    const v4i16 b1 = load4(Wnt + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
-  }
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = base + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
-    if (row >= n) continue;
+    for (int q = 0; q < 4; ++q) {
+      const int ch = (lane + 64 * q) & 7;
+      const int nb4[4] = {nq[q].x, nq[q].y, nq[q].z, nq[q].w};
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = 32 * c + (lane & 31);
-      const float v = (c == 0 ? acc0[r] : acc1[r]) + bias[col];
-      Hout[(size_t)row * H + col] = f2bf(fmaxf(v, 0.0f));
+      for (int k = 0; k < 4; ++k) {
+        const int j = nb4[k] >= 0 ? nb4[k] : rows[q];
+        hn[q][k] = *reinterpret_cast<const v8u16*>(Hin + (size_t)j * H + 8 * ch);
+      }
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = lane + 64 * q, r = c >> 3, ch = c & 7;
+      const bool live = base + r < n;
+      const int nb4[4] = {nq[q].x, nq[q].y, nq[q].z, nq[q].w};
+      int d = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d += nb4[k] >= 0;
+      const float inv = d > 0 ? 1.0f / (float)d : 0.0f;
+      float m[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const float t = m[e] + bf2f(hn[q][k][e]);
+          m[e] = nb4[k] >= 0 ? t : m[e];
+        }
+      }
+      v8u16 mv = {}, hz = {};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) mv[e] = f2bf(m[e] * inv);
+      lds_store8(ts + r * kLdsRow + 8 * ch, live ? hv[q] : hz);
+      lds_store8(tm + r * kLdsRow + 8 * ch, live ? mv : hz);
+    }
+    __syncthreads();
+    v16f32 acc0 = {}, acc1 = {};
+#pragma unroll
+    for (int s = 0; s < H / 8; ++s) {
+      const v4i16 a = load4(ts + rl * kLdsRow + 8 * s + kq);
+      const v4i16 b0 = load4(wl[0] + rl * kLdsRow + 8 * s + kq);
+      const v4i16 b1 = load4(wl[0] + (32 + rl) * kLdsRow + 8 * s + kq);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < H / 8; ++s) {
+      const v4i16 a = load4(tm + rl * kLdsRow + 8 * s + kq);
+      const v4i16 b0 = load4(wl[1] + rl * kLdsRow + 8 * s + kq);
+      const v4i16 b1 = load4(wl[1] + (32 + rl) * kLdsRow + 8 * s + kq);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    }
+    __syncthreads();   // every fragment read of the tile is done before it is overwritten
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int rr = 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+      ts[rr * kLdsRow + rl] = f2bf(fmaxf(acc0[r] + bias0, 0.0f));
+      ts[rr * kLdsRow + 32 + rl] = f2bf(fmaxf(acc1[r] + bias1, 0.0f));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = lane + 64 * q, r = c >> 3, ch = c & 7, row = base + r;
+      if (row < n)
+        *reinterpret_cast<v8u16*>(Hout + (size_t)row * H + 8 * ch) = lds_load8(ts + r * kLdsRow + 8 * ch);
+    }
+    __syncthreads();   // the tile's LDS rows are free for the next tile
   }
 }
 
@@ -728,7 +790,7 @@ int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
   GnnWorkspace W;
   carve(ws, n, W);
   k_encoder8<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, x, params, W.h[0]);
-  const int gg = (n + 127) / 128;   // 4 wavefronts x 32 rows per block
+  const int gg = std::min((n + 32 * kTileWaves - 1) / (32 * kTileWaves), kLayerBlocks);
   for (int l = 0; l < NL; ++l) {
     const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
