@@ -90,8 +90,9 @@ def solver_args(mesh, mu_t):
 
 
 def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
-    """Algorithmic HBM bytes per launch; per_nnz = 8 for FP32 values + int32 columns, 4 for
-    FP16/BF16 values + 16-bit column deltas (each array counted once). With the Chebyshev
+    """Algorithmic HBM bytes per launch; per_nnz = 8 for FP32 values + int32 columns, 6 for FP32
+    values + 16-bit LDS offsets (windowed level 0), 4 for FP16/BF16 values + 16-bit column
+    offsets or LDS offsets (each array counted once). With the Chebyshev
     smoother the level-0 sweeps also read the previous iterate (all steps but the Jacobi start;
     the last post-step reads the unscaling factor too): 4 n bytes more on average over a
     V-cycle's 4 + 1 + 1 step kinds."""
@@ -169,16 +170,17 @@ def measured_traffic(workload, kind, precision):
     return None
 
 
-def kernel_stats(ctx, n, nnz, precision, chebyshev=False):
+def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8):
     from controllable_pbd_3d_amd import _lib
     ms = np.zeros(8, np.float64)
     cnt = np.zeros(8, np.int64)
     ctx("mgpbd_ctx_profile_read", ms.ctypes.data_as(_lib.F64P),
         cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     kern = {}
-    # the outer PCG SpMV is FP32; the V-cycle passes use the precision under test
+    # the outer PCG SpMV is FP32 (values beside int32 columns, or beside 16-bit LDS offsets on
+    # a windowed level 0); the V-cycle passes use the precision under test
     vb = 8 if precision == "fp32" else 4
-    for kind, slot, per in (("spmv", 0, 8), ("jacobi", 1, vb), ("resid", 2, vb)):
+    for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
             by = kernel_bytes(n, nnz, kind, per, chebyshev)
@@ -311,9 +313,11 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     for k in range(profile_steps):
         load.step(s)
     cheb = kw.get("smoother") == "chebyshev"
-    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"], cheb)
-    kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"], cheb).items()
-                 if k not in kern})
+    win0 = any(w["level"] == 0 and w["woff"] is not None for w in s._gpu_solver._win)
+    per = 6 if win0 else 8
+    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"], cheb, per)
+    kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"], cheb,
+                                               per).items() if k not in kern})
     ctx("mgpbd_ctx_profile", 0)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 0)
     res["kernels"] = kern

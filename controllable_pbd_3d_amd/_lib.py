@@ -94,7 +94,7 @@ SIGNATURES = {
     "mgpbd_mg_coarse_stats": (c_int, [_V, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "mgpbd_mg_set_precision": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_level_half": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
-    "mgpbd_mg_set_level_window": (c_int, [_V, c_int, _V, _V, c_int, c_int]),
+    "mgpbd_mg_set_level_window": (c_int, [_V, c_int, _V, _V, c_int, c_int, _V]),
     "mgpbd_mg_convert_half": (c_int, [_V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_mg_set_cycle": (c_int, [_V, c_int, c_float]),

@@ -1394,10 +1394,11 @@ __device__ __forceinline__ int xcd_row_block(int b, int nb) {
   return k * q + min(k, r) + (b >> 3);
 }
 
-// Stages block blk's window of x into xs (all threads of the block take part).
+// Stages block blk's window of x (n_x entries; the last segment's tail past n_x reads entry
+// n_x - 1, which no column refers to) into xs (all threads of the block take part).
 __device__ __forceinline__ void stage_window(float* xs, const int* __restrict__ seg_ptr,
                                              const int* __restrict__ segs, int blk,
-                                             const float* __restrict__ x) {
+                                             const float* __restrict__ x, int n_x) {
   const int s0 = seg_ptr[blk];
   const int ns = seg_ptr[blk + 1] - s0;   // >= 1: every block holds a row with a diagonal
   const int sub = threadIdx.x >> 5, ln = threadIdx.x & 31;
@@ -1408,7 +1409,7 @@ __device__ __forceinline__ void stage_window(float* xs, const int* __restrict__ 
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int k = min(k0 + kPer * u + sub, ns - 1);
-      v[u] = x[segs[s0 + k] * kWinSeg + ln];
+      v[u] = x[min(segs[s0 + k] * kWinSeg + ln, n_x - 1)];
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -1449,7 +1450,7 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
   if (wp) omega = *wp;
   if (done && *done) return;
   const int blk = xcd_row_block(blockIdx.x, gridDim.x);
-  stage_window(xs, seg_ptr, segs, blk, x);
+  stage_window(xs, seg_ptr, segs, blk, x, n);
   const int i = blk * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
   const float ax = sellw_row<VT>(i, slice_ptr, hpk, xs);
@@ -1477,7 +1478,7 @@ k_power_step_w(int n, int n_slices, const int* __restrict__ slice_ptr,
                RedWS ws) {
   extern __shared__ float xs[];
   const int blk = xcd_row_block(blockIdx.x, gridDim.x);
-  stage_window(xs, seg_ptr, segs, blk, v);
+  stage_window(xs, seg_ptr, segs, blk, v, n);
   double vav = 0.0, vv = 0.0, ww = 0.0;
   const int i = blk * kBlock + threadIdx.x;
   if ((i >> 6) < n_slices) {
@@ -1499,13 +1500,15 @@ k_power_step_w(int n, int n_slices, const int* __restrict__ slice_ptr,
   }
 }
 
-// High halves of a windowed level's packed words: the LDS byte offset of each slot's column
-// in its block's window (binary search of the sorted segment list); 0 for padding rows, whose
-// words carry value 0. Low halves (the value bits) are kept.
+// LDS byte offset of each slot's column in its block's window (binary search of the sorted
+// segment list; 0 for padding rows, whose values are 0): into the high halves of a windowed
+// level's packed words (hpk, low halves -- the value bits -- kept) and into the 16-bit offset
+// array of the windowed FP32 kernels (woff); either may be NULL.
 __global__ void __launch_bounds__(kBlock)
 k_window_offsets(int n, int n_slices, const int* __restrict__ slice_ptr,
                  const int* __restrict__ idx, const int* __restrict__ seg_ptr,
-                 const int* __restrict__ segs, uint32_t* __restrict__ hpk, int* __restrict__ bad) {
+                 const int* __restrict__ segs, uint32_t* __restrict__ hpk,
+                 uint16_t* __restrict__ woff, int* __restrict__ bad) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
   const int p0 = sell_base(slice_ptr, i);
@@ -1526,8 +1529,92 @@ k_window_offsets(int n, int n_slices, const int* __restrict__ slice_ptr,
       if (lo < hi0 && segs[lo] == sg) off = (uint32_t)((lo - lo0) * kWinSeg + c % kWinSeg) * 4u;
       else atomicOr(bad, 1);
     }
-    hpk[p] = (off << 16) | (hpk[p] & 0xFFFFu);
+    if (hpk) hpk[p] = (off << 16) | (hpk[p] & 0xFFFFu);
+    if (woff) woff[p] = (uint16_t)off;
   }
+}
+
+// Windowed FP32 rows: FP32 values beside 16-bit LDS offsets (6 bytes per entry), x from the
+// block's window in LDS; products summed in the entry order of sell_row / sell_row_d, so the
+// results equal theirs bit for bit.
+template <bool ACC64>
+__device__ __forceinline__ double sellw_row_f32(int i, const int* __restrict__ slice_ptr,
+                                                const uint16_t* __restrict__ woff,
+                                                const float* __restrict__ val, const float* xs) {
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  const char* xl = reinterpret_cast<const char*>(xs);
+  double accd = 0.0;
+  float accf = 0.0f;
+#pragma unroll 4
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const int p = p0 + 64 * k0;
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(val + p));
+    const v4u16 o = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(woff + p));
+    const float x0 = *reinterpret_cast<const float*>(xl + o.x);
+    const float x1 = *reinterpret_cast<const float*>(xl + o.y);
+    const float x2 = *reinterpret_cast<const float*>(xl + o.z);
+    const float x3 = *reinterpret_cast<const float*>(xl + o.w);
+    if (ACC64) {
+      accd = fma((double)v.x, (double)x0, accd);
+      accd = fma((double)v.y, (double)x1, accd);
+      accd = fma((double)v.z, (double)x2, accd);
+      accd = fma((double)v.w, (double)x3, accd);
+    } else {
+      accf = fmaf(v.x, x0, accf);
+      accf = fmaf(v.y, x1, accf);
+      accf = fmaf(v.z, x2, accf);
+      accf = fmaf(v.w, x3, accf);
+    }
+  }
+  return ACC64 ? accd : (double)accf;
+}
+
+// Windowed k_pcg_spmv_dot (partials at the row block's slot: the same totals)
+template <bool ACC64>
+__global__ void __launch_bounds__(kBlock)
+k_pcg_spmv_dot_w(int n, int n_slices, const int* __restrict__ slice_ptr,
+                 const uint16_t* __restrict__ woff, const float* __restrict__ val,
+                 const int* __restrict__ seg_ptr, const int* __restrict__ segs,
+                 const float* __restrict__ p, float* __restrict__ Ap, RedWS ws,
+                 const PcgState* st) {
+  extern __shared__ float xs[];
+  if (st->done) return;
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, p, n);
+  double acc = 0.0;
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const float ax = (float)sellw_row_f32<ACC64>(i, slice_ptr, woff, val, xs);
+    if (i < n) {
+      Ap[i] = ax;
+      acc = (double)p[i] * (double)ax;
+    }
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) ws.partials[blk] = t;
+}
+
+// Windowed k_true_resid (partials at the row block's slot)
+__global__ void __launch_bounds__(kBlock)
+k_true_resid_w(int n, int n_slices, const int* __restrict__ slice_ptr,
+               const uint16_t* __restrict__ woff, const float* __restrict__ val,
+               const int* __restrict__ seg_ptr, const int* __restrict__ segs,
+               const float* __restrict__ x, const float* __restrict__ b, RedWS ws) {
+  extern __shared__ float xs[];
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, x, n);
+  double acc = 0.0;
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const double ax = sellw_row_f32<true>(i, slice_ptr, woff, val, xs);
+    if (i < n) {
+      const double ri = (double)b[i] - ax;
+      acc = ri * ri;
+    }
+  }
+  const double t = block_sum(acc);
+  if (threadIdx.x == 0) ws.partials[blk] = t;
 }
 
 // sd = sqrt(A_ii), isd = 1 / sd (1 where A_ii <= 0)
@@ -1722,6 +1809,7 @@ struct MgLevel {
   const int* seg_ptr = nullptr;
   const int* segs = nullptr;
   int win_segs = 0;         // largest window (segments) of the level
+  uint16_t* woff = nullptr; // LDS offsets beside the FP32 values (windowed FP32 kernels)
   float* sd = nullptr;      // sqrt(A_ii)
   float* isd = nullptr;     // 1 / sqrt(A_ii)
   const int* diag_pos = nullptr;
@@ -2228,7 +2316,7 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
   L.slice_ptr = slice_ptr; L.idx = idx; L.val = val; L.dinv = dinv;
   L.agg = agg; L.mptr = mptr; L.mem = mem; L.work = work;
   L.diagonal = diagonal_only;
-  L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0;
+  L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0; L.woff = nullptr;
   return 0;
 }
 
@@ -2749,20 +2837,20 @@ int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, uint32_t*
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_half: invalid arguments");
   MgLevel& L = ctx->L[level];
   L.hval = hpacked ? nullptr : hval; L.hpk = hpacked; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
-  L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0;
+  L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0; L.woff = nullptr;
   if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   return 0;
 }
 
 int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr,
-                              const int32_t* segs, int n_blocks, int max_segs) {
+                              const int32_t* segs, int n_blocks, int max_segs, uint16_t* woff) {
   CTX_CHECK("mgpbd_mg_set_level_window");
   if (level < 0 || level >= ctx->n_levels || !seg_ptr || !segs)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: invalid arguments");
   MgLevel& L = ctx->L[level];
-  if (!L.hpk || !L.idx || L.n <= 0)
+  if (!L.idx || L.n <= 0 || (!L.hpk && !woff))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: level needs packed half words "
-                                    "(mgpbd_mg_set_level_half with hpacked)");
+                                    "(mgpbd_mg_set_level_half with hpacked) or woff");
   if (n_blocks != nblocks(64LL * L.n_slices) || max_segs < 1 || max_segs > kWinMaxSegs)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: n_blocks must be ceil(64 n_slices / " +
                                     std::to_string(kBlock) + "), max_segs in [1, " +
@@ -2770,14 +2858,14 @@ int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr,
   int* bad = reinterpret_cast<int*>(ctx->ws.partials + (size_t)3 * kPartialStride);
   MGPBD_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), ctx->stream));
   k_window_offsets<<<n_blocks, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx,
-                                                           seg_ptr, segs, L.hpk, bad);
+                                                           seg_ptr, segs, L.hpk, woff, bad);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_set_level_window");
   int hb = 0;
   MGPBD_HIP_TRY(hipMemcpyAsync(&hb, bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   MGPBD_HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (hb) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: a column lies outside its "
                                           "block's window");
-  L.seg_ptr = seg_ptr; L.segs = segs; L.win_segs = max_segs;
+  L.seg_ptr = seg_ptr; L.segs = segs; L.win_segs = max_segs; L.woff = woff;
   if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
   return 0;
 }
@@ -2847,7 +2935,14 @@ static void pcg_iteration(mgpbd_ctx* ctx, float* x, double* x64, float* r, float
   const int n = L.n;
   hipStream_t s = ctx->stream;
   int pe = prof ? prof_begin(ctx, MGPBD_PROF_SPMV_L0) : -1;
-  if (x64)
+  const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
+  if (L.woff && x64)
+    k_pcg_spmv_dot_w<true><<<gs, kBlock, lds, s>>>(n, L.n_slices, L.slice_ptr, L.woff, L.val,
+                                                   L.seg_ptr, L.segs, p, Ap, ctx->ws, ctx->st);
+  else if (L.woff)
+    k_pcg_spmv_dot_w<false><<<gs, kBlock, lds, s>>>(n, L.n_slices, L.slice_ptr, L.woff, L.val,
+                                                    L.seg_ptr, L.segs, p, Ap, ctx->ws, ctx->st);
+  else if (x64)
     k_pcg_spmv_dot<true><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, p, Ap,
                                                ctx->ws, ctx->st);
   else
@@ -3049,7 +3144,11 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     if (!ctx->verify) break;
     // FP64 residual of the returned x (every solve); a restart from x when a converged one
     // misses the tolerance
-    k_true_resid<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, b, ctx->ws);
+    if (L.woff)
+      k_true_resid_w<<<gs, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
+          n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, x, b, ctx->ws);
+    else
+      k_true_resid<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, b, ctx->ws);
     k_fin_true<<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
     MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(verify)");
     MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));

@@ -231,17 +231,23 @@ class GPUMultigridSolver:
         return seg_ptr, segs, widest
 
     def _setup_half(self):
-        """Reduced-precision copies of every level for the V-cycle (extension)."""
+        """Reduced-precision copies of every level for the V-cycle (extension), and the LDS
+        windows: of every packed level, and of level 0 for the PCG's FP32 kernels."""
         prec = {"fp32": 0, "fp16": 1, "bf16": 2}[self.precision]
         self.ctx("mgpbd_mg_set_precision", prec, int(self.flexible))
         self._half = []
-        if prec == 0:
-            return
         dev = self.device
+        wins = [self._window(L) if (prec != 0 or l == 0) else None
+                for l, L in enumerate(self.levels)]
+        self._win = []
+        if prec == 0:
+            if wins[0] is not None:
+                self._set_window(0, wins[0], hpk=False)
+            return
         for l, L in enumerate(self.levels):
             rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
             delta = L.indices.astype(np.int64) - rows
-            win = self._window(L)
+            win = wins[l]
             use16 = bool(win is not None or len(delta) == 0 or
                          np.abs(delta).max() < self.col16_limit)
             slots = max(L.padded, 1)
@@ -255,12 +261,23 @@ class GPUMultigridSolver:
             self.ctx("mgpbd_mg_set_level_half", l, dptr(hb["hval"]), dptr(hb["hpk"]),
                      dptr(hb["sd"]), dptr(hb["isd"]), dptr(L.diag_pos))
             hb["seg_ptr"] = hb["segs"] = None
-            if win is not None:
-                hb["seg_ptr"] = torch.from_numpy(win[0]).to(dev)
-                hb["segs"] = torch.from_numpy(win[1]).to(dev)
-                self.ctx("mgpbd_mg_set_level_window", l, dptr(hb["seg_ptr"]), dptr(hb["segs"]),
-                         len(win[0]) - 1, win[2])
             self._half.append(hb)
+            if win is not None:
+                self._set_window(l, win, hpk=True)
+
+    def _set_window(self, l, win, hpk):
+        """Uploads level l's window lists and switches its kernels to the windowed ones; level
+        0 also gets the 16-bit offsets of the PCG's FP32 kernels (woff)."""
+        dev = self.device
+        L = self.levels[l]
+        seg_ptr = torch.from_numpy(win[0]).to(dev)
+        segs = torch.from_numpy(win[1]).to(dev)
+        woff = torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev) if l == 0 else None
+        self.ctx("mgpbd_mg_set_level_window", l, dptr(seg_ptr), dptr(segs), len(win[0]) - 1,
+                 win[2], dptr(woff))
+        if hpk:
+            self._half[l]["seg_ptr"], self._half[l]["segs"] = seg_ptr, segs
+        self._win.append({"level": l, "seg_ptr": seg_ptr, "segs": segs, "woff": woff})
 
     @staticmethod
     def _host_refill(fine_vals, cidx, coff):
@@ -325,7 +342,7 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_ctx_device_bytes", ctypes.byref(ctx_bytes))
         return device_bytes(self.levels, self.transfers, self.refill, self._work,
                             self._pcg_work, self._x, getattr(self, "_half", None),
-                            seen=seen) + ctx_bytes.value
+                            getattr(self, "_win", None), seen=seen) + ctx_bytes.value
 
     def pcg_counters(self):
         """(stall rescues, verification restarts) over this solver's solves."""

@@ -352,15 +352,19 @@ int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible);
  * mgpbd_mg_convert_half), diag_pos_dev (SELL position of each row's diagonal). */
 int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, uint32_t* hpacked_dev,
                             float* sd_dev, float* isd_dev, const int32_t* diag_pos_dev);
-/* Switches a packed level to the windowed sweeps (no reference counterpart; same results as
- * the packed sweeps, bit for bit): each workgroup stages its block's window of x in LDS
+/* Switches a level to the windowed kernels (no reference counterpart; same results as the
+ * gathering kernels, bit for bit): each workgroup stages its block's window of x in LDS
  * (seg_ptr_dev / segs_dev from mgpbd_sell_window on the level's CSR, n_blocks blocks, largest
- * window max_segs <= MGPBD_WINDOW_MAX_SEGS segments) and the packed words' high halves become
- * LDS byte offsets (rewritten here; mgpbd_mg_convert_half keeps them). The arrays must stay
- * valid while the level is used; mgpbd_mg_set_level / _set_level_half reset the window.
- * Synchronises the context's stream. */
+ * window max_segs <= MGPBD_WINDOW_MAX_SEGS segments). On a packed level (mgpbd_mg_set_level_half
+ * with hpacked) the words' high halves become LDS byte offsets (rewritten here;
+ * mgpbd_mg_convert_half keeps them) for the reduced-precision sweeps and power steps; with
+ * woff_dev (uint16 per SELL slot, filled here) the level-0 FP32 kernels of the PCG (A p, the
+ * FP64 residual check) read x from the window too. At least one of the two must be present.
+ * The arrays must stay valid while the level is used; mgpbd_mg_set_level / _set_level_half
+ * reset the window. Synchronises the context's stream. */
 int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr_dev,
-                              const int32_t* segs_dev, int n_blocks, int max_segs);
+                              const int32_t* segs_dev, int n_blocks, int max_segs,
+                              uint16_t* woff_dev);
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx);

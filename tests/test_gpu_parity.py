@@ -446,9 +446,13 @@ def test_half_levels_are_scaled_and_rounded(precision):
                 assert np.array_equal(c16, L.indices - rows)
 
 
-def _window_columns(L, hb):
-    """Columns decoded from a windowed level's LDS offsets (CSR order)."""
-    off = ((hb["hpk"].cpu().numpy().view(np.uint32) >> 16)[L.sell_pos_host]).astype(np.int64)
+def _window_columns(L, hb, woff=None):
+    """Columns decoded from a windowed level's LDS offsets (CSR order): the packed words' high
+    halves, or the FP32 kernels' 16-bit offsets."""
+    if woff is not None:
+        off = woff.cpu().numpy().view(np.uint16)[L.sell_pos_host].astype(np.int64)
+    else:
+        off = ((hb["hpk"].cpu().numpy().view(np.uint32) >> 16)[L.sell_pos_host]).astype(np.int64)
     seg_ptr, segs = hb["seg_ptr"].cpu().numpy(), hb["segs"].cpu().numpy()
     rows = np.repeat(np.arange(L.n_rows), np.diff(L.indptr))
     assert np.all(off % 4 == 0)
@@ -481,42 +485,50 @@ def test_half_levels_with_int32_columns_match_packed_offsets(monkeypatch):
     assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "fp32"])
 def test_windowed_levels_match_global_gathers(monkeypatch, precision):
-    """Windowed levels (x staged in LDS per block of 256 rows, words holding LDS offsets) give
-    the packed sweeps' results bit for bit: PCG solves with the Chebyshev smoother (sweeps,
-    residuals, power iteration) on every level agree in iterations and in every bit, against
-    packed column offsets and, with the 16-bit bound forced off, against int32 columns (the
-    windowed levels stay packed, as their words hold no column offsets)."""
+    """Windowed kernels (x staged in LDS per block of 256 rows, LDS offsets in place of column
+    indices) give the gathering kernels' results bit for bit: PCG solves with the Chebyshev
+    smoother (sweeps, residuals, power iteration on every windowed level; the FP32 A p and the
+    FP64 residual check on level 0) agree in iterations and in every bit, against packed column
+    offsets and, with the 16-bit bound forced off, against int32 columns (the windowed levels
+    stay packed, as their words hold no column offsets)."""
     from controllable_pbd_3d_amd.mgpcg_gpu import GPUMultigridSolver
     mesh, s, o = _extended_pair(n=7, seed=11)
     v0 = s.v_p.cpu().numpy()
     kw = dict(mu_per_tet=o.mu_per_tet, hierarchy="extended", galerkin="structural",
               max_coarse=64, coarse_solver="cholesky", precision=precision, relative_tol=True,
               tol_cg=1e-6, smoother="chebyshev", n_smooth=3, cheb_ratio=30.0,
-              smoother_safety=1.2)
+              smoother_safety=1.2, verify_residual=1)
     sw = _solver(mesh, v0, **kw)
     mg = sw._gpu_solver
+    w0 = mg._win[0]
+    assert w0["level"] == 0 and w0["woff"] is not None
+    assert np.array_equal(_window_columns(mg.levels[0], w0, w0["woff"]), mg.levels[0].indices)
     windowed = [hb["segs"] is not None for hb in mg._half]
-    assert windowed[0] and all(hb["hpk"] is not None for hb in mg._half)
-    for L, hb in zip(mg.levels, mg._half):
-        if hb["segs"] is not None:
-            assert np.array_equal(_window_columns(L, hb), L.indices)
+    if precision != "fp32":
+        assert windowed[0] and all(hb["hpk"] is not None for hb in mg._half)
+        for L, hb in zip(mg.levels, mg._half):
+            if hb["segs"] is not None:
+                assert np.array_equal(_window_columns(L, hb), L.indices)
     monkeypatch.setattr(GPUMultigridSolver, "window_segs_limit", 0)
-    sp_ = _solver(mesh, v0, **kw)
-    assert all(hb["segs"] is None and hb["hpk"] is not None for hb in sp_._gpu_solver._half)
-    monkeypatch.setattr(GPUMultigridSolver, "col16_limit", 0)
-    si = _solver(mesh, v0, **kw)
-    assert all(hb["hval"] is not None for hb in si._gpu_solver._half)
+    runs = [sw, _solver(mesh, v0, **kw)]
+    assert not runs[1]._gpu_solver._win
+    assert all(hb["segs"] is None and hb["hpk"] is not None for hb in runs[1]._gpu_solver._half)
+    if precision != "fp32":
+        monkeypatch.setattr(GPUMultigridSolver, "col16_limit", 0)
+        runs.append(_solver(mesh, v0, **kw))
+        assert all(hb["hval"] is not None for hb in runs[2]._gpu_solver._half)
     out = []
-    for z in (sw, sp_, si):
+    for z in runs:
         z._compute_C_and_gradC()
         it = z._gpu_solver.solve(z._b_gpu, z._dlam_gpu)
-        out.append((it, z._dlam_gpu.cpu().numpy()))
-    print(f"[window] {precision}: levels windowed {windowed}, iterations {[r[0] for r in out]}",
-          flush=True)
+        out.append((it, z._dlam_gpu.cpu().numpy(), z._gpu_solver.last_info.get("rr_true")))
+    print(f"[window] {precision}: levels windowed {windowed}, iterations {[r[0] for r in out]}, "
+          f"FP64 rr {[r[2] for r in out]}", flush=True)
     for r in out[1:]:
         assert out[0][0] == r[0] and np.array_equal(out[0][1].view(np.uint32), r[1].view(np.uint32))
+        assert out[0][2] == r[2]
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
