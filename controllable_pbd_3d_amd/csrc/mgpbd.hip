@@ -1675,6 +1675,40 @@ k_half_values(int n, int n_slices, const int* __restrict__ slice_ptr,
   }
 }
 
+// k_half_values on a windowed level with FP32 offsets (level 0): isd staged through the
+// block's window, each word built from the FP32 value and the slot's LDS offset, so per slot
+// 4 + 2 bytes are read and 4 written (no column indices, no old words, no isd gathers); the
+// words equal k_half_values' bit for bit (same product order; padding rows: offset 0, value 0).
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_half_values_w(int n, int n_slices, const int* __restrict__ slice_ptr,
+                const uint16_t* __restrict__ woff, const float* __restrict__ val,
+                const int* __restrict__ seg_ptr, const int* __restrict__ segs,
+                const float* __restrict__ isd, uint32_t* __restrict__ hpk) {
+  extern __shared__ float xs[];
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, isd, n);
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const int p0 = sell_base(slice_ptr, i);
+  const int w = sell_width(slice_ptr, i);
+  const bool row = i < n;
+  const float si = row ? isd[i] : 0.0f;
+  const char* xl = reinterpret_cast<const char*>(xs);
+#pragma unroll 4
+  for (int k0 = 0; k0 < w; k0 += 4) {
+    const int p = p0 + 64 * k0;
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(val + p));
+    const v4u16 o = __builtin_nontemporal_load(reinterpret_cast<const v4u16*>(woff + p));
+    v4u32 e;
+    e.x = ((uint32_t)o.x << 16) | f2h<VT>(row ? v.x * si * *reinterpret_cast<const float*>(xl + o.x) : 0.0f);
+    e.y = ((uint32_t)o.y << 16) | f2h<VT>(row ? v.y * si * *reinterpret_cast<const float*>(xl + o.y) : 0.0f);
+    e.z = ((uint32_t)o.z << 16) | f2h<VT>(row ? v.z * si * *reinterpret_cast<const float*>(xl + o.z) : 0.0f);
+    e.w = ((uint32_t)o.w << 16) | f2h<VT>(row ? v.w * si * *reinterpret_cast<const float*>(xl + o.w) : 0.0f);
+    __builtin_nontemporal_store(e, reinterpret_cast<v4u32*>(hpk + p));
+  }
+}
+
 // x^0 = omega isd r (first sweep on the scaled level 0) and b^0 = isd r
 __global__ void __launch_bounds__(kBlock)
 k_half_first(int n, const float* __restrict__ isd, const float* __restrict__ r,
@@ -2879,7 +2913,14 @@ int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
     if (!L.hval && !L.hpk) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
     k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
     const int g = nblocks(64LL * L.n_slices);
-    if (ctx->precision == MGPBD_PREC_FP16)
+    const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
+    if (L.woff && L.hpk && L.segs && ctx->precision == MGPBD_PREC_FP16)
+      k_half_values_w<1><<<g, kBlock, lds, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.woff,
+                                                         L.val, L.seg_ptr, L.segs, L.isd, L.hpk);
+    else if (L.woff && L.hpk && L.segs)
+      k_half_values_w<2><<<g, kBlock, lds, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.woff,
+                                                         L.val, L.seg_ptr, L.segs, L.isd, L.hpk);
+    else if (ctx->precision == MGPBD_PREC_FP16)
       k_half_values<1><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.isd, L.hval, L.hpk,
                                                         L.segs != nullptr);
     else
