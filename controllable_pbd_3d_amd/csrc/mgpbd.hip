@@ -1595,6 +1595,68 @@ k_pcg_spmv_dot_w(int n, int n_slices, const int* __restrict__ slice_ptr,
   if (threadIdx.x == 0) ws.partials[blk] = t;
 }
 
+// Windowed k_sell (same MODE semantics and arithmetic, FP32 values beside LDS offsets)
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_sellw_f32(int n, int n_slices, const int* __restrict__ slice_ptr,
+            const uint16_t* __restrict__ woff, const float* __restrict__ val,
+            const int* __restrict__ seg_ptr, const int* __restrict__ segs,
+            const float* __restrict__ x, float* y, const float* __restrict__ dinv,
+            const float* __restrict__ b, float omega, const float* __restrict__ wp,
+            const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
+  extern __shared__ float xs[];
+  if (wp) omega = *wp;
+  if (done && *done) return;
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, x, n);
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) >= n_slices) return;
+  const float ax = (float)sellw_row_f32<false>(i, slice_ptr, woff, val, xs);
+  if (i >= n) return;
+  if (MODE == 0) y[i] = ax;
+  else if (MODE == 1) y[i] = fmaf(omega * dinv[i], b[i] - ax, x[i]);
+  else if (MODE == 2) y[i] = b[i] - ax;
+  else {
+    const float xi = x[i];
+    const float u = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
+    y[i] = fmaf(cc[1] * dinv[i], b[i] - ax, u);
+  }
+}
+
+// Windowed k_power_step (partials at the row block's slot)
+__global__ void __launch_bounds__(kBlock)
+k_power_step_wf(int n, int n_slices, const int* __restrict__ slice_ptr,
+                const uint16_t* __restrict__ woff, const float* __restrict__ val,
+                const int* __restrict__ seg_ptr, const int* __restrict__ segs,
+                const float* __restrict__ dinv, const float* __restrict__ v,
+                float* __restrict__ w, RedWS ws) {
+  extern __shared__ float xs[];
+  const int blk = xcd_row_block(blockIdx.x, gridDim.x);
+  stage_window(xs, seg_ptr, segs, blk, v, n);
+  double vav = 0.0, vdv = 0.0, wdw = 0.0;
+  const int i = blk * kBlock + threadIdx.x;
+  if ((i >> 6) < n_slices) {
+    const float ax = (float)sellw_row_f32<false>(i, slice_ptr, woff, val, xs);
+    if (i < n) {
+      const float di = dinv[i];
+      w[i] = di * ax;
+      vav = (double)v[i] * ax;
+      if (di > 0.0f) {
+        vdv = (double)v[i] * v[i] / di;
+        wdw = (double)ax * ax * di;
+      }
+    }
+  }
+  const double s0 = block_sum(vav);
+  const double s1 = block_sum(vdv);
+  const double s2 = block_sum(wdw);
+  if (threadIdx.x == 0) {
+    ws.partials[blk] = s0;
+    ws.partials[(size_t)kPartialStride + blk] = s1;
+    ws.partials[(size_t)2 * kPartialStride + blk] = s2;
+  }
+}
+
 // Windowed k_true_resid (partials at the row block's slot)
 __global__ void __launch_bounds__(kBlock)
 k_true_resid_w(int n, int n_slices, const int* __restrict__ slice_ptr,
@@ -2375,6 +2437,23 @@ static int check_ready(mgpbd_ctx* ctx) {
   return 0;
 }
 
+// FP32 row kernels of a level: windowed when the level has FP32 LDS offsets
+extern "C++" {
+template <int MODE>
+static void sell_launch(mgpbd_ctx* ctx, const MgLevel& L, const float* x, float* y,
+                        const float* dinv, const float* b, float omega, const float* wp,
+                        const int* done, const float* xp = nullptr, const float* cc = nullptr) {
+  const int g = nblocks(64LL * L.n_slices);
+  if (L.woff)
+    k_sellw_f32<MODE><<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, ctx->stream>>>(
+        L.n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, x, y, dinv, b, omega, wp,
+        done, xp, cc);
+  else
+    k_sell<MODE><<<g, kBlock, 0, ctx->stream>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, x, y,
+                                               dinv, b, omega, wp, done, xp, cc);
+}
+}  // extern "C++"
+
 // Jacobi sweeps on level l from the first-sweep result already in *cur; returns the buffer
 // holding the result (cur or the level's other x buffer).
 // `count` smoothing steps on level l from the iterate in `cur` (one of the level's two x
@@ -2408,13 +2487,9 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
     if (cheb && step >= 2) {
       const float* xp = (step == 2 && from_zero) ? nullptr : nxt;
-      k_sell<3><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
-          L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, nullptr,
-          done, xp, cl(ctx, l, step));
+      sell_launch<3>(ctx, L, cur, nxt, L.dinv, b, ctx->omega, nullptr, done, xp, cl(ctx, l, step));
     } else {
-      k_sell<1><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
-          L.n, L.n_slices, L.slice_ptr, L.idx, L.val, cur, nxt, L.dinv, b, ctx->omega, wl(ctx, l),
-          done);
+      sell_launch<1>(ctx, L, cur, nxt, L.dinv, b, ctx->omega, wl(ctx, l), done);
     }
     prof_end(ctx, pe);
     cur = nxt;
@@ -2543,8 +2618,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
   // residual r = b - A x (x == 0 when n_smooth == 0: then r = b)
   if (ctx->n_smooth > 0) {
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_RESID_L0 : MGPBD_PROF_COARSE);
-    k_sell<2><<<nblocks(64LL * L.n_slices), kBlock, 0, ctx->stream>>>(
-        L.n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr, b, 0.f, nullptr, done);
+    sell_launch<2>(ctx, L, x, r, nullptr, b, 0.f, nullptr, done);
     prof_end(ctx, pe);
   } else {
     k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, b, r);
@@ -2565,8 +2639,7 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
     float* save = C.wb(5);
     float* rc = C.wb(4);
     k_copy_d<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, xcr, save, done);
-    k_sell<2><<<nblocks(64LL * C.n_slices), kBlock, 0, ctx->stream>>>(
-        C.n, C.n_slices, C.slice_ptr, C.idx, C.val, xcr, rc, nullptr, bc, 0.f, nullptr, done);
+    sell_launch<2>(ctx, C, xcr, rc, nullptr, bc, 0.f, nullptr, done);
     float* x2 = C.wb(1);
     if (ctx->n_smooth > 0)
       k_first_sweep<<<nblocks(C.n), kBlock, 0, ctx->stream>>>(C.n, C.dinv, rc, x2, ctx->omega,
@@ -2782,7 +2855,10 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     const int g = nblocks(64LL * L.n_slices);
     double* pw = ctx->pw_dev + 4 * l;
     for (int it = 0; it < iters; ++it) {
-      if (ctx->precision == MGPBD_PREC_FP32)
+      if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
+        k_power_step_wf<<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
+            L.n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, L.dinv, v, w, ctx->ws);
+      else if (ctx->precision == MGPBD_PREC_FP32)
         k_power_step<<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.val, L.dinv, v,
                                           w, ctx->ws);
       else if (L.segs && ctx->precision == MGPBD_PREC_FP16)
@@ -3093,11 +3169,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       if (x64 && (restarts > 0 || rescued))
         k_resid64<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x64, b, r);
       else
-        k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, r, nullptr,
-                                        b, 0.f, nullptr, nullptr);
+        sell_launch<2>(ctx, L, x, r, nullptr, b, 0.f, nullptr, nullptr);
       if (alt)
-        k_sell<2><<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, alt, Ap,
-                                        nullptr, b, 0.f, nullptr, nullptr);
+        sell_launch<2>(ctx, L, alt, Ap, nullptr, b, 0.f, nullptr, nullptr);
       k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, alt ? Ap : nullptr, ctx->ws);
       k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, alt != nullptr, ctx->st);
       k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, alt, Ap, L.dinv,

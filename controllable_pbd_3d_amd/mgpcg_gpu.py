@@ -237,12 +237,12 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_mg_set_precision", prec, int(self.flexible))
         self._half = []
         dev = self.device
-        wins = [self._window(L) if (prec != 0 or l == 0) else None
-                for l, L in enumerate(self.levels)]
+        wins = [self._window(L) for L in self.levels]
         self._win = []
         if prec == 0:
-            if wins[0] is not None:
-                self._set_window(0, wins[0], hpk=False)
+            for l, win in enumerate(wins):
+                if win is not None:
+                    self._set_window(l, win, hpk=False)
             return
         for l, L in enumerate(self.levels):
             rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
@@ -266,13 +266,15 @@ class GPUMultigridSolver:
                 self._set_window(l, win, hpk=True)
 
     def _set_window(self, l, win, hpk):
-        """Uploads level l's window lists and switches its kernels to the windowed ones; level
-        0 also gets the 16-bit offsets of the PCG's FP32 kernels (woff)."""
+        """Uploads level l's window lists and switches its kernels to the windowed ones. The
+        16-bit offsets of the FP32 kernels (woff) go to level 0, whose FP32 matrix the PCG
+        multiplies, and to every level of an FP32 V-cycle."""
         dev = self.device
         L = self.levels[l]
         seg_ptr = torch.from_numpy(win[0]).to(dev)
         segs = torch.from_numpy(win[1]).to(dev)
-        woff = torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev) if l == 0 else None
+        fp32 = l == 0 or not hpk
+        woff = torch.zeros(max(L.padded, 1), dtype=torch.int16, device=dev) if fp32 else None
         self.ctx("mgpbd_mg_set_level_window", l, dptr(seg_ptr), dptr(segs), len(win[0]) - 1,
                  win[2], dptr(woff))
         if hpk:

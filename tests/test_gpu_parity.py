@@ -506,7 +506,12 @@ def test_windowed_levels_match_global_gathers(monkeypatch, precision):
     assert w0["level"] == 0 and w0["woff"] is not None
     assert np.array_equal(_window_columns(mg.levels[0], w0, w0["woff"]), mg.levels[0].indices)
     windowed = [hb["segs"] is not None for hb in mg._half]
-    if precision != "fp32":
+    if precision == "fp32":   # every level of an FP32 V-cycle reads x from its windows
+        assert [w["level"] for w in mg._win] == list(range(len(mg.levels)))
+        for w in mg._win:
+            L = mg.levels[w["level"]]
+            assert np.array_equal(_window_columns(L, w, w["woff"]), L.indices)
+    else:
         assert windowed[0] and all(hb["hpk"] is not None for hb in mg._half)
         for L, hb in zip(mg.levels, mg._half):
             if hb["segs"] is not None:
