@@ -1966,13 +1966,15 @@ struct mgpbd_ctx {
   float rescue_boost = 1.25f;
   double stall_ratio = 0.25;   // ||r||^2 ratio over 8 iterations counted as a stall
   int64_t n_rescues = 0, n_restarts = 0;
-  // HIP graph of `poll` PCG iterations (+ the state copy), replayed on `work` stream
+  // HIP graphs of k PCG iterations (+ the state copy), k = 1 .. kGraphMax, captured when a
+  // replay of that length is first needed for the same buffers, replayed on `work` stream
   bool use_graph = true;
   hipStream_t work = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraphExec_t graph = nullptr;
+  static constexpr int kGraphMax = 64;
+  hipGraphExec_t graphs[kGraphMax + 1] = {};
   const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
-  int graph_chunk = 0;
+  int pred_iters = 0;   // iterations of the previous solve (length of the next first replay)
   // optional per-kernel timing with HIP events on the context stream
   bool prof = false;
   std::vector<hipEvent_t> ev;     // 2 per recorded launch
@@ -1981,6 +1983,12 @@ struct mgpbd_ctx {
   double prof_ms[MGPBD_PROF_KINDS] = {};
   int64_t prof_cnt[MGPBD_PROF_KINDS] = {};
 };
+
+// Every captured iteration graph is dropped (a level, a setting or a buffer changed).
+static void drop_graphs(mgpbd_ctx* c) {
+  for (hipGraphExec_t& g : c->graphs)
+    if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+}
 
 // Device pointer of level l's smoothing weight, or nullptr for the fixed host weight.
 // smoothing steps per phase on level l
@@ -2076,7 +2084,7 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (!c) return;
   (void)hipStreamSynchronize(c->stream);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
-  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  drop_graphs(c);
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->work) (void)hipStreamDestroy(c->work);
@@ -2376,7 +2384,7 @@ int mgpbd_mg_configure(mgpbd_ctx* ctx, int n_levels, int n_smooth, float omega, 
       (coarse_mode != MGPBD_COARSE_JACOBI && coarse_mode != MGPBD_COARSE_CHOLESKY))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_configure: invalid arguments");
   ctx->dense_ready = false;
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   ctx->n_levels = n_levels;
   ctx->n_smooth = n_smooth;
   for (int l = 0; l < MGPBD_MAX_LEVELS; ++l) ctx->level_smooth[l] = 0;
@@ -2405,7 +2413,7 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level: coarsest level too large for the dense "
                                     "solve (" + std::to_string(n_rows) + " > " +
                                     std::to_string(MGPBD_MAX_DENSE) + ")");
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   MgLevel& L = ctx->L[level];
   L.n = n_rows;
   L.n_slices = (n_rows + 63) / 64;
@@ -2783,7 +2791,7 @@ int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps) {
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_smoothing: level in range, steps in "
                                     "[0, 8], n_smooth >= 1");
   ctx->level_smooth[level] = steps;
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   return 0;
 }
 
@@ -2793,7 +2801,7 @@ int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect) {
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_cycle: cycle 1 (V) or 2 (W), overcorrect in (0, 4)");
   ctx->cycle = cycle;
   ctx->overcorrect = overcorrect;
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   return 0;
 }
 
@@ -2814,7 +2822,7 @@ int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, 
     ctx->pv_ready[l] = false;
     k_set_float<<<1, 64, 0, ctx->stream>>>(ctx->omega_dev + l, ctx->omega);
   }
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_set_smoother");
   return 0;
 }
@@ -2936,7 +2944,7 @@ int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible) {
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_precision: the reduced-precision V-cycle needs n_smooth >= 1");
   ctx->precision = precision;
   ctx->flexible = flexible != 0;
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   return 0;
 }
 
@@ -2948,7 +2956,7 @@ int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, uint32_t*
   MgLevel& L = ctx->L[level];
   L.hval = hpacked ? nullptr : hval; L.hpk = hpacked; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
   L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0; L.woff = nullptr;
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   return 0;
 }
 
@@ -2976,7 +2984,7 @@ int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr,
   if (hb) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_window: a column lies outside its "
                                           "block's window");
   L.seg_ptr = seg_ptr; L.segs = segs; L.win_segs = max_segs; L.woff = woff;
-  if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+  drop_graphs(ctx);
   return 0;
 }
 
@@ -3087,10 +3095,6 @@ int mgpbd_ctx_use_graphs(mgpbd_ctx* ctx, int enable, int poll) {
   CTX_CHECK("mgpbd_ctx_use_graphs");
   if (poll < 1 || poll > 64) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_use_graphs: poll in [1, 64]");
   ctx->use_graph = enable != 0;
-  if (ctx->poll != poll && ctx->graph) {
-    (void)hipGraphExecDestroy(ctx->graph);
-    ctx->graph = nullptr;
-  }
   ctx->poll = poll;
   return 0;
 }
@@ -3130,27 +3134,35 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
   } restore{ctx, caller};
   hipStream_t s = ctx->stream;
 
-  // capture (or reuse) the iteration graph before any work is queued on the stream
+  // Replay lengths: a round's first replay runs as many iterations as the previous solve
+  // took, the following ones single iterations up to one past that count, then `poll` at a
+  // time. Iterations past convergence are skipped on the device's done flag and change
+  // nothing, so the schedule only trades host polls against launches that do no work.
+  const int first_len = std::max(1, std::min(ctx->pred_iters > 0 ? ctx->pred_iters : ctx->poll,
+                                             (int)mgpbd_ctx::kGraphMax));
+  const int pred_end = ctx->pred_iters > 0 ? ctx->pred_iters + 1 : 0;
+  // capture (or reuse) the graphs of those lengths before any work is queued on the stream
   if (graph_mode) {
     const void* key[4] = {b, x, work, L.work};
-    bool stale = !ctx->graph || ctx->graph_chunk != ctx->poll ||
-                 std::memcmp(key, ctx->graph_key, sizeof(key)) != 0;
-    if (stale) {
-      if (ctx->graph) { (void)hipGraphExecDestroy(ctx->graph); ctx->graph = nullptr; }
+    if (std::memcmp(key, ctx->graph_key, sizeof(key)) != 0) {
+      drop_graphs(ctx);
+      std::memcpy(ctx->graph_key, key, sizeof(key));
+    }
+    const int lens[3] = {first_len, 1, ctx->poll};
+    for (int len : lens) {
+      if (ctx->graphs[len]) continue;
       hipGraph_t g = nullptr;
       MGPBD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      for (int c = 0; c < ctx->poll; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, false);
+      for (int c = 0; c < len; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, false);
       (void)hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s);
       hipError_t ce = hipStreamEndCapture(s, &g);
       if (ce != hipSuccess) return mgpbd_fail(MGPBD_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
-      ce = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+      ce = hipGraphInstantiate(&ctx->graphs[len], g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       if (ce != hipSuccess) {
-        ctx->graph = nullptr;
+        ctx->graphs[len] = nullptr;
         return mgpbd_fail(MGPBD_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ce));
       }
-      std::memcpy(ctx->graph_key, key, sizeof(key));
-      ctx->graph_chunk = ctx->poll;
     }
   }
 
@@ -3198,14 +3210,16 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     int n_poll = 0;
     bool stalled = false;
     while (issued < maxiter) {
-      if (graph_mode && maxiter - issued >= ctx->poll) {
-        MGPBD_HIP_TRY(hipGraphLaunch(ctx->graph, s));
-        issued += ctx->poll;
-        round_issued += ctx->poll;
+      const int want = std::min(round_issued == 0 ? first_len
+                                : round_issued < pred_end ? 1 : ctx->poll, maxiter - issued);
+      if (graph_mode && ctx->graphs[want]) {
+        MGPBD_HIP_TRY(hipGraphLaunch(ctx->graphs[want], s));
+        issued += want;
+        round_issued += want;
       } else {
         // profiling times every launch: one iteration per host check, so no launch past
         // convergence (skipped by the done flag) dilutes the per-kernel averages
-        int chunk = std::min(ctx->prof ? 1 : ctx->poll, maxiter - issued);
+        int chunk = ctx->prof ? 1 : want;
         for (int c = 0; c < chunk; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, true);
         MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(iteration)");
         issued += chunk;
@@ -3247,6 +3261,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       first_info[3] = h.rr_alt;
     }
     total_iters += stalled ? round_issued : ctx->st_host->iters;
+    if (!stalled && restarts == 0 && !rescued) ctx->pred_iters = std::max(1, ctx->st_host->iters);
     if (stalled && issued < maxiter) {
       // the V-cycle no longer contracts: the spectral estimates lag the matrix
       rescued = true;
@@ -3314,10 +3329,7 @@ int mgpbd_pcg_counters(mgpbd_ctx* ctx, int64_t* rescues, int64_t* restarts) {
 int mgpbd_pcg_set_fp64_iterate(mgpbd_ctx* ctx, int enable) {
   CTX_CHECK("mgpbd_pcg_set_fp64_iterate");
   const bool on = enable != 0;
-  if (on != ctx->fp64_iterate && ctx->graph) {   // the captured iteration changes
-    (void)hipGraphExecDestroy(ctx->graph);
-    ctx->graph = nullptr;
-  }
+  if (on != ctx->fp64_iterate) drop_graphs(ctx);   // the captured iteration changes
   ctx->fp64_iterate = on;
   return 0;
 }

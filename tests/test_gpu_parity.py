@@ -384,23 +384,24 @@ def test_extended_levels_and_cholesky_vcycle():
 @pytest.mark.parametrize("coarse_solver", ["jacobi", "cholesky"])
 def test_graph_replay_equals_eager(coarse_solver):
     """The PCG replayed as HIP graphs gives the same bits as eager launches, over several
-    solves with eager work in between (each solve replays the graph several times)."""
+    solves with eager work in between (each solve replays graphs of several lengths: the
+    first replay follows the previous solve's iteration count, then single iterations, then
+    `poll` at a time), for several poll lengths."""
     mesh, s, o = _extended_pair(coarse_solver=coarse_solver)
     mg = s._gpu_solver
     ctx = s.ctx
-    results = {}
-    for mode in (0, 1, 0, 1):
-        ctx("mgpbd_ctx_use_graphs", mode, 4)
+    results = []
+    for mode, poll in ((0, 4), (1, 4), (0, 4), (1, 4), (1, 1), (1, 7), (1, 2)):
+        ctx("mgpbd_ctx_use_graphs", mode, poll)
         s._compute_C_and_gradC()
         s._fill_A_and_update_gpu()
         it = mg.solve(s._b_gpu, s._dlam_gpu)
         torch.cuda.synchronize()
-        x = s._dlam_gpu.cpu().numpy().copy()
-        if mode in results:
-            assert results[mode][0] == it and np.array_equal(results[mode][1], x)
-        results[mode] = (it, x)
-    assert results[0][0] == results[1][0] > 4
-    assert np.array_equal(results[0][1].view(np.uint32), results[1][1].view(np.uint32))
+        results.append((it, s._dlam_gpu.cpu().numpy().copy()))
+    it0, x0 = results[0]
+    assert it0 > 4
+    for it, x in results[1:]:
+        assert it == it0 and np.array_equal(x.view(np.uint32), x0.view(np.uint32))
     ctx("mgpbd_ctx_use_graphs", 1, 4)
 
 
