@@ -106,6 +106,20 @@ def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
     raise ValueError(kind)
 
 
+def factored_bytes(n_tets, n_verts, kind, chebyshev=False):
+    """Algorithmic HBM bytes of one application of the factored level 0 (vertex pass + tet
+    pass), each array counted once per pass: vertex pass = 4 incidence slots per tet x (4-byte
+    incidence + 8-byte K) + x (4 n) + u written (16 per vertex); tet pass = t_i (16) + K (24)
+    + a^ (4) + x (4) + b (4) + y (4) per tet + u read (16 per vertex), + the previous iterate
+    (4 n) on average with the Chebyshev smoother (as for the stored levels). The residual pass
+    reads and writes the same arrays."""
+    if kind not in ("jacobi", "resid"):
+        raise ValueError(kind)
+    vertex_pass = 48 * n_tets + 4 * n_tets + 16 * n_verts
+    tet_pass = 56 * n_tets + 16 * n_verts
+    return vertex_pass + tet_pass + (4 * n_tets if chebyshev and kind == "jacobi" else 0)
+
+
 def cpu_baseline(mesh, mu_t, solver, newton_per_substep, pcg_per_substep, pcg_sample=4):
     """CPU baseline: the oracle's C restatement of the reference solver (FP32 arithmetic, one
     thread) on the same mesh, material and solver configuration (the same five-level
@@ -170,7 +184,7 @@ def measured_traffic(workload, kind, precision):
     return None
 
 
-def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8):
+def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factored_nv=None):
     from controllable_pbd_3d_amd import _lib
     ms = np.zeros(8, np.float64)
     cnt = np.zeros(8, np.int64)
@@ -183,7 +197,8 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8):
     for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
-            by = kernel_bytes(n, nnz, kind, per, chebyshev)
+            by = factored_bytes(n, factored_nv, kind, chebyshev) if factored_nv and \
+                kind != "spmv" else kernel_bytes(n, nnz, kind, per, chebyshev)
             kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
                           "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
     for kind, slot in (("fill_A", 3), ("constraints", 4), ("refill", 5), ("coarse", 6)):
@@ -224,14 +239,14 @@ def make_solver(mesh, mu_t, cfg, kw, model=None, record=None):
     return s
 
 
-def train_learned(cfg, seeds, substeps, epochs):
+def train_learned(cfg, seeds, substeps, epochs, level0="factored"):
     """Warm-start network trained on first-Newton-iteration pairs the solver produces on
     other seeds of the same workload (mesh jitter and per-tet E differ), A-norm loss."""
     import torch
     from controllable_pbd_3d_amd.warm_start import LearnedWarmStart
     t0 = time.perf_counter()
     samples, model = [], None
-    kw = dict(RUNS["fp16_previous"])
+    kw = dict(RUNS["fp16_previous"], level0=level0)
     for seed in seeds:
         mesh, mu_t = build_workload(cfg, seed)
         if model is None:
@@ -258,11 +273,14 @@ def train_learned(cfg, seeds, substeps, epochs):
     return model, info
 
 
-def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, seed=0):
+def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, seed=0,
+               level0="factored"):
     import torch
     from controllable_pbd_3d_amd.runtime import get_context
     ctx = get_context(0)
-    kw = RUNS[name]
+    kw = dict(RUNS[name])
+    if kw["precision"] != "fp32":
+        kw["level0"] = level0
     t0 = time.perf_counter()
     s = make_solver(mesh, mu_t, cfg, kw, model=model if kw.get("warm_start") == "learned" else None)
     torch.cuda.synchronize()
@@ -315,9 +333,10 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     cheb = kw.get("smoother") == "chebyshev"
     win0 = any(w["level"] == 0 and w["woff"] is not None for w in s._gpu_solver._win)
     per = 6 if win0 else 8
-    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"], cheb, per)
+    fnv = s.n_verts if kw.get("level0") == "factored" else None
+    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"], cheb, per, fnv)
     kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"], cheb,
-                                               per).items() if k not in kern})
+                                               per, fnv).items() if k not in kern})
     ctx("mgpbd_ctx_profile", 0)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 0)
     res["kernels"] = kern
@@ -338,6 +357,9 @@ def main():
     ap.add_argument("--runs", default="all",
                     help="comma list of " + ",".join(RUNS) + " (default: all)")
     ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--level0", default="factored", choices=("factored", "assembled"),
+                    help="level 0 of the reduced-precision V-cycles: factored K K^T + diag "
+                         "from the gradients, or the stored half values")
     ap.add_argument("--train-seeds", default="1,2")
     ap.add_argument("--train-substeps", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=20)
@@ -361,11 +383,13 @@ def main():
     model, learn_info = None, None
     if "fp16_learned" in names:
         model, learn_info = train_learned(cfg, [int(x) for x in args.train_seeds.split(",")],
-                                          args.train_substeps, args.train_epochs)
+                                          args.train_substeps, args.train_epochs, args.level0)
     runs, keep, pos = {}, None, {}
     for name in names:
         s, res = run_config(mesh, mu_t, cfg, name, args.steps, args.warmup, args.profile_steps,
-                            model, args.seed)
+                            model, args.seed, args.level0)
+        if RUNS[name]["precision"] != "fp32":
+            res["level0"] = args.level0
         runs[name] = res
         pos[name] = s.v_p.detach().cpu().numpy().astype(np.float64)
         if keep is None:
@@ -412,7 +436,10 @@ def main():
                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": k["GBs"] / HBM_PEAK_GBS,
                            "algorithmic_bytes": k["bytes"],
-                           "traffic": measured_traffic(cfg["label"], dom, prec)}
+                           "traffic": measured_traffic(
+                               cfg["label"], dom,
+                               prec + ("-factored" if prec != "fp32" and
+                                       args.level0 == "factored" else ""))}
     if args.cpu_baseline:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         ref = runs.get("fp32_reference", best)

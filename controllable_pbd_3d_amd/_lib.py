@@ -96,6 +96,8 @@ SIGNATURES = {
     "mgpbd_mg_set_level_half": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
     "mgpbd_mg_set_level_window": (c_int, [_V, c_int, _V, _V, c_int, c_int, _V]),
     "mgpbd_mg_convert_half": (c_int, [_V]),
+    "mgpbd_mg_set_level0_factored": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_mg_level_residual": (c_int, [_V, c_int, _V, _V, _V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_mg_set_cycle": (c_int, [_V, c_int, c_float]),
     "mgpbd_mg_set_level_smoothing": (c_int, [_V, c_int, c_int]),

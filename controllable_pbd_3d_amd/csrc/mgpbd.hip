@@ -1877,6 +1877,191 @@ k_pcg_p_flex(int n, const float* __restrict__ z, float* __restrict__ p, float* _
   zo[i] = zi;
 }
 
+// ------------------------------------------------------------------------------------------
+// Factored level 0 of the reduced-precision V-cycle. The level-0 matrix is
+// A = J W J^T + diag(alpha~): row t of J holds tet t's vertex gradients g_tk at its vertices'
+// columns, W = diag(inverse masses). Its scaled form D^-1/2 A D^-1/2 is therefore
+// K K^T + diag(a^) with K_tk = sqrt(w_v) g_tk / sqrt(A_tt) (|K_tk| <= 1, so FP16 keeps ~11
+// bits of every component) and a^_t = alpha~_t / A_tt. y = A^ x takes two passes:
+//   per vertex  u_v = sum over the tets t holding v of K_tk x_t      (k_fac_gather)
+//   per tet     y_t = a^_t x_t + sum_k K_tk . u_v(t,k)                (k_fac_apply)
+// The vertex pass reads a vertex-sliced incidence list (64 vertices per slice, slot k of
+// vertex 64s+l at vs_ptr[s] + 64k + l, vs_inc = 4 tet + local vertex) with K beside it (four
+// halves per slot, the fourth zero); the tet pass reads K in tet order (12 halves per tet in
+// three planes of four halves). Both copies are rounded from the same FP32 product, so the
+// operator is exactly symmetric. Per application this streams ~12 bytes per incidence slot
+// and ~44 per tet (~120 MB at 1M tets) instead of ~4 bytes per stored entry of the assembled
+// half level (~307 MB), and the per-Newton-iteration conversion reads the gradients instead of
+// the 72.5M FP32 entries.
+// ------------------------------------------------------------------------------------------
+typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
+
+template <int VT>
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {
+  return (uint32_t)f2h<VT>(a) | ((uint32_t)f2h<VT>(b) << 16);
+}
+template <int VT>
+__device__ __forceinline__ float lo_h(uint32_t w) { return h2f<VT>((unsigned short)w); }
+template <int VT>
+__device__ __forceinline__ float hi_h(uint32_t w) { return h2f<VT>((unsigned short)(w >> 16)); }
+
+// K in tet order (planes tk[0..n), tk[n..2n), tk[2n..3n)) and a^ from the current gradients
+// and the level-0 scales; k_fac_vert writes the same K bits in vertex-slot order.
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_tet(int n, const int4* __restrict__ t_i, const float* __restrict__ invm,
+          const float* __restrict__ gradC, const float* __restrict__ alpha,
+          const float* __restrict__ isd, v2u32* __restrict__ tk, float* __restrict__ ahat) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= n) return;
+  const int4 vi = t_i[t];
+  const float is = isd[t];
+  const float4* g4 = reinterpret_cast<const float4*>(gradC + 12 * (size_t)t);
+  const float4 a = g4[0], b = g4[1], c = g4[2];
+  // g_0 = (a.x a.y a.z), g_1 = (a.w b.x b.y), g_2 = (b.z b.w c.x), g_3 = (c.y c.z c.w)
+  const float s0 = sqrtf(invm[vi.x]) * is, s1 = sqrtf(invm[vi.y]) * is;
+  const float s2 = sqrtf(invm[vi.z]) * is, s3 = sqrtf(invm[vi.w]) * is;
+  tk[t] = v2u32{pack_h2<VT>(s0 * a.x, s0 * a.y), pack_h2<VT>(s0 * a.z, s1 * a.w)};
+  tk[(size_t)n + t] = v2u32{pack_h2<VT>(s1 * b.x, s1 * b.y), pack_h2<VT>(s2 * b.z, s2 * b.w)};
+  tk[2 * (size_t)n + t] = v2u32{pack_h2<VT>(s2 * c.x, s3 * c.y), pack_h2<VT>(s3 * c.z, s3 * c.w)};
+  ahat[t] = (alpha[t] * is) * is;
+}
+
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_vert(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
+           const int* __restrict__ vs_inc, const float* __restrict__ invm,
+           const float* __restrict__ gradC, const float* __restrict__ isd,
+           v2u32* __restrict__ vs_k) {
+  const int v = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (v >= nv) return;
+  const int base = vs_ptr[v >> 6] + (v & 63);
+  const int cnt = vs_cnt[v];
+  const float sw = sqrtf(invm[v]);
+  for (int k = 0; k < cnt; ++k) {
+    const int p = base + 64 * k;
+    const int inc = vs_inc[p];
+    const float s = sw * isd[inc >> 2];
+    const float* g = gradC + 12 * (size_t)(inc >> 2) + 3 * (inc & 3);
+    vs_k[p] = v2u32{pack_h2<VT>(s * g[0], s * g[1]), pack_h2<VT>(s * g[2], 0.0f)};
+  }
+}
+
+// u_v = sum_k K_{t_k, v} x_{t_k} over the vertex's incidence slots (FP32 accumulation, slot
+// order), eight slots' loads issued before their products.
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
+             const int* __restrict__ vs_inc, const v2u32* __restrict__ vs_k,
+             const float* __restrict__ x, float4* __restrict__ u, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int v = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (v >= nv) return;
+  const int base = vs_ptr[v >> 6] + (v & 63);
+  const int cnt = vs_cnt[v];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int k0 = 0; k0 < cnt; k0 += 8) {
+    int t[8];
+    v2u32 h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = base + 64 * min(k0 + j, cnt - 1);   // past the end: repeats the last slot
+      t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
+      h[j] = __builtin_nontemporal_load(vs_k + p);
+    }
+    float xv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xv[j] = x[t[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (k0 + j < cnt) {
+        ax = fmaf(lo_h<VT>(h[j].x), xv[j], ax);
+        ay = fmaf(hi_h<VT>(h[j].x), xv[j], ay);
+        az = fmaf(lo_h<VT>(h[j].y), xv[j], az);
+      }
+    }
+  }
+  u[v] = make_float4(ax, ay, az, 0.0f);
+}
+
+// (A^ x)_t from u: a^_t x_t + K_t0 . u_v0 + ... + K_t3 . u_v3 (FP32)
+template <int VT>
+__device__ __forceinline__ float fac_row(int t, int n, const int4* __restrict__ t_i,
+                                         const v2u32* __restrict__ tk,
+                                         const float* __restrict__ ahat,
+                                         const float4* __restrict__ u, float xt) {
+  const int4 vi = t_i[t];
+  const v2u32 q0 = __builtin_nontemporal_load(tk + t);
+  const v2u32 q1 = __builtin_nontemporal_load(tk + (size_t)n + t);
+  const v2u32 q2 = __builtin_nontemporal_load(tk + 2 * (size_t)n + t);
+  const float4 u0 = u[vi.x], u1 = u[vi.y], u2 = u[vi.z], u3 = u[vi.w];
+  float acc = ahat[t] * xt;
+  acc = fmaf(lo_h<VT>(q0.x), u0.x, acc);
+  acc = fmaf(hi_h<VT>(q0.x), u0.y, acc);
+  acc = fmaf(lo_h<VT>(q0.y), u0.z, acc);
+  acc = fmaf(hi_h<VT>(q0.y), u1.x, acc);
+  acc = fmaf(lo_h<VT>(q1.x), u1.y, acc);
+  acc = fmaf(hi_h<VT>(q1.x), u1.z, acc);
+  acc = fmaf(lo_h<VT>(q1.y), u2.x, acc);
+  acc = fmaf(hi_h<VT>(q1.y), u2.y, acc);
+  acc = fmaf(lo_h<VT>(q2.x), u2.z, acc);
+  acc = fmaf(hi_h<VT>(q2.x), u3.x, acc);
+  acc = fmaf(lo_h<VT>(q2.y), u3.y, acc);
+  acc = fmaf(hi_h<VT>(q2.y), u3.z, acc);
+  return acc;
+}
+
+// Factored counterpart of k_sellw on level 0 (same MODE semantics, rows = tets).
+template <int VT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_fac_apply(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
+            const float* __restrict__ ahat, const float4* __restrict__ u,
+            const float* __restrict__ x, float* y, const float* __restrict__ b,
+            float omega, const float* __restrict__ wp, const float* __restrict__ isd,
+            const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
+  if (wp) omega = *wp;
+  if (done && *done) return;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float xi = x[i];
+  const float ax = fac_row<VT>(i, n, t_i, tk, ahat, u, xi);
+  if (MODE == 2) {
+    y[i] = b[i] - ax;
+  } else if (MODE == 1 || MODE == 3) {
+    const float xn = fmaf(omega, b[i] - ax, xi);
+    y[i] = MODE == 3 ? isd[i] * xn : xn;
+  } else {
+    const float uu = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
+    const float xn = fmaf(cc[1], b[i] - ax, uu);
+    y[i] = MODE == 5 ? isd[i] * xn : xn;
+  }
+}
+
+// Power step on the factored level (w = A^ v after k_fac_gather of v) with the partials
+// v.A^v, v.v, w.w of k_power_step_h; the grid covers at least n rows.
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_power(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
+            const float* __restrict__ ahat, const float4* __restrict__ u,
+            const float* __restrict__ v, float* __restrict__ w, RedWS ws) {
+  double vav = 0.0, vv = 0.0, ww = 0.0;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i < n) {
+    const float vi = v[i];
+    const float ax = fac_row<VT>(i, n, t_i, tk, ahat, u, vi);
+    w[i] = ax;
+    vav = (double)vi * ax;
+    vv = (double)vi * vi;
+    ww = (double)ax * ax;
+  }
+  const double s0 = block_sum(vav);
+  const double s1 = block_sum(vv);
+  const double s2 = block_sum(ww);
+  store_partial(ws, 0, s0);
+  store_partial(ws, 1, s1);
+  store_partial(ws, 2, s2);
+}
+
 }  // namespace mgpbd
 
 using namespace mgpbd;
@@ -1954,6 +2139,23 @@ struct mgpbd_ctx {
   int pv_updates[MGPBD_MAX_LEVELS] = {};   // updates since the level was set
   double* pw_host = nullptr;    // pinned copy of pw_dev from the previous update (read lagged)
   MgLevel L[MGPBD_MAX_LEVELS];
+  // factored level 0 of the reduced-precision V-cycle (A^ = K K^T + diag(a^), K from the
+  // caller's gradients; mgpbd_mg_set_level0_factored): caller-owned arrays
+  struct Factored {
+    bool on = false;
+    int nv = 0;
+    const int4* t_i = nullptr;
+    const float* invm = nullptr;
+    const float* gradC = nullptr;
+    const float* alpha = nullptr;
+    const int* vs_ptr = nullptr;
+    const int* vs_cnt = nullptr;
+    const int* vs_inc = nullptr;
+    v2u32* vs_k = nullptr;   // K per vertex slot (4 halves)
+    v2u32* tk = nullptr;     // K per tet, three planes of n_tets
+    float* ahat = nullptr;   // a^ per tet
+    float4* u = nullptr;     // per-vertex products of the gather pass
+  } fac;
   int precision = MGPBD_PREC_FP32;   // V-cycle storage precision
   int flexible = 0;                   // Polak-Ribiere beta
   int poll = 4;
@@ -2421,6 +2623,7 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
   L.agg = agg; L.mptr = mptr; L.mem = mem; L.work = work;
   L.diagonal = diagonal_only;
   L.seg_ptr = nullptr; L.segs = nullptr; L.win_segs = 0; L.woff = nullptr;
+  if (level == 0) ctx->fac = mgpbd_ctx::Factored{};
   return 0;
 }
 
@@ -2440,7 +2643,7 @@ static int check_ready(mgpbd_ctx* ctx) {
     return mgpbd_fail(MGPBD_EINVAL, "multigrid: the dense coarsest solve needs two levels or more");
   if (ctx->precision != MGPBD_PREC_FP32)
     for (int l = 0; l < ctx->n_levels; ++l)
-      if (!ctx->L[l].hval && !ctx->L[l].hpk)
+      if (!ctx->L[l].hval && !ctx->L[l].hpk && !(l == 0 && ctx->fac.on))
         return mgpbd_fail(MGPBD_EINVAL, "multigrid: reduced precision without half buffers on level " + std::to_string(l));
   return 0;
 }
@@ -2703,11 +2906,38 @@ static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float
 #undef SELLH
 #undef SELLW
 }
+
+template <int VT>
+static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const float* b,
+                       const int* done, const float* wp, const float* xp, const float* cc) {
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  const MgLevel& L = ctx->L[0];
+  hipStream_t s = ctx->stream;
+  k_fac_gather<VT><<<nblocks(F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k,
+                                                    x, F.u, done);
+  const int g = nblocks(L.n);
+  const float w = ctx->omega;
+#define FAC(M) k_fac_apply<VT, M><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b, w, \
+                                                       wp, L.isd, done, xp, cc)
+  switch (mode) {
+    case 1: FAC(1); break;
+    case 2: FAC(2); break;
+    case 3: FAC(3); break;
+    case 4: FAC(4); break;
+    default: FAC(5); break;
+  }
+#undef FAC
+}
 }  // extern "C++"
 
 static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                   const float* b, const int* done, const float* wp, const float* xp = nullptr,
                   const float* cc = nullptr) {
+  if (&L == &ctx->L[0] && ctx->fac.on) {
+    if (ctx->precision == MGPBD_PREC_FP16) launch_fac<1>(ctx, mode, x, y, b, done, wp, xp, cc);
+    else launch_fac<2>(ctx, mode, x, y, b, done, wp, xp, cc);
+    return;
+  }
   if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp, xp, cc);
   else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp, xp, cc);
 }
@@ -2862,8 +3092,18 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     ++ctx->pv_updates[l];
     const int g = nblocks(64LL * L.n_slices);
     double* pw = ctx->pw_dev + 4 * l;
+    const mgpbd_ctx::Factored& F = ctx->fac;
+    const bool fac = l == 0 && F.on && ctx->precision != MGPBD_PREC_FP32;
     for (int it = 0; it < iters; ++it) {
-      if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
+      if (fac && ctx->precision == MGPBD_PREC_FP16) {
+        k_fac_gather<1><<<nblocks(F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
+                                                         F.vs_k, v, F.u, nullptr);
+        k_fac_power<1><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
+      } else if (fac) {
+        k_fac_gather<2><<<nblocks(F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
+                                                         F.vs_k, v, F.u, nullptr);
+        k_fac_power<2><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
+      } else if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
         k_power_step_wf<<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
             L.n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, L.dinv, v, w, ctx->ws);
       else if (ctx->precision == MGPBD_PREC_FP32)
@@ -2951,7 +3191,8 @@ int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible) {
 int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval, uint32_t* hpacked,
                             float* sd, float* isd, const int32_t* diag_pos) {
   CTX_CHECK("mgpbd_mg_set_level_half");
-  if (level < 0 || level >= ctx->n_levels || (!hval) == (!hpacked) || !sd || !isd || !diag_pos)
+  if (level < 0 || level >= ctx->n_levels || (hval && hpacked) || (!hval && !hpacked && level != 0) ||
+      !sd || !isd || !diag_pos)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_half: invalid arguments");
   MgLevel& L = ctx->L[level];
   L.hval = hpacked ? nullptr : hval; L.hpk = hpacked; L.sd = sd; L.isd = isd; L.diag_pos = diag_pos;
@@ -2988,14 +3229,66 @@ int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr,
   return 0;
 }
 
+int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i,
+                                 const float* v_invm, const float* gradC,
+                                 const float* alpha_tilde, const int32_t* vs_ptr,
+                                 const int32_t* vs_cnt, const int32_t* vs_inc, uint32_t* vs_k,
+                                 uint32_t* tk, float* ahat, float* u) {
+  CTX_CHECK("mgpbd_mg_set_level0_factored");
+  if (ctx->n_levels < 1 || ctx->L[0].n <= 0)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: level 0 not set");
+  drop_graphs(ctx);
+  if (!t_i) {   // back to the level's stored half values
+    ctx->fac = mgpbd_ctx::Factored{};
+    return 0;
+  }
+  if (n_verts <= 0 || !v_invm || !gradC || !alpha_tilde || !vs_ptr || !vs_cnt || !vs_inc ||
+      !vs_k || !tk || !ahat || !u)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: invalid arguments");
+  const auto al = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; };
+  if (!al(t_i, 16) || !al(gradC, 16) || !al(vs_k, 8) || !al(tk, 8) || !al(u, 16))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: t_i, gradC and u need 16-byte, "
+                                    "vs_k and tk 8-byte alignment");
+  if (!ctx->L[0].sd || !ctx->L[0].isd)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: call mgpbd_mg_set_level_half "
+                                    "for level 0 first (its scales)");
+  mgpbd_ctx::Factored& F = ctx->fac;
+  F.nv = n_verts;
+  F.t_i = reinterpret_cast<const int4*>(t_i);
+  F.invm = v_invm; F.gradC = gradC; F.alpha = alpha_tilde;
+  F.vs_ptr = vs_ptr; F.vs_cnt = vs_cnt; F.vs_inc = vs_inc;
+  F.vs_k = reinterpret_cast<v2u32*>(vs_k);
+  F.tk = reinterpret_cast<v2u32*>(tk);
+  F.ahat = ahat;
+  F.u = reinterpret_cast<float4*>(u);
+  F.on = true;
+  return 0;
+}
+
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
   CTX_CHECK("mgpbd_mg_convert_half");
   if (int e = check_levels(ctx)) return e;
   if (ctx->precision == MGPBD_PREC_FP32) return 0;
   for (int l = 0; l < ctx->n_levels; ++l) {
     MgLevel& L = ctx->L[l];
-    if (!L.hval && !L.hpk) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
+    const bool fac = l == 0 && ctx->fac.on;
+    if (!L.hval && !L.hpk && !fac) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
     k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
+    if (fac) {
+      const mgpbd_ctx::Factored& F = ctx->fac;
+      if (ctx->precision == MGPBD_PREC_FP16) {
+        k_fac_tet<1><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
+                                                               L.isd, F.tk, F.ahat);
+        k_fac_vert<1><<<nblocks(F.nv), kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
+                                                                 F.invm, F.gradC, L.isd, F.vs_k);
+      } else {
+        k_fac_tet<2><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
+                                                               L.isd, F.tk, F.ahat);
+        k_fac_vert<2><<<nblocks(F.nv), kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
+                                                                 F.invm, F.gradC, L.isd, F.vs_k);
+      }
+      continue;
+    }
     const int g = nblocks(64LL * L.n_slices);
     const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
     if (L.woff && L.hpk && L.segs && ctx->precision == MGPBD_PREC_FP16)
@@ -3036,6 +3329,18 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   float* res = vcycle_from(ctx, 0, b, x1, nullptr);
   k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(2, L.n, 0.f, res, x_out);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_vcycle");
+  return 0;
+}
+
+int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b, const float* x, float* r) {
+  CTX_CHECK("mgpbd_mg_level_residual");
+  if (int e = check_ready(ctx)) return e;
+  if (level < 0 || level >= ctx->n_levels || !b || !x || !r || x == r)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_level_residual: invalid arguments");
+  const MgLevel& L = ctx->L[level];
+  if (half_mode(ctx)) sellh(ctx, L, 2, x, r, b, nullptr, nullptr);
+  else sell_launch<2>(ctx, L, x, r, L.dinv, b, ctx->omega, nullptr, nullptr);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_level_residual");
   return 0;
 }
 

@@ -34,6 +34,9 @@ last entry repeating), ``f_ext``
 substep's lambda, or from the network's prediction with the previous lambda as the fallback;
 the device keeps the start with the smaller residual and iterates to the same tolerance.
 ``record_samples`` (a list) collects training pairs from the first Newton iteration.
+``level0`` ('assembled' | 'factored'; reduced precision only): the V-cycle applies level 0 as
+K K^T + diag(a^) from the tets' gradients instead of its stored half values (the matrix
+J W J^T + alpha~ in factored form; fewer bytes per sweep, no level-0 half conversion).
 """
 
 from __future__ import annotations
@@ -80,10 +83,14 @@ class MGPBDSolverGPU:
                  fp64_iterate: bool = False, power_iters: int = 2,
                  smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
-                 warm_start_model=None, record_samples=None,
+                 warm_start_model=None, record_samples=None, level0: str = "assembled",
                  device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
+        if level0 not in ("assembled", "factored"):
+            raise ValueError("level0 must be 'assembled' or 'factored'")
+        if level0 == "factored" and precision == "fp32":
+            raise ValueError("level0='factored' is a reduced-precision V-cycle option")
         self.ctx = get_context(device)
         dev = self.device = self.ctx.device
         self.v_p = to_device(v_p, torch.float32, dev, (-1, 3))
@@ -129,6 +136,7 @@ class MGPBDSolverGPU:
         self.cheb_ratio = float(cheb_ratio)
         self.stall_rescue = bool(stall_rescue)
         self.smooth_schedule = smooth_schedule
+        self.level0 = level0
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -186,8 +194,11 @@ class MGPBDSolverGPU:
             smoother_safety=self.smoother_safety, cheb_ratio=self.cheb_ratio,
             stall_rescue=self.stall_rescue, smooth_schedule=self.smooth_schedule,
             device=self.device.index)
-        self._gpu_solver.setup(self._A, self._aggs)
         self._build_incidence()
+        if self.level0 == "factored":
+            self._gpu_solver.set_factored_level0(self.t_i, self.v_invm, self.gradC,
+                                                 self.alpha_tilde, self._vptr_np, self._vinc_np)
+        self._gpu_solver.setup(self._A, self._aggs)
         if self.verbose:
             print(f"MGPBD-GPU: {self.n_verts} verts, {self.n_tets} tets, {self._nnz} nnz, "
                   f"aggregation={self.aggregation}, levels={self._gpu_solver._level_sizes}")
@@ -252,6 +263,7 @@ class MGPBDSolverGPU:
         vinc = np.empty(max(4 * self.n_tets, 1), np.int32)
         check(lib.mgpbd_vertex_incidence(self.n_tets, self.n_verts, np_ptr(self._t_i_np),
                                          np_ptr(vptr), np_ptr(vinc)), "mgpbd_vertex_incidence")
+        self._vptr_np, self._vinc_np = vptr, vinc
         self._vptr = torch.from_numpy(vptr).to(self.device)
         self._vinc = torch.from_numpy(vinc).to(self.device)
 

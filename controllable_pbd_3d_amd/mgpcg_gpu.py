@@ -27,7 +27,12 @@ Extensions (keyword-only, no reference counterpart):
   products and sums (on stiff high-contrast systems the rounding of an FP32 iterate alone keeps
   the true residual above 1e-6);
 * ``stall_rescue`` (default on with the spectral smoothers): a PCG that stops contracting
-  re-estimates the spectra from a fresh vector with a raised safety factor and restarts.
+  re-estimates the spectra from a fresh vector with a raised safety factor and restarts;
+* ``set_factored_level0`` (before ``setup``, reduced precision only): the V-cycle applies
+  level 0 as K K^T + diag(a^) from the tets' gradients (K_tk = sqrt(w_v) g_tk / sqrt(A_tt),
+  stored FP16/BF16) instead of its stored half values, streaming ~12 bytes per vertex-tet
+  incidence and ~44 per tet instead of ~4 per matrix entry; the PCG's own A p keeps the
+  assembled FP32 level.
 """
 
 from __future__ import annotations
@@ -49,6 +54,30 @@ def _members(agg: np.ndarray, n_c: int):
     ptr = np.zeros(n_c + 1, np.int32)
     ptr[1:] = np.cumsum(np.bincount(agg, minlength=n_c))
     return ptr, order
+
+
+def vertex_slices(vptr: np.ndarray, vinc: np.ndarray):
+    """Vertex-sliced incidence of the factored level 0: vertices binned 64 per slice, slot k
+    of vertex 64 s + l at vs_ptr[s] + 64 k + l for k < vs_cnt[v] (the vertex's incidences
+    4 tet + local index in vinc order), slices as wide as their busiest vertex. Returns
+    (vs_ptr int32[n_slices + 1], vs_cnt int32[n_verts], vs_inc int32[vs_ptr[-1]])."""
+    vptr = np.asarray(vptr, np.int64)
+    nv = len(vptr) - 1
+    cnt = np.diff(vptr)
+    ns = (nv + 63) // 64
+    padded = np.zeros(64 * ns, np.int64)
+    padded[:nv] = cnt
+    width = padded.reshape(ns, 64).max(axis=1) if ns else np.zeros(0, np.int64)
+    vs_ptr = np.zeros(ns + 1, np.int64)
+    vs_ptr[1:] = np.cumsum(64 * width)
+    v_of = np.repeat(np.arange(nv, dtype=np.int64), cnt)
+    k_of = np.arange(len(v_of), dtype=np.int64) - np.repeat(vptr[:-1], cnt)
+    pos = vs_ptr[v_of // 64] + 64 * k_of + v_of % 64
+    vs_inc = np.zeros(max(int(vs_ptr[-1]), 1), np.int32)
+    vs_inc[pos] = np.asarray(vinc, np.int32)[:len(pos)]
+    if vs_ptr[-1] >= 2 ** 31:
+        raise ValueError("vertex incidence exceeds 32-bit positions")
+    return vs_ptr.astype(np.int32), cnt.astype(np.int32), vs_inc
 
 
 def _aggs_from_Ps(Ps):
@@ -124,6 +153,20 @@ class GPUMultigridSolver:
         self.ctx = new_context(device)
         self.device = self.ctx.device
         self.last_info = None
+        self._fac_src = None
+        self._fac = None
+
+    def set_factored_level0(self, t_i, v_invm, gradC, alpha_tilde, vptr, vinc):
+        """Level 0 of the reduced-precision V-cycle applied in factored form (extension).
+        Call before ``setup``. The device tensors t_i (int32, 4 per tet), v_invm, gradC (12
+        floats per tet) and alpha_tilde are read at every ``update_all_coarse_matrices`` and
+        must stay allocated; vptr / vinc: the host vertex incidence (4 tet + local index)."""
+        if self.precision == "fp32":
+            raise ValueError("the factored level 0 is a reduced-precision V-cycle option")
+        if self._initialized:
+            raise RuntimeError("set_factored_level0 must precede setup()")
+        self._fac_src = dict(t_i=t_i, v_invm=v_invm, gradC=gradC, alpha_tilde=alpha_tilde,
+                             vptr=np.asarray(vptr), vinc=np.asarray(vinc))
 
     def setup(self, A_level0, Ps_or_aggs: list):
         """Hierarchy from the fine matrix and prolongations (cons/mgpcg_gpu.py:30-173).
@@ -245,6 +288,9 @@ class GPUMultigridSolver:
                     self._set_window(l, win, hpk=False)
             return
         for l, L in enumerate(self.levels):
+            if l == 0 and self._fac_src is not None:
+                self._setup_factored(wins[0])
+                continue
             rows = np.repeat(np.arange(L.n_rows, dtype=np.int64), np.diff(L.indptr))
             delta = L.indices.astype(np.int64) - rows
             win = wins[l]
@@ -264,6 +310,43 @@ class GPUMultigridSolver:
             self._half.append(hb)
             if win is not None:
                 self._set_window(l, win, hpk=True)
+
+    def _setup_factored(self, win):
+        """Level 0 without stored half values: scales, the FP32 kernels' window, and the
+        factored operator's buffers."""
+        dev = self.device
+        L = self.levels[0]
+        src = self._fac_src
+        hb = {"hpk": None, "hval": None,
+              "sd": torch.ones(L.n_rows, dtype=torch.float32, device=dev),
+              "isd": torch.ones(L.n_rows, dtype=torch.float32, device=dev),
+              "seg_ptr": None, "segs": None}
+        self.ctx("mgpbd_mg_set_level_half", 0, None, None, dptr(hb["sd"]), dptr(hb["isd"]),
+                 dptr(L.diag_pos))
+        self._half.append(hb)
+        if win is not None:
+            self._set_window(0, win, hpk=False)
+        vs_ptr, vs_cnt, vs_inc = vertex_slices(src["vptr"], src["vinc"])
+        nv, nt = len(vs_cnt), L.n_rows
+        self._fac = {
+            "vs_ptr": torch.from_numpy(vs_ptr).to(dev),
+            "vs_cnt": torch.from_numpy(vs_cnt).to(dev),
+            "vs_inc": torch.from_numpy(vs_inc).to(dev),
+            "vs_k": torch.zeros(2 * len(vs_inc), dtype=torch.int32, device=dev),
+            "tk": torch.zeros(6 * nt, dtype=torch.int32, device=dev),
+            "ahat": torch.zeros(nt, dtype=torch.float32, device=dev),
+            "u": torch.zeros(4 * nv, dtype=torch.float32, device=dev),
+        }
+        f = self._fac
+        self.ctx("mgpbd_mg_set_level0_factored", nv, dptr(src["t_i"]), dptr(src["v_invm"]),
+                 dptr(src["gradC"]), dptr(src["alpha_tilde"]), dptr(f["vs_ptr"]),
+                 dptr(f["vs_cnt"]), dptr(f["vs_inc"]), dptr(f["vs_k"]), dptr(f["tk"]),
+                 dptr(f["ahat"]), dptr(f["u"]))
+
+    def level_residual(self, level: int, b: torch.Tensor, x: torch.Tensor, r: torch.Tensor):
+        """r = b - A_l x with the operator the V-cycle applies on level l (scaled variables in
+        reduced precision)."""
+        self.ctx("mgpbd_mg_level_residual", int(level), dptr(b), dptr(x), dptr(r))
 
     def _set_window(self, l, win, hpk):
         """Uploads level l's window lists and switches its kernels to the windowed ones. The
@@ -344,7 +427,7 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_ctx_device_bytes", ctypes.byref(ctx_bytes))
         return device_bytes(self.levels, self.transfers, self.refill, self._work,
                             self._pcg_work, self._x, getattr(self, "_half", None),
-                            getattr(self, "_win", None), seen=seen) + ctx_bytes.value
+                            getattr(self, "_win", None), self._fac, seen=seen) + ctx_bytes.value
 
     def pcg_counters(self):
         """(stall rescues, verification restarts) over this solver's solves."""

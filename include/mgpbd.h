@@ -349,7 +349,8 @@ int mgpbd_mg_set_precision(mgpbd_ctx* ctx, int precision, int flexible);
 /* Per-level buffers, exactly one of: hpacked_dev (uint32 per SELL slot, (col - row) << 16 |
  * half bits; only valid when |col - row| < 32768 on the whole level) or hval_dev (uint16 per
  * SELL slot, with the level's int32 columns); sd_dev / isd_dev (n_rows floats, written by
- * mgpbd_mg_convert_half), diag_pos_dev (SELL position of each row's diagonal). */
+ * mgpbd_mg_convert_half), diag_pos_dev (SELL position of each row's diagonal). Level 0 may
+ * pass neither buffer when it runs factored (mgpbd_mg_set_level0_factored). */
 int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, uint32_t* hpacked_dev,
                             float* sd_dev, float* isd_dev, const int32_t* diag_pos_dev);
 /* Switches a level to the windowed kernels (no reference counterpart; same results as the
@@ -365,9 +366,36 @@ int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, uint3
 int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr_dev,
                               const int32_t* segs_dev, int n_blocks, int max_segs,
                               uint16_t* woff_dev);
+/* Factored level 0 for the reduced-precision V-cycle (extension; no reference counterpart --
+ * the reference stores level 0 only as the assembled CSR of cons/mgpbd_gpu.py:273-324). The
+ * level-0 matrix is A = J W J^T + diag(alpha~) with J's rows the tets' vertex gradients, so
+ * its scaled form is K K^T + diag(a^), K_tk = sqrt(invm_v) gradC_tk / sqrt(A_tt),
+ * a^_t = alpha~_t / A_tt; the V-cycle's level-0 sweeps, residuals and power steps then apply
+ * it in two passes (per-vertex gather of K x, per-tet product) from K stored FP16/BF16 instead
+ * of the level's stored half values, which need not exist (mgpbd_mg_set_level_half(ctx, 0,
+ * NULL, NULL, sd, isd, diag_pos) sets the scales only). mgpbd_mg_convert_half recomputes K
+ * and a^ from the arrays' current contents. Caller-owned device arrays, valid while used:
+ * t_i (int32[4 n_tets], 16-byte aligned), v_invm (n_verts), gradC (float[12 n_tets], 16-byte
+ * aligned), alpha_tilde (n_tets); the vertex-sliced incidence: slot k of vertex 64 s + l at
+ * vs_ptr[s] + 64 k + l for k < vs_cnt[v], holding vs_inc = 4 tet + local vertex index;
+ * outputs vs_k (2 uint32 per slot), tk (6 uint32 per tet), ahat (n_tets floats), u (4 floats
+ * per vertex, 16-byte aligned). t_i NULL switches the factored operator off. Level 0 must be
+ * set (mgpbd_mg_set_level) and its scales given (mgpbd_mg_set_level_half) first; setting
+ * level 0 again switches it off. */
+int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i,
+                                 const float* v_invm, const float* gradC,
+                                 const float* alpha_tilde, const int32_t* vs_ptr,
+                                 const int32_t* vs_cnt, const int32_t* vs_inc, uint32_t* vs_k,
+                                 uint32_t* tk, float* ahat, float* u);
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx);
+/* r = b - A_l x with the operator the V-cycle applies on level l (no reference counterpart;
+ * the reference computes this residual inside vcycle, cons/mgpcg_gpu.py:230-234): the FP32
+ * level, or in reduced precision its scaled form (stored half values, or the factored level
+ * 0) with b, x, r in the scaled variables. Device arrays of n_l floats; x != r. */
+int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b_dev, const float* x_dev,
+                            float* r_dev);
 /* x = M^-1 b with one V-cycle from x = 0. Replaces vcycle(0) (cons/mgpcg_gpu.py:214-259). */
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
 /* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:

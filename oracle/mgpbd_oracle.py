@@ -111,6 +111,28 @@ def galerkin_contribution_maps_fast(A_fine: sp.csr_matrix, agg: np.ndarray, Ac: 
     return fine_idx, offsets
 
 
+def factored_level0(t_i, v_invm, gradC, alpha_tilde, A_diag):
+    """The scaled level-0 matrix D^-1/2 A D^-1/2 in factored form K K^T + diag(a^), FP64.
+    A = J W J^T + diag(alpha~) (cons/mgpbd_gpu.py:273-309: A_ij = sum over shared vertices of
+    w_v g_iv . g_jv, A_ii adds alpha~_i), so K_tk = sqrt(w_v(t,k)) g_tk / sqrt(A_tt) and
+    a^_t = alpha~_t / A_tt. Returns K (n_tets, 4, 3) and a^ (n_tets,)."""
+    tv = np.asarray(t_i, np.int64).reshape(-1, 4)
+    isd = 1.0 / np.sqrt(np.asarray(A_diag, np.float64))
+    K = (np.sqrt(np.asarray(v_invm, np.float64))[tv][:, :, None] *
+         np.asarray(gradC, np.float64).reshape(-1, 4, 3) * isd[:, None, None])
+    return K, np.asarray(alpha_tilde, np.float64) * isd * isd
+
+
+def factored_apply(t_i, K, ahat, x, n_verts):
+    """(K K^T + diag(a^)) x: per-vertex sums u_v = sum_t K_tk x_t, then per tet
+    a^_t x_t + sum_k K_tk . u_v(t,k) (FP64)."""
+    tv = np.asarray(t_i, np.int64).reshape(-1, 4)
+    x = np.asarray(x, np.float64)
+    u = np.zeros((n_verts, 3))
+    np.add.at(u, tv.reshape(-1), (K * x[:, None, None]).reshape(-1, 3))
+    return ahat * x + np.einsum("tkc,tkc->t", K, u[tv])
+
+
 class _OrcLevel(ctypes.Structure):
     pass
 

@@ -462,6 +462,69 @@ def _window_columns(L, hb, woff=None):
     return segs[seg_ptr[rows // 256] + k] * 32 + (off // 4) % 32
 
 
+def _decode_half(raw, precision):
+    return raw.view(np.float16).astype(np.float64) if precision == "fp16" else \
+        _bf16_to_f32(raw).astype(np.float64)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_factored_level0_operator(precision):
+    """Factored level 0 (K K^T + diag(a^) from the gradients): (1) the stored K equals the
+    oracle's FP64 K rounded once (half an ulp of the format), the vertex-slot copy equal to the
+    tet-order copy bit for bit (an exactly symmetric operator); (2) the V-cycle's level-0
+    residual b - A^ x equals the FP64 product with the stored K within FP32 accumulation
+    (1e-5 of max |A^ x|); (3) it agrees with the assembled half level's residual within the
+    format's rounding (norm-wise 1e-2 FP16, 5e-2 BF16)."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(6, seed=4)
+    v0 = deformed(mesh, seed=4, amp=0.05, twist=0.1)
+    kw = dict(mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended", galerkin="structural",
+              max_coarse=64, coarse_solver="cholesky", relative_tol=True, tol_cg=1e-6,
+              precision=precision)
+    sa = _solver(mesh, v0, **kw)
+    sf = _solver(mesh, v0, level0="factored", **kw)
+    mg, mga = sf._gpu_solver, sa._gpu_solver
+    assert mg._half[0]["hpk"] is None and mg._half[0]["hval"] is None
+    nt, nv = sf.n_tets, sf.n_verts
+    g = sf.gradC.cpu().numpy().reshape(-1).astype(np.float64)
+    assert np.array_equal(g, sa.gradC.cpu().numpy().reshape(-1))
+    d = mg.levels[0].to_scipy().diagonal().astype(np.float64)
+    K64, a64 = om.factored_level0(mesh.tets, mesh.invm, g, sf.alpha_tilde.cpu().numpy(), d)
+    f = mg._fac
+    tk = f["tk"].cpu().numpy().view(np.uint16).reshape(3, nt, 4).transpose(1, 0, 2).reshape(nt, 12)
+    Kd = _decode_half(tk, precision).reshape(nt, 4, 3)
+    eps = 2.0 ** -11 if precision == "fp16" else 2.0 ** -8
+    tiny = 2.0 ** -25 if precision == "fp16" else 1e-38
+    assert np.all(np.abs(Kd - K64) <= eps * np.abs(K64) + tiny)
+    ahat = f["ahat"].cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(ahat, a64, rtol=1e-6)
+    vs_ptr, vs_cnt = f["vs_ptr"].cpu().numpy(), f["vs_cnt"].cpu().numpy()
+    vs_inc = f["vs_inc"].cpu().numpy()
+    vk = f["vs_k"].cpu().numpy().view(np.uint16).reshape(-1, 4)
+    v = np.repeat(np.arange(nv), vs_cnt)
+    k = np.arange(len(v)) - np.repeat(np.concatenate([[0], np.cumsum(vs_cnt)[:-1]]), vs_cnt)
+    pos = vs_ptr[v // 64] + 64 * k + v % 64
+    inc = vs_inc[pos]
+    assert np.array_equal(mesh.tets.reshape(-1, 4)[inc >> 2, inc & 3], v)
+    tk3 = tk.reshape(nt, 4, 3)
+    assert np.array_equal(vk[pos, :3], tk3[inc >> 2, inc & 3])
+    assert np.all(vk[pos, 3] == 0)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(nt).astype(np.float32)
+    b = rng.standard_normal(nt).astype(np.float32)
+    xd, bd = torch.from_numpy(x).cuda(), torch.from_numpy(b).cuda()
+    rf, ra = torch.empty_like(xd), torch.empty_like(xd)
+    mg.level_residual(0, bd, xd, rf)
+    mga.level_residual(0, bd, xd, ra)
+    ax = om.factored_apply(mesh.tets, Kd, ahat, x, nv)
+    got = rf.cpu().numpy().astype(np.float64)
+    scale = np.abs(ax).max()
+    assert np.abs(got - (b - ax)).max() <= 1e-5 * scale
+    diff = np.linalg.norm(got - ra.cpu().numpy()) / np.linalg.norm(ax)
+    print(f"[factored] {precision}: |K - K64| ok, residual vs assembled half {diff:.2e}", flush=True)
+    assert diff <= (1e-2 if precision == "fp16" else 5e-2)
+
+
 def test_half_levels_with_int32_columns_match_packed_offsets(monkeypatch):
     """Levels whose column offsets do not fit 16 bits keep int32 columns beside 16-bit values;
     the arithmetic is that of the packed levels, so a PCG solve with every level forced onto
@@ -537,16 +600,17 @@ def test_windowed_levels_match_global_gathers(monkeypatch, precision):
         assert out[0][2] == r[2]
 
 
-@pytest.mark.parametrize("precision", ["fp16", "bf16"])
-def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision):
-    """Reduced-precision V-cycle inside the flexible FP32 PCG: relative residual <= 2e-6 when
-    re-evaluated in FP64, solution within 1e-5 of scipy's FP64 direct solve, and at most 25%
-    more iterations than the FP32 V-cycle on the same system."""
+@pytest.mark.parametrize("precision,level0", [("fp16", "assembled"), ("bf16", "assembled"),
+                                              ("fp16", "factored"), ("bf16", "factored")])
+def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision, level0):
+    """Reduced-precision V-cycle (stored or factored level 0) inside the flexible FP32 PCG:
+    relative residual <= 2e-6 when re-evaluated in FP64, solution within 1e-5 of scipy's FP64
+    direct solve, and at most 25% more iterations than the FP32 V-cycle on the same system."""
     mesh, s, o = _extended_pair(n=6, seed=7)
     v0 = s.v_p.cpu().numpy()
     s2 = _solver(mesh, v0, mu_per_tet=o.mu_per_tet, hierarchy="extended", galerkin="structural",
                  max_coarse=64, coarse_solver="cholesky", precision=precision,
-                 relative_tol=True, tol_cg=1e-6)
+                 relative_tol=True, tol_cg=1e-6, level0=level0)
     s._compute_C_and_gradC()
     s2._compute_C_and_gradC()
     b = s._b_gpu.cpu().numpy()
@@ -558,13 +622,14 @@ def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision):
     res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
     x64 = sp.linalg.spsolve(A.tocsc(), b.astype(np.float64))
     err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
-    print(f"[mixed] {precision}: iters {it} (fp32 {it32}) rel residual {res:.2e} err {err:.2e}",
-          flush=True)
+    print(f"[mixed] {precision} {level0}: iters {it} (fp32 {it32}) rel residual {res:.2e} "
+          f"err {err:.2e}", flush=True)
     assert res <= 2e-6 and err <= 1e-5
     assert it <= int(np.ceil(1.25 * it32)) + 1
 
 
-def test_mixed_precision_substeps_match_fp32():
+@pytest.mark.parametrize("level0", ["assembled", "factored"])
+def test_mixed_precision_substeps_match_fp32(level0):
     """Ten substeps with the FP16 V-cycle stay within 1e-4 h of the FP32 V-cycle run, the two
     solvers stepping side by side on one device. Both solve every linear system to the same
     1e-6 relative PCG tolerance and run Newton to rtol 1e-6, so neither run can stop a Newton
@@ -576,13 +641,13 @@ def test_mixed_precision_substeps_match_fp32():
               relative_tol=True, tol_cg=1e-6, max_coarse=64, coarse_solver="cholesky",
               rtol=1e-6, atol=1e-9, maxiter=30)
     a = _solver(mesh, None, **kw)
-    b = _solver(mesh, None, precision="fp16", **kw)
+    b = _solver(mesh, None, precision="fp16", level0=level0, **kw)
     worst = 0.0
     for k in range(10):
         a.step(0)
         b.step(0)
         worst = max(worst, float((a.v_p - b.v_p).abs().max()))
-    print(f"[mixed] substeps max|dx|/h = {worst / mesh.h:.2e} fp32 {a.last_stats} "
+    print(f"[mixed] {level0} substeps max|dx|/h = {worst / mesh.h:.2e} fp32 {a.last_stats} "
           f"fp16 {b.last_stats}", flush=True)
     assert worst <= 1e-4 * mesh.h
 
