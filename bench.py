@@ -108,11 +108,16 @@ def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
 
 def factored_bytes(n_tets, n_verts, kind, chebyshev=False):
     """Algorithmic HBM bytes of one application of the factored level 0 (vertex pass + tet
-    pass), each array counted once per pass: vertex pass = 4 incidence slots per tet x (4-byte
+    pass), each array counted once per pass. V-cycle sweeps and residuals: vertex pass = 4 incidence slots per tet x (4-byte
     incidence + 8-byte K) + x (4 n) + u written (16 per vertex); tet pass = t_i (16) + K (24)
     + a^ (4) + x (4) + b (4) + y (4) per tet + u read (16 per vertex), + the previous iterate
     (4 n) on average with the Chebyshev smoother (as for the stored levels). The residual pass
     reads and writes the same arrays."""
+    if kind == "spmv":
+        # the PCG's FP64 product from the FP32 gradients: 4 slots per tet x (4-byte incidence +
+        # 16-byte gradient) + p (4 n) + u (32 per vertex) written; t_i (16) + gradC (48) +
+        # alpha~ (4) + p (4) + A p (4) per tet + u read
+        return 80 * n_tets + 4 * n_tets + 32 * n_verts + 76 * n_tets + 32 * n_verts
     if kind not in ("jacobi", "resid"):
         raise ValueError(kind)
     vertex_pass = 48 * n_tets + 4 * n_tets + 16 * n_verts
@@ -197,8 +202,8 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
     for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
-            by = factored_bytes(n, factored_nv, kind, chebyshev) if factored_nv and \
-                kind != "spmv" else kernel_bytes(n, nnz, kind, per, chebyshev)
+            by = factored_bytes(n, factored_nv, kind, chebyshev) if factored_nv else \
+                kernel_bytes(n, nnz, kind, per, chebyshev)
             kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
                           "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
     for kind, slot in (("fill_A", 3), ("constraints", 4), ("refill", 5), ("coarse", 6)):

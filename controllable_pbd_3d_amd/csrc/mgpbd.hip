@@ -1885,10 +1885,13 @@ k_pcg_p_flex(int n, const float* __restrict__ z, float* __restrict__ p, float* _
 // bits of every component) and a^_t = alpha~_t / A_tt. y = A^ x takes two passes:
 //   per vertex  u_v = sum over the tets t holding v of K_tk x_t      (k_fac_gather)
 //   per tet     y_t = a^_t x_t + sum_k K_tk . u_v(t,k)                (k_fac_apply)
-// The vertex pass reads a vertex-sliced incidence list (64 vertices per slice, slot k of
-// vertex 64s+l at vs_ptr[s] + 64k + l, vs_inc = 4 tet + local vertex) with K beside it (four
-// halves per slot, the fourth zero); the tet pass reads K in tet order (12 halves per tet in
-// three planes of four halves). Both copies are rounded from the same FP32 product, so the
+// The vertex pass reads a vertex-sliced incidence list (vs_inc = 4 tet + local vertex) with K
+// beside it (four halves per slot, the fourth zero): four lanes per vertex, 16 vertices per
+// 64-lane slice, slot k of vertex 16s+l at vs_ptr[s] + 64 (k/4) + 4l + k%4, so lane q of a
+// vertex takes its slots q, q+4, ... with coalesced loads across the wavefront and the four
+// partial sums meet in two shuffles (one lane per vertex left too few wavefronts to cover the
+// gathers' latency). The tet pass reads K in tet order (12 halves per tet in three planes of
+// four halves). Both copies are rounded from the same FP32 product, so the
 // operator is exactly symmetric. Per application this streams ~12 bytes per incidence slot
 // and ~44 per tet (~120 MB at 1M tets) instead of ~4 bytes per stored entry of the assembled
 // half level (~307 MB), and the per-Newton-iteration conversion reads the gradients instead of
@@ -1905,26 +1908,45 @@ __device__ __forceinline__ float lo_h(uint32_t w) { return h2f<VT>((unsigned sho
 template <int VT>
 __device__ __forceinline__ float hi_h(uint32_t w) { return h2f<VT>((unsigned short)(w >> 16)); }
 
-// K in tet order (planes tk[0..n), tk[n..2n), tk[2n..3n)) and a^ from the current gradients
-// and the level-0 scales; k_fac_vert writes the same K bits in vertex-slot order.
+// The level-0 diagonal A_tt = sum_k w_k |g_tk|^2 + alpha~_t from the gradients (sd, isd, 1/A_tt),
+// then K in tet order (planes tk[0..n), tk[n..2n), tk[2n..3n)) and a^; k_fac_vert writes the
+// same K bits in vertex-slot order from the isd written here.
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_tet(int n, const int4* __restrict__ t_i, const float* __restrict__ invm,
           const float* __restrict__ gradC, const float* __restrict__ alpha,
-          const float* __restrict__ isd, v2u32* __restrict__ tk, float* __restrict__ ahat) {
+          float* __restrict__ sd, float* __restrict__ isd, float* __restrict__ dinv,
+          v2u32* __restrict__ tk, float* __restrict__ ahat) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
   const int4 vi = t_i[t];
-  const float is = isd[t];
   const float4* g4 = reinterpret_cast<const float4*>(gradC + 12 * (size_t)t);
   const float4 a = g4[0], b = g4[1], c = g4[2];
   // g_0 = (a.x a.y a.z), g_1 = (a.w b.x b.y), g_2 = (b.z b.w c.x), g_3 = (c.y c.z c.w)
-  const float s0 = sqrtf(invm[vi.x]) * is, s1 = sqrtf(invm[vi.y]) * is;
-  const float s2 = sqrtf(invm[vi.z]) * is, s3 = sqrtf(invm[vi.w]) * is;
+  const float w0 = invm[vi.x], w1 = invm[vi.y], w2 = invm[vi.z], w3 = invm[vi.w];
+  float d = alpha[t];
+  d = fmaf(w0, fmaf(a.x, a.x, fmaf(a.y, a.y, a.z * a.z)), d);
+  d = fmaf(w1, fmaf(a.w, a.w, fmaf(b.x, b.x, b.y * b.y)), d);
+  d = fmaf(w2, fmaf(b.z, b.z, fmaf(b.w, b.w, c.x * c.x)), d);
+  d = fmaf(w3, fmaf(c.y, c.y, fmaf(c.z, c.z, c.w * c.w)), d);
+  const float r = d > 0.0f ? sqrtf(d) : 1.0f;
+  const float is = 1.0f / r;
+  sd[t] = r;
+  isd[t] = is;
+  dinv[t] = d > 0.0f ? 1.0f / d : 0.0f;
+  const float s0 = sqrtf(w0) * is, s1 = sqrtf(w1) * is;
+  const float s2 = sqrtf(w2) * is, s3 = sqrtf(w3) * is;
   tk[t] = v2u32{pack_h2<VT>(s0 * a.x, s0 * a.y), pack_h2<VT>(s0 * a.z, s1 * a.w)};
   tk[(size_t)n + t] = v2u32{pack_h2<VT>(s1 * b.x, s1 * b.y), pack_h2<VT>(s2 * b.z, s2 * b.w)};
   tk[2 * (size_t)n + t] = v2u32{pack_h2<VT>(s2 * c.x, s3 * c.y), pack_h2<VT>(s3 * c.z, s3 * c.w)};
   ahat[t] = (alpha[t] * is) * is;
+}
+
+constexpr int kFacLanes = 4;   // lanes per vertex in the vertex pass
+
+// First own slot of lane q of vertex v and the number of its own slots (k = q, q + 4, ...).
+__device__ __forceinline__ int fac_slot0(const int* __restrict__ vs_ptr, int v, int q) {
+  return vs_ptr[v >> 4] + ((v & 15) << 2) + q;
 }
 
 template <int VT>
@@ -1932,40 +1954,44 @@ __global__ void __launch_bounds__(kBlock)
 k_fac_vert(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
            const int* __restrict__ vs_inc, const float* __restrict__ invm,
            const float* __restrict__ gradC, const float* __restrict__ isd,
-           v2u32* __restrict__ vs_k) {
-  const int v = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+           v2u32* __restrict__ vs_k, float4* __restrict__ vs_g) {
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
   if (v >= nv) return;
-  const int base = vs_ptr[v >> 6] + (v & 63);
-  const int cnt = vs_cnt[v];
+  const int base = fac_slot0(vs_ptr, v, q);
+  const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
   const float sw = sqrtf(invm[v]);
-  for (int k = 0; k < cnt; ++k) {
+  for (int k = 0; k < own; ++k) {
     const int p = base + 64 * k;
     const int inc = vs_inc[p];
     const float s = sw * isd[inc >> 2];
     const float* g = gradC + 12 * (size_t)(inc >> 2) + 3 * (inc & 3);
     vs_k[p] = v2u32{pack_h2<VT>(s * g[0], s * g[1]), pack_h2<VT>(s * g[2], 0.0f)};
+    vs_g[p] = make_float4(g[0], g[1], g[2], 0.0f);
   }
 }
 
-// u_v = sum_k K_{t_k, v} x_{t_k} over the vertex's incidence slots (FP32 accumulation, slot
-// order), eight slots' loads issued before their products.
+// u_v = sum_k K_{t_k, v} x_{t_k} over the vertex's incidence slots (FP32 accumulation): each
+// of the vertex's four lanes sums its own slots (loads of up to eight issued before their
+// products), then lanes q and q^1, q and q^2 add (fixed order, deterministic).
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
              const int* __restrict__ vs_inc, const v2u32* __restrict__ vs_k,
              const float* __restrict__ x, float4* __restrict__ u, const int* __restrict__ done) {
   if (done && *done) return;
-  const int v = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (v >= nv) return;
-  const int base = vs_ptr[v >> 6] + (v & 63);
-  const int cnt = vs_cnt[v];
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;   // all four lanes of a vertex leave together
+  const int base = fac_slot0(vs_ptr, v, q);
+  const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int k0 = 0; k0 < cnt; k0 += 8) {
+  for (int k0 = 0; k0 < own; k0 += 8) {
     int t[8];
     v2u32 h[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int p = base + 64 * min(k0 + j, cnt - 1);   // past the end: repeats the last slot
+      const int p = base + 64 * min(k0 + j, own - 1);   // past the end: repeats the last slot
       t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
       h[j] = __builtin_nontemporal_load(vs_k + p);
     }
@@ -1974,14 +2000,16 @@ k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_
     for (int j = 0; j < 8; ++j) xv[j] = x[t[j]];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      if (k0 + j < cnt) {
+      if (k0 + j < own) {
         ax = fmaf(lo_h<VT>(h[j].x), xv[j], ax);
         ay = fmaf(hi_h<VT>(h[j].x), xv[j], ay);
         az = fmaf(lo_h<VT>(h[j].y), xv[j], az);
       }
     }
   }
-  u[v] = make_float4(ax, ay, az, 0.0f);
+  ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
+  ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
+  if (q == 0) u[v] = make_float4(ax, ay, az, 0.0f);
 }
 
 // (A^ x)_t from u: a^_t x_t + K_t0 . u_v0 + ... + K_t3 . u_v3 (FP32)
@@ -2060,6 +2088,156 @@ k_fac_power(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
   store_partial(ws, 0, s0);
   store_partial(ws, 1, s1);
   store_partial(ws, 2, s2);
+}
+
+// ------------------------------------------------------------------------------------------
+// The PCG's operator in factored form: A x = J W J^T x + alpha~ x with FP64 products and sums
+// of the FP32 gradients (the exact operator of the FP32 gradients; the assembled FP32 entries
+// of cons/mgpbd_gpu.py:273-309 round each A_ij once). Vertex pass u_v = w_v sum_t g_tv x_t
+// (four lanes per vertex over the FP32 vertex-order gradients vs_g), tet pass
+// (A x)_t = alpha~_t x_t + sum_k g_tk . u_v(t,k). With it the solve, its guessed-start and
+// restart residuals and the FP64 check all use one operator and level 0 is never assembled
+// per Newton iteration.
+// ------------------------------------------------------------------------------------------
+template <typename XT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
+              const int* __restrict__ vs_inc, const float4* __restrict__ vs_g,
+              const float* __restrict__ invm, const XT* __restrict__ x,
+              double4* __restrict__ u, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;   // all four lanes of a vertex leave together
+  const int base = fac_slot0(vs_ptr, v, q);
+  const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int k0 = 0; k0 < own; k0 += 8) {
+    int t[8];
+    float4 g[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = base + 64 * min(k0 + j, own - 1);
+      t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
+      g[j] = vs_g[p];
+    }
+    double xv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) xv[j] = (double)x[t[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (k0 + j < own) {
+        ax = fma((double)g[j].x, xv[j], ax);
+        ay = fma((double)g[j].y, xv[j], ay);
+        az = fma((double)g[j].z, xv[j], az);
+      }
+    }
+  }
+  ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
+  ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
+  if (q == 0) {
+    const double w = (double)invm[v];
+    u[v] = make_double4(w * ax, w * ay, w * az, 0.0);
+  }
+}
+
+__device__ __forceinline__ double fac_orow(int t, const int4* __restrict__ t_i,
+                                           const float* __restrict__ gradC,
+                                           const double4* __restrict__ u, double acc) {
+  const int4 vi = t_i[t];
+  const float4* g4 = reinterpret_cast<const float4*>(gradC + 12 * (size_t)t);
+  const float4 a = g4[0], b = g4[1], c = g4[2];
+  const double4 u0 = u[vi.x], u1 = u[vi.y], u2 = u[vi.z], u3 = u[vi.w];
+  acc = fma((double)a.x, u0.x, acc);
+  acc = fma((double)a.y, u0.y, acc);
+  acc = fma((double)a.z, u0.z, acc);
+  acc = fma((double)a.w, u1.x, acc);
+  acc = fma((double)b.x, u1.y, acc);
+  acc = fma((double)b.y, u1.z, acc);
+  acc = fma((double)b.z, u2.x, acc);
+  acc = fma((double)b.w, u2.y, acc);
+  acc = fma((double)c.x, u2.z, acc);
+  acc = fma((double)c.y, u3.x, acc);
+  acc = fma((double)c.z, u3.y, acc);
+  acc = fma((double)c.w, u3.z, acc);
+  return acc;
+}
+
+// MODE 0: y = A x rounded to FP32 and the partial x.y (k_pcg_spmv_dot's outputs); MODE 1:
+// y = b - A x rounded once; MODE 2: the partial (b - A x)^2 (k_true_resid's). The grid covers
+// at least n rows; every block stores its partial (modes 0 and 2).
+template <typename XT, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_fac_oapply(int n, const int4* __restrict__ t_i, const float* __restrict__ gradC,
+             const float* __restrict__ alpha, const double4* __restrict__ u,
+             const XT* __restrict__ x, const float* __restrict__ b, float* __restrict__ y,
+             RedWS ws, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  double part = 0.0;
+  if (i < n) {
+    const double xi = (double)x[i];
+    const double ax = fac_orow(i, t_i, gradC, u, (double)alpha[i] * xi);
+    if (MODE == 0) {
+      const float af = (float)ax;
+      y[i] = af;
+      part = xi * (double)af;
+    } else if (MODE == 1) {
+      y[i] = (float)((double)b[i] - ax);
+    } else {
+      const double ri = (double)b[i] - ax;
+      part = ri * ri;
+    }
+  }
+  if (MODE != 1) store_partial(ws, 0, block_sum(part));
+}
+
+// ------------------------------------------------------------------------------------------
+// Level 1 from the gradients. With P[i, agg[i]] = 1, A1 = P^T A P = G W G^T + P^T diag(alpha~) P
+// where G_(I,v) = sum of g_tv over the tets t of aggregate I holding vertex v. Per (aggregate,
+// vertex) pair: G and w_v (k_fac_gsum); per level-1 entry (I, J): the sum over the vertices
+// shared by I and J of w_v G_(I,v) . G_(J,v), plus sum_{i in I} alpha~_i on the diagonal
+// (k_fac_galerkin, FP64 sums rounded once, written at the entry's SELL position). This reads
+// ~2 gathered 16-byte G per shared vertex instead of the ~13 level-0 entries each level-1
+// entry sums in the refill (cons/mgpcg_gpu.py:183-212 contribution maps).
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock)
+k_fac_gsum(int ng, const int* __restrict__ gptr, const int* __restrict__ gmem,
+           const int* __restrict__ gvert, const float* __restrict__ gradC,
+           const float* __restrict__ invm, float4* __restrict__ G) {
+  const int gp = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (gp >= ng) return;
+  float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+  for (int m = gptr[gp]; m < gptr[gp + 1]; ++m) {
+    const int inc = gmem[m];
+    const float* g = gradC + 12 * (size_t)(inc >> 2) + 3 * (inc & 3);
+    sx += g[0];
+    sy += g[1];
+    sz += g[2];
+  }
+  G[gp] = make_float4(sx, sy, sz, invm[gvert[gp]]);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_fac_galerkin(int ne, const int* __restrict__ tptr, const int2* __restrict__ terms,
+               const float4* __restrict__ G, const int* __restrict__ cpos,
+               const int* __restrict__ cagg, const int* __restrict__ mptr,
+               const int* __restrict__ mem, const float* __restrict__ alpha,
+               float* __restrict__ cval) {
+  const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (c >= ne) return;
+  double acc = 0.0;
+  for (int m = tptr[c]; m < tptr[c + 1]; ++m) {
+    const int2 ab = terms[m];
+    const float4 ga = G[ab.x], gb = G[ab.y];
+    const double d = fma((double)ga.x, (double)gb.x,
+                         fma((double)ga.y, (double)gb.y, (double)ga.z * (double)gb.z));
+    acc = fma((double)ga.w, d, acc);
+  }
+  const int I = cagg[c];
+  if (I >= 0)
+    for (int m = mptr[I]; m < mptr[I + 1]; ++m) acc += (double)alpha[mem[m]];
+  cval[cpos[c]] = (float)acc;
 }
 
 }  // namespace mgpbd
@@ -2152,9 +2330,24 @@ struct mgpbd_ctx {
     const int* vs_cnt = nullptr;
     const int* vs_inc = nullptr;
     v2u32* vs_k = nullptr;   // K per vertex slot (4 halves)
+    float4* vs_g = nullptr;  // FP32 gradient per vertex slot (the PCG's operator)
     v2u32* tk = nullptr;     // K per tet, three planes of n_tets
     float* ahat = nullptr;   // a^ per tet
-    float4* u = nullptr;     // per-vertex products of the gather pass
+    float4* u = nullptr;     // per-vertex products of the V-cycle's gather pass
+    double4* u64 = nullptr;  // per-vertex products of the PCG operator's gather pass
+    float* dinv = nullptr;   // 1 / A_tt from the gradients (level 0's dinv array)
+    // level 1 from the gradients (mgpbd_mg_set_level1_factored)
+    bool galerkin = false;
+    int ng = 0, ne = 0;
+    const int* gptr = nullptr;
+    const int* gmem = nullptr;
+    const int* gvert = nullptr;
+    float4* G = nullptr;
+    const int* tptr = nullptr;
+    const int2* terms = nullptr;
+    const int* cpos = nullptr;
+    const int* cagg = nullptr;
+    float* cval = nullptr;
   } fac;
   int precision = MGPBD_PREC_FP32;   // V-cycle storage precision
   int flexible = 0;                   // Polak-Ribiere beta
@@ -2913,8 +3106,8 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
   const mgpbd_ctx::Factored& F = ctx->fac;
   const MgLevel& L = ctx->L[0];
   hipStream_t s = ctx->stream;
-  k_fac_gather<VT><<<nblocks(F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k,
-                                                    x, F.u, done);
+  k_fac_gather<VT><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
+      F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k, x, F.u, done);
   const int g = nblocks(L.n);
   const float w = ctx->omega;
 #define FAC(M) k_fac_apply<VT, M><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b, w, \
@@ -3096,11 +3289,11 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     const bool fac = l == 0 && F.on && ctx->precision != MGPBD_PREC_FP32;
     for (int it = 0; it < iters; ++it) {
       if (fac && ctx->precision == MGPBD_PREC_FP16) {
-        k_fac_gather<1><<<nblocks(F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
+        k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
                                                          F.vs_k, v, F.u, nullptr);
         k_fac_power<1><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
       } else if (fac) {
-        k_fac_gather<2><<<nblocks(F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
+        k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
                                                          F.vs_k, v, F.u, nullptr);
         k_fac_power<2><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
       } else if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
@@ -3233,7 +3426,8 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
                                  const float* v_invm, const float* gradC,
                                  const float* alpha_tilde, const int32_t* vs_ptr,
                                  const int32_t* vs_cnt, const int32_t* vs_inc, uint32_t* vs_k,
-                                 uint32_t* tk, float* ahat, float* u) {
+                                 float* vs_g, uint32_t* tk, float* ahat, float* u, double* u64,
+                                 float* dinv) {
   CTX_CHECK("mgpbd_mg_set_level0_factored");
   if (ctx->n_levels < 1 || ctx->L[0].n <= 0)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: level 0 not set");
@@ -3243,12 +3437,13 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
     return 0;
   }
   if (n_verts <= 0 || !v_invm || !gradC || !alpha_tilde || !vs_ptr || !vs_cnt || !vs_inc ||
-      !vs_k || !tk || !ahat || !u)
+      !vs_k || !vs_g || !tk || !ahat || !u || !u64 || !dinv)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: invalid arguments");
   const auto al = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; };
-  if (!al(t_i, 16) || !al(gradC, 16) || !al(vs_k, 8) || !al(tk, 8) || !al(u, 16))
-    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: t_i, gradC and u need 16-byte, "
-                                    "vs_k and tk 8-byte alignment");
+  if (!al(t_i, 16) || !al(gradC, 16) || !al(vs_k, 8) || !al(tk, 8) || !al(u, 16) ||
+      !al(vs_g, 16) || !al(u64, 32))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: t_i, gradC, vs_g and u need "
+                                    "16-byte, u64 32-byte, vs_k and tk 8-byte alignment");
   if (!ctx->L[0].sd || !ctx->L[0].isd)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: call mgpbd_mg_set_level_half "
                                     "for level 0 first (its scales)");
@@ -3258,10 +3453,54 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
   F.invm = v_invm; F.gradC = gradC; F.alpha = alpha_tilde;
   F.vs_ptr = vs_ptr; F.vs_cnt = vs_cnt; F.vs_inc = vs_inc;
   F.vs_k = reinterpret_cast<v2u32*>(vs_k);
+  F.vs_g = reinterpret_cast<float4*>(vs_g);
   F.tk = reinterpret_cast<v2u32*>(tk);
   F.ahat = ahat;
   F.u = reinterpret_cast<float4*>(u);
+  F.u64 = reinterpret_cast<double4*>(u64);
+  F.dinv = dinv;
+  F.galerkin = false;
   F.on = true;
+  return 0;
+}
+
+int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
+                                 const int32_t* gmem, const int32_t* gvert, float* G,
+                                 int n_entries, const int32_t* tptr, const int32_t* terms,
+                                 const int32_t* cpos, const int32_t* cagg, float* c_val) {
+  CTX_CHECK("mgpbd_mg_set_level1_factored");
+  mgpbd_ctx::Factored& F = ctx->fac;
+  if (!F.on || ctx->n_levels < 2 || !ctx->L[0].mptr)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level1_factored: needs the factored level 0 "
+                                    "(mgpbd_mg_set_level0_factored) and a level 1");
+  if (!gptr) {
+    F.galerkin = false;
+    return 0;
+  }
+  if (n_pairs <= 0 || n_entries <= 0 || !gmem || !gvert || !G || !tptr || !terms || !cpos ||
+      !cagg || !c_val || (reinterpret_cast<uintptr_t>(G) % 16) != 0 ||
+      (reinterpret_cast<uintptr_t>(terms) % 8) != 0)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level1_factored: invalid arguments (G 16-byte, "
+                                    "terms 8-byte aligned)");
+  F.ng = n_pairs; F.gptr = gptr; F.gmem = gmem; F.gvert = gvert;
+  F.G = reinterpret_cast<float4*>(G);
+  F.ne = n_entries; F.tptr = tptr; F.terms = reinterpret_cast<const int2*>(terms);
+  F.cpos = cpos; F.cagg = cagg; F.cval = c_val;
+  F.galerkin = true;
+  return 0;
+}
+
+int mgpbd_mg_factored_refill(mgpbd_ctx* ctx) {
+  CTX_CHECK("mgpbd_mg_factored_refill");
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  if (!F.on || !F.galerkin)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_factored_refill: call mgpbd_mg_set_level1_factored first");
+  hipStream_t s = ctx->stream;
+  const MgLevel& L = ctx->L[0];
+  k_fac_gsum<<<nblocks(F.ng), kBlock, 0, s>>>(F.ng, F.gptr, F.gmem, F.gvert, F.gradC, F.invm, F.G);
+  k_fac_galerkin<<<nblocks(F.ne), kBlock, 0, s>>>(F.ne, F.tptr, F.terms, F.G, F.cpos, F.cagg,
+                                                  L.mptr, L.mem, F.alpha, F.cval);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_factored_refill");
   return 0;
 }
 
@@ -3273,22 +3512,23 @@ int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
     MgLevel& L = ctx->L[l];
     const bool fac = l == 0 && ctx->fac.on;
     if (!L.hval && !L.hpk && !fac) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");
-    k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
-    if (fac) {
+    if (fac) {   // scales and K from the gradients (level 0 is not assembled per iteration)
       const mgpbd_ctx::Factored& F = ctx->fac;
+      const int gv = nblocks((int64_t)kFacLanes * F.nv);
       if (ctx->precision == MGPBD_PREC_FP16) {
         k_fac_tet<1><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
-                                                               L.isd, F.tk, F.ahat);
-        k_fac_vert<1><<<nblocks(F.nv), kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
-                                                                 F.invm, F.gradC, L.isd, F.vs_k);
+                                                               L.sd, L.isd, F.dinv, F.tk, F.ahat);
+        k_fac_vert<1><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.invm,
+                                                      F.gradC, L.isd, F.vs_k, F.vs_g);
       } else {
         k_fac_tet<2><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
-                                                               L.isd, F.tk, F.ahat);
-        k_fac_vert<2><<<nblocks(F.nv), kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
-                                                                 F.invm, F.gradC, L.isd, F.vs_k);
+                                                               L.sd, L.isd, F.dinv, F.tk, F.ahat);
+        k_fac_vert<2><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.invm,
+                                                      F.gradC, L.isd, F.vs_k, F.vs_g);
       }
       continue;
     }
+    k_half_scales<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.diag_pos, L.val, L.sd, L.isd);
     const int g = nblocks(64LL * L.n_slices);
     const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
     if (L.woff && L.hpk && L.segs && ctx->precision == MGPBD_PREC_FP16)
@@ -3344,6 +3584,34 @@ int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b, const flo
   return 0;
 }
 
+// The PCG's level-0 operator in factored form (vertex pass, tet pass), modes of k_fac_oapply;
+// x in FP32 (xf) or FP64 (xd); grid gs >= the row blocks (its partials when mode != 1).
+static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* xd,
+                      const float* b, float* y, const int* done, int gs) {
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  const int n = ctx->L[0].n;
+  hipStream_t s = ctx->stream;
+  const int gv = nblocks((int64_t)kFacLanes * F.nv);
+  if (xd) {
+    k_fac_ogather<double><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm,
+                                                xd, F.u64, done);
+    k_fac_oapply<double, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xd, b, y,
+                                                  ctx->ws, done);
+    return;
+  }
+  k_fac_ogather<float><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm,
+                                             xf, F.u64, done);
+  if (mode == 0)
+    k_fac_oapply<float, 0><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
+                                                 ctx->ws, done);
+  else if (mode == 1)
+    k_fac_oapply<float, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
+                                                 ctx->ws, done);
+  else
+    k_fac_oapply<float, 2><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
+                                                 ctx->ws, done);
+}
+
 // One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
 // z = M^-1 r for the PCG (first sweep included), FP32 or scaled reduced-precision V-cycle.
 static float* pcg_precondition(mgpbd_ctx* ctx, float* r, float* xv, bool first) {
@@ -3366,7 +3634,9 @@ static void pcg_iteration(mgpbd_ctx* ctx, float* x, double* x64, float* r, float
   hipStream_t s = ctx->stream;
   int pe = prof ? prof_begin(ctx, MGPBD_PROF_SPMV_L0) : -1;
   const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
-  if (L.woff && x64)
+  if (ctx->fac.on)
+    fac_outer(ctx, 0, p, nullptr, nullptr, Ap, &ctx->st->done, gs);
+  else if (L.woff && x64)
     k_pcg_spmv_dot_w<true><<<gs, kBlock, lds, s>>>(n, L.n_slices, L.slice_ptr, L.woff, L.val,
                                                    L.seg_ptr, L.segs, p, Ap, ctx->ws, ctx->st);
   else if (L.woff)
@@ -3483,12 +3753,18 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     if (from_guess) {
       // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
       // start with the smallest residual; a restart of the FP64 iterate takes r = b - A x64
-      if (x64 && (restarts > 0 || rescued))
-        k_resid64<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x64, b, r);
-      else
-        sell_launch<2>(ctx, L, x, r, nullptr, b, 0.f, nullptr, nullptr);
-      if (alt)
-        sell_launch<2>(ctx, L, alt, Ap, nullptr, b, 0.f, nullptr, nullptr);
+      if (ctx->fac.on) {
+        if (x64 && (restarts > 0 || rescued)) fac_outer(ctx, 1, nullptr, x64, b, r, nullptr, gs);
+        else fac_outer(ctx, 1, x, nullptr, b, r, nullptr, gs);
+        if (alt) fac_outer(ctx, 1, alt, nullptr, b, Ap, nullptr, gs);
+      } else {
+        if (x64 && (restarts > 0 || rescued))
+          k_resid64<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x64, b, r);
+        else
+          sell_launch<2>(ctx, L, x, r, nullptr, b, 0.f, nullptr, nullptr);
+        if (alt)
+          sell_launch<2>(ctx, L, alt, Ap, nullptr, b, 0.f, nullptr, nullptr);
+      }
       k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, alt ? Ap : nullptr, ctx->ws);
       k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, alt != nullptr, ctx->st);
       k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, alt, Ap, L.dinv,
@@ -3579,7 +3855,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     if (!ctx->verify) break;
     // FP64 residual of the returned x (every solve); a restart from x when a converged one
     // misses the tolerance
-    if (L.woff)
+    if (ctx->fac.on)
+      fac_outer(ctx, 2, x, nullptr, b, nullptr, nullptr, gs);
+    else if (L.woff)
       k_true_resid_w<<<gs, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
           n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, x, b, ctx->ws);
     else

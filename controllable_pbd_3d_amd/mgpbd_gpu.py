@@ -34,9 +34,10 @@ last entry repeating), ``f_ext``
 substep's lambda, or from the network's prediction with the previous lambda as the fallback;
 the device keeps the start with the smaller residual and iterates to the same tolerance.
 ``record_samples`` (a list) collects training pairs from the first Newton iteration.
-``level0`` ('assembled' | 'factored'; reduced precision only): the V-cycle applies level 0 as
-K K^T + diag(a^) from the tets' gradients instead of its stored half values (the matrix
-J W J^T + alpha~ in factored form; fewer bytes per sweep, no level-0 half conversion).
+``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is
+used as J W J^T + alpha~ in factored form from the tets' gradients (V-cycle sweeps from FP16/BF16
+K K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
+aggregates' summed gradients) and is not reassembled per Newton iteration.
 """
 
 from __future__ import annotations
@@ -89,8 +90,9 @@ class MGPBDSolverGPU:
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
         if level0 not in ("assembled", "factored"):
             raise ValueError("level0 must be 'assembled' or 'factored'")
-        if level0 == "factored" and precision == "fp32":
-            raise ValueError("level0='factored' is a reduced-precision V-cycle option")
+        if level0 == "factored" and (precision == "fp32" or galerkin != "structural"):
+            raise ValueError("level0='factored' needs precision 'fp16' or 'bf16' and "
+                             "galerkin='structural'")
         self.ctx = get_context(device)
         dev = self.device = self.ctx.device
         self.v_p = to_device(v_p, torch.float32, dev, (-1, 3))
@@ -197,7 +199,8 @@ class MGPBDSolverGPU:
         self._build_incidence()
         if self.level0 == "factored":
             self._gpu_solver.set_factored_level0(self.t_i, self.v_invm, self.gradC,
-                                                 self.alpha_tilde, self._vptr_np, self._vinc_np)
+                                                 self.alpha_tilde, self._vptr_np, self._vinc_np,
+                                                 t_i_host=self._t_i_np)
         self._gpu_solver.setup(self._A, self._aggs)
         if self.verbose:
             print(f"MGPBD-GPU: {self.n_verts} verts, {self.n_tets} tets, {self._nnz} nnz, "
@@ -332,7 +335,10 @@ class MGPBDSolverGPU:
                  dptr(self.gradC), dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
 
     def _fill_A_and_update_gpu(self):
-        self._fill_A()
+        # the factored level 0 reads the gradients; its assembled values are only wanted for
+        # recorded training pairs
+        if not self._gpu_solver.factored or self.record_samples is not None:
+            self._fill_A()
         self._gpu_solver.update_all_coarse_matrices()
 
     def _scatter_dlam_gpu(self):

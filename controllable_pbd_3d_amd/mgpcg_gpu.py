@@ -28,11 +28,13 @@ Extensions (keyword-only, no reference counterpart):
   the true residual above 1e-6);
 * ``stall_rescue`` (default on with the spectral smoothers): a PCG that stops contracting
   re-estimates the spectra from a fresh vector with a raised safety factor and restarts;
-* ``set_factored_level0`` (before ``setup``, reduced precision only): the V-cycle applies
-  level 0 as K K^T + diag(a^) from the tets' gradients (K_tk = sqrt(w_v) g_tk / sqrt(A_tt),
-  stored FP16/BF16) instead of its stored half values, streaming ~12 bytes per vertex-tet
-  incidence and ~44 per tet instead of ~4 per matrix entry; the PCG's own A p keeps the
-  assembled FP32 level.
+* ``set_factored_level0`` (before ``setup``, reduced precision and the structural Galerkin
+  pattern only): level 0 is used in factored form A = J W J^T + diag(alpha~) from the tets'
+  gradients and never reassembled: the V-cycle applies K K^T + diag(a^) (K_tk = sqrt(w_v) g_tk
+  / sqrt(A_tt), stored FP16/BF16; ~12 bytes per vertex-tet incidence and ~44 per tet instead of
+  ~4 per matrix entry), the PCG and its FP64 check apply J W J^T + alpha~ with FP64 sums of the
+  FP32 gradients (~180 MB per product at 1M tets instead of ~450 MB), and level 1 is
+  G W G^T + P^T diag(alpha~) P with G the aggregates' summed gradients per vertex.
 """
 
 from __future__ import annotations
@@ -57,27 +59,78 @@ def _members(agg: np.ndarray, n_c: int):
 
 
 def vertex_slices(vptr: np.ndarray, vinc: np.ndarray):
-    """Vertex-sliced incidence of the factored level 0: vertices binned 64 per slice, slot k
-    of vertex 64 s + l at vs_ptr[s] + 64 k + l for k < vs_cnt[v] (the vertex's incidences
-    4 tet + local index in vinc order), slices as wide as their busiest vertex. Returns
-    (vs_ptr int32[n_slices + 1], vs_cnt int32[n_verts], vs_inc int32[vs_ptr[-1]])."""
+    """Vertex-sliced incidence of the factored level 0, four lanes per vertex: vertices binned
+    16 per 64-lane slice, slot k of vertex 16 s + l at vs_ptr[s] + 64 (k // 4) + 4 l + k % 4
+    for k < vs_cnt[v] (the vertex's incidences 4 tet + local index, in vinc order); a slice
+    holds ceil(busiest vertex / 4) rows of 64 slots. Returns (vs_ptr int32[n_slices + 1],
+    vs_cnt int32[n_verts], vs_inc int32[vs_ptr[-1]], zero where unused)."""
     vptr = np.asarray(vptr, np.int64)
     nv = len(vptr) - 1
     cnt = np.diff(vptr)
-    ns = (nv + 63) // 64
-    padded = np.zeros(64 * ns, np.int64)
+    ns = (nv + 15) // 16
+    padded = np.zeros(16 * ns, np.int64)
     padded[:nv] = cnt
-    width = padded.reshape(ns, 64).max(axis=1) if ns else np.zeros(0, np.int64)
+    rows = (padded.reshape(ns, 16).max(axis=1) + 3) // 4 if ns else np.zeros(0, np.int64)
     vs_ptr = np.zeros(ns + 1, np.int64)
-    vs_ptr[1:] = np.cumsum(64 * width)
-    v_of = np.repeat(np.arange(nv, dtype=np.int64), cnt)
-    k_of = np.arange(len(v_of), dtype=np.int64) - np.repeat(vptr[:-1], cnt)
-    pos = vs_ptr[v_of // 64] + 64 * k_of + v_of % 64
-    vs_inc = np.zeros(max(int(vs_ptr[-1]), 1), np.int32)
-    vs_inc[pos] = np.asarray(vinc, np.int32)[:len(pos)]
+    vs_ptr[1:] = np.cumsum(64 * rows)
     if vs_ptr[-1] >= 2 ** 31:
         raise ValueError("vertex incidence exceeds 32-bit positions")
+    v_of = np.repeat(np.arange(nv, dtype=np.int64), cnt)
+    k_of = np.arange(len(v_of), dtype=np.int64) - np.repeat(vptr[:-1], cnt)
+    pos = vs_ptr[v_of // 16] + 64 * (k_of // 4) + 4 * (v_of % 16) + k_of % 4
+    vs_inc = np.zeros(max(int(vs_ptr[-1]), 1), np.int32)
+    vs_inc[pos] = np.asarray(vinc, np.int32)[:len(pos)]
     return vs_ptr.astype(np.int32), cnt.astype(np.int32), vs_inc
+
+
+def factored_galerkin_maps(t_i, n_verts: int, agg, c_indptr, c_indices):
+    """Maps of level 1 from the gradients (A1 = G W G^T + P^T diag(alpha~) P). Returns
+    gptr, gmem, gvert: the (aggregate, vertex) pairs in ascending (aggregate, vertex) order with
+    their incidences 4 tet + local vertex (ascending) and vertex; tptr, terms: per level-1 CSR
+    entry (I, J) the pairs (p_Iv, p_Jv) of the vertices v that I and J share, in vertex order;
+    cagg: I for a diagonal entry, else -1. Raises when the level-1 pattern lacks a shared-vertex
+    entry or holds an entry with no shared vertex (the pattern must be the structural one)."""
+    tv = np.asarray(t_i, np.int64).reshape(-1, 4)
+    nt = tv.shape[0]
+    inc = np.arange(4 * nt, dtype=np.int64)
+    agg = np.asarray(agg, np.int64)
+    key = agg[inc >> 2] * n_verts + tv.reshape(-1)
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    first = np.ones(len(ks), bool)
+    first[1:] = ks[1:] != ks[:-1]
+    starts = np.flatnonzero(first)
+    pair_key = ks[starts]
+    ng = len(pair_key)
+    gptr = np.append(starts, len(ks)).astype(np.int64)
+    gmem = inc[order]
+    gI, gv = pair_key // n_verts, pair_key % n_verts
+    pv = np.argsort(gv, kind="stable")          # pairs by vertex, aggregates ascending within
+    m = np.bincount(gv, minlength=n_verts).astype(np.int64)
+    vstart = np.concatenate([[0], np.cumsum(m)])
+    m_each = m[gv[pv]]
+    T = int(m_each.sum())
+    a_pos = np.repeat(np.arange(ng, dtype=np.int64), m_each)
+    offs = np.arange(T, dtype=np.int64) - np.repeat(np.cumsum(m_each) - m_each, m_each)
+    b_pos = np.repeat(vstart[gv[pv]], m_each) + offs
+    a, b = pv[a_pos], pv[b_pos]
+    n1 = len(c_indptr) - 1
+    rows = np.repeat(np.arange(n1, dtype=np.int64), np.diff(c_indptr))
+    ckey = rows * n1 + np.asarray(c_indices, np.int64)
+    tkey = gI[a] * n1 + gI[b]
+    c = np.searchsorted(ckey, tkey)
+    if np.any(c >= len(ckey)) or np.any(ckey[np.minimum(c, len(ckey) - 1)] != tkey):
+        raise ValueError("level-1 pattern lacks an entry of two aggregates sharing a vertex")
+    o = np.argsort(c, kind="stable")
+    cnt = np.bincount(c, minlength=len(ckey))
+    if np.any(cnt == 0):
+        raise ValueError("a level-1 entry has no shared vertex")
+    terms = np.stack([a[o], b[o]], axis=1)
+    tptr = np.concatenate([[0], np.cumsum(cnt)])
+    if tptr[-1] >= 2 ** 31 or len(gmem) >= 2 ** 31:
+        raise ValueError("level-1 maps exceed 32-bit offsets")
+    cagg = np.where(rows == np.asarray(c_indices, np.int64), rows, -1)
+    return gptr, gmem, gv, tptr, terms, cagg
 
 
 def _aggs_from_Ps(Ps):
@@ -156,17 +209,27 @@ class GPUMultigridSolver:
         self._fac_src = None
         self._fac = None
 
-    def set_factored_level0(self, t_i, v_invm, gradC, alpha_tilde, vptr, vinc):
-        """Level 0 of the reduced-precision V-cycle applied in factored form (extension).
-        Call before ``setup``. The device tensors t_i (int32, 4 per tet), v_invm, gradC (12
-        floats per tet) and alpha_tilde are read at every ``update_all_coarse_matrices`` and
-        must stay allocated; vptr / vinc: the host vertex incidence (4 tet + local index)."""
+    def set_factored_level0(self, t_i, v_invm, gradC, alpha_tilde, vptr, vinc, t_i_host=None):
+        """Level 0 in factored form (extension; see the module notes). Call before ``setup``.
+        The device tensors t_i (int32, 4 per tet), v_invm, gradC (12 floats per tet) and
+        alpha_tilde are read at every ``update_all_coarse_matrices`` and every solve and must
+        stay allocated; vptr / vinc: the host vertex incidence (4 tet + local index)."""
         if self.precision == "fp32":
-            raise ValueError("the factored level 0 is a reduced-precision V-cycle option")
+            raise ValueError("the factored level 0 is a reduced-precision option")
+        if self.galerkin != "structural":
+            raise ValueError("the factored level 0 needs galerkin='structural' (level 1 from "
+                             "the gradients fills every shared-vertex entry)")
         if self._initialized:
             raise RuntimeError("set_factored_level0 must precede setup()")
+        th = t_i.cpu().numpy() if t_i_host is None else np.asarray(t_i_host)
         self._fac_src = dict(t_i=t_i, v_invm=v_invm, gradC=gradC, alpha_tilde=alpha_tilde,
-                             vptr=np.asarray(vptr), vinc=np.asarray(vinc))
+                             vptr=np.asarray(vptr), vinc=np.asarray(vinc),
+                             t_i_host=np.ascontiguousarray(th, np.int32).reshape(-1))
+
+    @property
+    def factored(self) -> bool:
+        """Level 0 runs factored (its stored values are not read by the solve)."""
+        return self._fac is not None
 
     def setup(self, A_level0, Ps_or_aggs: list):
         """Hierarchy from the fine matrix and prolongations (cons/mgpcg_gpu.py:30-173).
@@ -312,8 +375,9 @@ class GPUMultigridSolver:
                 self._set_window(l, win, hpk=True)
 
     def _setup_factored(self, win):
-        """Level 0 without stored half values: scales, the FP32 kernels' window, and the
-        factored operator's buffers."""
+        """Level 0 without stored half values or LDS window (nothing reads its stored values
+        while it runs factored): scale arrays, the factored operators' buffers and level 1's
+        maps from the gradients."""
         dev = self.device
         L = self.levels[0]
         src = self._fac_src
@@ -324,24 +388,36 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_mg_set_level_half", 0, None, None, dptr(hb["sd"]), dptr(hb["isd"]),
                  dptr(L.diag_pos))
         self._half.append(hb)
-        if win is not None:
-            self._set_window(0, win, hpk=False)
         vs_ptr, vs_cnt, vs_inc = vertex_slices(src["vptr"], src["vinc"])
         nv, nt = len(vs_cnt), L.n_rows
-        self._fac = {
+        f = self._fac = {
             "vs_ptr": torch.from_numpy(vs_ptr).to(dev),
             "vs_cnt": torch.from_numpy(vs_cnt).to(dev),
             "vs_inc": torch.from_numpy(vs_inc).to(dev),
             "vs_k": torch.zeros(2 * len(vs_inc), dtype=torch.int32, device=dev),
+            "vs_g": torch.zeros(4 * len(vs_inc), dtype=torch.float32, device=dev),
             "tk": torch.zeros(6 * nt, dtype=torch.int32, device=dev),
             "ahat": torch.zeros(nt, dtype=torch.float32, device=dev),
             "u": torch.zeros(4 * nv, dtype=torch.float32, device=dev),
+            "u64": torch.zeros(4 * nv, dtype=torch.float64, device=dev),
         }
-        f = self._fac
         self.ctx("mgpbd_mg_set_level0_factored", nv, dptr(src["t_i"]), dptr(src["v_invm"]),
                  dptr(src["gradC"]), dptr(src["alpha_tilde"]), dptr(f["vs_ptr"]),
-                 dptr(f["vs_cnt"]), dptr(f["vs_inc"]), dptr(f["vs_k"]), dptr(f["tk"]),
-                 dptr(f["ahat"]), dptr(f["u"]))
+                 dptr(f["vs_cnt"]), dptr(f["vs_inc"]), dptr(f["vs_k"]), dptr(f["vs_g"]),
+                 dptr(f["tk"]), dptr(f["ahat"]), dptr(f["u"]), dptr(f["u64"]),
+                 dptr(L.diag_inv))
+        if self.n_levels < 2:
+            return
+        C = self.levels[1]
+        gptr, gmem, gvert, tptr, terms, cagg = factored_galerkin_maps(
+            src["t_i_host"], nv, self.transfers[0]["agg"].cpu().numpy(), C.indptr, C.indices)
+        up = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
+        f.update(gptr=up(gptr), gmem=up(gmem), gvert=up(gvert), tptr=up(tptr),
+                 terms=up(terms.reshape(-1)), cagg=up(cagg), cpos=up(C.sell_pos_host),
+                 G=torch.zeros(4 * max(len(gvert), 1), dtype=torch.float32, device=dev))
+        self.ctx("mgpbd_mg_set_level1_factored", len(gvert), dptr(f["gptr"]), dptr(f["gmem"]),
+                 dptr(f["gvert"]), dptr(f["G"]), len(cagg), dptr(f["tptr"]), dptr(f["terms"]),
+                 dptr(f["cpos"]), dptr(f["cagg"]), dptr(C.data))
 
     def level_residual(self, level: int, b: torch.Tensor, x: torch.Tensor, r: torch.Tensor):
         """r = b - A_l x with the operator the V-cycle applies on level l (scaled variables in
@@ -408,7 +484,9 @@ class GPUMultigridSolver:
         (cons/mgpcg_gpu.py:183-212)."""
         for l, rf in enumerate(self.refill):
             F, C = self.levels[l], self.levels[l + 1]
-            if rf["n"] > 0:
+            if l == 0 and self._fac is not None:
+                self.ctx("mgpbd_mg_factored_refill")
+            elif rf["n"] > 0:
                 self.ctx("mgpbd_coarse_refill", rf["n"], dptr(rf["off"]), dptr(rf["fpos"]),
                          dptr(rf["cpos"]), dptr(F.data), dptr(C.data))
             C.cache_diag_inv()

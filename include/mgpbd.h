@@ -366,27 +366,53 @@ int mgpbd_mg_set_level_half(mgpbd_ctx* ctx, int level, uint16_t* hval_dev, uint3
 int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr_dev,
                               const int32_t* segs_dev, int n_blocks, int max_segs,
                               uint16_t* woff_dev);
-/* Factored level 0 for the reduced-precision V-cycle (extension; no reference counterpart --
- * the reference stores level 0 only as the assembled CSR of cons/mgpbd_gpu.py:273-324). The
- * level-0 matrix is A = J W J^T + diag(alpha~) with J's rows the tets' vertex gradients, so
- * its scaled form is K K^T + diag(a^), K_tk = sqrt(invm_v) gradC_tk / sqrt(A_tt),
- * a^_t = alpha~_t / A_tt; the V-cycle's level-0 sweeps, residuals and power steps then apply
- * it in two passes (per-vertex gather of K x, per-tet product) from K stored FP16/BF16 instead
- * of the level's stored half values, which need not exist (mgpbd_mg_set_level_half(ctx, 0,
- * NULL, NULL, sd, isd, diag_pos) sets the scales only). mgpbd_mg_convert_half recomputes K
- * and a^ from the arrays' current contents. Caller-owned device arrays, valid while used:
- * t_i (int32[4 n_tets], 16-byte aligned), v_invm (n_verts), gradC (float[12 n_tets], 16-byte
- * aligned), alpha_tilde (n_tets); the vertex-sliced incidence: slot k of vertex 64 s + l at
- * vs_ptr[s] + 64 k + l for k < vs_cnt[v], holding vs_inc = 4 tet + local vertex index;
- * outputs vs_k (2 uint32 per slot), tk (6 uint32 per tet), ahat (n_tets floats), u (4 floats
- * per vertex, 16-byte aligned). t_i NULL switches the factored operator off. Level 0 must be
- * set (mgpbd_mg_set_level) and its scales given (mgpbd_mg_set_level_half) first; setting
- * level 0 again switches it off. */
+/* Factored level 0 for the reduced-precision solve (extension; no reference counterpart --
+ * the reference assembles level 0 as the CSR of cons/mgpbd_gpu.py:273-324 every Newton
+ * iteration and its PCG multiplies that CSR, cons/mgpcg_gpu.py:261-329). The level-0 matrix
+ * is A = J W J^T + diag(alpha~) with J's rows the tets' vertex gradients, so
+ *  - the V-cycle's level-0 sweeps, residuals and power steps apply its scaled form
+ *    K K^T + diag(a^), K_tk = sqrt(invm_v) gradC_tk / sqrt(A_tt), a^_t = alpha~_t / A_tt,
+ *    from K stored FP16/BF16 (two passes: per-vertex gather of K x, per-tet product);
+ *  - the PCG's A p, its guessed-start and restart residuals and the FP64 check apply A itself
+ *    with FP64 products and sums of the FP32 gradients (the exact operator of those
+ *    gradients, where the assembled FP32 entries round each A_ij once);
+ *  - mgpbd_mg_convert_half takes A_tt = sum_k invm |gradC_tk|^2 + alpha~_t, the scales and
+ *    K from the arrays' current contents and writes 1 / A_tt to dinv.
+ * So level 0's stored values are not read while it runs factored (they need not be current),
+ * and its half values need not exist (mgpbd_mg_set_level_half(ctx, 0, NULL, NULL, sd, isd,
+ * diag_pos) sets the scale arrays only). Caller-owned device arrays, valid while used: t_i
+ * (int32[4 n_tets], 16-byte aligned), v_invm (n_verts), gradC (float[12 n_tets], 16-byte
+ * aligned), alpha_tilde (n_tets); the vertex-sliced incidence, four lanes per vertex: slot k of
+ * vertex 16 s + l at vs_ptr[s] + 64 (k / 4) + 4 l + k % 4 for k < vs_cnt[v], holding vs_inc =
+ * 4 tet + local vertex index; outputs vs_k (2 uint32 per slot), vs_g (4 floats per slot,
+ * 16-byte aligned), tk (6 uint32 per tet), ahat (n_tets floats), u (4 floats per vertex,
+ * 16-byte aligned), u64 (4 doubles per vertex, 32-byte aligned), dinv (n_tets floats). Needs
+ * a reduced precision (mgpbd_mg_set_precision). t_i NULL switches the factored operator off.
+ * Level 0 must be set (mgpbd_mg_set_level) and its scale arrays given
+ * (mgpbd_mg_set_level_half) first; setting level 0 again switches it off. */
 int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i,
                                  const float* v_invm, const float* gradC,
                                  const float* alpha_tilde, const int32_t* vs_ptr,
                                  const int32_t* vs_cnt, const int32_t* vs_inc, uint32_t* vs_k,
-                                 uint32_t* tk, float* ahat, float* u);
+                                 float* vs_g, uint32_t* tk, float* ahat, float* u, double* u64,
+                                 float* dinv);
+/* Level 1 from the gradients with the factored level 0 (extension; replaces the level 0 -> 1
+ * refill of cons/mgpcg_gpu.py:183-212 when level 0 is not assembled): with P[i, agg[i]] = 1,
+ * A1 = G W G^T + P^T diag(alpha~) P, G_(I,v) = sum of gradC over the tets of aggregate I
+ * holding vertex v. Per (aggregate, vertex) pair p < n_pairs: its incidences (4 tet + local
+ * vertex) gmem[gptr[p] .. gptr[p+1]) and vertex gvert[p]; G (4 floats per pair, 16-byte
+ * aligned) is written by mgpbd_mg_factored_refill. Per level-1 CSR entry c < n_entries: the
+ * pairs (a, b) of the vertices its two aggregates share, terms[2 m], terms[2 m + 1] for m in
+ * [tptr[c], tptr[c+1]) (8-byte aligned), its SELL position cpos[c] in c_val (level 1's
+ * values) and cagg[c] = the aggregate when c is a diagonal entry, else -1 (the diagonal adds
+ * the aggregate's alpha~ over level 0's member lists). gptr NULL switches it off. */
+int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
+                                 const int32_t* gmem, const int32_t* gvert, float* G,
+                                 int n_entries, const int32_t* tptr, const int32_t* terms,
+                                 const int32_t* cpos, const int32_t* cagg, float* c_val);
+/* Level 1's values from the current gradients and alpha~ (FP64 sums rounded once; padding
+ * slots untouched). */
+int mgpbd_mg_factored_refill(mgpbd_ctx* ctx);
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx);

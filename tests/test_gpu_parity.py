@@ -462,6 +462,59 @@ def _window_columns(L, hb, woff=None):
     return segs[seg_ptr[rows // 256] + k] * 32 + (off // 4) % 32
 
 
+def _fp64_level0(s):
+    """A = J W J^T + alpha~ assembled in FP64 (the oracle's fill) from the solver's current
+    FP32 gradients: the system the factored level 0 solves."""
+    L64, R64, _ = om.lib("f64")
+    A = s._A
+    data = np.zeros(len(A.indices))
+    g = s.gradC.cpu().numpy().astype(np.float64).reshape(-1)
+    invm = s.v_invm.cpu().numpy().astype(np.float64)
+    al = s.alpha_tilde.cpu().numpy().astype(np.float64)
+    L64.orc_fill_A(s.n_tets, om.ptr(A.indptr, om.I32P), om.ptr(A.indices, om.I32P),
+                   om.ptr(s._t_i_np, om.I32P), om.ptr(invm, R64), om.ptr(g, R64),
+                   om.ptr(al, R64), om.ptr(data, R64))
+    return sp.csr_matrix((data, A.indices, A.indptr), shape=(s.n_tets, s.n_tets))
+
+
+def test_factored_level1_and_diagonal_from_gradients():
+    """With the factored level 0, level 1 comes from the aggregates' summed gradients: equal to
+    P^T A P of the FP64-assembled A (rounded once: 1e-6 of the largest entry), padding slots
+    zero; level 0's 1/diag from the gradients equals the assembled one (1e-6); after a substep
+    (no level-0 reassembly) both still match the FP64 assembly of the new gradients."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(6, seed=5)
+    v0 = deformed(mesh, seed=5, amp=0.05, twist=0.1)
+    s = _solver(mesh, v0, mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended",
+                galerkin="structural", max_coarse=64, coarse_solver="cholesky",
+                relative_tol=True, tol_cg=1e-6, precision="fp16", level0="factored")
+    mg = s._gpu_solver
+    assert mg.factored
+    for rnd in range(2):
+        A64 = _fp64_level0(s)
+        agg = mg.transfers[0]["agg"].cpu().numpy()
+        P = sp.csr_matrix((np.ones(len(agg)), (np.arange(len(agg)), agg)),
+                          shape=(len(agg), mg.levels[1].n_rows))
+        A1 = (P.T @ A64 @ P).tocsr()
+        C = mg.levels[1]
+        got = C.to_scipy().tocsr()
+        rows = np.repeat(np.arange(C.n_rows), np.diff(C.indptr))
+        want = np.asarray(A1[rows, C.indices]).reshape(-1)
+        gv = np.asarray(got[rows, C.indices]).reshape(-1)
+        err = np.abs(gv - want).max() / np.abs(want).max()
+        dinv = mg.levels[0].diag_inv.cpu().numpy().astype(np.float64)
+        derr = np.abs(dinv * A64.diagonal() - 1.0).max()
+        pad = np.ones(C.data.numel(), bool)
+        pad[C.sell_pos_host] = False
+        print(f"[factored] round {rnd}: level-1 rel err {err:.2e}, 1/diag err {derr:.2e}",
+              flush=True)
+        assert err <= 1e-6 and derr <= 1e-6
+        assert np.all(C.data.cpu().numpy()[pad] == 0.0)
+        s.step(0)
+        s._compute_C_and_gradC()
+        mg.update_all_coarse_matrices()
+
+
 def _decode_half(raw, precision):
     return raw.view(np.float16).astype(np.float64) if precision == "fp16" else \
         _bf16_to_f32(raw).astype(np.float64)
@@ -503,7 +556,7 @@ def test_factored_level0_operator(precision):
     vk = f["vs_k"].cpu().numpy().view(np.uint16).reshape(-1, 4)
     v = np.repeat(np.arange(nv), vs_cnt)
     k = np.arange(len(v)) - np.repeat(np.concatenate([[0], np.cumsum(vs_cnt)[:-1]]), vs_cnt)
-    pos = vs_ptr[v // 64] + 64 * k + v % 64
+    pos = vs_ptr[v // 16] + 64 * (k // 4) + 4 * (v % 16) + k % 4
     inc = vs_inc[pos]
     assert np.array_equal(mesh.tets.reshape(-1, 4)[inc >> 2, inc & 3], v)
     tk3 = tk.reshape(nt, 4, 3)
@@ -618,7 +671,9 @@ def test_mixed_precision_pcg_reaches_fp64_verified_tolerance(precision, level0):
     it32 = s._gpu_solver.solve(s._b_gpu, s._dlam_gpu)
     it = s2._gpu_solver.solve(s2._b_gpu, s2._dlam_gpu)
     x = s2._dlam_gpu.cpu().numpy().astype(np.float64)
-    A = s2._A.to_scipy().astype(np.float64)
+    # the factored level 0 solves J W J^T + alpha~ of the FP32 gradients exactly; the stored
+    # path the FP32-assembled entries
+    A = _fp64_level0(s2) if level0 == "factored" else s2._A.to_scipy().astype(np.float64)
     res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
     x64 = sp.linalg.spsolve(A.tocsc(), b.astype(np.float64))
     err = np.linalg.norm(x - x64) / np.linalg.norm(x64)

@@ -176,8 +176,9 @@ def test_factored_level0_equals_scaled_assembled_matrix():
 
 
 def test_vertex_slices_layout():
-    """Vertex-sliced incidence: every (vertex, k) incidence at vs_ptr[v // 64] + 64 k + v % 64,
-    slices as wide as their busiest vertex, nothing else written."""
+    """Vertex-sliced incidence (four lanes per vertex): every (vertex, k) incidence at
+    vs_ptr[v // 16] + 64 (k // 4) + 4 (v % 16) + k % 4, slices ceil(busiest / 4) rows of 64,
+    nothing else written."""
     from controllable_pbd_3d_amd.mgpcg_gpu import vertex_slices
     m = box_mesh(3, 0.1, seed=1)
     nt, nv = m.n_tets, m.n_verts
@@ -196,14 +197,59 @@ def test_vertex_slices_layout():
     assert np.array_equal(vs_cnt, np.diff(vptr))
     seen = np.zeros(len(vs_inc), bool)
     for v in range(nv):
-        s = v // 64
+        s = v // 16
         w = (vs_ptr[s + 1] - vs_ptr[s]) // 64
-        assert w == np.diff(vptr)[64 * s:64 * s + 64].max()
+        assert w == (np.diff(vptr)[16 * s:16 * s + 16].max() + 3) // 4
         for k in range(vs_cnt[v]):
-            p = vs_ptr[s] + 64 * k + v % 64
+            p = vs_ptr[s] + 64 * (k // 4) + 4 * (v % 16) + k % 4
             assert vs_inc[p] == vinc[vptr[v] + k] and tv[vs_inc[p] >> 2, vs_inc[p] & 3] == v
             seen[p] = True
     assert np.all(vs_inc[~seen] == 0)
+
+
+def test_factored_galerkin_maps_give_PtAP():
+    """Level 1 from the gradients: sum over each entry's terms of w_v G_(I,v) . G_(J,v), plus the
+    aggregate's alpha~ on the diagonal, equals P^T (J W J^T + alpha~) P (FP64) on the structural
+    Galerkin pattern, with every pattern entry covered; a term list built on the wrong vertex
+    or a missing alpha~ fails it."""
+    from controllable_pbd_3d_amd.mgpcg_gpu import factored_galerkin_maps
+    m, x, Dm, al, lam, C, g, b, _ = _state(n=3, seed=6)
+    nt, nv = m.n_tets, m.n_verts
+    invm = m.invm.astype(np.float64)
+    agg = (np.arange(nt) // 5).astype(np.int32)
+    n1 = int(agg.max()) + 1
+    J = np.zeros((nt, 3 * nv))
+    for t in range(nt):
+        for l in range(4):
+            v = m.tets[4 * t + l]
+            J[t, 3 * v:3 * v + 3] += g[12 * t + 3 * l:12 * t + 3 * l + 3]
+    A = (J * np.repeat(invm, 3)) @ J.T + np.diag(al)
+    P = np.zeros((nt, n1))
+    P[np.arange(nt), agg] = 1.0
+    A1 = P.T @ A @ P
+    indptr, indices = structure.build_sparsity_pattern(m.tets, nt)
+    S = sp.csr_matrix((np.ones(len(indices)), indices, indptr), shape=(nt, nt))
+    Ps = sp.csr_matrix(P)
+    pat = (Ps.T @ S @ Ps).tocsr()
+    pat.sort_indices()
+    gptr, gmem, gvert, tptr, terms, cagg = factored_galerkin_maps(m.tets, nv, agg, pat.indptr,
+                                                                    pat.indices)
+    G = np.zeros((len(gvert), 3))
+    for p in range(len(gvert)):
+        for inc in gmem[gptr[p]:gptr[p + 1]]:
+            assert m.tets[inc] == gvert[p]
+            G[p] += g[12 * (inc >> 2) + 3 * (inc & 3):12 * (inc >> 2) + 3 * (inc & 3) + 3]
+    rows = np.repeat(np.arange(n1), np.diff(pat.indptr))
+    got = np.zeros(len(pat.indices))
+    for c in range(len(got)):
+        for a_, b_ in terms[tptr[c]:tptr[c + 1]]:
+            assert gvert[a_] == gvert[b_]
+            got[c] += invm[gvert[a_]] * G[a_] @ G[b_]
+        if cagg[c] >= 0:
+            assert cagg[c] == rows[c] == pat.indices[c]
+            got[c] += al[agg == rows[c]].sum()
+    want = A1[rows, pat.indices]
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
 
 
 def test_scatter_equals_MinvJt():
