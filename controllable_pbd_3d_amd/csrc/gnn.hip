@@ -533,13 +533,25 @@ k_feature_sums(int n, const float* __restrict__ b, const float* __restrict__ din
   if (threadIdx.x < 8) partial[blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_feature_stats(int n, int nblk, const double* __restrict__ partial,
-                                float* __restrict__ stats) {
+// One block of 256: strided per-thread sums of the block partials, then a fixed-order tree
+// (deterministic).
+__global__ void __launch_bounds__(256)
+k_feature_stats(int n, int nblk, const double* __restrict__ partial, float* __restrict__ stats) {
   // stats: [s, sC, mean la, std la, mean ld, std ld, mean vol, unused]
-  if (threadIdx.x != 0) return;
+  __shared__ double sh[8][256];
+  const int tid = threadIdx.x;
   double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int bk = 0; bk < nblk; ++bk)
-    for (int q = 0; q < 8; ++q) t[q] += partial[bk * 8 + q];
+  for (int bk = tid; bk < nblk; bk += 256)
+    for (int q = 0; q < 8; ++q) t[q] += partial[(size_t)bk * 8 + q];
+  for (int q = 0; q < 8; ++q) sh[q][tid] = t[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int q = 0; q < 8; ++q) sh[q][tid] += sh[q][tid + w];
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  for (int q = 0; q < 8; ++q) t[q] = sh[q][0];
   const double nn = (double)(n > 0 ? n : 1);
   const double sj = sqrt(t[0] / nn), sl = sqrt(t[1] / nn);
   const double s = fmax(fmax(sj, sl), 1e-30);
@@ -752,7 +764,7 @@ int mgpbd_gnn_features(mgpbd_ctx* ctx, int n, const float* b, const float* dinv,
   carve(ws, n, W);
   const int g = std::min(nb(n), 1024);
   k_feature_sums<<<g, kBlock, 0, s>>>(n, b, dinv, lam_prev, C, alpha, vol, W.dpartial);
-  k_feature_stats<<<1, 64, 0, s>>>(n, g, W.dpartial, stats);
+  k_feature_stats<<<1, 256, 0, s>>>(n, g, W.dpartial, stats);
   k_features<<<nb(n), kBlock, 0, s>>>(n, b, dinv, lam_prev, C, alpha, vol, nbr, stats, x);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_features");
   return 0;

@@ -260,18 +260,20 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
              const float* __restrict__ C_cur, int with_current, RedWS ws) {
   double acc_cur = 0.0, acc_trial = 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    int4 tv = t_i[t];
-    float B[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
-    double F[9];
-    deformation_gradient(tv, x, dx, omega, B, F);
-    double U[3][3], s[3], V[3][3];
-    svd3(F, U, s, V);
-    float c = (float)arap_C(s);
     float al = alpha[t] * lambda[t];
-    float r = -(c + al);
-    acc_trial += (double)r * (double)r;
+    if (with_current != 2) {   // 2: the current norm only
+      int4 tv = t_i[t];
+      float B[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
+      double F[9];
+      deformation_gradient(tv, x, dx, omega, B, F);
+      double U[3][3], s[3], V[3][3];
+      svd3(F, U, s, V);
+      float c = (float)arap_C(s);
+      float r = -(c + al);
+      acc_trial += (double)r * (double)r;
+    }
     if (with_current) {
       float rc = -(C_cur[t] + al);
       acc_cur += (double)rc * (double)rc;
@@ -816,17 +818,24 @@ k_jacobi_small(int n, const int* __restrict__ slice_ptr, const int* __restrict__
 // triangle), mirrored to a full symmetric inverse, and every V-cycle applies it as a dense
 // mat-vec with FP64 accumulation.
 __global__ void __launch_bounds__(kBlock)
+k_dense_zero(int64_t len, double* __restrict__ D) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < len) D[i] = 0.0;
+}
+
+// One wavefront per row after k_dense_zero: lane k stores entry k of row i (a row's nonzero
+// columns are distinct; zero padding, whose column is the row itself, is skipped).
+__global__ void __launch_bounds__(kBlock)
 k_dense_from_sell(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
                   const float* __restrict__ val, double* __restrict__ D) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (i >= n) return;
-  for (int j = 0; j < n; ++j) D[(size_t)j * n + i] = 0.0;
   const int p0 = sell_base(slice_ptr, i);
   const int w = sell_width(slice_ptr, i);
-  for (int k = 0; k < w; ++k) {
+  for (int k = threadIdx.x & 63; k < w; k += 64) {
     const int p = p0 + 64 * (k & ~3) + (k & 3);
     const float v = val[p];
-    if (v != 0.0f) D[(size_t)idx[p] * n + i] += (double)v;   // column idx, row i
+    if (v != 0.0f) D[(size_t)idx[p] * n + i] = (double)v;   // column idx, row i
   }
 }
 
@@ -2141,6 +2150,51 @@ k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs
   }
 }
 
+// dpos_v = w_v sum_t dlambda_t g_tv over the vertex's incidences (FP32, four lanes per vertex
+// over the vertex-order gradients, then two shuffles) and lambda += dlambda: the factored
+// context's counterpart of k_scatter (same quantity, another summation order).
+__global__ void __launch_bounds__(kBlock)
+k_fac_scatter(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
+              const int* __restrict__ vs_inc, const float4* __restrict__ vs_g,
+              const float* __restrict__ invm, const float* __restrict__ dlam,
+              float* __restrict__ dpos) {
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;
+  const int base = fac_slot0(vs_ptr, v, q);
+  const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  for (int k0 = 0; k0 < own; k0 += 8) {
+    int t[8];
+    float4 g[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int p = base + 64 * min(k0 + j, own - 1);
+      t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
+      g[j] = vs_g[p];
+    }
+    float dl[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dl[j] = dlam[t[j]];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (k0 + j < own) {
+        ax = fmaf(g[j].x, dl[j], ax);
+        ay = fmaf(g[j].y, dl[j], ay);
+        az = fmaf(g[j].z, dl[j], az);
+      }
+    }
+  }
+  ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
+  ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
+  if (q == 0) {
+    const float w = invm[v];
+    dpos[3 * (size_t)v + 0] = w * ax;
+    dpos[3 * (size_t)v + 1] = w * ay;
+    dpos[3 * (size_t)v + 2] = w * az;
+  }
+}
+
 __device__ __forceinline__ double fac_orow(int t, const int4* __restrict__ t_i,
                                            const float* __restrict__ gradC,
                                            const double4* __restrict__ u, double acc) {
@@ -2925,7 +2979,8 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
     return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_create_handle failed");
   if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success)
     return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_set_stream failed");
-  k_dense_from_sell<<<nblocks(n), kBlock, 0, s>>>(n, L.slice_ptr, L.idx, L.val, ctx->dA);
+  k_dense_zero<<<nblocks((int64_t)n * n), kBlock, 0, s>>>((int64_t)n * n, ctx->dA);
+  k_dense_from_sell<<<(n + 3) / 4, kBlock, 0, s>>>(n, L.slice_ptr, L.idx, L.val, ctx->dA);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(assemble)");
   bool refactor = !ctx->dense_ready || !ctx->ns_update;
   if (!refactor) {
@@ -3490,6 +3545,21 @@ int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gpt
   return 0;
 }
 
+int mgpbd_mg_factored_scatter(mgpbd_ctx* ctx, const float* dlam, float* lambda, float* dpos) {
+  CTX_CHECK("mgpbd_mg_factored_scatter");
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  if (!F.on || !dlam || !lambda || !dpos)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_factored_scatter: needs the factored level 0 and "
+                                    "device arrays");
+  hipStream_t s = ctx->stream;
+  const int n = ctx->L[0].n;
+  k_add_inplace<<<nblocks(n), kBlock, 0, s>>>(n, dlam, lambda);
+  k_fac_scatter<<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
+      F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm, dlam, dpos);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_factored_scatter");
+  return 0;
+}
+
 int mgpbd_mg_factored_refill(mgpbd_ctx* ctx) {
   CTX_CHECK("mgpbd_mg_factored_refill");
   const mgpbd_ctx::Factored& F = ctx->fac;
@@ -3497,9 +3567,11 @@ int mgpbd_mg_factored_refill(mgpbd_ctx* ctx) {
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_factored_refill: call mgpbd_mg_set_level1_factored first");
   hipStream_t s = ctx->stream;
   const MgLevel& L = ctx->L[0];
+  const int pe = prof_begin(ctx, MGPBD_PROF_REFILL);
   k_fac_gsum<<<nblocks(F.ng), kBlock, 0, s>>>(F.ng, F.gptr, F.gmem, F.gvert, F.gradC, F.invm, F.G);
   k_fac_galerkin<<<nblocks(F.ne), kBlock, 0, s>>>(F.ne, F.tptr, F.terms, F.G, F.cpos, F.cagg,
                                                   L.mptr, L.mem, F.alpha, F.cval);
+  prof_end(ctx, pe);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factored_refill");
   return 0;
 }

@@ -168,6 +168,7 @@ class MGPBDSolverGPU:
         self._dlam_gpu = torch.zeros(nt, **f32)
         self._lam_prev = torch.zeros(nt, **f32)
         self._norms = torch.zeros(4, dtype=torch.float64, device=dev)
+        self._x_save = torch.zeros(nv, 3, **f32)
         self._gpu_solver = None
         self._amg_Ps = None
         self.last_stats = []
@@ -342,6 +343,11 @@ class MGPBDSolverGPU:
         self._gpu_solver.update_all_coarse_matrices()
 
     def _scatter_dlam_gpu(self):
+        mg = self._gpu_solver
+        if mg is not None and mg.factored:   # vertex-order gradients of the factored level 0
+            mg.ctx("mgpbd_mg_factored_scatter", dptr(self._dlam_gpu), dptr(self.lambdaf),
+                   dptr(self.dpos))
+            return
         self.ctx("mgpbd_scatter", self.n_tets, self.n_verts, dptr(self._vptr), dptr(self._vinc),
                  dptr(self.v_invm), dptr(self.gradC), dptr(self._dlam_gpu), dptr(self.lambdaf),
                  dptr(self.dpos))
@@ -350,25 +356,41 @@ class MGPBDSolverGPU:
         self.ctx("mgpbd_apply_dpos", self.n_verts, dptr(self.v_p), dptr(self.dpos),
                  dptr(self.v_invm), float(omega))
 
-    def _dual_norms(self, omega: float, with_current: bool):
+    def _dual_norms(self, omega: float, with_current: int, read: bool = True):
+        """[current, trial] squared dual norms into _norms[1:3] (with_current: 0 trial only,
+        1 both, 2 current only); read=False leaves them on the device."""
         self.ctx("mgpbd_dual_norms", self.n_tets, dptr(self.t_i), dptr(self.v_p),
                  dptr(self.dpos), float(omega), dptr(self.Dm_inv), dptr(self.alpha_tilde),
                  dptr(self.lambdaf), dptr(self.constraints), int(with_current),
                  dptr(self._norms[1:3]))
-        return self._norms[1:3].tolist()
+        return self._norms[1:3].tolist() if read else None
 
     def _line_search(self, beta=0.5, max_steps=10):
-        """Backtracking on the dual residual norm (cons/mgpbd_gpu.py:379-391); the current
-        norm uses the C stored for this Newton iterate (same positions, same kernel)."""
-        cur2, trial2 = self._dual_norms(1.0, True)
+        """Backtracking on the dual residual norm (cons/mgpbd_gpu.py:379-391): trials at
+        omega = 1, 1/2, ..., 1/2^9 against the current norm (from the C stored for this Newton
+        iterate), 1/2^10 when none is smaller. Applies the step and returns (omega, r): the full
+        step is tried by moving the positions and running the constraint pass there, whose
+        ||b|| is the trial norm and, when the step is taken, the next Newton iteration's C,
+        gradC, b and residual r (None when a shorter step was taken: the positions were
+        restored, the shorter step applied, and the next iteration evaluates them)."""
+        self._dual_norms(1.0, 2, read=False)           # current norm into _norms[1]
+        gpu_copy(self.v_p.view(-1), self._x_save.view(-1), 3 * self.n_verts)
+        self._apply_dpos(1.0)
+        self._compute_C_and_gradC()
+        trial2, cur2 = self._norms[:2].tolist()
+        if trial2 < cur2:
+            return 1.0, math.sqrt(trial2)
+        gpu_copy(self._x_save.view(-1), self.v_p.view(-1), 3 * self.n_verts)
         omega = 1.0
-        for k in range(max_steps):
-            if k > 0:
-                _, trial2 = self._dual_norms(omega, False)
-            if trial2 < cur2:
-                return omega
+        for k in range(1, max_steps):
             omega *= beta
-        return omega
+            _, trial2 = self._dual_norms(omega, 0)
+            if trial2 < cur2:
+                self._apply_dpos(omega)
+                return omega, None
+        omega *= beta
+        self._apply_dpos(omega)
+        return omega, None
 
     # ------------------------------------------------------------------ Newton loop
     def solve(self, frame: int = 0):
@@ -379,9 +401,14 @@ class MGPBDSolverGPU:
         r0 = 0.0
         stats = []
         self.last_solve_info = []
+        r_next = None   # residual of the positions the line search's full step left evaluated
         for ite in range(self.maxiter):
-            self._compute_C_and_gradC()
-            r = self._residual_norm()
+            if r_next is None:
+                self._compute_C_and_gradC()
+                r = self._residual_norm()
+            else:
+                r = r_next
+            r_next = None
             if ite == 0:
                 r0 = r
             if r < self.atol or (r0 > 1e-12 and r < self.rtol * r0):
@@ -402,8 +429,11 @@ class MGPBDSolverGPU:
             if ite == 0:
                 self.first_solve_info = self.last_solve_info[0]
             self._scatter_dlam_gpu()
-            omega = self._line_search() if self.use_line_search else 1.0
-            self._apply_dpos(omega)
+            if self.use_line_search:
+                omega, r_next = self._line_search()
+            else:
+                omega = 1.0
+                self._apply_dpos(omega)
             stats.append((ite, r, cg_iters, omega))
             if self.benchmark is not None:
                 self.benchmark.record_iteration(ite, r, cg_iters=cg_iters, omega=omega)

@@ -221,7 +221,8 @@ int mgpbd_scatter(mgpbd_ctx* ctx, int n_tets, int n_verts, const int32_t* vptr_d
 
 /* Dual-residual norms for the line search (cons/mgpbd_gpu.py:360-391): out[0] =
  * ||C(x) + alpha lambda||^2 from the stored C, out[1] = same at x + omega dpos (C
- * recomputed). FP64, device out (2 doubles). */
+ * recomputed). FP64, device out (2 doubles). with_current: 0 out[1] only, 1 both, 2 out[0]
+ * only (no constraint evaluation). */
 int mgpbd_dual_norms(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
                      const float* dpos_dev, float omega, const float* Dm_inv_dev,
                      const float* alpha_dev, const float* lambda_dev, const float* C_dev,
@@ -413,6 +414,12 @@ int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gpt
 /* Level 1's values from the current gradients and alpha~ (FP64 sums rounded once; padding
  * slots untouched). */
 int mgpbd_mg_factored_refill(mgpbd_ctx* ctx);
+/* lambda += dlambda and dpos_v = invm_v sum_t dlambda_t gradC_tv with the factored level 0's
+ * vertex-order gradients (current after mgpbd_mg_convert_half): the scatter of
+ * _scatter_dlam_gpu (cons/mgpbd_gpu.py:330-343) with four lanes per vertex (the summation
+ * order differs from mgpbd_scatter's). Device arrays: dlam, lambda (n_tets), dpos (3 n_verts). */
+int mgpbd_mg_factored_scatter(mgpbd_ctx* ctx, const float* dlam_dev, float* lambda_dev,
+                              float* dpos_dev);
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx);
