@@ -6,10 +6,11 @@ nu = 0.45, rho = 1000 kg/m^3, gravity -9.8 m/s^2, top face held, dt = 1/300 s (6
 substeps), Newton defaults of the reference (maxiter 20, atol 1e-4, rtol 1e-2, line search),
 PCG to relative residual 1e-6 (max 100 iterations), tet-aggregation hierarchy extended to
 5 levels, coarsest level solved exactly (FP64 dense inverse). Runs: the reference's algorithm
-(2+2 weighted-Jacobi sweeps, omega 2/3, FP32) and the Chebyshev-smoothed solver (5+5 steps on
-level 0 and 1+1 on the coarse levels, on the device's spectral estimates, FP64 iterate,
-FP64-verified solves) in FP32, FP16 and BF16,
-and FP16 with the previous-lambda and the learned warm starts. A step is one substep
+(2+2 weighted-Jacobi sweeps, omega 2/3, FP32, assembled CSR) and the Chebyshev-smoothed solver
+(8+8 steps on level 0 and 1+1 on the coarse levels on [1.2 lambda / 40, 1.2 lambda] of the
+device's spectral estimates, FP64 iterate, FP64-verified solves) in FP32 (assembled CSR), FP16
+and BF16 (factored level 0), and FP16 with the previous-lambda and the learned warm starts.
+A step is one substep
 (predict, Newton/MGPCG solve, velocity update) from the state the previous step left; inputs
 are resident in HBM.
 
@@ -40,9 +41,13 @@ CONFIGS = {
     # (5 substeps) a seeded random free vertex is pulled by a 1e3 N/m spring toward a target
     # uniform in a 0.5 m cube around its rest position. maxiter_cg replaces the reference's
     # 100-iteration PCG cap where some systems need more to reach 1e-6.
-    "c1": dict(kind="box", n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets"),
+    # smoothing: Chebyshev steps per phase on level 0 and on the coarse levels and the
+    # interval ratio of every Chebyshev run of the workload (all precisions alike; measured
+    # per workload, default 5 / 1 on [1.2 lambda / 30, 1.2 lambda])
+    "c1": dict(kind="box", n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets",
+               smoothing=((8, 1), 40.0)),
     "c2": dict(kind="box", n=56, h=0.02, E=(1e4, 1e8), max_levels=5,
-               label="box56_1053696tets_E1e4-1e8"),
+               label="box56_1053696tets_E1e4-1e8", smoothing=((8, 1), 40.0)),
     "c3": dict(kind="box", n=32, h=0.03, E=(1e4, 1e6), max_levels=4, drag=True,
                label="box32_196608tets_drag"),
     "c4": dict(kind="ellipsoid", h=0.01, E=(1e5, 1e8), max_levels=6, drag=True,
@@ -220,8 +225,8 @@ RUNS = {
     # is measured, not enforced
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
     # Chebyshev smoothing on the device's spectral estimates (5 + 5 steps on level 0, 1 + 1 on
-    # the coarse levels, on [1.2 lambda / 30, 1.2 lambda]), FP64 PCG iterate, every solve
-    # FP64-verified
+    # the coarse levels, on [1.2 lambda / 30, 1.2 lambda], unless the workload sets its own
+    # schedule: CONFIGS "smoothing"), FP64 PCG iterate, every solve FP64-verified
     "fp32": dict(precision="fp32", **CHEBYSHEV),
     "fp16": dict(precision="fp16", **CHEBYSHEV),
     "bf16": dict(precision="bf16", **CHEBYSHEV),
@@ -252,6 +257,9 @@ def train_learned(cfg, seeds, substeps, epochs, level0="factored"):
     t0 = time.perf_counter()
     samples, model = [], None
     kw = dict(RUNS["fp16_previous"], level0=level0)
+    if cfg.get("smoothing"):
+        (k0, k1), ratio = cfg["smoothing"]
+        kw.update(n_smooth=max(k0, k1), smooth_schedule=[k0, k1], cheb_ratio=ratio)
     for seed in seeds:
         mesh, mu_t = build_workload(cfg, seed)
         if model is None:
@@ -286,6 +294,9 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     kw = dict(RUNS[name])
     if kw["precision"] != "fp32":
         kw["level0"] = level0
+    if kw.get("smoother") == "chebyshev" and cfg.get("smoothing"):
+        (k0, k1), ratio = cfg["smoothing"]
+        kw.update(n_smooth=max(k0, k1), smooth_schedule=[k0, k1], cheb_ratio=ratio)
     t0 = time.perf_counter()
     s = make_solver(mesh, mu_t, cfg, kw, model=model if kw.get("warm_start") == "learned" else None)
     torch.cuda.synchronize()

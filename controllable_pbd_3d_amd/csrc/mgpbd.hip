@@ -839,6 +839,17 @@ k_dense_from_sell(int n, const int* __restrict__ slice_ptr, const int* __restric
   }
 }
 
+// Newton-Schulz acceptance on the device: X replaces the inverse when ||I - A X||_F^2 < 0.01
+// (f2 from k_fin_sum); flag = 1 accepted, 2 rejected (the previous inverse stays).
+__global__ void __launch_bounds__(kBlock)
+k_ns_accept(int64_t len, const double* __restrict__ f2, const double* __restrict__ X,
+            double* __restrict__ D, int* __restrict__ flag) {
+  const bool ok = *f2 < 0.01;   // false for a diverged (non-finite) iteration too
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) *flag = ok ? 1 : 2;
+  if (ok && i < len) D[i] = X[i];
+}
+
 // R = I (column-major n x n)
 __global__ void __launch_bounds__(kBlock)
 k_set_identity(int n, double* __restrict__ R) {
@@ -2249,11 +2260,12 @@ k_fac_oapply(int n, const int4* __restrict__ t_i, const float* __restrict__ grad
 // ------------------------------------------------------------------------------------------
 // Level 1 from the gradients. With P[i, agg[i]] = 1, A1 = P^T A P = G W G^T + P^T diag(alpha~) P
 // where G_(I,v) = sum of g_tv over the tets t of aggregate I holding vertex v. Per (aggregate,
-// vertex) pair: G and w_v (k_fac_gsum); per level-1 entry (I, J): the sum over the vertices
-// shared by I and J of w_v G_(I,v) . G_(J,v), plus sum_{i in I} alpha~_i on the diagonal
-// (k_fac_galerkin, FP64 sums rounded once, written at the entry's SELL position). This reads
-// ~2 gathered 16-byte G per shared vertex instead of the ~13 level-0 entries each level-1
-// entry sums in the refill (cons/mgpcg_gpu.py:183-212 contribution maps).
+// vertex) pair: G and w_v (k_fac_gsum); per level-1 entry (I, J) with I <= J: the sum over
+// the vertices shared by I and J of w_v G_(I,v) . G_(J,v), plus sum_{i in I} alpha~_i on the
+// diagonal (k_fac_galerkin, FP64 sums rounded once, written at the entry's SELL position and
+// at its transpose's, so A1 is exactly symmetric). This reads ~2 gathered 16-byte G per shared
+// vertex for half the entries instead of the ~13 level-0 entries each level-1 entry sums in
+// the refill (cons/mgpcg_gpu.py:183-212 contribution maps).
 // ------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 k_fac_gsum(int ng, const int* __restrict__ gptr, const int* __restrict__ gmem,
@@ -2275,7 +2287,8 @@ k_fac_gsum(int ng, const int* __restrict__ gptr, const int* __restrict__ gmem,
 __global__ void __launch_bounds__(kBlock)
 k_fac_galerkin(int ne, const int* __restrict__ tptr, const int2* __restrict__ terms,
                const float4* __restrict__ G, const int* __restrict__ cpos,
-               const int* __restrict__ cagg, const int* __restrict__ mptr,
+               const int* __restrict__ cpos_t, const int* __restrict__ cagg,
+               const int* __restrict__ mptr,
                const int* __restrict__ mem, const float* __restrict__ alpha,
                float* __restrict__ cval) {
   const int c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -2291,7 +2304,9 @@ k_fac_galerkin(int ne, const int* __restrict__ tptr, const int2* __restrict__ te
   const int I = cagg[c];
   if (I >= 0)
     for (int m = mptr[I]; m < mptr[I + 1]; ++m) acc += (double)alpha[mem[m]];
-  cval[cpos[c]] = (float)acc;
+  const float v = (float)acc;
+  cval[cpos[c]] = v;
+  cval[cpos_t[c]] = v;   // the transposed entry (the same slot on the diagonal)
 }
 
 }  // namespace mgpbd
@@ -2354,6 +2369,13 @@ struct mgpbd_ctx {
   double* dT = nullptr;      // next X
   double* dnorm = nullptr;   // ||R||_F^2 (device)
   int64_t ns_steps = 0, ns_refactors = 0;
+  // the last refresh's acceptance flag (1 accepted, 2 rejected), copied to pinned host memory
+  // without a wait and read at the next refresh (ns_event orders it)
+  int* ns_flag = nullptr;
+  int* ns_flag_host = nullptr;
+  hipEvent_t ns_event = nullptr;
+  bool ns_pending = false;
+  bool ns_rejected = false;
   float omega = 2.0f / 3.0f;
   int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
   float overcorrect = 1.0f;   // scale of the prolongated coarse correction
@@ -2400,6 +2422,7 @@ struct mgpbd_ctx {
     const int* tptr = nullptr;
     const int2* terms = nullptr;
     const int* cpos = nullptr;
+    const int* cpos_t = nullptr;
     const int* cagg = nullptr;
     float* cval = nullptr;
   } fac;
@@ -2545,6 +2568,9 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (c->dX2) (void)hipFree(c->dX2);
   if (c->dT) (void)hipFree(c->dT);
   if (c->dnorm) (void)hipFree(c->dnorm);
+  if (c->ns_flag) (void)hipFree(c->ns_flag);
+  if (c->ns_flag_host) (void)hipHostFree(c->ns_flag_host);
+  if (c->ns_event) (void)hipEventDestroy(c->ns_event);
   (void)hipFree(c->ws.partials);
   (void)hipFree(c->st);
   (void)hipHostFree(c->st_host);
@@ -2955,6 +2981,17 @@ static float* sweeps(mgpbd_ctx* ctx, int l, const float* b, float* cur, int coun
   return cur;
 }
 
+// Counts the outcome of the last Newton-Schulz refresh once its flag has arrived (the event
+// completed long before in the solve's own synchronisations).
+static int ns_settle(mgpbd_ctx* ctx) {
+  if (!ctx->ns_pending) return 0;
+  MGPBD_HIP_TRY(hipEventSynchronize(ctx->ns_event));
+  ctx->ns_pending = false;
+  if (*ctx->ns_flag_host == 1) ctx->ns_steps += 2;
+  else ctx->ns_rejected = true;
+  return 0;
+}
+
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
   CTX_CHECK("mgpbd_mg_factor_coarsest");
   if (int e = check_levels(ctx)) return e;
@@ -2983,6 +3020,11 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
   k_dense_from_sell<<<(n + 3) / 4, kBlock, 0, s>>>(n, L.slice_ptr, L.idx, L.val, ctx->dA);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(assemble)");
   bool refactor = !ctx->dense_ready || !ctx->ns_update;
+  if (int e = ns_settle(ctx)) return e;
+  if (ctx->ns_rejected) {   // the previous refresh's solves ran on the older inverse
+    ctx->ns_rejected = false;
+    refactor = true;
+  }
   if (!refactor) {
     // two Newton-Schulz steps from the previous inverse, X <- X + X (I - A X), then an a
     // posteriori check: ||I - A X||_F < 0.1 bounds the spectral radius of I - A X, so the
@@ -3006,18 +3048,20 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
     }
     k_sumsq<<<gq, kBlock, 0, s>>>((int64_t)need, ctx->dR, ctx->ws);
     k_fin_sum<<<1, kBlock, 0, s>>>(ctx->ws, gq, 0, 1, ctx->dnorm);
+    // accepted X is copied (not swapped: captured PCG graphs hold the address of ctx->dense)
+    // on the device, without a host wait; a rejected one leaves the previous inverse (SPD, so
+    // the V-cycle stays SPD; the verified PCG still reaches its tolerance) and the next
+    // refresh refactors
+    if (!ctx->ns_flag) MGPBD_HIP_TRY(hipMalloc(&ctx->ns_flag, sizeof(int)));
+    if (!ctx->ns_flag_host) MGPBD_HIP_TRY(hipHostMalloc(&ctx->ns_flag_host, sizeof(int)));
+    if (!ctx->ns_event) MGPBD_HIP_TRY(hipEventCreateWithFlags(&ctx->ns_event, hipEventDisableTiming));
+    k_ns_accept<<<nblocks((int64_t)need), kBlock, 0, s>>>((int64_t)need, ctx->dnorm, X,
+                                                          ctx->dense, ctx->ns_flag);
     MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(newton-schulz)");
-    double f2 = 0.0;
-    MGPBD_HIP_TRY(hipMemcpyAsync(&f2, ctx->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
-    MGPBD_HIP_TRY(hipStreamSynchronize(s));
-    if (f2 < 0.01) {
-      // copied rather than swapped: captured PCG graphs hold the address of ctx->dense
-      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, X, need * sizeof(double),
-                                   hipMemcpyDeviceToDevice, s));
-      ctx->ns_steps += 2;
-    } else {
-      refactor = true;   // also when the iteration diverged (f2 not finite)
-    }
+    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->ns_flag_host, ctx->ns_flag, sizeof(int),
+                                 hipMemcpyDeviceToHost, s));
+    MGPBD_HIP_TRY(hipEventRecord(ctx->ns_event, s));
+    ctx->ns_pending = true;
   }
   if (refactor) {
     ++ctx->ns_refactors;
@@ -3046,6 +3090,7 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
 
 int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors) {
   CTX_CHECK("mgpbd_mg_coarse_stats");
+  if (int e = ns_settle(ctx)) return e;
   if (newton_schulz_steps) *newton_schulz_steps = ctx->ns_steps;
   if (refactors) *refactors = ctx->ns_refactors;
   return 0;
@@ -3522,7 +3567,8 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,
-                                 const int32_t* cpos, const int32_t* cagg, float* c_val) {
+                                 const int32_t* cpos, const int32_t* cpos_t,
+                                 const int32_t* cagg, float* c_val) {
   CTX_CHECK("mgpbd_mg_set_level1_factored");
   mgpbd_ctx::Factored& F = ctx->fac;
   if (!F.on || ctx->n_levels < 2 || !ctx->L[0].mptr)
@@ -3533,14 +3579,14 @@ int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gpt
     return 0;
   }
   if (n_pairs <= 0 || n_entries <= 0 || !gmem || !gvert || !G || !tptr || !terms || !cpos ||
-      !cagg || !c_val || (reinterpret_cast<uintptr_t>(G) % 16) != 0 ||
+      !cpos_t || !cagg || !c_val || (reinterpret_cast<uintptr_t>(G) % 16) != 0 ||
       (reinterpret_cast<uintptr_t>(terms) % 8) != 0)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level1_factored: invalid arguments (G 16-byte, "
                                     "terms 8-byte aligned)");
   F.ng = n_pairs; F.gptr = gptr; F.gmem = gmem; F.gvert = gvert;
   F.G = reinterpret_cast<float4*>(G);
   F.ne = n_entries; F.tptr = tptr; F.terms = reinterpret_cast<const int2*>(terms);
-  F.cpos = cpos; F.cagg = cagg; F.cval = c_val;
+  F.cpos = cpos; F.cpos_t = cpos_t; F.cagg = cagg; F.cval = c_val;
   F.galerkin = true;
   return 0;
 }
@@ -3569,8 +3615,8 @@ int mgpbd_mg_factored_refill(mgpbd_ctx* ctx) {
   const MgLevel& L = ctx->L[0];
   const int pe = prof_begin(ctx, MGPBD_PROF_REFILL);
   k_fac_gsum<<<nblocks(F.ng), kBlock, 0, s>>>(F.ng, F.gptr, F.gmem, F.gvert, F.gradC, F.invm, F.G);
-  k_fac_galerkin<<<nblocks(F.ne), kBlock, 0, s>>>(F.ne, F.tptr, F.terms, F.G, F.cpos, F.cagg,
-                                                  L.mptr, L.mem, F.alpha, F.cval);
+  k_fac_galerkin<<<nblocks(F.ne), kBlock, 0, s>>>(F.ne, F.tptr, F.terms, F.G, F.cpos, F.cpos_t,
+                                                  F.cagg, L.mptr, L.mem, F.alpha, F.cval);
   prof_end(ctx, pe);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factored_refill");
   return 0;

@@ -84,12 +84,14 @@ def vertex_slices(vptr: np.ndarray, vinc: np.ndarray):
 
 
 def factored_galerkin_maps(t_i, n_verts: int, agg, c_indptr, c_indices):
-    """Maps of level 1 from the gradients (A1 = G W G^T + P^T diag(alpha~) P). Returns
-    gptr, gmem, gvert: the (aggregate, vertex) pairs in ascending (aggregate, vertex) order with
-    their incidences 4 tet + local vertex (ascending) and vertex; tptr, terms: per level-1 CSR
-    entry (I, J) the pairs (p_Iv, p_Jv) of the vertices v that I and J share, in vertex order;
-    cagg: I for a diagonal entry, else -1. Raises when the level-1 pattern lacks a shared-vertex
-    entry or holds an entry with no shared vertex (the pattern must be the structural one)."""
+    """Maps of level 1 from the gradients (A1 = G W G^T + P^T diag(alpha~) P, symmetric: each
+    entry (I, J) with I <= J is summed once and stored at both positions). Returns gptr, gmem,
+    gvert: the (aggregate, vertex) pairs in ascending (aggregate, vertex) order with their
+    incidences 4 tet + local vertex (ascending) and vertex; tptr, terms: per upper entry the
+    pairs (p_Iv, p_Jv) of the vertices v that I and J share, in vertex order; cup, clo: the CSR
+    index of the entry and of its transpose; cagg: I for a diagonal entry, else -1. Raises when
+    the level-1 pattern lacks a shared-vertex entry or holds an entry with no shared vertex
+    (the pattern must be the structural one)."""
     tv = np.asarray(t_i, np.int64).reshape(-1, 4)
     nt = tv.shape[0]
     inc = np.arange(4 * nt, dtype=np.int64)
@@ -116,21 +118,34 @@ def factored_galerkin_maps(t_i, n_verts: int, agg, c_indptr, c_indices):
     a, b = pv[a_pos], pv[b_pos]
     n1 = len(c_indptr) - 1
     rows = np.repeat(np.arange(n1, dtype=np.int64), np.diff(c_indptr))
-    ckey = rows * n1 + np.asarray(c_indices, np.int64)
-    tkey = gI[a] * n1 + gI[b]
-    c = np.searchsorted(ckey, tkey)
-    if np.any(c >= len(ckey)) or np.any(ckey[np.minimum(c, len(ckey) - 1)] != tkey):
-        raise ValueError("level-1 pattern lacks an entry of two aggregates sharing a vertex")
-    o = np.argsort(c, kind="stable")
-    cnt = np.bincount(c, minlength=len(ckey))
-    if np.any(cnt == 0):
+    cols = np.asarray(c_indices, np.int64)
+    ckey = rows * n1 + cols
+
+    def find(k):
+        c = np.searchsorted(ckey, k)
+        if np.any(c >= len(ckey)) or np.any(ckey[np.minimum(c, len(ckey) - 1)] != k):
+            raise ValueError("level-1 pattern lacks an entry of two aggregates sharing a vertex")
+        return c
+
+    cnt_all = np.bincount(find(gI[a] * n1 + gI[b]), minlength=len(ckey))
+    if np.any(cnt_all == 0):
         raise ValueError("a level-1 entry has no shared vertex")
+    up = gI[a] <= gI[b]
+    a, b = a[up], b[up]
+    c = find(gI[a] * n1 + gI[b])
+    cup = np.flatnonzero(rows <= cols)          # upper entries, CSR order
+    slot = np.full(len(ckey), -1, np.int64)
+    slot[cup] = np.arange(len(cup))
+    e = slot[c]
+    o = np.argsort(e, kind="stable")
+    cnt = np.bincount(e, minlength=len(cup))
     terms = np.stack([a[o], b[o]], axis=1)
     tptr = np.concatenate([[0], np.cumsum(cnt)])
     if tptr[-1] >= 2 ** 31 or len(gmem) >= 2 ** 31:
         raise ValueError("level-1 maps exceed 32-bit offsets")
-    cagg = np.where(rows == np.asarray(c_indices, np.int64), rows, -1)
-    return gptr, gmem, gv, tptr, terms, cagg
+    clo = find(cols[cup] * n1 + rows[cup])
+    cagg = np.where(rows[cup] == cols[cup], rows[cup], -1)
+    return gptr, gmem, gv, tptr, terms, cup, clo, cagg
 
 
 def _aggs_from_Ps(Ps):
@@ -409,15 +424,16 @@ class GPUMultigridSolver:
         if self.n_levels < 2:
             return
         C = self.levels[1]
-        gptr, gmem, gvert, tptr, terms, cagg = factored_galerkin_maps(
+        gptr, gmem, gvert, tptr, terms, cup, clo, cagg = factored_galerkin_maps(
             src["t_i_host"], nv, self.transfers[0]["agg"].cpu().numpy(), C.indptr, C.indices)
         up = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.int32)).to(dev)
         f.update(gptr=up(gptr), gmem=up(gmem), gvert=up(gvert), tptr=up(tptr),
-                 terms=up(terms.reshape(-1)), cagg=up(cagg), cpos=up(C.sell_pos_host),
+                 terms=up(terms.reshape(-1)), cagg=up(cagg), cpos=up(C.sell_pos_host[cup]),
+                 cpos_t=up(C.sell_pos_host[clo]),
                  G=torch.zeros(4 * max(len(gvert), 1), dtype=torch.float32, device=dev))
         self.ctx("mgpbd_mg_set_level1_factored", len(gvert), dptr(f["gptr"]), dptr(f["gmem"]),
                  dptr(f["gvert"]), dptr(f["G"]), len(cagg), dptr(f["tptr"]), dptr(f["terms"]),
-                 dptr(f["cpos"]), dptr(f["cagg"]), dptr(C.data))
+                 dptr(f["cpos"]), dptr(f["cpos_t"]), dptr(f["cagg"]), dptr(C.data))
 
     def level_residual(self, level: int, b: torch.Tensor, x: torch.Tensor, r: torch.Tensor):
         """r = b - A_l x with the operator the V-cycle applies on level l (scaled variables in

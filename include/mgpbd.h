@@ -297,14 +297,17 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
                        int diagonal_only);
 /* Coarsest level with MGPBD_COARSE_CHOLESKY: assemble it densely in FP64, factor and invert
  * (rocSOLVER dpotrf/dpotri), keep the symmetric inverse. Call after the coarsest values
- * change (every coarse refill). Synchronises; MGPBD_ENONFINITE if not positive definite.
- * No reference counterpart (the reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
+ * change (every coarse refill). A Cholesky refactorization synchronises (MGPBD_ENONFINITE if
+ * not positive definite); a Newton-Schulz refresh does not. No reference counterpart (the
+ * reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
-/* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level first
+/* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level
  * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse (rocBLAS dgemm)
- * and keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue of X A lies
- * within 0.1 of 1 and the V-cycle stays an SPD preconditioner); otherwise, or when the
- * iteration diverged, it refactors with Cholesky. */
+ * and the device keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue
+ * of X A lies within 0.1 of 1 and the V-cycle stays an SPD preconditioner). Otherwise, or when
+ * the iteration diverged, the previous inverse stays for the solves that follow (SPD, so the
+ * V-cycle stays SPD) and the next refresh refactors with Cholesky; the host learns the outcome
+ * at that refresh, without waiting for it in this one. */
 int mgpbd_mg_set_coarse_update(mgpbd_ctx* ctx, int newton_schulz);
 /* Counts since the context was created: Newton-Schulz steps kept, Cholesky refactorizations. */
 int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors);
@@ -402,15 +405,18 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
  * A1 = G W G^T + P^T diag(alpha~) P, G_(I,v) = sum of gradC over the tets of aggregate I
  * holding vertex v. Per (aggregate, vertex) pair p < n_pairs: its incidences (4 tet + local
  * vertex) gmem[gptr[p] .. gptr[p+1]) and vertex gvert[p]; G (4 floats per pair, 16-byte
- * aligned) is written by mgpbd_mg_factored_refill. Per level-1 CSR entry c < n_entries: the
+ * aligned) is written by mgpbd_mg_factored_refill. Per level-1 entry c < n_entries: the
  * pairs (a, b) of the vertices its two aggregates share, terms[2 m], terms[2 m + 1] for m in
- * [tptr[c], tptr[c+1]) (8-byte aligned), its SELL position cpos[c] in c_val (level 1's
- * values) and cagg[c] = the aggregate when c is a diagonal entry, else -1 (the diagonal adds
- * the aggregate's alpha~ over level 0's member lists). gptr NULL switches it off. */
+ * [tptr[c], tptr[c+1]) (8-byte aligned), over the entries (I, J) with I <= J only (A1 is
+ * symmetric): its SELL position cpos[c] in c_val (level 1's values), its transpose's
+ * cpos_t[c] (the same on the diagonal), and cagg[c] = the aggregate when c is a diagonal
+ * entry, else -1 (the diagonal adds the aggregate's alpha~ over level 0's member lists).
+ * gptr NULL switches it off. */
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,
-                                 const int32_t* cpos, const int32_t* cagg, float* c_val);
+                                 const int32_t* cpos, const int32_t* cpos_t,
+                                 const int32_t* cagg, float* c_val);
 /* Level 1's values from the current gradients and alpha~ (FP64 sums rounded once; padding
  * slots untouched). */
 int mgpbd_mg_factored_refill(mgpbd_ctx* ctx);

@@ -232,22 +232,27 @@ def test_factored_galerkin_maps_give_PtAP():
     Ps = sp.csr_matrix(P)
     pat = (Ps.T @ S @ Ps).tocsr()
     pat.sort_indices()
-    gptr, gmem, gvert, tptr, terms, cagg = factored_galerkin_maps(m.tets, nv, agg, pat.indptr,
-                                                                    pat.indices)
+    gptr, gmem, gvert, tptr, terms, cup, clo, cagg = factored_galerkin_maps(
+        m.tets, nv, agg, pat.indptr, pat.indices)
     G = np.zeros((len(gvert), 3))
     for p in range(len(gvert)):
         for inc in gmem[gptr[p]:gptr[p + 1]]:
             assert m.tets[inc] == gvert[p]
             G[p] += g[12 * (inc >> 2) + 3 * (inc & 3):12 * (inc >> 2) + 3 * (inc & 3) + 3]
     rows = np.repeat(np.arange(n1), np.diff(pat.indptr))
-    got = np.zeros(len(pat.indices))
-    for c in range(len(got)):
-        for a_, b_ in terms[tptr[c]:tptr[c + 1]]:
+    got = np.full(len(pat.indices), np.nan)
+    assert np.all(rows[cup] <= pat.indices[cup])
+    assert np.array_equal(rows[clo], pat.indices[cup]) and np.array_equal(pat.indices[clo], rows[cup])
+    for e, c in enumerate(cup):
+        v = 0.0
+        for a_, b_ in terms[tptr[e]:tptr[e + 1]]:
             assert gvert[a_] == gvert[b_]
-            got[c] += invm[gvert[a_]] * G[a_] @ G[b_]
-        if cagg[c] >= 0:
-            assert cagg[c] == rows[c] == pat.indices[c]
-            got[c] += al[agg == rows[c]].sum()
+            v += invm[gvert[a_]] * G[a_] @ G[b_]
+        if cagg[e] >= 0:
+            assert cagg[e] == rows[c] == pat.indices[c]
+            v += al[agg == rows[c]].sum()
+        got[c] = got[clo[e]] = v
+    assert not np.any(np.isnan(got))
     want = A1[rows, pat.indices]
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
 
