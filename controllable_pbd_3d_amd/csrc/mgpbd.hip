@@ -697,6 +697,30 @@ k_power_fin(RedWS ws, int nparts, double* __restrict__ pw) {
   pw[1] = sqrt(wdw);
 }
 
+// k_power_fin and k_power_norm in one launch for levels with few partials: every block sums
+// the same partials in the same order (identical totals), block 0 stores the estimates.
+__global__ void __launch_bounds__(kBlock)
+k_power_fin_norm(int n, RedWS ws, int nparts, const float* __restrict__ w,
+                 float* __restrict__ v, double* __restrict__ pw) {
+  const double vav = sum_partials(ws, 0, nparts);
+  const double vdv = sum_partials(ws, 1, nparts);
+  const double wdw = sum_partials(ws, 2, nparts);
+  __shared__ double nrm_s;
+  if (threadIdx.x == 0) {
+    nrm_s = sqrt(wdw);
+    if (blockIdx.x == 0) {
+      pw[2] = pw[0];
+      pw[0] = vdv > 0.0 ? vav / vdv : 0.0;
+      pw[1] = nrm_s;
+    }
+  }
+  __syncthreads();
+  const double nrm = nrm_s;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || !(nrm > 0.0) || !(nrm < 1e300)) return;
+  v[i] = (float)(w[i] / nrm);
+}
+
 // v = w / ||w||_D (v unchanged when the norm is 0 or not finite)
 __global__ void __launch_bounds__(kBlock)
 k_power_norm(int n, const float* __restrict__ w, float* __restrict__ v,
@@ -1022,6 +1046,7 @@ struct PcgState {
   int k;        // iteration index
   int relative;
   int start;    // 0: x = 0, 1: the guess in x, 2: the alternative guess
+  int verified; // 1: rr_true was computed on the device after convergence (graph tail)
 };
 
 __global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
@@ -1030,6 +1055,7 @@ __global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
   st->rr0 = st->rr_guess = st->rr_alt = 0.0;
   st->rr_true = -1.0;
   st->start = 0;
+  st->verified = 0;
   st->tol2 = tol * tol;
 }
 
@@ -1124,6 +1150,26 @@ __global__ void __launch_bounds__(kBlock)
 k_fin_true(RedWS ws, int nparts, PcgState* st) {
   const double t = sum_partials(ws, 0, nparts);
   if (threadIdx.x == 0) st->rr_true = t;
+}
+
+// The verification tail of a captured PCG replay: runs only once the solve has converged
+// (done == 1, a value every thread reads alike), then marks the state verified.
+__global__ void __launch_bounds__(kBlock)
+k_fin_true_gated(RedWS ws, int nparts, PcgState* st) {
+  if (st->done != 1 || st->verified) return;
+  const double t = sum_partials(ws, 0, nparts);
+  if (threadIdx.x == 0) {
+    st->rr_true = t;
+    st->verified = 1;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_x_from64_gated(int n, const double* __restrict__ x64, float* __restrict__ x,
+                 const PcgState* __restrict__ st) {
+  if (st->done != 1 || st->verified) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = (float)x64[i];
 }
 
 // Restart residual of the FP64 iterate: r = b - A x64 (FP64 products and sums), rounded once
@@ -2119,13 +2165,17 @@ k_fac_power(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
 // restart residuals and the FP64 check all use one operator and level 0 is never assembled
 // per Newton iteration.
 // ------------------------------------------------------------------------------------------
+// gate 0: skip when *done is set (PCG launches past convergence); gate 1: run only when the
+// solve converged and is not yet verified (st: the verification tail of a replay).
 template <typename XT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
               const int* __restrict__ vs_inc, const float4* __restrict__ vs_g,
               const float* __restrict__ invm, const XT* __restrict__ x,
-              double4* __restrict__ u, const int* __restrict__ done) {
+              double4* __restrict__ u, const int* __restrict__ done,
+              const PcgState* __restrict__ gate_st) {
   if (done && *done) return;
+  if (gate_st && (gate_st->done != 1 || gate_st->verified)) return;
   const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int v = gid >> 2, q = gid & 3;
   if (v >= nv) return;   // all four lanes of a vertex leave together
@@ -2236,8 +2286,9 @@ __global__ void __launch_bounds__(kBlock)
 k_fac_oapply(int n, const int4* __restrict__ t_i, const float* __restrict__ gradC,
              const float* __restrict__ alpha, const double4* __restrict__ u,
              const XT* __restrict__ x, const float* __restrict__ b, float* __restrict__ y,
-             RedWS ws, const int* __restrict__ done) {
+             RedWS ws, const int* __restrict__ done, const PcgState* __restrict__ gate_st) {
   if (done && *done) return;
+  if (gate_st && (gate_st->done != 1 || gate_st->verified)) return;
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   double part = 0.0;
   if (i < n) {
@@ -3420,8 +3471,12 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
       else
         k_power_step_h<2, false><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, L.hpk,
                                                       L.hval, v, w, ctx->ws);
-      k_power_fin<<<1, kBlock, 0, s>>>(ctx->ws, g, pw);
-      k_power_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, w, v, pw);
+      if (g <= 1024) {   // every block re-sums the (few) partials: one launch instead of two
+        k_power_fin_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, ctx->ws, g, w, v, pw);
+      } else {
+        k_power_fin<<<1, kBlock, 0, s>>>(ctx->ws, g, pw);
+        k_power_norm<<<nblocks(L.n), kBlock, 0, s>>>(L.n, w, v, pw);
+      }
     }
     k_power_omega<<<1, 64, 0, s>>>(pw, ctx->sm_target, ctx->sm_safety * boost, ctx->omega,
                                    cheb_level(ctx, l) ? 1 : 0, ctx->cheb_ratio,
@@ -3705,29 +3760,30 @@ int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b, const flo
 // The PCG's level-0 operator in factored form (vertex pass, tet pass), modes of k_fac_oapply;
 // x in FP32 (xf) or FP64 (xd); grid gs >= the row blocks (its partials when mode != 1).
 static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* xd,
-                      const float* b, float* y, const int* done, int gs) {
+                      const float* b, float* y, const int* done, int gs,
+                      const PcgState* gate = nullptr) {
   const mgpbd_ctx::Factored& F = ctx->fac;
   const int n = ctx->L[0].n;
   hipStream_t s = ctx->stream;
   const int gv = nblocks((int64_t)kFacLanes * F.nv);
   if (xd) {
     k_fac_ogather<double><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm,
-                                                xd, F.u64, done);
+                                                xd, F.u64, done, gate);
     k_fac_oapply<double, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xd, b, y,
-                                                  ctx->ws, done);
+                                                  ctx->ws, done, gate);
     return;
   }
   k_fac_ogather<float><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm,
-                                             xf, F.u64, done);
+                                             xf, F.u64, done, gate);
   if (mode == 0)
     k_fac_oapply<float, 0><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
-                                                 ctx->ws, done);
+                                                 ctx->ws, done, gate);
   else if (mode == 1)
     k_fac_oapply<float, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
-                                                 ctx->ws, done);
+                                                 ctx->ws, done, gate);
   else
     k_fac_oapply<float, 2><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
-                                                 ctx->ws, done);
+                                                 ctx->ws, done, gate);
 }
 
 // One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
@@ -3834,6 +3890,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
   const int first_len = std::max(1, std::min(ctx->pred_iters > 0 ? ctx->pred_iters : ctx->poll,
                                              (int)mgpbd_ctx::kGraphMax));
   const int pred_end = ctx->pred_iters > 0 ? ctx->pred_iters + 1 : 0;
+  // with the factored level 0 every captured replay ends with the FP64 check, gated on the
+  // device's done flag (a converged solve needs no second launch sequence and host wait)
+  const bool tail_verify = graph_mode && ctx->verify && ctx->fac.on;
   // capture (or reuse) the graphs of those lengths before any work is queued on the stream
   if (graph_mode) {
     const void* key[4] = {b, x, work, L.work};
@@ -3847,6 +3906,11 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       hipGraph_t g = nullptr;
       MGPBD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       for (int c = 0; c < len; ++c) pcg_iteration(ctx, x, x64, r, p, Ap, xv, zold, first, gn, gs, false);
+      if (tail_verify) {   // runs only once converged: the host reads rr_true with done
+        if (x64) k_x_from64_gated<<<nblocks(n), kBlock, 0, s>>>(n, x64, x, ctx->st);
+        fac_outer(ctx, 2, x, nullptr, b, nullptr, nullptr, gs, ctx->st);
+        k_fin_true_gated<<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
+      }
       (void)hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s);
       hipError_t ce = hipStreamEndCapture(s, &g);
       if (ce != hipSuccess) return mgpbd_fail(MGPBD_EHIP, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -3947,7 +4011,8 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       MGPBD_HIP_TRY(hipStreamSynchronize(s));
     }
-    if (x64) {
+    const bool tail_verified = tail_verify && ctx->st_host->verified == 1;
+    if (x64 && !tail_verified) {
       k_x_from64<<<nblocks(n), kBlock, 0, s>>>(n, x64, x);
       MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(x)");
     }
@@ -3973,17 +4038,19 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     if (!ctx->verify) break;
     // FP64 residual of the returned x (every solve); a restart from x when a converged one
     // misses the tolerance
-    if (ctx->fac.on)
-      fac_outer(ctx, 2, x, nullptr, b, nullptr, nullptr, gs);
-    else if (L.woff)
-      k_true_resid_w<<<gs, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
-          n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, x, b, ctx->ws);
-    else
-      k_true_resid<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, b, ctx->ws);
-    k_fin_true<<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
-    MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(verify)");
-    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
-    MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    if (!tail_verified) {
+      if (ctx->fac.on)
+        fac_outer(ctx, 2, x, nullptr, b, nullptr, nullptr, gs);
+      else if (L.woff)
+        k_true_resid_w<<<gs, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
+            n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, x, b, ctx->ws);
+      else
+        k_true_resid<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x, b, ctx->ws);
+      k_fin_true<<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
+      MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(verify)");
+      MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+      MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    }
     rr_true = ctx->st_host->rr_true;
     if (ctx->verify == 2 || ctx->st_host->done != 1 || rr_true < ctx->st_host->tol2 ||
         issued >= maxiter || restarts >= 3)
@@ -4038,6 +4105,7 @@ int mgpbd_pcg_set_fp64_iterate(mgpbd_ctx* ctx, int enable) {
 int mgpbd_pcg_set_verify(mgpbd_ctx* ctx, int enable) {
   CTX_CHECK("mgpbd_pcg_set_verify");
   if (enable < 0 || enable > 2) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_set_verify: 0, 1 or 2");
+  if ((enable != 0) != (ctx->verify != 0)) drop_graphs(ctx);   // the replays' check tail
   ctx->verify = enable;
   return 0;
 }

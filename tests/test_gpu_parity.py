@@ -389,7 +389,7 @@ def test_graph_replay_equals_eager(coarse_solver):
     `poll` at a time), for several poll lengths."""
     mesh, s, o = _extended_pair(coarse_solver=coarse_solver)
     mg = s._gpu_solver
-    ctx = s.ctx
+    ctx = mg.ctx   # the multigrid solver's own context runs the PCG
     results = []
     for mode, poll in ((0, 4), (1, 4), (0, 4), (1, 4), (1, 1), (1, 7), (1, 2)):
         ctx("mgpbd_ctx_use_graphs", mode, poll)
@@ -403,6 +403,37 @@ def test_graph_replay_equals_eager(coarse_solver):
     for it, x in results[1:]:
         assert it == it0 and np.array_equal(x.view(np.uint32), x0.view(np.uint32))
     ctx("mgpbd_ctx_use_graphs", 1, 4)
+
+
+@pytest.mark.parametrize("level0", ["assembled", "factored"])
+def test_graph_replay_with_verification_equals_eager(level0):
+    """With FP64 verification on (the factored level 0 folds the check into each replay's
+    gated tail), graph replays of several lengths and eager launches of the same system give
+    the same iteration count, the same x bits and the same verified FP64 residual."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(6, seed=3)
+    v0 = deformed(mesh, seed=3, amp=0.05, twist=0.1)
+    s = _solver(mesh, v0, mu_per_tet=mu_t, hold_pinned=True, hierarchy="extended",
+                galerkin="structural", max_coarse=64, coarse_solver="cholesky",
+                relative_tol=True, tol_cg=1e-6, precision="fp16", smoother="chebyshev",
+                verify_residual=1, fp64_iterate=True, level0=level0)
+    mg = s._gpu_solver
+    s._compute_C_and_gradC()
+    s._fill_A_and_update_gpu()          # one refresh: every solve below sees the same system
+    results = []
+    for mode, poll in ((0, 4), (1, 4), (1, 1), (0, 4), (1, 3), (1, 4)):
+        mg.ctx("mgpbd_ctx_use_graphs", mode, poll)
+        it = mg.solve(s._b_gpu, s._dlam_gpu)
+        torch.cuda.synchronize()
+        info = dict(mg.last_info)
+        results.append((it, s._dlam_gpu.cpu().numpy().copy(), info["rr_true"], info["reason"]))
+    it0, x0, rt0, re0 = results[0]
+    print(f"[graph-verify] {level0}: iters {it0}, rr_true {rt0:.3e}, reason {re0}", flush=True)
+    assert it0 >= 2 and re0 == 1 and 0.0 <= rt0 < 1e-12 * max(mg.last_info["bb"], 1e-300)
+    for it, x, rt, re in results[1:]:
+        assert it == it0 and re == re0 and rt == rt0
+        assert np.array_equal(x.view(np.uint32), x0.view(np.uint32))
+    mg.ctx("mgpbd_ctx_use_graphs", 1, 4)
 
 
 def _bf16_to_f32(u16):
