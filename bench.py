@@ -113,11 +113,11 @@ def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
 
 def factored_bytes(n_tets, n_verts, kind, chebyshev=False):
     """Algorithmic HBM bytes of one application of the factored level 0 (vertex pass + tet
-    pass), each array counted once per pass. V-cycle sweeps and residuals: vertex pass = 4 incidence slots per tet x (4-byte
-    incidence + 8-byte K) + x (4 n) + u written (16 per vertex); tet pass = t_i (16) + K (24)
-    + a^ (4) + x (4) + b (4) + y (4) per tet + u read (16 per vertex), + the previous iterate
-    (4 n) on average with the Chebyshev smoother (as for the stored levels). The residual pass
-    reads and writes the same arrays."""
+    pass), each array counted once per pass. V-cycle sweeps and residuals: vertex pass = 4
+    incidence slots per tet x (4-byte incidence + 8-byte K) + x (4 n) + u written (16 per
+    vertex); tet pass = t_i (16) + K (24) + a^ (4) + x (4) + b (4) + y (4) per tet + u read (16
+    per vertex), + the previous iterate (4 n) on average with the Chebyshev smoother (as for
+    the stored levels). The residual pass reads and writes the same arrays."""
     if kind == "spmv":
         # the PCG's FP64 product from the FP32 gradients: 4 slots per tet x (4-byte incidence +
         # 16-byte gradient) + p (4 n) + u (32 per vertex) written; t_i (16) + gradC (48) +
@@ -433,7 +433,9 @@ def main():
         "value": best["value"], "unit": "substeps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": best["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if prec == "fp32" else f"f32 outer PCG (f64 dots), {prec} V-cycle",
+        "dtype": "f32" if prec == "fp32" else (
+            f"{prec} V-cycle (f32 accumulation), f64 PCG products and dots (factored level 0)"
+            if args.level0 == "factored" else f"f32 outer PCG (f64 dots), {prec} V-cycle"),
         "data": "synthetic",
         "config": {"workload": cfg["label"], "seed": args.seed, "drag": bool(cfg.get("drag")),
                    "n_tets": mesh.n_tets, "n_verts": mesh.n_verts,

@@ -353,6 +353,12 @@ k_encoder(int n, const float* __restrict__ x, const float* __restrict__ P,
 __global__ void __launch_bounds__(kBlock)
 k_encoder8(int n, const float* __restrict__ x, const float* __restrict__ P,
            uint16_t* __restrict__ h0) {
+  // the encoder's weights and bias (F x H + H floats) staged in LDS once per block: the eight
+  // lanes of a row read eight distinct columns, conflict-free
+  __shared__ float wsh[F * H + H];
+  for (int k = threadIdx.x; k < F * H + H; k += kBlock)
+    wsh[k] = k < F * H ? P[off_win() + k] : P[off_bin() + (k - F * H)];
+  __syncthreads();
   const int t = blockIdx.x * kBlock + threadIdx.x;
   const int i = t >> 3, c0 = 8 * (t & 7);
   if (i >= n) return;
@@ -362,9 +368,9 @@ k_encoder8(int n, const float* __restrict__ x, const float* __restrict__ P,
   uint16_t out[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    float v = P[off_bin() + c0 + e];
+    float v = wsh[F * H + c0 + e];
 #pragma unroll
-    for (int k = 0; k < F; ++k) v = fmaf(xi[k], P[off_win() + k * H + c0 + e], v);
+    for (int k = 0; k < F; ++k) v = fmaf(xi[k], wsh[k * H + c0 + e], v);
     out[e] = f2bf(fmaxf(v, 0.0f));
   }
   uint4 pk;
