@@ -219,14 +219,16 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
 
 
 CHEBYSHEV = dict(smoother="chebyshev", n_smooth=5, smooth_schedule=[5, 1], cheb_ratio=30.0,
-                 smoother_safety=1.2, verify_residual=1, fp64_iterate=True)
+                 smoother_safety=1.2, verify_residual=1, fp64_iterate=True,
+                 spectral_refresh="substep")
 RUNS = {
     # the reference's algorithm (fixed 2/3-weighted Jacobi); the FP64 residual of every solve
     # is measured, not enforced
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
     # Chebyshev smoothing on the device's spectral estimates (5 + 5 steps on level 0, 1 + 1 on
     # the coarse levels, on [1.2 lambda / 30, 1.2 lambda], unless the workload sets its own
-    # schedule: CONFIGS "smoothing"), FP64 PCG iterate, every solve FP64-verified
+    # schedule: CONFIGS "smoothing"; spectral estimates refreshed at each substep's first
+    # Newton iteration), FP64 PCG iterate, every solve FP64-verified
     "fp32": dict(precision="fp32", **CHEBYSHEV),
     "fp16": dict(precision="fp16", **CHEBYSHEV),
     "bf16": dict(precision="bf16", **CHEBYSHEV),

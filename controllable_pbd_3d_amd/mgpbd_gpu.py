@@ -34,6 +34,10 @@ last entry repeating), ``f_ext``
 substep's lambda, or from the network's prediction with the previous lambda as the fallback;
 the device keeps the start with the smaller residual and iterates to the same tolerance.
 ``record_samples`` (a list) collects training pairs from the first Newton iteration.
+``spectral_refresh`` ('newton' | 'substep'): the spectral smoothers re-estimate every level's
+spectrum after every Newton iteration's refill, or only at a substep's first Newton iteration
+(the later iterations of a substep change the matrices little; the verified PCG and the stall
+rescue cover a lagging estimate).
 ``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is
 used as J W J^T + alpha~ in factored form from the tets' gradients (V-cycle sweeps from FP16/BF16
 K K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
@@ -85,9 +89,11 @@ class MGPBDSolverGPU:
                  smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
-                 device=None, verbose: bool = True) -> None:
+                 spectral_refresh: str = "newton", device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
+        if spectral_refresh not in ("newton", "substep"):
+            raise ValueError("spectral_refresh must be 'newton' or 'substep'")
         if level0 not in ("assembled", "factored"):
             raise ValueError("level0 must be 'assembled' or 'factored'")
         if level0 == "factored" and (precision == "fp32" or galerkin != "structural"):
@@ -139,6 +145,7 @@ class MGPBDSolverGPU:
         self.stall_rescue = bool(stall_rescue)
         self.smooth_schedule = smooth_schedule
         self.level0 = level0
+        self.spectral_refresh = spectral_refresh
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -335,12 +342,13 @@ class MGPBDSolverGPU:
                  dptr(A.row_len), dptr(self._A_desc), dptr(self.t_i), dptr(self.v_invm),
                  dptr(self.gradC), dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
 
-    def _fill_A_and_update_gpu(self):
+    def _fill_A_and_update_gpu(self, newton_iter: int = 0):
         # the factored level 0 reads the gradients; its assembled values are only wanted for
         # recorded training pairs
         if not self._gpu_solver.factored or self.record_samples is not None:
             self._fill_A()
-        self._gpu_solver.update_all_coarse_matrices()
+        refresh = self.spectral_refresh == "newton" or newton_iter == 0
+        self._gpu_solver.update_all_coarse_matrices(refresh_smoother=refresh)
 
     def _scatter_dlam_gpu(self):
         mg = self._gpu_solver
@@ -416,7 +424,7 @@ class MGPBDSolverGPU:
                 if self.benchmark is not None:
                     self.benchmark.record_iteration(ite, r)
                 break
-            self._fill_A_and_update_gpu()
+            self._fill_A_and_update_gpu(ite)
             if ite == 0 and self.warm_start != "none":
                 cg_iters = self._solve_warm_started()
             else:

@@ -495,9 +495,10 @@ class GPUMultigridSolver:
         self.levels[0].update_data(data)
         self.levels[0].cache_diag_inv()
 
-    def update_all_coarse_matrices(self):
+    def update_all_coarse_matrices(self, refresh_smoother: bool = True):
         """Coarse values from the fine ones through the refill maps
-        (cons/mgpcg_gpu.py:183-212)."""
+        (cons/mgpcg_gpu.py:183-212). refresh_smoother=False keeps the spectral smoother's
+        current estimates (extension: the caller knows the matrices changed little)."""
         for l, rf in enumerate(self.refill):
             F, C = self.levels[l], self.levels[l + 1]
             if l == 0 and self._fac is not None:
@@ -510,7 +511,7 @@ class GPUMultigridSolver:
             self.ctx("mgpbd_mg_factor_coarsest")
         if getattr(self, "_half", None):
             self.ctx("mgpbd_mg_convert_half")
-        if self.smoother != "jacobi":
+        if self.smoother != "jacobi" and refresh_smoother:
             self.ctx("mgpbd_mg_update_smoother", None)
 
     def device_bytes(self, seen=None) -> int:
