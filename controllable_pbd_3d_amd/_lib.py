@@ -96,6 +96,7 @@ SIGNATURES = {
     "mgpbd_mg_set_level_half": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
     "mgpbd_mg_set_level_window": (c_int, [_V, c_int, _V, _V, c_int, c_int, _V]),
     "mgpbd_mg_convert_half": (c_int, [_V]),
+    "mgpbd_mg_convert_half_levels": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_level0_factored": (c_int, [_V, c_int] + [_V] * 14),
     "mgpbd_mg_set_level1_factored": (c_int, [_V, c_int, _V, _V, _V, _V, c_int, _V, _V, _V, _V, _V, _V]),
     "mgpbd_mg_factored_refill": (c_int, [_V]),

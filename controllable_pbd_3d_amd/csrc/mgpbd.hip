@@ -3677,11 +3677,25 @@ int mgpbd_mg_factored_refill(mgpbd_ctx* ctx) {
   return 0;
 }
 
+static int convert_half_levels(mgpbd_ctx* ctx, int first, int last);
+
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx) {
   CTX_CHECK("mgpbd_mg_convert_half");
   if (int e = check_levels(ctx)) return e;
+  return convert_half_levels(ctx, 0, ctx->n_levels);
+}
+
+int mgpbd_mg_convert_half_levels(mgpbd_ctx* ctx, int first, int last) {
+  CTX_CHECK("mgpbd_mg_convert_half_levels");
+  if (int e = check_levels(ctx)) return e;
+  if (first < 0 || last > ctx->n_levels || first > last)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half_levels: 0 <= first <= last <= n_levels");
+  return convert_half_levels(ctx, first, last);
+}
+
+static int convert_half_levels(mgpbd_ctx* ctx, int first, int last) {
   if (ctx->precision == MGPBD_PREC_FP32) return 0;
-  for (int l = 0; l < ctx->n_levels; ++l) {
+  for (int l = first; l < last; ++l) {
     MgLevel& L = ctx->L[l];
     const bool fac = l == 0 && ctx->fac.on;
     if (!L.hval && !L.hpk && !fac) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_convert_half: level " + std::to_string(l) + " has no half buffers");

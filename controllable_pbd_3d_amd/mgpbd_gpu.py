@@ -38,6 +38,8 @@ the device keeps the start with the smaller residual and iterates to the same to
 spectrum after every Newton iteration's refill, or only at a substep's first Newton iteration
 (the later iterations of a substep change the matrices little; the verified PCG and the stall
 rescue cover a lagging estimate).
+``overlap_inference`` (default True; learned start with the factored level 0): the network runs
+on a second stream while the coarse levels refill (bit-identical to running it after them).
 ``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is
 used as J W J^T + alpha~ in factored form from the tets' gradients (V-cycle sweeps from FP16/BF16
 K K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
@@ -54,7 +56,7 @@ import torch
 from . import _lib
 from ._lib import check, dptr, np_ptr
 from .mgpcg_gpu import GPUMultigridSolver
-from .runtime import get_context, to_device
+from .runtime import get_context, new_context, to_device
 from .sparse_gpu import GPUSparseMatrix, gpu_copy, gpu_fill_zero
 from . import tet_aggregation
 
@@ -89,7 +91,8 @@ class MGPBDSolverGPU:
                  smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
-                 spectral_refresh: str = "newton", device=None, verbose: bool = True) -> None:
+                 spectral_refresh: str = "newton", overlap_inference: bool = True,
+                 device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
         if spectral_refresh not in ("newton", "substep"):
@@ -146,6 +149,9 @@ class MGPBDSolverGPU:
         self.smooth_schedule = smooth_schedule
         self.level0 = level0
         self.spectral_refresh = spectral_refresh
+        self.overlap_inference = bool(overlap_inference)
+        self._side = None          # (stream, context, events) of the overlapped inference
+        self._predicted = False
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
@@ -345,10 +351,35 @@ class MGPBDSolverGPU:
     def _fill_A_and_update_gpu(self, newton_iter: int = 0):
         # the factored level 0 reads the gradients; its assembled values are only wanted for
         # recorded training pairs
-        if not self._gpu_solver.factored or self.record_samples is not None:
+        mg = self._gpu_solver
+        if not mg.factored or self.record_samples is not None:
             self._fill_A()
         refresh = self.spectral_refresh == "newton" or newton_iter == 0
-        self._gpu_solver.update_all_coarse_matrices(refresh_smoother=refresh)
+        hook = None
+        if (newton_iter == 0 and self.warm_start == "learned" and self.overlap_inference and
+                mg.factored):
+            hook = self._launch_prediction
+        mg.update_all_coarse_matrices(refresh_smoother=refresh, after_level0=hook)
+        if hook is not None:
+            torch.cuda.current_stream(self.device).wait_event(self._side[3])
+
+    def _launch_prediction(self):
+        """The network's start for the first solve on a second stream (default priority), once
+        level 0's 1/diag is queued: it needs only the constraint pass's outputs and that
+        diagonal, so it runs while the coarse levels refill on the caller's stream."""
+        if self._side is None:
+            stream = torch.cuda.Stream(self.device)
+            with torch.cuda.stream(stream):
+                ctx = new_context(self.device.index)   # bound to the side stream
+            self._side = (stream, ctx, torch.cuda.Event(), torch.cuda.Event())
+        stream, ctx, ev_in, ev_out = self._side
+        ev_in.record(torch.cuda.current_stream(self.device))
+        stream.wait_event(ev_in)
+        self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
+                                      self.constraints, self.alpha_tilde, self.rest_vol,
+                                      self._dlam_gpu, ctx=ctx)
+        ev_out.record(stream)
+        self._predicted = True
 
     def _scatter_dlam_gpu(self):
         mg = self._gpu_solver
@@ -457,7 +488,9 @@ class MGPBDSolverGPU:
         if self.warm_start == "previous":
             gpu_copy(self._lam_prev, self._dlam_gpu, self.n_tets)
             return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu)
-        self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
-                                      self.constraints, self.alpha_tilde, self.rest_vol,
-                                      self._dlam_gpu)
+        if not self._predicted:   # else already queued beside the hierarchy refresh
+            self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
+                                          self.constraints, self.alpha_tilde, self.rest_vol,
+                                          self._dlam_gpu)
+        self._predicted = False
         return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu, self._lam_prev)

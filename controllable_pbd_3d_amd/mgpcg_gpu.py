@@ -495,10 +495,17 @@ class GPUMultigridSolver:
         self.levels[0].update_data(data)
         self.levels[0].cache_diag_inv()
 
-    def update_all_coarse_matrices(self, refresh_smoother: bool = True):
+    def update_all_coarse_matrices(self, refresh_smoother: bool = True, after_level0=None):
         """Coarse values from the fine ones through the refill maps
         (cons/mgpcg_gpu.py:183-212). refresh_smoother=False keeps the spectral smoother's
-        current estimates (extension: the caller knows the matrices changed little)."""
+        current estimates (extension: the caller knows the matrices changed little).
+        after_level0 (factored level 0 only): called once level 0's scales, 1/diag and K are
+        queued, before the coarse levels' work, so the caller can start work that needs only
+        those on another stream."""
+        early = after_level0 is not None and self._fac is not None
+        if early:
+            self.ctx("mgpbd_mg_convert_half_levels", 0, 1)
+            after_level0()
         for l, rf in enumerate(self.refill):
             F, C = self.levels[l], self.levels[l + 1]
             if l == 0 and self._fac is not None:
@@ -510,7 +517,10 @@ class GPUMultigridSolver:
         if getattr(self, "_coarse_mode", 0) == 1:
             self.ctx("mgpbd_mg_factor_coarsest")
         if getattr(self, "_half", None):
-            self.ctx("mgpbd_mg_convert_half")
+            if early:
+                self.ctx("mgpbd_mg_convert_half_levels", 1, self.n_levels)
+            else:
+                self.ctx("mgpbd_mg_convert_half")
         if self.smoother != "jacobi" and refresh_smoother:
             self.ctx("mgpbd_mg_update_smoother", None)
 

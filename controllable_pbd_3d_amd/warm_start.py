@@ -99,11 +99,12 @@ class LearnedWarmStart:
         return cls(neighbour_table(t_i, n_tets), **kw)
 
     # ------------------------------------------------------------------ inference
-    def features(self, b, dinv, lam_prev, C, alpha, vol, x=None, stats=None):
+    def features(self, b, dinv, lam_prev, C, alpha, vol, x=None, stats=None, ctx=None):
         x = self.x if x is None else x
         stats = self.stats if stats is None else stats
-        self.ctx("mgpbd_gnn_features", self.n, dptr(b), dptr(dinv), dptr(lam_prev), dptr(C),
-                 dptr(alpha), dptr(vol), dptr(self.nbr), dptr(x), dptr(stats), dptr(self.ws))
+        ctx = self.ctx if ctx is None else ctx
+        ctx("mgpbd_gnn_features", self.n, dptr(b), dptr(dinv), dptr(lam_prev), dptr(C),
+            dptr(alpha), dptr(vol), dptr(self.nbr), dptr(x), dptr(stats), dptr(self.ws))
         return x, stats
 
     def forward(self, x, stats, lam_prev, out=None):
@@ -111,17 +112,20 @@ class LearnedWarmStart:
                  dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
         return out
 
-    def infer(self, x, stats, lam_prev, out):
+    def infer(self, x, stats, lam_prev, out, ctx=None):
         """The forward's prediction through the fused inference kernels (bit-identical;
-        keeps no activations for a backward pass)."""
-        self.ctx("mgpbd_gnn_infer", self.n, dptr(self.nbr), dptr(x), dptr(stats),
-                 dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
+        keeps no activations for a backward pass). ctx: another context of the same device
+        (e.g. bound to a second stream), default the model's."""
+        ctx = self.ctx if ctx is None else ctx
+        ctx("mgpbd_gnn_infer", self.n, dptr(self.nbr), dptr(x), dptr(stats),
+            dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
         return out
 
-    def predict(self, b, dinv, lam_prev, C, alpha, vol, out):
-        """out = predicted dlambda of the first Newton iteration (device tensors, n floats)."""
-        x, st = self.features(b, dinv, lam_prev, C, alpha, vol)
-        return self.infer(x, st, lam_prev, out)
+    def predict(self, b, dinv, lam_prev, C, alpha, vol, out, ctx=None):
+        """out = predicted dlambda of the first Newton iteration (device tensors, n floats);
+        ctx: the context (stream) to run on, default the model's."""
+        x, st = self.features(b, dinv, lam_prev, C, alpha, vol, ctx=ctx)
+        return self.infer(x, st, lam_prev, out, ctx=ctx)
 
     # ------------------------------------------------------------------ training
     def make_sample(self, b, dinv, lam_prev, C, alpha, vol, target, matrix=None) -> dict:

@@ -429,6 +429,9 @@ int mgpbd_mg_factored_scatter(mgpbd_ctx* ctx, const float* dlam_dev, float* lamb
 /* Recomputes scales and reduced-precision values of every level from the FP32 values; call
  * after the values change (after the coarse refill). */
 int mgpbd_mg_convert_half(mgpbd_ctx* ctx);
+/* The same for levels first .. last-1 only (extension; e.g. the factored level 0 first, so
+ * that work needing its scales can start on another stream while the coarse levels refill). */
+int mgpbd_mg_convert_half_levels(mgpbd_ctx* ctx, int first, int last);
 /* r = b - A_l x with the operator the V-cycle applies on level l (no reference counterpart;
  * the reference computes this residual inside vcycle, cons/mgpcg_gpu.py:230-234): the FP32
  * level, or in reduced precision its scaled form (stored half values, or the factored level

@@ -350,3 +350,34 @@ def test_learned_start_gains_over_previous_lambda():
     assert p["first_solve_iters_learned"] <= p["first_solve_iters_previous"] + 0.25
     assert ev["learned"]["fp64_rel_residual_max"] < 1e-6
     assert ev["previous"]["fp64_rel_residual_max"] < 1e-6
+
+
+@pytest.mark.gpu
+def test_overlapped_inference_equals_serial():
+    """The network's first-solve start queued on a second stream beside the coarse-level
+    refill (factored level 0) gives the same positions, bit for bit, as running it after the
+    refill on the solver's stream."""
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    from controllable_pbd_3d_amd.warm_start import LearnedWarmStart
+    mesh, mu_t = physical_block(5, seed=31)
+    v0 = deformed(mesh, seed=31, amp=0.02, twist=0.05)
+    kw = solver_kwargs(mesh, mu_per_tet=mu_t, hold_pinned=True, relative_tol=True, tol_cg=1e-6,
+                       hierarchy="extended", galerkin="structural", max_coarse=64,
+                       coarse_solver="cholesky", smoother="chebyshev", verify_residual=1,
+                       fp64_iterate=True, rtol=1e-6, atol=1e-9, maxiter=30,
+                       precision="fp16", level0="factored", spectral_refresh="substep")
+    kw["v_p"] = v0
+    runs = []
+    for overlap in (False, True):
+        model = LearnedWarmStart.for_mesh(mesh.tets, mesh.n_tets, seed=3)
+        model.load_state_dict({"params": _trained_like_params(model, seed=7)})
+        s = MGPBDSolverGPU(**kw, warm_start="learned", warm_start_model=model,
+                           overlap_inference=overlap, verbose=False)
+        s.init()
+        for _ in range(3):
+            s.step(0)
+        assert all(d["reason"] != 0 for d in s.last_solve_info)
+        runs.append((_host(s.v_p), s._side is not None))
+    print(f"[overlap] box5 n_tets={mesh.n_tets}: side stream used {runs[1][1]}", flush=True)
+    assert not runs[0][1] and runs[1][1]
+    assert np.array_equal(runs[0][0], runs[1][0])
