@@ -203,7 +203,7 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
     kern = {}
     # the outer PCG SpMV is FP32 (values beside int32 columns, or beside 16-bit LDS offsets on
     # a windowed level 0); the V-cycle passes use the precision under test
-    vb = 8 if precision == "fp32" else 4
+    vb = spmv_per_nnz if precision == "fp32" else 4   # FP32 sweeps use the same level-0 layout
     for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3

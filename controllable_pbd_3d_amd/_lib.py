@@ -139,8 +139,15 @@ SIGNATURES = {
 _lib = None
 
 
+MGPBD_EINVAL, MGPBD_EHIP, MGPBD_ENONFINITE, MGPBD_EINTERNAL, MGPBD_ENOMEM = -1, -2, -3, -4, -5
+
+
 class MGPBDError(RuntimeError):
-    pass
+    """A libmgpbd entry point failed; ``status`` is its MGPBD_E* code when one was returned."""
+
+    def __init__(self, msg: str, status: int | None = None):
+        super().__init__(msg)
+        self.status = status
 
 
 def load():
@@ -163,7 +170,7 @@ def load():
 def check(status: int, what: str = "") -> None:
     if status != 0:
         msg = load().mgpbd_last_error().decode(errors="replace")
-        raise MGPBDError(f"{what or 'libmgpbd'} failed ({status}): {msg}")
+        raise MGPBDError(f"{what or 'libmgpbd'} failed ({status}): {msg}", status)
 
 
 def np_ptr(a, P=I32P):

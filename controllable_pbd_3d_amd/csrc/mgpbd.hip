@@ -2364,6 +2364,7 @@ k_fac_galerkin(int ne, const int* __restrict__ tptr, const int2* __restrict__ te
 
 using namespace mgpbd;
 
+
 // ============================================================================================
 // Context
 // ============================================================================================
@@ -3944,6 +3945,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
   const float* alt = x_alt;
   double first_info[4] = {0.0, 0.0, 0.0, 0.0};   // rr0, start, rr_guess, rr_alt of round 0
   double rr_true = -1.0;
+  bool nonfinite = false;   // NaN/Inf in ||b||^2 or the recurrence residual at a host poll
   for (;;) {
     k_pcg_reset<<<1, 1, 0, s>>>(ctx->st, tol, relative);
     if (from_guess) {
@@ -4006,6 +4008,10 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       MGPBD_HIP_TRY(hipStreamSynchronize(s));
       if (int e = prof_flush(ctx)) return e;
       if (ctx->st_host->done) break;
+      if (!std::isfinite(ctx->st_host->bb) || !std::isfinite(ctx->st_host->rr)) {
+        nonfinite = true;   // NaN never meets the stop tests: stop polling at once
+        break;
+      }
       if (watch) {
         rr_poll[n_poll % 64] = ctx->st_host->rr;
         it_poll[n_poll % 64] = round_issued;
@@ -4024,6 +4030,10 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     if (round_issued == 0) {
       MGPBD_HIP_TRY(hipMemcpyAsync(ctx->st_host, ctx->st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
       MGPBD_HIP_TRY(hipStreamSynchronize(s));
+    }
+    if (nonfinite || !std::isfinite(ctx->st_host->bb)) {
+      nonfinite = true;
+      break;
     }
     const bool tail_verified = tail_verify && ctx->st_host->verified == 1;
     if (x64 && !tail_verified) {
@@ -4077,6 +4087,14 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
   if (graph_mode) {
     MGPBD_HIP_TRY(hipEventRecord(ctx->ev_out, s));
     MGPBD_HIP_TRY(hipStreamWaitEvent(caller, ctx->ev_out, 0));
+  }
+  if (nonfinite) {
+    const PcgState& h = *ctx->st_host;
+    ctx->pred_iters = 0;
+    return mgpbd_fail(MGPBD_ENONFINITE,
+                      "mgpbd_pcg_solve: non-finite value (||b||^2 = " + std::to_string(h.bb) +
+                          ", ||r||^2 = " + std::to_string(h.rr) + " after " +
+                          std::to_string(issued) + " iterations)");
   }
   if (iters_out) *iters_out = total_iters;
   if (info_out) {

@@ -54,7 +54,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, dptr, np_ptr
+from ._lib import MGPBDError, check, dptr, np_ptr
 from .mgpcg_gpu import GPUMultigridSolver
 from .runtime import get_context, new_context, to_device
 from .sparse_gpu import GPUSparseMatrix, gpu_copy, gpu_fill_zero
@@ -340,7 +340,13 @@ class MGPBDSolverGPU:
                  dptr(self.constraints), dptr(self.gradC), dptr(self._b_gpu), dptr(self._norms))
 
     def _residual_norm(self) -> float:
-        return math.sqrt(float(self._norms[0].item()))
+        r2 = float(self._norms[0].item())
+        if not math.isfinite(r2):
+            # a NaN dual residual never meets the stop test and would run every Newton and
+            # PCG iteration to its cap; report it instead (status MGPBD_ENONFINITE)
+            raise MGPBDError(f"non-finite dual residual ||b||^2 = {r2} "
+                             f"(status {_lib.MGPBD_ENONFINITE}): NaN/Inf in positions or state")
+        return math.sqrt(r2)
 
     def _fill_A(self):
         A = self._A

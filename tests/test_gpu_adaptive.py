@@ -138,4 +138,7 @@ def test_solver_benchmark_hooks_record_the_newton_loop():
         assert fr["frame_id"] == f and fr["t_end"] >= fr["t_start"]
         assert [(d["iteration"], d["residual"], d["cg_iters"], d["omega"]) for d in fr["iterations"]] == \
             [(i, float(r), c, float(w)) for i, r, c, w in s.last_stats]
+        # device-timeline marks: ordered within the frame
+        ts = [fr["t_start"]] + [d["t"] for d in fr["iterations"]] + [fr["t_end"]]
+        assert all(a <= b for a, b in zip(ts, ts[1:])), ts
     assert b.summary()["n_frames"] == 3
