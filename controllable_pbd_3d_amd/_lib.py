@@ -66,6 +66,7 @@ SIGNATURES = {
                               c_float, c_float, c_int]),
     "mgpbd_update_vel": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float]),
     "mgpbd_constraints": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_constraints_noalloc": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_fill_A": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_fill_A_desc": (c_int, [_V, c_int, _V, _V, _V, _V, _V]),
     "mgpbd_fill_A_shared": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),

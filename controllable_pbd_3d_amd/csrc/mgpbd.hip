@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2360,10 +2361,45 @@ k_fac_galerkin(int ne, const int* __restrict__ tptr, const int2* __restrict__ te
   cval[cpos_t[c]] = v;   // the transposed entry (the same slot on the diagonal)
 }
 
+// ------------------------------------------------------------------------------------------
+// Allocation-free constraint pass: the FP64 block partials of ||rhs||^2 live in this
+// file-scope array instead of a context's workspace. k_constraints runs at most
+// kMaxRedBlocks blocks (grid-stride), so kMaxRedBlocks partials always suffice.
+// ------------------------------------------------------------------------------------------
+constexpr int kNoAllocPartials = kMaxRedBlocks;               // 2048 doubles = 16 KiB per GPU
+constexpr int64_t kNoAllocMaxTets = int64_t(1) << 28;         // 12 * n_tets floats fit size_t math
+__device__ __align__(256) double scratch[kNoAllocPartials];
+
 }  // namespace mgpbd
 
 using namespace mgpbd;
 
+namespace {
+// Device address of `scratch` on each device, looked up once per device with that device
+// current (every GPU holds its own copy of the array at its own address). The host-side
+// symbol is only a stand-in: kernels and copies must use these addresses.
+constexpr int kMaxScratchDevices = 64;
+double* g_scratch_dev[kMaxScratchDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_address(double** out) {
+  int dev = -1;
+  MGPBD_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(mgpbd::scratch));
+    if (e != hipSuccess || !p)
+      return mgpbd_fail(MGPBD_EHIP, std::string("mgpbd_constraints_noalloc: hipGetSymbolAddress: ") +
+                                        hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+}  // namespace
 
 // ============================================================================================
 // Context
@@ -4160,5 +4196,30 @@ int mgpbd_pcg_solve_guess(mgpbd_ctx* ctx, const float* b, float* x, const float*
 }
 
 void* mgpbd_internal_stream(mgpbd_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int mgpbd_constraints_noalloc(void* stream, int n_tets, const int32_t* t_i, const float* v_p,
+                              const float* Dm_inv, const float* alpha, const float* lambda,
+                              float* C, float* gradC, float* rhs, double* norm2) {
+  if (n_tets < 0 || n_tets > kNoAllocMaxTets)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints_noalloc: n_tets must be in [0, 2^28] "
+                                    "(got " + std::to_string(n_tets) + ")");
+  if (n_tets > 0 && (!t_i || !v_p || !Dm_inv || !alpha || !lambda || !C || !gradC || !rhs))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints_noalloc: null array");
+  if (!norm2) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints_noalloc: null norm2");
+  double* part = nullptr;
+  if (int e = scratch_address(&part)) return e;
+  const int g = red_blocks(n_tets);
+  if (g > kNoAllocPartials)
+    return mgpbd_fail(MGPBD_EINTERNAL, "mgpbd_constraints_noalloc: grid exceeds the scratch");
+  // every block stores its partial and the finaliser sums exactly g of them: nothing in the
+  // scratch has to start at zero
+  RedWS ws{part};
+  hipStream_t s = (hipStream_t)stream;
+  k_constraints<<<g, kBlock, 0, s>>>(n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv,
+                                     alpha, lambda, C, gradC, rhs, ws);
+  k_fin_sum<<<1, kBlock, 0, s>>>(ws, g, 0, 1, norm2);
+  MGPBD_LAUNCH_CHECK("mgpbd_constraints_noalloc");
+  return 0;
+}
 
 }  // extern "C"

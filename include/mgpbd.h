@@ -189,6 +189,18 @@ int mgpbd_update_vel(mgpbd_ctx* ctx, int n_verts, const float* v_p_dev,
 int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
                       const float* Dm_inv_dev, const float* alpha_dev, const float* lambda_dev,
                       float* C_dev, float* gradC_dev, float* rhs_dev, double* norm2_dev);
+/* The same constraint pass without a context and without allocating: no counterpart in the
+ * reference. For callers that cannot call hipMalloc (HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). Same kernels and results as mgpbd_constraints, queued
+ * on `stream` (a hipStream_t, NULL = the null stream). The ||rhs||^2 block partials live in
+ * a file-scope device array of 2048 doubles (16 KiB), held by every GPU the library runs on
+ * until the process ends; its address is looked up once per device on the first call there.
+ * n_tets must be in [0, 2^28] (MGPBD_EINVAL otherwise). Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+int mgpbd_constraints_noalloc(void* stream, int n_tets, const int32_t* t_i_dev,
+                              const float* v_p_dev, const float* Dm_inv_dev,
+                              const float* alpha_dev, const float* lambda_dev, float* C_dev,
+                              float* gradC_dev, float* rhs_dev, double* norm2_dev);
 
 /* A values on the level-0 sliced pattern plus 1/diag (0 when |diag| <= 1e-30).
  * Replaces _fill_A_kernel (:273-309) with the shared-vertex sum order of _intersect_tets
