@@ -62,6 +62,10 @@ SIGNATURES = {
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),
+    "mgpbd_rest_shape_vol": (c_int, [_V, c_int, _V, _V, _V, c_double, _V, c_double, c_double,
+                                     _V, _V, _V]),
+    "mgpbd_constraints_vol": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_dual_norms_vol": (c_int, [_V, c_int, _V, _V, _V, c_float, _V, _V, _V, _V, c_int, _V]),
     "mgpbd_predict": (c_int, [_V, c_int, _V, _V, _V, _V, _V, c_float, c_float, c_float,
                               c_float, c_float, c_int]),
     "mgpbd_update_vel": (c_int, [_V, c_int, _V, _V, _V, c_float, c_float]),
@@ -108,6 +112,7 @@ SIGNATURES = {
     "mgpbd_mg_set_level_smoothing": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_smoother": (c_int, [_V, c_int, c_float, c_float, c_int, c_float]),
     "mgpbd_mg_update_smoother": (c_int, [_V, F64P]),
+    "mgpbd_mg_set_level_weights": (c_int, [_V, c_int, F32P]),
     "mgpbd_mg_get_smoother": (c_int, [_V, F32P, F32P]),
     "mgpbd_pcg_solve": (c_int, [_V, _V, _V, _V, c_int, c_double, c_int,
                                 ctypes.POINTER(c_int), F64P]),

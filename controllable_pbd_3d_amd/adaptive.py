@@ -99,13 +99,17 @@ class MGPBDAdaptiveSolver:
 
     def _apply_active_mask(self):
         s = self.solver
-        s.ctx("mgpbd_zone_mask_alpha", self.n_tets, dptr(self.zone.tet_active),
+        per = getattr(s, "per_tet", 1)
+        # with the volume constraint both rows of a tet follow the tet's flag
+        flags = self.zone.tet_active if per == 1 else self.zone.tet_active.repeat_interleave(per)
+        s.ctx("mgpbd_zone_mask_alpha", per * self.n_tets, dptr(flags),
               dptr(s.alpha_tilde), dptr(self._original_alpha), self.PASSIVE_ALPHA)
         if hasattr(s, "_alpha_tilde_np"):
             s._alpha_tilde_np = s.alpha_tilde.cpu().numpy()
 
     def _restore_mask(self):
         s = self.solver
-        s.ctx("mgpbd_copy", self.n_tets, dptr(self._original_alpha), dptr(s.alpha_tilde))
+        s.ctx("mgpbd_copy", s.alpha_tilde.numel(), dptr(self._original_alpha),
+              dptr(s.alpha_tilde))
         if hasattr(s, "_alpha_tilde_np"):
             s._alpha_tilde_np = s.alpha_tilde.cpu().numpy()

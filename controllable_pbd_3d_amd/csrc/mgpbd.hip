@@ -119,10 +119,12 @@ __device__ __forceinline__ void load3(const float* __restrict__ p, int v, float*
 }
 
 // Rest shape in FP64 (rounded once to FP32): Dm^-1 = adj(Dm) / det, vol = |det| / 6,
-// alpha = 1 / (mu dt^2 vol).
+// alpha = 1 / (mu dt^2 vol). With the volume constraint (lam_t or lam > 0 given, vol_rows)
+// tet t owns constraints 2t (ARAP) and 2t+1 (volume, alpha = 1 / (lambda dt^2 vol)).
 __global__ void __launch_bounds__(kBlock)
 k_rest_shape(int n, const int4* __restrict__ t_i, const float* __restrict__ xr,
-             const float* __restrict__ mu_t, double mu, double dt, float* __restrict__ Dm_inv,
+             const float* __restrict__ mu_t, double mu, const float* __restrict__ lam_t,
+             double lam, int vol_rows, double dt, float* __restrict__ Dm_inv,
              float* __restrict__ vol_out, float* __restrict__ alpha) {
   int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
@@ -153,7 +155,13 @@ k_rest_shape(int n, const int4* __restrict__ t_i, const float* __restrict__ xr,
   double vol = fabs(det) / 6.0;
   vol_out[t] = (float)vol;
   double m = mu_t ? (double)mu_t[t] : mu;
-  alpha[t] = (float)(1.0 / (m * dt * dt * vol));
+  if (vol_rows) {
+    double l = lam_t ? (double)lam_t[t] : lam;
+    alpha[2 * (size_t)t] = (float)(1.0 / (m * dt * dt * vol));
+    alpha[2 * (size_t)t + 1] = (float)(1.0 / (l * dt * dt * vol));
+  } else {
+    alpha[t] = (float)(1.0 / (m * dt * dt * vol));
+  }
 }
 
 // F = Ds Dm^-1 (FP64) for tet t at positions x (+ omega * dx in FP32 when dx != nullptr,
@@ -197,6 +205,47 @@ __device__ __forceinline__ double arap_C(const double s[3]) {
   return sqrt(a * a + b * b + c * c);
 }
 
+// Volume constraint det F - 1 and dC/dF = cof(F): column c of the cofactor matrix is the
+// cross product of F's columns c+1 and c+2 (cyclic). G[r][c], returns det F - 1.
+__device__ __forceinline__ double volume_C(const double F[9], double G[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int a = (c + 1) % 3, b = (c + 2) % 3;
+    G[0][c] = F[3 + a] * F[6 + b] - F[6 + a] * F[3 + b];
+    G[1][c] = F[6 + a] * F[0 + b] - F[0 + a] * F[6 + b];
+    G[2][c] = F[0 + a] * F[3 + b] - F[3 + a] * F[0 + b];
+  }
+  const double det = F[0] * G[0][0] + F[3] * G[1][0] + F[6] * G[2][0];
+  return det - 1.0;
+}
+
+// Vertex gradients g_k = G Dm^-T[:, k-1] (k = 1..3), g_0 = -(g_1 + g_2 + g_3), rounded to
+// FP32 and stored as three float4 at gradC + 12 c.
+__device__ __forceinline__ void store_gradient(const double G[3][3], const float Bf[9],
+                                               float* __restrict__ gradC, size_t c) {
+  double gd[4][3];
+#pragma unroll
+  for (int k = 1; k <= 3; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      gd[k][r] = G[r][0] * (double)Bf[3 * (k - 1)] + G[r][1] * (double)Bf[3 * (k - 1) + 1] +
+                 G[r][2] * (double)Bf[3 * (k - 1) + 2];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) gd[0][r] = -gd[1][r] - gd[2][r] - gd[3][r];
+  float g[12];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) g[3 * k + r] = (float)gd[k][r];
+  float4* go = reinterpret_cast<float4*>(gradC + 12 * c);
+  go[0] = make_float4(g[0], g[1], g[2], g[3]);
+  go[1] = make_float4(g[4], g[5], g[6], g[7]);
+  go[2] = make_float4(g[8], g[9], g[10], g[11]);
+}
+
+// VOL: two constraints per tet, 2t the ARAP constraint and 2t+1 the volume constraint (its
+// gradient is never zeroed); the per-constraint arrays hold 2 n entries.
+template <bool VOL>
 __global__ void __launch_bounds__(kBlock)
 k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
               const float* __restrict__ Dm_inv, const float* __restrict__ alpha,
@@ -204,6 +253,7 @@ k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
               float* __restrict__ gradC, float* __restrict__ rhs, RedWS ws) {
   double acc = 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
+    const size_t c0 = VOL ? 2 * (size_t)t : (size_t)t;
     int4 tv = t_i[t];
     float Bf[9];
 #pragma unroll
@@ -241,19 +291,29 @@ k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
 #pragma unroll
       for (int k = 0; k < 12; ++k) g[k] = 0.0f;
     }
-    C_out[t] = C;
-    float4* go = reinterpret_cast<float4*>(gradC + 12 * (size_t)t);
+    C_out[c0] = C;
+    float4* go = reinterpret_cast<float4*>(gradC + 12 * c0);
     go[0] = make_float4(g[0], g[1], g[2], g[3]);
     go[1] = make_float4(g[4], g[5], g[6], g[7]);
     go[2] = make_float4(g[8], g[9], g[10], g[11]);
-    float b = -(C + alpha[t] * lambda[t]);
-    rhs[t] = b;
+    float b = -(C + alpha[c0] * lambda[c0]);
+    rhs[c0] = b;
     acc += (double)b * (double)b;
+    if (VOL) {
+      double Gh[3][3];
+      const float Ch = (float)volume_C(F, Gh);
+      store_gradient(Gh, Bf, gradC, c0 + 1);
+      C_out[c0 + 1] = Ch;
+      const float bh = -(Ch + alpha[c0 + 1] * lambda[c0 + 1]);
+      rhs[c0 + 1] = bh;
+      acc += (double)bh * (double)bh;
+    }
   }
   double bs = block_sum(acc);
   store_partial(ws, 0, bs);
 }
 
+template <bool VOL>
 __global__ void __launch_bounds__(kBlock)
 k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
              const float* __restrict__ dx, float omega, const float* __restrict__ Dm_inv,
@@ -261,7 +321,8 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
              const float* __restrict__ C_cur, int with_current, RedWS ws) {
   double acc_cur = 0.0, acc_trial = 0.0;
   for (int t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
-    float al = alpha[t] * lambda[t];
+    const size_t c0 = VOL ? 2 * (size_t)t : (size_t)t;
+    float al = alpha[c0] * lambda[c0];
     if (with_current != 2) {   // 2: the current norm only
       int4 tv = t_i[t];
       float B[9];
@@ -274,10 +335,20 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
       float c = (float)arap_C(s);
       float r = -(c + al);
       acc_trial += (double)r * (double)r;
+      if (VOL) {
+        double Gh[3][3];
+        const float ch = (float)volume_C(F, Gh);
+        const float rh = -(ch + alpha[c0 + 1] * lambda[c0 + 1]);
+        acc_trial += (double)rh * (double)rh;
+      }
     }
     if (with_current) {
-      float rc = -(C_cur[t] + al);
+      float rc = -(C_cur[c0] + al);
       acc_cur += (double)rc * (double)rc;
+      if (VOL) {
+        const float rh = -(C_cur[c0 + 1] + alpha[c0 + 1] * lambda[c0 + 1]);
+        acc_cur += (double)rh * (double)rh;
+      }
     }
   }
   double b0 = block_sum(acc_cur);
@@ -383,8 +454,11 @@ k_fill_A(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
 // Shared-vertex descriptors of the level-0 pattern, one 16-bit word per SELL slot: bits 12-13
 // the number of vertices row i shares with column j (1-3), and per shared vertex, in ascending
 // local slot a of tet i, 4 bits (a | b << 2) with b the vertex's local slot in tet j;
-// kDescDiag marks the diagonal, 0 padding. The pattern and t_i fix them: built once.
+// kDescDiag marks the diagonal, 0 padding. Rows that share all four vertices (the ARAP and
+// volume constraints of one tet) are kDescAll with, per local slot a of row i, 2 bits at 2a:
+// the slot b of the same vertex in row j. The pattern and t_i fix them: built once.
 constexpr uint16_t kDescDiag = 0x8000;
+constexpr uint16_t kDescAll = 0x4000;
 
 __global__ void __launch_bounds__(kBlock)
 k_fill_desc(int n, const int* __restrict__ slice_ptr, const int* __restrict__ idx,
@@ -407,11 +481,15 @@ k_fill_desc(int n, const int* __restrict__ slice_ptr, const int* __restrict__ id
       } else {
         const int4 tj = t_i[j];
         const int vj[4] = {tj.x, tj.y, tj.z, tj.w};
-        int cnt = 0, code = 0;
+        int cnt = 0, code = 0, perm = 0;
         for (int a = 0; a < 4; ++a)
           for (int b = 0; b < 4; ++b)
-            if (vi[a] == vj[b]) { code |= (a | (b << 2)) << (4 * cnt); ++cnt; }
-        d = (uint16_t)((cnt << 12) | code);
+            if (vi[a] == vj[b]) {
+              if (cnt < 3) code |= (a | (b << 2)) << (4 * cnt);
+              perm |= b << (2 * a);
+              ++cnt;
+            }
+        d = cnt == 4 ? (uint16_t)(kDescAll | perm) : (uint16_t)((cnt << 12) | code);
       }
     }
     desc[p] = d;
@@ -450,15 +528,16 @@ k_fill_A_desc(int n, const int* __restrict__ slice_ptr, const int* __restrict__ 
     const int cj[4] = {c.x, c.y, c.z, c.w};
     const uint32_t dq[4] = {dd.x, dd.y, dd.z, dd.w};
     // gathers of every shared vertex's gradient first, then the arithmetic
-    float g[4][3][3];
+    float g[4][4][3];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const uint32_t d = dq[q];
-      const int cnt = (d & kDescDiag) ? 0 : (int)((d >> 12) & 3u);
+      const bool all = (d & kDescAll) != 0;
+      const int cnt = (d & kDescDiag) ? 0 : all ? 4 : (int)((d >> 12) & 3u);
       const float* gj = gradC + 12 * (size_t)cj[q];
 #pragma unroll
-      for (int e = 0; e < 3; ++e) {
-        const int b = (int)((d >> (4 * e + 2)) & 3u);
+      for (int e = 0; e < 4; ++e) {
+        const int b = all ? (int)((d >> (2 * e)) & 3u) : (int)((d >> (4 * e + 2)) & 3u);
         if (e < cnt) {
           g[q][e][0] = gj[3 * b];
           g[q][e][1] = gj[3 * b + 1];
@@ -481,11 +560,12 @@ k_fill_A_desc(int n, const int* __restrict__ slice_ptr, const int* __restrict__ 
         v[q] = dv + al;
         diag = v[q];
       } else {
-        const int cnt = (int)((d >> 12) & 3u);
+        const bool all = (d & kDescAll) != 0;
+        const int cnt = all ? 4 : (int)((d >> 12) & 3u);
 #pragma unroll
-        for (int e = 0; e < 3; ++e) {
+        for (int e = 0; e < 4; ++e) {
           if (e < cnt) {
-            const int a = (int)((d >> (4 * e)) & 3u);
+            const int a = all ? e : (int)((d >> (4 * e)) & 3u);
             const float ga[3] = {a == 0 ? gi[0][0] : a == 1 ? gi[1][0] : a == 2 ? gi[2][0] : gi[3][0],
                                  a == 0 ? gi[0][1] : a == 1 ? gi[1][1] : a == 2 ? gi[2][1] : gi[3][1],
                                  a == 0 ? gi[0][2] : a == 1 ? gi[1][2] : a == 2 ? gi[2][2] : gi[3][2]};
@@ -2470,6 +2550,7 @@ struct mgpbd_ctx {
   // spectral smoother: per-level weights omega_l = target / (safety lambda_l) in device memory,
   // lambda_l from a power iteration on D^-1 A warm-started across updates
   bool spectral = false;
+  bool fixed_weights = false;   // per-level weights set by the caller (no spectral refresh)
   bool chebyshev = false;       // spectral weights as a Chebyshev polynomial smoother
   float cheb_ratio = 12.0f;     // smoothed interval [lambda_max / ratio, lambda_max]
   float* cheb_dev = nullptr;    // MGPBD_MAX_LEVELS x kChebSteps x (c1, c2)
@@ -2718,8 +2799,24 @@ int mgpbd_rest_shape(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float
   if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_rest_shape: n_tets < 0");
   if (n_tets == 0) return 0;
   k_rest_shape<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(
-      n_tets, reinterpret_cast<const int4*>(t_i), v_ref, mu_t, mu, dt, Dm_inv, vol, alpha);
+      n_tets, reinterpret_cast<const int4*>(t_i), v_ref, mu_t, mu, nullptr, 0.0, 0, dt, Dm_inv,
+      vol, alpha);
   MGPBD_LAUNCH_CHECK("mgpbd_rest_shape");
+  return 0;
+}
+
+int mgpbd_rest_shape_vol(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_ref,
+                         const float* mu_t, double mu, const float* lam_t, double lam, double dt,
+                         float* Dm_inv, float* vol, float* alpha) {
+  CTX_CHECK("mgpbd_rest_shape_vol");
+  if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_rest_shape_vol: n_tets < 0");
+  if (!lam_t && !(lam > 0.0))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_rest_shape_vol: lambda must be > 0 (or per tet)");
+  if (n_tets == 0) return 0;
+  k_rest_shape<<<nblocks(n_tets), kBlock, 0, ctx->stream>>>(
+      n_tets, reinterpret_cast<const int4*>(t_i), v_ref, mu_t, mu, lam_t, lam, 1, dt, Dm_inv,
+      vol, alpha);
+  MGPBD_LAUNCH_CHECK("mgpbd_rest_shape_vol");
   return 0;
 }
 
@@ -2761,12 +2858,28 @@ int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const floa
   if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints: n_tets < 0");
   int pe = prof_begin(ctx, MGPBD_PROF_CONSTRAINTS);
   const int g = red_blocks(n_tets);
-  k_constraints<<<g, kBlock, 0, ctx->stream>>>(
+  k_constraints<false><<<g, kBlock, 0, ctx->stream>>>(
       n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv, alpha, lambda, C, gradC, rhs,
       ctx->ws);
   k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, 0, 1, norm2);
   prof_end(ctx, pe);
   MGPBD_LAUNCH_CHECK("mgpbd_constraints");
+  return 0;
+}
+
+int mgpbd_constraints_vol(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_p,
+                          const float* Dm_inv, const float* alpha, const float* lambda, float* C,
+                          float* gradC, float* rhs, double* norm2) {
+  CTX_CHECK("mgpbd_constraints_vol");
+  if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_constraints_vol: n_tets < 0");
+  int pe = prof_begin(ctx, MGPBD_PROF_CONSTRAINTS);
+  const int g = red_blocks(n_tets);
+  k_constraints<true><<<g, kBlock, 0, ctx->stream>>>(
+      n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv, alpha, lambda, C, gradC, rhs,
+      ctx->ws);
+  k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, 0, 1, norm2);
+  prof_end(ctx, pe);
+  MGPBD_LAUNCH_CHECK("mgpbd_constraints_vol");
   return 0;
 }
 
@@ -2834,11 +2947,25 @@ int mgpbd_dual_norms(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float
   CTX_CHECK("mgpbd_dual_norms");
   if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_dual_norms: n < 0");
   const int g = red_blocks(n_tets);
-  k_dual_norms<<<g, kBlock, 0, ctx->stream>>>(
+  k_dual_norms<false><<<g, kBlock, 0, ctx->stream>>>(
       n_tets, reinterpret_cast<const int4*>(t_i), v_p, dpos, omega, Dm_inv, alpha, lambda, C,
       with_current, ctx->ws);
   k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, with_current ? 0 : 1, 2, out);
   MGPBD_LAUNCH_CHECK("mgpbd_dual_norms");
+  return 0;
+}
+
+int mgpbd_dual_norms_vol(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i, const float* v_p,
+                         const float* dpos, float omega, const float* Dm_inv, const float* alpha,
+                         const float* lambda, const float* C, int with_current, double* out) {
+  CTX_CHECK("mgpbd_dual_norms_vol");
+  if (n_tets < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_dual_norms_vol: n < 0");
+  const int g = red_blocks(n_tets);
+  k_dual_norms<true><<<g, kBlock, 0, ctx->stream>>>(
+      n_tets, reinterpret_cast<const int4*>(t_i), v_p, dpos, omega, Dm_inv, alpha, lambda, C,
+      with_current, ctx->ws);
+  k_fin_sum<<<1, kBlock, 0, ctx->stream>>>(ctx->ws, g, with_current ? 0 : 1, 2, out);
+  MGPBD_LAUNCH_CHECK("mgpbd_dual_norms_vol");
   return 0;
 }
 
@@ -3424,6 +3551,7 @@ int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, 
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_smoother: mode 0/1/2, target in (0, 2), "
                                     "safety >= 1, power_iters in [1, 64], cheb_ratio > 1");
   ctx->spectral = mode >= 1;
+  ctx->fixed_weights = false;
   ctx->chebyshev = mode == 2;
   ctx->cheb_ratio = cheb_ratio;
   ctx->sm_target = target;
@@ -3525,10 +3653,26 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
   return 0;
 }
 
+int mgpbd_mg_set_level_weights(mgpbd_ctx* ctx, int n, const float* omega) {
+  CTX_CHECK("mgpbd_mg_set_level_weights");
+  if (n < 1 || !omega) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_weights: n >= 1 weights");
+  for (int i = 0; i < n; ++i)
+    if (!(omega[i] > 0.0f && omega[i] < 2.0f))
+      return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_weights: weights in (0, 2)");
+  ctx->spectral = true;        // the sweeps read their weight from omega_dev
+  ctx->fixed_weights = true;
+  ctx->chebyshev = false;
+  for (int l = 0; l < MGPBD_MAX_LEVELS; ++l)
+    k_set_float<<<1, 64, 0, ctx->stream>>>(ctx->omega_dev + l, omega[std::min(l, n - 1)]);
+  drop_graphs(ctx);
+  MGPBD_LAUNCH_CHECK("mgpbd_mg_set_level_weights");
+  return 0;
+}
+
 int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out) {
   CTX_CHECK("mgpbd_mg_update_smoother");
   if (int e = check_levels(ctx)) return e;
-  if (!ctx->spectral) return 0;
+  if (!ctx->spectral || ctx->fixed_weights) return 0;
   hipStream_t s = ctx->stream;
   if (int e = smoother_refresh(ctx, false, 1.0f)) return e;
   if (lambda_out) {
@@ -4019,7 +4163,7 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     MGPBD_LAUNCH_CHECK("mgpbd_pcg_solve(init)");
 
     int round_issued = 0;
-    const bool watch = ctx->rescue && ctx->spectral && !rescued;
+    const bool watch = ctx->rescue && ctx->spectral && !ctx->fixed_weights && !rescued;
     double rr_poll[64];    // recurrence residual and iteration count at each host poll
     int it_poll[64];
     int n_poll = 0;
@@ -4215,7 +4359,7 @@ int mgpbd_constraints_noalloc(void* stream, int n_tets, const int32_t* t_i, cons
   // scratch has to start at zero
   RedWS ws{part};
   hipStream_t s = (hipStream_t)stream;
-  k_constraints<<<g, kBlock, 0, s>>>(n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv,
+  k_constraints<false><<<g, kBlock, 0, s>>>(n_tets, reinterpret_cast<const int4*>(t_i), v_p, Dm_inv,
                                      alpha, lambda, C, gradC, rhs, ws);
   k_fin_sum<<<1, kBlock, 0, s>>>(ws, g, 0, 1, norm2);
   MGPBD_LAUNCH_CHECK("mgpbd_constraints_noalloc");

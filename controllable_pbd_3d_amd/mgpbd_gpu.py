@@ -27,7 +27,8 @@ steps per refresh of the spectral estimates once settled), ``smoother_safety`` a
 ``cheb_ratio`` (upper end safety factor over the estimate and interval ratio of the spectral
 smoothers), ``stall_rescue`` (re-estimate the spectra and restart a PCG that stops
 contracting; spectral smoothers only), ``smooth_schedule`` (smoothing steps per level, the
-last entry repeating), ``f_ext``
+last entry repeating), ``omega_levels`` (fixed weighted-Jacobi weight per level, the last entry
+repeating, instead of the common ``omega_jacobi``), ``f_ext``
 (external vertex forces), ``device``,
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
@@ -44,6 +45,13 @@ on a second stream while the coarse levels refill (bit-identical to running it a
 used as J W J^T + alpha~ in factored form from the tets' gradients (V-cycle sweeps from FP16/BF16
 K K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
 aggregates' summed gradients) and is not reassembled per Newton iteration.
+``volume_lambda`` (None: the reference's single ARAP constraint per tet; a float or a per-tet
+array: the Lame lambda of cons/vbd.py:44-47): every tet also carries the volume constraint
+C_H = det F - 1 (the hydrostatic constraint of cons/deform3d.py:81-91) with compliance
+1 / (lambda dt^2 vol). Tet t then owns constraints 2t (ARAP) and 2t+1 (volume): the
+per-constraint arrays (``alpha_tilde``, ``lambdaf``, ``constraints``, ``gradC``) hold
+``n_cons = 2 n_tets`` entries, A has one row per constraint (sibling rows couple through all
+four vertices) and both rows of a tet share its aggregate.
 """
 
 from __future__ import annotations
@@ -92,7 +100,8 @@ class MGPBDSolverGPU:
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
                  spectral_refresh: str = "newton", overlap_inference: bool = True,
-                 device=None, verbose: bool = True) -> None:
+                 volume_lambda=None, omega_levels=None, device=None,
+                 verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
         if spectral_refresh not in ("newton", "substep"):
@@ -147,6 +156,7 @@ class MGPBDSolverGPU:
         self.cheb_ratio = float(cheb_ratio)
         self.stall_rescue = bool(stall_rescue)
         self.smooth_schedule = smooth_schedule
+        self.omega_levels = omega_levels
         self.level0 = level0
         self.spectral_refresh = spectral_refresh
         self.overlap_inference = bool(overlap_inference)
@@ -154,6 +164,22 @@ class MGPBDSolverGPU:
         self._predicted = False
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
+        self.volume = volume_lambda is not None
+        self.lam_per_tet, self.lam = None, 0.0
+        if self.volume:
+            if np.ndim(volume_lambda) == 0 and not isinstance(volume_lambda, torch.Tensor):
+                self.lam = float(volume_lambda)
+                if not self.lam > 0:
+                    raise ValueError("volume_lambda must be > 0")
+            else:
+                self.lam_per_tet = to_device(volume_lambda, torch.float32, dev, (-1,))
+                if self.lam_per_tet.numel() != self.n_tets:
+                    raise ValueError("volume_lambda needs one value per tet")
+        self.per_tet = 2 if self.volume else 1
+        self.n_cons = self.per_tet * self.n_tets
+        # vertex ids of every constraint (each tet's ids twice with the volume constraint)
+        self.t_c = (self.t_i.view(-1, 4).repeat_interleave(2, dim=0).reshape(-1).contiguous()
+                    if self.volume else self.t_i)
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
         if warm_start not in ("none", "previous", "learned"):
             raise ValueError("warm_start must be 'none', 'previous' or 'learned'")
@@ -165,7 +191,7 @@ class MGPBDSolverGPU:
         self.warm_start_model = warm_start_model
         self.record_samples = record_samples
 
-        nv, nt = self.n_verts, self.n_tets
+        nv, nt, nc = self.n_verts, self.n_tets, self.n_cons
         f32 = dict(dtype=torch.float32, device=dev)
         self.v_v = torch.zeros(nv, 3, **f32)
         self.v_p_old = torch.zeros(nv, 3, **f32)
@@ -173,13 +199,13 @@ class MGPBDSolverGPU:
         self.dpos = torch.zeros(nv, 3, **f32)
         self.Dm_inv = torch.zeros(nt, 3, 3, **f32)
         self.rest_vol = torch.zeros(nt, **f32)
-        self.alpha_tilde = torch.zeros(nt, **f32)
-        self.lambdaf = torch.zeros(nt, **f32)
-        self.constraints = torch.zeros(nt, **f32)
-        self.gradC = torch.zeros(nt, 4, 3, **f32)
-        self._b_gpu = torch.zeros(nt, **f32)
-        self._dlam_gpu = torch.zeros(nt, **f32)
-        self._lam_prev = torch.zeros(nt, **f32)
+        self.alpha_tilde = torch.zeros(nc, **f32)
+        self.lambdaf = torch.zeros(nc, **f32)
+        self.constraints = torch.zeros(nc, **f32)
+        self.gradC = torch.zeros(nc, 4, 3, **f32)
+        self._b_gpu = torch.zeros(nc, **f32)
+        self._dlam_gpu = torch.zeros(nc, **f32)
+        self._lam_prev = torch.zeros(nc, **f32)
         self._norms = torch.zeros(4, dtype=torch.float64, device=dev)
         self._x_save = torch.zeros(nv, 3, **f32)
         self._gpu_solver = None
@@ -188,13 +214,16 @@ class MGPBDSolverGPU:
         self.first_solve_info = {}
         self.last_solve_info = []
         self._drag = None
-        if warm_start_model is not None and warm_start_model.n != nt:
-            raise ValueError("warm_start_model was built for a different tet count")
+        if warm_start_model is not None and warm_start_model.n != nc:
+            raise ValueError("warm_start_model was built for a different constraint count")
 
     # ------------------------------------------------------------------ setup
     def init(self):
         """Rest shape, pattern, hierarchy, multigrid setup (cons/mgpbd_gpu.py:92-114)."""
         self._precompute_rest_shape()
+        # rest volume of every constraint's tet (a network feature)
+        self._vol_rows = (self.rest_vol.repeat_interleave(self.per_tet) if self.volume
+                          else self.rest_vol)
         self._build_sparsity_pattern()
         self._alpha_tilde_np = self.alpha_tilde.cpu().numpy()
         self._compute_C_and_gradC()
@@ -209,18 +238,24 @@ class MGPBDSolverGPU:
             fp64_iterate=self.fp64_iterate, power_iters=self.power_iters,
             smoother_safety=self.smoother_safety, cheb_ratio=self.cheb_ratio,
             stall_rescue=self.stall_rescue, smooth_schedule=self.smooth_schedule,
-            device=self.device.index)
+            omega_levels=self.omega_levels, device=self.device.index)
         self._build_incidence()
         if self.level0 == "factored":
-            self._gpu_solver.set_factored_level0(self.t_i, self.v_invm, self.gradC,
+            self._gpu_solver.set_factored_level0(self.t_c, self.v_invm, self.gradC,
                                                  self.alpha_tilde, self._vptr_np, self._vinc_np,
-                                                 t_i_host=self._t_i_np)
+                                                 t_i_host=self._t_c_np)
         self._gpu_solver.setup(self._A, self._aggs)
         if self.verbose:
-            print(f"MGPBD-GPU: {self.n_verts} verts, {self.n_tets} tets, {self._nnz} nnz, "
+            print(f"MGPBD-GPU: {self.n_verts} verts, {self.n_tets} tets, {self.n_cons} "
+                  f"constraints, {self._nnz} nnz, "
                   f"aggregation={self.aggregation}, levels={self._gpu_solver._level_sizes}")
 
     def _precompute_rest_shape(self):
+        if self.volume:
+            self.ctx("mgpbd_rest_shape_vol", self.n_tets, dptr(self.t_i), dptr(self.v_p_ref),
+                     dptr(self.mu_per_tet), self.mu, dptr(self.lam_per_tet), self.lam, self.dt,
+                     dptr(self.Dm_inv), dptr(self.rest_vol), dptr(self.alpha_tilde))
+            return
         self.ctx("mgpbd_rest_shape", self.n_tets, dptr(self.t_i), dptr(self.v_p_ref),
                  dptr(self.mu_per_tet), self.mu, self.dt, dptr(self.Dm_inv),
                  dptr(self.rest_vol), dptr(self.alpha_tilde))
@@ -228,25 +263,26 @@ class MGPBDSolverGPU:
     def _build_sparsity_pattern(self):
         """CSR pattern of A (cons/mgpbd_gpu.py:153-194) in the C++ setup code."""
         lib = _lib.load()
-        t = self.t_i.cpu().numpy()
-        self._t_i_np = t
-        p = lib.mgpbd_pattern_build(self.n_tets, self.n_verts, np_ptr(t))
+        self._t_i_np = self.t_i.cpu().numpy()
+        t = self._t_c_np = self.t_c.cpu().numpy() if self.volume else self._t_i_np
+        nc = self.n_cons
+        p = lib.mgpbd_pattern_build(nc, self.n_verts, np_ptr(t))
         if not p:
             check(-1, "mgpbd_pattern_build")
         nnz = int(lib.mgpbd_pattern_nnz(p))
-        indptr = np.empty(self.n_tets + 1, np.int32)
+        indptr = np.empty(nc + 1, np.int32)
         indices = np.empty(max(nnz, 1), np.int32)
         check(lib.mgpbd_pattern_copy(p, np_ptr(indptr), np_ptr(indices)), "mgpbd_pattern_copy")
         lib.mgpbd_pattern_free(p)
         self._A_indptr, self._A_indices = indptr, indices[:nnz]
         self._nnz = nnz
-        self._A = GPUSparseMatrix(self.n_tets, nnz, device=self.device.index)
+        self._A = GPUSparseMatrix(nc, nnz, device=self.device.index)
         self._A.set_pattern(indptr, indices[:nnz], diag_csr_pos=indptr[1:] - 1)
         # which vertices each (row, column) pair shares, per SELL slot: built once
         A = self._A
         self._A_desc = torch.zeros(max(A.padded, 1), dtype=torch.int16, device=self.device)
-        self.ctx("mgpbd_fill_A_desc", self.n_tets, dptr(A.slice_ptr), dptr(A.sell_idx),
-                 dptr(A.row_len), dptr(self.t_i), dptr(self._A_desc))
+        self.ctx("mgpbd_fill_A_desc", nc, dptr(A.slice_ptr), dptr(A.sell_idx),
+                 dptr(A.row_len), dptr(self.t_c), dptr(self._A_desc))
 
     def _build_hierarchy(self):
         if self.aggregation == "pyamg":
@@ -265,6 +301,11 @@ class MGPBDSolverGPU:
                 max_levels=(self.max_levels - 1) if self.max_levels else 8)
         else:
             raise ValueError("hierarchy must be 'reference' or 'extended'")
+        if self.volume and self.aggregation != "pyamg" and self._aggs:
+            # both constraints of a tet go to the tet's aggregate
+            agg0, n0 = self._aggs[0]
+            self._aggs = [(np.repeat(np.asarray(agg0, np.int32), self.per_tet), n0)] + \
+                list(self._aggs[1:])
         self._amg_Ps = None   # built lazily by amg_Ps (scipy objects, for inspection)
 
     @property
@@ -277,8 +318,8 @@ class MGPBDSolverGPU:
     def _build_incidence(self):
         lib = _lib.load()
         vptr = np.empty(self.n_verts + 1, np.int32)
-        vinc = np.empty(max(4 * self.n_tets, 1), np.int32)
-        check(lib.mgpbd_vertex_incidence(self.n_tets, self.n_verts, np_ptr(self._t_i_np),
+        vinc = np.empty(max(4 * self.n_cons, 1), np.int32)
+        check(lib.mgpbd_vertex_incidence(self.n_cons, self.n_verts, np_ptr(self._t_c_np),
                                          np_ptr(vptr), np_ptr(vinc)), "mgpbd_vertex_incidence")
         self._vptr_np, self._vinc_np = vptr, vinc
         self._vptr = torch.from_numpy(vptr).to(self.device)
@@ -335,7 +376,8 @@ class MGPBDSolverGPU:
     # ------------------------------------------------------------------ kernels
     def _compute_C_and_gradC(self):
         """C, gradC, rhs b = -(C + alpha lambda) and ||b||^2 in one pass."""
-        self.ctx("mgpbd_constraints", self.n_tets, dptr(self.t_i), dptr(self.v_p),
+        self.ctx("mgpbd_constraints_vol" if self.volume else "mgpbd_constraints", self.n_tets,
+                 dptr(self.t_i), dptr(self.v_p),
                  dptr(self.Dm_inv), dptr(self.alpha_tilde), dptr(self.lambdaf),
                  dptr(self.constraints), dptr(self.gradC), dptr(self._b_gpu), dptr(self._norms))
 
@@ -350,8 +392,8 @@ class MGPBDSolverGPU:
 
     def _fill_A(self):
         A = self._A
-        self.ctx("mgpbd_fill_A_shared", self.n_tets, dptr(A.slice_ptr), dptr(A.sell_idx),
-                 dptr(A.row_len), dptr(self._A_desc), dptr(self.t_i), dptr(self.v_invm),
+        self.ctx("mgpbd_fill_A_shared", self.n_cons, dptr(A.slice_ptr), dptr(A.sell_idx),
+                 dptr(A.row_len), dptr(self._A_desc), dptr(self.t_c), dptr(self.v_invm),
                  dptr(self.gradC), dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
 
     def _fill_A_and_update_gpu(self, newton_iter: int = 0):
@@ -382,7 +424,7 @@ class MGPBDSolverGPU:
         ev_in.record(torch.cuda.current_stream(self.device))
         stream.wait_event(ev_in)
         self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
-                                      self.constraints, self.alpha_tilde, self.rest_vol,
+                                      self.constraints, self.alpha_tilde, self._vol_rows,
                                       self._dlam_gpu, ctx=ctx)
         ev_out.record(stream)
         self._predicted = True
@@ -393,7 +435,7 @@ class MGPBDSolverGPU:
             mg.ctx("mgpbd_mg_factored_scatter", dptr(self._dlam_gpu), dptr(self.lambdaf),
                    dptr(self.dpos))
             return
-        self.ctx("mgpbd_scatter", self.n_tets, self.n_verts, dptr(self._vptr), dptr(self._vinc),
+        self.ctx("mgpbd_scatter", self.n_cons, self.n_verts, dptr(self._vptr), dptr(self._vinc),
                  dptr(self.v_invm), dptr(self.gradC), dptr(self._dlam_gpu), dptr(self.lambdaf),
                  dptr(self.dpos))
 
@@ -404,7 +446,8 @@ class MGPBDSolverGPU:
     def _dual_norms(self, omega: float, with_current: int, read: bool = True):
         """[current, trial] squared dual norms into _norms[1:3] (with_current: 0 trial only,
         1 both, 2 current only); read=False leaves them on the device."""
-        self.ctx("mgpbd_dual_norms", self.n_tets, dptr(self.t_i), dptr(self.v_p),
+        self.ctx("mgpbd_dual_norms_vol" if self.volume else "mgpbd_dual_norms", self.n_tets,
+                 dptr(self.t_i), dptr(self.v_p),
                  dptr(self.dpos), float(omega), dptr(self.Dm_inv), dptr(self.alpha_tilde),
                  dptr(self.lambdaf), dptr(self.constraints), int(with_current),
                  dptr(self._norms[1:3]))
@@ -469,7 +512,7 @@ class MGPBDSolverGPU:
             if ite == 0 and self.record_samples is not None:
                 self.record_samples.append(self.warm_start_model.make_sample(
                     self._b_gpu, self._A.diag_inv, self._lam_prev, self.constraints,
-                    self.alpha_tilde, self.rest_vol, self._dlam_gpu, self._A))
+                    self.alpha_tilde, self._vol_rows, self._dlam_gpu, self._A))
             self.last_solve_info.append(dict(self._gpu_solver.last_info, cg_iters=cg_iters))
             if ite == 0:
                 self.first_solve_info = self.last_solve_info[0]
@@ -484,7 +527,7 @@ class MGPBDSolverGPU:
                 self.benchmark.record_iteration(ite, r, cg_iters=cg_iters, omega=omega)
         self.last_stats = stats
         if self.warm_start != "none" or self.record_samples is not None:
-            gpu_copy(self.lambdaf, self._lam_prev, self.n_tets)
+            gpu_copy(self.lambdaf, self._lam_prev, self.n_cons)
         if self.benchmark is not None:
             self.benchmark.end_frame()
 
@@ -492,11 +535,11 @@ class MGPBDSolverGPU:
         """First PCG of the substep from the previous substep's lambda, or from the learned
         prediction with the previous lambda as the alternative start."""
         if self.warm_start == "previous":
-            gpu_copy(self._lam_prev, self._dlam_gpu, self.n_tets)
+            gpu_copy(self._lam_prev, self._dlam_gpu, self.n_cons)
             return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu)
         if not self._predicted:   # else already queued beside the hierarchy refresh
             self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
-                                          self.constraints, self.alpha_tilde, self.rest_vol,
+                                          self.constraints, self.alpha_tilde, self._vol_rows,
                                           self._dlam_gpu)
         self._predicted = False
         return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu, self._lam_prev)

@@ -179,8 +179,10 @@ class GPUMultigridSolver:
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
                  smoother_safety: float = 1.1, power_iters: int = 2, cheb_ratio: float = 12.0,
                  verify: int = 0, fp64_iterate: bool = False, stall_rescue: bool = True,
-                 smooth_schedule=None, device=None):
+                 smooth_schedule=None, omega_levels=None, device=None):
         self.n_smooth = int(n_smooth)
+        # fixed per-level Jacobi weights (the last entry repeating), with smoother='jacobi'
+        self.omega_levels = None if omega_levels is None else [float(w) for w in omega_levels]
         self.omega_jacobi = float(omega_jacobi)
         self.maxiter_cg = int(maxiter_cg)
         self.tol_cg = float(tol_cg)
@@ -203,6 +205,8 @@ class GPUMultigridSolver:
         if smoother not in ("jacobi", "spectral", "chebyshev"):
             raise ValueError("smoother must be 'jacobi' (reference), 'spectral' or 'chebyshev'")
         self.smoother = smoother
+        if self.omega_levels is not None and (smoother != "jacobi" or not self.omega_levels):
+            raise ValueError("omega_levels (a non-empty list) goes with smoother='jacobi'")
         self.smoother_target = float(smoother_target)
         self.smoother_safety = float(smoother_safety)
         self.power_iters = int(power_iters)
@@ -472,6 +476,9 @@ class GPUMultigridSolver:
         ctx("mgpbd_mg_set_cycle", 2 if self.cycle == "W" else 1, self.overcorrect)
         ctx("mgpbd_mg_set_smoother", {"jacobi": 0, "spectral": 1, "chebyshev": 2}[self.smoother],
             self.smoother_target, self.smoother_safety, self.power_iters, self.cheb_ratio)
+        if self.omega_levels:
+            w = np.asarray(self.omega_levels, np.float32)
+            ctx("mgpbd_mg_set_level_weights", len(w), w.ctypes.data_as(_lib.F32P))
         ctx("mgpbd_pcg_set_verify", int(self.verify))
         ctx("mgpbd_pcg_set_fp64_iterate", int(self.fp64_iterate))
         ctx("mgpbd_pcg_set_rescue", int(self.stall_rescue and self.smoother != "jacobi"), 1.25,

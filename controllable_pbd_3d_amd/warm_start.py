@@ -50,11 +50,19 @@ def initial_parameters(seed: int = 0) -> np.ndarray:
     return P
 
 
-def neighbour_table(t_i, n_tets: int) -> np.ndarray:
-    """(n_tets, 4) int32 face neighbours, -1 padded (libmgpbd's face adjacency)."""
+def neighbour_table(t_i, n_tets: int, per_tet: int = 1) -> np.ndarray:
+    """(n_tets * per_tet, 4) int32 face neighbours, -1 padded (libmgpbd's face adjacency).
+    With two constraints per tet (ARAP 2t, volume 2t+1) each constraint's neighbours are the
+    same kind of constraint on the face-adjacent tets."""
     from .tet_aggregation import face_adjacency_arrays
     adj, _ = face_adjacency_arrays(np.asarray(t_i, np.int32).reshape(-1), n_tets)
-    return adj.reshape(n_tets, 4)
+    adj = adj.reshape(n_tets, 4)
+    if per_tet == 1:
+        return adj
+    out = np.repeat(adj, per_tet, axis=0).astype(np.int64)
+    kind = np.tile(np.arange(per_tet), n_tets)[:, None]
+    out = np.where(out >= 0, per_tet * out + kind, -1)
+    return out.astype(np.int32)
 
 
 class LearnedWarmStart:
@@ -95,8 +103,10 @@ class LearnedWarmStart:
         self.loss = torch.zeros(1, dtype=torch.float64, device=dev)
 
     @classmethod
-    def for_mesh(cls, t_i, n_tets: int, **kw) -> "LearnedWarmStart":
-        return cls(neighbour_table(t_i, n_tets), **kw)
+    def for_mesh(cls, t_i, n_tets: int, per_tet: int = 1, **kw) -> "LearnedWarmStart":
+        """Network over the constraints of a tet mesh: one per tet, or two (ARAP and volume)
+        with ``per_tet=2``."""
+        return cls(neighbour_table(t_i, n_tets, per_tet), **kw)
 
     # ------------------------------------------------------------------ inference
     def features(self, b, dinv, lam_prev, C, alpha, vol, x=None, stats=None, ctx=None):

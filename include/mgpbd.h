@@ -170,6 +170,16 @@ int mgpbd_rest_shape(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev,
                      const float* v_ref_dev, const float* mu_dev, double mu, double dt,
                      float* Dm_inv_dev, float* rest_vol_dev, float* alpha_dev);
 
+/* Volume constraint (extension; the reference's MGPBD solve has one ARAP constraint per
+ * tet): tet t owns constraints 2t (ARAP) and 2t+1 (volume C_H = det F - 1, the hydrostatic
+ * constraint of cons/deform3d.py:81-91). As mgpbd_rest_shape, with alpha (2 n_tets entries)
+ * alpha[2t] = 1/(mu dt^2 vol) and alpha[2t+1] = 1/(lambda dt^2 vol), lambda the Lame
+ * parameter (cons/vbd.py:44-47; lam_dev per tet overrides the scalar lam). */
+int mgpbd_rest_shape_vol(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev,
+                         const float* v_ref_dev, const float* mu_dev, double mu,
+                         const float* lam_dev, double lam, double dt, float* Dm_inv_dev,
+                         float* rest_vol_dev, float* alpha_dev);
+
 /* v += dt g (+ dt invm f_ext); v *= damp; x_old = x; x += dt v.
  * Replaces make_prediction (:206-213), which moves every vertex, pinned ones included;
  * hold_pinned != 0 (extension) keeps vertices with invm == 0 in place with zero velocity.
@@ -189,6 +199,14 @@ int mgpbd_update_vel(mgpbd_ctx* ctx, int n_verts, const float* v_p_dev,
 int mgpbd_constraints(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const float* v_p_dev,
                       const float* Dm_inv_dev, const float* alpha_dev, const float* lambda_dev,
                       float* C_dev, float* gradC_dev, float* rhs_dev, double* norm2_dev);
+/* Two constraints per tet (extension, see mgpbd_rest_shape_vol): C, gradC, rhs, alpha and
+ * lambda hold 2 n_tets entries; constraint 2t as mgpbd_constraints, constraint 2t+1 the
+ * volume constraint det F - 1 with gradient from cof(F) (never zeroed); ||rhs||^2 over
+ * both. */
+int mgpbd_constraints_vol(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev,
+                          const float* v_p_dev, const float* Dm_inv_dev, const float* alpha_dev,
+                          const float* lambda_dev, float* C_dev, float* gradC_dev,
+                          float* rhs_dev, double* norm2_dev);
 /* The same constraint pass without a context and without allocating: no counterpart in the
  * reference. For callers that cannot call hipMalloc (HIP graph capture, tight serving loops,
  * harnesses that forbid allocation). Same kernels and results as mgpbd_constraints, queued
@@ -213,9 +231,10 @@ int mgpbd_fill_A(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr_dev,
  * (extension): mgpbd_fill_A_desc writes, once per pattern, one 16-bit word per SELL slot
  * (desc_dev: padded slots, zero-initialised): the diagonal flag 0x8000, or the count of
  * vertices rows i and j share (bits 12-13) and, per shared vertex in ascending local slot a of
- * tet i, (a | b << 2) with b its slot in tet j; mgpbd_fill_A_shared then gathers only tet j's
- * gradients at the shared vertices (12 bytes each instead of its vertex ids and whole
- * gradient, 64 bytes). */
+ * tet i, (a | b << 2) with b its slot in tet j; rows that share all four vertices (the two
+ * constraints of one tet, mgpbd_constraints_vol) get 0x4000 | (slot in j of i's vertex a) << 2a;
+ * mgpbd_fill_A_shared then gathers only row j's gradients at the shared vertices (12 bytes
+ * each instead of its vertex ids and whole gradient, 64 bytes). */
 int mgpbd_fill_A_desc(mgpbd_ctx* ctx, int n_tets, const int32_t* slice_ptr_dev,
                       const int32_t* sell_idx_dev, const int32_t* row_len_dev,
                       const int32_t* t_i_dev, uint16_t* desc_dev);
@@ -239,6 +258,13 @@ int mgpbd_dual_norms(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev, const f
                      const float* dpos_dev, float omega, const float* Dm_inv_dev,
                      const float* alpha_dev, const float* lambda_dev, const float* C_dev,
                      int with_current, double* out_dev);
+
+/* mgpbd_dual_norms with two constraints per tet (see mgpbd_constraints_vol). */
+int mgpbd_dual_norms_vol(mgpbd_ctx* ctx, int n_tets, const int32_t* t_i_dev,
+                         const float* v_p_dev, const float* dpos_dev, float omega,
+                         const float* Dm_inv_dev, const float* alpha_dev,
+                         const float* lambda_dev, const float* C_dev, int with_current,
+                         double* out_dev);
 
 /* x += omega dpos where invm > 0. Replaces _apply_dpos (:345-349). */
 int mgpbd_apply_dpos(mgpbd_ctx* ctx, int n_verts, float* v_p_dev, const float* dpos_dev,
@@ -350,6 +376,12 @@ int mgpbd_mg_set_smoother(mgpbd_ctx* ctx, int mode, float target, float safety, 
  * no-op in mode 0). lambda_out (host, n_levels doubles, may be NULL) receives the estimates
  * (then the call synchronises). */
 int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out);
+/* Fixed per-level weighted-Jacobi weights (extension): level l relaxes with omega[min(l, n-1)]
+ * (host array, each in (0, 2)) instead of the common omega of cons/mgpcg_gpu.py:232-246; no
+ * spectral estimates are made. For matrices whose lambda_max(D^-1 A) exceeds 3 on some level
+ * (the volume constraint's rows), where the common 2/3 makes the V-cycle indefinite. Reset by
+ * mgpbd_mg_set_smoother. */
+int mgpbd_mg_set_level_weights(mgpbd_ctx* ctx, int n, const float* omega);
 /* The weight each level relaxes with (host, n_levels floats) and, when cheb_out is given,
  * the Chebyshev (c1, c2) of steps 2..9 per level (n_levels x 16 floats, zero unless mode 2);
  * synchronises. */

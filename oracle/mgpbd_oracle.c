@@ -25,11 +25,25 @@
  *   orc_scatter         cons/mgpbd_gpu.py:330-354
  *   orc_dual_norm2      cons/mgpbd_gpu.py:360-377
  *   orc_apply_dpos      cons/mgpbd_gpu.py:345-349
+ *
+ * Volume constraint (extension; the reference MGPBD solve has one ARAP constraint per tet):
+ *   orc_rest_shape_vol, orc_constraints_vol, orc_dual_norm2_vol -- constraint 2t is tet t's
+ *   ARAP constraint as above, constraint 2t+1 its volume constraint C_H = det F - 1 with
+ *   dC_H/dF = cof(F) (the hydrostatic constraint of cons/deform3d.py:81-91) and compliance
+ *   1 / (lambda dt^2 vol), lambda the Lame parameter (cons/vbd.py:44-47).
+ *
+ * Threads: OpenMP over rows / tets / vertices. Every row-parallel loop computes each output
+ * from its own inputs in the sequential order, so results do not depend on the thread count;
+ * FP64 reductions sum fixed chunks of ORC_CHUNK terms in parallel and the chunk totals in
+ * chunk order (the same total for any thread count).
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+#define ORC_CHUNK 8192
+#define ORC_PAR_MIN 4096   /* shorter loops stay on one thread */
 
 #ifdef ORACLE_F64
 typedef double real;
@@ -42,9 +56,9 @@ typedef float real;
 #endif
 
 /* ---------------------------------------------------------------- rest shape (FP64) */
-void orc_rest_shape(int nt, const int32_t* t_i, const real* xr, const real* mu_t, double mu,
-                    double dt, real* Dm_inv, real* vol_out, real* alpha) {
-  for (int t = 0; t < nt; ++t) {
+/* Dm^-1 of tet t into Dm_inv, returns vol = |det Dm| / 6 (FP64). */
+static double rest_one(int t, const int32_t* t_i, const real* xr, real* Dm_inv) {
+  {
     const int32_t* v = t_i + 4 * (int64_t)t;
     double D[3][3];
     for (int r = 0; r < 3; ++r)
@@ -58,10 +72,34 @@ void orc_rest_shape(int nt, const int32_t* t_i, const real* xr, const real* mu_t
         c01 / det, (D[0][0] * D[2][2] - D[0][2] * D[2][0]) / det, (D[0][2] * D[1][0] - D[0][0] * D[1][2]) / det,
         c02 / det, (D[0][1] * D[2][0] - D[0][0] * D[2][1]) / det, (D[0][0] * D[1][1] - D[0][1] * D[1][0]) / det};
     for (int k = 0; k < 9; ++k) Dm_inv[9 * (int64_t)t + k] = (real)inv[k];
-    double vol = fabs(det) / 6.0;
+    return fabs(det) / 6.0;
+  }
+}
+
+void orc_rest_shape(int nt, const int32_t* t_i, const real* xr, const real* mu_t, double mu,
+                    double dt, real* Dm_inv, real* vol_out, real* alpha) {
+#pragma omp parallel for schedule(static)
+  for (int t = 0; t < nt; ++t) {
+    double vol = rest_one(t, t_i, xr, Dm_inv);
     vol_out[t] = (real)vol;
     double m = mu_t ? (double)mu_t[t] : mu;
     alpha[t] = (real)(1.0 / (m * dt * dt * vol));
+  }
+}
+
+/* As orc_rest_shape with two constraints per tet: alpha[2t] = 1/(mu dt^2 vol) (ARAP),
+ * alpha[2t+1] = 1/(lambda dt^2 vol) (volume); lam_t (per tet) overrides the scalar lam. */
+void orc_rest_shape_vol(int nt, const int32_t* t_i, const real* xr, const real* mu_t, double mu,
+                        const real* lam_t, double lam, double dt, real* Dm_inv, real* vol_out,
+                        real* alpha) {
+#pragma omp parallel for schedule(static)
+  for (int t = 0; t < nt; ++t) {
+    double vol = rest_one(t, t_i, xr, Dm_inv);
+    vol_out[t] = (real)vol;
+    double m = mu_t ? (double)mu_t[t] : mu;
+    double l = lam_t ? (double)lam_t[t] : lam;
+    alpha[2 * (int64_t)t] = (real)(1.0 / (m * dt * dt * vol));
+    alpha[2 * (int64_t)t + 1] = (real)(1.0 / (l * dt * dt * vol));
   }
 }
 
@@ -69,6 +107,7 @@ void orc_rest_shape(int nt, const int32_t* t_i, const real* xr, const real* mu_t
 void orc_predict(int nv, real* x, real* v, real* x_old, const real* invm, const real* fext,
                  real gx, real gy, real gz, real dt, real damp, int hold_pinned) {
   real g[3] = {gx, gy, gz};
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nv; ++k) {
     int pinned = hold_pinned && !(invm[k] > 0);
     for (int r = 0; r < 3; ++r) {
@@ -85,10 +124,12 @@ void orc_predict(int nv, real* x, real* v, real* x_old, const real* invm, const 
 }
 
 void orc_update_vel(int nv, const real* x, const real* x_old, real* v, real dt, real damp) {
+#pragma omp parallel for schedule(static)
   for (int64_t q = 0; q < 3 * (int64_t)nv; ++q) v[q] = damp * (x[q] - x_old[q]) / dt;
 }
 
 void orc_apply_dpos(int nv, real* x, const real* dpos, const real* invm, real omega) {
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nv; ++k)
     if (invm[k] > 0)
       for (int r = 0; r < 3; ++r) x[3 * (int64_t)k + r] = FMA(omega, dpos[3 * (int64_t)k + r], x[3 * (int64_t)k + r]);
@@ -169,9 +210,9 @@ static void svd3_f64(const double F[3][3], double U[3][3], double s[3], double V
   s[m] = sm;
 }
 
-/* C and dC/dx of one tet from positions (double). g[4][3]; returns C. */
-static double tet_C_grad(const double p[4][3], const real* Bm, double g[4][3], int want_grad) {
-  double Ds[3][3], B[3][3], F[3][3];
+/* F = Ds Dm^-1 of one tet from positions (double); B receives Dm^-1. */
+static void def_grad(const double p[4][3], const real* Bm, double B[3][3], double F[3][3]) {
+  double Ds[3][3];
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) {
       Ds[r][c] = p[c + 1][r] - p[0][r];
@@ -179,6 +220,20 @@ static double tet_C_grad(const double p[4][3], const real* Bm, double g[4][3], i
     }
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) F[r][c] = Ds[r][0] * B[0][c] + Ds[r][1] * B[1][c] + Ds[r][2] * B[2][c];
+}
+
+/* vertex gradients from dC/dF = G: dC/dDs = G B^T, column k-1 is vertex k's gradient
+ * (cons/mgpbd.py:41-67), vertex 0 takes minus their sum */
+static void grad_from_dF(const double G[3][3], const double B[3][3], double g[4][3]) {
+  for (int k = 1; k <= 3; ++k)
+    for (int r = 0; r < 3; ++r) g[k][r] = G[r][0] * B[k - 1][0] + G[r][1] * B[k - 1][1] + G[r][2] * B[k - 1][2];
+  for (int r = 0; r < 3; ++r) g[0][r] = -g[1][r] - g[2][r] - g[3][r];
+}
+
+/* ARAP C = ||sigma - 1|| and dC/dx of one tet from positions (double). g[4][3]; returns C. */
+static double tet_C_grad(const double p[4][3], const real* Bm, double g[4][3], int want_grad) {
+  double B[3][3], F[3][3];
+  def_grad(p, Bm, B, F);
   double U[3][3], s[3], V[3][3];
   svd3_f64(F, U, s, V);
   double C = sqrt((s[0] - 1) * (s[0] - 1) + (s[1] - 1) * (s[1] - 1) + (s[2] - 1) * (s[2] - 1));
@@ -187,55 +242,140 @@ static double tet_C_grad(const double p[4][3], const real* Bm, double g[4][3], i
   double G[3][3];
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) G[r][c] = U[r][0] * V[c][0] * d[0] + U[r][1] * V[c][1] * d[1] + U[r][2] * V[c][2] * d[2];
-  /* dC/dDs = G B^T; column k-1 is the gradient of vertex k (cons/mgpbd.py:41-67) */
-  for (int k = 1; k <= 3; ++k)
-    for (int r = 0; r < 3; ++r) g[k][r] = G[r][0] * B[k - 1][0] + G[r][1] * B[k - 1][1] + G[r][2] * B[k - 1][2];
-  for (int r = 0; r < 3; ++r) g[0][r] = -g[1][r] - g[2][r] - g[3][r];
+  grad_from_dF(G, B, g);
   return C;
+}
+
+/* Volume constraint C_H = det F - 1 of one tet, dC_H/dF = cof(F): column c is the cross
+ * product of F's other two columns in cyclic order (cons/deform3d.py:81-91). */
+static double tet_vol_grad(const double p[4][3], const real* Bm, double g[4][3], int want_grad) {
+  double B[3][3], F[3][3];
+  def_grad(p, Bm, B, F);
+  double G[3][3];
+  for (int c = 0; c < 3; ++c) {
+    const int a = (c + 1) % 3, b = (c + 2) % 3;
+    G[0][c] = F[1][a] * F[2][b] - F[2][a] * F[1][b];
+    G[1][c] = F[2][a] * F[0][b] - F[0][a] * F[2][b];
+    G[2][c] = F[0][a] * F[1][b] - F[1][a] * F[0][b];
+  }
+  double det = F[0][0] * G[0][0] + F[1][0] * G[1][0] + F[2][0] * G[2][0];
+  if (want_grad) grad_from_dF(G, B, g);
+  return det - 1.0;
+}
+
+static void tet_positions(const int32_t* t_i, int t, const real* x, const real* dx, real omega,
+                          double p[4][3]) {
+  for (int l = 0; l < 4; ++l)
+    for (int r = 0; r < 3; ++r) {
+      int64_t q = 3 * (int64_t)t_i[4 * (int64_t)t + l] + r;
+      real xv = x[q];
+      if (dx) xv = xv + omega * dx[q];
+      p[l][r] = (double)xv;
+    }
+}
+
+/* Deterministic parallel FP64 sum: fn(lo, hi, arg) returns the sequential sum over rows
+ * [lo, hi); chunks of ORC_CHUNK rows are summed in parallel, their totals in chunk order. */
+typedef double (*orc_range_fn)(int lo, int hi, void* arg);
+static double chunked_sum(int n, orc_range_fn fn, void* arg) {
+  int nch = (n + ORC_CHUNK - 1) / ORC_CHUNK;
+  if (nch <= 1) return n > 0 ? fn(0, n, arg) : 0.0;
+  double* part = malloc(sizeof(double) * (size_t)nch);
+#pragma omp parallel for schedule(static)
+  for (int ch = 0; ch < nch; ++ch) {
+    int lo = ch * ORC_CHUNK, hi = lo + ORC_CHUNK < n ? lo + ORC_CHUNK : n;
+    part[ch] = fn(lo, hi, arg);
+  }
+  double tot = 0.0;
+  for (int ch = 0; ch < nch; ++ch) tot += part[ch];
+  free(part);
+  return tot;
+}
+
+typedef struct {
+  const int32_t* t_i; const real *x, *dx; real omega; const real *Dm_inv, *alpha, *lambda;
+  real *C_out, *gradC, *rhs;
+  int volume;   /* 0: one ARAP constraint per tet; 1: ARAP (2t) and volume (2t+1) */
+} ConsArgs;
+
+static void store_cons(const ConsArgs* a, int64_t c, double C, const double g[4][3],
+                       int zero_small, double* acc) {
+  real Cr = (real)C;
+  a->C_out[c] = Cr;
+  if (!zero_small || Cr > (real)1e-6) {
+    for (int l = 0; l < 4; ++l)
+      for (int r = 0; r < 3; ++r) a->gradC[12 * c + 3 * l + r] = (real)g[l][r];
+  } else {
+    for (int k = 0; k < 12; ++k) a->gradC[12 * c + k] = 0;
+  }
+  real b = -(Cr + a->alpha[c] * a->lambda[c]);
+  a->rhs[c] = b;
+  *acc += (double)b * (double)b;
+}
+
+static double cons_range(int lo, int hi, void* arg) {
+  const ConsArgs* a = (const ConsArgs*)arg;
+  double acc = 0.0;
+  for (int t = lo; t < hi; ++t) {
+    double p[4][3], g[4][3];
+    tet_positions(a->t_i, t, a->x, NULL, 0, p);
+    const int64_t c0 = a->volume ? 2 * (int64_t)t : t;
+    double C = tet_C_grad(p, a->Dm_inv + 9 * (int64_t)t, g, 1);
+    store_cons(a, c0, C, g, 1, &acc);
+    if (a->volume) {
+      double Ch = tet_vol_grad(p, a->Dm_inv + 9 * (int64_t)t, g, 1);
+      store_cons(a, c0 + 1, Ch, g, 0, &acc);
+    }
+  }
+  return acc;
 }
 
 void orc_constraints(int nt, const int32_t* t_i, const real* x, const real* Dm_inv,
                      const real* alpha, const real* lambda, real* C_out, real* gradC,
                      real* rhs, double* norm2) {
+  ConsArgs a = {t_i, x, NULL, 0, Dm_inv, alpha, lambda, C_out, gradC, rhs, 0};
+  *norm2 = chunked_sum(nt, cons_range, &a);
+}
+
+/* Two constraints per tet (arrays of 2 nt entries, 12 gradient floats each): constraint 2t
+ * the ARAP constraint, 2t+1 the volume constraint (gradient never zeroed). */
+void orc_constraints_vol(int nt, const int32_t* t_i, const real* x, const real* Dm_inv,
+                         const real* alpha, const real* lambda, real* C_out, real* gradC,
+                         real* rhs, double* norm2) {
+  ConsArgs a = {t_i, x, NULL, 0, Dm_inv, alpha, lambda, C_out, gradC, rhs, 1};
+  *norm2 = chunked_sum(nt, cons_range, &a);
+}
+
+static double dual_range(int lo, int hi, void* arg) {
+  const ConsArgs* a = (const ConsArgs*)arg;
   double acc = 0.0;
-  for (int t = 0; t < nt; ++t) {
+  for (int t = lo; t < hi; ++t) {
     double p[4][3], g[4][3];
-    for (int l = 0; l < 4; ++l)
-      for (int r = 0; r < 3; ++r) p[l][r] = (double)x[3 * (int64_t)t_i[4 * (int64_t)t + l] + r];
-    double C = tet_C_grad(p, Dm_inv + 9 * (int64_t)t, g, 1);
-    real Cr = (real)C;
-    C_out[t] = Cr;
-    if (Cr > (real)1e-6) {
-      for (int l = 0; l < 4; ++l)
-        for (int r = 0; r < 3; ++r) gradC[12 * (int64_t)t + 3 * l + r] = (real)g[l][r];
-    } else {
-      for (int k = 0; k < 12; ++k) gradC[12 * (int64_t)t + k] = 0;
+    tet_positions(a->t_i, t, a->x, a->dx, a->omega, p);
+    const int64_t c0 = a->volume ? 2 * (int64_t)t : t;
+    real c = (real)tet_C_grad(p, a->Dm_inv + 9 * (int64_t)t, g, 0);
+    real rr = -(c + a->alpha[c0] * a->lambda[c0]);
+    acc += (double)rr * (double)rr;
+    if (a->volume) {
+      real ch = (real)tet_vol_grad(p, a->Dm_inv + 9 * (int64_t)t, g, 0);
+      real rh = -(ch + a->alpha[c0 + 1] * a->lambda[c0 + 1]);
+      acc += (double)rh * (double)rh;
     }
-    real b = -(Cr + alpha[t] * lambda[t]);
-    rhs[t] = b;
-    acc += (double)b * (double)b;
   }
-  *norm2 = acc;
+  return acc;
 }
 
 /* squared dual norm sum_t (C_t(x + omega dx) + alpha_t lambda_t)^2 */
 double orc_dual_norm2(int nt, const int32_t* t_i, const real* x, const real* dx, real omega,
                       const real* Dm_inv, const real* alpha, const real* lambda) {
-  double acc = 0.0;
-  for (int t = 0; t < nt; ++t) {
-    double p[4][3], g[4][3];
-    for (int l = 0; l < 4; ++l)
-      for (int r = 0; r < 3; ++r) {
-        int64_t q = 3 * (int64_t)t_i[4 * (int64_t)t + l] + r;
-        real xv = x[q];
-        if (dx) xv = xv + omega * dx[q];
-        p[l][r] = (double)xv;
-      }
-    real c = (real)tet_C_grad(p, Dm_inv + 9 * (int64_t)t, g, 0);
-    real rr = -(c + alpha[t] * lambda[t]);
-    acc += (double)rr * (double)rr;
-  }
-  return acc;
+  ConsArgs a = {t_i, x, dx, omega, Dm_inv, alpha, lambda, NULL, NULL, NULL, 0};
+  return chunked_sum(nt, dual_range, &a);
+}
+
+double orc_dual_norm2_vol(int nt, const int32_t* t_i, const real* x, const real* dx, real omega,
+                          const real* Dm_inv, const real* alpha, const real* lambda) {
+  ConsArgs a = {t_i, x, dx, omega, Dm_inv, alpha, lambda, NULL, NULL, NULL, 1};
+  return chunked_sum(nt, dual_range, &a);
 }
 
 /* ---------------------------------------------------------------- A fill (CSR) */
@@ -243,6 +383,7 @@ static real dot3r(const real* a, const real* b) { return FMA(a[2], b[2], FMA(a[1
 
 void orc_fill_A(int nt, const int32_t* indptr, const int32_t* indices, const int32_t* t_i,
                 const real* invm, const real* gradC, const real* alpha, real* data) {
+#pragma omp parallel for schedule(dynamic, 256) if (nt > ORC_PAR_MIN)
   for (int i = 0; i < nt; ++i) {
     const int32_t* a = t_i + 4 * (int64_t)i;
     const real* gi = gradC + 12 * (int64_t)i;
@@ -272,6 +413,7 @@ void orc_fill_A(int nt, const int32_t* indptr, const int32_t* indices, const int
 
 void orc_cache_diag_inv(int n, const int32_t* indptr, const int32_t* indices, const real* data,
                         real* dinv) {
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
   for (int i = 0; i < n; ++i) {
     real d = 0;
     for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k)
@@ -282,6 +424,7 @@ void orc_cache_diag_inv(int n, const int32_t* indptr, const int32_t* indices, co
 
 void orc_spmv(int n, const int32_t* indptr, const int32_t* indices, const real* data,
               const real* x, real* y) {
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
   for (int i = 0; i < n; ++i) {
     real s = 0;
     for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) s = FMA(data[k], x[indices[k]], s);
@@ -291,6 +434,7 @@ void orc_spmv(int n, const int32_t* indptr, const int32_t* indices, const real* 
 
 void orc_jacobi(int n, const int32_t* indptr, const int32_t* indices, const real* data,
                 const real* dinv, const real* b, const real* x, real* x_new, real omega) {
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
   for (int i = 0; i < n; ++i) {
     real s = 0;
     for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) s = FMA(data[k], x[indices[k]], s);
@@ -301,6 +445,7 @@ void orc_jacobi(int n, const int32_t* indptr, const int32_t* indices, const real
 /* coarse[c] = sum_k p_ki * fine[idx[k]] * p_lj (unit p: a plain sequential sum) */
 void orc_refill(int nc, const int32_t* off, const int32_t* fine_idx, const real* pki,
                 const real* plj, const real* fine, real* coarse) {
+#pragma omp parallel for schedule(static) if (nc > ORC_PAR_MIN)
   for (int c = 0; c < nc; ++c) {
     real v = 0;
     for (int32_t k = off[c]; k < off[c + 1]; ++k) v = v + (pki[k] * fine[fine_idx[k]]) * plj[k];
@@ -333,6 +478,7 @@ typedef struct {
 static void orc_cheb_step(int n, const int32_t* indptr, const int32_t* indices, const real* data,
                           const real* dinv, const real* b, const real* x, const real* xp,
                           real* x_new, real c1, real c2) {
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
   for (int i = 0; i < n; ++i) {
     real s = 0;
     for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) s = FMA(data[k], x[indices[k]], s);
@@ -366,6 +512,7 @@ static void vc(const OrcMG* mg, int l) {
   const real w = A->omega > 0 ? A->omega : mg->omega;
   if (l == mg->nl - 1 && A->dense) {
     /* x = A^-1 b with FP64 accumulation */
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
     for (int i = 0; i < n; ++i) {
       double acc = 0;
       for (int j = 0; j < n; ++j) acc += A->dense[(int64_t)j * n + i] * (double)A->b[j];
@@ -384,19 +531,27 @@ static void vc(const OrcMG* mg, int l) {
   const OrcLevel* C = &mg->L[l + 1];
   smooth_phase(A, mg->n_smooth, w);
   orc_spmv(n, A->ip, A->ix, A->a, A->x, A->r);
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
   for (int i = 0; i < n; ++i) A->r[i] = A->b[i] - A->r[i];
   orc_spmv(C->n, A->tip, A->tix, A->ta, A->r, C->b);
   memset(C->x, 0, sizeof(real) * C->n);
   vc(mg, l + 1);
   orc_spmv(n, A->pip, A->pix, A->pa, C->x, A->tmp);
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
   for (int i = 0; i < n; ++i) A->x[i] = FMA((real)1, A->tmp[i], A->x[i]);
   smooth_phase(A, mg->n_smooth, w);
 }
 
-static double dotd(int n, const real* a, const real* b) {
+typedef struct { const real *a, *b; } DotArgs;
+static double dot_range(int lo, int hi, void* arg) {
+  const DotArgs* d = (const DotArgs*)arg;
   double s = 0;
-  for (int i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
+  for (int i = lo; i < hi; ++i) s += (double)d->a[i] * (double)d->b[i];
   return s;
+}
+static double dotd(int n, const real* a, const real* b) {
+  DotArgs d = {a, b};
+  return chunked_sum(n, dot_range, &d);
 }
 
 /* z = V-cycle(r) with x_0 = 0 */
@@ -427,14 +582,18 @@ int orc_pcg(int nl, OrcLevel* L, int n_smooth, real omega, int maxiter, double t
     double pAp = dotd(n, p, Ap);
     if (fabs(pAp) < 1e-30) { reason = 2; break; }
     real al = (real)(rz / pAp);
-    for (int i = 0; i < n; ++i) x[i] = FMA(al, p[i], x[i]);
-    for (int i = 0; i < n; ++i) r[i] = FMA(-al, Ap[i], r[i]);
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
+    for (int i = 0; i < n; ++i) {
+      x[i] = FMA(al, p[i], x[i]);
+      r[i] = FMA(-al, Ap[i], r[i]);
+    }
     rr = dotd(n, r, r);
     if (rr < tol2) { cg_iters = k + 1; reason = 1; break; }
     orc_vcycle(nl, L, n_smooth, omega, r, z);
     double rz_new = dotd(n, r, z);
     if (fabs(rz) < 1e-30) { cg_iters = k + 1; reason = 3; break; }
     real be = (real)(rz_new / rz);
+#pragma omp parallel for schedule(static) if (n > ORC_PAR_MIN)
     for (int i = 0; i < n; ++i) p[i] = FMA(be, p[i], z[i]);
     rz = rz_new;
     cg_iters = k + 1;

@@ -48,9 +48,14 @@ def lib(precision: str = "f32"):
         L.orc_predict.argtypes = [ctypes.c_int, R, R, R, R, R, rt, rt, rt, rt, rt, ctypes.c_int]
         L.orc_update_vel.argtypes = [ctypes.c_int, R, R, R, rt, rt]
         L.orc_apply_dpos.argtypes = [ctypes.c_int, R, R, R, rt]
+        L.orc_rest_shape_vol.argtypes = [ctypes.c_int, I32P, R, R, ctypes.c_double, R,
+                                         ctypes.c_double, ctypes.c_double, R, R, R]
         L.orc_constraints.argtypes = [ctypes.c_int, I32P, R, R, R, R, R, R, R, F64P]
+        L.orc_constraints_vol.argtypes = [ctypes.c_int, I32P, R, R, R, R, R, R, R, F64P]
         L.orc_dual_norm2.argtypes = [ctypes.c_int, I32P, R, R, rt, R, R, R]
         L.orc_dual_norm2.restype = ctypes.c_double
+        L.orc_dual_norm2_vol.argtypes = [ctypes.c_int, I32P, R, R, rt, R, R, R]
+        L.orc_dual_norm2_vol.restype = ctypes.c_double
         L.orc_fill_A.argtypes = [ctypes.c_int, I32P, I32P, I32P, R, R, R, R]
         L.orc_cache_diag_inv.argtypes = [ctypes.c_int, I32P, I32P, R, R]
         L.orc_spmv.argtypes = [ctypes.c_int, I32P, I32P, R, R, R]
@@ -288,14 +293,35 @@ class OracleMG:
         return it
 
 
+def constraint_vertices(t_i, per_tet: int) -> np.ndarray:
+    """Vertex ids of every constraint (4 per constraint, flat int32): the tets' own ids, or
+    each tet's ids twice (constraints 2t and 2t+1) with the volume constraint."""
+    t = np.ascontiguousarray(t_i, np.int32).reshape(-1, 4)
+    return np.ascontiguousarray(np.repeat(t, per_tet, axis=0).reshape(-1))
+
+
+def expand_prolongation(P, per_tet: int):
+    """Level-0 prolongation of the tets' aggregates over their constraints: both constraints
+    of a tet go to the tet's aggregate (rows 2t and 2t+1 copy row t)."""
+    if per_tet == 1:
+        return P
+    return P.tocsr()[np.repeat(np.arange(P.shape[0]), per_tet)].tocsr()
+
+
 class OracleMGPBD:
-    """MGPBDSolverGPU restated on the CPU (cons/mgpbd_gpu.py:20-446)."""
+    """MGPBDSolverGPU restated on the CPU (cons/mgpbd_gpu.py:20-446).
+
+    ``volume_lambda`` (extension; None keeps the reference's single ARAP constraint per tet):
+    Lame lambda per tet (array) or for all tets (float); every tet then also carries the volume
+    constraint det F - 1 with compliance 1 / (lambda dt^2 vol), as constraint 2t+1 beside its
+    ARAP constraint 2t."""
 
     def __init__(self, v_p, v_p_ref, v_invm, t_i, t_m, gravity, dt, mu=1e6, damp=1.0,
                  maxiter=20, atol=1e-4, rtol=1e-2, maxiter_cg=100, tol_cg=1e-5, n_smooth=2,
                  omega_jacobi=2.0 / 3.0, setup_interval=10000, use_line_search=True,
                  aggregation="tet", benchmark=None, precision="f32", hold_pinned=False,
-                 mu_per_tet=None, relative_tol=False, f_ext=None):
+                 mu_per_tet=None, relative_tol=False, f_ext=None, volume_lambda=None,
+                 omega_levels=None):
         self.L, self.R, self.dtype = lib(precision)
         dtp = self.dtype
         self.precision = precision
@@ -316,18 +342,29 @@ class OracleMGPBD:
         self.relative_tol = relative_tol
         self.mu_per_tet = None if mu_per_tet is None else np.asarray(mu_per_tet, dtp).copy()
         self.f_ext = None if f_ext is None else np.asarray(f_ext, dtp).reshape(-1).copy()
+        self.volume = volume_lambda is not None
+        self.lam_per_tet, self.lam = None, 0.0
+        if self.volume:
+            if np.ndim(volume_lambda) == 0:
+                self.lam = float(volume_lambda)
+            else:
+                self.lam_per_tet = np.asarray(volume_lambda, dtp).reshape(-1).copy()
+        self.per_tet = 2 if self.volume else 1
+        self.omega_levels = omega_levels   # fixed Jacobi weight per level (last repeating)
         nt, nv = self.n_tets, self.n_verts
+        nc = self.n_cons = self.per_tet * nt
+        self.t_c = constraint_vertices(self.t_i, self.per_tet) if self.volume else self.t_i
         self.v_v = np.zeros((nv, 3), dtp)
         self.v_p_old = np.zeros((nv, 3), dtp)
         self.dpos = np.zeros((nv, 3), dtp)
         self.Dm_inv = np.zeros(9 * nt, dtp)
         self.rest_vol = np.zeros(nt, dtp)
-        self.alpha_tilde = np.zeros(nt, dtp)
-        self.lambdaf = np.zeros(nt, dtp)
-        self.constraints = np.zeros(nt, dtp)
-        self.gradC = np.zeros(12 * nt, dtp)
-        self.b = np.zeros(nt, dtp)
-        self.dlam = np.zeros(nt, dtp)
+        self.alpha_tilde = np.zeros(nc, dtp)
+        self.lambdaf = np.zeros(nc, dtp)
+        self.constraints = np.zeros(nc, dtp)
+        self.gradC = np.zeros(12 * nc, dtp)
+        self.b = np.zeros(nc, dtp)
+        self.dlam = np.zeros(nc, dtp)
         self.history = []
 
     def _p(self, a):
@@ -338,37 +375,52 @@ class OracleMGPBD:
         the oracle the same extended levels as a GPU run); ``structural`` builds coarse
         patterns from the structure of A instead of its init-time values."""
         L = self.L
-        L.orc_rest_shape(self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p_ref),
-                         self._p(self.mu_per_tet), float(self.mu), float(self.dt),
-                         self._p(self.Dm_inv), self._p(self.rest_vol), self._p(self.alpha_tilde))
-        self.indptr, self.indices = sparsity_pattern(self.t_i, self.n_tets, self.n_verts)
+        if self.volume:
+            L.orc_rest_shape_vol(self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p_ref),
+                                 self._p(self.mu_per_tet), float(self.mu),
+                                 self._p(self.lam_per_tet), float(self.lam), float(self.dt),
+                                 self._p(self.Dm_inv), self._p(self.rest_vol),
+                                 self._p(self.alpha_tilde))
+        else:
+            L.orc_rest_shape(self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p_ref),
+                             self._p(self.mu_per_tet), float(self.mu), float(self.dt),
+                             self._p(self.Dm_inv), self._p(self.rest_vol),
+                             self._p(self.alpha_tilde))
+        nc = self.n_cons
+        self.indptr, self.indices = sparsity_pattern(self.t_c, nc, self.n_verts)
         self._compute_C_and_gradC()
         data = np.zeros(len(self.indices), self.dtype)
         self._fill_A(data)
         A = sp.csr_matrix((data.astype(np.float32).astype(np.float64), self.indices.copy(),
-                           self.indptr.copy()), shape=(self.n_tets, self.n_tets))
+                           self.indptr.copy()), shape=(nc, nc))
         if Ps is None:
             Ps, sizes = structure.build_tet_hierarchy(self.t_i, self.v_p_ref.astype(np.float64),
                                                       self.n_tets)
+            Ps = [expand_prolongation(Ps[0], self.per_tet)] + list(Ps[1:]) if Ps else Ps
+            sizes = [nc] + list(sizes[1:])
         else:
-            sizes = [self.n_tets] + [P.shape[1] for P in Ps]
+            sizes = [nc] + [P.shape[1] for P in Ps]
         self.Ps, self.sizes = Ps, sizes
         self.mg = OracleMG(self.n_smooth, self.omega_jacobi, self.maxiter_cg, self.tol_cg,
                            self.precision, relative=self.relative_tol, structural=structural)
         self.mg.setup(A, Ps)
+        if self.omega_levels:
+            w = list(self.omega_levels)
+            self.mg.set_level_omegas([w[min(l, len(w) - 1)] for l in range(len(self.mg.levels))])
         self.A_data = data
 
     def _compute_C_and_gradC(self):
         n2 = np.zeros(1, np.float64)
-        self.L.orc_constraints(self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p),
+        fn = self.L.orc_constraints_vol if self.volume else self.L.orc_constraints
+        fn(self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p),
                                self._p(self.Dm_inv), self._p(self.alpha_tilde),
                                self._p(self.lambdaf), self._p(self.constraints),
                                self._p(self.gradC), self._p(self.b), ptr(n2, F64P))
         return float(np.sqrt(n2[0]))
 
     def _fill_A(self, data):
-        self.L.orc_fill_A(self.n_tets, ptr(self.indptr, I32P), ptr(self.indices, I32P),
-                          ptr(self.t_i, I32P), self._p(self.v_invm), self._p(self.gradC),
+        self.L.orc_fill_A(self.n_cons, ptr(self.indptr, I32P), ptr(self.indices, I32P),
+                          ptr(self.t_c, I32P), self._p(self.v_invm), self._p(self.gradC),
                           self._p(self.alpha_tilde), self._p(data))
 
     def step(self, frame=0):
@@ -388,7 +440,8 @@ class OracleMGPBD:
 
     def _dual_norm(self, omega=None):
         dx = None if omega is None else self.dpos
-        return float(np.sqrt(self.L.orc_dual_norm2(
+        fn = self.L.orc_dual_norm2_vol if self.volume else self.L.orc_dual_norm2
+        return float(np.sqrt(fn(
             self.n_tets, ptr(self.t_i, I32P), self._p(self.v_p), self._p(dx),
             0.0 if omega is None else omega, self._p(self.Dm_inv), self._p(self.alpha_tilde),
             self._p(self.lambdaf))))
@@ -424,7 +477,7 @@ class OracleMGPBD:
             self.mg.update_all_coarse_matrices()
             self.dlam[:] = 0
             cg_iters = self.mg.solve(self.b, self.dlam)
-            self.L.orc_scatter(self.n_tets, self.n_verts, ptr(self.t_i, I32P),
+            self.L.orc_scatter(self.n_cons, self.n_verts, ptr(self.t_c, I32P),
                                self._p(self.v_invm), self._p(self.gradC), self._p(self.dlam),
                                self._p(self.lambdaf), self._p(self.dpos))
             omega = self._line_search() if self.use_line_search else 1.0
