@@ -1,18 +1,27 @@
 """Benchmark: MGPBD substeps/s on one MI355X.
 
-Default workload (the 1M-tet configuration): jittered block of 56^3 Kuhn cubes (185,193
-vertices, 1,053,696 tets), h = 0.02 m, per-tet E log-uniform in [1e4, 1e6] Pa (seed 0),
-nu = 0.45, rho = 1000 kg/m^3, gravity -9.8 m/s^2, top face held, dt = 1/300 s (60 fps x 5
-substeps), Newton defaults of the reference (maxiter 20, atol 1e-4, rtol 1e-2, line search),
-PCG to relative residual 1e-6 (max 100 iterations), tet-aggregation hierarchy extended to
-5 levels, coarsest level solved exactly (FP64 dense inverse). Runs: the reference's algorithm
-(2+2 weighted-Jacobi sweeps, omega 2/3, FP32, assembled CSR) and the Chebyshev-smoothed solver
-(8+8 steps on level 0 and 1+1 on the coarse levels on [1.2 lambda / 40, 1.2 lambda] of the
-device's spectral estimates, FP64 iterate, FP64-verified solves) in FP32 (assembled CSR), FP16
-and BF16 (factored level 0), and FP16 with the previous-lambda and the learned warm starts.
-A step is one substep
-(predict, Newton/MGPCG solve, velocity update) from the state the previous step left; inputs
-are resident in HBM.
+Default workload (the 1M-tet configuration, controllable_pbd_3d_amd/workloads.py "c1"):
+jittered block of 56^3 Kuhn cubes (185,193 vertices, 1,053,696 tets), h = 0.02 m, per-tet E
+log-uniform in [1e4, 1e6] Pa (seed 0), nu = 0.45, rho = 1000 kg/m^3, gravity -9.8 m/s^2, top
+face held, dt = 1/300 s (60 fps x 5 substeps), Newton defaults of the reference (maxiter 20,
+atol 1e-4, rtol 1e-2, line search), PCG to relative residual 1e-6 (max 100 iterations),
+tet-aggregation hierarchy extended to 5 levels, coarsest level solved exactly (FP64 dense
+inverse). One ARAP constraint per tet (the reference's material) unless a run says "volume".
+
+Runs (each its own solver on the same mesh, same loading):
+  fp32_reference  the reference's algorithm (2 + 2 weighted-Jacobi sweeps, omega 2/3, FP32,
+                  assembled CSR; the FP64 residual is measured, not enforced)
+  fp32            Chebyshev smoothing on the device's spectral estimates, FP64 iterate, every
+                  solve FP64-verified; FP32 assembled level 0 (the metric's denominator)
+  fp16_assembled  the same with the stored FP16 level 0 (same operator as fp32: the precision's
+                  own gain)
+  fp16, bf16      FP16 / BF16 V-cycle on the factored level 0 J W J^T + alpha~
+  fp16_previous   fp16 started from the previous substep's lambda
+  fp16_learned    fp16 started from the learned warm start (network trained in the setup)
+  fp32_volume, fp16_volume   fp32 / fp16 with the volume constraint beside ARAP (lambda from
+                  nu = 0.45: 2 constraints per tet)
+A step is one substep (predict, Newton/MGPCG solve, velocity update) from the state the
+previous step left; inputs are resident in HBM.
 
 Prints one JSON line. `--gpus N` with N > 1 exits: this build measures one GPU.
 """
@@ -22,8 +31,8 @@ from __future__ import annotations
 import argparse
 import ctypes
 import json
-import math
 import os
+import platform
 import sys
 import time
 
@@ -32,165 +41,61 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from controllable_pbd_3d_amd.workloads import (CONFIGS, Loading, build_workload,   # noqa: E402
+                                               solver_args)
+
 HBM_PEAK_GBS = 8000.0
-
-CONFIGS = {
-    # box: n^3 Kuhn cubes of edge h, per-tet E log-uniform in E (constant 1e5 when None), top
-    # face held; ellipsoid: grid of edge h clipped to radii (1.0, 0.6, 0.4) m, stiff sphere of
-    # radius 0.2 m (E 1e8) in an E 1e5 matrix, lowest 5% of vertices held. drag: every frame
-    # (5 substeps) a seeded random free vertex is pulled by a 1e3 N/m spring toward a target
-    # uniform in a 0.5 m cube around its rest position. maxiter_cg replaces the reference's
-    # 100-iteration PCG cap where some systems need more to reach 1e-6.
-    # smoothing: Chebyshev steps per phase on level 0 and on the coarse levels and the
-    # interval ratio of every Chebyshev run of the workload (all precisions alike; measured
-    # per workload, default 5 / 1 on [1.2 lambda / 30, 1.2 lambda])
-    "c1": dict(kind="box", n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets",
-               smoothing=((8, 1), 40.0)),
-    "c2": dict(kind="box", n=56, h=0.02, E=(1e4, 1e8), max_levels=5,
-               label="box56_1053696tets_E1e4-1e8", smoothing=((8, 1), 40.0)),
-    "c3": dict(kind="box", n=32, h=0.03, E=(1e4, 1e6), max_levels=4, drag=True,
-               label="box32_196608tets_drag"),
-    "c4": dict(kind="ellipsoid", h=0.01, E=(1e5, 1e8), max_levels=6, drag=True,
-               maxiter_cg=1000, label="ellipsoid_h0.01_inclusion_drag"),
-    "c0": dict(kind="box", n=8, h=0.05, E=None, max_levels=3, label="box8_3072tets"),
-}
-DRAG_K, DRAG_BOX, SUBSTEPS_PER_FRAME = 1e3, 0.5, 5
-
-
-def build_workload(cfg, seed=0):
-    from controllable_pbd_3d_amd.mesh import box_mesh, ellipsoid_mesh
-    if cfg["kind"] == "ellipsoid":
-        mesh = ellipsoid_mesh(cfg["h"], jitter=0.05, seed=seed, rho=1000.0,
-                              E_matrix=cfg["E"][0], E_inclusion=cfg["E"][1])
-    else:
-        mesh = box_mesh(cfg["n"], cfg["h"], jitter=0.05, seed=seed, rho=1000.0,
-                        E_range=cfg["E"] if cfg["E"] else (1e5, 1e5))
-    mu_t = (mesh.E / (2.0 * (1.0 + 0.45))).astype(np.float32)
-    return mesh, mu_t
-
-
-class Loading:
-    """Per-substep loading of a workload: nothing, or the drag schedule of ``seed``."""
-
-    def __init__(self, cfg, mesh, seed, substeps):
-        from controllable_pbd_3d_amd.mesh import drag_schedule
-        self.frames = None
-        if cfg.get("drag"):
-            self.frames = drag_schedule(mesh, seed, substeps // SUBSTEPS_PER_FRAME + 1, DRAG_BOX)
-        self.k = 0
-
-    def step(self, s):
-        if self.frames is not None and self.k % SUBSTEPS_PER_FRAME == 0:
-            v, target = self.frames[self.k // SUBSTEPS_PER_FRAME]
-            s.set_drag(v, target, DRAG_K)
-        s.step(self.k // SUBSTEPS_PER_FRAME)
-        self.k += 1
-
-
-def solver_args(mesh, mu_t):
-    return dict(v_p=mesh.verts.copy(), v_p_ref=mesh.verts.copy(), v_invm=mesh.invm.copy(),
-                t_i=mesh.tets.copy(), t_m=mesh.tet_mass.copy(), gravity=(0.0, -9.8, 0.0),
-                dt=1.0 / 60.0 / 5.0, maxiter=20, atol=1e-4, rtol=1e-2, maxiter_cg=100,
-                n_smooth=2, omega_jacobi=2.0 / 3.0, use_line_search=True, mu_per_tet=mu_t)
 
 
 def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
-    """Algorithmic HBM bytes per launch; per_nnz = 8 for FP32 values + int32 columns, 6 for FP32
-    values + 16-bit LDS offsets (windowed level 0), 4 for FP16/BF16 values + 16-bit column
-    offsets or LDS offsets (each array counted once). With the Chebyshev
-    smoother the level-0 sweeps also read the previous iterate (all steps but the Jacobi start;
-    the last post-step reads the unscaling factor too): 4 n bytes more on average over a
-    V-cycle's 4 + 1 + 1 step kinds."""
+    """Algorithmic HBM bytes per launch of a stored level; per_nnz = 8 for FP32 values + int32
+    columns, 6 for FP32 values + 16-bit LDS offsets (windowed level 0), 4 for FP16/BF16 values +
+    16-bit column or LDS offsets (each array counted once). With the Chebyshev smoother the
+    level-0 sweeps also read the previous iterate (all steps but the Jacobi start): 4 n bytes
+    more, counted on every sweep."""
     if kind == "spmv":      # values + columns + row pointers + x + y
         return per_nnz * nnz + 4 * (n_rows + 1) + 4 * n_rows + 4 * n_rows
     if kind == "jacobi":    # + b, D^-1 (FP32 only: the scaled levels have unit diagonal), x -> x_new
-        return per_nnz * nnz + 4 * (n_rows + 1) + (16 if per_nnz == 8 else 12) * n_rows + \
+        return per_nnz * nnz + 4 * (n_rows + 1) + (16 if per_nnz >= 6 else 12) * n_rows + \
             (4 * n_rows if chebyshev else 0)
     if kind == "resid":     # values + columns + ptr + x + b -> r
         return per_nnz * nnz + 4 * (n_rows + 1) + 12 * n_rows
     raise ValueError(kind)
 
 
-def factored_bytes(n_tets, n_verts, kind, chebyshev=False):
-    """Algorithmic HBM bytes of one application of the factored level 0 (vertex pass + tet
-    pass), each array counted once per pass. V-cycle sweeps and residuals: vertex pass = 4
-    incidence slots per tet x (4-byte incidence + 8-byte K) + x (4 n) + u written (16 per
-    vertex); tet pass = t_i (16) + K (24) + a^ (4) + x (4) + b (4) + y (4) per tet + u read (16
-    per vertex), + the previous iterate (4 n) on average with the Chebyshev smoother (as for
-    the stored levels). The residual pass reads and writes the same arrays."""
+def factored_bytes(n_rows, n_verts, kind, chebyshev=False):
+    """Algorithmic HBM bytes of the factored level 0 (each array counted once per pass; 4
+    incidence slots per row). kind "gather" (vertex pass): 4 slots per row x (4-byte incidence
+    + 8-byte K) + x (4 per row) + u written (16 per vertex) = 52 n + 16 n_v. "apply" (tet pass
+    of a sweep): t_i 16 + K 24 + a^ 4 + x 4 + b 4 + y 4 per row + u read (16 per vertex)
+    = 56 n + 16 n_v. "jacobi"/"resid": both passes (+ the previous iterate, 4 n, for the
+    Chebyshev sweeps). "spmv": the PCG's FP64 product from the FP32 gradients, 4 slots x
+    (4-byte incidence + 16-byte gradient) + p + u (32 per vertex) written; t_i 16 + gradC 48 +
+    alpha~ 4 + p 4 + A p 4 per row + u read."""
+    gather = 52 * n_rows + 16 * n_verts
+    apply = 56 * n_rows + 16 * n_verts
+    if kind == "gather":
+        return gather
+    if kind == "apply":
+        return apply
     if kind == "spmv":
-        # the PCG's FP64 product from the FP32 gradients: 4 slots per tet x (4-byte incidence +
-        # 16-byte gradient) + p (4 n) + u (32 per vertex) written; t_i (16) + gradC (48) +
-        # alpha~ (4) + p (4) + A p (4) per tet + u read
-        return 80 * n_tets + 4 * n_tets + 32 * n_verts + 76 * n_tets + 32 * n_verts
-    if kind not in ("jacobi", "resid"):
-        raise ValueError(kind)
-    vertex_pass = 48 * n_tets + 4 * n_tets + 16 * n_verts
-    tet_pass = 56 * n_tets + 16 * n_verts
-    return vertex_pass + tet_pass + (4 * n_tets if chebyshev and kind == "jacobi" else 0)
+        return 80 * n_rows + 4 * n_rows + 32 * n_verts + 76 * n_rows + 32 * n_verts
+    if kind in ("jacobi", "resid"):
+        return gather + apply + (4 * n_rows if chebyshev and kind == "jacobi" else 0)
+    raise ValueError(kind)
 
 
-def cpu_baseline(mesh, mu_t, solver, newton_per_substep, pcg_per_substep, pcg_sample=4):
-    """CPU baseline: the oracle's C restatement of the reference solver (FP32 arithmetic, one
-    thread) on the same mesh, material and solver configuration (the same five-level
-    hierarchy, relative 1e-6 PCG). Bounded sample: one Newton iteration of the first substep
-    -- constraints + A fill + coarse refill, ``pcg_sample`` PCG iterations, scatter + one
-    line-search trial + apply -- timed piecewise and scaled to substeps with the Newton and
-    PCG iteration counts per substep the GPU run measured (same algorithm, same counts)."""
-    import scipy.sparse as sp
-    from oracle import mgpbd_oracle as om
-    args = solver_args(mesh, mu_t)
-    o = om.OracleMGPBD(**args, precision="f32", hold_pinned=True, relative_tol=True,
-                       tol_cg=1e-6)
-    Ps = [sp.csr_matrix((np.ones(len(a)), (np.arange(len(a)), a)), shape=(len(a), n))
-          for a, n in solver._aggs]
-    t0 = time.perf_counter()
-    o.init(Ps=Ps, structural=True)
-    setup_s = time.perf_counter() - t0
-    o.make_prediction()
-    o.lambdaf[:] = 0
-    t0 = time.perf_counter()
-    o._compute_C_and_gradC()
-    o._fill_A(o.A_data)
-    o.mg.update_fine_matrix_data(o.A_data)
-    o.mg.update_all_coarse_matrices()
-    t_assemble = time.perf_counter() - t0
-    o.mg.maxiter_cg = pcg_sample
-    o.dlam[:] = 0
-    t0 = time.perf_counter()
-    it = o.mg.solve(o.b, o.dlam)
-    t_pcg = (time.perf_counter() - t0) / max(it, 1)
-    t0 = time.perf_counter()
-    o.L.orc_scatter(o.n_tets, o.n_verts, om.ptr(o.t_i, om.I32P), o._p(o.v_invm), o._p(o.gradC),
-                    o._p(o.dlam), o._p(o.lambdaf), o._p(o.dpos))
-    o._dual_norm(None)
-    o._dual_norm(1.0)
-    o.L.orc_apply_dpos(o.n_verts, o._p(o.v_p), o._p(o.dpos), o._p(o.v_invm), 1.0)
-    t_update = time.perf_counter() - t0
-    per_substep = newton_per_substep * (t_assemble + t_update) + pcg_per_substep * t_pcg
-    sample_s = t_assemble + t_pcg * max(it, 1) + t_update
-    return {"value": 1.0 / per_substep, "unit": "substeps/s", "cores": 1, "kind": "port",
-            "sample": (f"one Newton iteration of the first substep on the same mesh "
-                       f"({mesh.n_tets} tets) and solver configuration ({it} PCG iterations), "
-                       f"FP32 C oracle, "
-                       f"1 thread: assemble {t_assemble:.2f} s, PCG {t_pcg:.2f} s/iteration, "
-                       f"update {t_update:.2f} s; scaled by the GPU run's "
-                       f"{newton_per_substep:.1f} Newton and {pcg_per_substep:.0f} PCG "
-                       f"iterations per substep; {sample_s:.1f} s of CPU work, setup "
-                       f"{setup_s:.1f} s excluded; host has {os.cpu_count()} cores")}
-
-
-def measured_traffic(workload, kind, precision):
-    """HBM bytes per launch of a level-0 kernel of this workload from the committed rocprofv3
-    counter summaries (``bench-traffic: <workload> <kind> <precision> <bytes>`` lines in
-    profiles/*.md), or None when that workload was not profiled."""
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel of this workload from the committed rocprofv3 counter
+    summaries (``bench-traffic: <workload> <kernel> <bytes>`` lines in profiles/*.md), or None
+    when that workload was not profiled."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             parts = line.split()
-            if len(parts) == 5 and parts[0] == "bench-traffic:" and parts[1] == workload and \
-                    parts[2] == kind and parts[3] == precision:
-                return float(parts[4])
+            if len(parts) == 4 and parts[0] == "bench-traffic:" and parts[1] == workload and \
+                    parts[2] == kernel:
+                return float(parts[3])
     return None
 
 
@@ -202,8 +107,8 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
         cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     kern = {}
     # the outer PCG SpMV is FP32 (values beside int32 columns, or beside 16-bit LDS offsets on
-    # a windowed level 0); the V-cycle passes use the precision under test
-    vb = spmv_per_nnz if precision == "fp32" else 4   # FP32 sweeps use the same level-0 layout
+    # a windowed level 0); the V-cycle passes use the precision under test on the same layout
+    vb = spmv_per_nnz if precision == "fp32" else 4
     for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
@@ -222,51 +127,55 @@ CHEBYSHEV = dict(smoother="chebyshev", n_smooth=5, smooth_schedule=[5, 1], cheb_
                  smoother_safety=1.2, verify_residual=1, fp64_iterate=True,
                  spectral_refresh="substep")
 RUNS = {
-    # the reference's algorithm (fixed 2/3-weighted Jacobi); the FP64 residual of every solve
-    # is measured, not enforced
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
-    # Chebyshev smoothing on the device's spectral estimates (5 + 5 steps on level 0, 1 + 1 on
-    # the coarse levels, on [1.2 lambda / 30, 1.2 lambda], unless the workload sets its own
-    # schedule: CONFIGS "smoothing"; spectral estimates refreshed at each substep's first
-    # Newton iteration), FP64 PCG iterate, every solve FP64-verified
     "fp32": dict(precision="fp32", **CHEBYSHEV),
-    "fp16": dict(precision="fp16", **CHEBYSHEV),
-    "bf16": dict(precision="bf16", **CHEBYSHEV),
-    "fp16_previous": dict(precision="fp16", warm_start="previous", **CHEBYSHEV),
-    "fp16_learned": dict(precision="fp16", warm_start="learned", **CHEBYSHEV),
+    "fp16_assembled": dict(precision="fp16", level0="assembled", **CHEBYSHEV),
+    "fp16": dict(precision="fp16", level0="factored", **CHEBYSHEV),
+    "bf16": dict(precision="bf16", level0="factored", **CHEBYSHEV),
+    "fp16_previous": dict(precision="fp16", level0="factored", warm_start="previous", **CHEBYSHEV),
+    "fp16_learned": dict(precision="fp16", level0="factored", warm_start="learned", **CHEBYSHEV),
+    "fp32_volume": dict(precision="fp32", volume=True, **CHEBYSHEV),
+    "fp16_volume": dict(precision="fp16", level0="factored", volume=True, **CHEBYSHEV),
 }
+DEFAULT_RUNS = list(RUNS)
 
 
-def make_solver(mesh, mu_t, cfg, kw, model=None, record=None):
+def run_kwargs(name, cfg):
+    kw = dict(RUNS[name])
+    if kw.get("smoother") == "chebyshev" and cfg.get("smoothing"):
+        (k0, k1), ratio = cfg["smoothing"]
+        kw.update(n_smooth=max(k0, k1), smooth_schedule=[k0, k1], cheb_ratio=ratio)
+    return kw
+
+
+def make_solver(mesh, mu_t, lam_t, cfg, kw, model=None, record=None):
     from controllable_pbd_3d_amd import MGPBDSolverGPU
-    args = solver_args(mesh, mu_t)
-    args["maxiter_cg"] = cfg.get("maxiter_cg", args["maxiter_cg"])
-    args.update(hold_pinned=True, relative_tol=True, tol_cg=1e-6, hierarchy="extended",
-                galerkin="structural", max_coarse=64, max_levels=cfg["max_levels"],
-                coarse_solver="cholesky", warm_start_model=model, record_samples=record,
-                verbose=False)
+    kw = dict(kw)
+    volume = kw.pop("volume", False)
+    args = solver_args(mesh, mu_t, lam_t, volume=volume,
+                       maxiter_cg=cfg.get("maxiter_cg", 100), hierarchy="extended",
+                       galerkin="structural", max_coarse=64, max_levels=cfg["max_levels"],
+                       coarse_solver="cholesky", warm_start_model=model, record_samples=record,
+                       verbose=False)
     args.update(kw)
     s = MGPBDSolverGPU(**args)
     s.init()
     return s
 
 
-def train_learned(cfg, seeds, substeps, epochs, level0="factored"):
+def train_learned(cfg, seeds, substeps, epochs):
     """Warm-start network trained on first-Newton-iteration pairs the solver produces on
     other seeds of the same workload (mesh jitter and per-tet E differ), A-norm loss."""
     import torch
     from controllable_pbd_3d_amd.warm_start import LearnedWarmStart
     t0 = time.perf_counter()
     samples, model = [], None
-    kw = dict(RUNS["fp16_previous"], level0=level0)
-    if cfg.get("smoothing"):
-        (k0, k1), ratio = cfg["smoothing"]
-        kw.update(n_smooth=max(k0, k1), smooth_schedule=[k0, k1], cheb_ratio=ratio)
+    kw = run_kwargs("fp16_previous", cfg)
     for seed in seeds:
-        mesh, mu_t = build_workload(cfg, seed)
+        mesh, mu_t, lam_t = build_workload(cfg, seed)
         if model is None:
             model = LearnedWarmStart.for_mesh(mesh.tets, mesh.n_tets, seed=0, lr=1e-3)
-        s = make_solver(mesh, mu_t, cfg, kw, model=model, record=samples)
+        s = make_solver(mesh, mu_t, lam_t, cfg, kw, model=model, record=samples)
         load = Loading(cfg, mesh, seed, substeps)
         for k in range(substeps):
             load.step(s)
@@ -284,23 +193,45 @@ def train_learned(cfg, seeds, substeps, epochs, level0="factored"):
             "epochs": epochs, "energy_loss_untrained": before, "energy_loss_epochs": hist,
             "generate_s": gen_s, "train_s": train_s}
     del samples
+    model.release_training_memory()
     torch.cuda.empty_cache()
     return model, info
 
 
-def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, seed=0,
-               level0="factored"):
+def probe_level0(s, reps=60, flush_gib=1.0):
+    """Per-kernel HIP-event timing of `reps` level-0 sweeps of this solver's V-cycle
+    (mgpbd_mg_probe_level0), back to back and with a flush_gib GiB buffer rewritten before each
+    sweep (larger than the 256 MB last-level cache: the sweep then reads from HBM)."""
+    import torch
+    from controllable_pbd_3d_amd import _lib
+    mg = s._gpu_solver
+    n = s.n_cons
+    dev = s.device
+    g = torch.Generator(device="cpu").manual_seed(1)
+    b = torch.randn(n, generator=g).to(dev)
+    x = torch.randn(n, generator=g).to(dev)
+    y = torch.zeros(n, dtype=torch.float32, device=dev)
+    out = {}
+    flush = torch.ones(int(flush_gib * (1 << 28)), dtype=torch.float32, device=dev)
+    for tag, fb in (("warm", None), ("flushed", flush)):
+        ms = np.zeros(3, np.float64)
+        mg.ctx("mgpbd_mg_probe_level0", reps, b.data_ptr(), x.data_ptr(), y.data_ptr(),
+               None if fb is None else fb.data_ptr(), 0 if fb is None else fb.numel(),
+               ms.ctypes.data_as(_lib.F64P))
+        out[tag] = {"first_us": ms[0] * 1e3, "second_us": ms[1] * 1e3, "sweep_us": ms[2] * 1e3}
+    del flush
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_config(mesh, mu_t, lam_t, cfg, name, steps, warmup, profile_steps, model=None, seed=0):
     import torch
     from controllable_pbd_3d_amd.runtime import get_context
     ctx = get_context(0)
-    kw = dict(RUNS[name])
-    if kw["precision"] != "fp32":
-        kw["level0"] = level0
-    if kw.get("smoother") == "chebyshev" and cfg.get("smoothing"):
-        (k0, k1), ratio = cfg["smoothing"]
-        kw.update(n_smooth=max(k0, k1), smooth_schedule=[k0, k1], cheb_ratio=ratio)
+    kw = run_kwargs(name, cfg)
     t0 = time.perf_counter()
-    s = make_solver(mesh, mu_t, cfg, kw, model=model if kw.get("warm_start") == "learned" else None)
+    s = make_solver(mesh, mu_t, lam_t, cfg, kw,
+                    model=model if kw.get("warm_start") == "learned" else None)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     print(f"[bench] {name}: setup {setup_s:.1f} s, levels {s._gpu_solver._level_sizes}",
@@ -309,7 +240,7 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     for k in range(warmup):
         load.step(s)
     torch.cuda.synchronize()
-    iters, newton, solves, starts, first_its = [], [], [], [], []
+    iters, newton, solves, starts, first_its, predicted = [], [], [], [], [], 0
     t0 = time.perf_counter()
     for k in range(steps):
         load.step(s)
@@ -318,6 +249,7 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
         solves.extend(s.last_solve_info)
         starts.append(s.first_solve_info.get("start", 0))
         first_its.append(s.first_solve_info.get("cg_iters", 0))
+        predicted += int(getattr(s, "last_predicted", False))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     rel = np.array([np.sqrt(max(d["rr_true"], 0.0) / d["bb"]) if d["bb"] > 0 else 0.0
@@ -327,6 +259,7 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
           f"converged {conv.mean() * 100:.0f}% of solves, FP64 rel residual max {rel.max():.2e}",
           file=sys.stderr, flush=True)
     res = {"value": steps / elapsed, "ms_per_step": 1e3 * elapsed / steps, "setup_s": setup_s,
+           "n_rows": s.n_cons, "nnz_A0": int(s._nnz),
            "pcg_iters_per_substep": float(np.mean(iters)),
            "newton_iters_per_substep": float(np.mean(newton)),
            "first_solve_pcg_iters": float(np.mean(first_its)),
@@ -340,6 +273,10 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     if kw.get("warm_start"):
         # start of each substep's first solve: 0 zero, 1 the guess, 2 the previous lambda
         res["first_solve_starts"] = np.bincount(starts, minlength=3).tolist()
+    if kw.get("warm_start") == "learned":
+        res["network_inferences"] = predicted
+    if kw.get("smoother") == "chebyshev":
+        res["spectral_estimates"] = [float(v) for v in s._gpu_solver.smoother_estimates()]
 
     # Kernel durations: HIP events around each launch of the main kernels, on the stream they
     # run on, over `profile_steps` further substeps (the event-bracketed PCG launches eagerly
@@ -352,13 +289,115 @@ def run_config(mesh, mu_t, cfg, name, steps, warmup, profile_steps, model=None, 
     win0 = any(w["level"] == 0 and w["woff"] is not None for w in s._gpu_solver._win)
     per = 6 if win0 else 8
     fnv = s.n_verts if kw.get("level0") == "factored" else None
-    kern = kernel_stats(s._gpu_solver.ctx, s.n_tets, s._nnz, kw["precision"], cheb, per, fnv)
-    kern.update({k: v for k, v in kernel_stats(ctx, s.n_tets, s._nnz, kw["precision"], cheb,
+    kern = kernel_stats(s._gpu_solver.ctx, s.n_cons, s._nnz, kw["precision"], cheb, per, fnv)
+    kern.update({k: v for k, v in kernel_stats(ctx, s.n_cons, s._nnz, kw["precision"], cheb,
                                                per, fnv).items() if k not in kern})
     ctx("mgpbd_ctx_profile", 0)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 0)
+    over = [k for k, v in kern.items() if v.get("GBs", 0.0) > HBM_PEAK_GBS]
+    if over:   # an algorithmic-byte count above the peak is a wrong byte model
+        raise SystemExit(f"bench: {name} kernels {over} exceed {HBM_PEAK_GBS} GB/s")
     res["kernels"] = kern
     return s, res
+
+
+def cpu_baseline(mesh, mu_t, lam_t, cfg, omegas, newton_ref, pcg_ref, volume=False):
+    """CPU baseline: the oracle's OpenMP C restatement (FP32 arithmetic) of the solver, on the
+    GPU host's cores, same mesh, material, hierarchy and PCG tolerance. Converging
+    configuration: weighted Jacobi 2 + 2 with fixed per-level weights (from the GPU run's
+    spectral estimates), 50 Jacobi sweeps on the coarsest level, relative 1e-6. Timed: one
+    full substep on all threads (measured, not scaled); then, on 1 thread, one assembly and
+    one PCG iteration, scaled by the all-thread substep's own Newton and PCG counts; and the
+    reference algorithm's cost (omega 2/3 on the two-level hierarchy, 50 coarse sweeps), one
+    PCG iteration's passes timed on all threads and scaled by the GPU reference run's PCG
+    count per substep (it does not converge, so its counts are the caps)."""
+    import scipy.sparse as sp
+    from oracle import mgpbd_oracle as om
+    L, R, _ = om.lib("f32")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L.orc_set_threads(threads)
+    args = solver_args(mesh, mu_t, lam_t, volume=volume)
+    okeys = ("v_p", "v_p_ref", "v_invm", "t_i", "t_m", "gravity", "dt", "maxiter", "atol",
+             "rtol", "maxiter_cg", "tol_cg", "relative_tol", "n_smooth", "omega_jacobi",
+             "use_line_search", "mu_per_tet", "hold_pinned", "volume_lambda")
+    o = om.OracleMGPBD(**{k: v for k, v in args.items() if k in okeys}, precision="f32",
+                       omega_levels=omegas)
+    from controllable_pbd_3d_amd import tet_aggregation
+    aggs = tet_aggregation.build_extended_hierarchy(mesh.tets, mesh.n_tets, max_coarse=64,
+                                                    max_levels=cfg["max_levels"] - 1)
+    Ps = [sp.csr_matrix((np.ones(len(a)), (np.arange(len(a)), a)), shape=(len(a), n))
+          for a, n in aggs]
+    t0 = time.perf_counter()
+    o.init(Ps=Ps, structural=True)
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step(0)
+    sub_s = time.perf_counter() - t0
+    hist = o.history[-1]
+    newton = len(hist)
+    pcg = sum(c for _, _, c, _ in hist)
+
+    def assembly():
+        t0 = time.perf_counter()
+        o._compute_C_and_gradC()
+        o._fill_A(o.A_data)
+        o.mg.update_fine_matrix_data(o.A_data)
+        o.mg.update_all_coarse_matrices()
+        return time.perf_counter() - t0
+
+    def pcg_iteration():
+        o.mg.maxiter_cg = 1
+        o.dlam[:] = 0
+        t0 = time.perf_counter()
+        o.mg.solve(o.b, o.dlam)
+        return time.perf_counter() - t0
+
+    t_asm_n = assembly()
+    L.orc_set_threads(1)
+    t_asm1, t_it1 = assembly(), pcg_iteration()
+    one_thread = 1.0 / (newton * t_asm1 + pcg * t_it1)
+    # the reference algorithm's PCG iteration on all threads: level 0 SpMV + 2 + 2 sweeps +
+    # residual (6 level-0 passes) and 50 Jacobi sweeps of its level 1 (this hierarchy's level
+    # 1: the reference's tet aggregation), 100 level-1 passes
+    L.orc_set_threads(threads)
+    lv0, lv1 = o.mg.levels[0], o.mg.levels[1]
+    v0 = np.zeros(lv0["n"], np.float32)
+    v1 = np.zeros(lv1["n"], np.float32)
+    t0 = time.perf_counter()
+    for _ in range(6):
+        L.orc_spmv(lv0["n"], om.ptr(lv0["ip"], om.I32P), om.ptr(lv0["ix"], om.I32P),
+                   om.ptr(lv0["a"], R), om.ptr(lv0["b"], R), om.ptr(v0, R))
+    for _ in range(100):
+        L.orc_spmv(lv1["n"], om.ptr(lv1["ip"], om.I32P), om.ptr(lv1["ix"], om.I32P),
+                   om.ptr(lv1["a"], R), om.ptr(lv1["b"], R), om.ptr(v1, R))
+    t_ref_it = time.perf_counter() - t0
+    cpu_model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": 1.0 / sub_s, "unit": "substeps/s", "cores": threads, "kind": "port",
+            "sample": (f"one full substep of the 1st frame on the same mesh ({mesh.n_tets} tets"
+                       f"{', volume constraint' if volume else ''}), FP32 OpenMP C oracle, "
+                       f"{threads} threads: {sub_s:.1f} s, {newton} Newton / {pcg} PCG "
+                       f"iterations to relative 1e-6 (weighted Jacobi 2+2 with per-level "
+                       f"weights {np.round(omegas, 3).tolist()}, same 5 levels, 50 coarsest "
+                       f"sweeps); setup {setup_s:.1f} s excluded"),
+            "one_thread_substeps_per_s": one_thread,
+            "one_thread_sample": (f"1 thread: assembly {t_asm1:.2f} s, PCG iteration "
+                                  f"{t_it1:.2f} s, scaled by the substep's {newton} Newton and "
+                                  f"{pcg} PCG iterations"),
+            "reference_algorithm_substeps_per_s": 1.0 / (newton_ref * t_asm_n +
+                                                         pcg_ref * t_ref_it),
+            "reference_algorithm_sample": (
+                f"{threads} threads: assembly {t_asm_n:.2f} s, one PCG iteration of the "
+                f"reference algorithm (6 level-0 + 100 level-1 passes) {t_ref_it:.2f} s, scaled "
+                f"by the GPU reference run's {newton_ref:.1f} Newton and {pcg_ref:.0f} PCG "
+                f"iterations per substep (capped at 100 per solve, unconverged)"),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model}
 
 
 def main():
@@ -375,9 +414,7 @@ def main():
     ap.add_argument("--runs", default="all",
                     help="comma list of " + ",".join(RUNS) + " (default: all)")
     ap.add_argument("--profile-steps", type=int, default=2)
-    ap.add_argument("--level0", default="factored", choices=("factored", "assembled"),
-                    help="level 0 of the reduced-precision V-cycles: factored K K^T + diag "
-                         "from the gradients, or the stored half values")
+    ap.add_argument("--probe-reps", type=int, default=60)
     ap.add_argument("--train-seeds", default="1,2")
     ap.add_argument("--train-substeps", type=int, default=8)
     ap.add_argument("--train-epochs", type=int, default=20)
@@ -391,42 +428,47 @@ def main():
 
     import torch
     cfg = CONFIGS[args.config]
-    names = list(RUNS) if args.runs == "all" else args.runs.split(",")
+    names = DEFAULT_RUNS if args.runs == "all" else args.runs.split(",")
     for n in names:
         if n not in RUNS:
             raise SystemExit(f"unknown run {n}")
     print(f"[bench] building {cfg['label']}", file=sys.stderr, flush=True)
-    mesh, mu_t = build_workload(cfg, args.seed)
+    mesh, mu_t, lam_t = build_workload(cfg, args.seed)
     torch.cuda.set_device(0)
     model, learn_info = None, None
     if "fp16_learned" in names:
         model, learn_info = train_learned(cfg, [int(x) for x in args.train_seeds.split(",")],
-                                          args.train_substeps, args.train_epochs, args.level0)
-    runs, keep, pos = {}, None, {}
+                                          args.train_substeps, args.train_epochs)
+    head = "fp16_learned" if "fp16_learned" in names else names[-1]
+    runs, pos, probe, omegas, keep = {}, {}, None, None, None
     for name in names:
-        s, res = run_config(mesh, mu_t, cfg, name, args.steps, args.warmup, args.profile_steps,
-                            model, args.seed, args.level0)
-        if RUNS[name]["precision"] != "fp32":
-            res["level0"] = args.level0
+        s, res = run_config(mesh, mu_t, lam_t, cfg, name, args.steps, args.warmup,
+                            args.profile_steps, model, args.seed)
         runs[name] = res
         pos[name] = s.v_p.detach().cpu().numpy().astype(np.float64)
+        if name == head:
+            probe = probe_level0(s, args.probe_reps)
+            res["level0_probe"] = probe
+        if name == "fp32":
+            # per-level Jacobi weights for the CPU baseline's converging configuration
+            omegas = [float((4.0 / 3.0) / (1.2 * max(l, 1.0))) for l in res["spectral_estimates"]]
         if keep is None:
-            keep = s            # hierarchy and nnz for the report and the CPU baseline
-        else:
-            del s
+            keep = (s._gpu_solver._level_sizes, int(s._nnz))
+        del s
         torch.cuda.empty_cache()
-    if "fp32" in pos:
-        # every run takes the same substeps under the same loading; positions after the last
-        # one, against the FP32 run with FP64-verified solves, in units of the edge length
-        for name in names:
-            d = np.abs(pos[name] - pos["fp32"]).max()
-            runs[name]["max_position_diff_vs_fp32_over_h"] = float(d / mesh.h)
-    head = "fp16_learned" if "fp16_learned" in runs else names[-1]
+    for group in (("fp32", "fp32_reference", "fp16_assembled", "fp16", "bf16", "fp16_previous",
+                   "fp16_learned"), ("fp32_volume", "fp16_volume")):
+        # every run of a group takes the same substeps under the same loading: positions after
+        # the last one against the group's FP32 run (FP64-verified solves), in edge lengths
+        ref = group[0]
+        if ref in pos:
+            for name in group:
+                if name in pos:
+                    d = np.abs(pos[name] - pos[ref]).max()
+                    runs[name][f"max_position_diff_vs_{ref}_over_h"] = float(d / mesh.h)
     best = runs[head]
-    kern = best["kernels"]
-    mat = {k: v for k, v in kern.items() if "bytes" in v}
-    dom = max(mat, key=lambda k: mat[k]["total_ms"]) if mat else None
     prec = RUNS[head]["precision"]
+    factored = RUNS[head].get("level0") == "factored"
     out = {
         "metric": "MGPBD substeps/s at 1M tets to 1e-6 relative residual (FP64-verified): "
                   "mixed-precision V-cycle + learned warm start vs plain FP32 GPU MGPCG"
@@ -437,33 +479,46 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if prec == "fp32" else (
             f"{prec} V-cycle (f32 accumulation), f64 PCG products and dots (factored level 0)"
-            if args.level0 == "factored" else f"f32 outer PCG (f64 dots), {prec} V-cycle"),
+            if factored else f"f32 outer PCG (f64 dots), {prec} V-cycle"),
         "data": "synthetic",
         "config": {"workload": cfg["label"], "seed": args.seed, "drag": bool(cfg.get("drag")),
                    "n_tets": mesh.n_tets, "n_verts": mesh.n_verts,
-                   "nnz_A0": int(keep._nnz), "levels": keep._gpu_solver._level_sizes,
+                   "nnz_A0": keep[1], "levels": keep[0],
                    "pcg_rel_tol": 1e-6, "pcg_maxiter": cfg.get("maxiter_cg", 100),
-                   "E_range": cfg["E"], "reported": head,
+                   "E_range": cfg["E"], "nu": 0.45, "reported": head,
                    "runs": runs, "learned_training": learn_info,
-                   "speedup_vs_fp32_reference": (best["value"] / runs["fp32_reference"]["value"]
-                                                 if "fp32_reference" in runs else None),
-                   "speedup_vs_fp32_spectral": (best["value"] / runs["fp32"]["value"]
-                                                if "fp32" in runs else None)},
+                   "speedup_vs_fp32": (best["value"] / runs["fp32"]["value"]
+                                       if "fp32" in runs else None),
+                   "speedup_fp16_assembled_vs_fp32": (
+                       runs["fp16_assembled"]["value"] / runs["fp32"]["value"]
+                       if "fp16_assembled" in runs and "fp32" in runs else None),
+                   "speedup_fp16_volume_vs_fp32_volume": (
+                       runs["fp16_volume"]["value"] / runs["fp32_volume"]["value"]
+                       if "fp16_volume" in runs and "fp32_volume" in runs else None)},
     }
-    if dom:
-        k = mat[dom]
-        out["roofline"] = {"bound": "hbm", "kernel": f"{dom} ({head})", "achieved": k["GBs"],
-                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": k["GBs"] / HBM_PEAK_GBS,
-                           "algorithmic_bytes": k["bytes"],
-                           "traffic": measured_traffic(
-                               cfg["label"], dom,
-                               prec + ("-factored" if prec != "fp32" and
-                                       args.level0 == "factored" else ""))}
+    if probe is not None and factored:
+        # dominant kernel of the reported run: the factored level 0's vertex pass
+        # (k_fac_gather), timed alone with HIP events over back-to-back sweeps
+        n_rows = best["n_rows"]
+        by = factored_bytes(n_rows, mesh.n_verts, "gather")
+        us = probe["warm"]["first_us"]
+        gbs = by / (us * 1e-6) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": f"k_fac_gather ({head})", "achieved": gbs,
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                           "algorithmic_bytes": by, "avg_us": us,
+                           "traffic": measured_traffic(cfg["label"], "k_fac_gather"),
+                           "apply_pass": {"kernel": "k_fac_apply<1>",
+                                          "avg_us": probe["warm"]["second_us"],
+                                          "GBs": factored_bytes(n_rows, mesh.n_verts, "apply") /
+                                          (probe["warm"]["second_us"] * 1e-6) / 1e9},
+                           "cache_flushed": probe["flushed"]}
     if args.cpu_baseline:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         ref = runs.get("fp32_reference", best)
-        out["cpu_baseline"] = cpu_baseline(mesh, mu_t, keep, ref["newton_iters_per_substep"],
+        if omegas is None:
+            omegas = [0.2, 0.5]
+        out["cpu_baseline"] = cpu_baseline(mesh, mu_t, lam_t, cfg, omegas,
+                                           ref["newton_iters_per_substep"],
                                            ref["pcg_iters_per_substep"])
     print(json.dumps(out), flush=True)
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "mgpbd_mg_factored_scatter": (c_int, [_V, _V, _V, _V]),
     "mgpbd_mg_level_residual": (c_int, [_V, c_int, _V, _V, _V]),
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
+    "mgpbd_mg_probe_level0": (c_int, [_V, c_int, _V, _V, _V, _V, c_int64, F64P]),
     "mgpbd_mg_set_cycle": (c_int, [_V, c_int, c_float]),
     "mgpbd_mg_set_level_smoothing": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_smoother": (c_int, [_V, c_int, c_float, c_float, c_int, c_float]),
@@ -128,6 +129,7 @@ SIGNATURES = {
     # learned warm start
     "mgpbd_gnn_param_count": (c_int, []),
     "mgpbd_gnn_workspace_bytes": (c_int64, [c_int]),
+    "mgpbd_gnn_infer_workspace_bytes": (c_int64, [c_int]),
     "mgpbd_gnn_features": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_forward": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_infer": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V]),

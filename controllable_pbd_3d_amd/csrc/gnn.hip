@@ -274,16 +274,31 @@ k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
   }
 }
 
-// out[e] (+)= sum over partial blocks (fixed order)
+// out[e] (+)= sum over the nblk partial rows of column e. A block owns `cols` = min(len, 64)
+// consecutive columns and splits the partial rows among G = 256 / cols thread groups
+// (group g sums rows g, g + G, ...); the group totals are added in group order through LDS,
+// so the result is the same every run. Grid: ceil(len / 64) blocks.
 __global__ void __launch_bounds__(kBlock)
 k_sum_partials(int nblk, int len, const float* __restrict__ partial, float* __restrict__ out,
                int accumulate) {
-  const int e = blockIdx.x * kBlock + threadIdx.x;
-  if (e >= len) return;
+  __shared__ float sh[kBlock];
+  const int cols = min(len, 64);
+  const int G = kBlock / cols;
+  const int c = threadIdx.x % cols, g = threadIdx.x / cols;
+  const int e = blockIdx.x * 64 + c;
   float s = 0.0f;
-  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * len + e];
-  out[e] = accumulate ? out[e] + s : s;
+  if (g < G && e < len)
+    for (int b = g; b < nblk; b += G) s += partial[(size_t)b * len + e];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < cols && e < len) {
+    float t = 0.0f;
+    for (int q = 0; q < G; ++q) t += sh[q * cols + threadIdx.x];
+    out[e] = accumulate ? out[e] + t : t;
+  }
 }
+
+static inline int sum_partials_grid(int len) { return (len + 63) / 64; }
 
 // ------------------------------------------------------------------------------------------
 // Elementwise / small kernels
@@ -447,19 +462,28 @@ __global__ void __launch_bounds__(kBlock)
 k_colsum_partial(int n, int width, const uint16_t* __restrict__ Zbf, const float* __restrict__ Zf,
                  const float* __restrict__ rowscale, const uint16_t* __restrict__ Sbf,
                  float* __restrict__ partial) {
-  // out[c] = sum_i (Z[i][c] or rowscale_i * S[i][c]) over the block's 1024 rows
-  const int c = threadIdx.x;
-  if (c >= width) return;
+  // out[c] = sum_i (Z[i][c] or rowscale_i * S[i][c]) over the block's 1024 rows: G = 256 /
+  // width row groups (group g takes rows g, g + G, ...), group totals added in group order
+  __shared__ float sh[kBlock];
+  const int G = kBlock / width;
+  const int c = threadIdx.x % width, g = threadIdx.x / width;
   const int r0 = blockIdx.x * 1024, r1 = min(n, r0 + 1024);
   float s = 0.0f;
-  for (int i = r0; i < r1; ++i) {
-    float v;
-    if (Zbf) v = bf2f(Zbf[(size_t)i * width + c]);
-    else if (Zf) v = Zf[(size_t)i * width + c];
-    else v = rowscale[i] * bf2f(Sbf[(size_t)i * H + c]);
-    s += v;
+  if (g < G)
+    for (int i = r0 + g; i < r1; i += G) {
+      float v;
+      if (Zbf) v = bf2f(Zbf[(size_t)i * width + c]);
+      else if (Zf) v = Zf[(size_t)i * width + c];
+      else v = rowscale[i] * bf2f(Sbf[(size_t)i * H + c]);
+      s += v;
+    }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < width) {
+    float t = 0.0f;
+    for (int q = 0; q < G; ++q) t += sh[q * width + threadIdx.x];
+    partial[(size_t)blockIdx.x * width + threadIdx.x] = t;
   }
-  partial[(size_t)blockIdx.x * width + c] = s;
 }
 
 // encoder weight gradient partials: P[b][k][c] = sum_i x[i][k] dz0[i][c]
@@ -727,9 +751,14 @@ struct GnnWorkspace {
   double* scal;
 };
 
+// Workspace layouts. Both start with the FP64 block partials and scalars (the features use
+// only those), so a training workspace also serves the feature pass. Training keeps every
+// layer's activations for the backward pass; inference ping-pongs two activation buffers.
 static void carve(void* ws, int n, GnnWorkspace& W) {
   char* p = (char*)ws;
   auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return (void*)q; };
+  W.dpartial = (double*)take((size_t)std::max(nb(n), 1) * 8 * 8);
+  W.scal = (double*)take(64);
   const size_t act = (size_t)n * H * sizeof(uint16_t);
   for (int l = 0; l <= NL; ++l) W.h[l] = (uint16_t*)take(act);
   W.m[0] = nullptr;
@@ -741,8 +770,24 @@ static void carve(void* ws, int n, GnnWorkspace& W) {
   W.dmb = (uint16_t*)take(act);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
   W.partial = (float*)take((size_t)std::max(nblk, 1) * H * H * 4);
+}
+
+static void carve_infer(void* ws, int n, GnnWorkspace& W) {
+  char* p = (char*)ws;
+  auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return (void*)q; };
   W.dpartial = (double*)take((size_t)std::max(nb(n), 1) * 8 * 8);
   W.scal = (double*)take(64);
+  const size_t act = (size_t)n * H * sizeof(uint16_t);
+  W.h[0] = (uint16_t*)take(act);
+  W.h[1] = (uint16_t*)take(act);
+  for (int l = 2; l <= NL; ++l) W.h[l] = W.h[l & 1];
+  W.y = (float*)take((size_t)n * 4);
+}
+
+static size_t infer_workspace_bytes(int n) {
+  const size_t act = ((size_t)n * H * 2 + 255) / 256 * 256;
+  return (((size_t)std::max(nb(n), 1) * 64 + 255) / 256 * 256) + 256 + 2 * act +
+         ((size_t)n * 4 + 255) / 256 * 256;
 }
 
 static size_t workspace_bytes(int n) {
@@ -759,6 +804,9 @@ extern "C" {
 
 int mgpbd_gnn_param_count(void) { return n_params(); }
 int64_t mgpbd_gnn_workspace_bytes(int n) { return n < 0 ? -1 : (int64_t)workspace_bytes(n); }
+int64_t mgpbd_gnn_infer_workspace_bytes(int n) {
+  return n < 0 ? -1 : (int64_t)infer_workspace_bytes(n);
+}
 
 int mgpbd_gnn_features(mgpbd_ctx* ctx, int n, const float* b, const float* dinv,
                        const float* lam_prev, const float* C, const float* alpha,
@@ -806,7 +854,7 @@ int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_infer: invalid arguments");
   hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
   GnnWorkspace W;
-  carve(ws, n, W);
+  carve_infer(ws, n, W);   // layer l reads h[l & 1] and writes h[(l + 1) & 1]
   k_encoder8<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, x, params, W.h[0]);
   const int gg = std::min((n + 32 * kTileWaves - 1) / (32 * kTileWaves), kLayerBlocks);
   for (int l = 0; l < NL; ++l) {
@@ -893,9 +941,9 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   const int ncs = (n + 1023) / 1024;
   // decoder: d w_out = sum_i dy_i h3_i, d b_out = sum_i dy_i, dh3 = dy w_out^T
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, nullptr, nullptr, dy, W.h[NL], W.partial);
-  k_sum_partials<<<1, kBlock, 0, s>>>(ncs, H, W.partial, grad + off_wout(), 0);
+  k_sum_partials<<<sum_partials_grid(H), kBlock, 0, s>>>(ncs, H, W.partial, grad + off_wout(), 0);
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, 1, nullptr, dy, nullptr, nullptr, W.partial);
-  k_sum_partials<<<1, kBlock, 0, s>>>(ncs, 1, W.partial, grad + off_bout(), 0);
+  k_sum_partials<<<sum_partials_grid(1), kBlock, 0, s>>>(ncs, 1, W.partial, grad + off_bout(), 0);
   k_decoder_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, dy, params, W.dh);
   const int gg = (n + 127) / 128;
   for (int l = NL - 1; l >= 0; --l) {
@@ -903,11 +951,11 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
     k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[l + 1], W.dz);
     // weight and bias gradients
     k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.h[l], W.dz, W.partial);
-    k_sum_partials<<<nb(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l), 0);
+    k_sum_partials<<<sum_partials_grid(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l), 0);
     k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.m[l + 1], W.dz, W.partial);
-    k_sum_partials<<<nb(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l) + H * H, 0);
+    k_sum_partials<<<sum_partials_grid(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l) + H * H, 0);
     k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
-    k_sum_partials<<<1, kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bl(l), 0);
+    k_sum_partials<<<sum_partials_grid(H), kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bl(l), 0);
     // input gradients: dh_l = dZ W_self^T + mean-backward(dZ W_nbr^T)
     const uint16_t* w_self = w + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = w + (size_t)(2 * l + 1) * H * H;
@@ -919,9 +967,9 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   // encoder: dZ0 = dh0 * (h0 > 0); dW_in = x^T dZ0; db_in = sum dZ0
   k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[0], W.dz);
   k_encoder_wgrad<<<ncs, kBlock, 0, s>>>(n, x, W.dz, W.partial);
-  k_sum_partials<<<nb(F * H), kBlock, 0, s>>>(ncs, F * H, W.partial, grad + off_win(), 0);
+  k_sum_partials<<<sum_partials_grid(F * H), kBlock, 0, s>>>(ncs, F * H, W.partial, grad + off_win(), 0);
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
-  k_sum_partials<<<1, kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bin(), 0);
+  k_sum_partials<<<sum_partials_grid(H), kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bin(), 0);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_backward");
   return 0;
 }
@@ -967,7 +1015,7 @@ int mgpbd_gnn_wgrad64(mgpbd_ctx* ctx, int n, const uint16_t* A, const uint16_t* 
   hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
   k_wgrad<<<nblk, kBlock, 0, s>>>(n, A, dZ, partial);
-  k_sum_partials<<<nb(H * H), kBlock, 0, s>>>(nblk, H * H, partial, G, 0);
+  k_sum_partials<<<sum_partials_grid(H * H), kBlock, 0, s>>>(nblk, H * H, partial, G, 0);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_wgrad64");
   return 0;
 }

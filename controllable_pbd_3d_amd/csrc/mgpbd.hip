@@ -3940,6 +3940,72 @@ int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b, float* x_out) {
   return 0;
 }
 
+int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b, const float* x, float* y,
+                          float* flush, int64_t flush_floats, double* ms_out) {
+  CTX_CHECK("mgpbd_mg_probe_level0");
+  if (int e = check_ready(ctx)) return e;
+  if (reps < 1 || reps > 1000 || !b || !x || !y || x == y || !ms_out ||
+      (flush_floats > 0 && !flush) || flush_floats < 0)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_probe_level0: invalid arguments");
+  const MgLevel& L = ctx->L[0];
+  hipStream_t s = ctx->stream;
+  const bool fac = ctx->fac.on && half_mode(ctx);
+  // three events per rep, all queued before one wait: the sweeps run back to back as in the
+  // V-cycle (no host round trip between them)
+  std::vector<hipEvent_t> e(3 * (size_t)reps, nullptr);
+  struct Free {
+    std::vector<hipEvent_t>& e;
+    ~Free() { for (hipEvent_t v : e) if (v) (void)hipEventDestroy(v); }
+  } fr{e};
+  for (auto& ev : e) MGPBD_HIP_TRY(hipEventCreate(&ev));
+  for (int r = 0; r < reps; ++r) {
+    if (flush_floats > 0) {   // a pass over a buffer larger than the caches before each rep
+      for (int64_t off = 0; off < flush_floats; off += (int64_t)1 << 30) {
+        const int m = (int)std::min<int64_t>((int64_t)1 << 30, flush_floats - off);
+        k_blas1<<<nblocks(m), kBlock, 0, s>>>(4, m, 1.0f, nullptr, flush + off);
+      }
+    }
+    MGPBD_HIP_TRY(hipEventRecord(e[3 * r], s));
+    if (fac) {
+      const mgpbd_ctx::Factored& F = ctx->fac;
+      if (ctx->precision == MGPBD_PREC_FP16)
+        k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
+            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k, x, F.u, nullptr);
+      else
+        k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
+            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k, x, F.u, nullptr);
+      MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 1], s));
+      if (ctx->precision == MGPBD_PREC_FP16)
+        k_fac_apply<1, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
+                                                         ctx->omega, nullptr, L.isd, nullptr,
+                                                         nullptr, nullptr);
+      else
+        k_fac_apply<2, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
+                                                         ctx->omega, nullptr, L.isd, nullptr,
+                                                         nullptr, nullptr);
+    } else {
+      if (half_mode(ctx)) sellh(ctx, L, 1, x, y, b, nullptr, nullptr);
+      else sell_launch<1>(ctx, L, x, y, L.dinv, b, ctx->omega, nullptr, nullptr);
+      MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 1], s));
+    }
+    MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 2], s));
+    MGPBD_LAUNCH_CHECK("mgpbd_mg_probe_level0");
+  }
+  MGPBD_HIP_TRY(hipEventSynchronize(e[3 * (size_t)reps - 1]));
+  double t0 = 0.0, t1 = 0.0, tp = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    float a = 0.f, c = 0.f, d = 0.f;
+    MGPBD_HIP_TRY(hipEventElapsedTime(&a, e[3 * r], e[3 * r + 1]));
+    MGPBD_HIP_TRY(hipEventElapsedTime(&c, e[3 * r + 1], e[3 * r + 2]));
+    MGPBD_HIP_TRY(hipEventElapsedTime(&d, e[3 * r], e[3 * r + 2]));
+    t0 += a; t1 += c; tp += d;
+  }
+  ms_out[0] = t0 / reps;
+  ms_out[1] = t1 / reps;
+  ms_out[2] = tp / reps;
+  return 0;
+}
+
 int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b, const float* x, float* r) {
   CTX_CHECK("mgpbd_mg_level_residual");
   if (int e = check_ready(ctx)) return e;

@@ -162,6 +162,7 @@ class MGPBDSolverGPU:
         self.overlap_inference = bool(overlap_inference)
         self._side = None          # (stream, context, events) of the overlapped inference
         self._predicted = False
+        self.last_predicted = False   # the network ran for this substep's first solve
         self.verbose = verbose
         self.mu_per_tet = None if mu_per_tet is None else to_device(mu_per_tet, torch.float32, dev, (-1,))
         self.volume = volume_lambda is not None
@@ -428,6 +429,7 @@ class MGPBDSolverGPU:
                                       self._dlam_gpu, ctx=ctx)
         ev_out.record(stream)
         self._predicted = True
+        self.last_predicted = True
 
     def _scatter_dlam_gpu(self):
         mg = self._gpu_solver
@@ -485,6 +487,7 @@ class MGPBDSolverGPU:
         """MGPBD outer loop (cons/mgpbd_gpu.py:397-446)."""
         if self.benchmark is not None:
             self.benchmark.begin_frame(frame)
+        self.last_predicted = False
         self.lambdaf.zero_()
         r0 = 0.0
         stats = []
@@ -541,5 +544,6 @@ class MGPBDSolverGPU:
             self.warm_start_model.predict(self._b_gpu, self._A.diag_inv, self._lam_prev,
                                           self.constraints, self.alpha_tilde, self._vol_rows,
                                           self._dlam_gpu)
+            self.last_predicted = True
         self._predicted = False
         return self._gpu_solver.solve_guess(self._b_gpu, self._dlam_gpu, self._lam_prev)

@@ -93,14 +93,47 @@ class LearnedWarmStart:
         self.wt = torch.zeros(nw, dtype=torch.int16, device=dev)
         self.w = torch.zeros(nw, dtype=torch.int16, device=dev)
         self.ctx("mgpbd_gnn_pack", dptr(self.params), dptr(self.wt), dptr(self.w))
-        ws = int(lib.mgpbd_gnn_workspace_bytes(self.n))
-        self.ws = torch.zeros(ws, dtype=torch.uint8, device=dev)
+        # features and inference: two ping-ponged activation buffers; the training workspace
+        # (every layer's activations, gradients, partials) is allocated on first training use
+        # and dropped by release_training_memory()
+        wi = int(lib.mgpbd_gnn_infer_workspace_bytes(self.n))
+        self.ws_infer = torch.zeros(wi, dtype=torch.uint8, device=dev)
+        self._train = None
         self.x = torch.zeros(self.n, FEATURES, **f32)
         self.stats = torch.zeros(8, **f32)
-        self.dy = torch.zeros(self.n, **f32)
-        self.dlam = torch.zeros(self.n, **f32)
-        self.tmp = torch.zeros(2 * self.n, **f32)
         self.loss = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    @property
+    def ws(self):
+        """The training workspace (allocated on first use)."""
+        if self._train is None:
+            f32 = dict(dtype=torch.float32, device=self.device)
+            nbytes = int(self.lib.mgpbd_gnn_workspace_bytes(self.n))
+            self._train = {"ws": torch.zeros(nbytes, dtype=torch.uint8, device=self.device),
+                           "dy": torch.zeros(self.n, **f32), "dlam": torch.zeros(self.n, **f32),
+                           "tmp": torch.zeros(2 * self.n, **f32)}
+        return self._train["ws"]
+
+    def _train_buf(self, key):
+        self.ws   # allocates the training buffers on first use
+        return self._train[key]
+
+    @property
+    def dy(self):
+        return self._train_buf("dy")
+
+    @property
+    def dlam(self):
+        return self._train_buf("dlam")
+
+    @property
+    def tmp(self):
+        return self._train_buf("tmp")
+
+    def release_training_memory(self) -> None:
+        """Frees the training workspace (activations of every layer, gradients, weight-gradient
+        partials); inference keeps its own small one. Training again reallocates it."""
+        self._train = None
 
     @classmethod
     def for_mesh(cls, t_i, n_tets: int, per_tet: int = 1, **kw) -> "LearnedWarmStart":
@@ -114,7 +147,7 @@ class LearnedWarmStart:
         stats = self.stats if stats is None else stats
         ctx = self.ctx if ctx is None else ctx
         ctx("mgpbd_gnn_features", self.n, dptr(b), dptr(dinv), dptr(lam_prev), dptr(C),
-            dptr(alpha), dptr(vol), dptr(self.nbr), dptr(x), dptr(stats), dptr(self.ws))
+            dptr(alpha), dptr(vol), dptr(self.nbr), dptr(x), dptr(stats), dptr(self.ws_infer))
         return x, stats
 
     def forward(self, x, stats, lam_prev, out=None):
@@ -128,7 +161,7 @@ class LearnedWarmStart:
         (e.g. bound to a second stream), default the model's."""
         ctx = self.ctx if ctx is None else ctx
         ctx("mgpbd_gnn_infer", self.n, dptr(self.nbr), dptr(x), dptr(stats),
-            dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws), dptr(out))
+            dptr(lam_prev), dptr(self.params), dptr(self.wt), dptr(self.ws_infer), dptr(out))
         return out
 
     def predict(self, b, dinv, lam_prev, C, alpha, vol, out, ctx=None):

@@ -484,6 +484,15 @@ int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b_dev, const
                             float* r_dev);
 /* x = M^-1 b with one V-cycle from x = 0. Replaces vcycle(0) (cons/mgpcg_gpu.py:214-259). */
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
+/* Measurement (no reference counterpart): `reps` (<= 1000) level-0 weighted-Jacobi sweeps
+ * y = x + omega D^-1 (b - A x) as the V-cycle launches them, back to back on the context
+ * stream with HIP events around each kernel (one wait after the last). ms_out[0]: average
+ * duration of the first kernel (the
+ * factored level's vertex pass, or the whole sweep of a stored level), ms_out[1]: of the
+ * second (the factored tet pass, 0 otherwise), ms_out[2]: of the sweep. With flush_floats > 0
+ * every rep first rewrites the flush buffer (scale by 1), evicting the caches. */
+int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b_dev, const float* x_dev,
+                          float* y_dev, float* flush_dev, int64_t flush_floats, double* ms_out);
 /* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:
  * 261-329: stop when ||r||^2 < tol^2, breakdown on |pAp| < 1e-30 and on the previous
  * |rz| < 1e-30). relative != 0 (extension) stops on ||r|| < tol ||b||. work_dev: 4*n floats
@@ -586,6 +595,10 @@ int mgpbd_zone_mask_alpha(mgpbd_ctx* ctx, int n_tets, const int32_t* active_dev,
 #define MGPBD_GNN_LAYERS 3
 int mgpbd_gnn_param_count(void);
 int64_t mgpbd_gnn_workspace_bytes(int n);
+/* Bytes of the smaller workspace that mgpbd_gnn_features and mgpbd_gnn_infer need (two
+ * ping-ponged activation buffers instead of every layer's): a model that is no longer trained
+ * keeps only this one. A training workspace serves them as well. */
+int64_t mgpbd_gnn_infer_workspace_bytes(int n);
 /* Per-tet features x_dev (n x 8): b/A_ii, lam_prev, C, log alpha, log A_ii, rest volume,
  * degree, 1 -- normalised by FP64 statistics written to stats_dev (8 floats: the scale s of
  * b/A_ii and lam_prev, the scale of C, mean/std of log alpha, mean/std of log A_ii, mean

@@ -42,6 +42,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #define ORC_CHUNK 8192
 #define ORC_PAR_MIN 4096   /* shorter loops stay on one thread */
 
@@ -54,6 +58,23 @@ typedef float real;
 #define FMA(a, b, c) fmaf((a), (b), (c))
 #define RABS(a) fabsf(a)
 #endif
+
+/* Threads of the OpenMP loops (the CPU baseline times 1 thread and all cores). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int orc_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 /* ---------------------------------------------------------------- rest shape (FP64) */
 /* Dm^-1 of tet t into Dm_inv, returns vol = |det Dm| / 6 (FP64). */

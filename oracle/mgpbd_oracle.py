@@ -68,6 +68,8 @@ def lib(precision: str = "f32"):
         L.orc_scatter.argtypes = [ctypes.c_int, ctypes.c_int, I32P, R, R, R, R, R]
         L.orc_pattern.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, I32P]
         L.orc_pattern.restype = ctypes.c_int64
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         _LIBS[precision] = (L, R, np.float32 if precision == "f32" else np.float64)
     return _LIBS[precision]
 
