@@ -21,7 +21,9 @@ Runs (each its own solver on the same mesh, same loading):
   fp32_volume, fp16_volume   fp32 / fp16 with the volume constraint beside ARAP (lambda from
                   nu = 0.45: 2 constraints per tet)
 A step is one substep (predict, Newton/MGPCG solve, velocity update) from the state the
-previous step left; inputs are resident in HBM.
+previous step left; inputs are resident in HBM. Substeps run in the library (mgpbd_substep, the
+C++ Newton loop); `--python-loop` runs the Python Newton loop instead (same kernels, same
+results bit for bit).
 
 Prints one JSON line. `--gpus N` with N > 1 exits: this build measures one GPU.
 """
@@ -45,6 +47,7 @@ from controllable_pbd_3d_amd.workloads import (CONFIGS, Loading, build_workload,
                                                solver_args)
 
 HBM_PEAK_GBS = 8000.0
+NATIVE_SUBSTEP = True      # --python-loop sets False
 
 
 def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
@@ -156,7 +159,7 @@ def make_solver(mesh, mu_t, lam_t, cfg, kw, model=None, record=None):
                        maxiter_cg=cfg.get("maxiter_cg", 100), hierarchy="extended",
                        galerkin="structural", max_coarse=64, max_levels=cfg["max_levels"],
                        coarse_solver="cholesky", warm_start_model=model, record_samples=record,
-                       verbose=False)
+                       native_substep=NATIVE_SUBSTEP, verbose=False)
     args.update(kw)
     s = MGPBDSolverGPU(**args)
     s.init()
@@ -420,7 +423,11 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--cpu-baseline", dest="cpu_baseline", action="store_true", default=True)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--python-loop", action="store_true",
+                    help="run the Newton loop in Python instead of mgpbd_substep")
     args = ap.parse_args()
+    global NATIVE_SUBSTEP
+    NATIVE_SUBSTEP = not args.python_loop
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no "
               "multi-GPU path", file=sys.stderr)
@@ -486,6 +493,7 @@ def main():
                    "nnz_A0": keep[1], "levels": keep[0],
                    "pcg_rel_tol": 1e-6, "pcg_maxiter": cfg.get("maxiter_cg", 100),
                    "E_range": cfg["E"], "nu": 0.45, "reported": head,
+                   "newton_loop": "mgpbd_substep (C++)" if NATIVE_SUBSTEP else "Python",
                    "runs": runs, "learned_training": learn_info,
                    "speedup_vs_fp32": (best["value"] / runs["fp32"]["value"]
                                        if "fp32" in runs else None),

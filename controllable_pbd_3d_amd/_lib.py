@@ -142,7 +142,38 @@ SIGNATURES = {
     "mgpbd_gnn_pack": (c_int, [_V, _V, _V, _V]),
     "mgpbd_gnn_gemm64": (c_int, [_V, c_int, _V, _V, _V, _V, _V, c_int, _V]),
     "mgpbd_gnn_wgrad64": (c_int, [_V, c_int, _V, _V, _V, _V]),
+    # native substep
+    "mgpbd_mg_set_refill": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, c_int, _V, _V]),
+    "mgpbd_mg_refresh": (c_int, [_V, c_int, c_int]),
+    "mgpbd_substep_struct_sizes": (c_int, [I32P]),
+    "mgpbd_substep_create": (_V, [_V]),
+    "mgpbd_substep_destroy": (None, [_V]),
+    "mgpbd_substep": (c_int, [_V, _V, _V, ctypes.POINTER(ctypes.c_int32),
+                              ctypes.POINTER(ctypes.c_int32)]),
 }
+
+
+class SubstepDesc(ctypes.Structure):
+    """mgpbd_substep_desc (include/mgpbd.h): device pointers as addresses."""
+    _fields_ = [(n, ctypes.c_int32) for n in ("n_verts", "n_tets", "per_tet", "n_cons")] + \
+        [(n, _V) for n in ("v_p", "v_v", "v_p_old", "v_invm", "f_ext", "t_i", "t_c", "Dm_inv",
+                           "alpha", "lam", "C", "gradC", "rhs", "dlam", "lam_prev", "norms",
+                           "x_save", "dpos", "slice_ptr", "sell_idx", "row_len", "fill_desc",
+                           "A_val", "A_dinv", "vptr", "vinc", "pcg_work", "nbr", "feat",
+                           "feat_stats", "params", "wt", "ws_infer", "vol_rows")] + \
+        [(n, c_double) for n in ("atol", "rtol", "tol_cg")] + \
+        [(n, c_float) for n in ("gx", "gy", "gz", "dt", "damp")] + \
+        [("drag_target", c_float * 3), ("drag_k", c_float)] + \
+        [(n, ctypes.c_int32) for n in ("drag_vertex", "hold_pinned", "maxiter", "maxiter_cg",
+                                       "relative_tol", "use_line_search",
+                                       "spectral_every_newton", "warm_start",
+                                       "overlap_inference", "factored")]
+
+
+class NewtonStat(ctypes.Structure):
+    """mgpbd_newton_stat (include/mgpbd.h)."""
+    _fields_ = [("ite", ctypes.c_int32), ("cg_iters", ctypes.c_int32), ("r", c_double),
+                ("omega", c_double), ("info", c_double * 8)]
 
 _lib = None
 

@@ -12,7 +12,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmgpbd.so")
 
-HIP_SOURCES = ["mgpbd.hip", "gnn.hip", "adaptive.hip"]
+HIP_SOURCES = ["mgpbd.hip", "gnn.hip", "adaptive.hip", "substep.hip"]
 CXX_SOURCES = ["setup.cpp", "common.cpp"]
 # packed FP32 instructions are disabled for every device object: some of their forms have
 # returned wrong results on this hardware without any error, so device code uses scalar FP32

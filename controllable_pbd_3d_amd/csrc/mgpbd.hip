@@ -2560,6 +2560,18 @@ struct mgpbd_ctx {
   double* pw_dev = nullptr;     // 4 per level: lambda estimate, ||w||_D, previous estimate
   bool pv_ready[MGPBD_MAX_LEVELS] = {};
   int pv_updates[MGPBD_MAX_LEVELS] = {};   // updates since the level was set
+  // coarse refill maps, level l -> l + 1 (mgpbd_mg_set_refill), for mgpbd_mg_refresh
+  struct Refill {
+    int n = -1;              // coarse entries refilled (-1: not registered)
+    const int* off = nullptr;
+    const int* fpos = nullptr;
+    const int* cpos = nullptr;
+    const float* fine = nullptr;
+    float* coarse = nullptr;
+    int n_coarse = 0;
+    const int* cdiag = nullptr;
+    float* cdinv = nullptr;
+  } refill[MGPBD_MAX_LEVELS];
   double* pw_host = nullptr;    // pinned copy of pw_dev from the previous update (read lagged)
   MgLevel L[MGPBD_MAX_LEVELS];
   // factored level 0 of the reduced-precision V-cycle (A^ = K K^T + diag(a^), K from the
@@ -3204,6 +3216,48 @@ static int ns_settle(mgpbd_ctx* ctx) {
   ctx->ns_pending = false;
   if (*ctx->ns_flag_host == 1) ctx->ns_steps += 2;
   else ctx->ns_rejected = true;
+  return 0;
+}
+
+static int convert_half_levels(mgpbd_ctx* ctx, int first, int last);
+
+int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t* off,
+                        const int32_t* fpos, const int32_t* cpos, const float* fine_val,
+                        float* coarse_val, int n_coarse, const int32_t* coarse_diag_pos,
+                        float* coarse_dinv) {
+  CTX_CHECK("mgpbd_mg_set_refill");
+  if (level < 0 || level >= MGPBD_MAX_LEVELS - 1 || n_entries < 0 || n_coarse < 0 ||
+      (n_entries > 0 && (!off || !fpos || !cpos || !fine_val || !coarse_val)) ||
+      (n_coarse > 0 && (!coarse_diag_pos || !coarse_dinv || !coarse_val)))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_refill: invalid arguments");
+  mgpbd_ctx::Refill& R = ctx->refill[level];
+  R.n = n_entries; R.off = off; R.fpos = fpos; R.cpos = cpos; R.fine = fine_val;
+  R.coarse = coarse_val; R.n_coarse = n_coarse; R.cdiag = coarse_diag_pos; R.cdinv = coarse_dinv;
+  return 0;
+}
+
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh_smoother, int half_from) {
+  CTX_CHECK("mgpbd_mg_refresh");
+  if (int e = check_levels(ctx)) return e;
+  if (half_from < 0 || half_from > ctx->n_levels)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: 0 <= half_from <= n_levels");
+  for (int l = 0; l + 1 < ctx->n_levels; ++l) {
+    const mgpbd_ctx::Refill& R = ctx->refill[l];
+    if (R.n < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: refill maps of level " +
+                                                     std::to_string(l) + " not registered");
+    if (l == 0 && ctx->fac.on) {
+      if (int e = mgpbd_mg_factored_refill(ctx)) return e;
+    } else if (R.n > 0) {
+      if (int e = mgpbd_coarse_refill(ctx, R.n, R.off, R.fpos, R.cpos, R.fine, R.coarse))
+        return e;
+    }
+    if (int e = mgpbd_diag_inv(ctx, R.n_coarse, R.cdiag, R.coarse, R.cdinv)) return e;
+  }
+  if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels > 1)
+    if (int e = mgpbd_mg_factor_coarsest(ctx)) return e;
+  if (int e = convert_half_levels(ctx, half_from, ctx->n_levels)) return e;
+  if (refresh_smoother && ctx->spectral && !ctx->fixed_weights)
+    if (int e = mgpbd_mg_update_smoother(ctx, nullptr)) return e;
   return 0;
 }
 

@@ -45,6 +45,11 @@ on a second stream while the coarse levels refill (bit-identical to running it a
 used as J W J^T + alpha~ in factored form from the tets' gradients (V-cycle sweeps from FP16/BF16
 K K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
 aggregates' summed gradients) and is not reassembled per Newton iteration.
+``native_substep`` (default False): ``step`` runs prediction, the Newton loop and the velocity
+update in the library (``mgpbd_substep``: the same kernels in the same order, positions
+bit-identical to the Python loop, the host reading one pair of norms per Newton iteration);
+the Python loop runs instead while a ``benchmark`` or ``record_samples`` is attached (their
+per-iteration hooks are Python).
 ``volume_lambda`` (None: the reference's single ARAP constraint per tet; a float or a per-tet
 array: the Lame lambda of cons/vbd.py:44-47): every tet also carries the volume constraint
 C_H = det F - 1 (the hydrostatic constraint of cons/deform3d.py:81-91) with compliance
@@ -56,6 +61,7 @@ four vertices) and both rows of a tet share its aggregate.
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -100,8 +106,8 @@ class MGPBDSolverGPU:
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
                  spectral_refresh: str = "newton", overlap_inference: bool = True,
-                 volume_lambda=None, omega_levels=None, device=None,
-                 verbose: bool = True) -> None:
+                 volume_lambda=None, omega_levels=None, native_substep: bool = False,
+                 device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
         if spectral_refresh not in ("newton", "substep"):
@@ -160,6 +166,8 @@ class MGPBDSolverGPU:
         self.level0 = level0
         self.spectral_refresh = spectral_refresh
         self.overlap_inference = bool(overlap_inference)
+        self.native_substep = bool(native_substep)
+        self._native = None        # (state handle, descriptor, stats buffer) of mgpbd_substep
         self._side = None          # (stream, context, events) of the overlapped inference
         self._predicted = False
         self.last_predicted = False   # the network ran for this substep's first solve
@@ -329,9 +337,99 @@ class MGPBDSolverGPU:
     # ------------------------------------------------------------------ time stepping
     def step(self, frame: int = 0):
         """predict -> solve -> velocity update (cons/mgpbd_gpu.py:200-204)."""
+        if self.native_substep and self.benchmark is None and self.record_samples is None:
+            self._native_step()
+            return
         self.make_prediction()
         self.solve(frame)
         self.update_vel()
+
+    def _native_descriptor(self):
+        """mgpbd_substep_desc over this solver's tensors (fixed after init)."""
+        mg = self._gpu_solver
+        A = self._A
+        d = _lib.SubstepDesc()
+        d.n_verts, d.n_tets, d.per_tet, d.n_cons = (self.n_verts, self.n_tets, self.per_tet,
+                                                    self.n_cons)
+        for field, t in (("v_p", self.v_p), ("v_v", self.v_v), ("v_p_old", self.v_p_old),
+                         ("v_invm", self.v_invm), ("t_i", self.t_i), ("t_c", self.t_c),
+                         ("Dm_inv", self.Dm_inv), ("alpha", self.alpha_tilde),
+                         ("lam", self.lambdaf), ("C", self.constraints), ("gradC", self.gradC),
+                         ("rhs", self._b_gpu), ("dlam", self._dlam_gpu),
+                         ("lam_prev", self._lam_prev), ("norms", self._norms),
+                         ("x_save", self._x_save), ("dpos", self.dpos),
+                         ("slice_ptr", A.slice_ptr), ("sell_idx", A.sell_idx),
+                         ("row_len", A.row_len), ("fill_desc", self._A_desc), ("A_val", A.data),
+                         ("A_dinv", A.diag_inv), ("vptr", self._vptr), ("vinc", self._vinc),
+                         ("pcg_work", mg._pcg_work), ("vol_rows", self._vol_rows)):
+            setattr(d, field, dptr(t))
+        m = self.warm_start_model
+        if self.warm_start == "learned":
+            for field, t in (("nbr", m.nbr), ("feat", m.x), ("feat_stats", m.stats),
+                             ("params", m.params), ("wt", m.wt), ("ws_infer", m.ws_infer)):
+                setattr(d, field, dptr(t))
+        d.atol, d.rtol, d.tol_cg = self.atol, self.rtol, mg.tol_cg
+        d.gx, d.gy, d.gz = self.gravity
+        d.dt = self.dt
+        d.hold_pinned, d.maxiter, d.maxiter_cg = int(self.hold_pinned), self.maxiter, mg.maxiter_cg
+        d.relative_tol, d.use_line_search = int(mg.relative), int(self.use_line_search)
+        d.spectral_every_newton = int(self.spectral_refresh == "newton")
+        d.warm_start = {"none": 0, "previous": 1, "learned": 2}[self.warm_start]
+        d.overlap_inference = int(self.overlap_inference)
+        d.factored = int(mg.factored)
+        return d
+
+    def _native_step(self):
+        """One substep through mgpbd_substep; fills last_stats, last_solve_info,
+        first_solve_info and last_predicted as the Python loop does."""
+        lib = _lib.load()
+        mg = self._gpu_solver
+        if self._native is None:
+            h = lib.mgpbd_substep_create(mg.ctx.handle)
+            if not h:
+                check(-1, "mgpbd_substep_create")
+            self._native = (h, self._native_descriptor(),
+                            (_lib.NewtonStat * max(self.maxiter, 1))())
+        h, d, stats = self._native
+        d.damp = self.damp[None]
+        d.f_ext = dptr(self.f_ext)
+        if self._drag is not None:
+            v, tx, ty, tz, k = self._drag
+            d.drag_vertex, d.drag_k = v, k
+            d.drag_target[0], d.drag_target[1], d.drag_target[2] = tx, ty, tz
+        else:
+            d.drag_vertex = -1
+        mg.ctx.bind_stream()
+        n, pred = ctypes.c_int32(0), ctypes.c_int32(0)
+        check(lib.mgpbd_substep(h, ctypes.byref(d), stats, ctypes.byref(n), ctypes.byref(pred)),
+              "mgpbd_substep")
+        guess = self.warm_start != "none"
+        self.last_stats, self.last_solve_info = [], []
+        for k in range(n.value):
+            s = stats[k]
+            self.last_stats.append((s.ite, s.r, s.cg_iters, s.omega))
+            i = s.info
+            if i[1] < 0:            # the iteration that met the stop test: no solve
+                continue
+            info = {"rr": i[0], "bb": i[1], "reason": int(i[2])}
+            if guess and s.ite == 0:
+                info.update(rr0=i[3], start=int(i[4]), rr_guess=i[5], rr_alt=i[6])
+            info["rr_true"] = i[7]
+            mg.last_info = dict(info)
+            info["cg_iters"] = s.cg_iters
+            self.last_solve_info.append(info)
+        if self.last_solve_info:   # else kept, as the Python loop keeps it
+            self.first_solve_info = self.last_solve_info[0]
+        self.last_predicted = bool(pred.value)
+
+    def __del__(self):
+        nat = getattr(self, "_native", None)
+        if nat is not None:
+            try:
+                _lib.load().mgpbd_substep_destroy(nat[0])
+            except Exception:
+                pass
+            self._native = None
 
     def device_bytes(self) -> dict:
         """HBM footprint in bytes: per-tet/per-vertex state and the level-0 pattern

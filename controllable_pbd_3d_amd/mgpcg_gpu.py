@@ -488,6 +488,11 @@ class GPUMultigridSolver:
                 k = self.smooth_schedule[min(l, len(self.smooth_schedule) - 1)]
                 ctx("mgpbd_mg_set_level_smoothing", l, k)
         self._coarse_mode = mode
+        for l, rf in enumerate(self.refill):   # for mgpbd_mg_refresh (the native substep)
+            F, C = self.levels[l], self.levels[l + 1]
+            ctx("mgpbd_mg_set_refill", l, rf["n"], dptr(rf["off"]), dptr(rf["fpos"]),
+                dptr(rf["cpos"]), dptr(F.data), dptr(C.data), C.n_rows, dptr(C.diag_pos),
+                dptr(C.diag_inv))
         for l, L in enumerate(self.levels):
             t = self.transfers[l] if l < len(self.transfers) else None
             ctx("mgpbd_mg_set_level", l, L.n_rows, dptr(L.slice_ptr), dptr(L.sell_idx),

@@ -339,6 +339,21 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
  * not positive definite); a Newton-Schulz refresh does not. No reference counterpart (the
  * reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
+/* Registers the coarse refill of level `level` + 1 from level `level` (the arguments of
+ * mgpbd_coarse_refill) with the coarse level's diagonal positions and 1/diag buffer, for
+ * mgpbd_mg_refresh. n_entries = 0 registers a level whose values come from elsewhere (the
+ * factored level 1). Device pointers stay owned by the caller. */
+int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t* off_dev,
+                        const int32_t* fpos_dev, const int32_t* cpos_dev,
+                        const float* fine_val_dev, float* coarse_val_dev, int n_coarse,
+                        const int32_t* coarse_diag_pos_dev, float* coarse_dinv_dev);
+/* One call for update_all_coarse_matrices (cons/mgpcg_gpu.py:183-212) after level 0's values
+ * changed: per level the coarse refill (mgpbd_mg_factored_refill for a factored level 1)
+ * and 1/diag, the coarsest inverse with MGPBD_COARSE_CHOLESKY, the reduced-precision
+ * copies of levels half_from .. n_levels-1 (a caller that converted level 0 early passes 1)
+ * and, with refresh_smoother != 0, the spectral smoother's estimates. Queues only (a
+ * Cholesky refactorization synchronises, as in mgpbd_mg_factor_coarsest). */
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh_smoother, int half_from);
 /* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level
  * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse (rocBLAS dgemm)
  * and the device keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue
@@ -657,6 +672,70 @@ int mgpbd_gnn_gemm64(mgpbd_ctx* ctx, int n, const uint16_t* A1_dev, const uint16
                      int relu, float* Y_dev);
 int mgpbd_gnn_wgrad64(mgpbd_ctx* ctx, int n, const uint16_t* A_dev, const uint16_t* dZ_dev,
                       float* partial_dev, float* G_dev);
+
+/* ===================================================================================== */
+/* Native substep (cons/mgpbd_gpu.py:200-219 and 397-446)                                */
+/* ===================================================================================== */
+
+/* Everything one substep reads and writes. Device arrays unless noted; the solver object
+ * owns them (the Python MGPBDSolverGPU fills this from its tensors). Sizes: v_p, v_v,
+ * v_p_old, f_ext, x_save, dpos 3 n_verts; v_invm n_verts; t_i 4 n_tets; t_c 4 n_cons;
+ * Dm_inv 9 n_tets; alpha, lambda, C, rhs, dlam, lam_prev, vol_rows n_cons; gradC 12 n_cons;
+ * norms 4 doubles. */
+typedef struct mgpbd_substep_desc {
+  int32_t n_verts, n_tets, per_tet, n_cons;    /* per_tet 2: ARAP + volume constraint */
+  float* v_p; float* v_v; float* v_p_old; const float* v_invm; float* f_ext; /* f_ext: NULL ok */
+  const int32_t* t_i; const int32_t* t_c; const float* Dm_inv; const float* alpha;
+  float* lambda; float* C; float* gradC; float* rhs; float* dlam; float* lam_prev;
+  double* norms; float* x_save; float* dpos;
+  /* assembled level 0 (mgpbd_fill_A_shared), its 1/diag and the scatter's incidence */
+  const int32_t* slice_ptr; const int32_t* sell_idx; const int32_t* row_len;
+  const uint16_t* fill_desc; float* A_val; float* A_dinv; const int32_t* vptr;
+  const int32_t* vinc;
+  float* pcg_work;                             /* mgpbd_pcg_solve's work buffer */
+  /* learned warm start (mgpbd_gnn_features + mgpbd_gnn_infer); NULL without one */
+  const int32_t* nbr; float* feat; float* feat_stats; const float* params; const uint16_t* wt;
+  void* ws_infer; const float* vol_rows;
+  double atol, rtol, tol_cg;                   /* Newton stop and PCG tolerance */
+  float gx, gy, gz, dt, damp;
+  float drag_target[3]; float drag_k;
+  int32_t drag_vertex;                         /* -1: no spring drag */
+  int32_t hold_pinned, maxiter, maxiter_cg, relative_tol, use_line_search;
+  int32_t spectral_every_newton;               /* 0: spectral refresh at iteration 0 only */
+  int32_t warm_start;                          /* 0 none, 1 previous lambda, 2 learned */
+  int32_t overlap_inference;                   /* learned + factored: network on a side stream */
+  int32_t factored;                            /* level 0 factored (mgpbd_mg_set_level0_factored) */
+} mgpbd_substep_desc;
+
+/* One Newton iteration's record: the reference's (ite, r, cg_iters, omega) plus the PCG's
+ * info {||r||^2, ||b||^2, stop reason, ||r0||^2 of the start, start, ||b - A x_guess||^2,
+ * ||b - A x_alt||^2, FP64 ||b - A x||^2} (entries 3-6 are -1 for a solve from zero). The
+ * iteration that meets the stop test has cg_iters = 0, omega = 1 and no solve (info all -1). */
+typedef struct mgpbd_newton_stat {
+  int32_t ite, cg_iters;
+  double r, omega;
+  double info[8];
+} mgpbd_newton_stat;
+
+typedef struct mgpbd_substep_state mgpbd_substep_state;
+/* Host state of the native substep on one multigrid context: a pinned read-back buffer and,
+ * once the learned start overlaps, a side stream (default priority) with its own context. */
+mgpbd_substep_state* mgpbd_substep_create(mgpbd_ctx* mg);
+void mgpbd_substep_destroy(mgpbd_substep_state* st);
+/* One substep in C++: spring drag (drag_vertex >= 0), prediction, the Newton loop of solve()
+ * (cons/mgpbd_gpu.py:397-446: lambda = 0; constraint pass; stop when r < atol or
+ * r < rtol r0; A fill or the factored level 0; mgpbd_mg_refresh; PCG, from zero or warm
+ * started; scatter; the backtracking line search of :379-391 with its full step evaluated by
+ * the constraint pass) and the velocity update, on mg's stream, with the same kernels in the
+ * same order as the Python loop (bit-identical positions). The host reads one pair of norms
+ * per Newton iteration, plus the PCG's polls. stats (host, >= maxiter entries): one record per
+ * Newton iteration, *n_stats their count; *predicted = 1 when the network ran.
+ * MGPBD_ENONFINITE when ||b||^2 is not finite. */
+int mgpbd_substep(mgpbd_substep_state* st, const mgpbd_substep_desc* d, mgpbd_newton_stat* stats,
+                  int32_t* n_stats, int32_t* predicted);
+/* sizeof(mgpbd_substep_desc) and sizeof(mgpbd_newton_stat), for bindings that restate the
+ * structures (ctypes). */
+int mgpbd_substep_struct_sizes(int32_t* sizes_out);
 
 #ifdef __cplusplus
 }

@@ -24,18 +24,20 @@ def _solver(**kw):
     return mesh, s
 
 
+@pytest.mark.parametrize("native", [False, True])
 @pytest.mark.parametrize("mode", ["reference", "extended_fp16"])
-def test_nan_position_reports_nonfinite(mode):
+def test_nan_position_reports_nonfinite(mode, native):
+    """Python Newton loop and the library's mgpbd_substep alike."""
     from controllable_pbd_3d_amd._lib import MGPBD_ENONFINITE, MGPBDError
     kw = {} if mode == "reference" else dict(
         hierarchy="extended", galerkin="structural", relative_tol=True, tol_cg=1e-6,
         max_coarse=64, coarse_solver="cholesky", smoother="chebyshev", verify_residual=1,
         fp64_iterate=True, precision="fp16", level0="factored")
-    mesh, s = _solver(**kw)
+    mesh, s = _solver(native_substep=native, **kw)
     s.step(0)                                   # a healthy substep first
     free = int(np.nonzero(mesh.invm > 0)[0][0])
     s.v_p[free, 1] = float("nan")               # host write into the device state
-    print(f"[errors] {mode}: NaN at vertex {free}", flush=True)
+    print(f"[errors] {mode} native={native}: NaN at vertex {free}", flush=True)
     with pytest.raises(MGPBDError) as e:
         s.step(1)
     assert "non-finite" in str(e.value)

@@ -29,6 +29,23 @@ def test_every_declared_symbol_is_exported_and_bound():
     assert lib.mgpbd_abi_version() == 1
 
 
+def test_substep_structures_match_the_header():
+    """The ctypes restatements of mgpbd_substep_desc / mgpbd_newton_stat have the C sizes,
+    and every field of the descriptor is distinct (none dropped or doubled)."""
+    import ctypes
+    lib = _lib.load()
+    sizes = (ctypes.c_int32 * 2)()
+    assert lib.mgpbd_substep_struct_sizes(sizes) == 0
+    assert (sizes[0], sizes[1]) == (ctypes.sizeof(_lib.SubstepDesc),
+                                    ctypes.sizeof(_lib.NewtonStat))
+    names = [f[0] for f in _lib.SubstepDesc._fields_]
+    assert len(names) == len(set(names))
+    src = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = src[src.index("typedef struct mgpbd_substep_desc"):src.index("} mgpbd_substep_desc;")]
+    c_fields = re.findall(r"\*?\s*(\w+)(?:\[\d+\])?\s*[;,]", body.split("{", 1)[1])
+    assert [("lam" if f == "lambda" else f) for f in c_fields] == names
+
+
 def test_host_setup_rejects_bad_input():
     lib = _lib.load()
     t = np.array([0, 1, 2, 7], np.int32)          # vertex id 7 with n_verts = 4
