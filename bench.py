@@ -66,23 +66,25 @@ def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
     raise ValueError(kind)
 
 
-def factored_bytes(n_rows, n_verts, kind, chebyshev=False):
+def factored_bytes(n_rows, n_verts, kind, chebyshev=False, offsets=False):
     """Algorithmic HBM bytes of the factored level 0 (each array counted once per pass; 4
     incidence slots per row). kind "gather" (vertex pass): 4 slots per row x (4-byte incidence
-    + 8-byte K) + x (4 per row) + u written (16 per vertex) = 52 n + 16 n_v. "apply" (tet pass
-    of a sweep): t_i 16 + K 24 + a^ 4 + x 4 + b 4 + y 4 per row + u read (16 per vertex)
-    = 56 n + 16 n_v. "jacobi"/"resid": both passes (+ the previous iterate, 4 n, for the
+    + 8-byte K) + x (4 per row) + u written (16 per vertex) = 52 n + 16 n_v; with 16-bit slot
+    offsets (the tet in K's fourth word, a 4-byte base tet per vertex) 36 n + 20 n_v. "apply"
+    (tet pass of a sweep): t_i 16 + K 24 + a^ 4 + x 4 + b 4 + y 4 per row + u read (16 per
+    vertex) = 56 n + 16 n_v. "jacobi"/"resid": both passes (+ the previous iterate, 4 n, for the
     Chebyshev sweeps). "spmv": the PCG's FP64 product from the FP32 gradients, 4 slots x
-    (4-byte incidence + 16-byte gradient) + p + u (32 per vertex) written; t_i 16 + gradC 48 +
-    alpha~ 4 + p 4 + A p 4 per row + u read."""
-    gather = 52 * n_rows + 16 * n_verts
+    (4-byte incidence + 16-byte gradient; 16 bytes with offsets, + 4 per vertex) + p + u (32
+    per vertex) written; t_i 16 + gradC 48 + alpha~ 4 + p 4 + A p 4 per row + u read."""
+    gather = (36 * n_rows + 20 * n_verts) if offsets else (52 * n_rows + 16 * n_verts)
     apply = 56 * n_rows + 16 * n_verts
     if kind == "gather":
         return gather
     if kind == "apply":
         return apply
     if kind == "spmv":
-        return 80 * n_rows + 4 * n_rows + 32 * n_verts + 76 * n_rows + 32 * n_verts
+        vert = (64 * n_rows + 4 * n_verts) if offsets else 80 * n_rows
+        return vert + 4 * n_rows + 32 * n_verts + 76 * n_rows + 32 * n_verts
     if kind in ("jacobi", "resid"):
         return gather + apply + (4 * n_rows if chebyshev and kind == "jacobi" else 0)
     raise ValueError(kind)
@@ -102,7 +104,8 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factored_nv=None):
+def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factored_nv=None,
+                 offsets=False):
     from controllable_pbd_3d_amd import _lib
     ms = np.zeros(8, np.float64)
     cnt = np.zeros(8, np.int64)
@@ -115,7 +118,7 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
     for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
-            by = factored_bytes(n, factored_nv, kind, chebyshev) if factored_nv else \
+            by = factored_bytes(n, factored_nv, kind, chebyshev, offsets) if factored_nv else \
                 kernel_bytes(n, nnz, kind, per, chebyshev)
             kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
                           "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
@@ -202,9 +205,10 @@ def train_learned(cfg, seeds, substeps, epochs):
 
 
 def probe_level0(s, reps=60, flush_gib=1.0):
-    """Per-kernel HIP-event timing of `reps` level-0 sweeps of this solver's V-cycle
-    (mgpbd_mg_probe_level0), back to back and with a flush_gib GiB buffer rewritten before each
-    sweep (larger than the 256 MB last-level cache: the sweep then reads from HBM)."""
+    """HIP-event timing of this solver's level-0 sweep kernels (mgpbd_mg_probe_level0):
+    "warm", `reps` launches of each kernel back to back (per-launch time, as the V-cycle runs
+    them); "flushed", events around each kernel with a flush_gib GiB buffer rewritten before
+    each sweep (larger than the 256 MB last-level cache: the sweep then reads from HBM)."""
     import torch
     from controllable_pbd_3d_amd import _lib
     mg = s._gpu_solver
@@ -292,9 +296,11 @@ def run_config(mesh, mu_t, lam_t, cfg, name, steps, warmup, profile_steps, model
     win0 = any(w["level"] == 0 and w["woff"] is not None for w in s._gpu_solver._win)
     per = 6 if win0 else 8
     fnv = s.n_verts if kw.get("level0") == "factored" else None
-    kern = kernel_stats(s._gpu_solver.ctx, s.n_cons, s._nnz, kw["precision"], cheb, per, fnv)
+    offs = res["slot_offsets"] = fnv is not None and "tbase" in s._gpu_solver._fac
+    kern = kernel_stats(s._gpu_solver.ctx, s.n_cons, s._nnz, kw["precision"], cheb, per, fnv,
+                        offs)
     kern.update({k: v for k, v in kernel_stats(ctx, s.n_cons, s._nnz, kw["precision"], cheb,
-                                               per, fnv).items() if k not in kern})
+                                               per, fnv, offs).items() if k not in kern})
     ctx("mgpbd_ctx_profile", 0)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 0)
     over = [k for k, v in kern.items() if v.get("GBs", 0.0) > HBM_PEAK_GBS]
@@ -506,9 +512,9 @@ def main():
     }
     if probe is not None and factored:
         # dominant kernel of the reported run: the factored level 0's vertex pass
-        # (k_fac_gather), timed alone with HIP events over back-to-back sweeps
+        # (k_fac_gather), `reps` launches back to back between two HIP events
         n_rows = best["n_rows"]
-        by = factored_bytes(n_rows, mesh.n_verts, "gather")
+        by = factored_bytes(n_rows, mesh.n_verts, "gather", offsets=best["slot_offsets"])
         us = probe["warm"]["first_us"]
         gbs = by / (us * 1e-6) / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": f"k_fac_gather ({head})", "achieved": gbs,

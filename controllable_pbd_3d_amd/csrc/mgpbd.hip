@@ -2096,57 +2096,73 @@ __device__ __forceinline__ int fac_slot0(const int* __restrict__ vs_ptr, int v, 
   return vs_ptr[v >> 4] + ((v & 15) << 2) + q;
 }
 
+// With tbase (mgpbd_mg_set_level0_offsets) the fourth 16-bit word of every K slot holds the
+// slot's tet minus the vertex's base tet (raw bits, not a half value) and the fourth float
+// of its FP32 gradient the same offset as integer bits: the vertex passes then take the tet
+// from the value they load anyway and never read the incidence list.
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_vert(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
-           const int* __restrict__ vs_inc, const float* __restrict__ invm,
-           const float* __restrict__ gradC, const float* __restrict__ isd,
-           v2u32* __restrict__ vs_k, float4* __restrict__ vs_g) {
+           const int* __restrict__ vs_inc, const int* __restrict__ tbase,
+           const float* __restrict__ invm, const float* __restrict__ gradC,
+           const float* __restrict__ isd, v2u32* __restrict__ vs_k,
+           float4* __restrict__ vs_g) {
   const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int v = gid >> 2, q = gid & 3;
   if (v >= nv) return;
   const int base = fac_slot0(vs_ptr, v, q);
   const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
   const float sw = sqrtf(invm[v]);
+  const int tb = tbase ? tbase[v] : 0;
   for (int k = 0; k < own; ++k) {
     const int p = base + 64 * k;
     const int inc = vs_inc[p];
-    const float s = sw * isd[inc >> 2];
-    const float* g = gradC + 12 * (size_t)(inc >> 2) + 3 * (inc & 3);
-    vs_k[p] = v2u32{pack_h2<VT>(s * g[0], s * g[1]), pack_h2<VT>(s * g[2], 0.0f)};
-    vs_g[p] = make_float4(g[0], g[1], g[2], 0.0f);
+    const int t = inc >> 2;
+    const uint32_t off = tbase ? (uint32_t)(t - tb) : 0u;
+    const float s = sw * isd[t];
+    const float* g = gradC + 12 * (size_t)t + 3 * (inc & 3);
+    vs_k[p] = v2u32{pack_h2<VT>(s * g[0], s * g[1]), (uint32_t)f2h<VT>(s * g[2]) | (off << 16)};
+    vs_g[p] = make_float4(g[0], g[1], g[2], __uint_as_float(off));
   }
 }
 
+// Tet of a vertex slot: from the incidence list, or (OFF) base + the offset word of its K / g.
+template <bool OFF>
+__device__ __forceinline__ int slot_tet(const int* __restrict__ vs_inc, int p, int tb,
+                                        uint32_t word) {
+  return OFF ? tb + (int)(word >> 16) : __builtin_nontemporal_load(vs_inc + p) >> 2;
+}
+
 // u_v = sum_k K_{t_k, v} x_{t_k} over the vertex's incidence slots (FP32 accumulation): each
-// of the vertex's four lanes sums its own slots (loads of up to eight issued before their
-// products), then lanes q and q^1, q and q^2 add (fixed order, deterministic).
-template <int VT>
-__global__ void __launch_bounds__(kBlock)
-k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
-             const int* __restrict__ vs_inc, const v2u32* __restrict__ vs_k,
-             const float* __restrict__ x, float4* __restrict__ u, const int* __restrict__ done) {
-  if (done && *done) return;
-  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  const int v = gid >> 2, q = gid & 3;
-  if (v >= nv) return;   // all four lanes of a vertex leave together
+// of the vertex's four lanes sums its own slots (loads of up to six issued before their
+// products: six slots per lane is an interior vertex of a Kuhn-cube mesh, and a batch longer
+// than the lane's slots costs reloads), then lanes q and q^1, q and q^2 add (fixed order,
+// deterministic). With slot offsets (tbase) the tets come from the K words themselves.
+constexpr int kFacBatch = 6;
+template <int VT, bool OFF>
+__device__ __forceinline__ void fac_gather_lane(int v, int q, const int* __restrict__ vs_ptr,
+                                                const int* __restrict__ vs_cnt,
+                                                const int* __restrict__ vs_inc, int tb,
+                                                const v2u32* __restrict__ vs_k,
+                                                const float* __restrict__ x,
+                                                float4* __restrict__ u) {
   const int base = fac_slot0(vs_ptr, v, q);
   const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for (int k0 = 0; k0 < own; k0 += 8) {
-    int t[8];
-    v2u32 h[8];
+  for (int k0 = 0; k0 < own; k0 += kFacBatch) {
+    int t[kFacBatch];
+    v2u32 h[kFacBatch];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < kFacBatch; ++j) {
       const int p = base + 64 * min(k0 + j, own - 1);   // past the end: repeats the last slot
-      t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
       h[j] = __builtin_nontemporal_load(vs_k + p);
+      t[j] = slot_tet<OFF>(vs_inc, p, tb, h[j].y);
     }
-    float xv[8];
+    float xv[kFacBatch];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = x[t[j]];
+    for (int j = 0; j < kFacBatch; ++j) xv[j] = x[t[j]];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < kFacBatch; ++j) {
       if (k0 + j < own) {
         ax = fmaf(lo_h<VT>(h[j].x), xv[j], ax);
         ay = fmaf(hi_h<VT>(h[j].x), xv[j], ay);
@@ -2157,6 +2173,22 @@ k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_
   ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
   ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
   if (q == 0) u[v] = make_float4(ax, ay, az, 0.0f);
+}
+
+template <int VT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
+             const int* __restrict__ vs_inc, const int* __restrict__ tbase,
+             const v2u32* __restrict__ vs_k, const float* __restrict__ x,
+             float4* __restrict__ u, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;   // all four lanes of a vertex leave together
+  if (tbase)
+    fac_gather_lane<VT, true>(v, q, vs_ptr, vs_cnt, vs_inc, tbase[v], vs_k, x, u);
+  else
+    fac_gather_lane<VT, false>(v, q, vs_ptr, vs_cnt, vs_inc, 0, vs_k, x, u);
 }
 
 // (A^ x)_t from u: a^_t x_t + K_t0 . u_v0 + ... + K_t3 . u_v3 (FP32)
@@ -2248,35 +2280,31 @@ k_fac_power(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
 // ------------------------------------------------------------------------------------------
 // gate 0: skip when *done is set (PCG launches past convergence); gate 1: run only when the
 // solve converged and is not yet verified (st: the verification tail of a replay).
-template <typename XT>
-__global__ void __launch_bounds__(kBlock)
-k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
-              const int* __restrict__ vs_inc, const float4* __restrict__ vs_g,
-              const float* __restrict__ invm, const XT* __restrict__ x,
-              double4* __restrict__ u, const int* __restrict__ done,
-              const PcgState* __restrict__ gate_st) {
-  if (done && *done) return;
-  if (gate_st && (gate_st->done != 1 || gate_st->verified)) return;
-  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  const int v = gid >> 2, q = gid & 3;
-  if (v >= nv) return;   // all four lanes of a vertex leave together
+template <typename XT, bool OFF>
+__device__ __forceinline__ void fac_ogather_lane(int v, int q, const int* __restrict__ vs_ptr,
+                                                 const int* __restrict__ vs_cnt,
+                                                 const int* __restrict__ vs_inc, int tb,
+                                                 const float4* __restrict__ vs_g,
+                                                 const float* __restrict__ invm,
+                                                 const XT* __restrict__ x,
+                                                 double4* __restrict__ u) {
   const int base = fac_slot0(vs_ptr, v, q);
   const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
   double ax = 0.0, ay = 0.0, az = 0.0;
-  for (int k0 = 0; k0 < own; k0 += 8) {
-    int t[8];
-    float4 g[8];
+  for (int k0 = 0; k0 < own; k0 += kFacBatch) {
+    int t[kFacBatch];
+    float4 g[kFacBatch];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < kFacBatch; ++j) {
       const int p = base + 64 * min(k0 + j, own - 1);
-      t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
       g[j] = vs_g[p];
+      t[j] = slot_tet<OFF>(vs_inc, p, tb, __float_as_uint(g[j].w) << 16);
     }
-    double xv[8];
+    double xv[kFacBatch];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) xv[j] = (double)x[t[j]];
+    for (int j = 0; j < kFacBatch; ++j) xv[j] = (double)x[t[j]];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < kFacBatch; ++j) {
       if (k0 + j < own) {
         ax = fma((double)g[j].x, xv[j], ax);
         ay = fma((double)g[j].y, xv[j], ay);
@@ -2290,6 +2318,24 @@ k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs
     const double w = (double)invm[v];
     u[v] = make_double4(w * ax, w * ay, w * az, 0.0);
   }
+}
+
+template <typename XT>
+__global__ void __launch_bounds__(kBlock)
+k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
+              const int* __restrict__ vs_inc, const int* __restrict__ tbase,
+              const float4* __restrict__ vs_g, const float* __restrict__ invm,
+              const XT* __restrict__ x, double4* __restrict__ u, const int* __restrict__ done,
+              const PcgState* __restrict__ gate_st) {
+  if (done && *done) return;
+  if (gate_st && (gate_st->done != 1 || gate_st->verified)) return;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;   // all four lanes of a vertex leave together
+  if (tbase)
+    fac_ogather_lane<XT, true>(v, q, vs_ptr, vs_cnt, vs_inc, tbase[v], vs_g, invm, x, u);
+  else
+    fac_ogather_lane<XT, false>(v, q, vs_ptr, vs_cnt, vs_inc, 0, vs_g, invm, x, u);
 }
 
 // dpos_v = w_v sum_t dlambda_t g_tv over the vertex's incidences (FP32, four lanes per vertex
@@ -2312,7 +2358,7 @@ k_fac_scatter(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int p = base + 64 * min(k0 + j, own - 1);
-      t[j] = __builtin_nontemporal_load(vs_inc + p) >> 2;
+      t[j] = slot_tet<false>(vs_inc, p, 0, 0u);
       g[j] = vs_g[p];
     }
     float dl[8];
@@ -2586,6 +2632,7 @@ struct mgpbd_ctx {
     const int* vs_ptr = nullptr;
     const int* vs_cnt = nullptr;
     const int* vs_inc = nullptr;
+    const int* tbase = nullptr;   // per-vertex base tet of the 16-bit slot offsets (or NULL)
     v2u32* vs_k = nullptr;   // K per vertex slot (4 halves)
     float4* vs_g = nullptr;  // FP32 gradient per vertex slot (the PCG's operator)
     v2u32* tk = nullptr;     // K per tet, three planes of n_tets
@@ -3476,7 +3523,7 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
   const MgLevel& L = ctx->L[0];
   hipStream_t s = ctx->stream;
   k_fac_gather<VT><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
-      F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k, x, F.u, done);
+      F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k, x, F.u, done);
   const int g = nblocks(L.n);
   const float w = ctx->omega;
 #define FAC(M) k_fac_apply<VT, M><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b, w, \
@@ -3660,11 +3707,11 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     for (int it = 0; it < iters; ++it) {
       if (fac && ctx->precision == MGPBD_PREC_FP16) {
         k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
-                                                         F.vs_k, v, F.u, nullptr);
+                                                         F.tbase, F.vs_k, v, F.u, nullptr);
         k_fac_power<1><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
       } else if (fac) {
         k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
-                                                         F.vs_k, v, F.u, nullptr);
+                                                         F.tbase, F.vs_k, v, F.u, nullptr);
         k_fac_power<2><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
       } else if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
         k_power_step_wf<<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
@@ -3854,6 +3901,15 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
   return 0;
 }
 
+int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase) {
+  CTX_CHECK("mgpbd_mg_set_level0_offsets");
+  mgpbd_ctx::Factored& F = ctx->fac;
+  if (!F.on) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_offsets: no factored level 0");
+  drop_graphs(ctx);
+  F.tbase = tbase;
+  return 0;
+}
+
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,
@@ -3940,12 +3996,12 @@ static int convert_half_levels(mgpbd_ctx* ctx, int first, int last) {
       if (ctx->precision == MGPBD_PREC_FP16) {
         k_fac_tet<1><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
                                                                L.sd, L.isd, F.dinv, F.tk, F.ahat);
-        k_fac_vert<1><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.invm,
+        k_fac_vert<1><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.invm,
                                                       F.gradC, L.isd, F.vs_k, F.vs_g);
       } else {
         k_fac_tet<2><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
                                                                L.sd, L.isd, F.dinv, F.tk, F.ahat);
-        k_fac_vert<2><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.invm,
+        k_fac_vert<2><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.invm,
                                                       F.gradC, L.isd, F.vs_k, F.vs_g);
       }
       continue;
@@ -4004,44 +4060,69 @@ int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b, const float*
   const MgLevel& L = ctx->L[0];
   hipStream_t s = ctx->stream;
   const bool fac = ctx->fac.on && half_mode(ctx);
-  // three events per rep, all queued before one wait: the sweeps run back to back as in the
-  // V-cycle (no host round trip between them)
-  std::vector<hipEvent_t> e(3 * (size_t)reps, nullptr);
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  auto first = [&]() {   // the factored vertex pass, or a whole stored-level sweep
+    if (fac) {
+      if (ctx->precision == MGPBD_PREC_FP16)
+        k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
+            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k, x, F.u, nullptr);
+      else
+        k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
+            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k, x, F.u, nullptr);
+    } else if (half_mode(ctx)) {
+      sellh(ctx, L, 1, x, y, b, nullptr, nullptr);
+    } else {
+      sell_launch<1>(ctx, L, x, y, L.dinv, b, ctx->omega, nullptr, nullptr);
+    }
+  };
+  auto second = [&]() {   // the factored tet pass (nothing for a stored level)
+    if (!fac) return;
+    if (ctx->precision == MGPBD_PREC_FP16)
+      k_fac_apply<1, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
+                                                       ctx->omega, nullptr, L.isd, nullptr,
+                                                       nullptr, nullptr);
+    else
+      k_fac_apply<2, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
+                                                       ctx->omega, nullptr, L.isd, nullptr,
+                                                       nullptr, nullptr);
+  };
+  const size_t ne = flush_floats > 0 ? 3 * (size_t)reps : 4;
+  std::vector<hipEvent_t> e(ne, nullptr);
   struct Free {
     std::vector<hipEvent_t>& e;
     ~Free() { for (hipEvent_t v : e) if (v) (void)hipEventDestroy(v); }
   } fr{e};
   for (auto& ev : e) MGPBD_HIP_TRY(hipEventCreate(&ev));
+  if (flush_floats == 0) {
+    // warm: `reps` launches of each kind back to back between two events (per-launch time
+    // including the dispatch gap, no event between kernels), then `reps` pairs
+    MGPBD_HIP_TRY(hipEventRecord(e[0], s));
+    for (int r = 0; r < reps; ++r) first();
+    MGPBD_HIP_TRY(hipEventRecord(e[1], s));
+    for (int r = 0; r < reps; ++r) second();
+    MGPBD_HIP_TRY(hipEventRecord(e[2], s));
+    for (int r = 0; r < reps; ++r) { first(); second(); }
+    MGPBD_HIP_TRY(hipEventRecord(e[3], s));
+    MGPBD_LAUNCH_CHECK("mgpbd_mg_probe_level0");
+    MGPBD_HIP_TRY(hipEventSynchronize(e[3]));
+    for (int k = 0; k < 3; ++k) {
+      float t = 0.f;
+      MGPBD_HIP_TRY(hipEventElapsedTime(&t, e[k], e[k + 1]));
+      ms_out[k] = (double)t / reps;
+    }
+    return 0;
+  }
+  // flushed: a pass over a buffer larger than the caches before each rep, events around each
+  // kernel (all queued before one wait)
   for (int r = 0; r < reps; ++r) {
-    if (flush_floats > 0) {   // a pass over a buffer larger than the caches before each rep
-      for (int64_t off = 0; off < flush_floats; off += (int64_t)1 << 30) {
-        const int m = (int)std::min<int64_t>((int64_t)1 << 30, flush_floats - off);
-        k_blas1<<<nblocks(m), kBlock, 0, s>>>(4, m, 1.0f, nullptr, flush + off);
-      }
+    for (int64_t off = 0; off < flush_floats; off += (int64_t)1 << 30) {
+      const int m = (int)std::min<int64_t>((int64_t)1 << 30, flush_floats - off);
+      k_blas1<<<nblocks(m), kBlock, 0, s>>>(4, m, 1.0f, nullptr, flush + off);
     }
     MGPBD_HIP_TRY(hipEventRecord(e[3 * r], s));
-    if (fac) {
-      const mgpbd_ctx::Factored& F = ctx->fac;
-      if (ctx->precision == MGPBD_PREC_FP16)
-        k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
-            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k, x, F.u, nullptr);
-      else
-        k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
-            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_k, x, F.u, nullptr);
-      MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 1], s));
-      if (ctx->precision == MGPBD_PREC_FP16)
-        k_fac_apply<1, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
-                                                         ctx->omega, nullptr, L.isd, nullptr,
-                                                         nullptr, nullptr);
-      else
-        k_fac_apply<2, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
-                                                         ctx->omega, nullptr, L.isd, nullptr,
-                                                         nullptr, nullptr);
-    } else {
-      if (half_mode(ctx)) sellh(ctx, L, 1, x, y, b, nullptr, nullptr);
-      else sell_launch<1>(ctx, L, x, y, L.dinv, b, ctx->omega, nullptr, nullptr);
-      MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 1], s));
-    }
+    first();
+    MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 1], s));
+    second();
     MGPBD_HIP_TRY(hipEventRecord(e[3 * r + 2], s));
     MGPBD_LAUNCH_CHECK("mgpbd_mg_probe_level0");
   }
@@ -4082,13 +4163,13 @@ static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* x
   hipStream_t s = ctx->stream;
   const int gv = nblocks((int64_t)kFacLanes * F.nv);
   if (xd) {
-    k_fac_ogather<double><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm,
+    k_fac_ogather<double><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_g, F.invm,
                                                 xd, F.u64, done, gate);
     k_fac_oapply<double, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xd, b, y,
                                                   ctx->ws, done, gate);
     return;
   }
-  k_fac_ogather<float><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.vs_g, F.invm,
+  k_fac_ogather<float><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_g, F.invm,
                                              xf, F.u64, done, gate);
   if (mode == 0)
     k_fac_oapply<float, 0><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,

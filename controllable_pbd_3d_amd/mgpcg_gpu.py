@@ -83,6 +83,25 @@ def vertex_slices(vptr: np.ndarray, vinc: np.ndarray):
     return vs_ptr.astype(np.int32), cnt.astype(np.int32), vs_inc
 
 
+def slot_offset_base(vptr: np.ndarray, vinc: np.ndarray):
+    """Per-vertex base tet (the smallest tet at the vertex) when every vertex's tets lie within
+    65535 of it, else None: the factored level 0 then stores each incidence slot's tet as a
+    16-bit offset beside its K and gradient (mgpbd_mg_set_level0_offsets)."""
+    vptr = np.asarray(vptr, np.int64)
+    cnt = np.diff(vptr)
+    nv = len(cnt)
+    if nv == 0:
+        return None
+    tet = np.asarray(vinc, np.int64)[:vptr[-1]] >> 2
+    v_of = np.repeat(np.arange(nv), cnt)
+    base = np.full(nv, np.iinfo(np.int64).max, np.int64)
+    np.minimum.at(base, v_of, tet)
+    base[cnt == 0] = 0
+    if len(tet) and int((tet - base[v_of]).max()) > 65535:
+        return None
+    return base.astype(np.int32)
+
+
 def factored_galerkin_maps(t_i, n_verts: int, agg, c_indptr, c_indices):
     """Maps of level 1 from the gradients (A1 = G W G^T + P^T diag(alpha~) P, symmetric: each
     entry (I, J) with I <= J is summed once and stored at both positions). Returns gptr, gmem,
@@ -333,6 +352,8 @@ class GPUMultigridSolver:
     # reduced-precision levels pack 16-bit column offsets j - i with the values when every
     # |j - i| is below this bound, and use int32 columns (shared with the FP32 copy) otherwise
     col16_limit = 32768
+    # factored level 0: 16-bit tet offsets in the vertex slots' unused words when they fit
+    slot_offsets = True
     # packed levels whose blocks of 256 rows read x from at most this many 32-column segments
     # stage them in LDS (windowed sweeps; the words then hold LDS offsets, so the 16-bit column
     # offset bound does not apply); 0 disables
@@ -425,6 +446,10 @@ class GPUMultigridSolver:
                  dptr(f["vs_cnt"]), dptr(f["vs_inc"]), dptr(f["vs_k"]), dptr(f["vs_g"]),
                  dptr(f["tk"]), dptr(f["ahat"]), dptr(f["u"]), dptr(f["u64"]),
                  dptr(L.diag_inv))
+        tbase = slot_offset_base(src["vptr"], src["vinc"])
+        if tbase is not None and self.slot_offsets:
+            f["tbase"] = torch.from_numpy(tbase).to(dev)
+            self.ctx("mgpbd_mg_set_level0_offsets", dptr(f["tbase"]))
         if self.n_levels < 2:
             return
         C = self.levels[1]

@@ -471,6 +471,13 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
  * cpos_t[c] (the same on the diagonal), and cagg[c] = the aggregate when c is a diagonal
  * entry, else -1 (the diagonal adds the aggregate's alpha~ over level 0's member lists).
  * gptr NULL switches it off. */
+/* 16-bit slot offsets for the factored level 0's vertex passes (no reference counterpart):
+ * tbase_dev (n_verts int32, device) holds a base tet per vertex such that every tet at the
+ * vertex lies in [tbase, tbase + 65535]; the per-iteration conversion then writes each slot's
+ * tet - tbase into the unused fourth word of its K and FP32 gradient, and the V-cycle's and
+ * the PCG's vertex passes read no incidence list (12 -> 8 bytes per slot). NULL switches it
+ * off; it takes effect at the next mgpbd_mg_convert_half. */
+int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase_dev);
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,
@@ -499,13 +506,15 @@ int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b_dev, const
                             float* r_dev);
 /* x = M^-1 b with one V-cycle from x = 0. Replaces vcycle(0) (cons/mgpcg_gpu.py:214-259). */
 int mgpbd_mg_vcycle(mgpbd_ctx* ctx, const float* b_dev, float* x_dev);
-/* Measurement (no reference counterpart): `reps` (<= 1000) level-0 weighted-Jacobi sweeps
- * y = x + omega D^-1 (b - A x) as the V-cycle launches them, back to back on the context
- * stream with HIP events around each kernel (one wait after the last). ms_out[0]: average
- * duration of the first kernel (the
- * factored level's vertex pass, or the whole sweep of a stored level), ms_out[1]: of the
- * second (the factored tet pass, 0 otherwise), ms_out[2]: of the sweep. With flush_floats > 0
- * every rep first rewrites the flush buffer (scale by 1), evicting the caches. */
+/* Measurement (no reference counterpart): level-0 weighted-Jacobi sweeps
+ * y = x + omega D^-1 (b - A x) with the kernels the V-cycle launches, on the context stream.
+ * The first kernel is the factored level's vertex pass (or the whole sweep of a stored level),
+ * the second the factored tet pass (none for a stored level). flush_floats = 0: `reps`
+ * (<= 1000) first kernels back to back, then `reps` second kernels, then `reps` sweeps, with
+ * HIP events only between the three batches; ms_out[0..2] = per-launch time of each batch
+ * (kernel plus dispatch gap). flush_floats > 0: before each of `reps` sweeps the flush buffer
+ * is rewritten (scale by 1), evicting the caches, and events around each kernel give
+ * ms_out[0] first kernel, ms_out[1] second, ms_out[2] sweep (averages). */
 int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b_dev, const float* x_dev,
                           float* y_dev, float* flush_dev, int64_t flush_floats, double* ms_out);
 /* PCG for A x = b, x from 0 (the reference's GPUMultigridSolver.solve, cons/mgpcg_gpu.py:

@@ -592,7 +592,9 @@ def test_factored_level0_operator(precision):
     assert np.array_equal(mesh.tets.reshape(-1, 4)[inc >> 2, inc & 3], v)
     tk3 = tk.reshape(nt, 4, 3)
     assert np.array_equal(vk[pos, :3], tk3[inc >> 2, inc & 3])
-    assert np.all(vk[pos, 3] == 0)
+    # fourth word: the slot's tet as an offset from the vertex's base tet (0 without offsets)
+    base = f["tbase"].cpu().numpy()[v] if "tbase" in f else (inc >> 2)
+    assert np.array_equal(vk[pos, 3].astype(np.int64), (inc >> 2) - base)
     rng = np.random.default_rng(11)
     x = rng.standard_normal(nt).astype(np.float32)
     b = rng.standard_normal(nt).astype(np.float32)
