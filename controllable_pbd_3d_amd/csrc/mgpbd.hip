@@ -3283,7 +3283,7 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
   return 0;
 }
 
-int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh_smoother, int half_from) {
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from) {
   CTX_CHECK("mgpbd_mg_refresh");
   if (int e = check_levels(ctx)) return e;
   if (half_from < 0 || half_from > ctx->n_levels)
@@ -3300,10 +3300,10 @@ int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh_smoother, int half_from) {
     }
     if (int e = mgpbd_diag_inv(ctx, R.n_coarse, R.cdiag, R.coarse, R.cdinv)) return e;
   }
-  if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels > 1)
+  if (refresh && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels > 1)
     if (int e = mgpbd_mg_factor_coarsest(ctx)) return e;
   if (int e = convert_half_levels(ctx, half_from, ctx->n_levels)) return e;
-  if (refresh_smoother && ctx->spectral && !ctx->fixed_weights)
+  if (refresh && ctx->spectral && !ctx->fixed_weights)
     if (int e = mgpbd_mg_update_smoother(ctx, nullptr)) return e;
   return 0;
 }

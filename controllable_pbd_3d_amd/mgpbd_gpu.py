@@ -36,9 +36,10 @@ substep's lambda, or from the network's prediction with the previous lambda as t
 the device keeps the start with the smaller residual and iterates to the same tolerance.
 ``record_samples`` (a list) collects training pairs from the first Newton iteration.
 ``spectral_refresh`` ('newton' | 'substep'): the spectral smoothers re-estimate every level's
-spectrum after every Newton iteration's refill, or only at a substep's first Newton iteration
-(the later iterations of a substep change the matrices little; the verified PCG and the stall
-rescue cover a lagging estimate).
+spectrum, and the dense coarsest solve (coarse_solver='cholesky') re-inverts its level, after
+every Newton iteration's refill, or only at a substep's first Newton iteration (the later
+iterations of a substep change the matrices little; the kept inverse is SPD, and the verified
+PCG and the stall rescue cover a lagging estimate).
 ``overlap_inference`` (default True; learned start with the factored level 0): the network runs
 on a second stream while the coarse levels refill (bit-identical to running it after them).
 ``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is

@@ -535,7 +535,9 @@ class GPUMultigridSolver:
     def update_all_coarse_matrices(self, refresh_smoother: bool = True, after_level0=None):
         """Coarse values from the fine ones through the refill maps
         (cons/mgpcg_gpu.py:183-212). refresh_smoother=False keeps the spectral smoother's
-        current estimates (extension: the caller knows the matrices changed little).
+        current estimates and the coarsest level's dense inverse (extension: the caller knows
+        the matrices changed little; the kept inverse is SPD, so the V-cycle stays an SPD
+        preconditioner).
         after_level0 (factored level 0 only): called once level 0's scales, 1/diag and K are
         queued, before the coarse levels' work, so the caller can start work that needs only
         those on another stream."""
@@ -551,7 +553,7 @@ class GPUMultigridSolver:
                 self.ctx("mgpbd_coarse_refill", rf["n"], dptr(rf["off"]), dptr(rf["fpos"]),
                          dptr(rf["cpos"]), dptr(F.data), dptr(C.data))
             C.cache_diag_inv()
-        if getattr(self, "_coarse_mode", 0) == 1:
+        if getattr(self, "_coarse_mode", 0) == 1 and refresh_smoother:
             self.ctx("mgpbd_mg_factor_coarsest")
         if getattr(self, "_half", None):
             if early:

@@ -349,11 +349,12 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
                         const int32_t* coarse_diag_pos_dev, float* coarse_dinv_dev);
 /* One call for update_all_coarse_matrices (cons/mgpcg_gpu.py:183-212) after level 0's values
  * changed: per level the coarse refill (mgpbd_mg_factored_refill for a factored level 1)
- * and 1/diag, the coarsest inverse with MGPBD_COARSE_CHOLESKY, the reduced-precision
- * copies of levels half_from .. n_levels-1 (a caller that converted level 0 early passes 1)
- * and, with refresh_smoother != 0, the spectral smoother's estimates. Queues only (a
+ * and 1/diag, the reduced-precision copies of levels half_from .. n_levels-1 (a caller that
+ * converted level 0 early passes 1) and, with refresh != 0, the coarsest inverse
+ * (MGPBD_COARSE_CHOLESKY) and the spectral smoother's estimates; refresh = 0 keeps both
+ * (the caller knows the matrices changed little; the kept inverse is SPD). Queues only (a
  * Cholesky refactorization synchronises, as in mgpbd_mg_factor_coarsest). */
-int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh_smoother, int half_from);
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from);
 /* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level
  * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse (rocBLAS dgemm)
  * and the device keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue
