@@ -276,7 +276,7 @@ def run_config(mesh, mu_t, lam_t, cfg, name, steps, warmup, profile_steps, model
     res["hbm_footprint_bytes"] = s.device_bytes()
     res["pcg_rescues"], res["pcg_restarts"] = s._gpu_solver.pcg_counters()
     ns, refac = s._gpu_solver.coarse_stats()
-    res["coarsest_inverse"] = {"newton_schulz_steps": ns, "cholesky_refactors": refac}
+    res["coarsest_inverse"] = {"newton_schulz_refresh_steps": ns, "full_inversions": refac}
     if kw.get("warm_start"):
         # start of each substep's first solve: 0 zero, 1 the guess, 2 the previous lambda
         res["first_solve_starts"] = np.bincount(starts, minlength=3).tolist()

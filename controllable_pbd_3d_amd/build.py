@@ -44,8 +44,7 @@ def build(verbose: bool = False) -> str:
              verbose)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp", "-L/opt/rocm/lib",
-          "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"], verbose)
+    _run([hipcc, "-shared", "-fPIC", "-o", tmp, *objs, "-lgomp"], verbose)
     os.replace(tmp, LIB)
     return LIB
 

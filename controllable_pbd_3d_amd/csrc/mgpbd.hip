@@ -25,8 +25,6 @@
 #include <string>
 #include <vector>
 
-#include <rocsolver/rocsolver.h>
-
 #include "mgpbd_internal.h"
 #include "svd3.h"
 
@@ -973,13 +971,79 @@ k_sumsq(int64_t len, const double* __restrict__ x, RedWS ws) {
   store_partial(ws, 0, block_sum(acc));
 }
 
-// copy the lower triangle (column-major) onto the upper one
+// C = alpha A B + beta C for n x n column-major FP64 matrices (beta = 0: C is not read): one
+// 32 x 32 tile of C per block, each thread 2 x 2 outputs (rows tx, tx + 16, columns ty,
+// ty + 16), the k dimension staged through LDS 32 at a time and summed in ascending order
+// (FP64 FMA; deterministic). Grid (ceil(n/32), ceil(n/32)).
+constexpr int kGemmTile = 32;
 __global__ void __launch_bounds__(kBlock)
-k_symmetrize_lower(int n, double* __restrict__ D) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= n * n) return;
-  const int i = t % n, j = t / n;   // element (i, j)
-  if (i < j) D[(size_t)j * n + i] = D[(size_t)i * n + j];
+k_dgemm(int n, double alpha, const double* __restrict__ A, const double* __restrict__ B,
+        double beta, double* __restrict__ C) {
+  __shared__ double As[kGemmTile][kGemmTile + 1];   // As[k][i] = A(i0 + i, k0 + k)
+  __shared__ double Bs[kGemmTile][kGemmTile + 1];   // Bs[k][j] = B(k0 + k, j0 + j)
+  const int i0 = blockIdx.x * kGemmTile, j0 = blockIdx.y * kGemmTile;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
+  for (int k0 = 0; k0 < n; k0 += kGemmTile) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = threadIdx.x + kBlock * r;
+      const int lo = e & 31, hi = e >> 5;           // lo runs along the contiguous index
+      const int ai = i0 + lo, ak = k0 + hi;          // A(ai, ak) at A[ak n + ai]
+      As[hi][lo] = (ai < n && ak < n) ? A[(size_t)ak * n + ai] : 0.0;
+      const int bk = k0 + lo, bj = j0 + hi;          // B(bk, bj) at B[bj n + bk]
+      Bs[lo][hi] = (bk < n && bj < n) ? B[(size_t)bj * n + bk] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = 0; k < kGemmTile; ++k) {
+      const double a0 = As[k][tx], a1 = As[k][tx + 16];
+      const double b0 = Bs[k][ty], b1 = Bs[k][ty + 16];
+      c00 = fma(a0, b0, c00);
+      c01 = fma(a0, b1, c01);
+      c10 = fma(a1, b0, c10);
+      c11 = fma(a1, b1, c11);
+    }
+    __syncthreads();
+  }
+  const double acc[2][2] = {{c00, c01}, {c10, c11}};
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
+      if (i < n && j < n) {
+        double* c = C + (size_t)j * n + i;
+        *c = beta == 0.0 ? alpha * acc[a][b] : fma(alpha, acc[a][b], beta * *c);
+      }
+    }
+}
+
+// out = max_i sum_j |A_ij| (one block; n x n column-major)
+__global__ void __launch_bounds__(kBlock)
+k_inf_norm(int n, const double* __restrict__ A, double* __restrict__ out) {
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    double r = 0.0;
+    for (int j = 0; j < n; ++j) r += fabs(A[(size_t)j * n + i]);
+    m = fmax(m, r);
+  }
+  __shared__ double sm[kBlock];
+  sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// X = I / *nrm (column-major n x n)
+__global__ void __launch_bounds__(kBlock)
+k_set_scaled_identity(int n, const double* __restrict__ nrm, double* __restrict__ X) {
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= (int64_t)n * n) return;
+  X[t] = (t % n == t / n) ? 1.0 / *nrm : 0.0;
 }
 
 // x = D b for a dense symmetric D (column-major): one wavefront per row i reads column i
@@ -2569,13 +2633,12 @@ struct mgpbd_ctx {
   int n_levels = 0, n_smooth = 2, coarse_mode = 0, coarse_sweeps = 50;
   int level_smooth[MGPBD_MAX_LEVELS] = {};   // smoothing steps per phase on level l (0: n_smooth)
   // dense coarsest-level inverse (coarse_mode == MGPBD_COARSE_CHOLESKY)
-  rocblas_handle blas = nullptr;
   double* dense = nullptr;
   size_t dense_cap = 0;
-  rocblas_int* dense_info = nullptr;
   bool dense_ready = false;
   // Newton-Schulz refresh of the coarsest inverse from the previous one: two steps
-  // X <- X + X (I - A X), kept when ||I - A X||_F < 0.1 afterwards, else a Cholesky inverse
+  // X <- X + X (I - A X), kept when ||I - A X||_F < 0.1 afterwards, else a full inversion
+  // (Newton-Schulz from I / ||A||_inf to convergence)
   bool ns_update = true;
   double* dA = nullptr;      // assembled coarsest matrix
   double* dR = nullptr;      // I - A X
@@ -2788,9 +2851,7 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (c->ev_in) (void)hipEventDestroy(c->ev_in);
   if (c->ev_out) (void)hipEventDestroy(c->ev_out);
   if (c->work) (void)hipStreamDestroy(c->work);
-  if (c->blas) (void)rocblas_destroy_handle(c->blas);
   if (c->dense) (void)hipFree(c->dense);
-  if (c->dense_info) (void)hipFree(c->dense_info);
   if (c->dA) (void)hipFree(c->dA);
   if (c->dR) (void)hipFree(c->dR);
   if (c->dX2) (void)hipFree(c->dX2);
@@ -3308,6 +3369,29 @@ int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from) {
   return 0;
 }
 
+// One Newton-Schulz step X <- X + X (I - A X) (T receives the new X; R = I - A X on exit).
+static void ns_step(mgpbd_ctx* ctx, int n, const double* X, double* T) {
+  hipStream_t s = ctx->stream;
+  const size_t need = (size_t)n * n;
+  const dim3 gg((n + kGemmTile - 1) / kGemmTile, (n + kGemmTile - 1) / kGemmTile);
+  k_set_identity<<<nblocks((int64_t)need), kBlock, 0, s>>>(n, ctx->dR);
+  k_dgemm<<<gg, kBlock, 0, s>>>(n, -1.0, ctx->dA, X, 1.0, ctx->dR);
+  (void)hipMemcpyAsync(T, X, need * sizeof(double), hipMemcpyDeviceToDevice, s);
+  k_dgemm<<<gg, kBlock, 0, s>>>(n, 1.0, X, ctx->dR, 1.0, T);
+}
+
+// ||I - A X||_F^2 into ctx->dnorm (queued)
+static void ns_residual(mgpbd_ctx* ctx, int n, const double* X) {
+  hipStream_t s = ctx->stream;
+  const size_t need = (size_t)n * n;
+  const dim3 gg((n + kGemmTile - 1) / kGemmTile, (n + kGemmTile - 1) / kGemmTile);
+  const int gq = std::min(nblocks((int64_t)need), 1024);
+  k_set_identity<<<nblocks((int64_t)need), kBlock, 0, s>>>(n, ctx->dR);
+  k_dgemm<<<gg, kBlock, 0, s>>>(n, -1.0, ctx->dA, X, 1.0, ctx->dR);
+  k_sumsq<<<gq, kBlock, 0, s>>>((int64_t)need, ctx->dR, ctx->ws);
+  k_fin_sum<<<1, kBlock, 0, s>>>(ctx->ws, gq, 0, 1, ctx->dnorm);
+}
+
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
   CTX_CHECK("mgpbd_mg_factor_coarsest");
   if (int e = check_levels(ctx)) return e;
@@ -3326,12 +3410,7 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
     ctx->dense_cap = need;
     ctx->dense_ready = false;
   }
-  if (!ctx->dense_info) MGPBD_HIP_TRY(hipMalloc(&ctx->dense_info, sizeof(rocblas_int)));
   if (!ctx->dnorm) MGPBD_HIP_TRY(hipMalloc(&ctx->dnorm, sizeof(double)));
-  if (!ctx->blas && rocblas_create_handle(&ctx->blas) != rocblas_status_success)
-    return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_create_handle failed");
-  if (rocblas_set_stream(ctx->blas, s) != rocblas_status_success)
-    return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_set_stream failed");
   k_dense_zero<<<nblocks((int64_t)n * n), kBlock, 0, s>>>((int64_t)n * n, ctx->dA);
   k_dense_from_sell<<<(n + 3) / 4, kBlock, 0, s>>>(n, L.slice_ptr, L.idx, L.val, ctx->dA);
   MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(assemble)");
@@ -3342,32 +3421,21 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
     refactor = true;
   }
   if (!refactor) {
-    // two Newton-Schulz steps from the previous inverse, X <- X + X (I - A X), then an a
-    // posteriori check: ||I - A X||_F < 0.1 bounds the spectral radius of I - A X, so the
-    // kept X is SPD with every eigenvalue of X A within 0.1 of 1
-    const int gq = std::min(nblocks((int64_t)need), 1024);
-    const double m1 = -1.0, p1 = 1.0;
+    // two Newton-Schulz steps from the previous inverse, then an a posteriori check:
+    // ||I - A X||_F < 0.1 bounds the spectral radius of I - A X, so the kept X is SPD with
+    // every eigenvalue of X A within 0.1 of 1
     double* X = ctx->dX2;
     double* T = ctx->dT;
     MGPBD_HIP_TRY(hipMemcpyAsync(X, ctx->dense, need * sizeof(double), hipMemcpyDeviceToDevice, s));
-    for (int step = 0; step <= 2; ++step) {
-      k_set_identity<<<nblocks((int64_t)need), kBlock, 0, s>>>(n, ctx->dR);
-      if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &m1,
-                        ctx->dA, n, X, n, &p1, ctx->dR, n) != rocblas_status_success)
-        return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_dgemm failed to launch");
-      if (step == 2) break;
-      MGPBD_HIP_TRY(hipMemcpyAsync(T, X, need * sizeof(double), hipMemcpyDeviceToDevice, s));
-      if (rocblas_dgemm(ctx->blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &p1,
-                        X, n, ctx->dR, n, &p1, T, n) != rocblas_status_success)
-        return mgpbd_fail(MGPBD_EINTERNAL, "rocblas_dgemm failed to launch");
+    for (int step = 0; step < 2; ++step) {
+      ns_step(ctx, n, X, T);
       std::swap(X, T);
     }
-    k_sumsq<<<gq, kBlock, 0, s>>>((int64_t)need, ctx->dR, ctx->ws);
-    k_fin_sum<<<1, kBlock, 0, s>>>(ctx->ws, gq, 0, 1, ctx->dnorm);
+    ns_residual(ctx, n, X);
     // accepted X is copied (not swapped: captured PCG graphs hold the address of ctx->dense)
     // on the device, without a host wait; a rejected one leaves the previous inverse (SPD, so
     // the V-cycle stays SPD; the verified PCG still reaches its tolerance) and the next
-    // refresh refactors
+    // refresh inverts from scratch
     if (!ctx->ns_flag) MGPBD_HIP_TRY(hipMalloc(&ctx->ns_flag, sizeof(int)));
     if (!ctx->ns_flag_host) MGPBD_HIP_TRY(hipHostMalloc(&ctx->ns_flag_host, sizeof(int)));
     if (!ctx->ns_event) MGPBD_HIP_TRY(hipEventCreateWithFlags(&ctx->ns_event, hipEventDisableTiming));
@@ -3380,25 +3448,40 @@ int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx) {
     ctx->ns_pending = true;
   }
   if (refactor) {
+    // Full inversion: Newton-Schulz from X0 = I / ||A||_inf. For SPD A every eigenvalue of
+    // X0 A lies in (0, 1], so ||I - X A||_2 < 1 and the iteration converges (quadratically
+    // once below 1). Rounds of four steps, each followed by a host check of ||I - A X||_F:
+    // done when it falls below 1e-10 or stops falling below 0.1 (rounding floor); an
+    // indefinite or singular level never converges (MGPBD_ENONFINITE after 64 steps).
     ++ctx->ns_refactors;
-    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, ctx->dA, need * sizeof(double),
-                                 hipMemcpyDeviceToDevice, s));
-    if (rocsolver_dpotrf(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
-        rocblas_status_success)
-      return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotrf failed to launch");
-    rocblas_int info = 0;
-    MGPBD_HIP_TRY(hipMemcpyAsync(&info, ctx->dense_info, sizeof(info), hipMemcpyDeviceToHost, s));
-    MGPBD_HIP_TRY(hipStreamSynchronize(s));
-    if (info != 0) {
-      ctx->dense_ready = false;
-      return mgpbd_fail(MGPBD_ENONFINITE, "coarsest level is not positive definite (dpotrf info " +
-                                              std::to_string(info) + ")");
+    double* X = ctx->dX2;
+    double* T = ctx->dT;
+    k_inf_norm<<<1, kBlock, 0, s>>>(n, ctx->dA, ctx->dnorm);
+    k_set_scaled_identity<<<nblocks((int64_t)need), kBlock, 0, s>>>(n, ctx->dnorm, X);
+    double prev = INFINITY, f = INFINITY;
+    bool done = false;
+    for (int round = 0; round < 16 && !done; ++round) {
+      for (int step = 0; step < 4; ++step) {
+        ns_step(ctx, n, X, T);
+        std::swap(X, T);
+      }
+      ns_residual(ctx, n, X);
+      MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(inversion)");
+      double f2 = 0.0;
+      MGPBD_HIP_TRY(hipMemcpyAsync(&f2, ctx->dnorm, sizeof(double), hipMemcpyDeviceToHost, s));
+      MGPBD_HIP_TRY(hipStreamSynchronize(s));
+      f = std::sqrt(f2);
+      if (!std::isfinite(f) || f > 1e6) break;
+      done = f < 1e-10 || (f < 0.1 && f > 0.5 * prev);
+      prev = f;
     }
-    if (rocsolver_dpotri(ctx->blas, rocblas_fill_lower, n, ctx->dense, n, ctx->dense_info) !=
-        rocblas_status_success)
-      return mgpbd_fail(MGPBD_EINTERNAL, "rocsolver_dpotri failed to launch");
-    k_symmetrize_lower<<<nblocks((int64_t)n * n), kBlock, 0, s>>>(n, ctx->dense);
-    MGPBD_LAUNCH_CHECK("mgpbd_mg_factor_coarsest(symmetrize)");
+    if (!done) {
+      ctx->dense_ready = false;
+      return mgpbd_fail(MGPBD_ENONFINITE, "coarsest level: the Newton-Schulz inverse did not "
+                                          "converge (not positive definite?), ||I - A X||_F = " +
+                                              std::to_string(f));
+    }
+    MGPBD_HIP_TRY(hipMemcpyAsync(ctx->dense, X, need * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   ctx->dense_ready = true;
   return 0;

@@ -580,7 +580,7 @@ class GPUMultigridSolver:
         return int(a.value), int(b.value)
 
     def coarse_stats(self):
-        """(Newton-Schulz steps kept, Cholesky refactorizations) of the coarsest inverse."""
+        """(Newton-Schulz refresh steps kept, full inversions) of the coarsest inverse."""
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self.ctx("mgpbd_mg_coarse_stats", ctypes.byref(a), ctypes.byref(b))
         return int(a.value), int(b.value)

@@ -333,11 +333,14 @@ int mgpbd_mg_set_level(mgpbd_ctx* ctx, int level, int n_rows, const int32_t* sli
                        const float* diag_inv_dev, const int32_t* agg_dev,
                        const int32_t* mem_ptr_dev, const int32_t* mem_dev, float* work_dev,
                        int diagonal_only);
-/* Coarsest level with MGPBD_COARSE_CHOLESKY: assemble it densely in FP64, factor and invert
- * (rocSOLVER dpotrf/dpotri), keep the symmetric inverse. Call after the coarsest values
- * change (every coarse refill). A Cholesky refactorization synchronises (MGPBD_ENONFINITE if
- * not positive definite); a Newton-Schulz refresh does not. No reference counterpart (the
- * reference smooths the coarsest level, mgpcg_gpu.py:216-224). */
+/* Coarsest level with MGPBD_COARSE_CHOLESKY: assemble it densely in FP64 and keep its
+ * inverse (library FP64 GEMM kernels, no vendor library): a full inversion runs Newton-Schulz
+ * X <- X + X (I - A X) from X = I / ||A||_inf until ||I - A X||_F stops falling (below 1e-10,
+ * or at its rounding floor below 0.1), checked by the host every four steps (it synchronises;
+ * MGPBD_ENONFINITE when it does not converge within 64 steps: A not positive definite); a
+ * refresh updates the previous inverse without a wait (mgpbd_mg_set_coarse_update). Call
+ * after the coarsest values change. No reference counterpart (the reference smooths the
+ * coarsest level, mgpcg_gpu.py:216-224). */
 int mgpbd_mg_factor_coarsest(mgpbd_ctx* ctx);
 /* Registers the coarse refill of level `level` + 1 from level `level` (the arguments of
  * mgpbd_coarse_refill) with the coarse level's diagonal positions and 1/diag buffer, for
@@ -356,14 +359,15 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
  * Cholesky refactorization synchronises, as in mgpbd_mg_factor_coarsest). */
 int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from);
 /* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level
- * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse (rocBLAS dgemm)
- * and the device keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue
- * of X A lies within 0.1 of 1 and the V-cycle stays an SPD preconditioner). Otherwise, or when
- * the iteration diverged, the previous inverse stays for the solves that follow (SPD, so the
- * V-cycle stays SPD) and the next refresh refactors with Cholesky; the host learns the outcome
- * at that refresh, without waiting for it in this one. */
+ * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse and the device
+ * keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue of X A lies
+ * within 0.1 of 1 and the V-cycle stays an SPD preconditioner). Otherwise, or when the
+ * iteration diverged, the previous inverse stays for the solves that follow (SPD, so the
+ * V-cycle stays SPD) and the next refresh inverts from scratch; the host learns the outcome at
+ * that refresh, without waiting for it in this one. newton_schulz = 0: every refresh inverts
+ * from scratch. */
 int mgpbd_mg_set_coarse_update(mgpbd_ctx* ctx, int newton_schulz);
-/* Counts since the context was created: Newton-Schulz steps kept, Cholesky refactorizations. */
+/* Counts since the context was created: Newton-Schulz refresh steps kept, full inversions. */
 int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors);
 /* Smoothing steps per phase on one level (extension; the reference uses n_smooth on every
  * level, cons/mgpcg_gpu.py:232-246): steps in [1, 8], 0 restores n_smooth. Coarse levels are
