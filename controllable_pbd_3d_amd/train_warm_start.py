@@ -12,7 +12,8 @@ the GPU with the A-norm loss (x0 - x*)^T A (x0 - x*) / x*^T A x* of the start x0
 converged dlambda x* (the quantity that bounds CG's iterations). Evaluation on a held-out seed
 solves every substep's first system twice, from the previous lambda and from the learned start
 (the device keeps whichever of prediction / previous lambda has the smaller residual), and
-also runs the whole sequence once per start to time it.
+also runs the whole sequence once per start to time it, under the strong preconditioner the
+solver uses and under the reference's weak V-cycle (PRECONDITIONERS).
 
   python -m controllable_pbd_3d_amd.train_warm_start --n 32 --seeds 0-3 --frames 25 \\
       --eval-seed 100 --eval-frames 10 --out warm_start_box32.npz
@@ -40,19 +41,36 @@ def workload(n: int, h: float, seed: int):
     return mesh, mu_t
 
 
+# preconditioners the start is evaluated under (the solution, and so the training target, is
+# the same for both): "chebyshev", the extended hierarchy with Chebyshev smoothing and the
+# dense coarsest solve (each PCG iteration contracts the error ~20x or more); "two_level", the
+# reference's hierarchy (tet aggregation, two levels, 50 coarsest sweeps, FP32) with 2 + 2
+# weighted-Jacobi sweeps whose weights come from the spectral estimates (the reference's fixed
+# 2/3 does not contract on every level of these systems), a weak preconditioner that needs
+# tens of iterations per solve
+PRECONDITIONERS = {
+    "chebyshev": dict(n_smooth=5, hierarchy="extended", galerkin="structural", max_coarse=64,
+                      coarse_solver="cholesky", smoother="chebyshev", cheb_ratio=30.0,
+                      smoother_safety=1.2, smooth_schedule=[5, 1], maxiter_cg=100),
+    "two_level": dict(n_smooth=2, hierarchy="reference", galerkin="values",
+                      coarse_solver="jacobi", smoother="spectral", precision="fp32",
+                      maxiter_cg=1000),
+}
+
+
 def make_solver(mesh, mu_t, max_levels, warm_start="none", model=None, record=None,
-                solver_cls=MGPBDSolverGPU, precision="fp16"):
+                solver_cls=MGPBDSolverGPU, precision="fp16", precond="chebyshev"):
+    kw = dict(max_levels=max_levels, precision=precision)
+    kw.update(PRECONDITIONERS[precond])
+    if precond == "two_level":
+        kw.pop("max_levels")
     s = solver_cls(v_p=mesh.verts.copy(), v_p_ref=mesh.verts.copy(), v_invm=mesh.invm.copy(),
                    t_i=mesh.tets.copy(), t_m=mesh.tet_mass.copy(), gravity=(0.0, -9.8, 0.0),
-                   dt=1.0 / 300.0, maxiter=20, atol=1e-4, rtol=1e-2, maxiter_cg=100,
-                   n_smooth=5, omega_jacobi=2.0 / 3.0, mu_per_tet=mu_t, hold_pinned=True,
-                   relative_tol=True, tol_cg=1e-6, hierarchy="extended", galerkin="structural",
-                   max_coarse=64, max_levels=max_levels, coarse_solver="cholesky",
-                   precision=precision, smoother="chebyshev", cheb_ratio=30.0,
-                   smoother_safety=1.2, smooth_schedule=[5, 1], verify_residual=1,
-                   fp64_iterate=True,
+                   dt=1.0 / 300.0, maxiter=20, atol=1e-4, rtol=1e-2,
+                   omega_jacobi=2.0 / 3.0, mu_per_tet=mu_t, hold_pinned=True,
+                   relative_tol=True, tol_cg=1e-6, verify_residual=1, fp64_iterate=True,
                    warm_start=warm_start, warm_start_model=model, record_samples=record,
-                   verbose=False)
+                   verbose=False, **kw)
     s.init()
     return s
 
@@ -83,11 +101,13 @@ class _Shadow(MGPBDSolverGPU):
         return it
 
 
-def evaluate(model, n, h, max_levels, seed, frames, warmup_frames=1, box=0.5):
+def evaluate(model, n, h, max_levels, seed, frames, warmup_frames=1, box=0.5,
+             precond="chebyshev"):
     mesh, mu_t = workload(n, h, seed)
-    out = {}
+    out = {"preconditioner": precond}
     # 1. the same systems from both starts
-    s = make_solver(mesh, mu_t, max_levels, "learned", model, solver_cls=_Shadow)
+    s = make_solver(mesh, mu_t, max_levels, "learned", model, solver_cls=_Shadow,
+                    precond=precond)
     s.shadow = []
     run_sequence(s, mesh, seed, frames, box=box)
     sh = s.shadow[5 * warmup_frames:]
@@ -102,7 +122,8 @@ def evaluate(model, n, h, max_levels, seed, frames, warmup_frames=1, box=0.5):
     del s
     # 2. whole sequences, timed
     for mode in ("previous", "learned"):
-        s = make_solver(mesh, mu_t, max_levels, mode, model if mode == "learned" else None)
+        s = make_solver(mesh, mu_t, max_levels, mode, model if mode == "learned" else None,
+                        precond=precond)
         run_sequence(s, mesh, seed, warmup_frames, box=box)
         stats = []
         torch.cuda.synchronize()
@@ -168,6 +189,9 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=100)
     ap.add_argument("--eval-frames", type=int, default=10)
     ap.add_argument("--out", default=None, help="checkpoint (.npz) to write")
+    ap.add_argument("--precond", default="chebyshev,two_level",
+                    help="comma list of preconditioners to evaluate under: " +
+                         ",".join(PRECONDITIONERS))
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
     seeds = parse_seeds(args.seeds)
@@ -185,8 +209,8 @@ def main(argv=None):
     torch.cuda.empty_cache()
     if args.out:
         model.save(args.out)
-    ev = evaluate(model, args.n, args.h, args.max_levels, args.eval_seed, args.eval_frames,
-                  box=args.box)
+    ev = {p: evaluate(model, args.n, args.h, args.max_levels, args.eval_seed, args.eval_frames,
+                      box=args.box, precond=p) for p in args.precond.split(",")}
     out = {"workload": f"box{args.n}_{mesh0.n_tets}tets_drag", "train_seeds": seeds,
            "frames_per_seed": args.frames, "pairs": n_pairs, "drag_box_m": args.box,
            "generate_s": gen_s, "train_s": train_s, "epochs": args.epochs,

@@ -327,9 +327,15 @@ def test_spring_drag_force():
 def test_learned_start_gains_over_previous_lambda():
     """Train on the spring-drag workload (8^3 cubes, two seeds), evaluate on a held-out seed.
     The previous-lambda start is the method without the network (an untrained network, zero
-    decoder, predicts exactly it). Gain shown on the same systems: the learned start's initial
-    residual and its first-solve PCG iterations against the previous lambda's; every solve still
-    reaches the FP64-verified 1e-6."""
+    decoder, predicts exactly it). Gains shown on the same held-out systems: the A-norm error
+    of the start (the quantity that bounds CG) falls below 0.8x the untrained network's, the
+    learned start's initial residual is below the previous lambda's, and under a weak
+    preconditioner (the reference's two-level V-cycle with 2 + 2 Jacobi sweeps: tens of PCG
+    iterations per solve) the first solve takes fewer iterations.
+    Under the strong preconditioner (Chebyshev smoothing, dense coarsest solve: each PCG
+    iteration contracts the error ~20x or more) the start saves less than one iteration, so the
+    first solve's iteration count only must not grow. Every solve reaches the FP64-verified
+    1e-6 either way."""
     from controllable_pbd_3d_amd import train_warm_start as tw
     from controllable_pbd_3d_amd.warm_start import LearnedWarmStart
     n, h, levels, box = 8, 0.03, 3, 0.1
@@ -338,18 +344,23 @@ def test_learned_start_gains_over_previous_lambda():
     samples = tw.generate_pairs(n, h, levels, [0, 1], 8, model, box=box)
     hist, before, after = tw.train_model(model, samples, epochs=8, n_heldout=20)
     del samples
-    ev = tw.evaluate(model, n, h, levels, seed=100, frames=5, box=box)
-    p = ev["paired"]
-    print(f"[learned gain] held-out A-norm loss {before:.4e} -> {after:.4e}; paired first "
-          f"solve: previous {p['first_solve_iters_previous']:.2f} it (r0 "
-          f"{p['initial_rel_residual_previous_median']:.3e}), learned "
-          f"{p['first_solve_iters_learned']:.2f} it (r0 "
-          f"{p['initial_rel_residual_learned_median']:.3e}), starts {p['starts_taken']}; "
-          f"timed previous {ev['previous']} learned {ev['learned']}", flush=True)
+    evs = {p: tw.evaluate(model, n, h, levels, seed=100, frames=5, box=box, precond=p)
+           for p in ("chebyshev", "two_level")}
+    for p, ev in evs.items():
+        q = ev["paired"]
+        print(f"[learned gain] {p}: held-out A-norm loss {before:.4e} -> {after:.4e}; paired "
+              f"first solve: previous {q['first_solve_iters_previous']:.2f} it (r0 "
+              f"{q['initial_rel_residual_previous_median']:.3e}), learned "
+              f"{q['first_solve_iters_learned']:.2f} it (r0 "
+              f"{q['initial_rel_residual_learned_median']:.3e}), starts {q['starts_taken']}; "
+              f"timed previous {ev['previous']} learned {ev['learned']}", flush=True)
+        assert ev["learned"]["fp64_rel_residual_max"] < 1e-6
+        assert ev["previous"]["fp64_rel_residual_max"] < 1e-6
+        assert q["initial_rel_residual_learned_median"] < q["initial_rel_residual_previous_median"]
     assert after < 0.8 * before
-    assert p["first_solve_iters_learned"] <= p["first_solve_iters_previous"] + 0.25
-    assert ev["learned"]["fp64_rel_residual_max"] < 1e-6
-    assert ev["previous"]["fp64_rel_residual_max"] < 1e-6
+    strong, weak = evs["chebyshev"]["paired"], evs["two_level"]["paired"]
+    assert strong["first_solve_iters_learned"] <= strong["first_solve_iters_previous"]
+    assert weak["first_solve_iters_learned"] < weak["first_solve_iters_previous"]
 
 
 @pytest.mark.gpu
