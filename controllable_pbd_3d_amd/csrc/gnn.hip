@@ -274,31 +274,44 @@ k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
   }
 }
 
-// out[e] (+)= sum over the nblk partial rows of column e. A block owns `cols` = min(len, 64)
-// consecutive columns and splits the partial rows among G = 256 / cols thread groups
-// (group g sums rows g, g + G, ...); the group totals are added in group order through LDS,
-// so the result is the same every run. Grid: ceil(len / 64) blocks.
 __global__ void __launch_bounds__(kBlock)
-k_sum_partials(int nblk, int len, const float* __restrict__ partial, float* __restrict__ out,
-               int accumulate) {
+k_sum_partials(int nblk, int len, int slice, int stride, const float* partial, float* out) {
+  // block (x, y): columns [64x, 64x + 64) summed over rows r in [y slice, min(nblk, (y + 1)
+  // slice)), row r at partial + r stride len; 256 / 64 = 4 row groups (fixed order), then the
+  // groups in order. The result goes to out + y slice stride len (in place into the slice's
+  // first row when out == partial).
   __shared__ float sh[kBlock];
   const int cols = min(len, 64);
   const int G = kBlock / cols;
   const int c = threadIdx.x % cols, g = threadIdx.x / cols;
   const int e = blockIdx.x * 64 + c;
+  const int r0 = blockIdx.y * slice, r1 = min(nblk, r0 + slice);
   float s = 0.0f;
   if (g < G && e < len)
-    for (int b = g; b < nblk; b += G) s += partial[(size_t)b * len + e];
+    for (int r = r0 + g; r < r1; r += G) s += partial[(size_t)r * stride * len + e];
   sh[threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.x < cols && e < len) {
     float t = 0.0f;
     for (int q = 0; q < G; ++q) t += sh[q * cols + threadIdx.x];
-    out[e] = accumulate ? out[e] + t : t;
+    out[(size_t)r0 * stride * len + e] = t;
   }
 }
 
-static inline int sum_partials_grid(int len) { return (len + 63) / 64; }
+// out[0 .. len) = sum over the nblk rows of partial (nblk x len, row-major), deterministic:
+// slices of 32 rows summed in place into each slice's first row by parallel blocks, then those
+// rows summed (a single pass when nblk <= 32). partial is overwritten.
+static void sum_partials(hipStream_t s, int nblk, int len, float* partial, float* out) {
+  const int gx = (len + 63) / 64;
+  constexpr int kSlice = 32;
+  if (nblk <= kSlice) {
+    k_sum_partials<<<dim3(gx, 1), kBlock, 0, s>>>(nblk, len, nblk, 1, partial, out);
+    return;
+  }
+  const int ns = (nblk + kSlice - 1) / kSlice;
+  k_sum_partials<<<dim3(gx, ns), kBlock, 0, s>>>(nblk, len, kSlice, 1, partial, partial);
+  k_sum_partials<<<dim3(gx, 1), kBlock, 0, s>>>(ns, len, ns, kSlice, partial, out);
+}
 
 // ------------------------------------------------------------------------------------------
 // Elementwise / small kernels
@@ -941,9 +954,9 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   const int ncs = (n + 1023) / 1024;
   // decoder: d w_out = sum_i dy_i h3_i, d b_out = sum_i dy_i, dh3 = dy w_out^T
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, nullptr, nullptr, dy, W.h[NL], W.partial);
-  k_sum_partials<<<sum_partials_grid(H), kBlock, 0, s>>>(ncs, H, W.partial, grad + off_wout(), 0);
+  sum_partials(s, ncs, H, W.partial, grad + off_wout());
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, 1, nullptr, dy, nullptr, nullptr, W.partial);
-  k_sum_partials<<<sum_partials_grid(1), kBlock, 0, s>>>(ncs, 1, W.partial, grad + off_bout(), 0);
+  sum_partials(s, ncs, 1, W.partial, grad + off_bout());
   k_decoder_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, dy, params, W.dh);
   const int gg = (n + 127) / 128;
   for (int l = NL - 1; l >= 0; --l) {
@@ -951,11 +964,11 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
     k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[l + 1], W.dz);
     // weight and bias gradients
     k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.h[l], W.dz, W.partial);
-    k_sum_partials<<<sum_partials_grid(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l), 0);
+    sum_partials(s, nblk, H * H, W.partial, grad + off_wl(l));
     k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.m[l + 1], W.dz, W.partial);
-    k_sum_partials<<<sum_partials_grid(H * H), kBlock, 0, s>>>(nblk, H * H, W.partial, grad + off_wl(l) + H * H, 0);
+    sum_partials(s, nblk, H * H, W.partial, grad + off_wl(l) + H * H);
     k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
-    k_sum_partials<<<sum_partials_grid(H), kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bl(l), 0);
+    sum_partials(s, ncs, H, W.partial, grad + off_bl(l));
     // input gradients: dh_l = dZ W_self^T + mean-backward(dZ W_nbr^T)
     const uint16_t* w_self = w + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = w + (size_t)(2 * l + 1) * H * H;
@@ -967,9 +980,9 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   // encoder: dZ0 = dh0 * (h0 > 0); dW_in = x^T dZ0; db_in = sum dZ0
   k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[0], W.dz);
   k_encoder_wgrad<<<ncs, kBlock, 0, s>>>(n, x, W.dz, W.partial);
-  k_sum_partials<<<sum_partials_grid(F * H), kBlock, 0, s>>>(ncs, F * H, W.partial, grad + off_win(), 0);
+  sum_partials(s, ncs, F * H, W.partial, grad + off_win());
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
-  k_sum_partials<<<sum_partials_grid(H), kBlock, 0, s>>>(ncs, H, W.partial, grad + off_bin(), 0);
+  sum_partials(s, ncs, H, W.partial, grad + off_bin());
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_backward");
   return 0;
 }
@@ -1015,7 +1028,7 @@ int mgpbd_gnn_wgrad64(mgpbd_ctx* ctx, int n, const uint16_t* A, const uint16_t* 
   hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
   k_wgrad<<<nblk, kBlock, 0, s>>>(n, A, dZ, partial);
-  k_sum_partials<<<sum_partials_grid(H * H), kBlock, 0, s>>>(nblk, H * H, partial, G, 0);
+  sum_partials(s, nblk, H * H, partial, G);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_wgrad64");
   return 0;
 }

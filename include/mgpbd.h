@@ -680,7 +680,7 @@ int mgpbd_gnn_pack(mgpbd_ctx* ctx, const float* params_dev, uint16_t* wt_dev, ui
 /* The MFMA building blocks on their own: Y (n x 64 FP32) = act(A1 W1 + A2 W2 + bias) with A*
  * BF16 n x 64 row-major and W*t BF16 [64 out][64 in] (A2/W2t/bias may be NULL, act = relu
  * when relu != 0); and G (64 x 64 FP32) = A^T dZ over n rows (partial_dev:
- * 4096 * ceil(n / 1024) floats). */
+ * 4096 * ceil(n / 1024) floats, overwritten). */
 int mgpbd_gnn_gemm64(mgpbd_ctx* ctx, int n, const uint16_t* A1_dev, const uint16_t* W1t_dev,
                      const uint16_t* A2_dev, const uint16_t* W2t_dev, const float* bias_dev,
                      int relu, float* Y_dev);
