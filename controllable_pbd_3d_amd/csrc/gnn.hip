@@ -339,29 +339,48 @@ k_mean_nbr(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ 
   for (int e = 0; e < 4; ++e) m[(size_t)i * H + c + e] = f2bf(s[e] * inv);
 }
 
-// Backward of the mean: dh_j += sum_{i in N(j)} dm_i / deg_i (the face graph is symmetric),
-// written into dh (FP32) on top of its current value.
+// Backward of the mean and of the ReLU below it: with dh_j = dh_j + sum_{i in N(j)} dm_i /
+// deg_i (the face graph is symmetric; FP32), dz_j = dh_j where h_j > 0, else 0, stored BF16
+// (the next layer down's dZ). Eight columns per thread (16-byte BF16 loads and stores).
 __global__ void __launch_bounds__(kBlock)
-k_mean_nbr_bwd(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ dm,
-               float* __restrict__ dh) {
+k_mean_nbr_relu_bwd(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ dm,
+                    const float* __restrict__ dh, const uint16_t* __restrict__ h,
+                    uint16_t* __restrict__ dz) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int j = t >> 4;
-  const int c = 4 * (t & 15);
+  const int j = t >> 3;
+  const int c = 8 * (t & 7);
   if (j >= n) return;
-  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const int4 nj = *reinterpret_cast<const int4*>(nbr + 4 * (size_t)j);
+  const int ids[4] = {nj.x, nj.y, nj.z, nj.w};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int i = nbr[4 * (size_t)j + q];
+    const int i = ids[q];
     if (i < 0) continue;
-    int d = 0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) d += nbr[4 * (size_t)i + u] >= 0;
+    const int4 ni = *reinterpret_cast<const int4*>(nbr + 4 * (size_t)i);
+    const int d = (ni.x >= 0) + (ni.y >= 0) + (ni.z >= 0) + (ni.w >= 0);
     const float inv = 1.0f / (float)d;
+    const uint4 v = *reinterpret_cast<const uint4*>(dm + (size_t)i * H + c);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) s[e] = fmaf(bf2f(dm[(size_t)i * H + c + e]), inv, s[e]);
+    for (int e = 0; e < 4; ++e) {
+      s[2 * e] = fmaf(bf2f((uint16_t)w[e]), inv, s[2 * e]);
+      s[2 * e + 1] = fmaf(bf2f((uint16_t)(w[e] >> 16)), inv, s[2 * e + 1]);
+    }
   }
+  const float4 a = *reinterpret_cast<const float4*>(dh + (size_t)j * H + c);
+  const float4 b = *reinterpret_cast<const float4*>(dh + (size_t)j * H + c + 4);
+  const float g[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  const uint4 hv = *reinterpret_cast<const uint4*>(h + (size_t)j * H + c);
+  const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
+  uint32_t o[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) dh[(size_t)j * H + c + e] += s[e];
+  for (int e = 0; e < 4; ++e) {
+    const float lo = bf2f((uint16_t)hw[e]) > 0.0f ? g[2 * e] + s[2 * e] : 0.0f;
+    const float hi = bf2f((uint16_t)(hw[e] >> 16)) > 0.0f ? g[2 * e + 1] + s[2 * e + 1] : 0.0f;
+    o[e] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+  }
+  *reinterpret_cast<uint4*>(dz + (size_t)j * H + c) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // h0 = relu(x W_in + b_in), FP32 weights, BF16 output
@@ -451,23 +470,25 @@ k_decoder(int n, const uint16_t* __restrict__ h3, const float* __restrict__ P,
   if (lane == 0) y[i] = v + P[off_bout()];
 }
 
-// dh3[i][c] = dy_i w_out[c]; masked by h3 > 0 later with the layer's ReLU
+// Decoder and top-layer ReLU backward: dz3[i][c] = dy_i w_out[c] where h3 > 0, else 0 (BF16),
+// eight columns per thread
 __global__ void __launch_bounds__(kBlock)
-k_decoder_bwd(int n, const float* __restrict__ dy, const float* __restrict__ P,
-              float* __restrict__ dh3) {
+k_decoder_relu_bwd(int n, const float* __restrict__ dy, const float* __restrict__ P,
+                   const uint16_t* __restrict__ h3, uint16_t* __restrict__ dz) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int i = t >> 6, c = t & 63;
+  const int i = t >> 3, c = 8 * (t & 7);
   if (i >= n) return;
-  dh3[(size_t)i * H + c] = dy[i] * P[off_wout() + c];
-}
-
-// dZ = dH (FP32) masked by h > 0, stored BF16
-__global__ void __launch_bounds__(kBlock)
-k_relu_bwd(int n, const float* __restrict__ dh, const uint16_t* __restrict__ h,
-           uint16_t* __restrict__ dz) {
-  const size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= (size_t)n * H) return;
-  dz[e] = f2bf(bf2f(h[e]) > 0.0f ? dh[e] : 0.0f);
+  const float d = dy[i];
+  const uint4 hv = *reinterpret_cast<const uint4*>(h3 + (size_t)i * H + c);
+  const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
+  uint32_t o[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float lo = bf2f((uint16_t)hw[e]) > 0.0f ? d * P[off_wout() + c + 2 * e] : 0.0f;
+    const float hi = bf2f((uint16_t)(hw[e] >> 16)) > 0.0f ? d * P[off_wout() + c + 2 * e + 1] : 0.0f;
+    o[e] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+  }
+  *reinterpret_cast<uint4*>(dz + (size_t)i * H + c) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
 // column sums over rows (bias / decoder gradients), block partials: P[b][c]
@@ -756,7 +777,6 @@ struct GnnWorkspace {
   uint16_t* m[NL + 1];
   float* y;
   float* dh;
-  float* dh2;
   uint16_t* dz;
   uint16_t* dmb;
   float* partial;   // max(blocks) * 64 * 64
@@ -778,7 +798,6 @@ static void carve(void* ws, int n, GnnWorkspace& W) {
   for (int l = 1; l <= NL; ++l) W.m[l] = (uint16_t*)take(act);
   W.y = (float*)take((size_t)n * 4);
   W.dh = (float*)take((size_t)n * H * 4);
-  W.dh2 = (float*)take((size_t)n * H * 4);
   W.dz = (uint16_t*)take(act);
   W.dmb = (uint16_t*)take(act);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
@@ -808,7 +827,7 @@ static size_t workspace_bytes(int n) {
   const size_t f4 = ((size_t)n * 4 + 255) / 256 * 256;
   const size_t fh = ((size_t)n * H * 4 + 255) / 256 * 256;
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
-  return act * (2 * NL + 1) + f4 + 2 * fh + 2 * act +
+  return act * (2 * NL + 1) + f4 + fh + 2 * act +
          (((size_t)std::max(nblk, 1) * H * H * 4 + 255) / 256 * 256) +
          (((size_t)std::max(nb(n), 1) * 64 + 255) / 256 * 256) + 256;
 }
@@ -957,11 +976,10 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   sum_partials(s, ncs, H, W.partial, grad + off_wout());
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, 1, nullptr, dy, nullptr, nullptr, W.partial);
   sum_partials(s, ncs, 1, W.partial, grad + off_bout());
-  k_decoder_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, dy, params, W.dh);
+  // dZ of the top layer = (dy w_out^T) * (h3 > 0)
+  k_decoder_relu_bwd<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, dy, params, W.h[NL], W.dz);
   const int gg = (n + 127) / 128;
   for (int l = NL - 1; l >= 0; --l) {
-    // dZ = dh_{l+1} * (h_{l+1} > 0)
-    k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[l + 1], W.dz);
     // weight and bias gradients
     k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.h[l], W.dz, W.partial);
     sum_partials(s, nblk, H * H, W.partial, grad + off_wl(l));
@@ -969,16 +987,15 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
     sum_partials(s, nblk, H * H, W.partial, grad + off_wl(l) + H * H);
     k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
     sum_partials(s, ncs, H, W.partial, grad + off_bl(l));
-    // input gradients: dh_l = dZ W_self^T + mean-backward(dZ W_nbr^T)
+    // input gradients: dh_l = dZ W_self^T + mean-backward(dZ W_nbr^T), then the ReLU of h_l:
+    // dZ of the layer below (of the encoder for l = 0) = dh_l * (h_l > 0)
     const uint16_t* w_self = w + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = w + (size_t)(2 * l + 1) * H * H;
     k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_nbr, nullptr, nullptr, nullptr, 0, W.dmb, nullptr);
-    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_self, nullptr, nullptr, nullptr, 0, nullptr, W.dh2);
-    k_mean_nbr_bwd<<<nb((int64_t)n * 16), kBlock, 0, s>>>(n, nbr, W.dmb, W.dh2);
-    std::swap(W.dh, W.dh2);
+    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_self, nullptr, nullptr, nullptr, 0, nullptr, W.dh);
+    k_mean_nbr_relu_bwd<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, nbr, W.dmb, W.dh, W.h[l], W.dz);
   }
-  // encoder: dZ0 = dh0 * (h0 > 0); dW_in = x^T dZ0; db_in = sum dZ0
-  k_relu_bwd<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, W.dh, W.h[0], W.dz);
+  // encoder: dW_in = x^T dZ0; db_in = sum dZ0
   k_encoder_wgrad<<<ncs, kBlock, 0, s>>>(n, x, W.dz, W.partial);
   sum_partials(s, ncs, F * H, W.partial, grad + off_win());
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
