@@ -105,7 +105,7 @@ k_gemm64(int n, const uint16_t* __restrict__ A1, const uint16_t* __restrict__ W1
 }
 
 // Inference layer with the neighbour mean fused in: Y = relu(H W_self + mean_nbr(H) W_nbr + b)
-// over 32-row MFMA tiles, with the arithmetic of k_mean_nbr + k_gemm64 (same fragments, same
+// over 32-row MFMA tiles, with the arithmetic of a mean pass followed by k_gemm64 (same fragments, same
 // MFMA sequence), so the output equals theirs bit for bit without the mean's n x 64 round trip
 // through memory. Memory side: each block stages both weight matrices in LDS once and its
 // wavefronts loop over tiles; a tile's own rows are loaded as 16-byte row chunks (eight lanes
@@ -134,7 +134,8 @@ __device__ __forceinline__ v8u16 lds_load8(const uint16_t* src) {
 __global__ void __launch_bounds__(kBlock)
 k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ Hin,
              const uint16_t* __restrict__ Wst, const uint16_t* __restrict__ Wnt,
-             const float* __restrict__ bias, uint16_t* __restrict__ Hout) {
+             const float* __restrict__ bias, uint16_t* __restrict__ Hout,
+             uint16_t* __restrict__ Mout) {
   __shared__ __align__(16) uint16_t wl[2][H * kLdsRow];
   __shared__ __align__(16) uint16_t tl[kTileWaves][2][32 * kLdsRow];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -195,6 +196,8 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
       v8u16 mv = {}, hz = {};
 #pragma unroll
       for (int e = 0; e < 8; ++e) mv[e] = f2bf(m[e] * inv);
+      if (Mout && live)   // the training forward keeps the mean for the weight gradient
+        *reinterpret_cast<v8u16*>(Mout + (size_t)(base + r) * H + 8 * ch) = mv;
       lds_store8(ts + r * kLdsRow + 8 * ch, live ? hv[q] : hz);
       lds_store8(tm + r * kLdsRow + 8 * ch, live ? mv : hz);
     }
@@ -316,29 +319,6 @@ static void sum_partials(hipStream_t s, int nblk, int len, float* partial, float
 // ------------------------------------------------------------------------------------------
 // Elementwise / small kernels
 // ------------------------------------------------------------------------------------------
-// m_i = mean of h_j over the face neighbours (BF16 in/out, FP32 sum in neighbour order)
-__global__ void __launch_bounds__(kBlock)
-k_mean_nbr(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ h,
-           uint16_t* __restrict__ m) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;   // one thread per (row, 4 columns)
-  const int i = t >> 4;
-  const int c = 4 * (t & 15);
-  if (i >= n) return;
-  float s[4] = {0.f, 0.f, 0.f, 0.f};
-  int d = 0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int j = nbr[4 * (size_t)i + q];
-    if (j < 0) continue;
-    ++d;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) s[e] += bf2f(h[(size_t)j * H + c + e]);
-  }
-  const float inv = d > 0 ? 1.0f / (float)d : 0.0f;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) m[(size_t)i * H + c + e] = f2bf(s[e] * inv);
-}
-
 // Backward of the mean and of the ReLU below it: with dh_j = dh_j + sum_{i in N(j)} dm_i /
 // deg_i (the face graph is symmetric; FP32), dz_j = dh_j where h_j > 0, else 0, stored BF16
 // (the next layer down's dZ). Eight columns per thread (16-byte BF16 loads and stores).
@@ -383,20 +363,8 @@ k_mean_nbr_relu_bwd(int n, const int32_t* __restrict__ nbr, const uint16_t* __re
   *reinterpret_cast<uint4*>(dz + (size_t)j * H + c) = make_uint4(o[0], o[1], o[2], o[3]);
 }
 
-// h0 = relu(x W_in + b_in), FP32 weights, BF16 output
-__global__ void __launch_bounds__(kBlock)
-k_encoder(int n, const float* __restrict__ x, const float* __restrict__ P,
-          uint16_t* __restrict__ h0) {
-  const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int i = t >> 6, c = t & 63;
-  if (i >= n) return;
-  float v = P[off_bin() + c];
-#pragma unroll
-  for (int k = 0; k < F; ++k) v = fmaf(x[(size_t)i * F + k], P[off_win() + k * H + c], v);
-  h0[(size_t)i * H + c] = f2bf(fmaxf(v, 0.0f));
-}
-
-// h0 for 8 channels per thread (same FP32 fma order as k_encoder), one 16-byte store
+// h0 = relu(x W_in + b_in) (FP32 weights, fma over the 8 features in order), 8 channels per
+// thread, one 16-byte BF16 store
 __global__ void __launch_bounds__(kBlock)
 k_encoder8(int n, const float* __restrict__ x, const float* __restrict__ P,
            uint16_t* __restrict__ h0) {
@@ -864,14 +832,15 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
   hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
   GnnWorkspace W;
   carve(ws, n, W);
-  k_encoder<<<nb((int64_t)n * H), kBlock, 0, s>>>(n, x, params, W.h[0]);
-  const int gg = (n + 127) / 128;   // 4 wavefronts x 32 rows per block
+  // the inference kernels, keeping every layer's input and neighbour mean for the backward
+  // pass
+  k_encoder8<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, x, params, W.h[0]);
+  const int gg = std::min((n + 32 * kTileWaves - 1) / (32 * kTileWaves), kLayerBlocks);
   for (int l = 0; l < NL; ++l) {
-    k_mean_nbr<<<nb((int64_t)n * 16), kBlock, 0, s>>>(n, nbr, W.h[l], W.m[l + 1]);
     const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
-    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.h[l], w_self, W.m[l + 1], w_nbr, params + off_bl(l), 1,
-                                   W.h[l + 1], nullptr);
+    k_layer_mean<<<gg, kBlock, 0, s>>>(n, nbr, W.h[l], w_self, w_nbr, params + off_bl(l),
+                                       W.h[l + 1], W.m[l + 1]);
   }
   k_decoder<<<nb((int64_t)n * 64), kBlock, 0, s>>>(n, W.h[NL], params, W.y);
   if (dlam_out) k_predict<<<nb(n), kBlock, 0, s>>>(n, W.y, lam_prev, stats, dlam_out);
@@ -893,7 +862,7 @@ int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
     const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
     k_layer_mean<<<gg, kBlock, 0, s>>>(n, nbr, W.h[l], w_self, w_nbr, params + off_bl(l),
-                                       W.h[l + 1]);
+                                       W.h[l + 1], nullptr);
   }
   k_decoder_predict<<<nb(n), kBlock, 0, s>>>(n, W.h[NL], params, lam_prev, stats, W.y, dlam_out);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_infer");
