@@ -241,6 +241,7 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
 // (A, dZ BF16 n x 64). Each of the 4 wavefronts owns one 32x32 quadrant of the 64x64 result;
 // the MFMA inner dimension runs over rows.
 constexpr int kWgradRows = 1024;
+constexpr int kColRows = 256;   // rows per block of the column-sum and encoder-gradient passes
 
 __device__ __forceinline__ v4i16 load4_strided(const uint16_t* p, int stride, int valid) {
   v4i16 v;
@@ -464,12 +465,12 @@ __global__ void __launch_bounds__(kBlock)
 k_colsum_partial(int n, int width, const uint16_t* __restrict__ Zbf, const float* __restrict__ Zf,
                  const float* __restrict__ rowscale, const uint16_t* __restrict__ Sbf,
                  float* __restrict__ partial) {
-  // out[c] = sum_i (Z[i][c] or rowscale_i * S[i][c]) over the block's 1024 rows: G = 256 /
+  // out[c] = sum_i (Z[i][c] or rowscale_i * S[i][c]) over the block's kColRows rows: G = 256 /
   // width row groups (group g takes rows g, g + G, ...), group totals added in group order
   __shared__ float sh[kBlock];
   const int G = kBlock / width;
   const int c = threadIdx.x % width, g = threadIdx.x / width;
-  const int r0 = blockIdx.x * 1024, r1 = min(n, r0 + 1024);
+  const int r0 = blockIdx.x * kColRows, r1 = min(n, r0 + kColRows);
   float s = 0.0f;
   if (g < G)
     for (int i = r0 + g; i < r1; i += G) {
@@ -488,22 +489,33 @@ k_colsum_partial(int n, int width, const uint16_t* __restrict__ Zbf, const float
   }
 }
 
-// encoder weight gradient partials: P[b][k][c] = sum_i x[i][k] dz0[i][c]
+// encoder weight gradient partials: P[b][k][c] = sum_i x[i][k] dz0[i][c] over the block's
+// kColRows rows: thread (g, c) takes column c of rows g, g + 4, ... for all eight k (the row's
+// features as two float4 loads), then the four groups are added in group order through LDS
 __global__ void __launch_bounds__(kBlock)
 k_encoder_wgrad(int n, const float* __restrict__ x, const uint16_t* __restrict__ dz0,
                 float* __restrict__ partial) {
-  const int e = threadIdx.x;   // 256 threads: k = e / 64 (+4), c = e % 64
-  const int r0 = blockIdx.x * 1024, r1 = min(n, r0 + 1024);
-  const int c = e & 63;
-  const int k0 = e >> 6;       // 0..3
-  float s0 = 0.f, s1 = 0.f;
-  for (int i = r0; i < r1; ++i) {
+  __shared__ float sh[F][kBlock];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * kColRows, r1 = min(n, r0 + kColRows);
+  float acc[F] = {};
+  for (int i = r0 + g; i < r1; i += 4) {
     const float dz = bf2f(dz0[(size_t)i * H + c]);
-    s0 = fmaf(x[(size_t)i * F + k0], dz, s0);
-    s1 = fmaf(x[(size_t)i * F + k0 + 4], dz, s1);
+    const float4 xa = reinterpret_cast<const float4*>(x)[2 * (size_t)i];
+    const float4 xb = reinterpret_cast<const float4*>(x)[2 * (size_t)i + 1];
+    const float xi[F] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+#pragma unroll
+    for (int k = 0; k < F; ++k) acc[k] = fmaf(xi[k], dz, acc[k]);
   }
-  partial[(size_t)blockIdx.x * F * H + k0 * H + c] = s0;
-  partial[(size_t)blockIdx.x * F * H + (k0 + 4) * H + c] = s1;
+#pragma unroll
+  for (int k = 0; k < F; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int o = threadIdx.x; o < F * H; o += kBlock) {
+    const int k = o / H, cc = o % H;
+    float t = 0.0f;
+    for (int q = 0; q < 4; ++q) t += sh[k][64 * q + cc];
+    partial[(size_t)blockIdx.x * F * H + o] = t;
+  }
 }
 
 // Adam on the FP32 parameters, then BF16 copies of the message-passing weights:
@@ -939,7 +951,7 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   GnnWorkspace W;
   carve(ws, n, W);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
-  const int ncs = (n + 1023) / 1024;
+  const int ncs = (n + kColRows - 1) / kColRows;
   // decoder: d w_out = sum_i dy_i h3_i, d b_out = sum_i dy_i, dh3 = dy w_out^T
   k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, nullptr, nullptr, dy, W.h[NL], W.partial);
   sum_partials(s, ncs, H, W.partial, grad + off_wout());
