@@ -252,22 +252,38 @@ __device__ __forceinline__ v4i16 load4_strided(const uint16_t* p, int stride, in
   return v;
 }
 
+// With A2 (nullable) the same block also forms P2 = A2^T dZ from the dZ fragments it loaded
+// for P, and with Pb (nullable) the column sums of dZ over its rows (the bias gradient): the
+// two waves with qk = 0 add the dZ values of their fragments (each lane four rows of one
+// column), and lanes l and l + 32 (the two row halves) combine in a fixed order.
 __global__ void __launch_bounds__(kBlock)
 k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
-        float* __restrict__ partial) {
+        float* __restrict__ partial, const uint16_t* __restrict__ A2,
+        float* __restrict__ partial2, float* __restrict__ Pb) {
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int qk = w >> 1, qj = w & 1;
   const int r0 = blockIdx.x * kWgradRows;
   const int r1 = min(n, r0 + kWgradRows);
   const int kq = 4 * (lane >> 5);
-  v16f32 acc = {};
+  v16f32 acc = {}, acc2 = {};
+  float cs = 0.0f;
   for (int i0 = r0; i0 < r1; i0 += 8) {
     const int i = i0 + kq;
     const int valid = r1 - i;
     const v4i16 a = load4_strided(A + (size_t)i * H + 32 * qk + (lane & 31), H, valid);
     const v4i16 b = load4_strided(dZ + (size_t)i * H + 32 * qj + (lane & 31), H, valid);
     acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    if (A2) {
+      const v4i16 a2 = load4_strided(A2 + (size_t)i * H + 32 * qk + (lane & 31), H, valid);
+      acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b, acc2, 0, 0, 0);
+    }
+    if (Pb && qk == 0) {
+      cs += bf2f((uint16_t)b.x);
+      cs += bf2f((uint16_t)b.y);
+      cs += bf2f((uint16_t)b.z);
+      cs += bf2f((uint16_t)b.w);
+    }
   }
   float* P = partial + (size_t)blockIdx.x * H * H;
 #pragma unroll
@@ -275,6 +291,19 @@ k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
     const int k = 32 * qk + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
     const int j = 32 * qj + (lane & 31);
     P[k * H + j] = acc[r];
+  }
+  if (A2) {
+    float* P2 = partial2 + (size_t)blockIdx.x * H * H;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int k = 32 * qk + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+      const int j = 32 * qj + (lane & 31);
+      P2[k * H + j] = acc2[r];
+    }
+  }
+  if (Pb && qk == 0) {
+    const float other = __shfl_xor(cs, 32, 64);
+    if (lane < 32) Pb[(size_t)blockIdx.x * H + 32 * qj + lane] = cs + other;
   }
 }
 
@@ -760,6 +789,8 @@ struct GnnWorkspace {
   uint16_t* dz;
   uint16_t* dmb;
   float* partial;   // max(blocks) * 64 * 64
+  float* partial2;  // the second weight gradient's block partials (same size)
+  float* partialb;  // the bias gradient's block partials, blocks * 64
   double* dpartial;
   double* scal;
 };
@@ -782,6 +813,8 @@ static void carve(void* ws, int n, GnnWorkspace& W) {
   W.dmb = (uint16_t*)take(act);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
   W.partial = (float*)take((size_t)std::max(nblk, 1) * H * H * 4);
+  W.partial2 = (float*)take((size_t)std::max(nblk, 1) * H * H * 4);
+  W.partialb = (float*)take((size_t)std::max(nblk, 1) * H * 4);
 }
 
 static void carve_infer(void* ws, int n, GnnWorkspace& W) {
@@ -808,7 +841,8 @@ static size_t workspace_bytes(int n) {
   const size_t fh = ((size_t)n * H * 4 + 255) / 256 * 256;
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
   return act * (2 * NL + 1) + f4 + fh + 2 * act +
-         (((size_t)std::max(nblk, 1) * H * H * 4 + 255) / 256 * 256) +
+         2 * (((size_t)std::max(nblk, 1) * H * H * 4 + 255) / 256 * 256) +
+         (((size_t)std::max(nblk, 1) * H * 4 + 255) / 256 * 256) +
          (((size_t)std::max(nb(n), 1) * 64 + 255) / 256 * 256) + 256;
 }
 
@@ -961,13 +995,12 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
   k_decoder_relu_bwd<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, dy, params, W.h[NL], W.dz);
   const int gg = (n + 127) / 128;
   for (int l = NL - 1; l >= 0; --l) {
-    // weight and bias gradients
-    k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.h[l], W.dz, W.partial);
+    // weight and bias gradients: one pass over dZ for W_self, W_nbr and b
+    k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.h[l], W.dz, W.partial, W.m[l + 1], W.partial2,
+                                    W.partialb);
     sum_partials(s, nblk, H * H, W.partial, grad + off_wl(l));
-    k_wgrad<<<nblk, kBlock, 0, s>>>(n, W.m[l + 1], W.dz, W.partial);
-    sum_partials(s, nblk, H * H, W.partial, grad + off_wl(l) + H * H);
-    k_colsum_partial<<<ncs, kBlock, 0, s>>>(n, H, W.dz, nullptr, nullptr, nullptr, W.partial);
-    sum_partials(s, ncs, H, W.partial, grad + off_bl(l));
+    sum_partials(s, nblk, H * H, W.partial2, grad + off_wl(l) + H * H);
+    sum_partials(s, nblk, H, W.partialb, grad + off_bl(l));
     // input gradients: dh_l = dZ W_self^T + mean-backward(dZ W_nbr^T), then the ReLU of h_l:
     // dZ of the layer below (of the encoder for l = 0) = dh_l * (h_l > 0)
     const uint16_t* w_self = w + (size_t)(2 * l) * H * H;
@@ -1025,7 +1058,7 @@ int mgpbd_gnn_wgrad64(mgpbd_ctx* ctx, int n, const uint16_t* A, const uint16_t* 
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_wgrad64: invalid arguments");
   hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
   const int nblk = (n + kWgradRows - 1) / kWgradRows;
-  k_wgrad<<<nblk, kBlock, 0, s>>>(n, A, dZ, partial);
+  k_wgrad<<<nblk, kBlock, 0, s>>>(n, A, dZ, partial, nullptr, nullptr, nullptr);
   sum_partials(s, nblk, H * H, partial, G);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_wgrad64");
   return 0;
