@@ -104,6 +104,43 @@ k_gemm64(int n, const uint16_t* __restrict__ A1, const uint16_t* __restrict__ W1
   }
 }
 
+// Both input-gradient products of a layer from one read of dZ: Y1 (BF16) = A W1 and
+// Y2 (FP32) = A W2, each with the MFMA sequence of k_gemm64 (bit-identical to two calls).
+__global__ void __launch_bounds__(kBlock)
+k_gemm64_dual(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ W1t,
+              const uint16_t* __restrict__ W2t, uint16_t* __restrict__ Y1bf,
+              float* __restrict__ Y2f) {
+  const int lane = threadIdx.x & 63;
+  const int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
+  if (base >= n) return;
+  const int ra = base + (lane & 31);
+  const int kq = 4 * (lane >> 5);
+  v16f32 acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
+  const v4i16 zero = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < H / 8; ++s) {
+    const v4i16 a = ra < n ? load4(A + (size_t)ra * H + 8 * s + kq) : zero;
+    const v4i16 b0 = load4(W1t + (size_t)(lane & 31) * H + 8 * s + kq);
+    const v4i16 b1 = load4(W1t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
+    const v4i16 b2 = load4(W2t + (size_t)(lane & 31) * H + 8 * s + kq);
+    const v4i16 b3 = load4(W2t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
+    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b2, acc2, 0, 0, 0);
+    acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b3, acc3, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = base + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+    if (row >= n) continue;
+    const int col = lane & 31;
+    Y1bf[(size_t)row * H + col] = f2bf(acc0[r] + 0.0f);
+    Y1bf[(size_t)row * H + 32 + col] = f2bf(acc1[r] + 0.0f);
+    Y2f[(size_t)row * H + col] = acc2[r] + 0.0f;
+    Y2f[(size_t)row * H + 32 + col] = acc3[r] + 0.0f;
+  }
+}
+
 // Inference layer with the neighbour mean fused in: Y = relu(H W_self + mean_nbr(H) W_nbr + b)
 // over 32-row MFMA tiles, with the arithmetic of a mean pass followed by k_gemm64 (same fragments, same
 // MFMA sequence), so the output equals theirs bit for bit without the mean's n x 64 round trip
@@ -1005,8 +1042,7 @@ int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x
     // dZ of the layer below (of the encoder for l = 0) = dh_l * (h_l > 0)
     const uint16_t* w_self = w + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = w + (size_t)(2 * l + 1) * H * H;
-    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_nbr, nullptr, nullptr, nullptr, 0, W.dmb, nullptr);
-    k_gemm64<<<gg, kBlock, 0, s>>>(n, W.dz, w_self, nullptr, nullptr, nullptr, 0, nullptr, W.dh);
+    k_gemm64_dual<<<gg, kBlock, 0, s>>>(n, W.dz, w_nbr, w_self, W.dmb, W.dh);
     k_mean_nbr_relu_bwd<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, nbr, W.dmb, W.dh, W.h[l], W.dz);
   }
   // encoder: dW_in = x^T dZ0; db_in = sum dZ0
