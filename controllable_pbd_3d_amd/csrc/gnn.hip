@@ -280,47 +280,72 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
 constexpr int kWgradRows = 1024;
 constexpr int kColRows = 256;   // rows per block of the column-sum and encoder-gradient passes
 
-__device__ __forceinline__ v4i16 load4_strided(const uint16_t* p, int stride, int valid) {
-  v4i16 v;
-  v.x = valid > 0 ? (short)p[0] : (short)0;
-  v.y = valid > 1 ? (short)p[stride] : (short)0;
-  v.z = valid > 2 ? (short)p[2 * stride] : (short)0;
-  v.w = valid > 3 ? (short)p[3 * stride] : (short)0;
-  return v;
-}
-
 // With A2 (nullable) the same block also forms P2 = A2^T dZ from the dZ fragments it loaded
 // for P, and with Pb (nullable) the column sums of dZ over its rows (the bias gradient): the
 // two waves with qk = 0 add the dZ values of their fragments (each lane four rows of one
-// column), and lanes l and l + 32 (the two row halves) combine in a fixed order.
+// column), and lanes l and l + 32 (the two row halves) combine in a fixed order. The rows go
+// through LDS 64 at a time (16-byte row-chunk loads, rows padded to 68 halves); the MFMA
+// sequence over 8-row steps is the same as reading the fragments from memory.
+constexpr int kWgChunk = 64;
 __global__ void __launch_bounds__(kBlock)
 k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
         float* __restrict__ partial, const uint16_t* __restrict__ A2,
         float* __restrict__ partial2, float* __restrict__ Pb) {
+  __shared__ __align__(16) uint16_t sa[kWgChunk * kLdsRow];
+  __shared__ __align__(16) uint16_t sa2[kWgChunk * kLdsRow];
+  __shared__ __align__(16) uint16_t sz[kWgChunk * kLdsRow];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int qk = w >> 1, qj = w & 1;
   const int r0 = blockIdx.x * kWgradRows;
   const int r1 = min(n, r0 + kWgradRows);
   const int kq = 4 * (lane >> 5);
+  const int ca = 32 * qk + (lane & 31), cz = 32 * qj + (lane & 31);
   v16f32 acc = {}, acc2 = {};
   float cs = 0.0f;
-  for (int i0 = r0; i0 < r1; i0 += 8) {
-    const int i = i0 + kq;
-    const int valid = r1 - i;
-    const v4i16 a = load4_strided(A + (size_t)i * H + 32 * qk + (lane & 31), H, valid);
-    const v4i16 b = load4_strided(dZ + (size_t)i * H + 32 * qj + (lane & 31), H, valid);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-    if (A2) {
-      const v4i16 a2 = load4_strided(A2 + (size_t)i * H + 32 * qk + (lane & 31), H, valid);
-      acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b, acc2, 0, 0, 0);
+  const v8u16 zero8 = {};
+  for (int c0 = r0; c0 < r1; c0 += kWgChunk) {
+    for (int q = threadIdx.x; q < kWgChunk * 8; q += kBlock) {
+      const int rr = q >> 3, ch = q & 7, row = c0 + rr;
+      const bool in = row < r1;
+      lds_store8(sa + rr * kLdsRow + 8 * ch,
+                 in ? *reinterpret_cast<const v8u16*>(A + (size_t)row * H + 8 * ch) : zero8);
+      lds_store8(sz + rr * kLdsRow + 8 * ch,
+                 in ? *reinterpret_cast<const v8u16*>(dZ + (size_t)row * H + 8 * ch) : zero8);
+      if (A2)
+        lds_store8(sa2 + rr * kLdsRow + 8 * ch,
+                   in ? *reinterpret_cast<const v8u16*>(A2 + (size_t)row * H + 8 * ch) : zero8);
     }
-    if (Pb && qk == 0) {
-      cs += bf2f((uint16_t)b.x);
-      cs += bf2f((uint16_t)b.y);
-      cs += bf2f((uint16_t)b.z);
-      cs += bf2f((uint16_t)b.w);
+    __syncthreads();
+    const int rows = min(kWgChunk, r1 - c0);
+    for (int s0 = 0; s0 < rows; s0 += 8) {
+      const int i = s0 + kq;
+      v4i16 a, b;
+      a.x = (short)sa[(i + 0) * kLdsRow + ca];
+      a.y = (short)sa[(i + 1) * kLdsRow + ca];
+      a.z = (short)sa[(i + 2) * kLdsRow + ca];
+      a.w = (short)sa[(i + 3) * kLdsRow + ca];
+      b.x = (short)sz[(i + 0) * kLdsRow + cz];
+      b.y = (short)sz[(i + 1) * kLdsRow + cz];
+      b.z = (short)sz[(i + 2) * kLdsRow + cz];
+      b.w = (short)sz[(i + 3) * kLdsRow + cz];
+      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      if (A2) {
+        v4i16 a2;
+        a2.x = (short)sa2[(i + 0) * kLdsRow + ca];
+        a2.y = (short)sa2[(i + 1) * kLdsRow + ca];
+        a2.z = (short)sa2[(i + 2) * kLdsRow + ca];
+        a2.w = (short)sa2[(i + 3) * kLdsRow + ca];
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b, acc2, 0, 0, 0);
+      }
+      if (Pb && qk == 0) {
+        cs += bf2f((uint16_t)b.x);
+        cs += bf2f((uint16_t)b.y);
+        cs += bf2f((uint16_t)b.z);
+        cs += bf2f((uint16_t)b.w);
+      }
     }
+    __syncthreads();   // the chunk's LDS rows are free for the next chunk
   }
   float* P = partial + (size_t)blockIdx.x * H * H;
 #pragma unroll
