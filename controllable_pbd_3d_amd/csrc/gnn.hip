@@ -488,8 +488,9 @@ k_encoder8(int n, const float* __restrict__ x, const float* __restrict__ P,
   reinterpret_cast<uint4*>(h0 + (size_t)i * H)[t & 7] = pk;
 }
 
-// dlambda = lam_prev + s (h3 w_out + b_out), one row per thread; the 64 products are summed in
-// the pairing order of k_decoder's butterfly (lane 0's view), so y matches it bit for bit
+// dlambda = lam_prev + s (h3 w_out + b_out) (and y = h3 w_out + b_out), one row per thread; the
+// 64 products are summed pairwise, c + 32, then c + 16, ... (the order of a 64-lane xor
+// butterfly seen from lane 0)
 __global__ void __launch_bounds__(kBlock)
 k_decoder_predict(int n, const uint16_t* __restrict__ h3, const float* __restrict__ P,
                   const float* __restrict__ lp, const float* __restrict__ st,
@@ -515,19 +516,6 @@ k_decoder_predict(int n, const uint16_t* __restrict__ h3, const float* __restric
   const float yi = p[0] + P[off_bout()];
   if (y) y[i] = yi;
   if (out) out[i] = (lp ? lp[i] : 0.0f) + st[0] * yi;
-}
-
-// y = h3 w_out + b_out
-__global__ void __launch_bounds__(kBlock)
-k_decoder(int n, const uint16_t* __restrict__ h3, const float* __restrict__ P,
-          float* __restrict__ y) {
-  const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (i >= n) return;
-  float v = bf2f(h3[(size_t)i * H + lane]) * P[off_wout() + lane];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if (lane == 0) y[i] = v + P[off_bout()];
 }
 
 // Decoder and top-layer ReLU backward: dz3[i][c] = dy_i w_out[c] where h3 > 0, else 0 (BF16),
@@ -726,16 +714,8 @@ k_features(int n, const float* __restrict__ b, const float* __restrict__ dinv,
   o[7] = 1.0f;
 }
 
-// prediction dlambda = s (lp/s + y) = lp + s y ; loss gradient dy = 2 (y - t) / n with the
-// normalised target t = (dlambda* - lp) / s; per-block partial of the squared error
-__global__ void __launch_bounds__(kBlock)
-k_predict(int n, const float* __restrict__ y, const float* __restrict__ lp,
-          const float* __restrict__ st, float* __restrict__ out) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  out[i] = (lp ? lp[i] : 0.0f) + st[0] * y[i];
-}
-
+// squared-error loss on the normalised output: dy = 2 (y - t) / n with t = (dlambda* - lp) / s,
+// and per-block partials of the squared error
 __global__ void __launch_bounds__(kBlock)
 k_loss_mse(int n, const float* __restrict__ y, const float* __restrict__ target,
            const float* __restrict__ lp, const float* __restrict__ st, float* __restrict__ dy,
@@ -950,8 +930,8 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
     k_layer_mean<<<gg, kBlock, 0, s>>>(n, nbr, W.h[l], w_self, w_nbr, params + off_bl(l),
                                        W.h[l + 1], W.m[l + 1]);
   }
-  k_decoder<<<nb((int64_t)n * 64), kBlock, 0, s>>>(n, W.h[NL], params, W.y);
-  if (dlam_out) k_predict<<<nb(n), kBlock, 0, s>>>(n, W.y, lam_prev, stats, dlam_out);
+  k_decoder_predict<<<nb(n), kBlock, 0, s>>>(n, W.h[NL], params, lam_prev, stats, W.y,
+                                             dlam_out);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_forward");
   return 0;
 }
