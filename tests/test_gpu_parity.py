@@ -67,6 +67,27 @@ def test_structure_matches_oracle(scene):
     assert np.array_equal(C.indptr, Ac.indptr) and np.array_equal(C.indices, Ac.indices)
 
 
+@pytest.mark.parametrize("per_tet_mu", [False, True])
+def test_rest_shape_matches_fp64_oracle(per_tet_mu):
+    """k_rest_shape on its own (the later kernel tests copy the oracle's Dm^-1 and alpha~ in):
+    Dm^-1, the rest volume and alpha~ = 1 / (mu dt^2 vol) are evaluated in FP64 and rounded once,
+    so each equals the FP64 oracle's value rounded to FP32 within one ulp (jittered block, per-tet
+    mu from E log-uniform in [1e4, 1e6] when per_tet_mu)."""
+    from tests.scenarios import physical_block
+    mesh, mu_t = physical_block(6, seed=19)
+    kw = dict(mu_per_tet=mu_t) if per_tet_mu else {}
+    s = _solver(mesh, None, **kw)
+    o = _oracle(mesh, None, precision="f64", **kw)
+    got = {"Dm_inv": s.Dm_inv.cpu().numpy().reshape(-1), "rest_vol": s.rest_vol.cpu().numpy(),
+           "alpha_tilde": s.alpha_tilde.cpu().numpy()}
+    for name, g in got.items():
+        ref = getattr(o, name).astype(np.float32)
+        np.testing.assert_array_max_ulp(g, ref, maxulp=1)
+    assert np.all(got["rest_vol"] > 0)
+    print(f"[rest shape] per_tet_mu={per_tet_mu}: {mesh.n_tets} tets within 1 ulp of FP64",
+          flush=True)
+
+
 def test_constraints_vs_fp64_svd(scene):
     mesh, v0 = scene
     s = _solver(mesh, v0)
