@@ -2255,6 +2255,40 @@ k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_
     fac_gather_lane<VT, false>(v, q, vs_ptr, vs_cnt, vs_inc, 0, vs_k, x, u);
 }
 
+// The vertex pass on the uniform layout: every slice R rows tall (slot k of vertex v at
+// (v / 16) 64 R + 64 (k / 4) + 4 (v % 16) + k % 4) and the tets in the K words' offsets, so a
+// lane issues its R K-word loads and the base-tet load at once, with no slice pointer or
+// count to wait for. Slots past a vertex's count hold K = 0 and offset 0 (never written), so
+// they add exact zeros after the lane's own slots: the sums equal k_fac_gather's bit for bit.
+template <int VT, int R>
+__global__ void __launch_bounds__(kBlock)
+k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ vs_k,
+               const float* __restrict__ x, float4* __restrict__ u,
+               const int* __restrict__ done) {
+  if (done && *done) return;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;
+  const int base = (v >> 4) * (64 * R) + ((v & 15) << 2) + q;
+  const int tb = tbase[v];
+  v2u32 h[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) h[j] = __builtin_nontemporal_load(vs_k + base + 64 * j);
+  float xv[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) xv[j] = x[tb + (int)(h[j].y >> 16)];
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    ax = fmaf(lo_h<VT>(h[j].x), xv[j], ax);
+    ay = fmaf(hi_h<VT>(h[j].x), xv[j], ay);
+    az = fmaf(lo_h<VT>(h[j].y), xv[j], az);
+  }
+  ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
+  ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
+  if (q == 0) u[v] = make_float4(ax, ay, az, 0.0f);
+}
+
 // (A^ x)_t from u: a^_t x_t + K_t0 . u_v0 + ... + K_t3 . u_v3 (FP32)
 template <int VT>
 __device__ __forceinline__ float fac_row(int t, int n, const int4* __restrict__ t_i,
@@ -2400,6 +2434,42 @@ k_fac_ogather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs
     fac_ogather_lane<XT, true>(v, q, vs_ptr, vs_cnt, vs_inc, tbase[v], vs_g, invm, x, u);
   else
     fac_ogather_lane<XT, false>(v, q, vs_ptr, vs_cnt, vs_inc, 0, vs_g, invm, x, u);
+}
+
+// k_fac_ogather on the uniform layout (see k_fac_gather_u; padded slots hold zero gradients
+// and offset 0, so the FP64 sums are the per-slice kernel's bit for bit).
+template <typename XT, int R>
+__global__ void __launch_bounds__(kBlock)
+k_fac_ogather_u(int nv, const int* __restrict__ tbase, const float4* __restrict__ vs_g,
+                const float* __restrict__ invm, const XT* __restrict__ x,
+                double4* __restrict__ u, const int* __restrict__ done,
+                const PcgState* __restrict__ gate_st) {
+  if (done && *done) return;
+  if (gate_st && (gate_st->done != 1 || gate_st->verified)) return;
+  const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int v = gid >> 2, q = gid & 3;
+  if (v >= nv) return;
+  const int base = (v >> 4) * (64 * R) + ((v & 15) << 2) + q;
+  const int tb = tbase[v];
+  float4 g[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) g[j] = vs_g[base + 64 * j];
+  double xv[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) xv[j] = (double)x[tb + (int)__float_as_uint(g[j].w)];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    ax = fma((double)g[j].x, xv[j], ax);
+    ay = fma((double)g[j].y, xv[j], ay);
+    az = fma((double)g[j].z, xv[j], az);
+  }
+  ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
+  ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
+  if (q == 0) {
+    const double w = (double)invm[v];
+    u[v] = make_double4(w * ax, w * ay, w * az, 0.0);
+  }
 }
 
 // dpos_v = w_v sum_t dlambda_t g_tv over the vertex's incidences (FP32, four lanes per vertex
@@ -2696,6 +2766,7 @@ struct mgpbd_ctx {
     const int* vs_cnt = nullptr;
     const int* vs_inc = nullptr;
     const int* tbase = nullptr;   // per-vertex base tet of the 16-bit slot offsets (or NULL)
+    int rows_u = 0;               // rows of the uniform vertex-slice layout (0: per slice)
     v2u32* vs_k = nullptr;   // K per vertex slot (4 halves)
     float4* vs_g = nullptr;  // FP32 gradient per vertex slot (the PCG's operator)
     v2u32* tk = nullptr;     // K per tet, three planes of n_tets
@@ -3599,14 +3670,30 @@ static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float
 #undef SELLW
 }
 
+// The factored level's vertex pass u = K^T x on the context stream: the uniform-layout kernel
+// when the layout is uniform, else the per-slice one (same sums, bit for bit).
+template <int VT>
+static void fac_gather(mgpbd_ctx* ctx, const float* x, const int* done) {
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  hipStream_t s = ctx->stream;
+  const int g = nblocks((int64_t)kFacLanes * F.nv);
+  switch (F.rows_u) {
+#define FGU(R) case R: k_fac_gather_u<VT, R><<<g, kBlock, 0, s>>>(F.nv, F.tbase, F.vs_k, x, F.u, done); return;
+    FGU(1) FGU(2) FGU(3) FGU(4) FGU(5) FGU(6) FGU(7) FGU(8)
+#undef FGU
+    default:
+      k_fac_gather<VT><<<g, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k,
+                                            x, F.u, done);
+  }
+}
+
 template <int VT>
 static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const float* b,
                        const int* done, const float* wp, const float* xp, const float* cc) {
   const mgpbd_ctx::Factored& F = ctx->fac;
   const MgLevel& L = ctx->L[0];
   hipStream_t s = ctx->stream;
-  k_fac_gather<VT><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
-      F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k, x, F.u, done);
+  fac_gather<VT>(ctx, x, done);
   const int g = nblocks(L.n);
   const float w = ctx->omega;
 #define FAC(M) k_fac_apply<VT, M><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b, w, \
@@ -3789,12 +3876,10 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     const bool fac = l == 0 && F.on && ctx->precision != MGPBD_PREC_FP32;
     for (int it = 0; it < iters; ++it) {
       if (fac && ctx->precision == MGPBD_PREC_FP16) {
-        k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
-                                                         F.tbase, F.vs_k, v, F.u, nullptr);
+        fac_gather<1>(ctx, v, nullptr);
         k_fac_power<1><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
       } else if (fac) {
-        k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc,
-                                                         F.tbase, F.vs_k, v, F.u, nullptr);
+        fac_gather<2>(ctx, v, nullptr);
         k_fac_power<2><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
       } else if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
         k_power_step_wf<<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
@@ -3984,12 +4069,16 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
   return 0;
 }
 
-int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase) {
+int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase, int rows_uniform) {
   CTX_CHECK("mgpbd_mg_set_level0_offsets");
   mgpbd_ctx::Factored& F = ctx->fac;
   if (!F.on) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_offsets: no factored level 0");
+  if (rows_uniform < 0 || rows_uniform > 8 || (rows_uniform > 0 && !tbase))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_offsets: rows_uniform in [0, 8], and "
+                                    "a uniform layout needs the base tets");
   drop_graphs(ctx);
   F.tbase = tbase;
+  F.rows_u = rows_uniform;
   return 0;
 }
 
@@ -4146,12 +4235,8 @@ int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b, const float*
   const mgpbd_ctx::Factored& F = ctx->fac;
   auto first = [&]() {   // the factored vertex pass, or a whole stored-level sweep
     if (fac) {
-      if (ctx->precision == MGPBD_PREC_FP16)
-        k_fac_gather<1><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
-            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k, x, F.u, nullptr);
-      else
-        k_fac_gather<2><<<nblocks((int64_t)kFacLanes * F.nv), kBlock, 0, s>>>(
-            F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k, x, F.u, nullptr);
+      if (ctx->precision == MGPBD_PREC_FP16) fac_gather<1>(ctx, x, nullptr);
+      else fac_gather<2>(ctx, x, nullptr);
     } else if (half_mode(ctx)) {
       sellh(ctx, L, 1, x, y, b, nullptr, nullptr);
     } else {
@@ -4236,6 +4321,22 @@ int mgpbd_mg_level_residual(mgpbd_ctx* ctx, int level, const float* b, const flo
   return 0;
 }
 
+// The PCG operator's vertex pass (uniform-layout kernel when the layout is uniform).
+extern "C++" template <typename XT>
+static void fac_ogather(mgpbd_ctx* ctx, const XT* x, const int* done, const PcgState* gate) {
+  const mgpbd_ctx::Factored& F = ctx->fac;
+  hipStream_t s = ctx->stream;
+  const int gv = nblocks((int64_t)kFacLanes * F.nv);
+  switch (F.rows_u) {
+#define FOU(R) case R: k_fac_ogather_u<XT, R><<<gv, kBlock, 0, s>>>(F.nv, F.tbase, F.vs_g, F.invm, x, F.u64, done, gate); return;
+    FOU(1) FOU(2) FOU(3) FOU(4) FOU(5) FOU(6) FOU(7) FOU(8)
+#undef FOU
+    default:
+      k_fac_ogather<XT><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_g,
+                                              F.invm, x, F.u64, done, gate);
+  }
+}
+
 // The PCG's level-0 operator in factored form (vertex pass, tet pass), modes of k_fac_oapply;
 // x in FP32 (xf) or FP64 (xd); grid gs >= the row blocks (its partials when mode != 1).
 static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* xd,
@@ -4244,16 +4345,13 @@ static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* x
   const mgpbd_ctx::Factored& F = ctx->fac;
   const int n = ctx->L[0].n;
   hipStream_t s = ctx->stream;
-  const int gv = nblocks((int64_t)kFacLanes * F.nv);
   if (xd) {
-    k_fac_ogather<double><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_g, F.invm,
-                                                xd, F.u64, done, gate);
+    fac_ogather<double>(ctx, xd, done, gate);
     k_fac_oapply<double, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xd, b, y,
                                                   ctx->ws, done, gate);
     return;
   }
-  k_fac_ogather<float><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_g, F.invm,
-                                             xf, F.u64, done, gate);
+  fac_ogather<float>(ctx, xf, done, gate);
   if (mode == 0)
     k_fac_oapply<float, 0><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
                                                  ctx->ws, done, gate);

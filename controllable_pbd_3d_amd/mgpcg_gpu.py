@@ -58,12 +58,32 @@ def _members(agg: np.ndarray, n_c: int):
     return ptr, order
 
 
-def vertex_slices(vptr: np.ndarray, vinc: np.ndarray):
+def uniform_rows(vptr: np.ndarray, max_growth: float = 1.25) -> int:
+    """Rows of four slots per vertex for a uniform vertex-slice layout (every slice as tall as
+    the busiest vertex), or 0 when that would exceed max_growth times the per-slice layout's
+    slots (or more than 8 rows). Kuhn-cube meshes (at most 24 tets per vertex) give 6."""
+    cnt = np.diff(np.asarray(vptr, np.int64))
+    nv = len(cnt)
+    if nv == 0:
+        return 0
+    ns = (nv + 15) // 16
+    padded = np.zeros(16 * ns, np.int64)
+    padded[:nv] = cnt
+    per_slice = int((64 * ((padded.reshape(ns, 16).max(axis=1) + 3) // 4)).sum())
+    rows = int((cnt.max() + 3) // 4)
+    if rows > 8 or 64 * rows * ns > max_growth * max(per_slice, 1):
+        return 0
+    return max(rows, 1)
+
+
+def vertex_slices(vptr: np.ndarray, vinc: np.ndarray, rows_uniform: int = 0):
     """Vertex-sliced incidence of the factored level 0, four lanes per vertex: vertices binned
     16 per 64-lane slice, slot k of vertex 16 s + l at vs_ptr[s] + 64 (k // 4) + 4 l + k % 4
     for k < vs_cnt[v] (the vertex's incidences 4 tet + local index, in vinc order); a slice
-    holds ceil(busiest vertex / 4) rows of 64 slots. Returns (vs_ptr int32[n_slices + 1],
-    vs_cnt int32[n_verts], vs_inc int32[vs_ptr[-1]], zero where unused)."""
+    holds ceil(busiest vertex / 4) rows of 64 slots, or rows_uniform rows each (then
+    vs_ptr[s] = 64 rows_uniform s, and the vertex passes need no slice pointer). Returns
+    (vs_ptr int32[n_slices + 1], vs_cnt int32[n_verts], vs_inc int32[vs_ptr[-1]], zero where
+    unused)."""
     vptr = np.asarray(vptr, np.int64)
     nv = len(vptr) - 1
     cnt = np.diff(vptr)
@@ -71,6 +91,10 @@ def vertex_slices(vptr: np.ndarray, vinc: np.ndarray):
     padded = np.zeros(16 * ns, np.int64)
     padded[:nv] = cnt
     rows = (padded.reshape(ns, 16).max(axis=1) + 3) // 4 if ns else np.zeros(0, np.int64)
+    if rows_uniform:
+        if ns and rows.max() > rows_uniform:
+            raise ValueError("a vertex has more slots than the uniform layout holds")
+        rows = np.full(ns, int(rows_uniform), np.int64)
     vs_ptr = np.zeros(ns + 1, np.int64)
     vs_ptr[1:] = np.cumsum(64 * rows)
     if vs_ptr[-1] >= 2 ** 31:
@@ -352,7 +376,8 @@ class GPUMultigridSolver:
     # reduced-precision levels pack 16-bit column offsets j - i with the values when every
     # |j - i| is below this bound, and use int32 columns (shared with the FP32 copy) otherwise
     col16_limit = 32768
-    # factored level 0: 16-bit tet offsets in the vertex slots' unused words when they fit
+    # factored level 0: 16-bit tet offsets in the vertex slots' unused words when they fit, and
+    # then (when it costs at most 25 % more slots) every vertex slice as tall as the busiest
     slot_offsets = True
     # packed levels whose blocks of 256 rows read x from at most this many 32-column segments
     # stage them in LDS (windowed sweeps; the words then hold LDS offsets, so the 16-bit column
@@ -428,7 +453,9 @@ class GPUMultigridSolver:
         self.ctx("mgpbd_mg_set_level_half", 0, None, None, dptr(hb["sd"]), dptr(hb["isd"]),
                  dptr(L.diag_pos))
         self._half.append(hb)
-        vs_ptr, vs_cnt, vs_inc = vertex_slices(src["vptr"], src["vinc"])
+        tbase = slot_offset_base(src["vptr"], src["vinc"]) if self.slot_offsets else None
+        rows_u = uniform_rows(src["vptr"]) if tbase is not None else 0
+        vs_ptr, vs_cnt, vs_inc = vertex_slices(src["vptr"], src["vinc"], rows_u)
         nv, nt = len(vs_cnt), L.n_rows
         f = self._fac = {
             "vs_ptr": torch.from_numpy(vs_ptr).to(dev),
@@ -446,10 +473,10 @@ class GPUMultigridSolver:
                  dptr(f["vs_cnt"]), dptr(f["vs_inc"]), dptr(f["vs_k"]), dptr(f["vs_g"]),
                  dptr(f["tk"]), dptr(f["ahat"]), dptr(f["u"]), dptr(f["u64"]),
                  dptr(L.diag_inv))
-        tbase = slot_offset_base(src["vptr"], src["vinc"])
-        if tbase is not None and self.slot_offsets:
+        if tbase is not None:
             f["tbase"] = torch.from_numpy(tbase).to(dev)
-            self.ctx("mgpbd_mg_set_level0_offsets", dptr(f["tbase"]))
+            f["rows_uniform"] = rows_u
+            self.ctx("mgpbd_mg_set_level0_offsets", dptr(f["tbase"]), rows_u)
         if self.n_levels < 2:
             return
         C = self.levels[1]

@@ -481,8 +481,13 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
  * vertex lies in [tbase, tbase + 65535]; the per-iteration conversion then writes each slot's
  * tet - tbase into the unused fourth word of its K and FP32 gradient, and the V-cycle's and
  * the PCG's vertex passes read no incidence list (12 -> 8 bytes per slot). NULL switches it
- * off; it takes effect at the next mgpbd_mg_convert_half. */
-int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase_dev);
+ * off; it takes effect at the next mgpbd_mg_convert_half. rows_uniform (0..8, 0 = off)
+ * declares that the vertex slices set with mgpbd_mg_set_level0_factored are uniform: every
+ * slice rows_uniform rows of 64 slots (vs_ptr[s] = 64 rows_uniform s) with the unused slots
+ * of vs_k / vs_g zero; the vertex passes then compute each lane's slot positions instead of
+ * loading the slice pointer and count (same sums bit for bit: the unused slots add exact
+ * zeros). */
+int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase_dev, int rows_uniform);
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,

@@ -1,7 +1,8 @@
-"""The factored level 0's 16-bit slot offsets (mgpbd_mg_set_level0_offsets): the vertex passes
-take each slot's tet from the offset word beside its K / gradient instead of the incidence
-list. Same tets, same summation order: positions bit-identical to the incidence-list passes.
-The host-side base (slot_offset_base) is checked on the CPU in tests/test_slot_offsets.py."""
+"""The factored level 0's vertex-pass layouts (mgpbd_mg_set_level0_offsets): the tet of each
+slot from the incidence list, from a 16-bit offset beside its K / gradient, and the uniform
+layout (every vertex slice as tall as the busiest vertex, slot positions computed, unused slots
+zero). Same tets, same summation order, zeros added last: positions bit-identical across the
+three. The host-side base and layout are checked on the CPU in tests/test_slot_offsets.py."""
 
 import numpy as np
 import pytest
@@ -14,8 +15,8 @@ from tests.scenarios import deformed, physical_block, solver_kwargs   # noqa: E4
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
-def test_slot_offsets_are_bit_identical(monkeypatch, precision):
-    from controllable_pbd_3d_amd import MGPBDSolverGPU
+def test_slot_layouts_are_bit_identical(monkeypatch, precision):
+    from controllable_pbd_3d_amd import MGPBDSolverGPU, mgpcg_gpu
     from controllable_pbd_3d_amd.mgpcg_gpu import GPUMultigridSolver
     mesh, mu_t = physical_block(6, seed=71)
     kw = solver_kwargs(mesh, mu_per_tet=mu_t, hold_pinned=True, relative_tol=True, tol_cg=1e-6,
@@ -25,14 +26,20 @@ def test_slot_offsets_are_bit_identical(monkeypatch, precision):
                        precision=precision, level0="factored")
     kw["v_p"] = deformed(mesh, seed=71, amp=0.02, twist=0.05)
     runs = []
-    for offsets in (False, True):
-        monkeypatch.setattr(GPUMultigridSolver, "slot_offsets", offsets)
+    uniform_rows = mgpcg_gpu.uniform_rows
+    for layout in ("incidence", "offsets", "uniform"):
+        monkeypatch.setattr(GPUMultigridSolver, "slot_offsets", layout != "incidence")
+        monkeypatch.setattr(mgpcg_gpu, "uniform_rows",
+                            uniform_rows if layout == "uniform" else (lambda vptr: 0))
         s = MGPBDSolverGPU(**kw, verbose=False)
         s.init()
-        assert ("tbase" in s._gpu_solver._fac) == offsets
+        f = s._gpu_solver._fac
+        assert ("tbase" in f) == (layout != "incidence")
+        assert (f.get("rows_uniform", 0) > 0) == (layout == "uniform")
         for _ in range(3):
             s.step(0)
         runs.append((s.v_p.cpu().numpy(), s.last_stats))
-    print(f"[offsets] {precision}: {runs[1][1]}", flush=True)
-    assert np.array_equal(runs[0][0].view(np.int32), runs[1][0].view(np.int32))
-    assert runs[0][1] == runs[1][1]
+    print(f"[layouts] {precision}: {runs[2][1]}", flush=True)
+    for v, st in runs[1:]:
+        assert np.array_equal(runs[0][0].view(np.int32), v.view(np.int32))
+        assert runs[0][1] == st

@@ -6,7 +6,7 @@ import numpy as np
 
 from controllable_pbd_3d_amd import _lib
 from controllable_pbd_3d_amd.mesh import box_mesh
-from controllable_pbd_3d_amd.mgpcg_gpu import slot_offset_base
+from controllable_pbd_3d_amd.mgpcg_gpu import slot_offset_base, uniform_rows, vertex_slices
 
 
 def _incidence(tets, nv):
@@ -44,3 +44,28 @@ def test_far_apart_tets_get_no_offsets():
     assert slot_offset_base(vptr, vinc) is None
     ok = slot_offset_base(*_incidence(tets[:-1], nv))
     assert ok is not None
+
+
+def test_uniform_layout():
+    """A Kuhn block has at most 24 tets per vertex: six rows of four slots per vertex, every
+    incidence at its computed position, the rest zero; a mesh with one busy vertex keeps the
+    per-slice layout."""
+    m = box_mesh(7, 0.05, jitter=0.05, seed=5)
+    vptr, vinc = _incidence(m.tets, m.n_verts)
+    rows = uniform_rows(vptr)
+    assert rows == 6
+    vs_ptr, vs_cnt, vs_inc = vertex_slices(vptr, vinc, rows)
+    assert np.array_equal(vs_ptr, 64 * rows * np.arange(len(vs_ptr)))
+    seen = np.zeros(len(vs_inc), bool)
+    for v in range(m.n_verts):
+        for k in range(vs_cnt[v]):
+            p = (v // 16) * 64 * rows + 64 * (k // 4) + 4 * (v % 16) + k % 4
+            assert vs_inc[p] == vinc[vptr[v] + k]
+            seen[p] = True
+    assert np.all(vs_inc[~seen] == 0)
+    # one vertex in 40 tets: a uniform layout would be 10 rows tall everywhere
+    t4 = m.tets.reshape(-1, 4).copy()
+    hub = int(t4[0, 0])
+    t4[:40, 0] = hub
+    vptr2, _ = _incidence(t4, m.n_verts)
+    assert uniform_rows(vptr2) == 0
