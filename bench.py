@@ -12,7 +12,10 @@ Runs (each its own solver on the same mesh, same loading):
   fp32_reference  the reference's algorithm (2 + 2 weighted-Jacobi sweeps, omega 2/3, FP32,
                   assembled CSR; the FP64 residual is measured, not enforced)
   fp32            Chebyshev smoothing on the device's spectral estimates, FP64 iterate, every
-                  solve FP64-verified; FP32 assembled level 0 (the metric's denominator)
+                  solve FP64-verified; FP32 assembled level 0 (the metric's denominator);
+                  spectral estimates, coarsest inverse and coarse levels 1.. rebuilt at each
+                  substep's first Newton iteration (level 0 at every one), as in every run
+                  below
   fp16_assembled  the same with the stored FP16 level 0 (same operator as fp32: the precision's
                   own gain)
   fp16, bf16      FP16 / BF16 V-cycle on the factored level 0 J W J^T + alpha~
@@ -131,7 +134,7 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
 
 CHEBYSHEV = dict(smoother="chebyshev", n_smooth=5, smooth_schedule=[5, 1], cheb_ratio=30.0,
                  smoother_safety=1.2, verify_residual=1, fp64_iterate=True,
-                 spectral_refresh="substep")
+                 spectral_refresh="substep", coarse_refresh="substep")
 RUNS = {
     "fp32_reference": dict(precision="fp32", smoother="jacobi", verify_residual=2),
     "fp32": dict(precision="fp32", **CHEBYSHEV),
@@ -297,6 +300,8 @@ def run_config(mesh, mu_t, lam_t, cfg, name, steps, warmup, profile_steps, model
     per = 6 if win0 else 8
     fnv = s.n_verts if kw.get("level0") == "factored" else None
     offs = res["slot_offsets"] = fnv is not None and "tbase" in s._gpu_solver._fac
+    if offs:    # 0: per-slice row counts; R: every 16-vertex slice padded to R rows
+        res["vertex_rows_uniform"] = int(s._gpu_solver._fac["rows_uniform"])
     kern = kernel_stats(s._gpu_solver.ctx, s.n_cons, s._nnz, kw["precision"], cheb, per, fnv,
                         offs)
     kern.update({k: v for k, v in kernel_stats(ctx, s.n_cons, s._nnz, kw["precision"], cheb,

@@ -145,7 +145,7 @@ SIGNATURES = {
     "mgpbd_gnn_wgrad64": (c_int, [_V, c_int, _V, _V, _V, _V]),
     # native substep
     "mgpbd_mg_set_refill": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, c_int, _V, _V]),
-    "mgpbd_mg_refresh": (c_int, [_V, c_int, c_int]),
+    "mgpbd_mg_refresh": (c_int, [_V, c_int, c_int, c_int]),
     "mgpbd_substep_struct_sizes": (c_int, [I32P]),
     "mgpbd_substep_create": (_V, [_V]),
     "mgpbd_substep_destroy": (None, [_V]),
@@ -167,7 +167,8 @@ class SubstepDesc(ctypes.Structure):
         [("drag_target", c_float * 3), ("drag_k", c_float)] + \
         [(n, ctypes.c_int32) for n in ("drag_vertex", "hold_pinned", "maxiter", "maxiter_cg",
                                        "relative_tol", "use_line_search",
-                                       "spectral_every_newton", "warm_start",
+                                       "spectral_every_newton", "coarse_every_newton",
+                                       "warm_start",
                                        "overlap_inference", "factored")]
 
 

@@ -3415,11 +3415,18 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
   return 0;
 }
 
-int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from) {
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from, int coarse) {
   CTX_CHECK("mgpbd_mg_refresh");
   if (int e = check_levels(ctx)) return e;
   if (half_from < 0 || half_from > ctx->n_levels)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: 0 <= half_from <= n_levels");
+  if (!coarse) {   // levels 1.. kept: only level 0's own reduced-precision data
+    if (half_from == 0)
+      if (int e = convert_half_levels(ctx, 0, 1)) return e;
+    if (refresh && ctx->spectral && !ctx->fixed_weights)
+      if (int e = mgpbd_mg_update_smoother(ctx, nullptr)) return e;
+    return 0;
+  }
   for (int l = 0; l + 1 < ctx->n_levels; ++l) {
     const mgpbd_ctx::Refill& R = ctx->refill[l];
     if (R.n < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: refill maps of level " +

@@ -213,6 +213,7 @@ int mgpbd_substep(mgpbd_substep_state* st, const mgpbd_substep_desc* d, mgpbd_ne
                                   d->fill_desc, d->t_c, d->v_invm, d->gradC, d->alpha,
                                   d->A_val, d->A_dinv));
     const int refresh = d->spectral_every_newton || ite == 0;
+    const int coarse = d->coarse_every_newton || ite == 0;
     bool have_guess = false;
     if (ite == 0 && d->warm_start == 2 && d->overlap_inference && d->factored) {
       // the network needs only the constraint pass and level 0's 1/diag: it runs on the
@@ -223,12 +224,12 @@ int mgpbd_substep(mgpbd_substep_state* st, const mgpbd_substep_desc* d, mgpbd_ne
       MGPBD_HIP_TRY(hipStreamWaitEvent(st->side, st->ev_in, 0));
       SUB_TRY(predict_start(st->side_ctx, d));
       MGPBD_HIP_TRY(hipEventRecord(st->ev_out, st->side));
-      SUB_TRY(mgpbd_mg_refresh(c, refresh, 1));
+      SUB_TRY(mgpbd_mg_refresh(c, refresh, 1, coarse));
       MGPBD_HIP_TRY(hipStreamWaitEvent(s, st->ev_out, 0));
       have_guess = true;
       *predicted = 1;
     } else {
-      SUB_TRY(mgpbd_mg_refresh(c, refresh, 0));
+      SUB_TRY(mgpbd_mg_refresh(c, refresh, 0, coarse));
     }
 
     int it = 0;

@@ -40,6 +40,10 @@ spectrum, and the dense coarsest solve (coarse_solver='cholesky') re-inverts its
 every Newton iteration's refill, or only at a substep's first Newton iteration (the later
 iterations of a substep change the matrices little; the kept inverse is SPD, and the verified
 PCG and the stall rescue cover a lagging estimate).
+``coarse_refresh`` ('newton' | 'substep'): the coarse levels 1.. are rebuilt from level 0 after
+every Newton iteration's refill (the reference), or only at a substep's first Newton iteration
+(extension: later iterations refresh level 0 alone; the V-cycle stays an SPD preconditioner of
+level 0's current matrix, and the PCG still solves that matrix to its tolerance).
 ``overlap_inference`` (default True; learned start with the factored level 0): the network runs
 on a second stream while the coarse levels refill (bit-identical to running it after them).
 ``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is
@@ -106,13 +110,16 @@ class MGPBDSolverGPU:
                  smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
-                 spectral_refresh: str = "newton", overlap_inference: bool = True,
+                 spectral_refresh: str = "newton", coarse_refresh: str = "newton",
+                 overlap_inference: bool = True,
                  volume_lambda=None, omega_levels=None, native_substep: bool = False,
                  device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
         if spectral_refresh not in ("newton", "substep"):
             raise ValueError("spectral_refresh must be 'newton' or 'substep'")
+        if coarse_refresh not in ("newton", "substep"):
+            raise ValueError("coarse_refresh must be 'newton' or 'substep'")
         if level0 not in ("assembled", "factored"):
             raise ValueError("level0 must be 'assembled' or 'factored'")
         if level0 == "factored" and (precision == "fp32" or galerkin != "structural"):
@@ -166,6 +173,7 @@ class MGPBDSolverGPU:
         self.omega_levels = omega_levels
         self.level0 = level0
         self.spectral_refresh = spectral_refresh
+        self.coarse_refresh = coarse_refresh
         self.overlap_inference = bool(overlap_inference)
         self.native_substep = bool(native_substep)
         self._native = None        # (state handle, descriptor, stats buffer) of mgpbd_substep
@@ -375,6 +383,7 @@ class MGPBDSolverGPU:
         d.hold_pinned, d.maxiter, d.maxiter_cg = int(self.hold_pinned), self.maxiter, mg.maxiter_cg
         d.relative_tol, d.use_line_search = int(mg.relative), int(self.use_line_search)
         d.spectral_every_newton = int(self.spectral_refresh == "newton")
+        d.coarse_every_newton = int(self.coarse_refresh == "newton")
         d.warm_start = {"none": 0, "previous": 1, "learned": 2}[self.warm_start]
         d.overlap_inference = int(self.overlap_inference)
         d.factored = int(mg.factored)
@@ -503,11 +512,12 @@ class MGPBDSolverGPU:
         if not mg.factored or self.record_samples is not None:
             self._fill_A()
         refresh = self.spectral_refresh == "newton" or newton_iter == 0
+        coarse = self.coarse_refresh == "newton" or newton_iter == 0
         hook = None
         if (newton_iter == 0 and self.warm_start == "learned" and self.overlap_inference and
                 mg.factored):
             hook = self._launch_prediction
-        mg.update_all_coarse_matrices(refresh_smoother=refresh, after_level0=hook)
+        mg.update_all_coarse_matrices(refresh_smoother=refresh, after_level0=hook, coarse=coarse)
         if hook is not None:
             torch.cuda.current_stream(self.device).wait_event(self._side[3])
 

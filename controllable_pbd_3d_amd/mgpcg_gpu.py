@@ -559,12 +559,14 @@ class GPUMultigridSolver:
         self.levels[0].update_data(data)
         self.levels[0].cache_diag_inv()
 
-    def update_all_coarse_matrices(self, refresh_smoother: bool = True, after_level0=None):
+    def update_all_coarse_matrices(self, refresh_smoother: bool = True, after_level0=None,
+                                   coarse: bool = True):
         """Coarse values from the fine ones through the refill maps
         (cons/mgpcg_gpu.py:183-212). refresh_smoother=False keeps the spectral smoother's
         current estimates and the coarsest level's dense inverse (extension: the caller knows
         the matrices changed little; the kept inverse is SPD, so the V-cycle stays an SPD
-        preconditioner).
+        preconditioner). coarse=False keeps levels 1.. as they are and refreshes only level
+        0's reduced-precision data (extension, mgpbd_mg_refresh with coarse = 0).
         after_level0 (factored level 0 only): called once level 0's scales, 1/diag and K are
         queued, before the coarse levels' work, so the caller can start work that needs only
         those on another stream."""
@@ -572,6 +574,12 @@ class GPUMultigridSolver:
         if early:
             self.ctx("mgpbd_mg_convert_half_levels", 0, 1)
             after_level0()
+        if not coarse:
+            if getattr(self, "_half", None) and not early:
+                self.ctx("mgpbd_mg_convert_half_levels", 0, 1)
+            if self.smoother != "jacobi" and refresh_smoother:
+                self.ctx("mgpbd_mg_update_smoother", None)
+            return
         for l, rf in enumerate(self.refill):
             F, C = self.levels[l], self.levels[l + 1]
             if l == 0 and self._fac is not None:

@@ -355,9 +355,14 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
  * and 1/diag, the reduced-precision copies of levels half_from .. n_levels-1 (a caller that
  * converted level 0 early passes 1) and, with refresh != 0, the coarsest inverse
  * (MGPBD_COARSE_CHOLESKY) and the spectral smoother's estimates; refresh = 0 keeps both
- * (the caller knows the matrices changed little; the kept inverse is SPD). Queues only (a
- * Cholesky refactorization synchronises, as in mgpbd_mg_factor_coarsest). */
-int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from);
+ * (the caller knows the matrices changed little; the kept inverse is SPD). coarse = 0 keeps
+ * levels 1 .. n_levels-1 as they are (extension, no reference counterpart: the caller knows
+ * level 0 changed little since they were built, e.g. within one substep's Newton loop; the
+ * V-cycle stays an SPD preconditioner of level 0's current matrix, which the PCG solves):
+ * then only level 0's reduced-precision data (half_from = 0) and, with refresh, the spectral
+ * estimates are refreshed. Queues only (a Cholesky refactorization synchronises, as in
+ * mgpbd_mg_factor_coarsest). */
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from, int coarse);
 /* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level
  * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse and the device
  * keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue of X A lies
@@ -721,6 +726,7 @@ typedef struct mgpbd_substep_desc {
   int32_t drag_vertex;                         /* -1: no spring drag */
   int32_t hold_pinned, maxiter, maxiter_cg, relative_tol, use_line_search;
   int32_t spectral_every_newton;               /* 0: spectral refresh at iteration 0 only */
+  int32_t coarse_every_newton;                 /* 0: levels 1.. rebuilt at iteration 0 only */
   int32_t warm_start;                          /* 0 none, 1 previous lambda, 2 learned */
   int32_t overlap_inference;                   /* learned + factored: network on a side stream */
   int32_t factored;                            /* level 0 factored (mgpbd_mg_set_level0_factored) */

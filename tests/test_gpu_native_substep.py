@@ -32,6 +32,10 @@ CASES = {
                                        warm_start="learned"), False, True),
     "fp16_learned_serial": (dict(MIXED, precision="fp16", level0="factored",
                                  warm_start="learned", overlap_inference=False), True, False),
+    # levels 1.. rebuilt at each substep's first Newton iteration only
+    "fp16_factored_coarse_lag_drag": (dict(MIXED, precision="fp16", level0="factored",
+                                           coarse_refresh="substep"), True, True),
+    "fp32_coarse_lag": (dict(EXT, smoother="spectral", coarse_refresh="substep"), True, False),
 }
 
 

@@ -176,6 +176,21 @@ def test_factored_mixed_precision_matches_fp32_at_full_size(name, volume, precis
     assert worst_h <= 1e-4
 
 
+@pytest.mark.parametrize("name,volume,kw_b", [
+    ("c1", False, dict(precision="fp16", level0="factored")),
+    ("c3", True, dict(precision="fp16", level0="factored")),
+    ("c1", False, {})])
+def test_lagged_coarse_levels_match_at_full_size(name, volume, kw_b):
+    """coarse_refresh='substep' (levels 1.. rebuilt at a substep's first Newton iteration
+    only) against the FP32 solver that rebuilds them at every Newton iteration: the PCG still
+    solves level 0's current matrix to 1e-6, so with Newton run to rtol 1e-6 the positions
+    agree as closely as two precisions do."""
+    worst_h, bad, rel_max = _pair_run(name, volume, 2, {},
+                                      dict(kw_b, coarse_refresh="substep"))
+    assert bad == 0 and rel_max <= 1e-6
+    assert worst_h <= 1e-4
+
+
 def test_c2_gap_is_the_newton_cap_not_precision():
     """C2 (E 1e4-1e8): every substep runs into the 30-Newton cap at rtol 1e-6 (the line search
     accepts 1/4 or 1/8 steps), so the positions are a mid-iteration Newton state that any
