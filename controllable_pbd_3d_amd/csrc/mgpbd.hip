@@ -203,6 +203,41 @@ __device__ __forceinline__ double arap_C(const double s[3]) {
   return sqrt(a * a + b * b + c * c);
 }
 
+// ARAP constraint C = sqrt(sum_i (s_i - 1)^2) over F's rotation-variant singular values and,
+// with GRAD, dC/dF = U diag((s_i - 1) / C) V^T into G when (float)C > 1e-6 (else G untouched).
+// For det F > 0 from the polar rotation R = U V^T (C = ||F - R||_F, dC/dF = (F - R) / C, a
+// few Newton steps); inverted, collapsed or strongly compressed tets take the Jacobi SVD.
+template <bool GRAD>
+__device__ __forceinline__ double arap_constraint(const double F[9], double (&G)[3][3]) {
+  double R[9];
+  if (polar_rotation(F, R)) {
+    double D[9], c2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      D[k] = F[k] - R[k];
+      c2 = fma(D[k], D[k], c2);
+    }
+    const double Cd = sqrt(c2);
+    if (GRAD && (float)Cd > 1e-6f) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) G[k / 3][k % 3] = D[k] / Cd;
+    }
+    return Cd;
+  }
+  double U[3][3], s[3], V[3][3];
+  svd3(F, U, s, V);
+  const double Cd = arap_C(s);
+  if (GRAD && (float)Cd > 1e-6f) {
+    const double d[3] = {(s[0] - 1.0) / Cd, (s[1] - 1.0) / Cd, (s[2] - 1.0) / Cd};
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        G[r][c] = U[0][r] * V[0][c] * d[0] + U[1][r] * V[1][c] * d[1] + U[2][r] * V[2][c] * d[2];
+  }
+  return Cd;
+}
+
 // Volume constraint det F - 1 and dC/dF = cof(F): column c of the cofactor matrix is the
 // cross product of F's columns c+1 and c+2 (cyclic). G[r][c], returns det F - 1.
 __device__ __forceinline__ double volume_C(const double F[9], double G[3][3]) {
@@ -258,19 +293,11 @@ k_constraints(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
     for (int k = 0; k < 9; ++k) Bf[k] = Dm_inv[9 * (size_t)t + k];
     double F[9];
     deformation_gradient(tv, x, nullptr, 0.0f, Bf, F);
-    double U[3][3], s[3], V[3][3];
-    svd3(F, U, s, V);
-    double Cd = arap_C(s);
+    double G[3][3];   // dC/dF
+    const double Cd = arap_constraint<true>(F, G);
     float C = (float)Cd;
     float g[12];
     if (C > 1e-6f) {
-      double d[3] = {(s[0] - 1.0) / Cd, (s[1] - 1.0) / Cd, (s[2] - 1.0) / Cd};
-      double G[3][3];   // dC/dF = U diag(d) V^T
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          G[r][c] = U[0][r] * V[0][c] * d[0] + U[1][r] * V[1][c] * d[1] + U[2][r] * V[2][c] * d[2];
       // g_k = G B^T[:, k-1] (k = 1..3), g_0 = -(g_1 + g_2 + g_3)
       double gd[4][3];
 #pragma unroll
@@ -328,9 +355,8 @@ k_dual_norms(int n, const int4* __restrict__ t_i, const float* __restrict__ x,
       for (int k = 0; k < 9; ++k) B[k] = Dm_inv[9 * (size_t)t + k];
       double F[9];
       deformation_gradient(tv, x, dx, omega, B, F);
-      double U[3][3], s[3], V[3][3];
-      svd3(F, U, s, V);
-      float c = (float)arap_C(s);
+      double G[3][3];
+      float c = (float)arap_constraint<false>(F, G);
       float r = -(c + al);
       acc_trial += (double)r * (double)r;
       if (VOL) {

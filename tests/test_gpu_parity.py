@@ -103,6 +103,42 @@ def test_constraints_vs_fp64_svd(scene):
     np.testing.assert_allclose(g_g, o.gradC, rtol=0, atol=2e-5 * max(1.0, np.abs(o.gradC).max()))
 
 
+def test_constraints_polar_and_svd_paths():
+    """k_constraints takes R = U V^T from Newton's polar iteration when det F is safely positive
+    and the Jacobi SVD otherwise. A block squashed to 30 % of its height, sheared, with a few
+    vertices pushed through their neighbours gives both kinds of tets (and strongly compressed
+    ones): C and dC/dx match the FP64-SVD oracle to the same bar as above."""
+    mesh = small_block(5, seed=7)
+    rng = np.random.default_rng(7)
+    v = mesh.verts.astype(np.float64) @ np.array([[1.0, 0.0, 0.4], [0.0, 1.0, 0.0],
+                                                 [0.0, 0.0, 0.3]])
+    v += rng.normal(0.0, 0.05 * mesh.h, v.shape)
+    for i in rng.choice(mesh.n_verts, 6, replace=False):
+        v[i] += rng.normal(0.0, 1.5 * mesh.h, 3)
+    v0 = v.astype(np.float32)
+    s = _solver(mesh, v0)
+    o = _oracle(mesh, v0, precision="f64")
+    s.Dm_inv.copy_(torch.from_numpy(o.Dm_inv.astype(np.float32).reshape(-1, 3, 3)))
+    s._compute_C_and_gradC()
+    o._compute_C_and_gradC()
+    t = mesh.tets.reshape(-1, 4)
+    p = v0.astype(np.float64)
+    Ds = np.stack([p[t[:, k]] - p[t[:, 0]] for k in (1, 2, 3)], axis=2)
+    F = Ds @ o.Dm_inv.astype(np.float32).astype(np.float64).reshape(-1, 3, 3)
+    det = np.linalg.det(F)
+    f2 = (F ** 2).sum(axis=(1, 2))
+    polar = (det > 0) & (det ** 2 > 1e-6 * f2 ** 3 / 27)
+    sv = np.linalg.svd(F, compute_uv=False)
+    print(f"[polar] {mesh.n_tets} tets: {int(polar.sum())} polar, {int((~polar).sum())} SVD "
+          f"({int((det <= 0).sum())} inverted); smallest singular value {sv.min():.3f}",
+          flush=True)
+    assert polar.sum() > 0 and (det <= 0).sum() > 0
+    C_g = s.constraints.cpu().numpy()
+    g_g = s.gradC.cpu().numpy().reshape(-1)
+    np.testing.assert_allclose(C_g, o.constraints, rtol=2e-5, atol=2e-5)
+    np.testing.assert_allclose(g_g, o.gradC, rtol=0, atol=2e-5 * max(1.0, np.abs(o.gradC).max()))
+
+
 def test_shared_vertex_fill_equals_full_fill():
     """The descriptor A fill (only the shared vertices' gradients of tet j gathered) writes the
     same bits as the reference-structured fill, on a jittered block and on the reference's
