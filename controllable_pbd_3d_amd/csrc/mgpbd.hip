@@ -997,52 +997,78 @@ k_sumsq(int64_t len, const double* __restrict__ x, RedWS ws) {
   store_partial(ws, 0, block_sum(acc));
 }
 
-// C = alpha A B + beta C for n x n column-major FP64 matrices (beta = 0: C is not read): one
-// 32 x 32 tile of C per block, each thread 2 x 2 outputs (rows tx, tx + 16, columns ty,
-// ty + 16), the k dimension staged through LDS 32 at a time and summed in ascending order
-// (FP64 FMA; deterministic). Grid (ceil(n/32), ceil(n/32)).
-constexpr int kGemmTile = 32;
+// C = alpha A B + beta C for n x n column-major FP64 matrices (beta = 0: C is not read) on
+// the FP64 matrix cores. A block owns one 16 x 16 tile of C; its four wavefronts split the k
+// range in four consecutive parts and each runs v_mfma_f64_16x16x4 along its part: in the
+// m-th MFMA of a 16-wide k step, lane l supplies A(i0 + l % 16, k) and B(k, j0 + l % 16) with
+// k = k0 + 4 (l / 16) + m, and accumulator r of lane l is C(i0 + l / 16 + 4 r, j0 + l % 16)
+// (the layout measured on gfx950 with a one-hot B operand). The next step's operands are
+// loaded before the current step's MFMAs issue; the split gives each SIMD several wavefronts
+// to cover the L2 latency at the coarsest level's sizes. The four partial tiles are added in
+// wavefront order through LDS, so the result is deterministic. Grid (ceil(n/16), ceil(n/16));
+// rows, columns and k past n read zeros.
+constexpr int kGemmTile = 16;
+constexpr int kGemmSplit = kBlock / 64;
+typedef double v4f64 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void dgemm_operands(int n, const double* __restrict__ A,
+                                               const double* __restrict__ Bc, int ia, bool ra,
+                                               bool rb, int kb, double (&a)[4],
+                                               double (&b)[4]) {
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {   // loads from clamped (in-bounds) addresses, then selects:
+    const int k = kb + m;         // no branches around the loads
+    const bool in = k < n;
+    const int kc = in ? k : n - 1;
+    const double av = A[(size_t)kc * n + (ra ? ia : 0)];
+    const double bv = Bc[kc];
+    a[m] = (ra && in) ? av : 0.0;
+    b[m] = (rb && in) ? bv : 0.0;
+  }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_dgemm(int n, double alpha, const double* __restrict__ A, const double* __restrict__ B,
         double beta, double* __restrict__ C) {
-  __shared__ double As[kGemmTile][kGemmTile + 1];   // As[k][i] = A(i0 + i, k0 + k)
-  __shared__ double Bs[kGemmTile][kGemmTile + 1];   // Bs[k][j] = B(k0 + k, j0 + j)
+  __shared__ v4f64 part[kGemmSplit - 1][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i0 = blockIdx.x * kGemmTile, j0 = blockIdx.y * kGemmTile;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double c00 = 0.0, c01 = 0.0, c10 = 0.0, c11 = 0.0;
-  for (int k0 = 0; k0 < n; k0 += kGemmTile) {
+  const int li = lane & 15, g = lane >> 4;
+  const int ia = i0 + li, jb = j0 + li;
+  const bool ra = ia < n, rb = jb < n;
+  const double* Bc = B + (size_t)(rb ? jb : 0) * n;
+  const int steps = (n + 15) / 16;
+  const int per = (steps + kGemmSplit - 1) / kGemmSplit;
+  const int s0 = w * per, s1 = min(steps, s0 + per);
+  v4f64 acc = {0.0, 0.0, 0.0, 0.0};
+  if (s0 < s1) {   // uniform across the wavefront
+    double a[4], b[4], an[4], bn[4];
+    dgemm_operands(n, A, Bc, ia, ra, rb, 16 * s0 + 4 * g, a, b);
+    for (int st = s0; st < s1; ++st) {
+      dgemm_operands(n, A, Bc, ia, ra, rb, 16 * (st + 1) + 4 * g, an, bn);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int e = threadIdx.x + kBlock * r;
-      const int lo = e & 31, hi = e >> 5;           // lo runs along the contiguous index
-      const int ai = i0 + lo, ak = k0 + hi;          // A(ai, ak) at A[ak n + ai]
-      As[hi][lo] = (ai < n && ak < n) ? A[(size_t)ak * n + ai] : 0.0;
-      const int bk = k0 + lo, bj = j0 + hi;          // B(bk, bj) at B[bj n + bk]
-      Bs[lo][hi] = (bk < n && bj < n) ? B[(size_t)bj * n + bk] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int k = 0; k < kGemmTile; ++k) {
-      const double a0 = As[k][tx], a1 = As[k][tx + 16];
-      const double b0 = Bs[k][ty], b1 = Bs[k][ty + 16];
-      c00 = fma(a0, b0, c00);
-      c01 = fma(a0, b1, c01);
-      c10 = fma(a1, b0, c10);
-      c11 = fma(a1, b1, c11);
-    }
-    __syncthreads();
-  }
-  const double acc[2][2] = {{c00, c01}, {c10, c11}};
+      for (int m = 0; m < 4; ++m)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[m], acc, 0, 0, 0);
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
-      if (i < n && j < n) {
-        double* c = C + (size_t)j * n + i;
-        *c = beta == 0.0 ? alpha * acc[a][b] : fma(alpha, acc[a][b], beta * *c);
+      for (int m = 0; m < 4; ++m) {
+        a[m] = an[m];
+        b[m] = bn[m];
       }
     }
+  }
+  if (w > 0) part[w - 1][lane] = acc;
+  __syncthreads();
+  if (w != 0) return;
+#pragma unroll
+  for (int q = 0; q < kGemmSplit - 1; ++q) acc += part[q][lane];
+  const int j = j0 + li;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + g + 4 * r;
+    if (i < n && j < n) {
+      double* c = C + (size_t)j * n + i;
+      *c = beta == 0.0 ? alpha * acc[r] : fma(alpha, acc[r], beta * *c);
+    }
+  }
 }
 
 // out = max_i sum_j |A_ij| (one block; n x n column-major)
