@@ -69,7 +69,7 @@ def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
     raise ValueError(kind)
 
 
-def factored_bytes(n_rows, n_verts, kind, chebyshev=False, offsets=False):
+def factored_bytes(n_rows, n_verts, kind, chebyshev=False, offsets=False, packed=False):
     """Algorithmic HBM bytes of the factored level 0 (each array counted once per pass; 4
     incidence slots per row). kind "gather" (vertex pass): 4 slots per row x (4-byte incidence
     + 8-byte K) + x (4 per row) + u written (16 per vertex) = 52 n + 16 n_v; with 16-bit slot
@@ -78,16 +78,18 @@ def factored_bytes(n_rows, n_verts, kind, chebyshev=False, offsets=False):
     vertex) = 56 n + 16 n_v. "jacobi"/"resid": both passes (+ the previous iterate, 4 n, for the
     Chebyshev sweeps). "spmv": the PCG's FP64 product from the FP32 gradients, 4 slots x
     (4-byte incidence + 16-byte gradient; 16 bytes with offsets, + 4 per vertex) + p + u (32
-    per vertex) written; t_i 16 + gradC 48 + alpha~ 4 + p 4 + A p 4 per row + u read."""
+    per vertex) written; t_i 16 + gradC 48 + alpha~ 4 + p 4 + A p 4 per row + u read. With
+    packed rows (packed) the tet passes read 8 bytes of row vertices instead of t_i's 16."""
     gather = (36 * n_rows + 20 * n_verts) if offsets else (52 * n_rows + 16 * n_verts)
-    apply = 56 * n_rows + 16 * n_verts
+    tv = 8 if packed else 16
+    apply = (40 + tv) * n_rows + 16 * n_verts
     if kind == "gather":
         return gather
     if kind == "apply":
         return apply
     if kind == "spmv":
         vert = (64 * n_rows + 4 * n_verts) if offsets else 80 * n_rows
-        return vert + 4 * n_rows + 32 * n_verts + 76 * n_rows + 32 * n_verts
+        return vert + 4 * n_rows + 32 * n_verts + (60 + tv) * n_rows + 32 * n_verts
     if kind in ("jacobi", "resid"):
         return gather + apply + (4 * n_rows if chebyshev and kind == "jacobi" else 0)
     raise ValueError(kind)
@@ -108,7 +110,7 @@ def measured_traffic(workload, kernel):
 
 
 def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factored_nv=None,
-                 offsets=False):
+                 offsets=False, packed=False):
     from controllable_pbd_3d_amd import _lib
     ms = np.zeros(8, np.float64)
     cnt = np.zeros(8, np.int64)
@@ -121,8 +123,10 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
     for kind, slot, per in (("spmv", 0, spmv_per_nnz), ("jacobi", 1, vb), ("resid", 2, vb)):
         if cnt[slot] > 0:
             avg = ms[slot] / cnt[slot] * 1e-3
-            by = factored_bytes(n, factored_nv, kind, chebyshev, offsets) if factored_nv else \
-                kernel_bytes(n, nnz, kind, per, chebyshev)
+            if factored_nv:
+                by = factored_bytes(n, factored_nv, kind, chebyshev, offsets, packed)
+            else:
+                by = kernel_bytes(n, nnz, kind, per, chebyshev)
             kern[kind] = {"avg_us": avg * 1e6, "launches": int(cnt[slot]),
                           "total_ms": float(ms[slot]), "GBs": by / avg / 1e9, "bytes": by}
     for kind, slot in (("fill_A", 3), ("constraints", 4), ("refill", 5), ("coarse", 6)):
@@ -302,10 +306,11 @@ def run_config(mesh, mu_t, lam_t, cfg, name, steps, warmup, profile_steps, model
     offs = res["slot_offsets"] = fnv is not None and "tbase" in s._gpu_solver._fac
     if offs:    # 0: per-slice row counts; R: every 16-vertex slice padded to R rows
         res["vertex_rows_uniform"] = int(s._gpu_solver._fac["rows_uniform"])
+    pk = res["packed_rows"] = fnv is not None and "rows_packed" in s._gpu_solver._fac
     kern = kernel_stats(s._gpu_solver.ctx, s.n_cons, s._nnz, kw["precision"], cheb, per, fnv,
-                        offs)
+                        offs, pk)
     kern.update({k: v for k, v in kernel_stats(ctx, s.n_cons, s._nnz, kw["precision"], cheb,
-                                               per, fnv, offs).items() if k not in kern})
+                                               per, fnv, offs, pk).items() if k not in kern})
     ctx("mgpbd_ctx_profile", 0)
     s._gpu_solver.ctx("mgpbd_ctx_profile", 0)
     over = [k for k, v in kern.items() if v.get("GBs", 0.0) > HBM_PEAK_GBS]
@@ -528,7 +533,8 @@ def main():
                            "traffic": measured_traffic(cfg["label"], "k_fac_gather"),
                            "apply_pass": {"kernel": "k_fac_apply<1>",
                                           "avg_us": probe["warm"]["second_us"],
-                                          "GBs": factored_bytes(n_rows, mesh.n_verts, "apply") /
+                                          "GBs": factored_bytes(n_rows, mesh.n_verts, "apply",
+                                                                packed=best["packed_rows"]) /
                                           (probe["warm"]["second_us"] * 1e-6) / 1e9},
                            "cache_flushed": probe["flushed"]}
     if args.cpu_baseline:

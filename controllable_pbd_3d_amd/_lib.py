@@ -105,6 +105,7 @@ SIGNATURES = {
     "mgpbd_mg_set_level0_factored": (c_int, [_V, c_int] + [_V] * 14),
     "mgpbd_mg_set_level1_factored": (c_int, [_V, c_int, _V, _V, _V, _V, c_int, _V, _V, _V, _V, _V, _V]),
     "mgpbd_mg_set_level0_offsets": (c_int, [_V, _V, c_int]),
+    "mgpbd_mg_set_level0_packed_rows": (c_int, [_V, _V, c_int, c_int]),
     "mgpbd_mg_factored_refill": (c_int, [_V]),
     "mgpbd_mg_factored_scatter": (c_int, [_V, _V, _V, _V]),
     "mgpbd_mg_level_residual": (c_int, [_V, c_int, _V, _V, _V]),

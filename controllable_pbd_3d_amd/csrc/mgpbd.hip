@@ -2341,13 +2341,34 @@ k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ 
   if (q == 0) u[v] = make_float4(ax, ay, az, 0.0f);
 }
 
+// The four vertices of a level-0 row for the tet passes: from the int4 list, or from one
+// packed 64-bit word (mgpbd_mg_set_level0_packed_rows): the first vertex in the low bb bits,
+// then the other three as signed ob-bit differences from it (8 bytes per row instead of 16).
+struct RowsI4 {
+  const int4* __restrict__ p;
+  __device__ __forceinline__ int4 operator()(int t) const { return p[t]; }
+};
+struct RowsPk {
+  const uint64_t* __restrict__ p;
+  int bb, ob;
+  __device__ __forceinline__ int4 operator()(int t) const {
+    const uint64_t w = p[t];
+    const int v0 = (int)(w & ((1ull << bb) - 1));
+    // field k (k = 1..3) at bits [bb + (k-1) ob, bb + k ob): shifted to the top, then
+    // arithmetic-shifted down to sign-extend it
+    const int d1 = (int)((int64_t)(w << (64 - bb - ob)) >> (64 - ob));
+    const int d2 = (int)((int64_t)(w << (64 - bb - 2 * ob)) >> (64 - ob));
+    const int d3 = (int)((int64_t)(w << (64 - bb - 3 * ob)) >> (64 - ob));
+    return make_int4(v0, v0 + d1, v0 + d2, v0 + d3);
+  }
+};
+
 // (A^ x)_t from u: a^_t x_t + K_t0 . u_v0 + ... + K_t3 . u_v3 (FP32)
-template <int VT>
-__device__ __forceinline__ float fac_row(int t, int n, const int4* __restrict__ t_i,
-                                         const v2u32* __restrict__ tk,
+template <int VT, class RS>
+__device__ __forceinline__ float fac_row(int t, int n, RS rows, const v2u32* __restrict__ tk,
                                          const float* __restrict__ ahat,
                                          const float4* __restrict__ u, float xt) {
-  const int4 vi = t_i[t];
+  const int4 vi = rows(t);
   const v2u32 q0 = __builtin_nontemporal_load(tk + t);
   const v2u32 q1 = __builtin_nontemporal_load(tk + (size_t)n + t);
   const v2u32 q2 = __builtin_nontemporal_load(tk + 2 * (size_t)n + t);
@@ -2369,9 +2390,9 @@ __device__ __forceinline__ float fac_row(int t, int n, const int4* __restrict__ 
 }
 
 // Factored counterpart of k_sellw on level 0 (same MODE semantics, rows = tets).
-template <int VT, int MODE>
+template <int VT, int MODE, class RS>
 __global__ void __launch_bounds__(kBlock)
-k_fac_apply(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
+k_fac_apply(int n, RS rows, const v2u32* __restrict__ tk,
             const float* __restrict__ ahat, const float4* __restrict__ u,
             const float* __restrict__ x, float* y, const float* __restrict__ b,
             float omega, const float* __restrict__ wp, const float* __restrict__ isd,
@@ -2381,7 +2402,7 @@ k_fac_apply(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const float xi = x[i];
-  const float ax = fac_row<VT>(i, n, t_i, tk, ahat, u, xi);
+  const float ax = fac_row<VT>(i, n, rows, tk, ahat, u, xi);
   if (MODE == 2) {
     y[i] = b[i] - ax;
   } else if (MODE == 1 || MODE == 3) {
@@ -2396,16 +2417,16 @@ k_fac_apply(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
 
 // Power step on the factored level (w = A^ v after k_fac_gather of v) with the partials
 // v.A^v, v.v, w.w of k_power_step_h; the grid covers at least n rows.
-template <int VT>
+template <int VT, class RS>
 __global__ void __launch_bounds__(kBlock)
-k_fac_power(int n, const int4* __restrict__ t_i, const v2u32* __restrict__ tk,
+k_fac_power(int n, RS rows, const v2u32* __restrict__ tk,
             const float* __restrict__ ahat, const float4* __restrict__ u,
             const float* __restrict__ v, float* __restrict__ w, RedWS ws) {
   double vav = 0.0, vv = 0.0, ww = 0.0;
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < n) {
     const float vi = v[i];
-    const float ax = fac_row<VT>(i, n, t_i, tk, ahat, u, vi);
+    const float ax = fac_row<VT>(i, n, rows, tk, ahat, u, vi);
     w[i] = ax;
     vav = (double)vi * ax;
     vv = (double)vi * vi;
@@ -2569,10 +2590,10 @@ k_fac_scatter(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs
   }
 }
 
-__device__ __forceinline__ double fac_orow(int t, const int4* __restrict__ t_i,
-                                           const float* __restrict__ gradC,
+template <class RS>
+__device__ __forceinline__ double fac_orow(int t, RS rows, const float* __restrict__ gradC,
                                            const double4* __restrict__ u, double acc) {
-  const int4 vi = t_i[t];
+  const int4 vi = rows(t);
   const float4* g4 = reinterpret_cast<const float4*>(gradC + 12 * (size_t)t);
   const float4 a = g4[0], b = g4[1], c = g4[2];
   const double4 u0 = u[vi.x], u1 = u[vi.y], u2 = u[vi.z], u3 = u[vi.w];
@@ -2594,9 +2615,9 @@ __device__ __forceinline__ double fac_orow(int t, const int4* __restrict__ t_i,
 // MODE 0: y = A x rounded to FP32 and the partial x.y (k_pcg_spmv_dot's outputs); MODE 1:
 // y = b - A x rounded once; MODE 2: the partial (b - A x)^2 (k_true_resid's). The grid covers
 // at least n rows; every block stores its partial (modes 0 and 2).
-template <typename XT, int MODE>
+template <typename XT, int MODE, class RS>
 __global__ void __launch_bounds__(kBlock)
-k_fac_oapply(int n, const int4* __restrict__ t_i, const float* __restrict__ gradC,
+k_fac_oapply(int n, RS rows, const float* __restrict__ gradC,
              const float* __restrict__ alpha, const double4* __restrict__ u,
              const XT* __restrict__ x, const float* __restrict__ b, float* __restrict__ y,
              RedWS ws, const int* __restrict__ done, const PcgState* __restrict__ gate_st) {
@@ -2606,7 +2627,7 @@ k_fac_oapply(int n, const int4* __restrict__ t_i, const float* __restrict__ grad
   double part = 0.0;
   if (i < n) {
     const double xi = (double)x[i];
-    const double ax = fac_orow(i, t_i, gradC, u, (double)alpha[i] * xi);
+    const double ax = fac_orow(i, rows, gradC, u, (double)alpha[i] * xi);
     if (MODE == 0) {
       const float af = (float)ax;
       y[i] = af;
@@ -2819,6 +2840,8 @@ struct mgpbd_ctx {
     const int* vs_inc = nullptr;
     const int* tbase = nullptr;   // per-vertex base tet of the 16-bit slot offsets (or NULL)
     int rows_u = 0;               // rows of the uniform vertex-slice layout (0: per slice)
+    const uint64_t* tpk = nullptr;  // packed row vertices (or NULL: t_i)
+    int pk_bb = 0, pk_ob = 0;       // bits of the first vertex and of each difference
     v2u32* vs_k = nullptr;   // K per vertex slot (4 halves)
     float4* vs_g = nullptr;  // FP32 gradient per vertex slot (the PCG's operator)
     v2u32* tk = nullptr;     // K per tet, three planes of n_tets
@@ -3688,6 +3711,14 @@ static float* vcycle_from(mgpbd_ctx* ctx, int l, const float* b, float* x_first,
 
 // ---------------- reduced-precision (scaled) V-cycle ----------------
 extern "C++" {
+// Calls f with the row-vertex source of the factored level 0's tet passes: the packed words
+// when mgpbd_mg_set_level0_packed_rows registered them, else the int4 list.
+template <class Fn>
+static void with_rows(const mgpbd_ctx::Factored& F, Fn&& f) {
+  if (F.tpk) f(RowsPk{F.tpk, F.pk_bb, F.pk_ob});
+  else f(RowsI4{F.t_i});
+}
+
 template <int VT>
 static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                          const float* b, const int* done, const float* wp, const float* xp,
@@ -3755,16 +3786,19 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
   fac_gather<VT>(ctx, x, done);
   const int g = nblocks(L.n);
   const float w = ctx->omega;
-#define FAC(M) k_fac_apply<VT, M><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b, w, \
-                                                       wp, L.isd, done, xp, cc)
-  switch (mode) {
-    case 1: FAC(1); break;
-    case 2: FAC(2); break;
-    case 3: FAC(3); break;
-    case 4: FAC(4); break;
-    default: FAC(5); break;
-  }
+  with_rows(F, [&](auto rows) {
+    using RS = decltype(rows);
+#define FAC(M) k_fac_apply<VT, M, RS><<<g, kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, x, y, b, \
+                                                           w, wp, L.isd, done, xp, cc)
+    switch (mode) {
+      case 1: FAC(1); break;
+      case 2: FAC(2); break;
+      case 3: FAC(3); break;
+      case 4: FAC(4); break;
+      default: FAC(5); break;
+    }
 #undef FAC
+  });
 }
 }  // extern "C++"
 
@@ -3936,10 +3970,16 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
     for (int it = 0; it < iters; ++it) {
       if (fac && ctx->precision == MGPBD_PREC_FP16) {
         fac_gather<1>(ctx, v, nullptr);
-        k_fac_power<1><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
+        with_rows(F, [&](auto rows) {
+          k_fac_power<1, decltype(rows)><<<g, kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, v, w,
+                                                              ctx->ws);
+        });
       } else if (fac) {
         fac_gather<2>(ctx, v, nullptr);
-        k_fac_power<2><<<g, kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, v, w, ctx->ws);
+        with_rows(F, [&](auto rows) {
+          k_fac_power<2, decltype(rows)><<<g, kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, v, w,
+                                                              ctx->ws);
+        });
       } else if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
         k_power_step_wf<<<g, kBlock, sizeof(float) * kWinSeg * L.win_segs, s>>>(
             L.n, L.n_slices, L.slice_ptr, L.woff, L.val, L.seg_ptr, L.segs, L.dinv, v, w, ctx->ws);
@@ -4141,6 +4181,22 @@ int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase, int rows_u
   return 0;
 }
 
+int mgpbd_mg_set_level0_packed_rows(mgpbd_ctx* ctx, const uint64_t* rows_dev, int base_bits,
+                                    int diff_bits) {
+  CTX_CHECK("mgpbd_mg_set_level0_packed_rows");
+  mgpbd_ctx::Factored& F = ctx->fac;
+  if (!F.on) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_packed_rows: no factored level 0");
+  if (rows_dev && (base_bits < 1 || base_bits > 31 || diff_bits < 2 || diff_bits > 20 ||
+                   base_bits + 3 * diff_bits > 64))
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_packed_rows: base_bits in [1, 31], "
+                                    "diff_bits in [2, 20], base_bits + 3 diff_bits <= 64");
+  drop_graphs(ctx);
+  F.tpk = rows_dev;
+  F.pk_bb = rows_dev ? base_bits : 0;
+  F.pk_ob = rows_dev ? diff_bits : 0;
+  return 0;
+}
+
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,
@@ -4304,14 +4360,17 @@ int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b, const float*
   };
   auto second = [&]() {   // the factored tet pass (nothing for a stored level)
     if (!fac) return;
-    if (ctx->precision == MGPBD_PREC_FP16)
-      k_fac_apply<1, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
-                                                       ctx->omega, nullptr, L.isd, nullptr,
-                                                       nullptr, nullptr);
-    else
-      k_fac_apply<2, 1><<<nblocks(L.n), kBlock, 0, s>>>(L.n, F.t_i, F.tk, F.ahat, F.u, x, y, b,
-                                                       ctx->omega, nullptr, L.isd, nullptr,
-                                                       nullptr, nullptr);
+    with_rows(F, [&](auto rows) {
+      using RS = decltype(rows);
+      if (ctx->precision == MGPBD_PREC_FP16)
+        k_fac_apply<1, 1, RS><<<nblocks(L.n), kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, x, y,
+                                                              b, ctx->omega, nullptr, L.isd,
+                                                              nullptr, nullptr, nullptr);
+      else
+        k_fac_apply<2, 1, RS><<<nblocks(L.n), kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, x, y,
+                                                              b, ctx->omega, nullptr, L.isd,
+                                                              nullptr, nullptr, nullptr);
+    });
   };
   const size_t ne = flush_floats > 0 ? 3 * (size_t)reps : 4;
   std::vector<hipEvent_t> e(ne, nullptr);
@@ -4406,20 +4465,26 @@ static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* x
   hipStream_t s = ctx->stream;
   if (xd) {
     fac_ogather<double>(ctx, xd, done, gate);
-    k_fac_oapply<double, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xd, b, y,
-                                                  ctx->ws, done, gate);
+    with_rows(F, [&](auto rows) {
+      k_fac_oapply<double, 1, decltype(rows)><<<gs, kBlock, 0, s>>>(n, rows, F.gradC, F.alpha,
+                                                                    F.u64, xd, b, y, ctx->ws,
+                                                                    done, gate);
+    });
     return;
   }
   fac_ogather<float>(ctx, xf, done, gate);
-  if (mode == 0)
-    k_fac_oapply<float, 0><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
-                                                 ctx->ws, done, gate);
-  else if (mode == 1)
-    k_fac_oapply<float, 1><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
-                                                 ctx->ws, done, gate);
-  else
-    k_fac_oapply<float, 2><<<gs, kBlock, 0, s>>>(n, F.t_i, F.gradC, F.alpha, F.u64, xf, b, y,
-                                                 ctx->ws, done, gate);
+  with_rows(F, [&](auto rows) {
+    using RS = decltype(rows);
+    if (mode == 0)
+      k_fac_oapply<float, 0, RS><<<gs, kBlock, 0, s>>>(n, rows, F.gradC, F.alpha, F.u64, xf, b,
+                                                       y, ctx->ws, done, gate);
+    else if (mode == 1)
+      k_fac_oapply<float, 1, RS><<<gs, kBlock, 0, s>>>(n, rows, F.gradC, F.alpha, F.u64, xf, b,
+                                                       y, ctx->ws, done, gate);
+    else
+      k_fac_oapply<float, 2, RS><<<gs, kBlock, 0, s>>>(n, rows, F.gradC, F.alpha, F.u64, xf, b,
+                                                       y, ctx->ws, done, gate);
+  });
 }
 
 // One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.

@@ -76,6 +76,27 @@ def uniform_rows(vptr: np.ndarray, max_growth: float = 1.25) -> int:
     return max(rows, 1)
 
 
+def packed_rows(rows: np.ndarray, n_verts: int):
+    """Level-0 row vertices packed one 64-bit word per row for the factored tet passes
+    (mgpbd_mg_set_level0_packed_rows): the first vertex in the low bb bits, the other three as
+    signed ob-bit differences from it. Returns (words uint64, bb, ob), or None when a
+    difference does not fit (the passes then read the int4 rows)."""
+    r = np.asarray(rows, np.int64).reshape(-1, 4)
+    bb = max(1, int(max(n_verts, 1) - 1).bit_length())
+    ob = min(20, (64 - bb) // 3)
+    if bb > 31 or ob < 2:
+        return None
+    d = r[:, 1:] - r[:, :1]
+    lim = 1 << (ob - 1)
+    if len(r) and (d.min() < -lim or d.max() >= lim):
+        return None
+    mask = (1 << ob) - 1
+    w = r[:, 0].astype(np.uint64)
+    for k in range(3):
+        w |= (d[:, k] & mask).astype(np.uint64) << np.uint64(bb + k * ob)
+    return w, bb, ob
+
+
 def vertex_slices(vptr: np.ndarray, vinc: np.ndarray, rows_uniform: int = 0):
     """Vertex-sliced incidence of the factored level 0, four lanes per vertex: vertices binned
     16 per 64-lane slice, slot k of vertex 16 s + l at vs_ptr[s] + 64 (k // 4) + 4 l + k % 4
@@ -378,7 +399,11 @@ class GPUMultigridSolver:
     col16_limit = 32768
     # factored level 0: 16-bit tet offsets in the vertex slots' unused words when they fit, and
     # then (when it costs at most 25 % more slots) every vertex slice as tall as the busiest
+    # vertex needs
     slot_offsets = True
+    # factored level 0: every row's four vertices in one 64-bit word for the tet passes when
+    # the differences to the first vertex fit
+    pack_rows = True
     # packed levels whose blocks of 256 rows read x from at most this many 32-column segments
     # stage them in LDS (windowed sweeps; the words then hold LDS offsets, so the 16-bit column
     # offset bound does not apply); 0 disables
@@ -477,6 +502,11 @@ class GPUMultigridSolver:
             f["tbase"] = torch.from_numpy(tbase).to(dev)
             f["rows_uniform"] = rows_u
             self.ctx("mgpbd_mg_set_level0_offsets", dptr(f["tbase"]), rows_u)
+        pk = packed_rows(src["t_i_host"], nv) if self.pack_rows else None
+        if pk is not None:
+            f["rows_packed"] = torch.from_numpy(pk[0].view(np.int64)).to(dev)
+            f["rows_bits"] = pk[1:]
+            self.ctx("mgpbd_mg_set_level0_packed_rows", dptr(f["rows_packed"]), pk[1], pk[2])
         if self.n_levels < 2:
             return
         C = self.levels[1]

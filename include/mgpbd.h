@@ -469,18 +469,6 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
                                  const int32_t* vs_cnt, const int32_t* vs_inc, uint32_t* vs_k,
                                  float* vs_g, uint32_t* tk, float* ahat, float* u, double* u64,
                                  float* dinv);
-/* Level 1 from the gradients with the factored level 0 (extension; replaces the level 0 -> 1
- * refill of cons/mgpcg_gpu.py:183-212 when level 0 is not assembled): with P[i, agg[i]] = 1,
- * A1 = G W G^T + P^T diag(alpha~) P, G_(I,v) = sum of gradC over the tets of aggregate I
- * holding vertex v. Per (aggregate, vertex) pair p < n_pairs: its incidences (4 tet + local
- * vertex) gmem[gptr[p] .. gptr[p+1]) and vertex gvert[p]; G (4 floats per pair, 16-byte
- * aligned) is written by mgpbd_mg_factored_refill. Per level-1 entry c < n_entries: the
- * pairs (a, b) of the vertices its two aggregates share, terms[2 m], terms[2 m + 1] for m in
- * [tptr[c], tptr[c+1]) (8-byte aligned), over the entries (I, J) with I <= J only (A1 is
- * symmetric): its SELL position cpos[c] in c_val (level 1's values), its transpose's
- * cpos_t[c] (the same on the diagonal), and cagg[c] = the aggregate when c is a diagonal
- * entry, else -1 (the diagonal adds the aggregate's alpha~ over level 0's member lists).
- * gptr NULL switches it off. */
 /* 16-bit slot offsets for the factored level 0's vertex passes (no reference counterpart):
  * tbase_dev (n_verts int32, device) holds a base tet per vertex such that every tet at the
  * vertex lies in [tbase, tbase + 65535]; the per-iteration conversion then writes each slot's
@@ -493,6 +481,26 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
  * loading the slice pointer and count (same sums bit for bit: the unused slots add exact
  * zeros). */
 int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase_dev, int rows_uniform);
+/* Packed row vertices for the factored level 0's tet passes (no reference counterpart): one
+ * 64-bit word per level-0 row (device, 8-byte aligned) holding the row's first vertex in the
+ * low base_bits bits and the other three as signed diff_bits-bit differences from it, in
+ * order above (base_bits + 3 diff_bits <= 64). The V-cycle's tet pass, the power steps and
+ * the PCG's tet pass then read 8 bytes per row instead of t_i's 16 (same vertices, same
+ * results bit for bit). NULL switches it off (t_i again). */
+int mgpbd_mg_set_level0_packed_rows(mgpbd_ctx* ctx, const uint64_t* rows_dev, int base_bits,
+                                    int diff_bits);
+/* Level 1 from the gradients with the factored level 0 (extension; replaces the level 0 -> 1
+ * refill of cons/mgpcg_gpu.py:183-212 when level 0 is not assembled): with P[i, agg[i]] = 1,
+ * A1 = G W G^T + P^T diag(alpha~) P, G_(I,v) = sum of gradC over the tets of aggregate I
+ * holding vertex v. Per (aggregate, vertex) pair p < n_pairs: its incidences (4 tet + local
+ * vertex) gmem[gptr[p] .. gptr[p+1]) and vertex gvert[p]; G (4 floats per pair, 16-byte
+ * aligned) is written by mgpbd_mg_factored_refill. Per level-1 entry c < n_entries: the
+ * pairs (a, b) of the vertices its two aggregates share, terms[2 m], terms[2 m + 1] for m in
+ * [tptr[c], tptr[c+1]) (8-byte aligned), over the entries (I, J) with I <= J only (A1 is
+ * symmetric): its SELL position cpos[c] in c_val (level 1's values), its transpose's
+ * cpos_t[c] (the same on the diagonal), and cagg[c] = the aggregate when c is a diagonal
+ * entry, else -1 (the diagonal adds the aggregate's alpha~ over level 0's member lists).
+ * gptr NULL switches it off. */
 int mgpbd_mg_set_level1_factored(mgpbd_ctx* ctx, int n_pairs, const int32_t* gptr,
                                  const int32_t* gmem, const int32_t* gvert, float* G,
                                  int n_entries, const int32_t* tptr, const int32_t* terms,

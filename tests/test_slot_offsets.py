@@ -6,7 +6,8 @@ import numpy as np
 
 from controllable_pbd_3d_amd import _lib
 from controllable_pbd_3d_amd.mesh import box_mesh
-from controllable_pbd_3d_amd.mgpcg_gpu import slot_offset_base, uniform_rows, vertex_slices
+from controllable_pbd_3d_amd.mgpcg_gpu import (packed_rows, slot_offset_base, uniform_rows,
+                                              vertex_slices)
 
 
 def _incidence(tets, nv):
@@ -69,3 +70,38 @@ def test_uniform_layout():
     t4[:40, 0] = hub
     vptr2, _ = _incidence(t4, m.n_verts)
     assert uniform_rows(vptr2) == 0
+
+
+def _unpack(words, bb, ob):
+    """Decodes packed row words with plain integer arithmetic (the kernels use shifts)."""
+    w = words.astype(object)
+    v0 = np.array([int(x) & ((1 << bb) - 1) for x in w], np.int64)
+    out = [v0]
+    for k in range(3):
+        f = np.array([(int(x) >> (bb + k * ob)) & ((1 << ob) - 1) for x in w], np.int64)
+        f = np.where(f >= 1 << (ob - 1), f - (1 << ob), f)
+        out.append(v0 + f)
+    return np.stack(out, axis=1)
+
+
+def test_packed_rows_round_trip():
+    """Every row's four vertices come back from its packed word, in order, on a block whose
+    tets span up to a layer of vertices (every other row reversed: differences of both
+    signs)."""
+    m = box_mesh(9, 0.1)
+    rows = m.tets.reshape(-1, 4).astype(np.int64).copy()
+    rows[1::2] = rows[1::2, ::-1]
+    pk = packed_rows(rows, m.n_verts)
+    assert pk is not None
+    words, bb, ob = pk
+    assert bb == int(m.n_verts - 1).bit_length() and bb + 3 * ob <= 64
+    d = rows[:, 1:] - rows[:, :1]
+    assert d.min() < 0 < d.max()
+    assert np.array_equal(_unpack(words, bb, ob), rows)
+
+
+def test_packed_rows_refuse_differences_that_do_not_fit():
+    rows = np.array([[0, 1, 2, 3], [5, 0, 1 << 21, 7]], np.int64)
+    assert packed_rows(rows, 1 << 22) is None          # 22 base bits leave 14 per difference
+    pk = packed_rows(rows[:1], 16)
+    assert pk is not None and np.array_equal(_unpack(*pk), rows[:1])
