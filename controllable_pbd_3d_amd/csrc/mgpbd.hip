@@ -76,12 +76,35 @@ __device__ __forceinline__ void store_partial(RedWS ws, int slot, double v) {
   if (threadIdx.x == 0) ws.partials[(size_t)slot * kPartialStride + blockIdx.x] = v;
 }
 
-// Phase 2 helper (one block): total of slot `slot` over `nparts` block partials (thread 0).
+// Phase 2 helper (one block): totals of slots first .. first + K - 1 over `nparts` block
+// partials (thread 0). Each thread keeps four independent partial sums per slot, so up to
+// 4 K loads are in flight at once instead of one dependent load per step (the partials come
+// from other XCDs' L2 write-backs: each round trip is long); the order is fixed.
+template <int K>
+__device__ __forceinline__ void sum_partials_k(RedWS ws, int first, int nparts,
+                                               double (&out)[K]) {
+  const double* p = ws.partials + (size_t)first * kPartialStride;
+  double a[K][4];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k][0] = a[k][1] = a[k][2] = a[k][3] = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * kBlock < nparts; i += 4 * kBlock) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) a[k][j] += p[(size_t)k * kPartialStride + i + j * kBlock];
+  }
+  for (; i < nparts; i += kBlock)
+#pragma unroll
+    for (int k = 0; k < K; ++k) a[k][0] += p[(size_t)k * kPartialStride + i];
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[k] = block_sum((a[k][0] + a[k][1]) + (a[k][2] + a[k][3]));
+}
+
 __device__ __forceinline__ double sum_partials(RedWS ws, int slot, int nparts) {
-  const double* p = ws.partials + (size_t)slot * kPartialStride;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += kBlock) s += p[i];
-  return block_sum(s);
+  double t[1];
+  sum_partials_k<1>(ws, slot, nparts, t);
+  return t[0];
 }
 
 // out[s] = total of slot s, for s in [first, nslots)
@@ -793,9 +816,9 @@ k_power_step(int n, int n_slices, const int* __restrict__ slice_ptr, const int* 
 // pw[0] = lambda = v.Av / v.Dv, pw[1] = ||w||_D, pw[2] = the previous iteration's lambda
 __global__ void __launch_bounds__(kBlock)
 k_power_fin(RedWS ws, int nparts, double* __restrict__ pw) {
-  const double vav = sum_partials(ws, 0, nparts);
-  const double vdv = sum_partials(ws, 1, nparts);
-  const double wdw = sum_partials(ws, 2, nparts);
+  double t[3];
+  sum_partials_k<3>(ws, 0, nparts, t);
+  const double vav = t[0], vdv = t[1], wdw = t[2];
   if (threadIdx.x != 0) return;
   pw[2] = pw[0];
   pw[0] = vdv > 0.0 ? vav / vdv : 0.0;
@@ -807,9 +830,9 @@ k_power_fin(RedWS ws, int nparts, double* __restrict__ pw) {
 __global__ void __launch_bounds__(kBlock)
 k_power_fin_norm(int n, RedWS ws, int nparts, const float* __restrict__ w,
                  float* __restrict__ v, double* __restrict__ pw) {
-  const double vav = sum_partials(ws, 0, nparts);
-  const double vdv = sum_partials(ws, 1, nparts);
-  const double wdw = sum_partials(ws, 2, nparts);
+  double t[3];
+  sum_partials_k<3>(ws, 0, nparts, t);
+  const double vav = t[0], vdv = t[1], wdw = t[2];
   __shared__ double nrm_s;
   if (threadIdx.x == 0) {
     nrm_s = sqrt(wdw);
@@ -1281,9 +1304,9 @@ k_pcg_guess_norms(int n, const float* __restrict__ b, const float* __restrict__ 
 // The stopping test stays relative to ||b|| (or absolute), as from a zero start.
 __global__ void __launch_bounds__(kBlock)
 k_pcg_fin_guess(RedWS ws, int nparts, int has_alt, PcgState* st) {
-  const double bb = sum_partials(ws, 0, nparts);
-  const double ra = sum_partials(ws, 1, nparts);
-  const double rb = sum_partials(ws, 2, nparts);
+  double t[3];
+  sum_partials_k<3>(ws, 0, nparts, t);
+  const double bb = t[0], ra = t[1], rb = t[2];
   if (threadIdx.x != 0) return;
   st->bb = bb;
   if (st->relative) st->tol2 = st->tol * st->tol * bb;
@@ -2115,8 +2138,9 @@ k_pcg_rz2(int n, const float* __restrict__ r, const float* __restrict__ z,
 __global__ void __launch_bounds__(kBlock)
 k_pcg_fin_flex(RedWS ws, int nparts, PcgState* st) {
   if (st->done) return;
-  const double rz = sum_partials(ws, 0, nparts);
-  const double rzo = sum_partials(ws, 1, nparts);
+  double t[2];
+  sum_partials_k<2>(ws, 0, nparts, t);
+  const double rz = t[0], rzo = t[1];
   if (threadIdx.x != 0) return;
   st->iters = st->k + 1;
   if (fabs(st->rz) < 1e-30) {
