@@ -2185,6 +2185,10 @@ k_pcg_p_flex(int n, const float* __restrict__ z, float* __restrict__ p, float* _
 // the 72.5M FP32 entries.
 // ------------------------------------------------------------------------------------------
 typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
+// Per-vertex products u of the vertex pass: three floats, 12 bytes (one dwordx3 access)
+struct fvec3 {
+  float x, y, z;
+};
 
 template <int VT>
 __device__ __forceinline__ uint32_t pack_h2(float a, float b) {
@@ -2285,7 +2289,7 @@ __device__ __forceinline__ void fac_gather_lane(int v, int q, const int* __restr
                                                 const int* __restrict__ vs_inc, int tb,
                                                 const v2u32* __restrict__ vs_k,
                                                 const float* __restrict__ x,
-                                                float4* __restrict__ u) {
+                                                fvec3* __restrict__ u) {
   const int base = fac_slot0(vs_ptr, v, q);
   const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -2312,7 +2316,7 @@ __device__ __forceinline__ void fac_gather_lane(int v, int q, const int* __restr
   }
   ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
   ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
-  if (q == 0) u[v] = make_float4(ax, ay, az, 0.0f);
+  if (q == 0) u[v] = fvec3{ax, ay, az};
 }
 
 template <int VT>
@@ -2320,7 +2324,7 @@ __global__ void __launch_bounds__(kBlock)
 k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
              const int* __restrict__ vs_inc, const int* __restrict__ tbase,
              const v2u32* __restrict__ vs_k, const float* __restrict__ x,
-             float4* __restrict__ u, const int* __restrict__ done) {
+             fvec3* __restrict__ u, const int* __restrict__ done) {
   if (done && *done) return;
   const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int v = gid >> 2, q = gid & 3;
@@ -2339,7 +2343,7 @@ k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_
 template <int VT, int R>
 __global__ void __launch_bounds__(kBlock)
 k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ vs_k,
-               const float* __restrict__ x, float4* __restrict__ u,
+               const float* __restrict__ x, fvec3* __restrict__ u,
                const int* __restrict__ done) {
   if (done && *done) return;
   const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -2362,7 +2366,7 @@ k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ 
   }
   ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
   ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
-  if (q == 0) u[v] = make_float4(ax, ay, az, 0.0f);
+  if (q == 0) u[v] = fvec3{ax, ay, az};
 }
 
 // The four vertices of a level-0 row for the tet passes: from the int4 list, or from one
@@ -2391,12 +2395,12 @@ struct RowsPk {
 template <int VT, class RS>
 __device__ __forceinline__ float fac_row(int t, int n, RS rows, const v2u32* __restrict__ tk,
                                          const float* __restrict__ ahat,
-                                         const float4* __restrict__ u, float xt) {
+                                         const fvec3* __restrict__ u, float xt) {
   const int4 vi = rows(t);
   const v2u32 q0 = __builtin_nontemporal_load(tk + t);
   const v2u32 q1 = __builtin_nontemporal_load(tk + (size_t)n + t);
   const v2u32 q2 = __builtin_nontemporal_load(tk + 2 * (size_t)n + t);
-  const float4 u0 = u[vi.x], u1 = u[vi.y], u2 = u[vi.z], u3 = u[vi.w];
+  const fvec3 u0 = u[vi.x], u1 = u[vi.y], u2 = u[vi.z], u3 = u[vi.w];
   float acc = ahat[t] * xt;
   acc = fmaf(lo_h<VT>(q0.x), u0.x, acc);
   acc = fmaf(hi_h<VT>(q0.x), u0.y, acc);
@@ -2417,7 +2421,7 @@ __device__ __forceinline__ float fac_row(int t, int n, RS rows, const v2u32* __r
 template <int VT, int MODE, class RS>
 __global__ void __launch_bounds__(kBlock)
 k_fac_apply(int n, RS rows, const v2u32* __restrict__ tk,
-            const float* __restrict__ ahat, const float4* __restrict__ u,
+            const float* __restrict__ ahat, const fvec3* __restrict__ u,
             const float* __restrict__ x, float* y, const float* __restrict__ b,
             float omega, const float* __restrict__ wp, const float* __restrict__ isd,
             const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
@@ -2444,7 +2448,7 @@ k_fac_apply(int n, RS rows, const v2u32* __restrict__ tk,
 template <int VT, class RS>
 __global__ void __launch_bounds__(kBlock)
 k_fac_power(int n, RS rows, const v2u32* __restrict__ tk,
-            const float* __restrict__ ahat, const float4* __restrict__ u,
+            const float* __restrict__ ahat, const fvec3* __restrict__ u,
             const float* __restrict__ v, float* __restrict__ w, RedWS ws) {
   double vav = 0.0, vv = 0.0, ww = 0.0;
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
@@ -2870,7 +2874,7 @@ struct mgpbd_ctx {
     float4* vs_g = nullptr;  // FP32 gradient per vertex slot (the PCG's operator)
     v2u32* tk = nullptr;     // K per tet, three planes of n_tets
     float* ahat = nullptr;   // a^ per tet
-    float4* u = nullptr;     // per-vertex products of the V-cycle's gather pass
+    fvec3* u = nullptr;     // per-vertex products of the V-cycle's gather pass
     double4* u64 = nullptr;  // per-vertex products of the PCG operator's gather pass
     float* dinv = nullptr;   // 1 / A_tt from the gradients (level 0's dinv array)
     // level 1 from the gradients (mgpbd_mg_set_level1_factored)
@@ -4184,7 +4188,7 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
   F.vs_g = reinterpret_cast<float4*>(vs_g);
   F.tk = reinterpret_cast<v2u32*>(tk);
   F.ahat = ahat;
-  F.u = reinterpret_cast<float4*>(u);
+  F.u = reinterpret_cast<fvec3*>(u);
   F.u64 = reinterpret_cast<double4*>(u64);
   F.dinv = dinv;
   F.galerkin = false;
