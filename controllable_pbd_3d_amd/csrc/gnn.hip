@@ -1,7 +1,7 @@
 // Learned warm start for the PCG (extension; no reference counterpart): a message-passing
 // network over the tet face-adjacency graph that predicts dlambda of the first Newton
 // iteration from per-constraint features. Forward and backward run here; the three
-// 128 -> 64 message-passing layers use BF16 MFMA (v_mfma_f32_32x32x8bf16_1k) with FP32
+// 128 -> 64 message-passing layers use BF16 MFMA (gfx950 v_mfma_f32_32x32x16_bf16) with FP32
 // accumulation, the 8 -> 64 encoder and 64 -> 1 decoder are FP32.
 //
 // Network (n tets, H = 64):
@@ -51,12 +51,28 @@ __host__ __device__ constexpr int n_params() { return off_bout() + 1; }
 // ------------------------------------------------------------------------------------------
 // MFMA GEMM: Y[n x 64] = act(A1 W1 + A2 W2 + bias), A* BF16 n x 64, W* as out-major BF16
 // [64 out][64 in]. One wavefront computes 32 rows x 64 columns (two 32x32 MFMA tiles).
-// MFMA 32x32x8 operand layout (wave64): A fragment lane l holds A[l%32][4(l/32) .. +3],
-// B fragment holds B[4(l/32) .. +3][l%32]; result register r of lane l is
-// D[8(r/4) + 4(l/32) + r%4][l%32].
+// gfx950's v_mfma_f32_32x32x16_bf16 operand layout (wave64): A fragment lane l holds
+// A[l%32][8(l/32) .. +7], B fragment holds B[8(l/32) .. +7][l%32]; result register r of lane
+// l is D[8(r/4) + 4(l/32) + r%4][l%32]. A 64-long inner dimension is four instructions.
 // ------------------------------------------------------------------------------------------
+typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
+typedef short v8i16 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v16f32 mfma16(v8i16 a, v8i16 b, v16f32 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf16, a),
+                                                 __builtin_bit_cast(v8bf16, b), c, 0, 0, 0);
+}
 __device__ __forceinline__ v4i16 load4(const uint16_t* p) {
   return *reinterpret_cast<const v4i16*>(p);
+}
+// eight BF16 values at a 16-byte aligned address (one 16-byte load)
+__device__ __forceinline__ v8i16 load8(const uint16_t* p) {
+  return *reinterpret_cast<const v8i16*>(p);
+}
+// eight BF16 values at an 8-byte aligned address (LDS rows padded to 68 halves): two loads
+__device__ __forceinline__ v8i16 load8_lds(const uint16_t* p) {
+  const v4i16 lo = load4(p), hi = load4(p + 4);
+  return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -68,25 +84,25 @@ k_gemm64(int n, const uint16_t* __restrict__ A1, const uint16_t* __restrict__ W1
   const int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
   if (base >= n) return;
   const int ra = base + (lane & 31);
-  const int kq = 4 * (lane >> 5);
+  const int kq = 8 * (lane >> 5);
   v16f32 acc0 = {}, acc1 = {};
-  const v4i16 zero = {0, 0, 0, 0};
+  const v8i16 zero = {};
 #pragma unroll
-  for (int s = 0; s < H / 8; ++s) {
-    const v4i16 a = ra < n ? load4(A1 + (size_t)ra * H + 8 * s + kq) : zero;
-    const v4i16 b0 = load4(W1t + (size_t)(lane & 31) * H + 8 * s + kq);
-    const v4i16 b1 = load4(W1t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+  for (int s = 0; s < H / 16; ++s) {
+    const v8i16 a = ra < n ? load8(A1 + (size_t)ra * H + 16 * s + kq) : zero;
+    const v8i16 b0 = load8(W1t + (size_t)(lane & 31) * H + 16 * s + kq);
+    const v8i16 b1 = load8(W1t + (size_t)(32 + (lane & 31)) * H + 16 * s + kq);
+    acc0 = mfma16(a, b0, acc0);
+    acc1 = mfma16(a, b1, acc1);
   }
   if (A2) {
 #pragma unroll
-    for (int s = 0; s < H / 8; ++s) {
-      const v4i16 a = ra < n ? load4(A2 + (size_t)ra * H + 8 * s + kq) : zero;
-      const v4i16 b0 = load4(W2t + (size_t)(lane & 31) * H + 8 * s + kq);
-      const v4i16 b1 = load4(W2t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    for (int s = 0; s < H / 16; ++s) {
+      const v8i16 a = ra < n ? load8(A2 + (size_t)ra * H + 16 * s + kq) : zero;
+      const v8i16 b0 = load8(W2t + (size_t)(lane & 31) * H + 16 * s + kq);
+      const v8i16 b1 = load8(W2t + (size_t)(32 + (lane & 31)) * H + 16 * s + kq);
+      acc0 = mfma16(a, b0, acc0);
+      acc1 = mfma16(a, b1, acc1);
     }
   }
 #pragma unroll
@@ -114,20 +130,20 @@ k_gemm64_dual(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict_
   const int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
   if (base >= n) return;
   const int ra = base + (lane & 31);
-  const int kq = 4 * (lane >> 5);
+  const int kq = 8 * (lane >> 5);
   v16f32 acc0 = {}, acc1 = {}, acc2 = {}, acc3 = {};
-  const v4i16 zero = {0, 0, 0, 0};
+  const v8i16 zero = {};
 #pragma unroll
-  for (int s = 0; s < H / 8; ++s) {
-    const v4i16 a = ra < n ? load4(A + (size_t)ra * H + 8 * s + kq) : zero;
-    const v4i16 b0 = load4(W1t + (size_t)(lane & 31) * H + 8 * s + kq);
-    const v4i16 b1 = load4(W1t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
-    const v4i16 b2 = load4(W2t + (size_t)(lane & 31) * H + 8 * s + kq);
-    const v4i16 b3 = load4(W2t + (size_t)(32 + (lane & 31)) * H + 8 * s + kq);
-    acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
-    acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b2, acc2, 0, 0, 0);
-    acc3 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b3, acc3, 0, 0, 0);
+  for (int s = 0; s < H / 16; ++s) {
+    const v8i16 a = ra < n ? load8(A + (size_t)ra * H + 16 * s + kq) : zero;
+    const v8i16 b0 = load8(W1t + (size_t)(lane & 31) * H + 16 * s + kq);
+    const v8i16 b1 = load8(W1t + (size_t)(32 + (lane & 31)) * H + 16 * s + kq);
+    const v8i16 b2 = load8(W2t + (size_t)(lane & 31) * H + 16 * s + kq);
+    const v8i16 b3 = load8(W2t + (size_t)(32 + (lane & 31)) * H + 16 * s + kq);
+    acc0 = mfma16(a, b0, acc0);
+    acc1 = mfma16(a, b1, acc1);
+    acc2 = mfma16(a, b2, acc2);
+    acc3 = mfma16(a, b3, acc3);
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -182,7 +198,7 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
     lds_store8(wl[m] + r * kLdsRow + 8 * ch, *reinterpret_cast<const v8u16*>(src));
   }
   const int n_tiles = (n + 31) / 32;
-  const int kq = 4 * (lane >> 5), rl = lane & 31;
+  const int kq = 8 * (lane >> 5), rl = lane & 31;
   uint16_t* ts = tl[w][0];
   uint16_t* tm = tl[w][1];
   const float bias0 = bias[rl], bias1 = bias[32 + rl];
@@ -241,20 +257,20 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
     __syncthreads();
     v16f32 acc0 = {}, acc1 = {};
 #pragma unroll
-    for (int s = 0; s < H / 8; ++s) {
-      const v4i16 a = load4(ts + rl * kLdsRow + 8 * s + kq);
-      const v4i16 b0 = load4(wl[0] + rl * kLdsRow + 8 * s + kq);
-      const v4i16 b1 = load4(wl[0] + (32 + rl) * kLdsRow + 8 * s + kq);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    for (int s = 0; s < H / 16; ++s) {
+      const v8i16 a = load8_lds(ts + rl * kLdsRow + 16 * s + kq);
+      const v8i16 b0 = load8_lds(wl[0] + rl * kLdsRow + 16 * s + kq);
+      const v8i16 b1 = load8_lds(wl[0] + (32 + rl) * kLdsRow + 16 * s + kq);
+      acc0 = mfma16(a, b0, acc0);
+      acc1 = mfma16(a, b1, acc1);
     }
 #pragma unroll
-    for (int s = 0; s < H / 8; ++s) {
-      const v4i16 a = load4(tm + rl * kLdsRow + 8 * s + kq);
-      const v4i16 b0 = load4(wl[1] + rl * kLdsRow + 8 * s + kq);
-      const v4i16 b1 = load4(wl[1] + (32 + rl) * kLdsRow + 8 * s + kq);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc1, 0, 0, 0);
+    for (int s = 0; s < H / 16; ++s) {
+      const v8i16 a = load8_lds(tm + rl * kLdsRow + 16 * s + kq);
+      const v8i16 b0 = load8_lds(wl[1] + rl * kLdsRow + 16 * s + kq);
+      const v8i16 b1 = load8_lds(wl[1] + (32 + rl) * kLdsRow + 16 * s + kq);
+      acc0 = mfma16(a, b0, acc0);
+      acc1 = mfma16(a, b1, acc1);
     }
     __syncthreads();   // every fragment read of the tile is done before it is overwritten
 #pragma unroll
@@ -282,10 +298,10 @@ constexpr int kColRows = 256;   // rows per block of the column-sum and encoder-
 
 // With A2 (nullable) the same block also forms P2 = A2^T dZ from the dZ fragments it loaded
 // for P, and with Pb (nullable) the column sums of dZ over its rows (the bias gradient): the
-// two waves with qk = 0 add the dZ values of their fragments (each lane four rows of one
+// two waves with qk = 0 add the dZ values of their fragments (each lane eight rows of one
 // column), and lanes l and l + 32 (the two row halves) combine in a fixed order. The rows go
 // through LDS 64 at a time (16-byte row-chunk loads, rows padded to 68 halves); the MFMA
-// sequence over 8-row steps is the same as reading the fragments from memory.
+// sequence over 16-row steps is the same as reading the fragments from memory.
 constexpr int kWgChunk = 64;
 __global__ void __launch_bounds__(kBlock)
 k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
@@ -299,7 +315,7 @@ k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
   const int qk = w >> 1, qj = w & 1;
   const int r0 = blockIdx.x * kWgradRows;
   const int r1 = min(n, r0 + kWgradRows);
-  const int kq = 4 * (lane >> 5);
+  const int kq = 8 * (lane >> 5);
   const int ca = 32 * qk + (lane & 31), cz = 32 * qj + (lane & 31);
   v16f32 acc = {}, acc2 = {};
   float cs = 0.0f;
@@ -318,31 +334,25 @@ k_wgrad(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict__ dZ,
     }
     __syncthreads();
     const int rows = min(kWgChunk, r1 - c0);
-    for (int s0 = 0; s0 < rows; s0 += 8) {
+    // rows past r1 in the chunk are zero in LDS, so a 16-row step may run past `rows`
+    for (int s0 = 0; s0 < rows; s0 += 16) {
       const int i = s0 + kq;
-      v4i16 a, b;
-      a.x = (short)sa[(i + 0) * kLdsRow + ca];
-      a.y = (short)sa[(i + 1) * kLdsRow + ca];
-      a.z = (short)sa[(i + 2) * kLdsRow + ca];
-      a.w = (short)sa[(i + 3) * kLdsRow + ca];
-      b.x = (short)sz[(i + 0) * kLdsRow + cz];
-      b.y = (short)sz[(i + 1) * kLdsRow + cz];
-      b.z = (short)sz[(i + 2) * kLdsRow + cz];
-      b.w = (short)sz[(i + 3) * kLdsRow + cz];
-      acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+      v8i16 a, b;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        a[e] = (short)sa[(i + e) * kLdsRow + ca];
+        b[e] = (short)sz[(i + e) * kLdsRow + cz];
+      }
+      acc = mfma16(a, b, acc);
       if (A2) {
-        v4i16 a2;
-        a2.x = (short)sa2[(i + 0) * kLdsRow + ca];
-        a2.y = (short)sa2[(i + 1) * kLdsRow + ca];
-        a2.z = (short)sa2[(i + 2) * kLdsRow + ca];
-        a2.w = (short)sa2[(i + 3) * kLdsRow + ca];
-        acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, b, acc2, 0, 0, 0);
+        v8i16 a2;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a2[e] = (short)sa2[(i + e) * kLdsRow + ca];
+        acc2 = mfma16(a2, b, acc2);
       }
       if (Pb && qk == 0) {
-        cs += bf2f((uint16_t)b.x);
-        cs += bf2f((uint16_t)b.y);
-        cs += bf2f((uint16_t)b.z);
-        cs += bf2f((uint16_t)b.w);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) cs += bf2f((uint16_t)b[e]);
       }
     }
     __syncthreads();   // the chunk's LDS rows are free for the next chunk

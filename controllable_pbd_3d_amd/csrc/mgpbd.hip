@@ -1171,6 +1171,37 @@ k_refill(int nc, const int* __restrict__ off, const int* __restrict__ fpos,
   coarse[cpos[c]] = v;
 }
 
+// Sum over the members of aggregate I (ascending) of f(i) = r_i (or sd_i r_i) in the member
+// order: the member indices of a batch of up to kAggBatch are loaded first, then their values,
+// then summed, so an aggregate (at most 8 members, a few more after singleton merges) costs two
+// dependent load rounds instead of two per member. Members past the end re-read the last one
+// and are dropped, so the sum equals the one-member-at-a-time loop bit for bit.
+constexpr int kAggBatch = 8;
+template <bool SCALED>
+__device__ __forceinline__ float agg_sum(int I, const int* __restrict__ mptr,
+                                         const int* __restrict__ mem,
+                                         const float* __restrict__ sd,
+                                         const float* __restrict__ r) {
+  const int m0 = mptr[I], m1 = mptr[I + 1];
+  float s = 0.0f;
+  for (int mb = m0; mb < m1; mb += kAggBatch) {
+    int id[kAggBatch];
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) id[j] = mem[min(mb + j, m1 - 1)];
+    float rv[kAggBatch], sv[kAggBatch];
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) {
+      rv[j] = r[id[j]];
+      sv[j] = SCALED ? sd[id[j]] : 1.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) {
+      if (mb + j < m1) s = SCALED ? fmaf(sv[j], rv[j], s) : s + rv[j];
+    }
+  }
+  return s;
+}
+
 // b_c[I] = sum of r over the members of aggregate I (ascending); with dinv_c given, also the
 // coarse level's first Jacobi sweep from zero, x_c = (omega dinv_c) b_c.
 __global__ void __launch_bounds__(kBlock)
@@ -1181,8 +1212,7 @@ k_restrict(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
   if (done && *done) return;
   int I = blockIdx.x * kBlock + threadIdx.x;
   if (I >= nc) return;
-  float s = 0.0f;
-  for (int m = mptr[I]; m < mptr[I + 1]; ++m) s = s + r[mem[m]];
+  const float s = agg_sum<false>(I, mptr, mem, nullptr, r);
   bc[I] = s;
   if (xc) xc[I] = (omega * dinv_c[I]) * s;
 }
@@ -2080,11 +2110,7 @@ k_restrict_h(int nc, const int* __restrict__ mptr, const int* __restrict__ mem,
   if (done && *done) return;
   const int I = blockIdx.x * kBlock + threadIdx.x;
   if (I >= nc) return;
-  float s = 0.0f;
-  for (int m = mptr[I]; m < mptr[I + 1]; ++m) {
-    const int i = mem[m];
-    s = fmaf(sd_f[i], r[i], s);
-  }
+  const float s = agg_sum<true>(I, mptr, mem, sd_f, r);
   const float v = isd_c[I] * s;
   bc[I] = v;
   if (xc) xc[I] = omega * v;
