@@ -112,6 +112,7 @@ SIGNATURES = {
     "mgpbd_mg_vcycle": (c_int, [_V, _V, _V]),
     "mgpbd_mg_probe_level0": (c_int, [_V, c_int, _V, _V, _V, _V, c_int64, F64P]),
     "mgpbd_mg_set_cycle": (c_int, [_V, c_int, c_float]),
+    "mgpbd_mg_set_launch_fusion": (c_int, [_V, c_int]),
     "mgpbd_mg_set_level_smoothing": (c_int, [_V, c_int, c_int]),
     "mgpbd_mg_set_smoother": (c_int, [_V, c_int, c_float, c_float, c_int, c_float]),
     "mgpbd_mg_update_smoother": (c_int, [_V, F64P]),

@@ -1202,6 +1202,40 @@ __device__ __forceinline__ float agg_sum(int I, const int* __restrict__ mptr,
   return s;
 }
 
+// Prolongation in push form, fused into the kernel that writes a coarse level's final values:
+// coarse row I adds its correction to the members of its aggregate on the next finer level,
+// x_j = fmaf((sd_j isd_I) g, xc_I, x_j) for j in mem[mptr[I] .. mptr[I + 1]) -- the arithmetic of
+// k_prolong_h on the same values, so the result is bit-identical to a separate prolongation
+// launch. Every fine row belongs to exactly one aggregate, so no two threads write one x_j.
+// x == nullptr: nothing to push.
+struct Push {
+  const int* mptr;
+  const int* mem;
+  const float* sd_f;
+  const float* isd_c;
+  float* x;
+  float g;
+};
+
+__device__ __forceinline__ void push_prolong(const Push& p, int I, float xc) {
+  const int m0 = p.mptr[I], m1 = p.mptr[I + 1];
+  const float sc = p.isd_c[I];
+  for (int mb = m0; mb < m1; mb += kAggBatch) {
+    int id[kAggBatch];
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) id[j] = p.mem[min(mb + j, m1 - 1)];
+    float sv[kAggBatch], xv[kAggBatch];
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) {
+      sv[j] = p.sd_f[id[j]];
+      xv[j] = p.x[id[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j)
+      if (mb + j < m1) p.x[id[j]] = fmaf((sv[j] * sc) * p.g, xc, xv[j]);
+  }
+}
+
 // b_c[I] = sum of r over the members of aggregate I (ascending); with dinv_c given, also the
 // coarse level's first Jacobi sweep from zero, x_c = (omega dinv_c) b_c.
 __global__ void __launch_bounds__(kBlock)
@@ -1507,7 +1541,7 @@ __global__ void __launch_bounds__(kBlock)
 k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* __restrict__ p,
              const float* __restrict__ Ap, const float* __restrict__ dinv0,
              float* __restrict__ xv, float omega, const float* __restrict__ wp, RedWS ws,
-             const PcgState* st, double* __restrict__ x64) {
+             const PcgState* st, double* __restrict__ x64, float* __restrict__ bh) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   if (st->done) return;
   const float al = (float)st->alpha;
@@ -1518,7 +1552,13 @@ k_pcg_update(int n, float* __restrict__ x, float* __restrict__ r, const float* _
     else x[i] = fmaf(al, p[i], x[i]);
     float ri = fmaf(-al, Ap[i], r[i]);
     r[i] = ri;
-    if (xv) xv[i] = (omega * dinv0[i]) * ri;
+    if (bh) {   // scaled level 0 (dinv0 = isd): k_half_first's b^ and first sweep
+      const float v = dinv0[i] * ri;
+      bh[i] = v;
+      xv[i] = omega * v;
+    } else if (xv) {
+      xv[i] = (omega * dinv0[i]) * ri;
+    }
     acc += (double)ri * (double)ri;
   }
   store_partial(ws, 0, block_sum(acc));
@@ -1641,24 +1681,27 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
         const uint32_t* __restrict__ hpk, const uint16_t* __restrict__ hval,
         const float* __restrict__ x, float* y, const float* __restrict__ b,
         float omega, const float* __restrict__ wp, const float* __restrict__ isd, const int* __restrict__ done,
-        const float* xp, const float* __restrict__ cc) {
+        const float* xp, const float* __restrict__ cc, Push push) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
   const float ax = sellh_row<VT, PACKED>(i, slice_ptr, idx, hpk, hval, x);
   if (i >= n) return;
+  float out;
   if (MODE == 2) {
-    y[i] = b[i] - ax;
+    out = b[i] - ax;
   } else if (MODE == 1 || MODE == 3) {
     const float xn = fmaf(omega, b[i] - ax, x[i]);
-    y[i] = MODE == 3 ? isd[i] * xn : xn;
+    out = MODE == 3 ? isd[i] * xn : xn;
   } else {
     const float xi = x[i];
     const float u = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
     const float xn = fmaf(cc[1], b[i] - ax, u);
-    y[i] = MODE == 5 ? isd[i] * xn : xn;
+    out = MODE == 5 ? isd[i] * xn : xn;
   }
+  y[i] = out;
+  if (push.x) push_prolong(push, i, out);
 }
 
 // Power iteration on the scaled reduced-precision level (unit diagonal, so D = I in these
@@ -1761,7 +1804,7 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
         const int* __restrict__ seg_ptr, const int* __restrict__ segs,
         const float* __restrict__ x, float* y, const float* __restrict__ b,
         float omega, const float* __restrict__ wp, const float* __restrict__ isd,
-        const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
+        const int* __restrict__ done, const float* xp, const float* __restrict__ cc, Push push) {
   extern __shared__ float xs[];
   if (wp) omega = *wp;
   if (done && *done) return;
@@ -1771,17 +1814,20 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
   if ((i >> 6) >= n_slices) return;
   const float ax = sellw_row<VT>(i, slice_ptr, hpk, xs);
   if (i >= n) return;
+  float out;
   if (MODE == 2) {
-    y[i] = b[i] - ax;
+    out = b[i] - ax;
   } else if (MODE == 1 || MODE == 3) {
     const float xn = fmaf(omega, b[i] - ax, x[i]);
-    y[i] = MODE == 3 ? isd[i] * xn : xn;
+    out = MODE == 3 ? isd[i] * xn : xn;
   } else {
     const float xi = x[i];
     const float u = fmaf(cc[0], xi - (xp ? xp[i] : 0.0f), xi);
     const float xn = fmaf(cc[1], b[i] - ax, u);
-    y[i] = MODE == 5 ? isd[i] * xn : xn;
+    out = MODE == 5 ? isd[i] * xn : xn;
   }
+  y[i] = out;
+  if (push.x) push_prolong(push, i, out);
 }
 
 // Windowed counterpart of k_power_step_h; partials stored at the row block's slot, so the
@@ -2131,7 +2177,7 @@ k_prolong_h(int n, const int* __restrict__ agg, const float* __restrict__ sd_f,
 __global__ void __launch_bounds__(kBlock)
 k_dense_matvec_scaled(int n, const double* __restrict__ D, const float* __restrict__ sd,
                       const float* __restrict__ b, float* __restrict__ x,
-                      const int* __restrict__ done) {
+                      const int* __restrict__ done, Push push) {
   if (done && *done) return;
   const int i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -2139,8 +2185,16 @@ k_dense_matvec_scaled(int n, const double* __restrict__ D, const float* __restri
   const double* col = D + (size_t)i * n;
   double acc = 0.0;
   for (int j = lane; j < n; j += 64) acc += col[j] * ((double)sd[j] * (double)b[j]);
-  acc = wave_sum(acc);
-  if (lane == 0) x[i] = (float)(acc * (double)sd[i]);
+  acc = wave_sum(acc);   // the xor butterfly leaves the same total in every lane
+  const float xi = (float)(acc * (double)sd[i]);
+  if (lane == 0) x[i] = xi;
+  if (push.x) {   // the row's members, one lane each (push_prolong's arithmetic)
+    const int m1 = push.mptr[i + 1];
+    for (int m = push.mptr[i] + lane; m < m1; m += 64) {
+      const int j = push.mem[m];
+      push.x[j] = fmaf((push.sd_f[j] * push.isd_c[i]) * push.g, xi, push.x[j]);
+    }
+  }
 }
 
 // Flexible PCG (Polak-Ribiere): partial r.z and r.z_old
@@ -2853,6 +2907,10 @@ struct mgpbd_ctx {
   float omega = 2.0f / 3.0f;
   int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
   float overcorrect = 1.0f;   // scale of the prolongated coarse correction
+  // reduced-precision V-cycle: each coarse level's last kernel prolongs onto the finer level
+  // itself (push_prolong) instead of a k_prolong_h launch, and the PCG update writes level 0's
+  // scaled rhs and first sweep instead of k_half_first (same values bit for bit)
+  bool fused_launch = true;
   // spectral smoother: per-level weights omega_l = target / (safety lambda_l) in device memory,
   // lambda_l from a power iteration on D^-1 A warm-started across updates
   bool spectral = false;
@@ -3776,14 +3834,14 @@ static void with_rows(const mgpbd_ctx::Factored& F, Fn&& f) {
 template <int VT>
 static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                          const float* b, const int* done, const float* wp, const float* xp,
-                         const float* cc) {
+                         const float* cc, Push push) {
   const int g = nblocks(64LL * L.n_slices);
   hipStream_t s = ctx->stream;
   const float w = ctx->omega;
 #define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
-                                                         L.hpk, L.hval, x, y, b, w, wp, L.isd, done, xp, cc)
+                                                         L.hpk, L.hval, x, y, b, w, wp, L.isd, done, xp, cc, push)
 #define SELLW(M) k_sellw<VT, M><<<g, kBlock, lds, s>>>(L.n, L.n_slices, L.slice_ptr, L.hpk, L.seg_ptr, \
-                                                     L.segs, x, y, b, w, wp, L.isd, done, xp, cc)
+                                                     L.segs, x, y, b, w, wp, L.isd, done, xp, cc, push)
   const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
   if (L.hpk && L.segs) {
     switch (mode) {
@@ -3856,23 +3914,27 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
 }
 }  // extern "C++"
 
+// With push.x set (a coarse level's last sweep of a V-cycle) the launch also prolongs its
+// output onto the next finer level's iterate (push_prolong); level 0 never pushes.
 static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                   const float* b, const int* done, const float* wp, const float* xp = nullptr,
-                  const float* cc = nullptr) {
+                  const float* cc = nullptr, Push push = Push{}) {
   if (&L == &ctx->L[0] && ctx->fac.on) {
     if (ctx->precision == MGPBD_PREC_FP16) launch_fac<1>(ctx, mode, x, y, b, done, wp, xp, cc);
     else launch_fac<2>(ctx, mode, x, y, b, done, wp, xp, cc);
     return;
   }
-  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp, xp, cc);
-  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp, xp, cc);
+  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp, xp, cc, push);
+  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp, xp, cc, push);
 }
 
 // `count` scaled sweeps from cur; with z_out the last one writes isd * x into z_out instead.
 // Scaled-level counterpart of `sweeps` (same step numbering); the last step of a phase that
-// ends the V-cycle on level 0 writes the unscaled result into z_out.
+// ends the V-cycle on level 0 writes the unscaled result into z_out. With push.x set and
+// count >= 1 the last sweep also prolongs its result onto the finer level (*pushed = true).
 static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count,
-                       const int* done, float* z_out, int first_step = 0, bool from_zero = false) {
+                       const int* done, float* z_out, int first_step = 0, bool from_zero = false,
+                       Push push = Push{}, bool* pushed = nullptr) {
   MgLevel& L = ctx->L[l];
   float* xa = L.wb(1);
   float* xb = L.wb(2);
@@ -3883,32 +3945,39 @@ static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int co
     float* nxt = last_out ? z_out : other;
     const int step = first_step + sw;
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
+    const Push ps = sw == count - 1 ? push : Push{};
     if (cheb && step >= 2) {
       const float* xp = (step == 2 && from_zero) ? nullptr : other;
-      sellh(ctx, L, last_out ? 5 : 4, cur, nxt, b, done, nullptr, xp, cl(ctx, l, step));
+      sellh(ctx, L, last_out ? 5 : 4, cur, nxt, b, done, nullptr, xp, cl(ctx, l, step), ps);
     } else {
-      sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l));
+      sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l), nullptr, nullptr, ps);
     }
     prof_end(ctx, pe);
     cur = nxt;
   }
+  if (pushed) *pushed = push.x && count > 0;
   return cur;
 }
 
 // Scaled V-cycle on level l for rhs b^ with the first sweep (x^ = omega b^) in x_first.
-// On level 0 the result is returned unscaled in z_out.
+// On level 0 the result is returned unscaled in z_out. With push.x set (V-cycles, l >= 1) the
+// kernel that writes level l's final values also prolongs them onto level l - 1's iterate and
+// *pushed reports it; otherwise the caller launches k_prolong_h.
 static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, const int* done,
-                       float* z_out) {
+                       float* z_out, Push push = Push{}, bool* pushed = nullptr) {
   MgLevel& L = ctx->L[l];
   hipStream_t s = ctx->stream;
   const bool coarsest = (l == ctx->n_levels - 1);
+  if (pushed) *pushed = false;
   if (coarsest) {
     if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY) {
-      k_dense_matvec_scaled<<<(L.n + 3) / 4, kBlock, 0, s>>>(L.n, ctx->dense, L.sd, b, x_first, done);
+      k_dense_matvec_scaled<<<(L.n + 3) / 4, kBlock, 0, s>>>(L.n, ctx->dense, L.sd, b, x_first,
+                                                             done, push);
+      if (pushed) *pushed = push.x != nullptr;
       return x_first;
     }
     const int total = std::min(ctx->coarse_sweeps, L.n);
-    return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr);
+    return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr, 0, false, push, pushed);
   }
   float* x = sweeps_h(ctx, l, b, x_first, ns(ctx, l) - 1, done, nullptr, 2, true);
   float* r = L.wb(3);
@@ -3922,8 +3991,14 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
   k_restrict_h<<<nblocks(C.n), kBlock, 0, s>>>(C.n, L.mptr, L.mem, L.sd, r, C.isd, bc,
                                                c_dense ? nullptr : xc, ctx->omega, wl(ctx, l + 1),
                                                done);
-  float* xcr = vcycle_h(ctx, l + 1, bc, xc, done, nullptr);
-  if (ctx->cycle == 2 && l + 1 < ctx->n_levels - 1) {
+  // V-cycle: the coarse level's last kernel prolongs onto x itself; a W-cycle combines two
+  // coarse results first, so it prolongs in a launch of its own
+  const bool w2 = ctx->cycle == 2 && l + 1 < ctx->n_levels - 1;
+  const Push pc = (w2 || !ctx->fused_launch)
+                      ? Push{} : Push{L.mptr, L.mem, L.sd, C.isd, x, ctx->overcorrect};
+  bool c_pushed = false;
+  float* xcr = vcycle_h(ctx, l + 1, bc, xc, done, nullptr, pc, &c_pushed);
+  if (w2) {
     float* save = C.wb(5);
     float* rc = C.wb(4);
     k_copy_d<<<nblocks(C.n), kBlock, 0, s>>>(C.n, xcr, save, done);
@@ -3933,9 +4008,10 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
     xcr = vcycle_h(ctx, l + 1, rc, x2, done, nullptr);
     k_add_d<<<nblocks(C.n), kBlock, 0, s>>>(C.n, save, xcr, done);
   }
-  k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x, ctx->overcorrect,
-                                              done);
-  return sweeps_h(ctx, l, b, x, ns(ctx, l), done, z_out, 1, false);
+  if (!c_pushed)
+    k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x,
+                                                ctx->overcorrect, done);
+  return sweeps_h(ctx, l, b, x, ns(ctx, l), done, z_out, 1, false, push, pushed);
 }
 
 int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps) {
@@ -3958,6 +4034,13 @@ int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect) {
   ctx->cycle = cycle;
   ctx->overcorrect = overcorrect;
   drop_graphs(ctx);
+  return 0;
+}
+
+int mgpbd_mg_set_launch_fusion(mgpbd_ctx* ctx, int enable) {
+  CTX_CHECK("mgpbd_mg_set_launch_fusion");
+  ctx->fused_launch = enable != 0;
+  drop_graphs(ctx);   // captured replays hold the launches of the previous choice
   return 0;
 }
 
@@ -4542,13 +4625,16 @@ static void fac_outer(mgpbd_ctx* ctx, int mode, const float* xf, const double* x
 }
 
 // One PCG iteration (cons/mgpcg_gpu.py:289-327) as a launch sequence on ctx->stream.
-// z = M^-1 r for the PCG (first sweep included), FP32 or scaled reduced-precision V-cycle.
-static float* pcg_precondition(mgpbd_ctx* ctx, float* r, float* xv, bool first) {
+// z = M^-1 r for the PCG (first sweep included), FP32 or scaled reduced-precision V-cycle;
+// scaled_ready: the PCG update already wrote the scaled level 0's b^ and first sweep.
+static float* pcg_precondition(mgpbd_ctx* ctx, float* r, float* xv, bool first,
+                               bool scaled_ready = false) {
   MgLevel& L = ctx->L[0];
   const int* done = &ctx->st->done;
   if (half_mode(ctx)) {
-    k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, r, L.wb(0), L.wb(1),
-                                                           ctx->omega, wl(ctx, 0), done);
+    if (!scaled_ready)
+      k_half_first<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, L.isd, r, L.wb(0), L.wb(1),
+                                                             ctx->omega, wl(ctx, 0), done);
     return vcycle_h(ctx, 0, L.wb(0), L.wb(1), done, L.wb(3));
   }
   if (!first) k_blas1<<<nblocks(L.n), kBlock, 0, ctx->stream>>>(3, L.n, 0.f, nullptr, xv);
@@ -4579,11 +4665,16 @@ static void pcg_iteration(mgpbd_ctx* ctx, float* x, double* x64, float* r, float
                                                 ctx->ws, ctx->st);
   prof_end(ctx, pe);
   k_pcg_fin<FIN_ALPHA><<<1, kBlock, 0, s>>>(ctx->ws, gs, ctx->st);
+  // the update also writes the V-cycle's first sweep: from 1/diag on an FP32 level 0's first
+  // iteration, from isd (with b^ = isd r, k_half_first's values) on a scaled level 0
   const bool fuse_first = first && !half_mode(ctx);
-  k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, L.dinv, fuse_first ? xv : nullptr,
-                                     ctx->omega, wl(ctx, 0), ctx->ws, ctx->st, x64);
+  const bool fuse_half = half_mode(ctx) && ctx->fused_launch;
+  k_pcg_update<<<gn, kBlock, 0, s>>>(n, x, r, p, Ap, fuse_half ? L.isd : L.dinv,
+                                     fuse_half ? L.wb(1) : (fuse_first ? xv : nullptr),
+                                     ctx->omega, wl(ctx, 0), ctx->ws, ctx->st, x64,
+                                     fuse_half ? L.wb(0) : nullptr);
   k_pcg_fin<FIN_RR><<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
-  float* zz = pcg_precondition(ctx, r, xv, first);
+  float* zz = pcg_precondition(ctx, r, xv, first, fuse_half);
   if (ctx->flexible) {
     k_pcg_rz2<<<gn, kBlock, 0, s>>>(n, r, zz, zold, ctx->ws, ctx->st);
     k_pcg_fin_flex<<<1, kBlock, 0, s>>>(ctx->ws, gn, ctx->st);
