@@ -170,6 +170,7 @@ k_gemm64_dual(int n, const uint16_t* __restrict__ A, const uint16_t* __restrict_
 constexpr int kLdsRow = H + 4;
 constexpr int kTileWaves = kBlock / 64;
 constexpr int kLayerBlocks = 768;   // 3 blocks (52 KiB of LDS each) per CU on 256 CUs
+constexpr int kEncoderBlocks = 2048; // encoder grid: 16 rows per lane group at 1M rows
 
 typedef unsigned short v8u16 __attribute__((ext_vector_type(8)));
 
@@ -184,27 +185,52 @@ __device__ __forceinline__ v8u16 lds_load8(const uint16_t* src) {
                (unsigned short)hi.x, (unsigned short)hi.y, (unsigned short)hi.z, (unsigned short)hi.w};
 }
 
+// Decoder fused into the last inference layer: with dlam set, the layer's output rows (BF16, in
+// LDS) go straight through k_decoder_predict's arithmetic instead of to memory.
+struct FusedDecoder {
+  const float* P;     // parameters (w_out, b_out)
+  const float* lp;    // previous lambda (nullable)
+  const float* st;    // feature statistics (st[0]: the prediction scale)
+  float* y;           // decoder output (nullable)
+  float* dlam;        // prediction; nullptr: no fused decoder
+};
+
+// Blocks go to the 8 XCDs round-robin (block b on XCD b % 8); the bijection below numbers
+// them so that each XCD's blocks are consecutive, and each block takes a contiguous run of
+// tile groups: an XCD then works on one contiguous range of rows, whose face neighbours
+// (a few hundred to ~20k rows away in the block meshes' ordering) its own L2 mostly holds.
+__device__ __forceinline__ int xcd_consecutive(int b, int G) {
+  const int x = b & 7, j = b >> 3, q = G >> 3, rem = G & 7;
+  return x * q + min(x, rem) + j;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ Hin,
              const uint16_t* __restrict__ Wst, const uint16_t* __restrict__ Wnt,
              const float* __restrict__ bias, uint16_t* __restrict__ Hout,
-             uint16_t* __restrict__ Mout) {
+             uint16_t* __restrict__ Mout, FusedDecoder dec) {
   __shared__ __align__(16) uint16_t wl[2][H * kLdsRow];
   __shared__ __align__(16) uint16_t tl[kTileWaves][2][32 * kLdsRow];
+  __shared__ float wo[H];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int c = threadIdx.x; c < 2 * H * 8; c += kBlock) {   // 16-byte chunks of both matrices
     const int m = c / (H * 8), r = (c / 8) % H, ch = c % 8;
     const uint16_t* src = (m ? Wnt : Wst) + r * H + 8 * ch;
     lds_store8(wl[m] + r * kLdsRow + 8 * ch, *reinterpret_cast<const v8u16*>(src));
   }
+  if (dec.dlam && threadIdx.x < H) wo[threadIdx.x] = dec.P[off_wout() + threadIdx.x];
   const int n_tiles = (n + 31) / 32;
   const int kq = 8 * (lane >> 5), rl = lane & 31;
   uint16_t* ts = tl[w][0];
   uint16_t* tm = tl[w][1];
   const float bias0 = bias[rl], bias1 = bias[32 + rl];
+  const int n_groups = (n_tiles + kTileWaves - 1) / kTileWaves;
+  const int vb = xcd_consecutive(blockIdx.x, gridDim.x);
+  const int g0 = (int)((int64_t)vb * n_groups / gridDim.x);
+  const int g1 = (int)((int64_t)(vb + 1) * n_groups / gridDim.x);
   // the loop bound is the block's, so every wavefront meets every barrier
-  for (int t0 = blockIdx.x * kTileWaves; t0 < n_tiles; t0 += gridDim.x * kTileWaves) {
-    const int base = (t0 + w) * 32;
+  for (int g = g0; g < g1; ++g) {
+    const int base = (g * kTileWaves + w) * 32;
     // all loads of the tile are issued before any is used: rows past n read row n - 1 and
     // absent neighbours re-read the row itself, and selects drop them (a dropped neighbour
     // leaves the sum untouched, as skipping it does)
@@ -280,11 +306,29 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
       ts[rr * kLdsRow + 32 + rl] = f2bf(fmaxf(acc1[r] + bias1, 0.0f));
     }
     __syncthreads();
+    if (dec.dlam) {
+      // one lane per row: k_decoder_predict's products and pairwise sums over the row's
+      // BF16 values (c + 32, then c + 16, ...)
+      const int row = base + rl;
+      if (lane < 32 && row < n) {
+        float p[H];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = lane + 64 * q, r = c >> 3, ch = c & 7, row = base + r;
-      if (row < n)
-        *reinterpret_cast<v8u16*>(Hout + (size_t)row * H + 8 * ch) = lds_load8(ts + r * kLdsRow + 8 * ch);
+        for (int c = 0; c < H; ++c) p[c] = bf2f(ts[rl * kLdsRow + c]) * wo[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+          for (int c = 0; c < off; ++c) p[c] = p[c] + p[c + off];
+        const float yi = p[0] + dec.P[off_bout()];
+        if (dec.y) dec.y[row] = yi;
+        dec.dlam[row] = (dec.lp ? dec.lp[row] : 0.0f) + dec.st[0] * yi;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int c = lane + 64 * q, r = c >> 3, ch = c & 7, row = base + r;
+        if (row < n)
+          *reinterpret_cast<v8u16*>(Hout + (size_t)row * H + 8 * ch) = lds_load8(ts + r * kLdsRow + 8 * ch);
+      }
     }
     __syncthreads();   // the tile's LDS rows are free for the next tile
   }
@@ -470,32 +514,38 @@ k_mean_nbr_relu_bwd(int n, const int32_t* __restrict__ nbr, const uint16_t* __re
 __global__ void __launch_bounds__(kBlock)
 k_encoder8(int n, const float* __restrict__ x, const float* __restrict__ P,
            uint16_t* __restrict__ h0) {
-  // the encoder's weights and bias (F x H + H floats) staged in LDS once per block: the eight
-  // lanes of a row read eight distinct columns, conflict-free
-  __shared__ float wsh[F * H + H];
-  for (int k = threadIdx.x; k < F * H + H; k += kBlock)
-    wsh[k] = k < F * H ? P[off_win() + k] : P[off_bin() + (k - F * H)];
-  __syncthreads();
+  // eight lanes per row, lane q owning output columns 8q .. 8q + 7: its 8 x 8 weights and 8
+  // biases stay in registers for all of its rows (grid-stride over rows), so a row costs two
+  // 16-byte feature loads, 64 FMAs and one 16-byte store per lane
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  const int i = t >> 3, c0 = 8 * (t & 7);
-  if (i >= n) return;
-  const float4 xa = reinterpret_cast<const float4*>(x)[2 * (size_t)i];
-  const float4 xb = reinterpret_cast<const float4*>(x)[2 * (size_t)i + 1];
-  const float xi[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
-  uint16_t out[8];
+  const int c0 = 8 * (t & 7);
+  float w[F][8], bias[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    float v = wsh[F * H + c0 + e];
+    bias[e] = P[off_bin() + c0 + e];
 #pragma unroll
-    for (int k = 0; k < F; ++k) v = fmaf(xi[k], wsh[k * H + c0 + e], v);
-    out[e] = f2bf(fmaxf(v, 0.0f));
+    for (int k = 0; k < F; ++k) w[k][e] = P[off_win() + k * H + c0 + e];
   }
-  uint4 pk;
-  pk.x = (uint32_t)out[0] | ((uint32_t)out[1] << 16);
-  pk.y = (uint32_t)out[2] | ((uint32_t)out[3] << 16);
-  pk.z = (uint32_t)out[4] | ((uint32_t)out[5] << 16);
-  pk.w = (uint32_t)out[6] | ((uint32_t)out[7] << 16);
-  reinterpret_cast<uint4*>(h0 + (size_t)i * H)[t & 7] = pk;
+  const int stride = (gridDim.x * kBlock) >> 3;
+  for (int i = t >> 3; i < n; i += stride) {
+    const float4 xa = reinterpret_cast<const float4*>(x)[2 * (size_t)i];
+    const float4 xb = reinterpret_cast<const float4*>(x)[2 * (size_t)i + 1];
+    const float xi[8] = {xa.x, xa.y, xa.z, xa.w, xb.x, xb.y, xb.z, xb.w};
+    uint16_t out[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float v = bias[e];
+#pragma unroll
+      for (int k = 0; k < F; ++k) v = fmaf(xi[k], w[k][e], v);
+      out[e] = f2bf(fmaxf(v, 0.0f));
+    }
+    uint4 pk;
+    pk.x = (uint32_t)out[0] | ((uint32_t)out[1] << 16);
+    pk.y = (uint32_t)out[2] | ((uint32_t)out[3] << 16);
+    pk.z = (uint32_t)out[4] | ((uint32_t)out[5] << 16);
+    pk.w = (uint32_t)out[6] | ((uint32_t)out[7] << 16);
+    reinterpret_cast<uint4*>(h0 + (size_t)i * H)[t & 7] = pk;
+  }
 }
 
 // dlambda = lam_prev + s (h3 w_out + b_out) (and y = h3 w_out + b_out), one row per thread; the
@@ -932,13 +982,14 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
   carve(ws, n, W);
   // the inference kernels, keeping every layer's input and neighbour mean for the backward
   // pass
-  k_encoder8<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, x, params, W.h[0]);
+  k_encoder8<<<std::min(nb((int64_t)n * 8), kEncoderBlocks), kBlock, 0, s>>>(n, x, params,
+                                                                             W.h[0]);
   const int gg = std::min((n + 32 * kTileWaves - 1) / (32 * kTileWaves), kLayerBlocks);
   for (int l = 0; l < NL; ++l) {
     const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
     k_layer_mean<<<gg, kBlock, 0, s>>>(n, nbr, W.h[l], w_self, w_nbr, params + off_bl(l),
-                                       W.h[l + 1], W.m[l + 1]);
+                                       W.h[l + 1], W.m[l + 1], FusedDecoder{});
   }
   k_decoder_predict<<<nb(n), kBlock, 0, s>>>(n, W.h[NL], params, lam_prev, stats, W.y,
                                              dlam_out);
@@ -954,15 +1005,18 @@ int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr, const float* x,
   hipStream_t s = (hipStream_t)mgpbd_internal_stream(ctx);
   GnnWorkspace W;
   carve_infer(ws, n, W);   // layer l reads h[l & 1] and writes h[(l + 1) & 1]
-  k_encoder8<<<nb((int64_t)n * 8), kBlock, 0, s>>>(n, x, params, W.h[0]);
+  k_encoder8<<<std::min(nb((int64_t)n * 8), kEncoderBlocks), kBlock, 0, s>>>(n, x, params,
+                                                                             W.h[0]);
   const int gg = std::min((n + 32 * kTileWaves - 1) / (32 * kTileWaves), kLayerBlocks);
   for (int l = 0; l < NL; ++l) {
     const uint16_t* w_self = wt + (size_t)(2 * l) * H * H;
     const uint16_t* w_nbr = wt + (size_t)(2 * l + 1) * H * H;
+    // the last layer feeds the decoder from LDS (its rows never reach memory)
+    const FusedDecoder dec = l == NL - 1 ? FusedDecoder{params, lam_prev, stats, W.y, dlam_out}
+                                         : FusedDecoder{};
     k_layer_mean<<<gg, kBlock, 0, s>>>(n, nbr, W.h[l], w_self, w_nbr, params + off_bl(l),
-                                       W.h[l + 1], nullptr);
+                                       W.h[l + 1], nullptr, dec);
   }
-  k_decoder_predict<<<nb(n), kBlock, 0, s>>>(n, W.h[NL], params, lam_prev, stats, W.y, dlam_out);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_infer");
   return 0;
 }
