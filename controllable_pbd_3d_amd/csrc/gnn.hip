@@ -195,15 +195,6 @@ struct FusedDecoder {
   float* dlam;        // prediction; nullptr: no fused decoder
 };
 
-// Blocks go to the 8 XCDs round-robin (block b on XCD b % 8); the bijection below numbers
-// them so that each XCD's blocks are consecutive, and each block takes a contiguous run of
-// tile groups: an XCD then works on one contiguous range of rows, whose face neighbours
-// (a few hundred to ~20k rows away in the block meshes' ordering) its own L2 mostly holds.
-__device__ __forceinline__ int xcd_consecutive(int b, int G) {
-  const int x = b & 7, j = b >> 3, q = G >> 3, rem = G & 7;
-  return x * q + min(x, rem) + j;
-}
-
 __global__ void __launch_bounds__(kBlock)
 k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict__ Hin,
              const uint16_t* __restrict__ Wst, const uint16_t* __restrict__ Wnt,
@@ -224,13 +215,12 @@ k_layer_mean(int n, const int32_t* __restrict__ nbr, const uint16_t* __restrict_
   uint16_t* ts = tl[w][0];
   uint16_t* tm = tl[w][1];
   const float bias0 = bias[rl], bias1 = bias[32 + rl];
-  const int n_groups = (n_tiles + kTileWaves - 1) / kTileWaves;
-  const int vb = xcd_consecutive(blockIdx.x, gridDim.x);
-  const int g0 = (int)((int64_t)vb * n_groups / gridDim.x);
-  const int g1 = (int)((int64_t)(vb + 1) * n_groups / gridDim.x);
-  // the loop bound is the block's, so every wavefront meets every barrier
-  for (int g = g0; g < g1; ++g) {
-    const int base = (g * kTileWaves + w) * 32;
+  // the blocks sweep the tiles as one front (tile groups dealt round-robin), so the face
+  // neighbours of the rows in flight (a few to a few hundred rows away, ~20k for the third
+  // axis of the block meshes) were mostly just read by another block; the loop bound is the
+  // block's, so every wavefront meets every barrier
+  for (int t0 = blockIdx.x * kTileWaves; t0 < n_tiles; t0 += gridDim.x * kTileWaves) {
+    const int base = (t0 + w) * 32;
     // all loads of the tile are issued before any is used: rows past n read row n - 1 and
     // absent neighbours re-read the row itself, and selects drop them (a dropped neighbour
     // leaves the sum untouched, as skipping it does)

@@ -877,6 +877,10 @@ k_power_init(int n, float* __restrict__ v) {
 //             omega_l = 1 / theta, step k >= 2 uses c1 = rho_k rho_{k-1}, c2 = 2 rho_k / delta with
 //             rho_1 = 1 / sigma, rho_k = 1 / (2 sigma - rho_{k-1}).
 // Not usable lambda: the fixed weight (and Chebyshev steps that reduce to it, c1 = 0).
+// Chebyshev coefficient pairs kept per level: steps 2 .. kChebSteps + 1 of a smoothing phase
+// (a phase of at most kChebSteps + 1 steps; the per-level step count is capped at kChebSteps)
+constexpr int kChebSteps = 16;
+
 __global__ void k_power_omega(const double* __restrict__ pw, float target, float safety,
                               float fallback, int cheb, float ratio, float* __restrict__ omega_l,
                               float* __restrict__ cheb_l) {
@@ -890,14 +894,14 @@ __global__ void k_power_omega(const double* __restrict__ pw, float target, float
   }
   if (!ok) {
     omega_l[0] = fallback;
-    for (int k = 0; k < 8; ++k) { cheb_l[2 * k] = 0.0f; cheb_l[2 * k + 1] = fallback; }
+    for (int k = 0; k < kChebSteps; ++k) { cheb_l[2 * k] = 0.0f; cheb_l[2 * k + 1] = fallback; }
     return;
   }
   const double b = safety * lam, a = b / ratio;
   const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
   omega_l[0] = (float)(1.0 / theta);
   double rho = 1.0 / sigma;
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < kChebSteps; ++k) {
     const double rn = 1.0 / (2.0 * sigma - rho);
     cheb_l[2 * k] = (float)(rn * rho);
     cheb_l[2 * k + 1] = (float)(2.0 * rn / delta);
@@ -1217,13 +1221,38 @@ struct Push {
   float g;
 };
 
-__device__ __forceinline__ void push_prolong(const Push& p, int I, float xc) {
-  const int m0 = p.mptr[I], m1 = p.mptr[I + 1];
-  const float sc = p.isd_c[I];
-  for (int mb = m0; mb < m1; mb += kAggBatch) {
+// The push's loads (member ids, their sd and current x, and isd_I) do not depend on the coarse
+// value, so the sweep kernels issue them before their own row (push_prefetch) and store after
+// it (push_finish); members past the first batch of kAggBatch are loaded and stored there.
+struct PushPre {
+  int m0, m1;
+  float sc;
+  int id[kAggBatch];
+  float sv[kAggBatch], xv[kAggBatch];
+};
+
+__device__ __forceinline__ void push_prefetch(const Push& p, int I, PushPre& q) {
+  q.m0 = p.mptr[I];
+  q.m1 = p.mptr[I + 1];
+  q.sc = p.isd_c[I];
+  const int last = max(q.m1 - 1, q.m0);   // aggregates are never empty; guard anyway
+#pragma unroll
+  for (int j = 0; j < kAggBatch; ++j) q.id[j] = q.m1 > q.m0 ? p.mem[min(q.m0 + j, last)] : 0;
+#pragma unroll
+  for (int j = 0; j < kAggBatch; ++j) {
+    q.sv[j] = q.m1 > q.m0 ? p.sd_f[q.id[j]] : 0.0f;
+    q.xv[j] = q.m1 > q.m0 ? p.x[q.id[j]] : 0.0f;
+  }
+}
+
+__device__ __forceinline__ void push_finish(const Push& p, const PushPre& q, float xc) {
+#pragma unroll
+  for (int j = 0; j < kAggBatch; ++j)
+    if (q.m0 + j < q.m1) p.x[q.id[j]] = fmaf((q.sv[j] * q.sc) * p.g, xc, q.xv[j]);
+  for (int mb = q.m0 + kAggBatch; mb < q.m1; mb += kAggBatch) {   // members past the first batch
     int id[kAggBatch];
 #pragma unroll
-    for (int j = 0; j < kAggBatch; ++j) id[j] = p.mem[min(mb + j, m1 - 1)];
+    for (int j = 0; j < kAggBatch; ++j) id[j] = p.mem[min(mb + j, q.m1 - 1)];
     float sv[kAggBatch], xv[kAggBatch];
 #pragma unroll
     for (int j = 0; j < kAggBatch; ++j) {
@@ -1232,7 +1261,7 @@ __device__ __forceinline__ void push_prolong(const Push& p, int I, float xc) {
     }
 #pragma unroll
     for (int j = 0; j < kAggBatch; ++j)
-      if (mb + j < m1) p.x[id[j]] = fmaf((sv[j] * sc) * p.g, xc, xv[j]);
+      if (mb + j < q.m1) p.x[id[j]] = fmaf((sv[j] * q.sc) * p.g, xc, xv[j]);
   }
 }
 
@@ -1686,6 +1715,9 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
+  const bool pushing = push.x && i < n;
+  PushPre pre;
+  if (pushing) push_prefetch(push, i, pre);
   const float ax = sellh_row<VT, PACKED>(i, slice_ptr, idx, hpk, hval, x);
   if (i >= n) return;
   float out;
@@ -1701,7 +1733,7 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
     out = MODE == 5 ? isd[i] * xn : xn;
   }
   y[i] = out;
-  if (push.x) push_prolong(push, i, out);
+  if (pushing) push_finish(push, pre, out);
 }
 
 // Power iteration on the scaled reduced-precision level (unit diagonal, so D = I in these
@@ -1809,8 +1841,11 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
   if (wp) omega = *wp;
   if (done && *done) return;
   const int blk = xcd_row_block(blockIdx.x, gridDim.x);
-  stage_window(xs, seg_ptr, segs, blk, x, n);
   const int i = blk * kBlock + threadIdx.x;
+  const bool pushing = push.x && i < n;
+  PushPre pre;
+  if (pushing) push_prefetch(push, i, pre);   // in flight while the window is staged
+  stage_window(xs, seg_ptr, segs, blk, x, n);
   if ((i >> 6) >= n_slices) return;
   const float ax = sellw_row<VT>(i, slice_ptr, hpk, xs);
   if (i >= n) return;
@@ -1827,7 +1862,7 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
     out = MODE == 5 ? isd[i] * xn : xn;
   }
   y[i] = out;
-  if (push.x) push_prolong(push, i, out);
+  if (pushing) push_finish(push, pre, out);
 }
 
 // Windowed counterpart of k_power_step_h; partials stored at the row block's slot, so the
@@ -2188,7 +2223,7 @@ k_dense_matvec_scaled(int n, const double* __restrict__ D, const float* __restri
   acc = wave_sum(acc);   // the xor butterfly leaves the same total in every lane
   const float xi = (float)(acc * (double)sd[i]);
   if (lane == 0) x[i] = xi;
-  if (push.x) {   // the row's members, one lane each (push_prolong's arithmetic)
+  if (push.x) {   // the row's members, one lane each (push_finish's arithmetic)
     const int m1 = push.mptr[i + 1];
     for (int m = push.mptr[i] + lane; m < m1; m += 64) {
       const int j = push.mem[m];
@@ -2873,7 +2908,6 @@ struct MgLevel {
   float* wb(int k) const { return work + (size_t)k * 64 * n_slices; }
 };
 
-constexpr int kChebSteps = 8;
 
 struct mgpbd_ctx {
   int device = 0;
@@ -2908,8 +2942,9 @@ struct mgpbd_ctx {
   int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
   float overcorrect = 1.0f;   // scale of the prolongated coarse correction
   // reduced-precision V-cycle: each coarse level's last kernel prolongs onto the finer level
-  // itself (push_prolong) instead of a k_prolong_h launch, and the PCG update writes level 0's
-  // scaled rhs and first sweep instead of k_half_first (same values bit for bit)
+  // itself (push_prefetch, push_finish) instead of a k_prolong_h launch, and the PCG update
+  // writes level 0's scaled rhs and first sweep instead of k_half_first (same values bit for
+  // bit)
   bool fused_launch = true;
   // spectral smoother: per-level weights omega_l = target / (safety lambda_l) in device memory,
   // lambda_l from a power iteration on D^-1 A warm-started across updates
@@ -3915,7 +3950,7 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
 }  // extern "C++"
 
 // With push.x set (a coarse level's last sweep of a V-cycle) the launch also prolongs its
-// output onto the next finer level's iterate (push_prolong); level 0 never pushes.
+// output onto the next finer level's iterate (push_finish); level 0 never pushes.
 static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                   const float* b, const int* done, const float* wp, const float* xp = nullptr,
                   const float* cc = nullptr, Push push = Push{}) {
@@ -4018,10 +4053,10 @@ int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps) {
   CTX_CHECK("mgpbd_mg_set_level_smoothing");
   // steps 0 restores the configured n_smooth; per-level counts need n_smooth >= 1 (the
   // first step of every phase structure is laid out for a smoothed cycle)
-  if (level < 0 || level >= ctx->n_levels || steps < 0 || steps > 8 ||
+  if (level < 0 || level >= ctx->n_levels || steps < 0 || steps > kChebSteps ||
       (steps > 0 && ctx->n_smooth < 1))
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level_smoothing: level in range, steps in "
-                                    "[0, 8], n_smooth >= 1");
+                                    "[0, 16], n_smooth >= 1");
   ctx->level_smooth[level] = steps;
   drop_graphs(ctx);
   return 0;

@@ -652,9 +652,9 @@ class GPUMultigridSolver:
 
     def smoother_weights(self, chebyshev: bool = False):
         """Jacobi weight of each level as the V-cycle uses it now (synchronises); with
-        ``chebyshev`` also the (c1, c2) of Chebyshev steps 2..9, shape (n_levels, 8, 2)."""
+        ``chebyshev`` also the (c1, c2) of Chebyshev steps 2..17, shape (n_levels, 16, 2)."""
         w = np.zeros(self.n_levels, np.float32)
-        c = np.zeros((self.n_levels, 8, 2), np.float32)
+        c = np.zeros((self.n_levels, 16, 2), np.float32)
         self.ctx("mgpbd_mg_get_smoother", w.ctypes.data_as(_lib.F32P),
                  c.ctypes.data_as(_lib.F32P) if chebyshev else None)
         return (w, c) if chebyshev else w

@@ -375,7 +375,7 @@ int mgpbd_mg_set_coarse_update(mgpbd_ctx* ctx, int newton_schulz);
 /* Counts since the context was created: Newton-Schulz refresh steps kept, full inversions. */
 int mgpbd_mg_coarse_stats(mgpbd_ctx* ctx, int64_t* newton_schulz_steps, int64_t* refactors);
 /* Smoothing steps per phase on one level (extension; the reference uses n_smooth on every
- * level, cons/mgpcg_gpu.py:232-246): steps in [1, 8], 0 restores n_smooth. Coarse levels are
+ * level, cons/mgpcg_gpu.py:232-246): steps in [1, 16], 0 restores n_smooth. Coarse levels are
  * cheap in bytes but each step is a launch, so fewer steps there can pay. */
 int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps);
 /* Cycle shape (extension; no reference counterpart: cons/mgpcg_gpu.py:214-259 is a V-cycle
@@ -417,7 +417,7 @@ int mgpbd_mg_update_smoother(mgpbd_ctx* ctx, double* lambda_out);
  * mgpbd_mg_set_smoother. */
 int mgpbd_mg_set_level_weights(mgpbd_ctx* ctx, int n, const float* omega);
 /* The weight each level relaxes with (host, n_levels floats) and, when cheb_out is given,
- * the Chebyshev (c1, c2) of steps 2..9 per level (n_levels x 16 floats, zero unless mode 2);
+ * the Chebyshev (c1, c2) of steps 2..17 per level (n_levels x 32 floats, zero unless mode 2);
  * synchronises. */
 int mgpbd_mg_get_smoother(mgpbd_ctx* ctx, float* omega_out, float* cheb_out);
 /* Reduced-precision V-cycle (extension; no reference counterpart). Every level is used in its

@@ -486,7 +486,7 @@ typedef struct {
   real *b, *x, *xn, *r, *tmp;
   const double* dense;      /* coarsest only: dense column-major inverse, or NULL (sweeps) */
   real omega;               /* this level's Jacobi weight; <= 0: the cycle's common weight */
-  const real* cheb;         /* Chebyshev (c1, c2) of smoothing steps 2..9, or NULL (Jacobi) */
+  const real* cheb;         /* Chebyshev (c1, c2) of smoothing steps 2..17, or NULL (Jacobi) */
 } OrcLevel;
 
 typedef struct {
@@ -516,7 +516,7 @@ static void smooth_phase(const OrcLevel* A, int count, real w) {
   for (int s = 0; s < count; ++s) {
     const int step = 1 + s;
     if (A->cheb && step >= 2) {
-      const int k = step - 2 < 7 ? step - 2 : 7;
+      const int k = step - 2 < 15 ? step - 2 : 15;
       orc_cheb_step(n, A->ip, A->ix, A->a, A->dinv, A->b, A->x, prev, A->xn, A->cheb[2 * k],
                     A->cheb[2 * k + 1]);
     } else {

@@ -212,8 +212,8 @@ class OracleMG:
 
     def set_level_omegas(self, omegas, cheb=None):
         """Per-level Jacobi weights (None: the common ``omega`` on every level) and, optionally,
-        per-level Chebyshev (c1, c2) of smoothing steps 2..9 (arrays of shape (8, 2), or None for
-        a Jacobi level)."""
+        per-level Chebyshev (c1, c2) of smoothing steps 2..17 (arrays of shape (16, 2), or None
+        for a Jacobi level)."""
         self.level_omegas = None if omegas is None else [float(w) for w in omegas]
         self.level_cheb = None if cheb is None else [
             None if c is None else np.ascontiguousarray(np.asarray(c).reshape(-1), self.dt)

@@ -431,7 +431,7 @@ def test_fixed_jacobi_vcycle_is_indefinite_and_spectral_weights_fix_it():
 
 
 def chebyshev_coefficients(lam, safety=1.1, ratio=12.0):
-    """(1/theta, [(c1, c2) of steps 2..9]) of the Chebyshev smoother on
+    """(1/theta, [(c1, c2) of steps 2..17]) of the Chebyshev smoother on
     [safety lam / ratio, safety lam] (Saad, Iterative Methods, Alg. 12.1 in three-term form)."""
     b = safety * lam
     a = b / ratio
@@ -439,7 +439,7 @@ def chebyshev_coefficients(lam, safety=1.1, ratio=12.0):
     sigma = theta / delta
     rho = 1.0 / sigma
     c = []
-    for _ in range(8):
+    for _ in range(16):
         rn = 1.0 / (2.0 * sigma - rho)
         c.append((rn * rho, 2.0 * rn / delta))
         rho = rn
