@@ -1206,12 +1206,14 @@ __device__ __forceinline__ float agg_sum(int I, const int* __restrict__ mptr,
   return s;
 }
 
-// Prolongation in push form, fused into the kernel that writes a coarse level's final values:
-// coarse row I adds its correction to the members of its aggregate on the next finer level,
-// x_j = fmaf((sd_j isd_I) g, xc_I, x_j) for j in mem[mptr[I] .. mptr[I + 1]) -- the arithmetic of
-// k_prolong_h on the same values, so the result is bit-identical to a separate prolongation
-// launch. Every fine row belongs to exactly one aggregate, so no two threads write one x_j.
-// x == nullptr: nothing to push.
+// Prolongation in push form, fused into the dense coarsest product: coarse row I adds its
+// correction to the members of its aggregate on the next finer level, x_j = fmaf((sd_j isd_I)
+// g, xc_I, x_j) for j in mem[mptr[I] .. mptr[I + 1]) -- the arithmetic of k_prolong_h on the
+// same values, so the result is bit-identical to a separate prolongation launch. Every fine row
+// belongs to exactly one aggregate, so no two lanes write one x_j. (Pushing from the sweep
+// kernels of the other coarse levels was measured slower than their row-parallel k_prolong_h:
+// each sweep thread then waits on scattered read-modify-writes, 3.5-6 us per launch against
+// 3.3-5 us for the separate pass.) x == nullptr: nothing to push.
 struct Push {
   const int* mptr;
   const int* mem;
@@ -1220,50 +1222,6 @@ struct Push {
   float* x;
   float g;
 };
-
-// The push's loads (member ids, their sd and current x, and isd_I) do not depend on the coarse
-// value, so the sweep kernels issue them before their own row (push_prefetch) and store after
-// it (push_finish); members past the first batch of kAggBatch are loaded and stored there.
-struct PushPre {
-  int m0, m1;
-  float sc;
-  int id[kAggBatch];
-  float sv[kAggBatch], xv[kAggBatch];
-};
-
-__device__ __forceinline__ void push_prefetch(const Push& p, int I, PushPre& q) {
-  q.m0 = p.mptr[I];
-  q.m1 = p.mptr[I + 1];
-  q.sc = p.isd_c[I];
-  const int last = max(q.m1 - 1, q.m0);   // aggregates are never empty; guard anyway
-#pragma unroll
-  for (int j = 0; j < kAggBatch; ++j) q.id[j] = q.m1 > q.m0 ? p.mem[min(q.m0 + j, last)] : 0;
-#pragma unroll
-  for (int j = 0; j < kAggBatch; ++j) {
-    q.sv[j] = q.m1 > q.m0 ? p.sd_f[q.id[j]] : 0.0f;
-    q.xv[j] = q.m1 > q.m0 ? p.x[q.id[j]] : 0.0f;
-  }
-}
-
-__device__ __forceinline__ void push_finish(const Push& p, const PushPre& q, float xc) {
-#pragma unroll
-  for (int j = 0; j < kAggBatch; ++j)
-    if (q.m0 + j < q.m1) p.x[q.id[j]] = fmaf((q.sv[j] * q.sc) * p.g, xc, q.xv[j]);
-  for (int mb = q.m0 + kAggBatch; mb < q.m1; mb += kAggBatch) {   // members past the first batch
-    int id[kAggBatch];
-#pragma unroll
-    for (int j = 0; j < kAggBatch; ++j) id[j] = p.mem[min(mb + j, q.m1 - 1)];
-    float sv[kAggBatch], xv[kAggBatch];
-#pragma unroll
-    for (int j = 0; j < kAggBatch; ++j) {
-      sv[j] = p.sd_f[id[j]];
-      xv[j] = p.x[id[j]];
-    }
-#pragma unroll
-    for (int j = 0; j < kAggBatch; ++j)
-      if (mb + j < q.m1) p.x[id[j]] = fmaf((sv[j] * q.sc) * p.g, xc, xv[j]);
-  }
-}
 
 // b_c[I] = sum of r over the members of aggregate I (ascending); with dinv_c given, also the
 // coarse level's first Jacobi sweep from zero, x_c = (omega dinv_c) b_c.
@@ -1710,14 +1668,11 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
         const uint32_t* __restrict__ hpk, const uint16_t* __restrict__ hval,
         const float* __restrict__ x, float* y, const float* __restrict__ b,
         float omega, const float* __restrict__ wp, const float* __restrict__ isd, const int* __restrict__ done,
-        const float* xp, const float* __restrict__ cc, Push push) {
+        const float* xp, const float* __restrict__ cc) {
   if (wp) omega = *wp;   // per-level weight held in device memory
   if (done && *done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
-  const bool pushing = push.x && i < n;
-  PushPre pre;
-  if (pushing) push_prefetch(push, i, pre);
   const float ax = sellh_row<VT, PACKED>(i, slice_ptr, idx, hpk, hval, x);
   if (i >= n) return;
   float out;
@@ -1733,7 +1688,6 @@ k_sellh(int n, int n_slices, const int* __restrict__ slice_ptr, const int* __res
     out = MODE == 5 ? isd[i] * xn : xn;
   }
   y[i] = out;
-  if (pushing) push_finish(push, pre, out);
 }
 
 // Power iteration on the scaled reduced-precision level (unit diagonal, so D = I in these
@@ -1836,16 +1790,13 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
         const int* __restrict__ seg_ptr, const int* __restrict__ segs,
         const float* __restrict__ x, float* y, const float* __restrict__ b,
         float omega, const float* __restrict__ wp, const float* __restrict__ isd,
-        const int* __restrict__ done, const float* xp, const float* __restrict__ cc, Push push) {
+        const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
   extern __shared__ float xs[];
   if (wp) omega = *wp;
   if (done && *done) return;
   const int blk = xcd_row_block(blockIdx.x, gridDim.x);
-  const int i = blk * kBlock + threadIdx.x;
-  const bool pushing = push.x && i < n;
-  PushPre pre;
-  if (pushing) push_prefetch(push, i, pre);   // in flight while the window is staged
   stage_window(xs, seg_ptr, segs, blk, x, n);
+  const int i = blk * kBlock + threadIdx.x;
   if ((i >> 6) >= n_slices) return;
   const float ax = sellw_row<VT>(i, slice_ptr, hpk, xs);
   if (i >= n) return;
@@ -1862,7 +1813,6 @@ k_sellw(int n, int n_slices, const int* __restrict__ slice_ptr, const uint32_t* 
     out = MODE == 5 ? isd[i] * xn : xn;
   }
   y[i] = out;
-  if (pushing) push_finish(push, pre, out);
 }
 
 // Windowed counterpart of k_power_step_h; partials stored at the row block's slot, so the
@@ -2223,7 +2173,7 @@ k_dense_matvec_scaled(int n, const double* __restrict__ D, const float* __restri
   acc = wave_sum(acc);   // the xor butterfly leaves the same total in every lane
   const float xi = (float)(acc * (double)sd[i]);
   if (lane == 0) x[i] = xi;
-  if (push.x) {   // the row's members, one lane each (push_finish's arithmetic)
+  if (push.x) {   // the row's members, one lane each (k_prolong_h's arithmetic)
     const int m1 = push.mptr[i + 1];
     for (int m = push.mptr[i] + lane; m < m1; m += 64) {
       const int j = push.mem[m];
@@ -2941,10 +2891,9 @@ struct mgpbd_ctx {
   float omega = 2.0f / 3.0f;
   int cycle = 1;              // 1: V-cycle, 2: W-cycle (two coarse cycles per level)
   float overcorrect = 1.0f;   // scale of the prolongated coarse correction
-  // reduced-precision V-cycle: each coarse level's last kernel prolongs onto the finer level
-  // itself (push_prefetch, push_finish) instead of a k_prolong_h launch, and the PCG update
-  // writes level 0's scaled rhs and first sweep instead of k_half_first (same values bit for
-  // bit)
+  // reduced-precision V-cycle: the dense coarsest product prolongs onto the level above it
+  // itself instead of a k_prolong_h launch, and the PCG update writes level 0's scaled rhs
+  // and first sweep instead of k_half_first (same values bit for bit)
   bool fused_launch = true;
   // spectral smoother: per-level weights omega_l = target / (safety lambda_l) in device memory,
   // lambda_l from a power iteration on D^-1 A warm-started across updates
@@ -3869,14 +3818,14 @@ static void with_rows(const mgpbd_ctx::Factored& F, Fn&& f) {
 template <int VT>
 static void launch_sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                          const float* b, const int* done, const float* wp, const float* xp,
-                         const float* cc, Push push) {
+                         const float* cc) {
   const int g = nblocks(64LL * L.n_slices);
   hipStream_t s = ctx->stream;
   const float w = ctx->omega;
 #define SELLH(D, M) k_sellh<VT, D, M><<<g, kBlock, 0, s>>>(L.n, L.n_slices, L.slice_ptr, L.idx, \
-                                                         L.hpk, L.hval, x, y, b, w, wp, L.isd, done, xp, cc, push)
+                                                         L.hpk, L.hval, x, y, b, w, wp, L.isd, done, xp, cc)
 #define SELLW(M) k_sellw<VT, M><<<g, kBlock, lds, s>>>(L.n, L.n_slices, L.slice_ptr, L.hpk, L.seg_ptr, \
-                                                     L.segs, x, y, b, w, wp, L.isd, done, xp, cc, push)
+                                                     L.segs, x, y, b, w, wp, L.isd, done, xp, cc)
   const size_t lds = sizeof(float) * kWinSeg * L.win_segs;
   if (L.hpk && L.segs) {
     switch (mode) {
@@ -3949,27 +3898,23 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
 }
 }  // extern "C++"
 
-// With push.x set (a coarse level's last sweep of a V-cycle) the launch also prolongs its
-// output onto the next finer level's iterate (push_finish); level 0 never pushes.
 static void sellh(mgpbd_ctx* ctx, const MgLevel& L, int mode, const float* x, float* y,
                   const float* b, const int* done, const float* wp, const float* xp = nullptr,
-                  const float* cc = nullptr, Push push = Push{}) {
+                  const float* cc = nullptr) {
   if (&L == &ctx->L[0] && ctx->fac.on) {
     if (ctx->precision == MGPBD_PREC_FP16) launch_fac<1>(ctx, mode, x, y, b, done, wp, xp, cc);
     else launch_fac<2>(ctx, mode, x, y, b, done, wp, xp, cc);
     return;
   }
-  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp, xp, cc, push);
-  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp, xp, cc, push);
+  if (ctx->precision == MGPBD_PREC_FP16) launch_sellh<1>(ctx, L, mode, x, y, b, done, wp, xp, cc);
+  else launch_sellh<2>(ctx, L, mode, x, y, b, done, wp, xp, cc);
 }
 
 // `count` scaled sweeps from cur; with z_out the last one writes isd * x into z_out instead.
 // Scaled-level counterpart of `sweeps` (same step numbering); the last step of a phase that
-// ends the V-cycle on level 0 writes the unscaled result into z_out. With push.x set and
-// count >= 1 the last sweep also prolongs its result onto the finer level (*pushed = true).
+// ends the V-cycle on level 0 writes the unscaled result into z_out.
 static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int count,
-                       const int* done, float* z_out, int first_step = 0, bool from_zero = false,
-                       Push push = Push{}, bool* pushed = nullptr) {
+                       const int* done, float* z_out, int first_step = 0, bool from_zero = false) {
   MgLevel& L = ctx->L[l];
   float* xa = L.wb(1);
   float* xb = L.wb(2);
@@ -3980,39 +3925,34 @@ static float* sweeps_h(mgpbd_ctx* ctx, int l, const float* b, float* cur, int co
     float* nxt = last_out ? z_out : other;
     const int step = first_step + sw;
     int pe = prof_begin(ctx, l == 0 ? MGPBD_PROF_JACOBI_L0 : MGPBD_PROF_COARSE);
-    const Push ps = sw == count - 1 ? push : Push{};
     if (cheb && step >= 2) {
       const float* xp = (step == 2 && from_zero) ? nullptr : other;
-      sellh(ctx, L, last_out ? 5 : 4, cur, nxt, b, done, nullptr, xp, cl(ctx, l, step), ps);
+      sellh(ctx, L, last_out ? 5 : 4, cur, nxt, b, done, nullptr, xp, cl(ctx, l, step));
     } else {
-      sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l), nullptr, nullptr, ps);
+      sellh(ctx, L, last_out ? 3 : 1, cur, nxt, b, done, wl(ctx, l));
     }
     prof_end(ctx, pe);
     cur = nxt;
   }
-  if (pushed) *pushed = push.x && count > 0;
   return cur;
 }
 
 // Scaled V-cycle on level l for rhs b^ with the first sweep (x^ = omega b^) in x_first.
-// On level 0 the result is returned unscaled in z_out. With push.x set (V-cycles, l >= 1) the
-// kernel that writes level l's final values also prolongs them onto level l - 1's iterate and
-// *pushed reports it; otherwise the caller launches k_prolong_h.
+// On level 0 the result is returned unscaled in z_out. On the dense coarsest level, push.x set
+// makes the product also prolong onto the level above (the caller then skips k_prolong_h).
 static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, const int* done,
-                       float* z_out, Push push = Push{}, bool* pushed = nullptr) {
+                       float* z_out, Push push = Push{}) {
   MgLevel& L = ctx->L[l];
   hipStream_t s = ctx->stream;
   const bool coarsest = (l == ctx->n_levels - 1);
-  if (pushed) *pushed = false;
   if (coarsest) {
     if (ctx->coarse_mode == MGPBD_COARSE_CHOLESKY) {
       k_dense_matvec_scaled<<<(L.n + 3) / 4, kBlock, 0, s>>>(L.n, ctx->dense, L.sd, b, x_first,
                                                              done, push);
-      if (pushed) *pushed = push.x != nullptr;
       return x_first;
     }
     const int total = std::min(ctx->coarse_sweeps, L.n);
-    return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr, 0, false, push, pushed);
+    return sweeps_h(ctx, l, b, x_first, total - 1, done, nullptr);
   }
   float* x = sweeps_h(ctx, l, b, x_first, ns(ctx, l) - 1, done, nullptr, 2, true);
   float* r = L.wb(3);
@@ -4026,13 +3966,11 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
   k_restrict_h<<<nblocks(C.n), kBlock, 0, s>>>(C.n, L.mptr, L.mem, L.sd, r, C.isd, bc,
                                                c_dense ? nullptr : xc, ctx->omega, wl(ctx, l + 1),
                                                done);
-  // V-cycle: the coarse level's last kernel prolongs onto x itself; a W-cycle combines two
-  // coarse results first, so it prolongs in a launch of its own
+  // the dense coarsest product prolongs onto x itself (no W-cycle doubling happens there)
   const bool w2 = ctx->cycle == 2 && l + 1 < ctx->n_levels - 1;
-  const Push pc = (w2 || !ctx->fused_launch)
-                      ? Push{} : Push{L.mptr, L.mem, L.sd, C.isd, x, ctx->overcorrect};
-  bool c_pushed = false;
-  float* xcr = vcycle_h(ctx, l + 1, bc, xc, done, nullptr, pc, &c_pushed);
+  const bool pushed = c_dense && ctx->fused_launch;
+  const Push pc = pushed ? Push{L.mptr, L.mem, L.sd, C.isd, x, ctx->overcorrect} : Push{};
+  float* xcr = vcycle_h(ctx, l + 1, bc, xc, done, nullptr, pc);
   if (w2) {
     float* save = C.wb(5);
     float* rc = C.wb(4);
@@ -4043,10 +3981,10 @@ static float* vcycle_h(mgpbd_ctx* ctx, int l, const float* b, float* x_first, co
     xcr = vcycle_h(ctx, l + 1, rc, x2, done, nullptr);
     k_add_d<<<nblocks(C.n), kBlock, 0, s>>>(C.n, save, xcr, done);
   }
-  if (!c_pushed)
+  if (!pushed)
     k_prolong_h<<<nblocks(L.n), kBlock, 0, s>>>(L.n, L.agg, L.sd, C.isd, xcr, x,
                                                 ctx->overcorrect, done);
-  return sweeps_h(ctx, l, b, x, ns(ctx, l), done, z_out, 1, false, push, pushed);
+  return sweeps_h(ctx, l, b, x, ns(ctx, l), done, z_out, 1, false);
 }
 
 int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps) {

@@ -34,7 +34,7 @@ DRAG_K, DRAG_BOX = 1e3, 0.5
 CONFIGS = {
     "c0": dict(kind="box", n=8, h=0.05, E=None, max_levels=3, label="box8_3072tets"),
     "c1": dict(kind="box", n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets",
-               smoothing=((8, 1), 40.0)),
+               smoothing=((13, 1), 40.0)),
     "c2": dict(kind="box", n=56, h=0.02, E=(1e4, 1e8), max_levels=5,
                label="box56_1053696tets_E1e4-1e8", smoothing=((8, 1), 40.0)),
     "c3": dict(kind="box", n=32, h=0.03, E=(1e4, 1e6), max_levels=4, drag=True,

@@ -385,12 +385,10 @@ int mgpbd_mg_set_level_smoothing(mgpbd_ctx* ctx, int level, int steps);
 int mgpbd_mg_set_cycle(mgpbd_ctx* ctx, int cycle, float overcorrect);
 /* Launch fusion of the reduced-precision V-cycle (extension; no reference counterpart: the
  * reference prolongs in kernels of their own, cons/mgpcg_gpu.py:251-253). enable 1 (the
- * default): the kernel that writes a coarse level's final values (its last sweep, or the dense
- * coarsest product) also adds each coarse row's correction to the members of its aggregate on
- * the finer level (one launch fewer per level and V-cycle; W-cycles still prolong the doubled
- * levels separately), and the PCG's update writes the scaled level 0's rhs isd r and first
- * sweep (one launch fewer per iteration); 0: separate launches. Both give the same iterates
- * bit for bit. */
+ * default): the dense coarsest product (coarse_mode Cholesky) also adds each coarse row's
+ * correction to the members of its aggregate on the level above, and the PCG's update writes
+ * the scaled level 0's rhs isd r and first sweep (one launch fewer per V-cycle and per
+ * iteration); 0: separate launches. Both give the same iterates bit for bit. */
 int mgpbd_mg_set_launch_fusion(mgpbd_ctx* ctx, int enable);
 /* Smoother (extension). mode 0 = the reference's fixed weighted Jacobi (the omega of
  * mgpbd_mg_configure on every level, cons/mgpcg_gpu.py:232-246). mode 1 = spectral Jacobi:

@@ -1,6 +1,7 @@
-"""Launch fusion of the reduced-precision V-cycle (mgpbd_mg_set_launch_fusion): prolongation in
-the coarse levels' last kernels, against the separate prolongation launch of the reference's
-V-cycle (cons/mgpcg_gpu.py:251-253), and level 0's scaled rhs and first sweep in the PCG update.
+"""Launch fusion of the reduced-precision V-cycle (mgpbd_mg_set_launch_fusion): prolongation from
+the dense coarsest level inside its product, against the separate prolongation launch of the
+reference's V-cycle (cons/mgpcg_gpu.py:251-253), and level 0's scaled rhs and first sweep in the
+PCG update.
 Two solvers built alike, one with the fusion switched off, stepped side by side: positions,
 lambda and the per-iteration records must agree bit for bit after every substep."""
 
@@ -18,13 +19,14 @@ EXT = dict(hierarchy="extended", galerkin="structural", max_coarse=16, relative_
 
 CASES = {
     # the benchmarked configuration: factored level 0, Chebyshev, dense coarsest (the dense
-    # product pushes onto the level above it, every other coarse level's last sweep onto its own)
+    # product pushes onto the level above it)
     "fp16_factored_chebyshev_dense": dict(EXT, precision="fp16", level0="factored",
                                           coarse_solver="cholesky", smoother="chebyshev"),
-    # assembled level 0, spectral Jacobi, Jacobi sweeps on the coarsest level
+    # assembled level 0, spectral Jacobi, Jacobi sweeps on the coarsest level (only the PCG
+    # update's fusion applies)
     "bf16_assembled_spectral_jacobi_coarsest": dict(EXT, precision="bf16", smoother="spectral",
                                                     coarse_solver="jacobi", coarse_sweeps=20),
-    # W-cycle: only the coarsest level pushes, the doubled levels prolong separately
+    # W-cycle: the coarsest level pushes, the doubled levels prolong separately
     "fp16_wcycle": dict(EXT, precision="fp16", level0="factored", coarse_solver="cholesky",
                         smoother="spectral", cycle="W"),
 }
