@@ -13,9 +13,9 @@ Runs (each its own solver on the same mesh, same loading):
                   assembled CSR; the FP64 residual is measured, not enforced)
   fp32            Chebyshev smoothing on the device's spectral estimates, FP64 iterate, every
                   solve FP64-verified; FP32 assembled level 0 (the metric's denominator);
-                  spectral estimates, coarsest inverse and coarse levels 1.. rebuilt at each
-                  substep's first Newton iteration (level 0 at every one), as in every run
-                  below
+                  spectral estimates and coarsest inverse rebuilt at each substep's first
+                  Newton iteration, coarse levels 1.. then and after every solve of >= 8 PCG
+                  iterations (level 0 at every one), as in every run below
   fp16_assembled  the same with the stored FP16 level 0 (same operator as fp32: the precision's
                   own gain)
   fp16, bf16      FP16 / BF16 V-cycle on the factored level 0 J W J^T + alpha~
@@ -138,7 +138,7 @@ def kernel_stats(ctx, n, nnz, precision, chebyshev=False, spmv_per_nnz=8, factor
 
 CHEBYSHEV = dict(smoother="chebyshev", n_smooth=5, smooth_schedule=[5, 1], cheb_ratio=30.0,
                  smoother_safety=1.2, verify_residual=1, fp64_iterate=True,
-                 spectral_refresh="substep", coarse_refresh="substep")
+                 spectral_refresh="substep", coarse_refresh="adaptive")
 # Level-0 / coarse Chebyshev steps and interval ratio of the runs whose measured best differs
 # from the workload's (at 1M tets: 13 + 13 level-0 steps for FP32 and FP16 with the ARAP rows,
 # 8 + 8 for BF16, whose 8-bit mantissa caps what longer polynomials gain, and for the volume
@@ -166,6 +166,8 @@ def run_kwargs(name, cfg):
     if kw.get("smoother") == "chebyshev" and sm:
         (k0, k1), ratio = sm
         kw.update(n_smooth=max(k0, k1), smooth_schedule=[k0, k1], cheb_ratio=ratio)
+    if "coarse_refresh" in kw and cfg.get("coarse_refresh"):
+        kw["coarse_refresh"] = cfg["coarse_refresh"]
     return kw
 
 
@@ -290,6 +292,7 @@ def run_config(mesh, mu_t, lam_t, cfg, name, steps, warmup, profile_steps, model
            "fp64_rel_residual_median": float(np.median(rel)) if len(rel) else None}
     res["hbm_footprint_bytes"] = s.device_bytes()
     res["pcg_rescues"], res["pcg_restarts"] = s._gpu_solver.pcg_counters()
+    res["pcg_lag_rescues"] = s._gpu_solver.lag_rescues()
     ns, refac = s._gpu_solver.coarse_stats()
     res["coarsest_inverse"] = {"newton_schulz_refresh_steps": ns, "full_inversions": refac}
     if kw.get("warm_start"):

@@ -59,6 +59,8 @@ SIGNATURES = {
     "mgpbd_pcg_set_fp64_iterate": (c_int, [_V, c_int]),
     "mgpbd_pcg_set_rescue": (c_int, [_V, c_int, c_float, c_double]),
     "mgpbd_pcg_counters": (c_int, [_V, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
+    "mgpbd_pcg_lag_rescues": (c_int, [_V, ctypes.POINTER(c_int64)]),
+    "mgpbd_pcg_set_lag_rescue": (c_int, [_V, c_int]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),
@@ -148,6 +150,7 @@ SIGNATURES = {
     # native substep
     "mgpbd_mg_set_refill": (c_int, [_V, c_int, c_int, _V, _V, _V, _V, _V, c_int, _V, _V]),
     "mgpbd_mg_refresh": (c_int, [_V, c_int, c_int, c_int]),
+    "mgpbd_mg_set_coarse_lagged": (c_int, [_V, c_int]),
     "mgpbd_substep_struct_sizes": (c_int, [I32P]),
     "mgpbd_substep_create": (_V, [_V]),
     "mgpbd_substep_destroy": (None, [_V]),

@@ -2971,6 +2971,14 @@ struct mgpbd_ctx {
   float rescue_boost = 1.25f;
   double stall_ratio = 0.25;   // ||r||^2 ratio over 8 iterations counted as a stall
   int64_t n_rescues = 0, n_restarts = 0;
+  // lag rescue: levels 1.. may be older than level 0 (mgpbd_mg_refresh with coarse = 0, or
+  // mgpbd_mg_set_coarse_lagged); a solve on such a hierarchy that runs past
+  // max(8, 2 fresh_iters + 2) iterations rebuilds levels 1.. and resumes from its iterate
+  // (once per solve). fresh_iters: the last solve run entirely on a current hierarchy.
+  bool coarse_lagged = false;
+  int fresh_iters = 0;
+  int lag_limit = -1;   // -1: max(8, 2 fresh_iters + 2); 0: no lag rescue; k > 0: k iterations
+  int64_t n_lag_rescues = 0;
   // HIP graphs of k PCG iterations (+ the state copy), k = 1 .. kGraphMax, captured when a
   // replay of that length is first needed for the same buffers, replayed on `work` stream
   bool use_graph = true;
@@ -3586,18 +3594,10 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
   return 0;
 }
 
-int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from, int coarse) {
-  CTX_CHECK("mgpbd_mg_refresh");
-  if (int e = check_levels(ctx)) return e;
-  if (half_from < 0 || half_from > ctx->n_levels)
-    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: 0 <= half_from <= n_levels");
-  if (!coarse) {   // levels 1.. kept: only level 0's own reduced-precision data
-    if (half_from == 0)
-      if (int e = convert_half_levels(ctx, 0, 1)) return e;
-    if (refresh && ctx->spectral && !ctx->fixed_weights)
-      if (int e = mgpbd_mg_update_smoother(ctx, nullptr)) return e;
-    return 0;
-  }
+// Levels 1.. from level 0's current values through the registered refill maps (1/diag of
+// each), the coarsest inverse refreshed when `factor`, and the reduced-precision copies of
+// levels half_from.. converted.
+static int rebuild_coarse(mgpbd_ctx* ctx, bool factor, int half_from) {
   for (int l = 0; l + 1 < ctx->n_levels; ++l) {
     const mgpbd_ctx::Refill& R = ctx->refill[l];
     if (R.n < 0) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: refill maps of level " +
@@ -3610,11 +3610,35 @@ int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from, int coarse) {
     }
     if (int e = mgpbd_diag_inv(ctx, R.n_coarse, R.cdiag, R.coarse, R.cdinv)) return e;
   }
-  if (refresh && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels > 1)
+  if (factor && ctx->coarse_mode == MGPBD_COARSE_CHOLESKY && ctx->n_levels > 1)
     if (int e = mgpbd_mg_factor_coarsest(ctx)) return e;
   if (int e = convert_half_levels(ctx, half_from, ctx->n_levels)) return e;
+  ctx->coarse_lagged = false;
+  return 0;
+}
+
+int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from, int coarse) {
+  CTX_CHECK("mgpbd_mg_refresh");
+  if (int e = check_levels(ctx)) return e;
+  if (half_from < 0 || half_from > ctx->n_levels)
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_refresh: 0 <= half_from <= n_levels");
+  if (!coarse) {   // levels 1.. kept: only level 0's own reduced-precision data
+    if (half_from == 0)
+      if (int e = convert_half_levels(ctx, 0, 1)) return e;
+    ctx->coarse_lagged = ctx->n_levels > 1;
+    if (refresh && ctx->spectral && !ctx->fixed_weights)
+      if (int e = mgpbd_mg_update_smoother(ctx, nullptr)) return e;
+    return 0;
+  }
+  if (int e = rebuild_coarse(ctx, refresh != 0, half_from)) return e;
   if (refresh && ctx->spectral && !ctx->fixed_weights)
     if (int e = mgpbd_mg_update_smoother(ctx, nullptr)) return e;
+  return 0;
+}
+
+int mgpbd_mg_set_coarse_lagged(mgpbd_ctx* ctx, int lagged) {
+  CTX_CHECK("mgpbd_mg_set_coarse_lagged");
+  ctx->coarse_lagged = lagged != 0 && ctx->n_levels > 1;
   return 0;
 }
 
@@ -4744,7 +4768,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
 
   const bool fuse_first = first && !half_mode(ctx);
   int issued = 0, total_iters = 0, restarts = 0;
-  bool rescued = false, first_round = true;
+  bool rescued = false, lag_rescued = false, first_round = true;
+  const bool lagged_at_start = ctx->coarse_lagged;
+  const int lag_limit = ctx->lag_limit < 0 ? std::max(8, 2 * ctx->fresh_iters + 2) : ctx->lag_limit;
   bool from_guess = guess;
   const float* alt = x_alt;
   double first_info[4] = {0.0, 0.0, 0.0, 0.0};   // rr0, start, rr_guess, rr_alt of round 0
@@ -4756,11 +4782,12 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       // r = b - A x, q = b - A x_alt (q lives in Ap until the first iteration), then the
       // start with the smallest residual; a restart of the FP64 iterate takes r = b - A x64
       if (ctx->fac.on) {
-        if (x64 && (restarts > 0 || rescued)) fac_outer(ctx, 1, nullptr, x64, b, r, nullptr, gs);
+        if (x64 && (restarts > 0 || rescued || lag_rescued))
+          fac_outer(ctx, 1, nullptr, x64, b, r, nullptr, gs);
         else fac_outer(ctx, 1, x, nullptr, b, r, nullptr, gs);
         if (alt) fac_outer(ctx, 1, alt, nullptr, b, Ap, nullptr, gs);
       } else {
-        if (x64 && (restarts > 0 || rescued))
+        if (x64 && (restarts > 0 || rescued || lag_rescued))
           k_resid64<<<gs, kBlock, 0, s>>>(n, L.n_slices, L.slice_ptr, L.idx, L.val, x64, b, r);
         else
           sell_launch<2>(ctx, L, x, r, nullptr, b, 0.f, nullptr, nullptr);
@@ -4773,7 +4800,9 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
                                                  fuse_first ? xv : nullptr, ctx->omega,
                                                  wl(ctx, 0), ctx->st);
       if (x64)
-        k_x64_from_start<<<nblocks(n), kBlock, 0, s>>>(n, x, x64, restarts > 0 || rescued, ctx->st);
+        k_x64_from_start<<<nblocks(n), kBlock, 0, s>>>(n, x, x64,
+                                                       restarts > 0 || rescued || lag_rescued,
+                                                       ctx->st);
     } else {
       k_pcg_init<<<gn, kBlock, 0, s>>>(n, b, x, r, L.dinv, fuse_first ? xv : nullptr,
                                         ctx->omega, wl(ctx, 0), ctx->ws, x64);
@@ -4791,7 +4820,8 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
     double rr_poll[64];    // recurrence residual and iteration count at each host poll
     int it_poll[64];
     int n_poll = 0;
-    bool stalled = false;
+    bool stalled = false, lagging = false;
+    const bool lag_watch = ctx->coarse_lagged && !lag_rescued && lag_limit > 0;
     while (issued < maxiter) {
       const int want = std::min(round_issued == 0 ? first_len
                                 : round_issued < pred_end ? 1 : ctx->poll, maxiter - issued);
@@ -4814,6 +4844,10 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       if (ctx->st_host->done) break;
       if (!std::isfinite(ctx->st_host->bb) || !std::isfinite(ctx->st_host->rr)) {
         nonfinite = true;   // NaN never meets the stop tests: stop polling at once
+        break;
+      }
+      if (lag_watch && round_issued >= lag_limit) {
+        lagging = true;   // levels 1.. older than level 0 and the solve runs long
         break;
       }
       if (watch) {
@@ -4852,8 +4886,20 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
       first_info[2] = h.rr_guess;
       first_info[3] = h.rr_alt;
     }
-    total_iters += stalled ? round_issued : ctx->st_host->iters;
-    if (!stalled && restarts == 0 && !rescued) ctx->pred_iters = std::max(1, ctx->st_host->iters);
+    total_iters += (stalled || lagging) ? round_issued : ctx->st_host->iters;
+    if (!stalled && !lagging && restarts == 0 && !rescued && !lag_rescued) {
+      ctx->pred_iters = std::max(1, ctx->st_host->iters);
+      if (!lagged_at_start) ctx->fresh_iters = ctx->st_host->iters;
+    }
+    if (lagging && issued < maxiter) {
+      // the lagged hierarchy no longer fits level 0: rebuild levels 1.. and resume
+      lag_rescued = true;
+      ++ctx->n_lag_rescues;
+      if (int e = rebuild_coarse(ctx, true, 1)) return e;
+      from_guess = true;
+      alt = nullptr;
+      continue;
+    }
     if (stalled && issued < maxiter) {
       // the V-cycle no longer contracts: the spectral estimates lag the matrix
       rescued = true;
@@ -4925,8 +4971,21 @@ int mgpbd_pcg_set_rescue(mgpbd_ctx* ctx, int enable, float boost, double stall_r
 
 int mgpbd_pcg_counters(mgpbd_ctx* ctx, int64_t* rescues, int64_t* restarts) {
   CTX_CHECK("mgpbd_pcg_counters");
-  if (rescues) *rescues = ctx->n_rescues;
+  if (rescues) *rescues = ctx->n_rescues + ctx->n_lag_rescues;
   if (restarts) *restarts = ctx->n_restarts;
+  return 0;
+}
+
+int mgpbd_pcg_set_lag_rescue(mgpbd_ctx* ctx, int limit) {
+  CTX_CHECK("mgpbd_pcg_set_lag_rescue");
+  if (limit < -1) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_set_lag_rescue: limit >= -1");
+  ctx->lag_limit = limit;
+  return 0;
+}
+
+int mgpbd_pcg_lag_rescues(mgpbd_ctx* ctx, int64_t* lag_rescues) {
+  CTX_CHECK("mgpbd_pcg_lag_rescues");
+  if (lag_rescues) *lag_rescues = ctx->n_lag_rescues;
   return 0;
 }
 

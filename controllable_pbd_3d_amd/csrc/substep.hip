@@ -188,6 +188,7 @@ int mgpbd_substep(mgpbd_substep_state* st, const mgpbd_substep_desc* d, mgpbd_ne
   // solve (cons/mgpbd_gpu.py:397-446)
   SUB_TRY(mgpbd_fill(c, d->n_cons, 0.0f, d->lambda));
   double r0 = 0.0, r_next = -1.0;
+  int prev_cg = 0;   // PCG iterations of the previous Newton iteration's solve
   for (int ite = 0; ite < d->maxiter; ++ite) {
     double r;
     if (r_next < 0.0) {
@@ -213,7 +214,9 @@ int mgpbd_substep(mgpbd_substep_state* st, const mgpbd_substep_desc* d, mgpbd_ne
                                   d->fill_desc, d->t_c, d->v_invm, d->gradC, d->alpha,
                                   d->A_val, d->A_dinv));
     const int refresh = d->spectral_every_newton || ite == 0;
-    const int coarse = d->coarse_every_newton || ite == 0;
+    // coarse levels: every iteration, the first only, or (k >= 2) also after a long solve
+    const int coarse = d->coarse_every_newton == 1 || ite == 0 ||
+                       (d->coarse_every_newton >= 2 && prev_cg >= d->coarse_every_newton);
     bool have_guess = false;
     if (ite == 0 && d->warm_start == 2 && d->overlap_inference && d->factored) {
       // the network needs only the constraint pass and level 0's 1/diag: it runs on the
@@ -254,6 +257,7 @@ int mgpbd_substep(mgpbd_substep_state* st, const mgpbd_substep_desc* d, mgpbd_ne
       rec.info[7] = info[3];
     }
     rec.cg_iters = it;
+    prev_cg = it;
 
     if (d->factored)
       SUB_TRY(mgpbd_mg_factored_scatter(c, d->dlam, d->lambda, d->dpos));

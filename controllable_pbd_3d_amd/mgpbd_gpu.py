@@ -40,10 +40,14 @@ spectrum, and the dense coarsest solve (coarse_solver='cholesky') re-inverts its
 every Newton iteration's refill, or only at a substep's first Newton iteration (the later
 iterations of a substep change the matrices little; the kept inverse is SPD, and the verified
 PCG and the stall rescue cover a lagging estimate).
-``coarse_refresh`` ('newton' | 'substep'): the coarse levels 1.. are rebuilt from level 0 after
-every Newton iteration's refill (the reference), or only at a substep's first Newton iteration
-(extension: later iterations refresh level 0 alone; the V-cycle stays an SPD preconditioner of
-level 0's current matrix, and the PCG still solves that matrix to its tolerance).
+``coarse_refresh`` ('newton' | 'substep' | 'adaptive'): the coarse levels 1.. are rebuilt from
+level 0 after every Newton iteration's refill (the reference), or only at a substep's first
+Newton iteration (extension: later iterations refresh level 0 alone; the V-cycle stays an SPD
+preconditioner of level 0's current matrix, and the PCG still solves that matrix to its
+tolerance), or ('adaptive') at the first Newton iteration and after every solve that took at
+least ``coarse_refresh_iters`` (default 8) PCG iterations: where the matrices move much within a
+substep (stiffness contrasts of 1e4, the drag of an inclusion) a lagging hierarchy multiplies the
+iteration counts, where they do not the later refreshes are skipped.
 ``overlap_inference`` (default True; learned start with the factored level 0): the network runs
 on a second stream while the coarse levels refill (bit-identical to running it after them).
 ``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is
@@ -111,15 +115,17 @@ class MGPBDSolverGPU:
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
                  spectral_refresh: str = "newton", coarse_refresh: str = "newton",
-                 overlap_inference: bool = True,
+                 coarse_refresh_iters: int = 8, overlap_inference: bool = True,
                  volume_lambda=None, omega_levels=None, native_substep: bool = False,
                  device=None, verbose: bool = True) -> None:
         if aggregation not in ("tet", "pyamg"):
             raise ValueError("aggregation must be 'tet' or 'pyamg'")
         if spectral_refresh not in ("newton", "substep"):
             raise ValueError("spectral_refresh must be 'newton' or 'substep'")
-        if coarse_refresh not in ("newton", "substep"):
-            raise ValueError("coarse_refresh must be 'newton' or 'substep'")
+        if coarse_refresh not in ("newton", "substep", "adaptive"):
+            raise ValueError("coarse_refresh must be 'newton', 'substep' or 'adaptive'")
+        if coarse_refresh == "adaptive" and int(coarse_refresh_iters) < 2:
+            raise ValueError("coarse_refresh_iters must be at least 2")
         if level0 not in ("assembled", "factored"):
             raise ValueError("level0 must be 'assembled' or 'factored'")
         if level0 == "factored" and (precision == "fp32" or galerkin != "structural"):
@@ -174,6 +180,7 @@ class MGPBDSolverGPU:
         self.level0 = level0
         self.spectral_refresh = spectral_refresh
         self.coarse_refresh = coarse_refresh
+        self.coarse_refresh_iters = int(coarse_refresh_iters)
         self.overlap_inference = bool(overlap_inference)
         self.native_substep = bool(native_substep)
         self._native = None        # (state handle, descriptor, stats buffer) of mgpbd_substep
@@ -383,7 +390,8 @@ class MGPBDSolverGPU:
         d.hold_pinned, d.maxiter, d.maxiter_cg = int(self.hold_pinned), self.maxiter, mg.maxiter_cg
         d.relative_tol, d.use_line_search = int(mg.relative), int(self.use_line_search)
         d.spectral_every_newton = int(self.spectral_refresh == "newton")
-        d.coarse_every_newton = int(self.coarse_refresh == "newton")
+        d.coarse_every_newton = {"newton": 1, "substep": 0}.get(self.coarse_refresh,
+                                                                self.coarse_refresh_iters)
         d.warm_start = {"none": 0, "previous": 1, "learned": 2}[self.warm_start]
         d.overlap_inference = int(self.overlap_inference)
         d.factored = int(mg.factored)
@@ -505,14 +513,15 @@ class MGPBDSolverGPU:
                  dptr(A.row_len), dptr(self._A_desc), dptr(self.t_c), dptr(self.v_invm),
                  dptr(self.gradC), dptr(self.alpha_tilde), dptr(A.data), dptr(A.diag_inv))
 
-    def _fill_A_and_update_gpu(self, newton_iter: int = 0):
+    def _fill_A_and_update_gpu(self, newton_iter: int = 0, prev_cg: int = 0):
         # the factored level 0 reads the gradients; its assembled values are only wanted for
         # recorded training pairs
         mg = self._gpu_solver
         if not mg.factored or self.record_samples is not None:
             self._fill_A()
         refresh = self.spectral_refresh == "newton" or newton_iter == 0
-        coarse = self.coarse_refresh == "newton" or newton_iter == 0
+        coarse = (self.coarse_refresh == "newton" or newton_iter == 0 or
+                  (self.coarse_refresh == "adaptive" and prev_cg >= self.coarse_refresh_iters))
         hook = None
         if (newton_iter == 0 and self.warm_start == "learned" and self.overlap_inference and
                 mg.factored):
@@ -602,6 +611,7 @@ class MGPBDSolverGPU:
         stats = []
         self.last_solve_info = []
         r_next = None   # residual of the positions the line search's full step left evaluated
+        cg_iters = 0    # the previous Newton iteration's PCG iterations
         for ite in range(self.maxiter):
             if r_next is None:
                 self._compute_C_and_gradC()
@@ -616,7 +626,7 @@ class MGPBDSolverGPU:
                 if self.benchmark is not None:
                     self.benchmark.record_iteration(ite, r)
                 break
-            self._fill_A_and_update_gpu(ite)
+            self._fill_A_and_update_gpu(ite, cg_iters)
             if ite == 0 and self.warm_start != "none":
                 cg_iters = self._solve_warm_started()
             else:

@@ -607,6 +607,7 @@ class GPUMultigridSolver:
         if not coarse:
             if getattr(self, "_half", None) and not early:
                 self.ctx("mgpbd_mg_convert_half_levels", 0, 1)
+            self.ctx("mgpbd_mg_set_coarse_lagged", 1)
             if self.smoother != "jacobi" and refresh_smoother:
                 self.ctx("mgpbd_mg_update_smoother", None)
             return
@@ -625,6 +626,7 @@ class GPUMultigridSolver:
                 self.ctx("mgpbd_mg_convert_half_levels", 1, self.n_levels)
             else:
                 self.ctx("mgpbd_mg_convert_half")
+        self.ctx("mgpbd_mg_set_coarse_lagged", 0)
         if self.smoother != "jacobi" and refresh_smoother:
             self.ctx("mgpbd_mg_update_smoother", None)
 
@@ -639,10 +641,17 @@ class GPUMultigridSolver:
                             getattr(self, "_win", None), self._fac, seen=seen) + ctx_bytes.value
 
     def pcg_counters(self):
-        """(stall rescues, verification restarts) over this solver's solves."""
+        """(rescues: stall and lagged-hierarchy, verification restarts) over this solver's
+        solves."""
         a, b = ctypes.c_int64(), ctypes.c_int64()
         self.ctx("mgpbd_pcg_counters", ctypes.byref(a), ctypes.byref(b))
         return int(a.value), int(b.value)
+
+    def lag_rescues(self) -> int:
+        """Solves that rebuilt a lagged hierarchy mid-solve (mgpbd_mg_set_coarse_lagged)."""
+        a = ctypes.c_int64()
+        self.ctx("mgpbd_pcg_lag_rescues", ctypes.byref(a))
+        return int(a.value)
 
     def coarse_stats(self):
         """(Newton-Schulz refresh steps kept, full inversions) of the coarsest inverse."""

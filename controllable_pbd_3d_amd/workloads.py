@@ -30,7 +30,9 @@ DRAG_K, DRAG_BOX = 1e3, 0.5
 # max_levels: levels of the extended hierarchy; maxiter_cg: PCG cap where some systems need
 # more than the reference's 100 iterations to reach 1e-6; smoothing: Chebyshev steps per
 # phase on level 0 and on the coarse levels and the interval ratio (measured per workload,
-# default 5 / 1 on [1.2 lambda / 30, 1.2 lambda]).
+# default 5 / 1 on [1.2 lambda / 30, 1.2 lambda]); coarse_refresh: when levels 1.. are rebuilt
+# (measured per workload: the inclusion under drag moves level 0 so far at every Newton
+# iteration that rebuilding each time beats the adaptive policy).
 CONFIGS = {
     "c0": dict(kind="box", n=8, h=0.05, E=None, max_levels=3, label="box8_3072tets"),
     "c1": dict(kind="box", n=56, h=0.02, E=(1e4, 1e6), max_levels=5, label="box56_1053696tets",
@@ -40,7 +42,8 @@ CONFIGS = {
     "c3": dict(kind="box", n=32, h=0.03, E=(1e4, 1e6), max_levels=4, drag=True,
                label="box32_196608tets_drag"),
     "c4": dict(kind="ellipsoid", h=0.01, E=(1e5, 1e8), max_levels=6, drag=True,
-               maxiter_cg=1000, label="ellipsoid_h0.01_inclusion_drag"),
+               maxiter_cg=1000, coarse_refresh="newton",
+               label="ellipsoid_h0.01_inclusion_drag"),
 }
 
 

@@ -363,6 +363,16 @@ int mgpbd_mg_set_refill(mgpbd_ctx* ctx, int level, int n_entries, const int32_t*
  * estimates are refreshed. Queues only (a Cholesky refactorization synchronises, as in
  * mgpbd_mg_factor_coarsest). */
 int mgpbd_mg_refresh(mgpbd_ctx* ctx, int refresh, int half_from, int coarse);
+/* Lagged hierarchy (extension; no reference counterpart: the reference rebuilds every level at
+ * every Newton iteration, cons/mgpcg_gpu.py:183-212). Records whether levels 1.. are older
+ * than level 0's current values; mgpbd_mg_refresh sets it itself (coarse = 0: lagged), a
+ * caller that refills the levels through the separate entry points says so here. While it is
+ * set, a PCG solve that is still running after max(8, 2 f + 2) iterations, f the iteration
+ * count of the last solve on a current hierarchy, rebuilds levels 1.. (refill maps, 1/diag,
+ * coarsest inverse, reduced-precision copies) and resumes from its iterate, once per solve:
+ * a hierarchy gone stale within a substep costs a bounded number of iterations instead of
+ * multiplying them. */
+int mgpbd_mg_set_coarse_lagged(mgpbd_ctx* ctx, int lagged);
 /* With newton_schulz != 0 (the default) a refresh of an already inverted coarsest level
  * takes two Newton-Schulz steps X <- X + X (I - A X) from the previous inverse and the device
  * keeps the result when ||I - A X||_F < 0.1 afterwards (then every eigenvalue of X A lies
@@ -585,8 +595,15 @@ int mgpbd_pcg_set_fp64_iterate(mgpbd_ctx* ctx, int enable);
  * eigenvalue; the polynomial smoother then amplifies the modes above its interval and the
  * V-cycle stops contracting. Default off. */
 int mgpbd_pcg_set_rescue(mgpbd_ctx* ctx, int enable, float boost, double stall_ratio);
-/* Cumulative counts over the context's solves: stall rescues and verification restarts. */
+/* Cumulative counts over the context's solves: rescues (stall and lagged-hierarchy) and
+ * verification restarts. */
 int mgpbd_pcg_counters(mgpbd_ctx* ctx, int64_t* rescues, int64_t* restarts);
+/* Iteration limit of the lagged-hierarchy rescue (extension; see mgpbd_mg_set_coarse_lagged):
+ * -1 (the default) max(8, 2 f + 2), 0 off, k > 0 after k iterations. */
+int mgpbd_pcg_set_lag_rescue(mgpbd_ctx* ctx, int limit);
+/* Cumulative count of the lagged-hierarchy rescues alone (extension; see
+ * mgpbd_mg_set_coarse_lagged). */
+int mgpbd_pcg_lag_rescues(mgpbd_ctx* ctx, int64_t* lag_rescues);
 
 /* ---- Warm start (extension; no reference counterpart: the reference zeroes lambda every
  * substep, cons/mgpbd_gpu.py:402, and starts every PCG from dlambda = 0,
@@ -741,7 +758,9 @@ typedef struct mgpbd_substep_desc {
   int32_t drag_vertex;                         /* -1: no spring drag */
   int32_t hold_pinned, maxiter, maxiter_cg, relative_tol, use_line_search;
   int32_t spectral_every_newton;               /* 0: spectral refresh at iteration 0 only */
-  int32_t coarse_every_newton;                 /* 0: levels 1.. rebuilt at iteration 0 only */
+  int32_t coarse_every_newton;                 /* levels 1.. rebuilt: 0 at iteration 0 only, 1 at
+                                                  every iteration, k >= 2 at iteration 0 and after
+                                                  a solve of >= k PCG iterations */
   int32_t warm_start;                          /* 0 none, 1 previous lambda, 2 learned */
   int32_t overlap_inference;                   /* learned + factored: network on a side stream */
   int32_t factored;                            /* level 0 factored (mgpbd_mg_set_level0_factored) */

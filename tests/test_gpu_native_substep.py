@@ -36,6 +36,10 @@ CASES = {
     "fp16_factored_coarse_lag_drag": (dict(MIXED, precision="fp16", level0="factored",
                                            coarse_refresh="substep"), True, True),
     "fp32_coarse_lag": (dict(EXT, smoother="spectral", coarse_refresh="substep"), True, False),
+    # levels 1.. rebuilt at the first Newton iteration and after solves of >= 3 PCG iterations
+    "fp16_factored_coarse_adaptive_drag": (dict(MIXED, precision="fp16", level0="factored",
+                                                coarse_refresh="adaptive",
+                                                coarse_refresh_iters=3), True, True),
 }
 
 
@@ -100,6 +104,41 @@ def test_native_substep_is_bit_identical(case):
         assert predicted == 6
         assert (nat._native[0] is not None) and ((py._side is None) ==
                                                  (not kw.get("overlap_inference", True)))
+
+
+def test_lag_rescue_is_bit_identical_and_verified():
+    """The lagged-hierarchy rescue (levels 1.. rebuilt only at a substep's first Newton
+    iteration; a lagged solve past 2 iterations rebuilds them and resumes) on a stiff-contrast
+    block under drag: it fires, the native and Python loops stay bit-identical, every solve is
+    verified at 1e-6."""
+    from controllable_pbd_3d_amd import MGPBDSolverGPU
+    mesh, mu_t = physical_block(6, seed=63, E_range=(1e4, 1e8))
+    args = solver_kwargs(mesh, mu_per_tet=mu_t, **dict(MIXED, precision="fp16",
+                                                       level0="factored",
+                                                       coarse_refresh="substep"))
+    args["v_p"] = deformed(mesh, seed=63, amp=0.02, twist=0.05)
+    py, nat = (MGPBDSolverGPU(**args, native_substep=native, verbose=False)
+               for native in (False, True))
+    for s in (py, nat):
+        s.init()
+        s._gpu_solver.ctx("mgpbd_pcg_set_lag_rescue", 2)
+    rng = np.random.default_rng(6)
+    free = np.nonzero(mesh.invm > 0)[0]
+    for k in range(4):
+        v = int(rng.choice(free))
+        target = mesh.verts[v] + rng.uniform(-0.05, 0.05, 3)
+        for s in (py, nat):
+            s.set_drag(v, target, k=1e3)
+        py.step(0)
+        nat.step(0)
+        for name in ("v_p", "v_v", "lambdaf"):
+            assert _same(getattr(py, name), getattr(nat, name)), (k, name)
+        assert py.last_stats == nat.last_stats, k
+        assert all(d["reason"] != 0 for d in nat.last_solve_info), k
+    fired = nat._gpu_solver.lag_rescues()
+    print(f"[lag rescue] 4 substeps bit-identical; lag rescues {fired} / "
+          f"{py._gpu_solver.lag_rescues()}; last {nat.last_stats}", flush=True)
+    assert fired > 0 and fired == py._gpu_solver.lag_rescues()
 
 
 def test_native_substep_falls_back_while_hooks_are_attached():

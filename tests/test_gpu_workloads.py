@@ -191,6 +191,42 @@ def test_lagged_coarse_levels_match_at_full_size(name, volume, kw_b):
     assert worst_h <= 1e-4
 
 
+def test_adaptive_coarse_refresh_keeps_the_iteration_counts_of_c2():
+    """C2 (E 1e4-1e8, every substep at the 20-Newton cap): a hierarchy rebuilt only at a
+    substep's first Newton iteration (coarse_refresh='substep') costs PCG iterations against
+    rebuilding it at every one, because level 0 moves far within the substep; the lag rescue
+    (a lagged solve past max(8, 2 f + 2) iterations rebuilds and resumes) keeps that within 2x
+    (3.1x and unverified solves without it); coarse_refresh='adaptive' (rebuilt after every
+    solve of >= 8 iterations, which here is all of them) keeps the every-iteration counts.
+    Every solve verified at 1e-6."""
+    from controllable_pbd_3d_amd.workloads import (CONFIGS, Loading, build_workload,
+                                                    solver_args)
+    cfg = CONFIGS["c2"]
+    mesh, mu_t, lam_t = build_workload(cfg, seed=0)
+    (k0, k1), ratio = cfg["smoothing"]
+    kw = solver_args(mesh, mu_t, lam_t, hierarchy="extended", galerkin="structural",
+                     max_levels=cfg["max_levels"], max_coarse=64, coarse_solver="cholesky",
+                     smoother="chebyshev", n_smooth=max(k0, k1), smooth_schedule=[k0, k1],
+                     cheb_ratio=ratio, smoother_safety=1.2, verify_residual=1,
+                     fp64_iterate=True, spectral_refresh="substep", precision="fp16",
+                     level0="factored")
+    its, unverified = {}, {}
+    for policy in ("newton", "adaptive", "substep"):
+        s = _gpu(**dict(kw, coarse_refresh=policy))
+        load = Loading(cfg, mesh, 0, 1)
+        load.step(s)
+        its[policy] = sum(c for _, _, c, _ in s.last_stats)
+        unverified[policy] = sum(d["reason"] != 1 for d in s.last_solve_info)
+        del s
+        torch.cuda.empty_cache()
+    print(f"[c2 coarse refresh] PCG iterations per substep {its}; solves not verified at 1e-6 "
+          f"{unverified}", flush=True)
+    assert unverified == {"newton": 0, "adaptive": 0, "substep": 0}
+    assert its["adaptive"] <= 1.1 * its["newton"]
+    # the lag rescue bounds what the lagging hierarchy costs (3.1x without it)
+    assert its["substep"] <= 2.0 * its["newton"]
+
+
 def test_c2_gap_is_the_newton_cap_not_precision():
     """C2 (E 1e4-1e8): every substep runs into the 30-Newton cap at rtol 1e-6 (the line search
     accepts 1/4 or 1/8 steps), so the positions are a mid-iteration Newton state that any
