@@ -61,6 +61,7 @@ SIGNATURES = {
     "mgpbd_pcg_counters": (c_int, [_V, ctypes.POINTER(c_int64), ctypes.POINTER(c_int64)]),
     "mgpbd_pcg_lag_rescues": (c_int, [_V, ctypes.POINTER(c_int64)]),
     "mgpbd_pcg_set_lag_rescue": (c_int, [_V, c_int]),
+    "mgpbd_pcg_set_start_criterion": (c_int, [_V, c_int]),
     "mgpbd_ctx_profile_read": (c_int, [_V, F64P, ctypes.POINTER(c_int64)]),
     # per-tet / per-vertex
     "mgpbd_rest_shape": (c_int, [_V, c_int, _V, _V, _V, c_double, c_double, _V, _V, _V]),

@@ -33,6 +33,7 @@ namespace mgpbd {
 constexpr int kBlock = 256;           // 4 wavefronts of 64
 constexpr int kMaxRedBlocks = 2048;   // grid cap for grid-stride reduction kernels
 constexpr int kPartialStride = 65536; // partials per reduction slot (full-grid kernels)
+constexpr int kPartialSlots = 5;       // reduction slots of a context (slots 0 .. 4 are used)
 
 // ------------------------------------------------------------------------------------------
 // Deterministic FP64 reductions in two phases. Phase 1: every block of the producing kernel
@@ -69,9 +70,10 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
 }
 
 struct RedWS {
-  double* partials;       // kPartialStride per slot, 4 slots
+  double* partials;       // kPartialStride per slot, kPartialSlots slots
 };
 
+// slot < kPartialSlots (every call site passes a constant)
 __device__ __forceinline__ void store_partial(RedWS ws, int slot, double v) {
   if (threadIdx.x == 0) ws.partials[(size_t)slot * kPartialStride + blockIdx.x] = v;
 }
@@ -1331,16 +1333,24 @@ __global__ void k_pcg_reset(PcgState* st, double tol, int relative) {
 }
 
 // Started from a guess: partials of b.b (slot 0), r.r (slot 1) and q.q (slot 2), where
-// r = b - A x_guess and q = b - A x_alt were written by k_sell<2>.
+// r = b - A x_guess and q = b - A x_alt were written by the residual pass; with `energy`, also
+// x.(b + r) (slot 3) and x_alt.(b + q) (slot 4), from which phi(x) = x.A x / 2 - b.x =
+// -x.(b + r) / 2, the CG energy (phi(x) - phi(x*) = ||x - x*||_A^2 / 2).
 __global__ void __launch_bounds__(kBlock)
 k_pcg_guess_norms(int n, const float* __restrict__ b, const float* __restrict__ r,
-                  const float* __restrict__ q, RedWS ws) {
-  double bb = 0.0, rr = 0.0, qq = 0.0;
+                  const float* __restrict__ q, const float* __restrict__ x,
+                  const float* __restrict__ x_alt, int energy, RedWS ws) {
+  double bb = 0.0, rr = 0.0, qq = 0.0, ea = 0.0, eb = 0.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double bi = b[i], ri = r[i];
     bb += bi * bi;
     rr += ri * ri;
-    if (q) { const double qi = q[i]; qq += qi * qi; }
+    if (energy) ea += (double)x[i] * (bi + ri);
+    if (q) {
+      const double qi = q[i];
+      qq += qi * qi;
+      if (energy) eb += (double)x_alt[i] * (bi + qi);
+    }
   }
   const double b0 = block_sum(bb);
   const double b1 = block_sum(rr);
@@ -1348,15 +1358,28 @@ k_pcg_guess_norms(int n, const float* __restrict__ b, const float* __restrict__ 
   store_partial(ws, 0, b0);
   store_partial(ws, 1, b1);
   store_partial(ws, 2, b2);
+  if (energy) {
+    const double b3 = block_sum(ea);
+    const double b4 = block_sum(eb);
+    store_partial(ws, 3, b3);
+    store_partial(ws, 4, b4);
+  }
 }
 
-// Picks the start with the smallest residual among the guess, the alternative guess and zero
-// (the guess wins a tie with the alternative; zero wins a tie and any non-finite residual).
-// The stopping test stays relative to ||b|| (or absolute), as from a zero start.
+// Picks the start among the guess, the alternative guess and zero: by the smallest residual,
+// or (energy) by the lowest CG energy phi, i.e. the smallest A-norm error (phi(0) = 0). The
+// guess wins a tie with the alternative; zero wins a tie and any non-finite value. The stopping
+// test stays relative to ||b|| (or absolute), as from a zero start.
 __global__ void __launch_bounds__(kBlock)
-k_pcg_fin_guess(RedWS ws, int nparts, int has_alt, PcgState* st) {
-  double t[3];
-  sum_partials_k<3>(ws, 0, nparts, t);
+k_pcg_fin_guess(RedWS ws, int nparts, int has_alt, int energy, PcgState* st) {
+  double t[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (energy) {
+    sum_partials_k<5>(ws, 0, nparts, t);
+  } else {
+    double t3[3];
+    sum_partials_k<3>(ws, 0, nparts, t3);
+    t[0] = t3[0]; t[1] = t3[1]; t[2] = t3[2];
+  }
   const double bb = t[0], ra = t[1], rb = t[2];
   if (threadIdx.x != 0) return;
   st->bb = bb;
@@ -1364,11 +1387,18 @@ k_pcg_fin_guess(RedWS ws, int nparts, int has_alt, PcgState* st) {
   st->rr_guess = ra;
   st->rr_alt = has_alt ? rb : 0.0;
   int start = 1;
-  double best = ra;
-  if (has_alt && rb < best) { start = 2; best = rb; }
-  if (!(best < bb)) { start = 0; best = bb; }   // also catches a non-finite guess
+  if (energy) {
+    const double pa = -0.5 * t[3], pb = -0.5 * t[4];
+    double best = pa;
+    if (has_alt && pb < best) { start = 2; best = pb; }
+    if (!(best < 0.0) || !(ra < 1e300) || (start == 2 && !(rb < 1e300))) start = 0;
+  } else {
+    double best = ra;
+    if (has_alt && rb < best) { start = 2; best = rb; }
+    if (!(best < bb)) start = 0;   // also catches a non-finite guess
+  }
   st->start = start;
-  st->rr0 = best;
+  st->rr0 = start == 0 ? bb : start == 2 ? rb : ra;
 }
 
 // Applies the chosen start: x, r (and the fused first sweep xv = (omega dinv0) r).
@@ -2975,6 +3005,8 @@ struct mgpbd_ctx {
   // mgpbd_mg_set_coarse_lagged); a solve on such a hierarchy that runs past
   // max(8, 2 fresh_iters + 2) iterations rebuilds levels 1.. and resumes from its iterate
   // (once per solve). fresh_iters: the last solve run entirely on a current hierarchy.
+  // warm-started solves pick their start by the CG energy instead of the residual
+  bool start_energy = false;
   bool coarse_lagged = false;
   int fresh_iters = 0;
   int lag_limit = -1;   // -1: max(8, 2 fresh_iters + 2); 0: no lag rescue; k > 0: k iterations
@@ -3059,7 +3091,8 @@ mgpbd_ctx* mgpbd_ctx_create(void* stream) {
   if (!c) { mgpbd_set_error("mgpbd_ctx_create: out of memory"); return nullptr; }
   c->stream = (hipStream_t)stream;
   hipError_t e = hipGetDevice(&c->device);
-  if (e == hipSuccess) e = hipMalloc(&c->ws.partials, sizeof(double) * kPartialStride * 4);
+  if (e == hipSuccess)
+    e = hipMalloc(&c->ws.partials, sizeof(double) * kPartialStride * kPartialSlots);
   if (e == hipSuccess) e = hipMalloc(&c->st, sizeof(PcgState));
   if (e == hipSuccess) e = hipMemset(c->st, 0, sizeof(PcgState));
   if (e == hipSuccess) e = hipHostMalloc(&c->st_host, sizeof(PcgState), hipHostMallocDefault);
@@ -3122,7 +3155,8 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
 
 int mgpbd_ctx_device_bytes(mgpbd_ctx* c, int64_t* bytes) {
   if (!c || !bytes) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_ctx_device_bytes: invalid arguments");
-  int64_t t = (int64_t)sizeof(double) * kPartialStride * 4 + (int64_t)sizeof(PcgState) +
+  int64_t t = (int64_t)sizeof(double) * kPartialStride * kPartialSlots +
+              (int64_t)sizeof(PcgState) +
               (int64_t)sizeof(float) * MGPBD_MAX_LEVELS * (1 + 2 * kChebSteps) +
               (int64_t)sizeof(double) * 4 * MGPBD_MAX_LEVELS;
   int nd = 0;
@@ -4794,8 +4828,10 @@ static int pcg_solve(mgpbd_ctx* ctx, const float* b, float* x, const float* x_al
         if (alt)
           sell_launch<2>(ctx, L, alt, Ap, nullptr, b, 0.f, nullptr, nullptr);
       }
-      k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, alt ? Ap : nullptr, ctx->ws);
-      k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, alt != nullptr, ctx->st);
+      k_pcg_guess_norms<<<gn, kBlock, 0, s>>>(n, b, r, alt ? Ap : nullptr, x, alt,
+                                               ctx->start_energy ? 1 : 0, ctx->ws);
+      k_pcg_fin_guess<<<1, kBlock, 0, s>>>(ctx->ws, gn, alt != nullptr,
+                                           ctx->start_energy ? 1 : 0, ctx->st);
       k_pcg_select<<<nblocks(n), kBlock, 0, s>>>(n, b, x, r, alt, Ap, L.dinv,
                                                  fuse_first ? xv : nullptr, ctx->omega,
                                                  wl(ctx, 0), ctx->st);
@@ -4980,6 +5016,12 @@ int mgpbd_pcg_set_lag_rescue(mgpbd_ctx* ctx, int limit) {
   CTX_CHECK("mgpbd_pcg_set_lag_rescue");
   if (limit < -1) return mgpbd_fail(MGPBD_EINVAL, "mgpbd_pcg_set_lag_rescue: limit >= -1");
   ctx->lag_limit = limit;
+  return 0;
+}
+
+int mgpbd_pcg_set_start_criterion(mgpbd_ctx* ctx, int energy) {
+  CTX_CHECK("mgpbd_pcg_set_start_criterion");
+  ctx->start_energy = energy != 0;
   return 0;
 }
 

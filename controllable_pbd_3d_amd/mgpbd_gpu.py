@@ -33,7 +33,9 @@ repeating, instead of the common ``omega_jacobi``), ``f_ext``
 ``warm_start`` ('none' | 'previous' | 'learned') with ``warm_start_model`` (a
 ``warm_start.LearnedWarmStart``): the first PCG of each substep starts from the previous
 substep's lambda, or from the network's prediction with the previous lambda as the fallback;
-the device keeps the start with the smaller residual and iterates to the same tolerance.
+the device keeps the start with the smaller residual (``start_criterion='residual'``, the
+default) or the lower CG energy x.A x / 2 - b.x, i.e. the smaller A-norm error
+(``start_criterion='energy'``), and iterates to the same tolerance.
 ``record_samples`` (a list) collects training pairs from the first Newton iteration.
 ``spectral_refresh`` ('newton' | 'substep'): the spectral smoothers re-estimate every level's
 spectrum, and the dense coarsest solve (coarse_solver='cholesky') re-inverts its level, after
@@ -113,6 +115,7 @@ class MGPBDSolverGPU:
                  fp64_iterate: bool = False, power_iters: int = 2,
                  smoother_safety: float = 1.1, cheb_ratio: float = 12.0,
                  stall_rescue: bool = True, smooth_schedule=None, f_ext=None, warm_start: str = "none",
+                 start_criterion: str = "residual",
                  warm_start_model=None, record_samples=None, level0: str = "assembled",
                  spectral_refresh: str = "newton", coarse_refresh: str = "newton",
                  coarse_refresh_iters: int = 8, overlap_inference: bool = True,
@@ -175,6 +178,7 @@ class MGPBDSolverGPU:
         self.smoother_safety = float(smoother_safety)
         self.cheb_ratio = float(cheb_ratio)
         self.stall_rescue = bool(stall_rescue)
+        self.start_criterion = start_criterion
         self.smooth_schedule = smooth_schedule
         self.omega_levels = omega_levels
         self.level0 = level0
@@ -208,6 +212,8 @@ class MGPBDSolverGPU:
         self.f_ext = None if f_ext is None else to_device(f_ext, torch.float32, dev, (-1, 3))
         if warm_start not in ("none", "previous", "learned"):
             raise ValueError("warm_start must be 'none', 'previous' or 'learned'")
+        if start_criterion not in ("residual", "energy"):
+            raise ValueError("start_criterion must be 'residual' or 'energy'")
         if warm_start == "learned" and warm_start_model is None:
             raise ValueError("warm_start='learned' needs warm_start_model")
         if record_samples is not None and warm_start_model is None:
@@ -263,6 +269,7 @@ class MGPBDSolverGPU:
             fp64_iterate=self.fp64_iterate, power_iters=self.power_iters,
             smoother_safety=self.smoother_safety, cheb_ratio=self.cheb_ratio,
             stall_rescue=self.stall_rescue, smooth_schedule=self.smooth_schedule,
+            start_criterion=self.start_criterion,
             omega_levels=self.omega_levels, device=self.device.index)
         self._build_incidence()
         if self.level0 == "factored":

@@ -243,7 +243,8 @@ class GPUMultigridSolver:
                  smoother: str = "jacobi", smoother_target: float = 4.0 / 3.0,
                  smoother_safety: float = 1.1, power_iters: int = 2, cheb_ratio: float = 12.0,
                  verify: int = 0, fp64_iterate: bool = False, stall_rescue: bool = True,
-                 smooth_schedule=None, omega_levels=None, device=None):
+                 smooth_schedule=None, omega_levels=None, start_criterion: str = "residual",
+                 device=None):
         self.n_smooth = int(n_smooth)
         # fixed per-level Jacobi weights (the last entry repeating), with smoother='jacobi'
         self.omega_levels = None if omega_levels is None else [float(w) for w in omega_levels]
@@ -278,6 +279,9 @@ class GPUMultigridSolver:
         self.verify = int(verify)   # 0 off, 1 verify + restart (True), 2 measure only
         self.fp64_iterate = bool(fp64_iterate)
         self.stall_rescue = bool(stall_rescue)
+        if start_criterion not in ("residual", "energy"):
+            raise ValueError("start_criterion must be 'residual' or 'energy'")
+        self.start_criterion = start_criterion
         # smoothing steps per level (the last entry repeats); None: n_smooth everywhere
         self.smooth_schedule = None if smooth_schedule is None else [int(k) for k in smooth_schedule]
         self.stall_ratio = 0.25   # squared-residual ratio over 8 iterations counted as a stall
@@ -563,6 +567,7 @@ class GPUMultigridSolver:
             ctx("mgpbd_mg_set_level_weights", len(w), w.ctypes.data_as(_lib.F32P))
         ctx("mgpbd_pcg_set_verify", int(self.verify))
         ctx("mgpbd_pcg_set_fp64_iterate", int(self.fp64_iterate))
+        ctx("mgpbd_pcg_set_start_criterion", int(self.start_criterion == "energy"))
         ctx("mgpbd_pcg_set_rescue", int(self.stall_rescue and self.smoother != "jacobi"), 1.25,
             self.stall_ratio)
         if self.smooth_schedule:
@@ -701,7 +706,8 @@ class GPUMultigridSolver:
     def solve_guess(self, b_field: torch.Tensor, x_field: torch.Tensor,
                     x_alt: torch.Tensor | None = None) -> int:
         """PCG from the guess held in ``x_field`` (extension): the device takes whichever of
-        the guess, ``x_alt`` and zero has the smallest residual ||b - A x0||, then iterates to
+        the guess, ``x_alt`` and zero has the smallest residual ||b - A x0|| (or, with
+        start_criterion='energy', the lowest CG energy x0.A x0 / 2 - b.x0), then iterates to
         the same stopping test as ``solve``. ``last_info['start']`` is 0 (zero), 1 (guess) or
         2 (x_alt)."""
         it = ctypes.c_int(0)

@@ -598,6 +598,11 @@ int mgpbd_pcg_set_rescue(mgpbd_ctx* ctx, int enable, float boost, double stall_r
 /* Cumulative counts over the context's solves: rescues (stall and lagged-hierarchy) and
  * verification restarts. */
 int mgpbd_pcg_counters(mgpbd_ctx* ctx, int64_t* rescues, int64_t* restarts);
+/* Start criterion of the warm-started solves (extension): 0 (the default) the candidate start
+ * (guess, alternative, zero) with the smallest residual ||b - A x0||; 1 the one with the lowest
+ * CG energy phi(x0) = x0.A x0 / 2 - b.x0, i.e. the smallest A-norm error, which is what bounds
+ * CG's iterations (a start with a small residual can carry a large smooth error). */
+int mgpbd_pcg_set_start_criterion(mgpbd_ctx* ctx, int energy);
 /* Iteration limit of the lagged-hierarchy rescue (extension; see mgpbd_mg_set_coarse_lagged):
  * -1 (the default) max(8, 2 f + 2), 0 off, k > 0 after k iterations. */
 int mgpbd_pcg_set_lag_rescue(mgpbd_ctx* ctx, int limit);

@@ -313,6 +313,37 @@ def test_bad_guess_falls_back_to_alternative_or_zero(pcg_scene):
     assert info["rr_guess"] > info["rr_alt"]
 
 
+def test_energy_criterion_picks_the_smaller_a_norm_error(pcg_scene):
+    """Two starts built from the exact solution x* and A's extreme eigenvectors: x1 = x* +
+    s1 v_min, x2 = x* + s2 v_max with s1 / s2 = kappa^(3/4), so x1 has the smaller residual
+    (lambda_min s1 < lambda_max s2) and the larger A-norm error (lambda_min s1^2 > lambda_max
+    s2^2). The residual criterion takes x1, the energy criterion x2 (checked against the FP64
+    energies), and both solves reach the tolerance."""
+    s, A, b, x_star = pcg_scene
+    lam, V = np.linalg.eigh(A.toarray())
+    kappa = lam[-1] / lam[0]
+    s2 = 1e-3 * np.linalg.norm(x_star) / np.sqrt(len(b))
+    s1 = s2 * kappa ** 0.75
+    x1 = x_star + s1 * V[:, 0]
+    x2 = x_star + s2 * V[:, -1]
+    phi = lambda x: 0.5 * x @ (A @ x) - b @ x
+    assert np.linalg.norm(b - A @ x1) < np.linalg.norm(b - A @ x2)
+    assert phi(x1) > phi(x2) and phi(x2) < 0.0
+    g = s._gpu_solver
+    chosen = {}
+    try:
+        for crit in ("residual", "energy"):
+            g.ctx("mgpbd_pcg_set_start_criterion", int(crit == "energy"))
+            it, x, info = _solve(s, x1, alt=x2)
+            chosen[crit] = info["start"]
+            assert info["rr"] < 1e-12 * info["bb"], crit
+    finally:
+        g.ctx("mgpbd_pcg_set_start_criterion", 0)
+    print(f"[start criterion] kappa {kappa:.1f}: residual takes {chosen['residual']}, energy "
+          f"takes {chosen['energy']}", flush=True)
+    assert chosen == {"residual": 1, "energy": 2}
+
+
 def test_spring_drag_force():
     from controllable_pbd_3d_amd.runtime import get_context
     ctx = get_context(0)
