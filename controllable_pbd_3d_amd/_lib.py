@@ -141,7 +141,8 @@ SIGNATURES = {
     "mgpbd_gnn_infer": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_loss_mse": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_loss_residual": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
-    "mgpbd_gnn_loss_energy": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "mgpbd_gnn_loss_energy": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V,
+                                      c_int]),
     "mgpbd_gnn_backward": (c_int, [_V, c_int, _V, _V, _V, _V, _V, _V, _V]),
     "mgpbd_gnn_adam": (c_int, [_V, _V, _V, _V, _V, c_float, c_float, c_float, c_float, c_int,
                                _V, _V]),

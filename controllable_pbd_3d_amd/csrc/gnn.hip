@@ -812,13 +812,29 @@ k_resid_norms(int n, float* __restrict__ ax, const float* __restrict__ b,
   if (threadIdx.x == 0) { partial[2 * blockIdx.x] = sh[0][0]; partial[2 * blockIdx.x + 1] = sh[1][0]; }
 }
 
-// scal[0] = r.r / b.b (the loss), scal[1] = 2 s / b.b (d loss / d y = scal[1] A r)
-__global__ void k_resid_fin(int nblk, const double* __restrict__ partial,
-                            const float* __restrict__ st, double* __restrict__ scal,
-                            double* __restrict__ loss) {
-  if (threadIdx.x != 0) return;
+// scal[0] = r.r / b.b (the loss), scal[1] = 2 s / b.b (d loss / d y = scal[1] A r), from the
+// (r.r, b.b) block pairs; one wavefront: lane l sums the pairs l, l + 64, ... in order, then an
+// xor butterfly (deterministic, the same total in every lane). With den_io and den_ready the
+// denominator b.b is read from den_io instead of the pairs' second entries; with den_io and
+// !den_ready it is also stored there (the energy loss's t.A t, constant per training pair).
+__global__ void __launch_bounds__(64)
+k_resid_fin(int nblk, const double* __restrict__ partial, const float* __restrict__ st,
+            double* __restrict__ scal, double* __restrict__ loss, double* __restrict__ den_io,
+            int den_ready) {
+  const int l = threadIdx.x;
   double rr = 0.0, bb = 0.0;
-  for (int b = 0; b < nblk; ++b) { rr += partial[2 * b]; bb += partial[2 * b + 1]; }
+  for (int b = l; b < nblk; b += 64) {
+    rr += partial[2 * b];
+    if (!den_ready) bb += partial[2 * b + 1];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    rr += __shfl_xor(rr, off, 64);
+    bb += __shfl_xor(bb, off, 64);
+  }
+  if (l != 0) return;
+  if (den_ready) bb = *den_io;
+  else if (den_io) *den_io = bb;
   const double inv = bb > 0.0 ? 1.0 / bb : 0.0;
   scal[0] = rr * inv;
   scal[1] = 2.0 * (double)st[0] * inv;
@@ -1040,7 +1056,7 @@ int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
   if (int e = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, dlam, r)) return e;
   const int gl = std::min(nb(n), 1024);
   k_resid_norms<<<gl, kBlock, 0, s>>>(n, r, b, W.dpartial);
-  k_resid_fin<<<1, 64, 0, s>>>(gl, W.dpartial, stats, W.scal, loss_out);
+  k_resid_fin<<<1, 64, 0, s>>>(gl, W.dpartial, stats, W.scal, loss_out, nullptr, 0);
   if (int e = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, r, ar)) return e;
   k_resid_dy<<<nb(n), kBlock, 0, s>>>(n, ar, W.scal, dy);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_loss_residual");
@@ -1050,7 +1066,7 @@ int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
 int mgpbd_gnn_loss_energy(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
                           const int32_t* sell_idx, const float* sell_val, const float* target,
                           const float* dlam, const float* stats, float* tmp, void* ws, float* dy,
-                          double* loss_out) {
+                          double* loss_out, double* tat_io, int tat_ready) {
   if (!ctx || n <= 0 || !slice_ptr || !sell_idx || !sell_val || !target || !dlam || !stats ||
       !tmp || !ws || !dy)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_gnn_loss_energy: invalid arguments");
@@ -1063,10 +1079,13 @@ int mgpbd_gnn_loss_energy(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr,
   k_energy_err<<<nb(n), kBlock, 0, s>>>(n, dlam, target, e);
   if (int r = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, e, ae)) return r;
   k_pair_dot<<<gl, kBlock, 0, s>>>(n, e, ae, W.dpartial, 0);          // e.Ae
-  if (int r = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, target, e)) return r;
-  k_pair_dot<<<gl, kBlock, 0, s>>>(n, target, e, W.dpartial, 1);     // t.At
+  const bool cached = tat_io && tat_ready;
+  if (!cached) {   // t.At, constant per training pair: once when the caller keeps it
+    if (int r = mgpbd_spmv(ctx, n, slice_ptr, sell_idx, sell_val, target, e)) return r;
+    k_pair_dot<<<gl, kBlock, 0, s>>>(n, target, e, W.dpartial, 1);   // t.At
+  }
   // same finalisation as the residual loss: scal = {e.Ae / t.At, 2 s / t.At}
-  k_resid_fin<<<1, 64, 0, s>>>(gl, W.dpartial, stats, W.scal, loss_out);
+  k_resid_fin<<<1, 64, 0, s>>>(gl, W.dpartial, stats, W.scal, loss_out, tat_io, cached ? 1 : 0);
   k_resid_dy<<<nb(n), kBlock, 0, s>>>(n, ae, W.scal, dy);
   MGPBD_LAUNCH_CHECK("mgpbd_gnn_loss_energy");
   return 0;

@@ -195,10 +195,14 @@ class LearnedWarmStart:
                      dptr(self.dlam), dptr(sample["stats"]), dptr(self.tmp), dptr(self.ws),
                      dptr(self.dy), dptr(self.loss))
         elif loss == "energy":
+            # t.A t is constant per pair: computed at the pair's first loss, then reused
+            ready = "tat" in sample
+            if not ready:
+                sample["tat"] = torch.zeros(1, dtype=torch.float64, device=self.dlam.device)
             self.ctx("mgpbd_gnn_loss_energy", self.n, dptr(sample["slice_ptr"]),
                      dptr(sample["sell_idx"]), dptr(sample["val"]), dptr(sample["target"]),
                      dptr(self.dlam), dptr(sample["stats"]), dptr(self.tmp), dptr(self.ws),
-                     dptr(self.dy), dptr(self.loss))
+                     dptr(self.dy), dptr(self.loss), dptr(sample["tat"]), int(ready))
         else:
             raise ValueError("loss must be 'energy', 'residual' or 'mse'")
 

@@ -691,7 +691,8 @@ int mgpbd_gnn_forward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float
 /* Inference-only forward (extension): the same network and the same result bit for bit as
  * mgpbd_gnn_forward's dlam_out, with the neighbour mean fused into each MFMA layer (no n x 64
  * mean written or read back), the encoder at 8 channels per thread and the decoder plus
- * prediction at one row per thread. Leaves no activations for a backward pass. */
+ * prediction computed in the last layer from its tile in LDS. Leaves no activations for a
+ * backward pass. */
 int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* x_dev,
                     const float* stats_dev, const float* lam_prev_dev, const float* params_dev,
                     const uint16_t* wt_dev, void* ws_dev, float* dlam_out_dev);
@@ -702,7 +703,9 @@ int mgpbd_gnn_infer(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* 
  *             SELL-64x4 matrix of the same solve (tmp_dev: 2 n floats). This is the quantity
  *             mgpbd_pcg_solve_guess compares between starts.
  *   energy:   (dlam - t)^T A (dlam - t) / t^T A t with t = target (the converged dlambda): the
- *             A-norm error that bounds CG's convergence from that start (tmp_dev: 2 n). */
+ *             A-norm error that bounds CG's convergence from that start (tmp_dev: 2 n).
+ *             tat_dev (nullable, one device double) caches t^T A t, which is constant per
+ *             training pair: tat_ready 0 computes and stores it, 1 reads it (one SpMV less). */
 int mgpbd_gnn_loss_mse(mgpbd_ctx* ctx, int n, const float* stats_dev, const float* lam_prev_dev,
                        const float* target_dev, void* ws_dev, float* dy_dev, double* loss_dev);
 int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr_dev,
@@ -712,7 +715,8 @@ int mgpbd_gnn_loss_residual(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr_dev,
 int mgpbd_gnn_loss_energy(mgpbd_ctx* ctx, int n, const int32_t* slice_ptr_dev,
                           const int32_t* sell_idx_dev, const float* sell_val_dev,
                           const float* target_dev, const float* dlam_dev, const float* stats_dev,
-                          float* tmp_dev, void* ws_dev, float* dy_dev, double* loss_dev);
+                          float* tmp_dev, void* ws_dev, float* dy_dev, double* loss_dev,
+                          double* tat_dev, int tat_ready);
 /* Backward pass from dy_dev = d loss / d y: grad_dev gets d loss / d params (FP32,
  * param_count values, overwritten). */
 int mgpbd_gnn_backward(mgpbd_ctx* ctx, int n, const int32_t* nbr_dev, const float* x_dev,
