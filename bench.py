@@ -72,17 +72,18 @@ def kernel_bytes(n_rows, nnz, kind, per_nnz=8, chebyshev=False):
 def factored_bytes(n_rows, n_verts, kind, chebyshev=False, offsets=False, packed=False):
     """Algorithmic HBM bytes of the factored level 0 (each array counted once per pass; 4
     incidence slots per row). kind "gather" (vertex pass): 4 slots per row x (4-byte incidence
-    + 8-byte K) + x (4 per row) + u written (12 per vertex) = 52 n + 12 n_v; with 16-bit slot
-    offsets (the tet in K's fourth word, a 4-byte base tet per vertex) 36 n + 16 n_v. "apply"
-    (tet pass of a sweep): t_i 16 + K 24 + a^ 4 + x 4 + b 4 + y 4 per row + u read (12 per
-    vertex) = 56 n + 12 n_v. "jacobi"/"resid": both passes (+ the previous iterate, 4 n, for the
-    Chebyshev sweeps). "spmv": the PCG's FP64 product from the FP32 gradients, 4 slots x
+    + 8-byte K) + x (4 per row) + the vertex weight read and u written (4 + 12 per vertex) =
+    52 n + 16 n_v; with 16-bit slot offsets (the tet in K's fourth word, a 4-byte base tet per
+    vertex) 36 n + 20 n_v. "apply" (tet pass of a sweep): t_i 16 + K 18 (three vertices' nine
+    halves; the fourth vertex's is their negative sum) + a^ 4 + x 4 + b 4 + y 4 per row + u
+    read (12 per vertex) = 50 n + 12 n_v. "jacobi"/"resid": both passes (+ the previous
+    iterate, 4 n, for the Chebyshev sweeps). "spmv": the PCG's FP64 product from the FP32 gradients, 4 slots x
     (4-byte incidence + 16-byte gradient; 16 bytes with offsets, + 4 per vertex) + p + u (32
     per vertex) written; t_i 16 + gradC 48 + alpha~ 4 + p 4 + A p 4 per row + u read. With
     packed rows (packed) the tet passes read 8 bytes of row vertices instead of t_i's 16."""
-    gather = (36 * n_rows + 16 * n_verts) if offsets else (52 * n_rows + 12 * n_verts)
+    gather = (36 * n_rows + 20 * n_verts) if offsets else (52 * n_rows + 16 * n_verts)
     tv = 8 if packed else 16
-    apply = (40 + tv) * n_rows + 12 * n_verts
+    apply = (34 + tv) * n_rows + 12 * n_verts
     if kind == "gather":
         return gather
     if kind == "apply":

@@ -2263,21 +2263,26 @@ k_pcg_p_flex(int n, const float* __restrict__ z, float* __restrict__ p, float* _
 // Factored level 0 of the reduced-precision V-cycle. The level-0 matrix is
 // A = J W J^T + diag(alpha~): row t of J holds tet t's vertex gradients g_tk at its vertices'
 // columns, W = diag(inverse masses). Its scaled form D^-1/2 A D^-1/2 is therefore
-// K K^T + diag(a^) with K_tk = sqrt(w_v) g_tk / sqrt(A_tt) (|K_tk| <= 1, so FP16 keeps ~11
-// bits of every component) and a^_t = alpha~_t / A_tt. y = A^ x takes two passes:
-//   per vertex  u_v = sum over the tets t holding v of K_tk x_t      (k_fac_gather)
-//   per tet     y_t = a^_t x_t + sum_k K_tk . u_v(t,k)                (k_fac_apply)
+// K W' K^T + diag(a^) with K_tk = c g_tk / sqrt(A_tt), W' = W / c^2 and a^_t = alpha~_t / A_tt,
+// c^2 = the largest inverse mass (so |K_tk| = sqrt(w_max / w_v) times what the unit-weight
+// form sqrt(w_v) g_tk / sqrt(A_tt) <= 1 holds: ~11 bits of every component in FP16).
+// A tet's constraint is a function of its deformation gradient, so its four vertex gradients
+// sum to zero: the tet order stores the first three rounded, and the fourth is defined as
+// the rounded negative sum of those three (k_fac_k3) in both copies. y = A^ x takes two
+// passes:
+//   per vertex  u_v = w'_v sum over the tets t holding v of K_tk x_t   (k_fac_gather)
+//   per tet     y_t = a^_t x_t + sum_k K_tk . u_v(t,k)                  (k_fac_apply)
 // The vertex pass reads a vertex-sliced incidence list (vs_inc = 4 tet + local vertex) with K
 // beside it (four halves per slot, the fourth zero): four lanes per vertex, 16 vertices per
 // 64-lane slice, slot k of vertex 16s+l at vs_ptr[s] + 64 (k/4) + 4l + k%4, so lane q of a
 // vertex takes its slots q, q+4, ... with coalesced loads across the wavefront and the four
 // partial sums meet in two shuffles (one lane per vertex left too few wavefronts to cover the
-// gathers' latency). The tet pass reads K in tet order (12 halves per tet in three planes of
-// four halves). Both copies are rounded from the same FP32 product, so the
-// operator is exactly symmetric. Per application this streams ~12 bytes per incidence slot
-// and ~44 per tet (~120 MB at 1M tets) instead of ~4 bytes per stored entry of the assembled
-// half level (~307 MB), and the per-Newton-iteration conversion reads the gradients instead of
-// the 72.5M FP32 entries.
+// gathers' latency). The tet pass reads K in tet order: nine halves per tet, eight in a
+// 16-byte plane and the ninth in a 2-byte plane (FacK). Both copies hold the same bits, so
+// the operator is exactly symmetric. Per application this streams ~12 bytes per incidence
+// slot and ~42 per tet (~115 MB at 1M tets) instead of ~4 bytes per stored entry of the
+// assembled half level (~307 MB), and the per-Newton-iteration conversion reads the
+// gradients instead of the 72.5M FP32 entries.
 // ------------------------------------------------------------------------------------------
 typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
 // Per-vertex products u of the vertex pass: three floats, 12 bytes (one dwordx3 access)
@@ -2294,15 +2299,38 @@ __device__ __forceinline__ float lo_h(uint32_t w) { return h2f<VT>((unsigned sho
 template <int VT>
 __device__ __forceinline__ float hi_h(uint32_t w) { return h2f<VT>((unsigned short)(w >> 16)); }
 
+// A K component in the storage format; FP16 saturates at +-65504 (a tet whose vertices are
+// almost all pinned can scale a pinned vertex's gradient far up, and an infinity there would
+// turn that vertex's zero weight into NaN).
+template <int VT>
+__device__ __forceinline__ unsigned short f2h_sat(float f) {
+  if (VT == 1) f = fminf(fmaxf(f, -65504.0f), 65504.0f);
+  return f2h<VT>(f);
+}
+
+// K of a level-0 tet in tet order: halves 0..7 (g_0 xyz, g_1 xyz, g_2 xy) in one 16-byte
+// plane, half 8 (g_2 z) in a 2-byte plane.
+struct FacK {
+  const v4u32* __restrict__ a;
+  const unsigned short* __restrict__ z;
+};
+
+// One component of the fourth vertex's K from the other three (stored) ones: the rounded
+// negative sum, in a fixed order, so every reader gets the same bits.
+template <int VT>
+__device__ __forceinline__ unsigned short k_fac_k3(float k0, float k1, float k2) {
+  return f2h_sat<VT>(-((k0 + k1) + k2));
+}
+
 // The level-0 diagonal A_tt = sum_k w_k |g_tk|^2 + alpha~_t from the gradients (sd, isd, 1/A_tt),
-// then K in tet order (planes tk[0..n), tk[n..2n), tk[2n..3n)) and a^; k_fac_vert writes the
-// same K bits in vertex-slot order from the isd written here.
+// then K = cs isd_t g_tk of the first three vertices in tet order (FacK) and a^; k_fac_vert
+// copies the same K bits (and the fourth vertex's, k_fac_k3) into vertex-slot order.
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_tet(int n, const int4* __restrict__ t_i, const float* __restrict__ invm,
-          const float* __restrict__ gradC, const float* __restrict__ alpha,
+          const float* __restrict__ gradC, const float* __restrict__ alpha, float cs,
           float* __restrict__ sd, float* __restrict__ isd, float* __restrict__ dinv,
-          v2u32* __restrict__ tk, float* __restrict__ ahat) {
+          v4u32* __restrict__ tka, unsigned short* __restrict__ tkz, float* __restrict__ ahat) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= n) return;
   const int4 vi = t_i[t];
@@ -2320,12 +2348,25 @@ k_fac_tet(int n, const int4* __restrict__ t_i, const float* __restrict__ invm,
   sd[t] = r;
   isd[t] = is;
   dinv[t] = d > 0.0f ? 1.0f / d : 0.0f;
-  const float s0 = sqrtf(w0) * is, s1 = sqrtf(w1) * is;
-  const float s2 = sqrtf(w2) * is, s3 = sqrtf(w3) * is;
-  tk[t] = v2u32{pack_h2<VT>(s0 * a.x, s0 * a.y), pack_h2<VT>(s0 * a.z, s1 * a.w)};
-  tk[(size_t)n + t] = v2u32{pack_h2<VT>(s1 * b.x, s1 * b.y), pack_h2<VT>(s2 * b.z, s2 * b.w)};
-  tk[2 * (size_t)n + t] = v2u32{pack_h2<VT>(s2 * c.x, s3 * c.y), pack_h2<VT>(s3 * c.z, s3 * c.w)};
+  const float s = cs * is;
+  const auto h2 = [&](float p, float q) {
+    return (uint32_t)f2h_sat<VT>(s * p) | ((uint32_t)f2h_sat<VT>(s * q) << 16);
+  };
+  tka[t] = v4u32{h2(a.x, a.y), h2(a.z, a.w), h2(b.x, b.y), h2(b.z, b.w)};
+  tkz[t] = f2h_sat<VT>(s * c.x);
   ahat[t] = (alpha[t] * is) * is;
+}
+
+// The four vertices' K of tet t from the tet-order planes (the fourth by k_fac_k3), FP32.
+template <int VT>
+__device__ __forceinline__ void fac_tet_k(FacK tk, int t, float (&k)[12]) {
+  const v4u32 q = __builtin_nontemporal_load(tk.a + t);
+  const unsigned short z = __builtin_nontemporal_load(tk.z + t);
+  k[0] = lo_h<VT>(q.x); k[1] = hi_h<VT>(q.x); k[2] = lo_h<VT>(q.y);
+  k[3] = hi_h<VT>(q.y); k[4] = lo_h<VT>(q.z); k[5] = hi_h<VT>(q.z);
+  k[6] = lo_h<VT>(q.w); k[7] = hi_h<VT>(q.w); k[8] = h2f<VT>(z);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) k[9 + c] = h2f<VT>(k_fac_k3<VT>(k[c], k[3 + c], k[6 + c]));
 }
 
 constexpr int kFacLanes = 4;   // lanes per vertex in the vertex pass
@@ -2343,24 +2384,27 @@ template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_vert(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
            const int* __restrict__ vs_inc, const int* __restrict__ tbase,
-           const float* __restrict__ invm, const float* __restrict__ gradC,
-           const float* __restrict__ isd, v2u32* __restrict__ vs_k,
-           float4* __restrict__ vs_g) {
+           const float* __restrict__ gradC, FacK tk, v2u32* __restrict__ vs_k,
+           float4* __restrict__ vs_g, const float* __restrict__ invm, float ic2,
+           int2* __restrict__ tbw) {
   const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int v = gid >> 2, q = gid & 3;
   if (v >= nv) return;
   const int base = fac_slot0(vs_ptr, v, q);
   const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
-  const float sw = sqrtf(invm[v]);
   const int tb = tbase ? tbase[v] : 0;
+  if (tbw && q == 0) tbw[v] = make_int2(tb, __float_as_int(invm[v] * ic2));
   for (int k = 0; k < own; ++k) {
     const int p = base + 64 * k;
     const int inc = vs_inc[p];
-    const int t = inc >> 2;
+    const int t = inc >> 2, j = inc & 3;
     const uint32_t off = tbase ? (uint32_t)(t - tb) : 0u;
-    const float s = sw * isd[t];
-    const float* g = gradC + 12 * (size_t)t + 3 * (inc & 3);
-    vs_k[p] = v2u32{pack_h2<VT>(s * g[0], s * g[1]), (uint32_t)f2h<VT>(s * g[2]) | (off << 16)};
+    float kt[12];
+    fac_tet_k<VT>(tk, t, kt);
+    const float* g = gradC + 12 * (size_t)t + 3 * j;
+    // re-rounding a value decoded from the storage format returns its bits
+    vs_k[p] = v2u32{pack_h2<VT>(kt[3 * j], kt[3 * j + 1]),
+                    (uint32_t)f2h<VT>(kt[3 * j + 2]) | (off << 16)};
     vs_g[p] = make_float4(g[0], g[1], g[2], __uint_as_float(off));
   }
 }
@@ -2372,7 +2416,7 @@ __device__ __forceinline__ int slot_tet(const int* __restrict__ vs_inc, int p, i
   return OFF ? tb + (int)(word >> 16) : __builtin_nontemporal_load(vs_inc + p) >> 2;
 }
 
-// u_v = sum_k K_{t_k, v} x_{t_k} over the vertex's incidence slots (FP32 accumulation): each
+// u_v = w'_v sum_k K_{t_k, v} x_{t_k} over the vertex's incidence slots (FP32 accumulation): each
 // of the vertex's four lanes sums its own slots (loads of up to six issued before their
 // products: six slots per lane is an interior vertex of a Kuhn-cube mesh, and a batch longer
 // than the lane's slots costs reloads), then lanes q and q^1, q and q^2 add (fixed order,
@@ -2383,7 +2427,7 @@ __device__ __forceinline__ void fac_gather_lane(int v, int q, const int* __restr
                                                 const int* __restrict__ vs_cnt,
                                                 const int* __restrict__ vs_inc, int tb,
                                                 const v2u32* __restrict__ vs_k,
-                                                const float* __restrict__ x,
+                                                const float* __restrict__ x, float w,
                                                 fvec3* __restrict__ u) {
   const int base = fac_slot0(vs_ptr, v, q);
   const int own = (vs_cnt[v] - q + kFacLanes - 1) / kFacLanes;
@@ -2411,33 +2455,35 @@ __device__ __forceinline__ void fac_gather_lane(int v, int q, const int* __restr
   }
   ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
   ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
-  if (q == 0) u[v] = fvec3{ax, ay, az};
+  if (q == 0) u[v] = fvec3{w * ax, w * ay, w * az};
 }
 
 template <int VT>
 __global__ void __launch_bounds__(kBlock)
 k_fac_gather(int nv, const int* __restrict__ vs_ptr, const int* __restrict__ vs_cnt,
              const int* __restrict__ vs_inc, const int* __restrict__ tbase,
-             const v2u32* __restrict__ vs_k, const float* __restrict__ x,
-             fvec3* __restrict__ u, const int* __restrict__ done) {
+             const v2u32* __restrict__ vs_k, const float* __restrict__ invm, float ic2,
+             const float* __restrict__ x, fvec3* __restrict__ u,
+             const int* __restrict__ done) {
   if (done && *done) return;
   const int gid = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int v = gid >> 2, q = gid & 3;
   if (v >= nv) return;   // all four lanes of a vertex leave together
+  const float w = invm[v] * ic2;
   if (tbase)
-    fac_gather_lane<VT, true>(v, q, vs_ptr, vs_cnt, vs_inc, tbase[v], vs_k, x, u);
+    fac_gather_lane<VT, true>(v, q, vs_ptr, vs_cnt, vs_inc, tbase[v], vs_k, x, w, u);
   else
-    fac_gather_lane<VT, false>(v, q, vs_ptr, vs_cnt, vs_inc, 0, vs_k, x, u);
+    fac_gather_lane<VT, false>(v, q, vs_ptr, vs_cnt, vs_inc, 0, vs_k, x, w, u);
 }
 
 // The vertex pass on the uniform layout: every slice R rows tall (slot k of vertex v at
 // (v / 16) 64 R + 64 (k / 4) + 4 (v % 16) + k % 4) and the tets in the K words' offsets, so a
-// lane issues its R K-word loads and the base-tet load at once, with no slice pointer or
-// count to wait for. Slots past a vertex's count hold K = 0 and offset 0 (never written), so
+// lane issues its R K-word loads and the load of the vertex's {base tet, w'} word at once,
+// with no slice pointer or count to wait for. Slots past a vertex's count hold K = 0 and offset 0 (never written), so
 // they add exact zeros after the lane's own slots: the sums equal k_fac_gather's bit for bit.
 template <int VT, int R>
 __global__ void __launch_bounds__(kBlock)
-k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ vs_k,
+k_fac_gather_u(int nv, const int2* __restrict__ tbw, const v2u32* __restrict__ vs_k,
                const float* __restrict__ x, fvec3* __restrict__ u,
                const int* __restrict__ done) {
   if (done && *done) return;
@@ -2445,7 +2491,9 @@ k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ 
   const int v = gid >> 2, q = gid & 3;
   if (v >= nv) return;
   const int base = (v >> 4) * (64 * R) + ((v & 15) << 2) + q;
-  const int tb = tbase[v];
+  const int2 tw = tbw[v];   // base tet, w' = invm_v / c^2
+  const int tb = tw.x;
+  const float w = __int_as_float(tw.y);
   v2u32 h[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) h[j] = __builtin_nontemporal_load(vs_k + base + 64 * j);
@@ -2461,7 +2509,7 @@ k_fac_gather_u(int nv, const int* __restrict__ tbase, const v2u32* __restrict__ 
   }
   ax += __shfl_xor(ax, 1, 64); ay += __shfl_xor(ay, 1, 64); az += __shfl_xor(az, 1, 64);
   ax += __shfl_xor(ax, 2, 64); ay += __shfl_xor(ay, 2, 64); az += __shfl_xor(az, 2, 64);
-  if (q == 0) u[v] = fvec3{ax, ay, az};
+  if (q == 0) u[v] = fvec3{w * ax, w * ay, w * az};
 }
 
 // The four vertices of a level-0 row for the tet passes: from the int4 list, or from one
@@ -2488,44 +2536,41 @@ struct RowsPk {
 
 // (A^ x)_t from u: a^_t x_t + K_t0 . u_v0 + ... + K_t3 . u_v3 (FP32)
 template <int VT, class RS>
-__device__ __forceinline__ float fac_row(int t, int n, RS rows, const v2u32* __restrict__ tk,
-                                         const float* __restrict__ ahat,
+__device__ __forceinline__ float fac_row(int t, RS rows, FacK tk, const float* __restrict__ ahat,
                                          const fvec3* __restrict__ u, float xt) {
   const int4 vi = rows(t);
-  const v2u32 q0 = __builtin_nontemporal_load(tk + t);
-  const v2u32 q1 = __builtin_nontemporal_load(tk + (size_t)n + t);
-  const v2u32 q2 = __builtin_nontemporal_load(tk + 2 * (size_t)n + t);
+  float k[12];
+  fac_tet_k<VT>(tk, t, k);
   const fvec3 u0 = u[vi.x], u1 = u[vi.y], u2 = u[vi.z], u3 = u[vi.w];
   float acc = ahat[t] * xt;
-  acc = fmaf(lo_h<VT>(q0.x), u0.x, acc);
-  acc = fmaf(hi_h<VT>(q0.x), u0.y, acc);
-  acc = fmaf(lo_h<VT>(q0.y), u0.z, acc);
-  acc = fmaf(hi_h<VT>(q0.y), u1.x, acc);
-  acc = fmaf(lo_h<VT>(q1.x), u1.y, acc);
-  acc = fmaf(hi_h<VT>(q1.x), u1.z, acc);
-  acc = fmaf(lo_h<VT>(q1.y), u2.x, acc);
-  acc = fmaf(hi_h<VT>(q1.y), u2.y, acc);
-  acc = fmaf(lo_h<VT>(q2.x), u2.z, acc);
-  acc = fmaf(hi_h<VT>(q2.x), u3.x, acc);
-  acc = fmaf(lo_h<VT>(q2.y), u3.y, acc);
-  acc = fmaf(hi_h<VT>(q2.y), u3.z, acc);
+  acc = fmaf(k[0], u0.x, acc);
+  acc = fmaf(k[1], u0.y, acc);
+  acc = fmaf(k[2], u0.z, acc);
+  acc = fmaf(k[3], u1.x, acc);
+  acc = fmaf(k[4], u1.y, acc);
+  acc = fmaf(k[5], u1.z, acc);
+  acc = fmaf(k[6], u2.x, acc);
+  acc = fmaf(k[7], u2.y, acc);
+  acc = fmaf(k[8], u2.z, acc);
+  acc = fmaf(k[9], u3.x, acc);
+  acc = fmaf(k[10], u3.y, acc);
+  acc = fmaf(k[11], u3.z, acc);
   return acc;
 }
 
 // Factored counterpart of k_sellw on level 0 (same MODE semantics, rows = tets).
 template <int VT, int MODE, class RS>
 __global__ void __launch_bounds__(kBlock)
-k_fac_apply(int n, RS rows, const v2u32* __restrict__ tk,
-            const float* __restrict__ ahat, const fvec3* __restrict__ u,
-            const float* __restrict__ x, float* y, const float* __restrict__ b,
-            float omega, const float* __restrict__ wp, const float* __restrict__ isd,
+k_fac_apply(int n, RS rows, FacK tk, const float* __restrict__ ahat,
+            const fvec3* __restrict__ u, const float* __restrict__ x, float* y,
+            const float* __restrict__ b, float omega, const float* __restrict__ wp, const float* __restrict__ isd,
             const int* __restrict__ done, const float* xp, const float* __restrict__ cc) {
   if (wp) omega = *wp;
   if (done && *done) return;
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const float xi = x[i];
-  const float ax = fac_row<VT>(i, n, rows, tk, ahat, u, xi);
+  const float ax = fac_row<VT>(i, rows, tk, ahat, u, xi);
   if (MODE == 2) {
     y[i] = b[i] - ax;
   } else if (MODE == 1 || MODE == 3) {
@@ -2542,14 +2587,14 @@ k_fac_apply(int n, RS rows, const v2u32* __restrict__ tk,
 // v.A^v, v.v, w.w of k_power_step_h; the grid covers at least n rows.
 template <int VT, class RS>
 __global__ void __launch_bounds__(kBlock)
-k_fac_power(int n, RS rows, const v2u32* __restrict__ tk,
-            const float* __restrict__ ahat, const fvec3* __restrict__ u,
-            const float* __restrict__ v, float* __restrict__ w, RedWS ws) {
+k_fac_power(int n, RS rows, FacK tk, const float* __restrict__ ahat,
+            const fvec3* __restrict__ u, const float* __restrict__ v, float* __restrict__ w,
+            RedWS ws) {
   double vav = 0.0, vv = 0.0, ww = 0.0;
   const int i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < n) {
     const float vi = v[i];
-    const float ax = fac_row<VT>(i, n, rows, tk, ahat, u, vi);
+    const float ax = fac_row<VT>(i, rows, tk, ahat, u, vi);
     w[i] = ax;
     vav = (double)vi * ax;
     vv = (double)vi * vi;
@@ -2916,6 +2961,10 @@ struct mgpbd_ctx {
   int* ns_flag = nullptr;
   int* ns_flag_host = nullptr;
   hipEvent_t ns_event = nullptr;
+  // per-vertex {base tet, vertex weight w'} of the factored level 0's uniform vertex pass
+  // (one 8-byte read per vertex; library-owned, grown by mgpbd_mg_set_level0_offsets)
+  int2* fac_tbw = nullptr;
+  int fac_tbw_cap = 0;
   bool ns_pending = false;
   bool ns_rejected = false;
   float omega = 2.0f / 3.0f;
@@ -2965,12 +3014,16 @@ struct mgpbd_ctx {
     const int* vs_cnt = nullptr;
     const int* vs_inc = nullptr;
     const int* tbase = nullptr;   // per-vertex base tet of the 16-bit slot offsets (or NULL)
+    int2* tbw = nullptr;          // {tbase, w'} per vertex, written by k_fac_vert (with tbase)
     int rows_u = 0;               // rows of the uniform vertex-slice layout (0: per slice)
     const uint64_t* tpk = nullptr;  // packed row vertices (or NULL: t_i)
     int pk_bb = 0, pk_ob = 0;       // bits of the first vertex and of each difference
     v2u32* vs_k = nullptr;   // K per vertex slot (4 halves)
     float4* vs_g = nullptr;  // FP32 gradient per vertex slot (the PCG's operator)
-    v2u32* tk = nullptr;     // K per tet, three planes of n_tets
+    v4u32* tka = nullptr;             // K per tet: halves 0..7 (16-byte plane of n_tets)
+    unsigned short* tkz = nullptr;    // K per tet: half 8 (2-byte plane of n_tets)
+    float cs = 1.0f, ic2 = 1.0f;      // c = sqrt(max inverse mass) in K, 1 / c^2 in W'
+    FacK k() const { return FacK{tka, tkz}; }
     float* ahat = nullptr;   // a^ per tet
     fvec3* u = nullptr;     // per-vertex products of the V-cycle's gather pass
     double4* u64 = nullptr;  // per-vertex products of the PCG operator's gather pass
@@ -3141,6 +3194,7 @@ void mgpbd_ctx_destroy(mgpbd_ctx* c) {
   if (c->dT) (void)hipFree(c->dT);
   if (c->dnorm) (void)hipFree(c->dnorm);
   if (c->ns_flag) (void)hipFree(c->ns_flag);
+  if (c->fac_tbw) (void)hipFree(c->fac_tbw);
   if (c->ns_flag_host) (void)hipHostFree(c->ns_flag_host);
   if (c->ns_event) (void)hipEventDestroy(c->ns_event);
   (void)hipFree(c->ws.partials);
@@ -3921,13 +3975,13 @@ static void fac_gather(mgpbd_ctx* ctx, const float* x, const int* done) {
   const mgpbd_ctx::Factored& F = ctx->fac;
   hipStream_t s = ctx->stream;
   const int g = nblocks((int64_t)kFacLanes * F.nv);
-  switch (F.rows_u) {
-#define FGU(R) case R: k_fac_gather_u<VT, R><<<g, kBlock, 0, s>>>(F.nv, F.tbase, F.vs_k, x, F.u, done); return;
+  switch (F.tbw ? F.rows_u : 0) {
+#define FGU(R) case R: k_fac_gather_u<VT, R><<<g, kBlock, 0, s>>>(F.nv, F.tbw, F.vs_k, x, F.u, done); return;
     FGU(1) FGU(2) FGU(3) FGU(4) FGU(5) FGU(6) FGU(7) FGU(8)
 #undef FGU
     default:
       k_fac_gather<VT><<<g, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.vs_k,
-                                            x, F.u, done);
+                                            F.invm, F.ic2, x, F.u, done);
   }
 }
 
@@ -3942,7 +3996,7 @@ static void launch_fac(mgpbd_ctx* ctx, int mode, const float* x, float* y, const
   const float w = ctx->omega;
   with_rows(F, [&](auto rows) {
     using RS = decltype(rows);
-#define FAC(M) k_fac_apply<VT, M, RS><<<g, kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, x, y, b, \
+#define FAC(M) k_fac_apply<VT, M, RS><<<g, kBlock, 0, s>>>(L.n, rows, F.k(), F.ahat, F.u, x, y, b, \
                                                            w, wp, L.isd, done, xp, cc)
     switch (mode) {
       case 1: FAC(1); break;
@@ -4139,13 +4193,13 @@ static int smoother_refresh(mgpbd_ctx* ctx, bool fresh, float boost) {
       if (fac && ctx->precision == MGPBD_PREC_FP16) {
         fac_gather<1>(ctx, v, nullptr);
         with_rows(F, [&](auto rows) {
-          k_fac_power<1, decltype(rows)><<<g, kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, v, w,
+          k_fac_power<1, decltype(rows)><<<g, kBlock, 0, s>>>(L.n, rows, F.k(), F.ahat, F.u, v, w,
                                                               ctx->ws);
         });
       } else if (fac) {
         fac_gather<2>(ctx, v, nullptr);
         with_rows(F, [&](auto rows) {
-          k_fac_power<2, decltype(rows)><<<g, kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, v, w,
+          k_fac_power<2, decltype(rows)><<<g, kBlock, 0, s>>>(L.n, rows, F.k(), F.ahat, F.u, v, w,
                                                               ctx->ws);
         });
       } else if (ctx->precision == MGPBD_PREC_FP32 && L.woff)
@@ -4312,21 +4366,34 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
       !vs_k || !vs_g || !tk || !ahat || !u || !u64 || !dinv)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: invalid arguments");
   const auto al = [](const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; };
-  if (!al(t_i, 16) || !al(gradC, 16) || !al(vs_k, 8) || !al(tk, 8) || !al(u, 16) ||
+  if (!al(t_i, 16) || !al(gradC, 16) || !al(vs_k, 8) || !al(tk, 16) || !al(u, 16) ||
       !al(vs_g, 16) || !al(u64, 32))
-    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: t_i, gradC, vs_g and u need "
-                                    "16-byte, u64 32-byte, vs_k and tk 8-byte alignment");
+    return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: t_i, gradC, vs_g, tk and u "
+                                    "need 16-byte, u64 32-byte, vs_k 8-byte alignment");
   if (!ctx->L[0].sd || !ctx->L[0].isd)
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_factored: call mgpbd_mg_set_level_half "
                                     "for level 0 first (its scales)");
+  // K's scale c: c^2 = the largest inverse mass (read once at set-up, after whatever stream
+  // wrote the masses; they are the mesh's)
+  std::vector<float> w(n_verts);
+  MGPBD_HIP_TRY(hipDeviceSynchronize());
+  MGPBD_HIP_TRY(hipMemcpy(w.data(), v_invm, sizeof(float) * (size_t)n_verts,
+                          hipMemcpyDeviceToHost));
+  float wmax = 0.0f;
+  for (float x : w)
+    if (std::isfinite(x)) wmax = std::max(wmax, x);
   mgpbd_ctx::Factored& F = ctx->fac;
+  F.cs = wmax > 0.0f ? std::sqrt(wmax) : 1.0f;
+  F.ic2 = 1.0f / (F.cs * F.cs);
   F.nv = n_verts;
   F.t_i = reinterpret_cast<const int4*>(t_i);
   F.invm = v_invm; F.gradC = gradC; F.alpha = alpha_tilde;
   F.vs_ptr = vs_ptr; F.vs_cnt = vs_cnt; F.vs_inc = vs_inc;
   F.vs_k = reinterpret_cast<v2u32*>(vs_k);
   F.vs_g = reinterpret_cast<float4*>(vs_g);
-  F.tk = reinterpret_cast<v2u32*>(tk);
+  const int nt = ctx->L[0].n;
+  F.tka = reinterpret_cast<v4u32*>(tk);
+  F.tkz = reinterpret_cast<unsigned short*>(tk + 4 * (size_t)nt);
   F.ahat = ahat;
   F.u = reinterpret_cast<fvec3*>(u);
   F.u64 = reinterpret_cast<double4*>(u64);
@@ -4344,7 +4411,15 @@ int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase, int rows_u
     return mgpbd_fail(MGPBD_EINVAL, "mgpbd_mg_set_level0_offsets: rows_uniform in [0, 8], and "
                                     "a uniform layout needs the base tets");
   drop_graphs(ctx);
+  if (tbase && ctx->fac_tbw_cap < F.nv) {
+    if (ctx->fac_tbw) MGPBD_HIP_TRY(hipFree(ctx->fac_tbw));
+    ctx->fac_tbw = nullptr;
+    ctx->fac_tbw_cap = 0;
+    MGPBD_HIP_TRY(hipMalloc(&ctx->fac_tbw, sizeof(int2) * (size_t)F.nv));
+    ctx->fac_tbw_cap = F.nv;
+  }
   F.tbase = tbase;
+  F.tbw = tbase ? ctx->fac_tbw : nullptr;
   F.rows_u = rows_uniform;
   return 0;
 }
@@ -4448,16 +4523,20 @@ static int convert_half_levels(mgpbd_ctx* ctx, int first, int last) {
     if (fac) {   // scales and K from the gradients (level 0 is not assembled per iteration)
       const mgpbd_ctx::Factored& F = ctx->fac;
       const int gv = nblocks((int64_t)kFacLanes * F.nv);
+      hipStream_t s = ctx->stream;
+      const int gt = nblocks(L.n);
       if (ctx->precision == MGPBD_PREC_FP16) {
-        k_fac_tet<1><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
-                                                               L.sd, L.isd, F.dinv, F.tk, F.ahat);
-        k_fac_vert<1><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.invm,
-                                                      F.gradC, L.isd, F.vs_k, F.vs_g);
+        k_fac_tet<1><<<gt, kBlock, 0, s>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha, F.cs, L.sd,
+                                           L.isd, F.dinv, F.tka, F.tkz, F.ahat);
+        k_fac_vert<1><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase,
+                                            F.gradC, F.k(), F.vs_k, F.vs_g, F.invm, F.ic2,
+                                            F.tbw);
       } else {
-        k_fac_tet<2><<<nblocks(L.n), kBlock, 0, ctx->stream>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha,
-                                                               L.sd, L.isd, F.dinv, F.tk, F.ahat);
-        k_fac_vert<2><<<gv, kBlock, 0, ctx->stream>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase, F.invm,
-                                                      F.gradC, L.isd, F.vs_k, F.vs_g);
+        k_fac_tet<2><<<gt, kBlock, 0, s>>>(L.n, F.t_i, F.invm, F.gradC, F.alpha, F.cs, L.sd,
+                                           L.isd, F.dinv, F.tka, F.tkz, F.ahat);
+        k_fac_vert<2><<<gv, kBlock, 0, s>>>(F.nv, F.vs_ptr, F.vs_cnt, F.vs_inc, F.tbase,
+                                            F.gradC, F.k(), F.vs_k, F.vs_g, F.invm, F.ic2,
+                                            F.tbw);
       }
       continue;
     }
@@ -4531,11 +4610,11 @@ int mgpbd_mg_probe_level0(mgpbd_ctx* ctx, int reps, const float* b, const float*
     with_rows(F, [&](auto rows) {
       using RS = decltype(rows);
       if (ctx->precision == MGPBD_PREC_FP16)
-        k_fac_apply<1, 1, RS><<<nblocks(L.n), kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, x, y,
+        k_fac_apply<1, 1, RS><<<nblocks(L.n), kBlock, 0, s>>>(L.n, rows, F.k(), F.ahat, F.u, x, y,
                                                               b, ctx->omega, nullptr, L.isd,
                                                               nullptr, nullptr, nullptr);
       else
-        k_fac_apply<2, 1, RS><<<nblocks(L.n), kBlock, 0, s>>>(L.n, rows, F.tk, F.ahat, F.u, x, y,
+        k_fac_apply<2, 1, RS><<<nblocks(L.n), kBlock, 0, s>>>(L.n, rows, F.k(), F.ahat, F.u, x, y,
                                                               b, ctx->omega, nullptr, L.isd,
                                                               nullptr, nullptr, nullptr);
     });

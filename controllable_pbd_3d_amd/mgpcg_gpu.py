@@ -492,7 +492,7 @@ class GPUMultigridSolver:
             "vs_inc": torch.from_numpy(vs_inc).to(dev),
             "vs_k": torch.zeros(2 * len(vs_inc), dtype=torch.int32, device=dev),
             "vs_g": torch.zeros(4 * len(vs_inc), dtype=torch.float32, device=dev),
-            "tk": torch.zeros(6 * nt, dtype=torch.int32, device=dev),
+            "tk": torch.zeros((9 * nt + 1) // 2, dtype=torch.int32, device=dev),
             "ahat": torch.zeros(nt, dtype=torch.float32, device=dev),
             "u": torch.zeros(4 * nv, dtype=torch.float32, device=dev),
             "u64": torch.zeros(4 * nv, dtype=torch.float64, device=dev),

@@ -461,8 +461,11 @@ int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr_
  * iteration and its PCG multiplies that CSR, cons/mgpcg_gpu.py:261-329). The level-0 matrix
  * is A = J W J^T + diag(alpha~) with J's rows the tets' vertex gradients, so
  *  - the V-cycle's level-0 sweeps, residuals and power steps apply its scaled form
- *    K K^T + diag(a^), K_tk = sqrt(invm_v) gradC_tk / sqrt(A_tt), a^_t = alpha~_t / A_tt,
- *    from K stored FP16/BF16 (two passes: per-vertex gather of K x, per-tet product);
+ *    K W' K^T + diag(a^), K_tk = c gradC_tk / sqrt(A_tt), W' = diag(invm) / c^2,
+ *    a^_t = alpha~_t / A_tt, c^2 = max invm (read from v_invm here, once), from K stored
+ *    FP16/BF16 (two passes: per-vertex gather of K x times w'_v, per-tet product); K of a
+ *    tet's fourth vertex is the rounded negative sum of the first three (a tet constraint's
+ *    gradients sum to zero), so the tet order stores nine values per tet;
  *  - the PCG's A p, its guessed-start and restart residuals and the FP64 check apply A itself
  *    with FP64 products and sums of the FP32 gradients (the exact operator of those
  *    gradients, where the assembled FP32 entries round each A_ij once);
@@ -475,9 +478,11 @@ int mgpbd_mg_set_level_window(mgpbd_ctx* ctx, int level, const int32_t* seg_ptr_
  * aligned), alpha_tilde (n_tets); the vertex-sliced incidence, four lanes per vertex: slot k of
  * vertex 16 s + l at vs_ptr[s] + 64 (k / 4) + 4 l + k % 4 for k < vs_cnt[v], holding vs_inc =
  * 4 tet + local vertex index; outputs vs_k (2 uint32 per slot), vs_g (4 floats per slot,
- * 16-byte aligned), tk (6 uint32 per tet), ahat (n_tets floats), u (4 floats per vertex,
+ * 16-byte aligned), tk (18 bytes per tet: a 16-byte plane of n_tets, then a 2-byte plane;
+ * 16-byte aligned), ahat (n_tets floats), u (4 floats per vertex,
  * 16-byte aligned), u64 (4 doubles per vertex, 32-byte aligned), dinv (n_tets floats). Needs
- * a reduced precision (mgpbd_mg_set_precision). t_i NULL switches the factored operator off.
+ * a reduced precision (mgpbd_mg_set_precision). Synchronises the device (the read of
+ * v_invm). t_i NULL switches the factored operator off.
  * Level 0 must be set (mgpbd_mg_set_level) and its scale arrays given
  * (mgpbd_mg_set_level_half) first; setting level 0 again switches it off. */
 int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i,

@@ -130,13 +130,46 @@ def factored_level0(t_i, v_invm, gradC, alpha_tilde, A_diag):
     return K, np.asarray(alpha_tilde, np.float64) * isd * isd
 
 
-def factored_apply(t_i, K, ahat, x, n_verts):
-    """(K K^T + diag(a^)) x: per-vertex sums u_v = sum_t K_tk x_t, then per tet
-    a^_t x_t + sum_k K_tk . u_v(t,k) (FP64)."""
+def factored_level0_weighted(t_i, v_invm, gradC, alpha_tilde, A_diag):
+    """The same scaled matrix as K W' K^T + diag(a^) with the vertex weights outside K:
+    K_tk = c g_tk / sqrt(A_tt), w'_v = w_v / c^2, c^2 = max w (FP64). A tet's constraint is a
+    function of its deformation gradient, so its four K_tk sum to zero. Returns K
+    (n_tets, 4, 3), a^ (n_tets,) and w' (n_verts,)."""
+    tv = np.asarray(t_i, np.int64).reshape(-1, 4)
+    w = np.asarray(v_invm, np.float64)
+    c2 = w.max() if w.max() > 0 else 1.0
+    isd = 1.0 / np.sqrt(np.asarray(A_diag, np.float64))
+    K = np.sqrt(c2) * np.asarray(gradC, np.float64).reshape(-1, 4, 3) * isd[:, None, None]
+    return K, np.asarray(alpha_tilde, np.float64) * isd * isd, w / c2
+
+
+def round_storage(x, fmt):
+    """float32 values rounded to the V-cycle's 16-bit storage format (round to nearest even;
+    FP16 saturates at +-65504), returned as float64."""
+    x = np.asarray(x, np.float32)
+    if fmt == "fp16":
+        return np.clip(x, -65504.0, 65504.0).astype(np.float16).astype(np.float64)
+    u = x.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def factored_fourth(K3, fmt):
+    """K of a tet's fourth vertex from the stored first three (K3: (..., 3, 3) values of the
+    storage format): the negative sum ((k0 + k1) + k2) in float32, rounded to the format."""
+    k = np.asarray(K3, np.float32)
+    return round_storage(-((k[..., 0, :] + k[..., 1, :]) + k[..., 2, :]), fmt)
+
+
+def factored_apply(t_i, K, ahat, x, n_verts, w=None):
+    """(K W K^T + diag(a^)) x: per-vertex sums u_v = w_v sum_t K_tk x_t (w = 1 when not
+    given), then per tet a^_t x_t + sum_k K_tk . u_v(t,k) (FP64)."""
     tv = np.asarray(t_i, np.int64).reshape(-1, 4)
     x = np.asarray(x, np.float64)
     u = np.zeros((n_verts, 3))
     np.add.at(u, tv.reshape(-1), (K * x[:, None, None]).reshape(-1, 3))
+    if w is not None:
+        u *= np.asarray(w, np.float64)[:, None]
     return ahat * x + np.einsum("tkc,tkc->t", K, u[tv])
 
 

@@ -603,6 +603,14 @@ def test_factored_level1_and_diagonal_from_gradients():
         mg.update_all_coarse_matrices()
 
 
+def _encode_half(v, precision):
+    """Raw 16-bit words of values already in the storage format."""
+    v = np.asarray(v, np.float32)
+    if precision == "fp16":
+        return v.astype(np.float16).view(np.uint16)
+    return (v.view(np.uint32) >> 16).astype(np.uint16)
+
+
 def _decode_half(raw, precision):
     return raw.view(np.float16).astype(np.float64) if precision == "fp16" else \
         _bf16_to_f32(raw).astype(np.float64)
@@ -610,12 +618,14 @@ def _decode_half(raw, precision):
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16"])
 def test_factored_level0_operator(precision):
-    """Factored level 0 (K K^T + diag(a^) from the gradients): (1) the stored K equals the
-    oracle's FP64 K rounded once (half an ulp of the format), the vertex-slot copy equal to the
-    tet-order copy bit for bit (an exactly symmetric operator); (2) the V-cycle's level-0
-    residual b - A^ x equals the FP64 product with the stored K within FP32 accumulation
-    (1e-5 of max |A^ x|); (3) it agrees with the assembled half level's residual within the
-    format's rounding (norm-wise 1e-2 FP16, 5e-2 BF16)."""
+    """Factored level 0 (K W' K^T + diag(a^) from the gradients): (1) the stored K of a tet's
+    first three vertices equals the oracle's FP64 K rounded once (half an ulp of the format),
+    the fourth equals the oracle's rebuild from those three bit for bit and the FP64 K within
+    the rounding of the four, and the vertex-slot copy equals the tet-order values bit for bit
+    (an exactly symmetric operator); (2) the V-cycle's level-0 residual b - A^ x equals the
+    FP64 product with the stored K and the weights w' = invm / max invm within FP32
+    accumulation (1e-5 of max |A^ x|); (3) it agrees with the assembled half level's residual
+    within the format's rounding (norm-wise 1e-2 FP16, 5e-2 BF16)."""
     from tests.scenarios import physical_block
     mesh, mu_t = physical_block(6, seed=4)
     v0 = deformed(mesh, seed=4, amp=0.05, twist=0.1)
@@ -630,13 +640,19 @@ def test_factored_level0_operator(precision):
     g = sf.gradC.cpu().numpy().reshape(-1).astype(np.float64)
     assert np.array_equal(g, sa.gradC.cpu().numpy().reshape(-1))
     d = mg.levels[0].to_scipy().diagonal().astype(np.float64)
-    K64, a64 = om.factored_level0(mesh.tets, mesh.invm, g, sf.alpha_tilde.cpu().numpy(), d)
+    K64, a64, w64 = om.factored_level0_weighted(mesh.tets, mesh.invm, g,
+                                                sf.alpha_tilde.cpu().numpy(), d)
     f = mg._fac
-    tk = f["tk"].cpu().numpy().view(np.uint16).reshape(3, nt, 4).transpose(1, 0, 2).reshape(nt, 12)
-    Kd = _decode_half(tk, precision).reshape(nt, 4, 3)
+    raw = f["tk"].cpu().numpy().view(np.uint16)
+    tk9 = np.concatenate([raw[:8 * nt].reshape(nt, 8), raw[8 * nt:9 * nt].reshape(nt, 1)], 1)
+    K9 = _decode_half(tk9, precision).reshape(nt, 3, 3)
     eps = 2.0 ** -11 if precision == "fp16" else 2.0 ** -8
     tiny = 2.0 ** -25 if precision == "fp16" else 1e-38
-    assert np.all(np.abs(Kd - K64) <= eps * np.abs(K64) + tiny)
+    assert np.all(np.abs(K9 - K64[:, :3]) <= eps * np.abs(K64[:, :3]) + tiny)
+    k4 = om.factored_fourth(K9, precision)
+    assert np.all(np.abs(k4 - K64[:, 3]) <= eps * np.abs(K64).sum(axis=1) + tiny)
+    Kd = np.concatenate([K9, k4[:, None, :]], axis=1)
+    tk = np.concatenate([tk9, _encode_half(k4, precision)], axis=1)
     ahat = f["ahat"].cpu().numpy().astype(np.float64)
     np.testing.assert_allclose(ahat, a64, rtol=1e-6)
     vs_ptr, vs_cnt = f["vs_ptr"].cpu().numpy(), f["vs_cnt"].cpu().numpy()
@@ -659,7 +675,7 @@ def test_factored_level0_operator(precision):
     rf, ra = torch.empty_like(xd), torch.empty_like(xd)
     mg.level_residual(0, bd, xd, rf)
     mga.level_residual(0, bd, xd, ra)
-    ax = om.factored_apply(mesh.tets, Kd, ahat, x, nv)
+    ax = om.factored_apply(mesh.tets, Kd, ahat, x, nv, w64)
     got = rf.cpu().numpy().astype(np.float64)
     scale = np.abs(ax).max()
     assert np.abs(got - (b - ax)).max() <= 1e-5 * scale

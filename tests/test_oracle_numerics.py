@@ -173,6 +173,43 @@ def test_factored_level0_equals_scaled_assembled_matrix():
     X = rng.standard_normal((nt, 3))
     got = np.stack([om.factored_apply(m.tets, K, ahat, X[:, j], nv) for j in range(3)], axis=1)
     np.testing.assert_allclose(got, S @ X, rtol=1e-12, atol=1e-12 * np.abs(S @ X).max())
+    # the weighted form (vertex weights outside K) is the same matrix; its K rows sum to zero
+    # over a tet's four vertices (translation invariance of the constraint), which the stored
+    # form uses to keep three of them
+    Kw, ahw, wp = om.factored_level0_weighted(m.tets, invm, g, al, d)
+    np.testing.assert_allclose(ahw, ahat, rtol=1e-15)
+    assert wp.max() == 1.0 and np.all(wp[invm == 0] == 0)
+    got = np.stack([om.factored_apply(m.tets, Kw, ahw, X[:, j], nv, wp) for j in range(3)],
+                   axis=1)
+    np.testing.assert_allclose(got, S @ X, rtol=1e-12, atol=1e-12 * np.abs(S @ X).max())
+    assert np.abs(Kw.sum(axis=1)).max() <= 1e-12 * np.abs(Kw).max()
+
+
+def test_factored_fourth_vertex_from_three():
+    """The stored form's fourth K: rounded from the float32 negative sum of the three stored
+    ones; on gradients that sum to zero it lands within the rounding of the four values
+    (eps (|k0| + |k1| + |k2| + |k3|)) of the true fourth; FP16 saturates instead of
+    overflowing; BF16 rounds to nearest even."""
+    m, x, Dm, al, lam, C, g, b, _ = _state(n=3, seed=5)
+    invm = m.invm.astype(np.float64)
+    indptr, indices = structure.build_sparsity_pattern(m.tets, m.n_tets)
+    data = np.zeros(len(indices))
+    L64.orc_fill_A(m.n_tets, om.ptr(indptr, I32P), om.ptr(indices, I32P), om.ptr(m.tets, I32P),
+                   om.ptr(invm, R64), om.ptr(g, R64), om.ptr(al, R64), om.ptr(data, R64))
+    d = sp.csr_matrix((data, indices, indptr)).diagonal()
+    K, _, _ = om.factored_level0_weighted(m.tets, invm, g, al, d)
+    for fmt, eps in (("fp16", 2.0 ** -11), ("bf16", 2.0 ** -8)):
+        K3 = om.round_storage(K[:, :3, :], fmt)
+        k4 = om.factored_fourth(K3, fmt)
+        bound = eps * (np.abs(K).sum(axis=1)) + 1e-30
+        assert np.all(np.abs(k4 - K[:, 3, :]) <= bound)
+        # a dropped sign or a missing term is far outside that bound
+        assert not np.all(np.abs(-k4 - K[:, 3, :]) <= bound)
+    assert om.round_storage(np.float32(1e6), "fp16") == 65504.0
+    assert om.round_storage(np.float32(-1e6), "fp16") == -65504.0
+    one_up = np.array([1.0 + 2.0 ** -8], np.float32)   # halfway between two BF16 values
+    assert om.round_storage(one_up, "bf16")[0] == 1.0   # ties to even
+    assert om.round_storage(np.array([1.0 + 3 * 2.0 ** -8], np.float32), "bf16")[0] == 1.0 + 2.0 ** -6
 
 
 def test_vertex_slices_layout():
