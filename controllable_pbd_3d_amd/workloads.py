@@ -40,7 +40,7 @@ CONFIGS = {
     "c2": dict(kind="box", n=56, h=0.02, E=(1e4, 1e8), max_levels=5,
                label="box56_1053696tets_E1e4-1e8", smoothing=((8, 1), 40.0)),
     "c3": dict(kind="box", n=32, h=0.03, E=(1e4, 1e6), max_levels=4, drag=True,
-               label="box32_196608tets_drag", smoothing=((12, 1), 40.0)),
+               label="box32_196608tets_drag", smoothing=((8, 1), 40.0)),
     "c4": dict(kind="ellipsoid", h=0.01, E=(1e5, 1e8), max_levels=6, drag=True,
                maxiter_cg=1000, coarse_refresh="newton",
                label="ellipsoid_h0.01_inclusion_drag"),
