@@ -3216,6 +3216,7 @@ int mgpbd_ctx_device_bytes(mgpbd_ctx* c, int64_t* bytes) {
   int nd = 0;
   for (double* b : {c->dense, c->dA, c->dR, c->dX2, c->dT}) nd += b != nullptr;
   t += (int64_t)nd * (int64_t)c->dense_cap * (int64_t)sizeof(double);
+  t += (int64_t)c->fac_tbw_cap * (int64_t)sizeof(int2);
   *bytes = t;
   return 0;
 }

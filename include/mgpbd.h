@@ -501,7 +501,9 @@ int mgpbd_mg_set_level0_factored(mgpbd_ctx* ctx, int n_verts, const int32_t* t_i
  * slice rows_uniform rows of 64 slots (vs_ptr[s] = 64 rows_uniform s) with the unused slots
  * of vs_k / vs_g zero; the vertex passes then compute each lane's slot positions instead of
  * loading the slice pointer and count (same sums bit for bit: the unused slots add exact
- * zeros). */
+ * zeros). With tbase_dev the context allocates (once, 8 bytes per vertex; counted by
+ * mgpbd_ctx_device_bytes) the {base tet, w'} words that the uniform vertex pass reads, which
+ * mgpbd_mg_convert_half fills. */
 int mgpbd_mg_set_level0_offsets(mgpbd_ctx* ctx, const int32_t* tbase_dev, int rows_uniform);
 /* Packed row vertices for the factored level 0's tet passes (no reference counterpart): one
  * 64-bit word per level-0 row (device, 8-byte aligned) holding the row's first vertex in the
