@@ -3001,7 +3001,7 @@ struct mgpbd_ctx {
   } refill[MGPBD_MAX_LEVELS];
   double* pw_host = nullptr;    // pinned copy of pw_dev from the previous update (read lagged)
   MgLevel L[MGPBD_MAX_LEVELS];
-  // factored level 0 of the reduced-precision V-cycle (A^ = K K^T + diag(a^), K from the
+  // factored level 0 of the reduced-precision V-cycle (A^ = K W' K^T + diag(a^), K from the
   // caller's gradients; mgpbd_mg_set_level0_factored): caller-owned arrays
   struct Factored {
     bool on = false;

@@ -54,7 +54,7 @@ iteration counts, where they do not the later refreshes are skipped.
 on a second stream while the coarse levels refill (bit-identical to running it after them).
 ``level0`` ('assembled' | 'factored'; reduced precision with galerkin='structural'): level 0 is
 used as J W J^T + alpha~ in factored form from the tets' gradients (V-cycle sweeps from FP16/BF16
-K K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
+K W' K^T + diag(a^), PCG products and FP64 checks from FP64 sums of the gradients, level 1 from the
 aggregates' summed gradients) and is not reassembled per Newton iteration.
 ``native_substep`` (default False): ``step`` runs prediction, the Newton loop and the velocity
 update in the library (``mgpbd_substep``: the same kernels in the same order, positions

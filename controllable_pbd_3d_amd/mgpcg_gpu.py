@@ -30,9 +30,10 @@ Extensions (keyword-only, no reference counterpart):
   re-estimates the spectra from a fresh vector with a raised safety factor and restarts;
 * ``set_factored_level0`` (before ``setup``, reduced precision and the structural Galerkin
   pattern only): level 0 is used in factored form A = J W J^T + diag(alpha~) from the tets'
-  gradients and never reassembled: the V-cycle applies K K^T + diag(a^) (K_tk = sqrt(w_v) g_tk
-  / sqrt(A_tt), stored FP16/BF16; ~12 bytes per vertex-tet incidence and ~44 per tet instead of
-  ~4 per matrix entry), the PCG and its FP64 check apply J W J^T + alpha~ with FP64 sums of the
+  gradients and never reassembled: the V-cycle applies K W' K^T + diag(a^) (K_tk = c g_tk /
+  sqrt(A_tt), W' = diag(invm) / c^2, c^2 = max invm, stored FP16/BF16 with a tet's fourth
+  vertex's K the negative sum of the other three; ~12 bytes per vertex-tet incidence and ~42
+  per tet instead of ~4 per matrix entry), the PCG and its FP64 check apply J W J^T + alpha~ with FP64 sums of the
   FP32 gradients (~180 MB per product at 1M tets instead of ~450 MB), and level 1 is
   G W G^T + P^T diag(alpha~) P with G the aggregates' summed gradients per vertex.
 """
